@@ -1,0 +1,179 @@
+/*
+ * myrt.h - C ABI of the MI355X path-tracing core (libmyrt_amd.so).
+ *
+ * Every entry point is extern "C", returns an int status (MRT_OK = 0, negative = error) and
+ * reports the reason through mrt_last_error(). Handles are opaque. Buffers are plain pointers
+ * plus element counts; nothing here depends on torch or on C++ types.
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * johannes-braun/myrt). Threading: a handle may be used by one host thread at a time.
+ */
+#ifndef MYRT_H
+#define MYRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+enum {
+  MRT_OK = 0,
+  MRT_ERR_INVALID = -1,      /* bad argument or handle */
+  MRT_ERR_HIP = -2,          /* a HIP runtime call failed */
+  MRT_ERR_NOT_PREPARED = -3, /* trace before mrt_scene_prepare */
+  MRT_ERR_LIMIT = -4,        /* a documented size limit was exceeded */
+  MRT_ERR_INTERNAL = -5
+};
+
+/* BVH build modes: the reference's primitive_split_func values (src/pathtracer/bvh.hpp:159-160). */
+enum { MRT_SPLIT_NONE = 0 /* dont_split */, MRT_SPLIT_AABBS = 1 /* split_aabbs */ };
+
+/* ---- plain records ---------------------------------------------------------------------- */
+
+/* Ray record, 32 B: origin, t_min, direction (not necessarily normalised), t_max.
+ * Replaces ray_t {origin, direction, length} (src/pathtracer/bvh.hpp:88-96); a hit is accepted
+ * for t_min < t < t_max. */
+typedef struct mrt_ray {
+  float o[3];
+  float tmin;
+  float d[3];
+  float tmax;
+} mrt_ray;
+
+/* Closest-hit record, 16 B: distance, triangle index within its geometry, instance index and
+ * barycentrics (b1 weights vertex 1, b2 vertex 2; each stored as unorm16, b1 in the low half).
+ * A miss has t = +inf and prim = inst = 0xFFFFFFFF. Replaces scene::hit
+ * (src/pathtracer/scene.hpp:160-165) and hit_t (src/pt/pt_trace.comp:56-66). */
+typedef struct mrt_hit {
+  float t;
+  uint32_t prim;
+  uint32_t inst;
+  uint32_t bary;
+} mrt_hit;
+
+/* Binary BVH node, 48 B, identical layout to aligned_node_t / bvh_node_t
+ * (src/pathtracer/bvh.hpp:98-139): bit 31 of type_and_parent marks a leaf, the low 31 bits hold
+ * the parent index. Leaves reference [first, second] (inclusive) of the reordered index array;
+ * inner nodes hold their two child node indices. */
+typedef struct mrt_bvh_node {
+  float min[3];
+  uint32_t type_and_parent;
+  float max[3];
+  uint32_t first;
+  uint32_t second;
+  uint32_t pad[3];
+} mrt_bvh_node;
+
+/* Axis-aligned box with optional weighted centroid: aabb_t (src/pathtracer/bvh.hpp:25-86). */
+typedef struct mrt_aabb {
+  float min[3];
+  float max[3];
+  float centroid[3];
+  uint32_t has_centroid;
+} mrt_aabb;
+
+typedef struct mrt_scene_s* mrt_scene;
+typedef struct mrt_bvh_s* mrt_bvh;
+
+/* Counters filled by mrt_scene_prepare (scene::prepare_result_t, src/pathtracer/scene.hpp:84-91,
+ * extended with the device-side sizes of the wide BVH). */
+typedef struct mrt_prepare_result {
+  int32_t geometries_changed;
+  int32_t drawables_changed;
+  uint32_t num_geometries;
+  uint32_t num_instances;
+  uint32_t blas_wide_nodes;
+  uint32_t blas_tri_records;
+  uint32_t tlas_wide_nodes;
+  uint32_t max_stack_entries;
+  uint64_t device_bytes;
+} mrt_prepare_result;
+
+/* Per-launch traversal counters (sum over the rays of one trace call). */
+typedef struct mrt_trace_stats {
+  uint64_t rays;
+  uint64_t node_visits; /* wide 64-B node fetches */
+  uint64_t tri_tests;   /* 48-B triangle record fetches */
+  uint64_t instance_visits;
+} mrt_trace_stats;
+
+/* ---- errors, device plumbing ------------------------------------------------------------- */
+const char* mrt_last_error(void);
+int mrt_device_info(char* name_out, size_t name_cap, int* num_cus);
+int mrt_device_alloc(size_t bytes, void** out);
+int mrt_device_free(void* ptr);
+int mrt_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int mrt_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int mrt_device_synchronize(void);
+/* Stream-ordered GPU timer: records start/stop events on `stream` (NULL = default stream). */
+typedef struct mrt_timer_s* mrt_timer;
+int mrt_timer_create(mrt_timer* out);
+int mrt_timer_destroy(mrt_timer t);
+int mrt_timer_start(mrt_timer t, void* stream);
+int mrt_timer_stop(mrt_timer t, void* stream);
+int mrt_timer_elapsed_ms(mrt_timer t, float* ms); /* synchronises on the stop event */
+
+/* ---- BVH builder (host) ------------------------------------------------------------------ */
+/* generate_triangle_bounds (src/pathtracer/bvh.cpp:136-152): one padded box per triangle with
+ * the weighted centroid (p0+p1+p2)/3. */
+int mrt_generate_triangle_bounds(const uint32_t* idx, size_t n_idx, const float* pos3, size_t n_vtx,
+                                 mrt_aabb* out_boxes);
+/* bvh::bvh(aabbs, split) (src/pathtracer/bvh.cpp:154-161, 356-512): binned-SAH build, bit-exact
+ * with the reference's node order and reordered index array. */
+int mrt_bvh_build(const mrt_aabb* boxes, size_t n, int split_mode, mrt_bvh* out);
+int mrt_bvh_build_triangles(const uint32_t* idx, size_t n_idx, const float* pos3, size_t n_vtx, int split_mode,
+                            mrt_bvh* out);
+int mrt_bvh_sizes(mrt_bvh b, size_t* n_nodes, size_t* n_indices);
+/* bvh::nodes() and bvh::reordered_indices() (src/pathtracer/bvh.hpp:174-175). */
+int mrt_bvh_copy(mrt_bvh b, mrt_bvh_node* nodes, uint32_t* indices);
+int mrt_bvh_destroy(mrt_bvh b);
+
+/* ---- scene ------------------------------------------------------------------------------- */
+/* scene::scene / ~scene (src/pathtracer/scene.hpp:116-117). */
+int mrt_scene_create(mrt_scene* out);
+int mrt_scene_destroy(mrt_scene s);
+/* scene::push_geometry (src/pathtracer/scene.hpp:128-129, scene.cpp:43-71). Copies its inputs.
+ * nrm3 and uv2 may be NULL. Builds the object BVH with split_aabbs, as the reference does. */
+int mrt_scene_push_geometry(mrt_scene s, const uint32_t* idx, size_t n_idx, const float* pos3, const float* nrm3,
+                            const float* uv2, size_t n_vtx, uint32_t* out_geom);
+/* scene::enqueue(geometry, material, mat4) (src/pathtracer/scene.hpp:144-145, scene.cpp:255-271).
+ * m4x4 is column-major like rnu::mat4. out_inst receives the instance index used in mrt_hit.inst
+ * after the next prepare (the enqueue order of this frame). */
+int mrt_scene_enqueue_geometry(mrt_scene s, uint32_t geom, uint32_t material, const float m4x4_colmajor[16],
+                               uint32_t* out_inst);
+/* scene::prepare (src/pathtracer/scene.hpp:182, scene.cpp:330-409): builds the instance BVH with
+ * dont_split over this frame's drawables, collapses every BVH to the 64-B wide format and uploads
+ * it. Drawables are consumed, as in the reference; an empty frame keeps the previous one. */
+int mrt_scene_prepare(mrt_scene s, mrt_prepare_result* out);
+/* The instance inverse matrices of the prepared frame (row-major 3x4 floats, 12 per instance). */
+int mrt_scene_instance_inverses(mrt_scene s, float* out, size_t n_inst);
+/* The host half of mrt_scene_prepare (BVH builds and the wide collapse) without touching the GPU,
+ * and a copy of the resulting device-format arrays: 64-B wide nodes, 48-B triangle records,
+ * TLAS leaf slots and 64-B instance records. Sizes come from mrt_prepare_result. These replace
+ * scene::get_scene_buffers (src/pathtracer/scene.hpp:184-186), which exposes the GL buffers. */
+int mrt_scene_build_host(mrt_scene s, mrt_prepare_result* out);
+int mrt_scene_copy_wide(mrt_scene s, void* blas_nodes, void* tris, void* tlas_nodes, uint32_t* tlas_items,
+                        void* instances);
+
+/* ---- ray queries (GPU) ------------------------------------------------------------------- */
+/* nearest_hit / any_hit (src/pt/pt_trace.comp:311-386) over the two-level BVH
+ * (src/pt/bvh.impl.glsl:8-72), with a watertight triangle test and lowest-(inst, prim) ties. */
+int mrt_trace_closest(mrt_scene s, const mrt_ray* rays, size_t n, mrt_hit* hits);      /* host buffers */
+int mrt_trace_any(mrt_scene s, const mrt_ray* rays, size_t n, uint8_t* occluded);      /* host buffers */
+int mrt_trace_closest_device(mrt_scene s, const mrt_ray* d_rays, size_t n, mrt_hit* d_hits, void* stream);
+int mrt_trace_any_device(mrt_scene s, const mrt_ray* d_rays, size_t n, uint8_t* d_occluded, void* stream);
+/* Same traversal with per-launch fetch counters (slower; used to price bytes per ray). */
+int mrt_trace_stats_device(mrt_scene s, const mrt_ray* d_rays, size_t n, int any_hit, mrt_trace_stats* out,
+                           void* stream);
+/* scene::pick (src/pathtracer/scene.cpp:273-314): closest hit of one ray; returns 1 in *hit_found
+ * when something was hit. */
+int mrt_scene_pick(mrt_scene s, const mrt_ray* ray, mrt_hit* hit, int* hit_found);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MYRT_H */

@@ -1,0 +1,340 @@
+"""Python binding of libmyrt_amd.so (ctypes over the C ABI in include/myrt.h).
+
+The library owns its device memory; numpy arrays are host buffers. Nothing here falls back to a
+CPU path: loading fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmyrt_amd.so")
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("bary", "<u4")])
+NODE_DTYPE = np.dtype([("min", "<f4", 3), ("type_and_parent", "<u4"), ("max", "<f4", 3), ("first", "<u4"),
+                       ("second", "<u4"), ("pad", "<u4", 3)])
+AABB_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("centroid", "<f4", 3), ("has_centroid", "<u4")])
+WIDE_NODE_DTYPE = np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("nchildren", "u1"), ("node_base", "<u4"),
+                            ("item_base", "<u4"), ("qlo", "u1", (3, 4)), ("qhi", "u1", (3, 4)),
+                            ("count", "u1", 4), ("offset", "<u2", 4), ("pad", "<u4")])
+TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("prim", "<u4"), ("v1", "<f4", 3), ("pad1", "<u4"), ("v2", "<f4", 3),
+                      ("pad2", "<u4")])
+INSTANCE_DTYPE = np.dtype([("inv", "<f4", 12), ("blas_root", "<u4"), ("geom", "<u4"), ("inst", "<u4"),
+                           ("pad", "<u4")])
+assert WIDE_NODE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48 and INSTANCE_DTYPE.itemsize == 64
+
+SPLIT_NONE = 0
+SPLIT_AABBS = 1
+MISS = 0xFFFFFFFF
+
+
+class MrtError(RuntimeError):
+    pass
+
+
+class _PrepareResult(ctypes.Structure):
+    _fields_ = [("geometries_changed", ctypes.c_int32), ("drawables_changed", ctypes.c_int32),
+                ("num_geometries", ctypes.c_uint32), ("num_instances", ctypes.c_uint32),
+                ("blas_wide_nodes", ctypes.c_uint32), ("blas_tri_records", ctypes.c_uint32),
+                ("tlas_wide_nodes", ctypes.c_uint32), ("max_stack_entries", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+class _TraceStats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_uint64), ("node_visits", ctypes.c_uint64), ("tri_tests", ctypes.c_uint64),
+                ("instance_visits", ctypes.c_uint64)]
+
+
+_lib = None
+
+# Every exported entry point and its argument types; include/myrt.h declares exactly these.
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_U32P = ctypes.POINTER(ctypes.c_uint32)
+SIGNATURES = {
+    "mrt_last_error": ([], ctypes.c_char_p),
+    "mrt_device_info": ([ctypes.c_char_p, _SZ, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "mrt_device_alloc": ([_SZ, ctypes.POINTER(_P)], ctypes.c_int),
+    "mrt_device_free": ([_P], ctypes.c_int),
+    "mrt_memcpy_h2d": ([_P, _P, _SZ], ctypes.c_int),
+    "mrt_memcpy_d2h": ([_P, _P, _SZ], ctypes.c_int),
+    "mrt_device_synchronize": ([], ctypes.c_int),
+    "mrt_timer_create": ([ctypes.POINTER(_P)], ctypes.c_int),
+    "mrt_timer_destroy": ([_P], ctypes.c_int),
+    "mrt_timer_start": ([_P, _P], ctypes.c_int),
+    "mrt_timer_stop": ([_P, _P], ctypes.c_int),
+    "mrt_timer_elapsed_ms": ([_P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "mrt_generate_triangle_bounds": ([_P, _SZ, _P, _SZ, _P], ctypes.c_int),
+    "mrt_bvh_build": ([_P, _SZ, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "mrt_bvh_build_triangles": ([_P, _SZ, _P, _SZ, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+    "mrt_bvh_sizes": ([_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], ctypes.c_int),
+    "mrt_bvh_copy": ([_P, _P, _P], ctypes.c_int),
+    "mrt_bvh_destroy": ([_P], ctypes.c_int),
+    "mrt_scene_create": ([ctypes.POINTER(_P)], ctypes.c_int),
+    "mrt_scene_destroy": ([_P], ctypes.c_int),
+    "mrt_scene_push_geometry": ([_P, _P, _SZ, _P, _P, _P, _SZ, _U32P], ctypes.c_int),
+    "mrt_scene_enqueue_geometry": ([_P, ctypes.c_uint32, ctypes.c_uint32, _P, _U32P], ctypes.c_int),
+    "mrt_scene_prepare": ([_P, ctypes.POINTER(_PrepareResult)], ctypes.c_int),
+    "mrt_scene_instance_inverses": ([_P, _P, _SZ], ctypes.c_int),
+    "mrt_scene_build_host": ([_P, ctypes.POINTER(_PrepareResult)], ctypes.c_int),
+    "mrt_scene_copy_wide": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "mrt_trace_closest": ([_P, _P, _SZ, _P], ctypes.c_int),
+    "mrt_trace_any": ([_P, _P, _SZ, _P], ctypes.c_int),
+    "mrt_trace_closest_device": ([_P, _P, _SZ, _P, _P], ctypes.c_int),
+    "mrt_trace_any_device": ([_P, _P, _SZ, _P, _P], ctypes.c_int),
+    "mrt_trace_stats_device": ([_P, _P, _SZ, ctypes.c_int, ctypes.POINTER(_TraceStats), _P], ctypes.c_int),
+    "mrt_scene_pick": ([_P, _P, _P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+}
+
+
+def lib():
+    """Loads libmyrt_amd.so once; raises if it is missing (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MrtError("libmyrt_amd.so is not built (run python -m myrt_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().mrt_last_error()
+        raise MrtError("myrt error %d: %s" % (rc, msg.decode() if msg else ""))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cus = ctypes.c_int(0)
+    _check(lib().mrt_device_info(name, 256, ctypes.byref(cus)))
+    return name.value.decode(), cus.value
+
+
+def synchronize():
+    _check(lib().mrt_device_synchronize())
+
+
+class DeviceBuffer:
+    """A block of device memory owned by libmyrt_amd."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().mrt_device_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        _check(lib().mrt_memcpy_h2d(b.ptr, _ptr(a), a.nbytes))
+        return b
+
+    def to_array(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        _check(lib().mrt_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().mrt_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Timer:
+    """Stream-ordered event timer (start/stop recorded on a HIP stream, NULL = default)."""
+
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _check(lib().mrt_timer_create(ctypes.byref(p)))
+        self.h = p
+
+    def start(self, stream=None):
+        _check(lib().mrt_timer_start(self.h, stream))
+
+    def stop(self, stream=None):
+        _check(lib().mrt_timer_stop(self.h, stream))
+
+    def elapsed_ms(self):
+        ms = ctypes.c_float()
+        _check(lib().mrt_timer_elapsed_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().mrt_timer_destroy(self.h)
+        except Exception:
+            pass
+
+
+def triangle_bounds(indices, points):
+    idx = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros(idx.size // 3, dtype=AABB_DTYPE)
+    _check(lib().mrt_generate_triangle_bounds(_ptr(idx), idx.size, _ptr(pts), pts.shape[0], _ptr(out)))
+    return out
+
+
+def _bvh_result(h):
+    nn, ni = ctypes.c_size_t(), ctypes.c_size_t()
+    try:
+        _check(lib().mrt_bvh_sizes(h, ctypes.byref(nn), ctypes.byref(ni)))
+        nodes = np.zeros(nn.value, dtype=NODE_DTYPE)
+        idx = np.zeros(ni.value, dtype=np.uint32)
+        _check(lib().mrt_bvh_copy(h, _ptr(nodes), _ptr(idx)))
+    finally:
+        lib().mrt_bvh_destroy(h)
+    return nodes, idx
+
+
+def bvh_build(boxes, split=SPLIT_NONE):
+    """bvh(aabbs, split) -> (nodes, reordered_indices)."""
+    boxes = np.ascontiguousarray(boxes, dtype=AABB_DTYPE)
+    h = ctypes.c_void_p()
+    _check(lib().mrt_bvh_build(_ptr(boxes), boxes.size, split, ctypes.byref(h)))
+    return _bvh_result(h)
+
+
+def bvh_build_triangles(indices, points, split=SPLIT_AABBS):
+    idx = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    h = ctypes.c_void_p()
+    _check(lib().mrt_bvh_build_triangles(_ptr(idx), idx.size, _ptr(pts), pts.shape[0], split, ctypes.byref(h)))
+    return _bvh_result(h)
+
+
+def make_rays(origins, directions, tmin=0.0, tmax=np.inf):
+    o = np.asarray(origins, dtype=np.float32).reshape(-1, 3)
+    d = np.asarray(directions, dtype=np.float32).reshape(-1, 3)
+    r = np.zeros(o.shape[0], dtype=RAY_DTYPE)
+    r["o"] = o
+    r["d"] = d
+    r["tmin"] = tmin
+    r["tmax"] = tmax
+    return r
+
+
+def unpack_bary(hits):
+    b = hits["bary"].astype(np.uint32)
+    return (b & 0xFFFF).astype(np.float32) / 65535.0, (b >> 16).astype(np.float32) / 65535.0
+
+
+class Scene:
+    """myrt::scene (src/pathtracer/scene.hpp:60-255) over the C ABI."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().mrt_scene_create(ctypes.byref(h)))
+        self.h = h
+        self.last_prepare = None
+
+    def close(self):
+        if self.h:
+            lib().mrt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def push_geometry(self, indices, points, normals=None, uvs=None):
+        idx = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        nrm = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        uv = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+        out = ctypes.c_uint32()
+        _check(lib().mrt_scene_push_geometry(self.h, _ptr(idx), idx.size, _ptr(pts), _ptr(nrm), _ptr(uv),
+                                             pts.shape[0], ctypes.byref(out)))
+        return out.value
+
+    def enqueue(self, geometry, transformation=None, material=0):
+        """scene::enqueue(geometry, material, mat4): `transformation` is a 4x4 array in the
+        usual row/column sense (m[r][c]); it is passed column-major like rnu::mat4."""
+        m = np.eye(4, dtype=np.float32) if transformation is None else np.asarray(transformation, np.float32)
+        colmajor = np.ascontiguousarray(m.T).ravel()
+        out = ctypes.c_uint32()
+        _check(lib().mrt_scene_enqueue_geometry(self.h, int(geometry), int(material), _ptr(colmajor),
+                                                ctypes.byref(out)))
+        return out.value
+
+    def _result(self, r):
+        return {k: getattr(r, k) for k, _ in _PrepareResult._fields_}
+
+    def prepare(self):
+        r = _PrepareResult()
+        _check(lib().mrt_scene_prepare(self.h, ctypes.byref(r)))
+        self.last_prepare = self._result(r)
+        return self.last_prepare
+
+    def build_host(self):
+        """BVH builds and wide collapse only (no GPU)."""
+        r = _PrepareResult()
+        _check(lib().mrt_scene_build_host(self.h, ctypes.byref(r)))
+        self.last_prepare = self._result(r)
+        return self.last_prepare
+
+    def wide_arrays(self):
+        p = self.last_prepare
+        blas = np.zeros(p["blas_wide_nodes"], WIDE_NODE_DTYPE)
+        tris = np.zeros(p["blas_tri_records"], TRI_DTYPE)
+        tlas = np.zeros(p["tlas_wide_nodes"], WIDE_NODE_DTYPE)
+        n_items = p["num_instances"]
+        items = np.zeros(max(n_items, 1), np.uint32)
+        inst = np.zeros(max(p["num_instances"], 1), INSTANCE_DTYPE)
+        _check(lib().mrt_scene_copy_wide(self.h, _ptr(blas), _ptr(tris), _ptr(tlas), _ptr(items), _ptr(inst)))
+        return blas, tris, tlas, items[:n_items], inst[:p["num_instances"]]
+
+    def instance_inverses(self, n):
+        out = np.zeros((max(n, 1), 12), np.float32)
+        _check(lib().mrt_scene_instance_inverses(self.h, _ptr(out), max(n, 1)))
+        return out[:n]
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(rays.size, HIT_DTYPE)
+        _check(lib().mrt_trace_closest(self.h, _ptr(rays), rays.size, _ptr(hits)))
+        return hits
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        occ = np.zeros(rays.size, np.uint8)
+        _check(lib().mrt_trace_any(self.h, _ptr(rays), rays.size, _ptr(occ)))
+        return occ
+
+    def trace_closest_device(self, d_rays, n, d_hits, stream=None):
+        _check(lib().mrt_trace_closest_device(self.h, d_rays.ptr, n, d_hits.ptr, stream))
+
+    def trace_any_device(self, d_rays, n, d_occ, stream=None):
+        _check(lib().mrt_trace_any_device(self.h, d_rays.ptr, n, d_occ.ptr, stream))
+
+    def trace_stats(self, d_rays, n, any_hit=False):
+        s = _TraceStats()
+        _check(lib().mrt_trace_stats_device(self.h, d_rays.ptr, n, 1 if any_hit else 0, ctypes.byref(s), None))
+        return {k: getattr(s, k) for k, _ in _TraceStats._fields_}
+
+    def pick(self, origin, direction, length=np.inf):
+        """scene::pick(ray_t) -> None or the hit record."""
+        r = make_rays([origin], [direction], 0.0, length)
+        h = np.zeros(1, HIT_DTYPE)
+        found = ctypes.c_int()
+        _check(lib().mrt_scene_pick(self.h, _ptr(r), _ptr(h), ctypes.byref(found)))
+        return h[0] if found.value else None

@@ -1,0 +1,667 @@
+// Scene registry, BVH preparation and the C ABI of libmyrt_amd.so.
+//
+// Mirrors myrt::scene (src/pathtracer/scene.hpp:60-255, scene.cpp:43-71, 255-409): geometries
+// are copied and get their object BVH (split_aabbs) at push time; drawables are enqueued per
+// frame and consumed by prepare(), which builds the instance BVH (dont_split) over the
+// transformed geometry boxes. Storage is HIP device memory instead of GL buffers, and every BVH
+// is collapsed to the 64-byte wide format the traversal kernels read.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/myrt.h"
+#include "bvh_build.hpp"
+#include "device_types.hpp"
+#include "scene_internal.hpp"
+#include "trace.hpp"
+#include "wide_bvh.hpp"
+
+namespace mrt {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(MRT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Inverse of a column-major 4x4 matrix by cofactor expansion in binary64, rounded to binary32.
+// Returns the first three rows of the inverse as row-major 3x4.
+bool invert_affine_rows(const float m[16], float out[12]) {
+  double a[4][4];  // a[r][c]
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) a[r][c] = double(m[c * 4 + r]);
+  // 2x2 minors of rows 0-1 and rows 2-3.
+  const double s0 = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+  const double s1 = a[0][0] * a[1][2] - a[1][0] * a[0][2];
+  const double s2 = a[0][0] * a[1][3] - a[1][0] * a[0][3];
+  const double s3 = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+  const double s4 = a[0][1] * a[1][3] - a[1][1] * a[0][3];
+  const double s5 = a[0][2] * a[1][3] - a[1][2] * a[0][3];
+  const double c5 = a[2][2] * a[3][3] - a[3][2] * a[2][3];
+  const double c4 = a[2][1] * a[3][3] - a[3][1] * a[2][3];
+  const double c3 = a[2][1] * a[3][2] - a[3][1] * a[2][2];
+  const double c2 = a[2][0] * a[3][3] - a[3][0] * a[2][3];
+  const double c1 = a[2][0] * a[3][2] - a[3][0] * a[2][2];
+  const double c0 = a[2][0] * a[3][1] - a[3][0] * a[2][1];
+  const double det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+  if (!(det != 0.0) || !std::isfinite(det)) return false;
+  const double id = 1.0 / det;
+  double inv[3][4];
+  inv[0][0] = (a[1][1] * c5 - a[1][2] * c4 + a[1][3] * c3) * id;
+  inv[0][1] = (-a[0][1] * c5 + a[0][2] * c4 - a[0][3] * c3) * id;
+  inv[0][2] = (a[3][1] * s5 - a[3][2] * s4 + a[3][3] * s3) * id;
+  inv[0][3] = (-a[2][1] * s5 + a[2][2] * s4 - a[2][3] * s3) * id;
+  inv[1][0] = (-a[1][0] * c5 + a[1][2] * c2 - a[1][3] * c1) * id;
+  inv[1][1] = (a[0][0] * c5 - a[0][2] * c2 + a[0][3] * c1) * id;
+  inv[1][2] = (-a[3][0] * s5 + a[3][2] * s2 - a[3][3] * s1) * id;
+  inv[1][3] = (a[2][0] * s5 - a[2][2] * s2 + a[2][3] * s1) * id;
+  inv[2][0] = (a[1][0] * c4 - a[1][1] * c2 + a[1][3] * c0) * id;
+  inv[2][1] = (-a[0][0] * c4 + a[0][1] * c2 - a[0][3] * c0) * id;
+  inv[2][2] = (a[3][0] * s4 - a[3][1] * s2 + a[3][3] * s0) * id;
+  inv[2][3] = (-a[2][0] * s4 + a[2][1] * s2 - a[2][3] * s0) * id;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 4; ++c) out[r * 4 + c] = float(inv[r][c]);
+  return true;
+}
+
+namespace {
+
+constexpr double kBlasPadRel = 1.0 / (1 << 20);
+constexpr double kTlasPadRel = 1.0 / (1 << 16);
+
+template <typename T>
+int upload(T*& dptr, size_t& cap, const std::vector<T>& v) {
+  const size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+  if (!dptr || cap < bytes) {
+    if (dptr) hipFree(dptr);
+    dptr = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&dptr), bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    cap = bytes;
+  }
+  if (!v.empty()) {
+    hipError_t e = hipMemcpy(dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+  }
+  return MRT_OK;
+}
+
+}  // namespace
+
+// scene::enqueue's instance box: the eight corners of the geometry box through M (scene.cpp:256-261).
+Box transform_box(const Box& b, const float m[16]) {
+  Box out;
+  for (unsigned i = 0; i < 8; ++i) {
+    const float x = (i & 1u) ? b.mx.x : b.mn.x;
+    const float y = (i & 2u) ? b.mx.y : b.mn.y;
+    const float z = (i & 4u) ? b.mx.z : b.mn.z;
+    vec3f p;
+    for (int r = 0; r < 3; ++r) p[r] = m[0 * 4 + r] * x + m[1 * 4 + r] * y + m[2 * 4 + r] * z + m[3 * 4 + r];
+    out.enclose(p);
+  }
+  return out;
+}
+
+int Scene::build_host() {
+  try {
+    if (geoms_dirty) {
+      blas_nodes.clear();
+      tris.clear();
+      blas_root.assign(geoms.size(), 0);
+      blas_depth = 0;
+      for (size_t g = 0; g < geoms.size(); ++g) {
+        const Geometry& geo = geoms[g];
+        const uint32_t node_off = uint32_t(blas_nodes.size());
+        const uint32_t item_off = uint32_t(tris.size());
+        blas_root[g] = node_off;
+        blas_depth = std::max(blas_depth, geo.wide.depth);
+        for (WideNode n : geo.wide.nodes) {
+          n.node_base += node_off;
+          n.item_base += item_off;
+          blas_nodes.push_back(n);
+        }
+        for (uint32_t item : geo.wide.items) {
+          const uint32_t prim = geo.bvh.indices[item];
+          TriRecord t{};
+          const float* p0 = &geo.pos[size_t(geo.idx[3 * size_t(prim) + 0]) * 3];
+          const float* p1 = &geo.pos[size_t(geo.idx[3 * size_t(prim) + 1]) * 3];
+          const float* p2 = &geo.pos[size_t(geo.idx[3 * size_t(prim) + 2]) * 3];
+          std::memcpy(t.v0, p0, 12);
+          std::memcpy(t.v1, p1, 12);
+          std::memcpy(t.v2, p2, 12);
+          t.prim = prim;
+          tris.push_back(t);
+        }
+      }
+      if (tris.size() >= kMaxItemIndex) return fail(MRT_ERR_LIMIT, "scene exceeds 16M triangle references");
+    }
+    if (inst_dirty) {
+      tlas_nodes.clear();
+      tlas_items.clear();
+      inst_records.clear();
+      tlas_depth = 0;
+      if (!current.empty()) {
+        std::vector<Box> boxes;
+        boxes.reserve(current.size());
+        for (const Instance& in : current) boxes.push_back(in.world_box);
+        tlas = build_binary_bvh(boxes, false);
+        WideBvh w = collapse_to_wide(tlas, kTlasPadRel);
+        tlas_nodes = std::move(w.nodes);
+        tlas_depth = w.depth;
+        for (uint32_t item : w.items) tlas_items.push_back(tlas.indices[item]);
+        if (tlas_items.size() >= kMaxItemIndex) return fail(MRT_ERR_LIMIT, "scene exceeds 16M instance references");
+        for (size_t i = 0; i < current.size(); ++i) {
+          InstanceRecord r{};
+          std::memcpy(r.inv, current[i].inv, sizeof(r.inv));
+          r.geom = current[i].geom;
+          r.inst = uint32_t(i);
+          inst_records.push_back(r);
+        }
+      } else {
+        tlas = BinaryBvh{};
+      }
+    }
+    for (InstanceRecord& r : inst_records) r.blas_root = blas_root[r.geom];
+    max_stack = 3 * (tlas_depth + blas_depth) + 2 + 4;
+  } catch (const std::exception& ex) {
+    return fail(MRT_ERR_INTERNAL, ex.what());
+  }
+  return MRT_OK;
+}
+
+int Scene::upload_all() {
+  int rc;
+  if (geoms_dirty) {
+    if ((rc = upload(d_blas, cap_blas, blas_nodes)) != MRT_OK) return rc;
+    if ((rc = upload(d_tris, cap_tris, tris)) != MRT_OK) return rc;
+  }
+  if (inst_dirty || geoms_dirty) {
+    if ((rc = upload(d_tlas, cap_tlas, tlas_nodes)) != MRT_OK) return rc;
+    if ((rc = upload(d_tlas_items, cap_items, tlas_items)) != MRT_OK) return rc;
+    if ((rc = upload(d_inst, cap_inst, inst_records)) != MRT_OK) return rc;
+  }
+  if (!d_ctrl) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_ctrl), 64);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    num_cus = prop.multiProcessorCount;
+  }
+  return MRT_OK;
+}
+
+Scene::~Scene() {
+  if (d_blas) hipFree(d_blas);
+  if (d_tris) hipFree(d_tris);
+  if (d_tlas) hipFree(d_tlas);
+  if (d_tlas_items) hipFree(d_tlas_items);
+  if (d_inst) hipFree(d_inst);
+  if (d_ctrl) hipFree(d_ctrl);
+  if (d_spill) hipFree(d_spill);
+}
+
+int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream) {
+  if (!prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
+  if (n == 0) return MRT_OK;
+  if (n >= 0x7fffffffu) return fail(MRT_ERR_LIMIT, "at most 2^31-1 rays per call");
+  const int per_cu = trace_blocks_per_cu(any, stats);
+  const size_t max_grid = size_t(num_cus) * size_t(per_cu);
+  const size_t want = (n + kTraceBlock - 1) / kTraceBlock;
+  const int grid = int(std::max<size_t>(1, std::min(max_grid, want)));
+  const uint32_t total_threads = uint32_t(grid) * kTraceBlock;
+  const uint32_t spill_depth = max_stack > kTraceStackLds ? max_stack - kTraceStackLds : 1;
+  const size_t spill_entries = size_t(spill_depth) * total_threads;
+  if (spill_entries > spill_cap) {
+    if (d_spill) hipFree(d_spill);
+    d_spill = nullptr;
+    spill_cap = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_spill), spill_entries * 4);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc spill");
+    spill_cap = spill_entries;
+  }
+  hipError_t e = hipMemsetAsync(d_ctrl, 0, 64, stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  TraceParams p{};
+  p.tlas_nodes = d_tlas;
+  p.tlas_items = d_tlas_items;
+  p.instances = d_inst;
+  p.blas_nodes = d_blas;
+  p.tris = d_tris;
+  p.rays = d_rays;
+  p.out = d_out;
+  p.n = uint32_t(n);
+  p.has_tlas = tlas_nodes.empty() ? 0u : 1u;
+  p.counter = d_ctrl;
+  p.flags = d_ctrl + 1;
+  p.stats = reinterpret_cast<unsigned long long*>(d_ctrl + 8);
+  p.spill = d_spill;
+  p.spill_depth = spill_depth;
+  p.total_threads = total_threads;
+  p.refill_below = refill_below;
+  if (launch_trace(p, any, stats, grid, stream) != 0) return fail(MRT_ERR_HIP, "trace kernel launch failed");
+  return MRT_OK;
+}
+
+int Scene::read_ctrl(uint32_t* flags, unsigned long long* stats, hipStream_t stream) {
+  uint32_t host[16];
+  hipError_t e = hipMemcpyAsync(host, d_ctrl, 64, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return hip_fail(e, "reading trace counters");
+  *flags = host[1];
+  if (stats) std::memcpy(stats, host + 8, 4 * sizeof(unsigned long long));
+  return MRT_OK;
+}
+
+}  // namespace mrt
+
+using namespace mrt;
+
+struct mrt_bvh_s {
+  BinaryBvh b;
+};
+
+struct mrt_timer_s {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
+#define MRT_GUARD_BEGIN try {
+#define MRT_GUARD_END                                              \
+  }                                                                \
+  catch (const std::bad_alloc&) {                                  \
+    return fail(MRT_ERR_LIMIT, "out of host memory");              \
+  }                                                                \
+  catch (const std::exception& ex) {                               \
+    return fail(MRT_ERR_INTERNAL, ex.what());                      \
+  }
+
+extern "C" {
+
+const char* mrt_last_error(void) { return g_last_error.c_str(); }
+
+int mrt_device_info(char* name_out, size_t name_cap, int* num_cus) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  if (name_out && name_cap) {
+    std::snprintf(name_out, name_cap, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (num_cus) *num_cus = prop.multiProcessorCount;
+  return MRT_OK;
+}
+
+int mrt_device_alloc(size_t bytes, void** out) {
+  if (!out) return fail(MRT_ERR_INVALID, "null output pointer");
+  hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 1));
+  return e == hipSuccess ? MRT_OK : hip_fail(e, "hipMalloc");
+}
+int mrt_device_free(void* ptr) {
+  hipError_t e = hipFree(ptr);
+  return e == hipSuccess ? MRT_OK : hip_fail(e, "hipFree");
+}
+int mrt_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? MRT_OK : hip_fail(e, "hipMemcpy H2D");
+}
+int mrt_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? MRT_OK : hip_fail(e, "hipMemcpy D2H");
+}
+int mrt_device_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? MRT_OK : hip_fail(e, "hipDeviceSynchronize");
+}
+
+int mrt_timer_create(mrt_timer* out) {
+  if (!out) return fail(MRT_ERR_INVALID, "null output pointer");
+  auto* t = new mrt_timer_s;
+  hipError_t e = hipEventCreate(&t->start);
+  if (e == hipSuccess) e = hipEventCreate(&t->stop);
+  if (e != hipSuccess) {
+    delete t;
+    return hip_fail(e, "hipEventCreate");
+  }
+  *out = t;
+  return MRT_OK;
+}
+int mrt_timer_destroy(mrt_timer t) {
+  if (!t) return MRT_OK;
+  hipEventDestroy(t->start);
+  hipEventDestroy(t->stop);
+  delete t;
+  return MRT_OK;
+}
+int mrt_timer_start(mrt_timer t, void* stream) {
+  if (!t) return fail(MRT_ERR_INVALID, "null timer");
+  hipError_t e = hipEventRecord(t->start, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? MRT_OK : hip_fail(e, "hipEventRecord");
+}
+int mrt_timer_stop(mrt_timer t, void* stream) {
+  if (!t) return fail(MRT_ERR_INVALID, "null timer");
+  hipError_t e = hipEventRecord(t->stop, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? MRT_OK : hip_fail(e, "hipEventRecord");
+}
+int mrt_timer_elapsed_ms(mrt_timer t, float* ms) {
+  if (!t || !ms) return fail(MRT_ERR_INVALID, "null argument");
+  hipError_t e = hipEventSynchronize(t->stop);
+  if (e == hipSuccess) e = hipEventElapsedTime(ms, t->start, t->stop);
+  return e == hipSuccess ? MRT_OK : hip_fail(e, "hipEventElapsedTime");
+}
+
+static void box_to_c(const Box& b, mrt_aabb* o) {
+  o->min[0] = b.mn.x; o->min[1] = b.mn.y; o->min[2] = b.mn.z;
+  o->max[0] = b.mx.x; o->max[1] = b.mx.y; o->max[2] = b.mx.z;
+  o->centroid[0] = b.wc.x; o->centroid[1] = b.wc.y; o->centroid[2] = b.wc.z;
+  o->has_centroid = b.has_wc ? 1u : 0u;
+}
+static Box box_from_c(const mrt_aabb& a) {
+  Box b;
+  b.mn = {a.min[0], a.min[1], a.min[2]};
+  b.mx = {a.max[0], a.max[1], a.max[2]};
+  b.wc = {a.centroid[0], a.centroid[1], a.centroid[2]};
+  b.has_wc = a.has_centroid != 0;
+  return b;
+}
+
+static int check_mesh(const uint32_t* idx, size_t n_idx, const float* pos3, size_t n_vtx) {
+  if (!idx || !pos3) return fail(MRT_ERR_INVALID, "null index or position pointer");
+  if (n_idx == 0 || n_idx % 3 != 0) return fail(MRT_ERR_INVALID, "index count must be a positive multiple of 3");
+  for (size_t i = 0; i < n_idx; ++i)
+    if (idx[i] >= n_vtx) return fail(MRT_ERR_INVALID, "index out of range of the vertex array");
+  return MRT_OK;
+}
+
+int mrt_generate_triangle_bounds(const uint32_t* idx, size_t n_idx, const float* pos3, size_t n_vtx,
+                                 mrt_aabb* out_boxes) {
+  MRT_GUARD_BEGIN
+  int rc = check_mesh(idx, n_idx, pos3, n_vtx);
+  if (rc != MRT_OK) return rc;
+  if (!out_boxes) return fail(MRT_ERR_INVALID, "null output");
+  const std::vector<Box> boxes = triangle_bounds(idx, n_idx, pos3);
+  for (size_t i = 0; i < boxes.size(); ++i) box_to_c(boxes[i], out_boxes + i);
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_bvh_build(const mrt_aabb* boxes, size_t n, int split_mode, mrt_bvh* out) {
+  MRT_GUARD_BEGIN
+  if (!boxes || !out || n == 0) return fail(MRT_ERR_INVALID, "need at least one box and an output handle");
+  if (split_mode != MRT_SPLIT_NONE && split_mode != MRT_SPLIT_AABBS) return fail(MRT_ERR_INVALID, "bad split mode");
+  std::vector<Box> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = box_from_c(boxes[i]);
+  auto* h = new mrt_bvh_s;
+  try {
+    h->b = build_binary_bvh(v, split_mode == MRT_SPLIT_AABBS);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_bvh_build_triangles(const uint32_t* idx, size_t n_idx, const float* pos3, size_t n_vtx, int split_mode,
+                            mrt_bvh* out) {
+  MRT_GUARD_BEGIN
+  int rc = check_mesh(idx, n_idx, pos3, n_vtx);
+  if (rc != MRT_OK) return rc;
+  if (!out) return fail(MRT_ERR_INVALID, "null output handle");
+  if (split_mode != MRT_SPLIT_NONE && split_mode != MRT_SPLIT_AABBS) return fail(MRT_ERR_INVALID, "bad split mode");
+  auto* h = new mrt_bvh_s;
+  try {
+    h->b = build_binary_bvh(triangle_bounds(idx, n_idx, pos3), split_mode == MRT_SPLIT_AABBS);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_bvh_sizes(mrt_bvh b, size_t* n_nodes, size_t* n_indices) {
+  if (!b) return fail(MRT_ERR_INVALID, "null bvh");
+  if (n_nodes) *n_nodes = b->b.nodes.size();
+  if (n_indices) *n_indices = b->b.indices.size();
+  return MRT_OK;
+}
+
+int mrt_bvh_copy(mrt_bvh b, mrt_bvh_node* nodes, uint32_t* indices) {
+  if (!b) return fail(MRT_ERR_INVALID, "null bvh");
+  static_assert(sizeof(mrt_bvh_node) == sizeof(BinNode), "node layouts differ");
+  if (nodes) std::memcpy(nodes, b->b.nodes.data(), b->b.nodes.size() * sizeof(BinNode));
+  if (indices) std::memcpy(indices, b->b.indices.data(), b->b.indices.size() * 4);
+  return MRT_OK;
+}
+
+int mrt_bvh_destroy(mrt_bvh b) {
+  delete b;
+  return MRT_OK;
+}
+
+int mrt_scene_create(mrt_scene* out) {
+  MRT_GUARD_BEGIN
+  if (!out) return fail(MRT_ERR_INVALID, "null output handle");
+  *out = reinterpret_cast<mrt_scene>(new Scene);
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_destroy(mrt_scene s) {
+  delete reinterpret_cast<Scene*>(s);
+  return MRT_OK;
+}
+
+int mrt_scene_push_geometry(mrt_scene sh, const uint32_t* idx, size_t n_idx, const float* pos3, const float* nrm3,
+                            const float* uv2, size_t n_vtx, uint32_t* out_geom) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s) return fail(MRT_ERR_INVALID, "null scene");
+  int rc = check_mesh(idx, n_idx, pos3, n_vtx);
+  if (rc != MRT_OK) return rc;
+  Geometry g;
+  g.idx.assign(idx, idx + n_idx);
+  g.pos.assign(pos3, pos3 + 3 * n_vtx);
+  if (nrm3) g.nrm.assign(nrm3, nrm3 + 3 * n_vtx);
+  if (uv2) g.uv.assign(uv2, uv2 + 2 * n_vtx);
+  else g.uv.assign(2 * n_vtx, 0.f);  // scene.cpp:55-58 zero-fills missing uvs
+  g.bvh = build_binary_bvh(triangle_bounds(idx, n_idx, pos3), true);
+  g.box.mn = {g.bvh.nodes[0].mn[0], g.bvh.nodes[0].mn[1], g.bvh.nodes[0].mn[2]};
+  g.box.mx = {g.bvh.nodes[0].mx[0], g.bvh.nodes[0].mx[1], g.bvh.nodes[0].mx[2]};
+  g.wide = collapse_to_wide(g.bvh, 1.0 / (1 << 20));
+  s->geoms.push_back(std::move(g));
+  s->geoms_dirty = true;
+  if (out_geom) *out_geom = uint32_t(s->geoms.size() - 1);
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_enqueue_geometry(mrt_scene sh, uint32_t geom, uint32_t material, const float m4x4_colmajor[16],
+                               uint32_t* out_inst) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s) return fail(MRT_ERR_INVALID, "null scene");
+  if (geom >= s->geoms.size()) return fail(MRT_ERR_INVALID, "unknown geometry");
+  if (!m4x4_colmajor) return fail(MRT_ERR_INVALID, "null matrix");
+  Instance in;
+  in.geom = geom;
+  in.material = material;
+  std::memcpy(in.m, m4x4_colmajor, sizeof(in.m));
+  if (!invert_affine_rows(in.m, in.inv)) return fail(MRT_ERR_INVALID, "transformation is not invertible");
+  in.world_box = transform_box(s->geoms[geom].box, in.m);
+  s->pending.push_back(in);
+  if (out_inst) *out_inst = uint32_t(s->pending.size() - 1);
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
+  if (!s->pending.empty()) {
+    s->current = std::move(s->pending);
+    s->pending.clear();
+    s->inst_dirty = true;
+  }
+  const bool gd = s->geoms_dirty, id = s->inst_dirty;
+  int rc = s->build_host();
+  if (rc != MRT_OK) return rc;
+  if (upload_too) {
+    rc = s->upload_all();
+    if (rc != MRT_OK) return rc;
+    s->geoms_dirty = false;
+    s->inst_dirty = false;
+    s->prepared = true;
+  }
+  if (out) {
+    std::memset(out, 0, sizeof(*out));
+    out->geometries_changed = gd ? 1 : 0;
+    out->drawables_changed = id ? 1 : 0;
+    out->num_geometries = uint32_t(s->geoms.size());
+    out->num_instances = uint32_t(s->current.size());
+    out->blas_wide_nodes = uint32_t(s->blas_nodes.size());
+    out->blas_tri_records = uint32_t(s->tris.size());
+    out->tlas_wide_nodes = uint32_t(s->tlas_nodes.size());
+    out->max_stack_entries = s->max_stack;
+    out->device_bytes = s->blas_nodes.size() * sizeof(WideNode) + s->tris.size() * sizeof(TriRecord) +
+                        s->tlas_nodes.size() * sizeof(WideNode) + s->tlas_items.size() * 4 +
+                        s->inst_records.size() * sizeof(InstanceRecord);
+  }
+  return MRT_OK;
+}
+
+int mrt_scene_prepare(mrt_scene sh, mrt_prepare_result* out) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s) return fail(MRT_ERR_INVALID, "null scene");
+  return prepare_impl(s, out, true);
+  MRT_GUARD_END
+}
+
+int mrt_scene_instance_inverses(mrt_scene sh, float* out, size_t n_inst) {
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !out) return fail(MRT_ERR_INVALID, "null argument");
+  if (n_inst < s->current.size()) return fail(MRT_ERR_INVALID, "output too small");
+  for (size_t i = 0; i < s->current.size(); ++i) std::memcpy(out + 12 * i, s->current[i].inv, 48);
+  return MRT_OK;
+}
+
+int mrt_scene_build_host(mrt_scene sh, mrt_prepare_result* out) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s) return fail(MRT_ERR_INVALID, "null scene");
+  return prepare_impl(s, out, false);
+  MRT_GUARD_END
+}
+
+int mrt_scene_copy_wide(mrt_scene sh, void* blas_nodes, void* tris, void* tlas_nodes, uint32_t* tlas_items,
+                        void* instances) {
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s) return fail(MRT_ERR_INVALID, "null scene");
+  if (blas_nodes) std::memcpy(blas_nodes, s->blas_nodes.data(), s->blas_nodes.size() * sizeof(WideNode));
+  if (tris) std::memcpy(tris, s->tris.data(), s->tris.size() * sizeof(TriRecord));
+  if (tlas_nodes) std::memcpy(tlas_nodes, s->tlas_nodes.data(), s->tlas_nodes.size() * sizeof(WideNode));
+  if (tlas_items) std::memcpy(tlas_items, s->tlas_items.data(), s->tlas_items.size() * 4);
+  if (instances) std::memcpy(instances, s->inst_records.data(), s->inst_records.size() * sizeof(InstanceRecord));
+  return MRT_OK;
+}
+
+static int trace_host(mrt_scene sh, const mrt_ray* rays, size_t n, void* out, size_t out_elem, bool any) {
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || (n && (!rays || !out))) return fail(MRT_ERR_INVALID, "null argument");
+  if (!s->prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
+  if (n == 0) return MRT_OK;
+  void *d_r = nullptr, *d_o = nullptr;
+  hipError_t e = hipMalloc(&d_r, n * sizeof(mrt_ray));
+  if (e == hipSuccess) e = hipMalloc(&d_o, n * out_elem);
+  if (e == hipSuccess) e = hipMemcpy(d_r, rays, n * sizeof(mrt_ray), hipMemcpyHostToDevice);
+  int rc = MRT_OK;
+  if (e != hipSuccess) rc = hip_fail(e, "staging rays");
+  if (rc == MRT_OK) rc = s->trace(d_r, n, d_o, any, false, nullptr);
+  uint32_t flags = 0;
+  if (rc == MRT_OK) rc = s->read_ctrl(&flags, nullptr, nullptr);
+  if (rc == MRT_OK) {
+    e = hipMemcpy(out, d_o, n * out_elem, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "reading results");
+  }
+  if (d_r) hipFree(d_r);
+  if (d_o) hipFree(d_o);
+  if (rc == MRT_OK && (flags & 1u)) rc = fail(MRT_ERR_INTERNAL, "traversal stack overflow");
+  return rc;
+}
+
+int mrt_trace_closest(mrt_scene s, const mrt_ray* rays, size_t n, mrt_hit* hits) {
+  MRT_GUARD_BEGIN
+  return trace_host(s, rays, n, hits, sizeof(mrt_hit), false);
+  MRT_GUARD_END
+}
+int mrt_trace_any(mrt_scene s, const mrt_ray* rays, size_t n, uint8_t* occluded) {
+  MRT_GUARD_BEGIN
+  return trace_host(s, rays, n, occluded, 1, true);
+  MRT_GUARD_END
+}
+int mrt_trace_closest_device(mrt_scene sh, const mrt_ray* d_rays, size_t n, mrt_hit* d_hits, void* stream) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || (n && (!d_rays || !d_hits))) return fail(MRT_ERR_INVALID, "null argument");
+  return s->trace(d_rays, n, d_hits, false, false, static_cast<hipStream_t>(stream));
+  MRT_GUARD_END
+}
+int mrt_trace_any_device(mrt_scene sh, const mrt_ray* d_rays, size_t n, uint8_t* d_occluded, void* stream) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || (n && (!d_rays || !d_occluded))) return fail(MRT_ERR_INVALID, "null argument");
+  return s->trace(d_rays, n, d_occluded, true, false, static_cast<hipStream_t>(stream));
+  MRT_GUARD_END
+}
+int mrt_trace_stats_device(mrt_scene sh, const mrt_ray* d_rays, size_t n, int any_hit, mrt_trace_stats* out,
+                           void* stream) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !out || (n && !d_rays)) return fail(MRT_ERR_INVALID, "null argument");
+  void* d_o = nullptr;
+  hipError_t e = hipMalloc(&d_o, std::max<size_t>(n, 1) * (any_hit ? 1 : sizeof(mrt_hit)));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  int rc = s->trace(d_rays, n, d_o, any_hit != 0, true, st);
+  uint32_t flags = 0;
+  unsigned long long st4[4] = {0, 0, 0, 0};
+  if (rc == MRT_OK) rc = s->read_ctrl(&flags, st4, st);
+  hipFree(d_o);
+  if (rc != MRT_OK) return rc;
+  if (flags & 1u) return fail(MRT_ERR_INTERNAL, "traversal stack overflow");
+  out->rays = st4[0];
+  out->node_visits = st4[1];
+  out->tri_tests = st4[2];
+  out->instance_visits = st4[3];
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_pick(mrt_scene s, const mrt_ray* ray, mrt_hit* hit, int* hit_found) {
+  MRT_GUARD_BEGIN
+  if (!ray || !hit || !hit_found) return fail(MRT_ERR_INVALID, "null argument");
+  int rc = trace_host(s, ray, 1, hit, sizeof(mrt_hit), false);
+  if (rc != MRT_OK) return rc;
+  *hit_found = hit->inst != 0xFFFFFFFFu ? 1 : 0;
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// Scene state behind the mrt_scene handle.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+#include <vector>
+
+#include "bvh_build.hpp"
+#include "device_types.hpp"
+#include "wide_bvh.hpp"
+
+namespace mrt {
+
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+bool invert_affine_rows(const float m[16], float out[12]);
+Box transform_box(const Box& b, const float m[16]);
+
+// geometry_t plus the arrays push_geometry appends (src/pathtracer/scene.cpp:7-12, 43-71).
+struct Geometry {
+  std::vector<uint32_t> idx;
+  std::vector<float> pos, nrm, uv;
+  BinaryBvh bvh;  // object BVH, split_aabbs
+  WideBvh wide;
+  Box box;        // bvh root box = geometry_t::aabb
+};
+
+// drawable_geometry_t (src/pathtracer/scene.hpp:97-104).
+struct Instance {
+  uint32_t geom = 0, material = 0;
+  float m[16];     // column-major
+  float inv[12];   // rows 0-2 of the inverse
+  Box world_box;
+};
+
+struct Scene {
+  std::vector<Geometry> geoms;
+  std::vector<Instance> pending;  // enqueued since the last prepare
+  std::vector<Instance> current;  // the prepared frame
+  bool geoms_dirty = false, inst_dirty = false, prepared = false;
+
+  // Host copies of the device arrays.
+  std::vector<WideNode> blas_nodes, tlas_nodes;
+  std::vector<TriRecord> tris;
+  std::vector<uint32_t> tlas_items, blas_root;
+  std::vector<InstanceRecord> inst_records;
+  BinaryBvh tlas;
+  uint32_t blas_depth = 0, tlas_depth = 0, max_stack = 0;
+
+  // Device arrays.
+  WideNode *d_blas = nullptr, *d_tlas = nullptr;
+  TriRecord* d_tris = nullptr;
+  uint32_t* d_tlas_items = nullptr;
+  InstanceRecord* d_inst = nullptr;
+  size_t cap_blas = 0, cap_tris = 0, cap_tlas = 0, cap_items = 0, cap_inst = 0;
+  uint32_t* d_ctrl = nullptr;  // [0] ray counter, [1] flags, [8..15] four u64 counters
+  uint32_t* d_spill = nullptr;
+  size_t spill_cap = 0;
+  int num_cus = 0;
+  uint32_t refill_below = 40;
+
+  ~Scene();
+  int build_host();
+  int upload_all();
+  int trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream);
+  int read_ctrl(uint32_t* flags, unsigned long long* stats, hipStream_t stream);
+};
+
+}  // namespace mrt

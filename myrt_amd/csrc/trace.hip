@@ -1,0 +1,360 @@
+// Closest-hit and any-hit ray traversal for gfx950.
+//
+// Replaces the reference's stackless binary traversal (src/pt/bvh.impl.glsl:8-72), its
+// Moeller-Trumbore test (src/pt/intersect.glsl:3-28) and the two-level visitors
+// (src/pt/pt_trace.comp:229-329) with:
+//   * 4-wide BVH nodes of one 64-byte line each, quantised child boxes, tested with a robust slab
+//     test (far distances scaled by 1 + 2*gamma(3), Ize 2013) so no box that contains a hit is
+//     culled;
+//   * the watertight ray/triangle test of Woop, Benthin and Wald (JCGT 2013) with the FP64
+//     fallback for zero edge functions, so the hit set does not depend on traversal order;
+//   * closest-hit ties broken by the lowest (instance, triangle) pair;
+//   * persistent waves that refill idle lanes from a global ray counter (ballot + one atomic per
+//     wave), and a per-lane ring stack of 16 entries in LDS that spills to a per-thread slice of
+//     global memory when a path needs more.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_types.hpp"
+#include "trace.hpp"
+
+namespace mrt {
+namespace dev {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kStackLds = 16;
+constexpr float kRobust = 1.0000005f;  // > 1 + 2*gamma(3) for binary32
+
+struct RayPre {  // per-space ray data for the watertight test
+  float ox, oy, oz;   // origin permuted to (kx, ky, kz)
+  float sx, sy, sz;   // shear constants
+  int kx, ky, kz;
+};
+
+__device__ inline float sel3(int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); }
+
+__device__ inline void make_pre(float ox, float oy, float oz, float dx, float dy, float dz, RayPre& r) {
+  const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+  int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  const float dkz = sel3(kz, dx, dy, dz);
+  if (dkz < 0.f) {
+    const int t = kx;
+    kx = ky;
+    ky = t;
+  }
+  r.kx = kx;
+  r.ky = ky;
+  r.kz = kz;
+  r.sx = sel3(kx, dx, dy, dz) / dkz;
+  r.sy = sel3(ky, dx, dy, dz) / dkz;
+  r.sz = 1.0f / dkz;
+  r.ox = sel3(kx, ox, oy, oz);
+  r.oy = sel3(ky, ox, oy, oz);
+  r.oz = sel3(kz, ox, oy, oz);
+}
+
+// Watertight test. Returns true when the ray's line crosses the triangle (either side);
+// t = T / det is then the hit distance and (b1, b2) the weights of v1 and v2.
+__device__ inline bool watertight(const RayPre& r, float4 a, float4 b, float4 c, float& t, float& b1, float& b2) {
+  const float Axk = sel3(r.kx, a.x, a.y, a.z) - r.ox, Ayk = sel3(r.ky, a.x, a.y, a.z) - r.oy,
+              Azk = sel3(r.kz, a.x, a.y, a.z) - r.oz;
+  const float Bxk = sel3(r.kx, b.x, b.y, b.z) - r.ox, Byk = sel3(r.ky, b.x, b.y, b.z) - r.oy,
+              Bzk = sel3(r.kz, b.x, b.y, b.z) - r.oz;
+  const float Cxk = sel3(r.kx, c.x, c.y, c.z) - r.ox, Cyk = sel3(r.ky, c.x, c.y, c.z) - r.oy,
+              Czk = sel3(r.kz, c.x, c.y, c.z) - r.oz;
+  const float Ax = Axk - r.sx * Azk, Ay = Ayk - r.sy * Azk;
+  const float Bx = Bxk - r.sx * Bzk, By = Byk - r.sy * Bzk;
+  const float Cx = Cxk - r.sx * Czk, Cy = Cyk - r.sy * Czk;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.f || V == 0.f || W == 0.f) {
+    U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+    V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+    W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+  }
+  if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+  const float det = U + V + W;
+  if (det == 0.f) return false;
+  const float Az = r.sz * Azk, Bz = r.sz * Bzk, Cz = r.sz * Czk;
+  const float T = U * Az + V * Bz + W * Cz;
+  t = T / det;
+  const float rcp = 1.0f / det;
+  b1 = V * rcp;
+  b2 = W * rcp;
+  return true;
+}
+
+__device__ inline uint32_t pack_bary(float b1, float b2) {
+  const uint32_t u = uint32_t(__saturatef(b1) * 65535.f + 0.5f);
+  const uint32_t v = uint32_t(__saturatef(b2) * 65535.f + 0.5f);
+  return u | (v << 16);
+}
+
+__device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
+__device__ inline float byte_of(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
+
+template <bool ANY, bool STATS>
+__global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
+  __shared__ uint32_t lds_stack[kStackLds][kBlock];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t gtid = blockIdx.x * kBlock + tid;
+  const int lane = __lane_id();
+
+  int ray_id = -1;
+  bool queue_empty = false;
+
+  float wox = 0, woy = 0, woz = 0, wdx = 0, wdy = 0, wdz = 0, wix = 0, wiy = 0, wiz = 0;
+  float tmin = 0, tmax = 0;
+  float ox = 0, oy = 0, oz = 0, ix = 0, iy = 0, iz = 0;
+  RayPre pre{};
+  float t_best = 0, b1_best = 0, b2_best = 0;
+  uint32_t inst_best = 0, prim_best = 0, cur_inst = 0;
+  uint32_t entry = 0, sp = 0;
+  int level = 0;
+  bool found = false;
+  uint32_t n_nodes = 0, n_tris = 0, n_inst = 0;
+
+  const auto push = [&](uint32_t e) {
+    const uint32_t slot = sp & (kStackLds - 1);
+    if (sp >= kStackLds) {
+      const uint32_t deep = sp - kStackLds;
+      if (deep < p.spill_depth) {
+        p.spill[size_t(deep) * p.total_threads + gtid] = lds_stack[slot][tid];
+      } else {
+        atomicOr(p.flags, 1u);  // overflow: the entry is dropped and the call reports an error
+      }
+    }
+    lds_stack[slot][tid] = e;
+    ++sp;
+  };
+  const auto pop = [&]() -> uint32_t {
+    --sp;
+    const uint32_t slot = sp & (kStackLds - 1);
+    const uint32_t e = lds_stack[slot][tid];
+    if (sp >= kStackLds) {
+      const uint32_t deep = sp - kStackLds;
+      if (deep < p.spill_depth) lds_stack[slot][tid] = p.spill[size_t(deep) * p.total_threads + gtid];
+    }
+    return e;
+  };
+
+  for (;;) {
+    // ---- refill idle lanes from the global ray counter ----
+    const bool want = ray_id < 0 && !queue_empty;
+    const uint64_t want_mask = __ballot(want);
+    if (want_mask) {
+      const int leader = __ffsll((unsigned long long)want_mask) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(p.counter, uint32_t(__popcll(want_mask)));
+      base = __shfl(base, leader);
+      if (want) {
+        const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
+        const uint32_t r = base + rank;
+        if (r < p.n) {
+          ray_id = int(r);
+          const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(r);
+          const float4 r0 = rp[0], r1 = rp[1];
+          wox = r0.x; woy = r0.y; woz = r0.z; tmin = r0.w;
+          wdx = r1.x; wdy = r1.y; wdz = r1.z; tmax = r1.w;
+          wix = 1.0f / wdx; wiy = 1.0f / wdy; wiz = 1.0f / wdz;
+          ox = wox; oy = woy; oz = woz; ix = wix; iy = wiy; iz = wiz;
+          t_best = tmax;
+          inst_best = prim_best = 0xFFFFFFFFu;
+          b1_best = b2_best = 0.f;
+          found = false;
+          level = 0;
+          sp = 0;
+          entry = 0;  // TLAS root
+          if (STATS) n_nodes = n_tris = n_inst = 0;
+          if (!p.has_tlas || !(wdx != 0.f || wdy != 0.f || wdz != 0.f)) entry = kStackReturn;  // finishes at once
+        }
+      }
+    }
+    if (__ballot(want && ray_id < 0)) queue_empty = true;
+    if (__ballot(ray_id >= 0) == 0) break;
+
+    // ---- traverse until too many lanes are idle ----
+    for (;;) {
+      if (ray_id >= 0) {
+        bool have_next = false;
+        uint32_t next = 0;
+        bool done = false;
+        if (entry == kStackReturn) {
+          if (level == 0) {
+            done = true;  // no scene or degenerate ray
+          } else {
+            level = 0;
+            ox = wox; oy = woy; oz = woz; ix = wix; iy = wiy; iz = wiz;
+          }
+        } else if (entry & 0x80000000u) {
+          const uint32_t cnt = (entry >> 24) & 0x7fu;
+          const uint32_t start = entry & 0xffffffu;
+          if (level == 0) {
+            // Enter an instance: the remaining instances of this leaf stay on the stack.
+            const uint32_t inst = p.tlas_items[start];
+            if (cnt > 1) push(0x80000000u | ((cnt - 1) << 24) | (start + 1));
+            push(kStackReturn);
+            const float4* ip = reinterpret_cast<const float4*>(p.instances + inst);
+            const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
+            const uint4 meta = reinterpret_cast<const uint4*>(ip)[3];
+            ox = m0.x * wox + m0.y * woy + m0.z * woz + m0.w;
+            oy = m1.x * wox + m1.y * woy + m1.z * woz + m1.w;
+            oz = m2.x * wox + m2.y * woy + m2.z * woz + m2.w;
+            const float dx = m0.x * wdx + m0.y * wdy + m0.z * wdz;
+            const float dy = m1.x * wdx + m1.y * wdy + m1.z * wdz;
+            const float dz = m2.x * wdx + m2.y * wdy + m2.z * wdz;
+            ix = 1.0f / dx; iy = 1.0f / dy; iz = 1.0f / dz;
+            make_pre(ox, oy, oz, dx, dy, dz, pre);
+            cur_inst = meta.z;
+            level = 1;
+            next = meta.x;  // BLAS root node
+            have_next = true;
+            if (STATS) ++n_inst;
+          } else {
+            const float4* tp = reinterpret_cast<const float4*>(p.tris) + 3 * size_t(start);
+            for (uint32_t k = 0; k < cnt; ++k) {
+              const float4 a = tp[3 * k], b = tp[3 * k + 1], c = tp[3 * k + 2];
+              if (STATS) ++n_tris;
+              float t, b1, b2;
+              if (!watertight(pre, a, b, c, t, b1, b2)) continue;
+              if (!(t > tmin && t < tmax)) continue;
+              const uint32_t prim = __float_as_uint(a.w);
+              if (ANY) {
+                found = true;
+                done = true;
+                break;
+              }
+              if (t < t_best || (t == t_best && (cur_inst < inst_best || (cur_inst == inst_best && prim < prim_best)))) {
+                t_best = t;
+                inst_best = cur_inst;
+                prim_best = prim;
+                b1_best = b1;
+                b2_best = b2;
+                found = true;
+              }
+            }
+          }
+        } else {
+          // ---- inner node: test the four quantised child boxes ----
+          const WideNode* nodes = level ? p.blas_nodes : p.tlas_nodes;
+          const uint4* np = reinterpret_cast<const uint4*>(nodes + entry);
+          const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
+          if (STATS) ++n_nodes;
+          const float orx = __uint_as_float(q0.x), ory = __uint_as_float(q0.y), orz = __uint_as_float(q0.z);
+          const float scx = exp2i(int(int8_t(q0.w & 0xffu)));
+          const float scy = exp2i(int(int8_t((q0.w >> 8) & 0xffu)));
+          const float scz = exp2i(int(int8_t((q0.w >> 16) & 0xffu)));
+          const uint32_t nch = q0.w >> 24;
+          float key[4];
+          uint32_t ent[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float lx = fmaf(byte_of(q1.z, c), scx, orx), hx = fmaf(byte_of(q2.y, c), scx, orx);
+            const float ly = fmaf(byte_of(q1.w, c), scy, ory), hy = fmaf(byte_of(q2.z, c), scy, ory);
+            const float lz = fmaf(byte_of(q2.x, c), scz, orz), hz = fmaf(byte_of(q2.w, c), scz, orz);
+            const float tx0 = (lx - ox) * ix, tx1 = (hx - ox) * ix;
+            const float ty0 = (ly - oy) * iy, ty1 = (hy - oy) * iy;
+            const float tz0 = (lz - oz) * iz, tz1 = (hz - oz) * iz;
+            const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+            const float tf = fminf(fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)), t_best) * kRobust;
+            const bool hit = (uint32_t(c) < nch) && tn <= tf;
+            const uint32_t cnt = (q3.x >> (8 * c)) & 0xffu;
+            const uint32_t off = (c < 2 ? (q3.y >> (16 * c)) : (q3.z >> (16 * (c - 2)))) & 0xffffu;
+            ent[c] = cnt == 0 ? (q1.x + off) : (0x80000000u | (cnt << 24) | (q1.y + off));
+            key[c] = hit ? tn : __int_as_float(0x7f800000);
+          }
+          // Sort the four (distance, entry) pairs ascending; misses carry +inf and sort last.
+#define MRT_CSWAP(i, j)                                  \
+  if (key[j] < key[i]) {                                 \
+    const float tk = key[i]; key[i] = key[j]; key[j] = tk; \
+    const uint32_t te = ent[i]; ent[i] = ent[j]; ent[j] = te; \
+  }
+          MRT_CSWAP(0, 1) MRT_CSWAP(2, 3) MRT_CSWAP(0, 2) MRT_CSWAP(1, 3) MRT_CSWAP(1, 2)
+#undef MRT_CSWAP
+          const float inf = __int_as_float(0x7f800000);
+          // A box whose near distance is +inf is never reached (a zero direction component with
+          // the origin outside that slab), so it counts as missed.
+          uint32_t nhit = 0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) nhit += (key[c] < inf) ? 1u : 0u;
+          if (nhit > 3) push(ent[3]);
+          if (nhit > 2) push(ent[2]);
+          if (nhit > 1) push(ent[1]);
+          if (nhit > 0) {
+            next = ent[0];
+            have_next = true;
+          }
+        }
+        if (!done && !have_next) {
+          if (sp == 0) {
+            done = true;
+          } else {
+            next = pop();
+            have_next = true;
+          }
+        }
+        if (done) {
+          if (ANY) {
+            reinterpret_cast<uint8_t*>(p.out)[ray_id] = found ? 1 : 0;
+          } else {
+            uint4 h;
+            h.x = found ? __float_as_uint(t_best) : 0x7f800000u;
+            h.y = found ? prim_best : 0xFFFFFFFFu;
+            h.z = found ? inst_best : 0xFFFFFFFFu;
+            h.w = found ? pack_bary(b1_best, b2_best) : 0u;
+            reinterpret_cast<uint4*>(p.out)[ray_id] = h;
+          }
+          if (STATS) {
+            atomicAdd(p.stats + 0, 1ull);
+            atomicAdd(p.stats + 1, (unsigned long long)n_nodes);
+            atomicAdd(p.stats + 2, (unsigned long long)n_tris);
+            atomicAdd(p.stats + 3, (unsigned long long)n_inst);
+          }
+          ray_id = -1;
+        } else {
+          entry = next;
+        }
+      }
+      const uint64_t active = __ballot(ray_id >= 0);
+      if (active == 0) break;
+      if (!queue_empty && __popcll(active) < p.refill_below) break;
+    }
+  }
+}
+
+}  // namespace dev
+
+int launch_trace(const TraceParams& p, bool any, bool stats, int grid, hipStream_t stream) {
+  const dim3 block(dev::kBlock), g(grid);
+  if (any) {
+    if (stats)
+      hipLaunchKernelGGL((dev::trace_kernel<true, true>), g, block, 0, stream, p);
+    else
+      hipLaunchKernelGGL((dev::trace_kernel<true, false>), g, block, 0, stream, p);
+  } else {
+    if (stats)
+      hipLaunchKernelGGL((dev::trace_kernel<false, true>), g, block, 0, stream, p);
+    else
+      hipLaunchKernelGGL((dev::trace_kernel<false, false>), g, block, 0, stream, p);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int trace_blocks_per_cu(bool any, bool stats) {
+  int blocks = 0;
+  hipError_t e;
+  if (any)
+    e = stats ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<true, true>, dev::kBlock, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<true, false>, dev::kBlock, 0);
+  else
+    e = stats ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<false, true>, dev::kBlock, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<false, false>, dev::kBlock, 0);
+  if (e != hipSuccess || blocks < 1) blocks = 1;
+  return blocks;
+}
+
+}  // namespace mrt

@@ -1,0 +1,35 @@
+// Host-visible interface of the traversal kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_types.hpp"
+
+namespace mrt {
+
+struct TraceParams {
+  const WideNode* tlas_nodes;
+  const uint32_t* tlas_items;      // TLAS leaf slot -> instance index
+  const InstanceRecord* instances;
+  const WideNode* blas_nodes;
+  const TriRecord* tris;
+  const void* rays;                // mrt_ray[n]
+  void* out;                       // mrt_hit[n] or uint8_t[n]
+  uint32_t n;
+  uint32_t has_tlas;
+  uint32_t* counter;               // ray counter, zero before the launch
+  uint32_t* flags;                 // bit 0: stack overflow
+  uint32_t* spill;                 // [spill_depth][total_threads]
+  uint32_t spill_depth;
+  uint32_t total_threads;
+  uint32_t refill_below;           // refill a wave when fewer lanes than this are busy
+  unsigned long long* stats;       // rays, node visits, triangle tests, instance visits
+};
+
+int launch_trace(const TraceParams& p, bool any, bool stats, int grid, hipStream_t stream);
+int trace_blocks_per_cu(bool any, bool stats);
+constexpr int kTraceBlock = 256;
+constexpr uint32_t kTraceStackLds = 16;
+
+}  // namespace mrt

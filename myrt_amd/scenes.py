@@ -1,0 +1,226 @@
+"""Seeded synthetic workloads: the fBm heightfield, icosphere fields and ray sets.
+
+All randomness comes from numpy's PCG64 generator with the seeds named here; the noise itself is
+integer-hashed value noise using only +, -, *, / so every platform produces the same vertices.
+"""
+import numpy as np
+
+# ---- geometry ---------------------------------------------------------------------------------
+
+
+def _hash01(ix, iz, seed):
+    """Integer lattice hash -> float in [0, 1) (a multiply/xorshift mix on uint32)."""
+    h = (ix.astype(np.uint32) * np.uint32(0x8DA6B343)) ^ (iz.astype(np.uint32) * np.uint32(0xD8163841))
+    h = h ^ np.uint32((seed * 0x9E3779B1) & 0xFFFFFFFF)
+    h ^= h >> np.uint32(15)
+    h *= np.uint32(0x2C1B3C6D)
+    h ^= h >> np.uint32(12)
+    h *= np.uint32(0x297A2D39)
+    h ^= h >> np.uint32(15)
+    return (h >> np.uint32(8)).astype(np.float64) / float(1 << 24)
+
+
+def value_noise(x, z, seed):
+    ix = np.floor(x)
+    iz = np.floor(z)
+    fx = x - ix
+    fz = z - iz
+    ix = ix.astype(np.int64)
+    iz = iz.astype(np.int64)
+    sx = fx * fx * (3.0 - 2.0 * fx)
+    sz = fz * fz * (3.0 - 2.0 * fz)
+    a = _hash01(ix, iz, seed)
+    b = _hash01(ix + 1, iz, seed)
+    c = _hash01(ix, iz + 1, seed)
+    d = _hash01(ix + 1, iz + 1, seed)
+    return (a + (b - a) * sx) * (1.0 - sz) + (c + (d - c) * sx) * sz
+
+
+def fbm(x, z, seed, octaves=6, lacunarity=2.0, gain=0.5):
+    total = np.zeros_like(x)
+    amp, freq = 1.0, 1.0
+    for o in range(octaves):
+        total += amp * (value_noise(x * freq, z * freq, seed + o) * 2.0 - 1.0)
+        amp *= gain
+        freq *= lacunarity
+    return total
+
+
+def heightfield(nx=1000, nz=500, seed=3, extent=(20.0, 10.0), height=1.5, freq=0.35):
+    """nx*nz cells, two triangles each, on (nx+1)*(nz+1) vertices over x in [-ex/2, ex/2],
+    z in [-ez/2, ez/2] with y = height * fBm(6 octaves, lacunarity 2, gain 0.5)."""
+    ex, ez = extent
+    xs = np.linspace(-ex / 2, ex / 2, nx + 1)
+    zs = np.linspace(-ez / 2, ez / 2, nz + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="xy")  # shape (nz+1, nx+1)
+    Y = height * fbm(X * freq, Z * freq, seed)
+    pts = np.stack([X, Y, Z], axis=-1).reshape(-1, 3).astype(np.float32)
+    # Normals from central differences of the height grid.
+    gy, gx = np.gradient(Y, zs, xs)
+    n = np.stack([-gx, np.ones_like(Y), -gy], axis=-1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    nrm = n.reshape(-1, 3).astype(np.float32)
+    i = np.arange(nx)[None, :]
+    j = np.arange(nz)[:, None]
+    v00 = j * (nx + 1) + i
+    v10 = v00 + 1
+    v01 = v00 + (nx + 1)
+    v11 = v01 + 1
+    t0 = np.stack([v00, v01, v11], axis=-1)
+    t1 = np.stack([v00, v11, v10], axis=-1)
+    idx = np.stack([t0, t1], axis=2).reshape(-1, 3).astype(np.uint32)
+    uv = np.stack([(X - X.min()) / ex, (Z - Z.min()) / ez], axis=-1).reshape(-1, 2).astype(np.float32)
+    return idx, pts, nrm, uv
+
+
+def icosphere(level):
+    """Unit icosphere with 20 * 4**level faces (level 2: 320, level 3: 1280)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    pts = np.array(verts, np.float32)
+    return np.array(faces, np.uint32), pts, pts.copy()
+
+
+def translate_scale(center, radius):
+    m = np.eye(4, dtype=np.float32)
+    m[0, 0] = m[1, 1] = m[2, 2] = radius
+    m[:3, 3] = center
+    return m
+
+
+def sphere_field(n, level, seed=1, radius=(0.1, 0.5), box=5.0):
+    """n instances of one icosphere: radius U[r0, r1], centre U[-box, box]^3."""
+    rng = np.random.default_rng(seed)
+    radii = rng.uniform(radius[0], radius[1], n)
+    centers = rng.uniform(-box, box, (n, 3))
+    return icosphere(level), [translate_scale(c, r) for c, r in zip(centers, radii)]
+
+
+def ground_quad(size=6.0, y=-5.0):
+    pts = np.array([[-size, y, -size], [size, y, -size], [size, y, size], [-size, y, size]], np.float32)
+    idx = np.array([[0, 2, 1], [0, 3, 2]], np.uint32)
+    nrm = np.tile(np.array([[0, 1, 0]], np.float32), (4, 1))
+    return idx, pts, nrm
+
+
+# ---- rays -------------------------------------------------------------------------------------
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+
+
+def look_at_basis(eye, target, up=(0.0, 1.0, 0.0)):
+    f = np.asarray(target, np.float64) - np.asarray(eye, np.float64)
+    f /= np.linalg.norm(f)
+    r = np.cross(f, up)
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    return f, r, u
+
+
+def primary_rays(width, height, spp, seed, eye=(0.0, 12.0, -7.0), target=(0.0, -1.0, 0.8), fov_deg=30.0):
+    """Pinhole camera, `spp` uniformly jittered samples per pixel, pixel-major order."""
+    rng = np.random.default_rng(seed)
+    f, r, u = look_at_basis(eye, target)
+    tan = np.tan(np.radians(fov_deg) / 2)
+    aspect = width / height
+    px = np.repeat(np.arange(width * height), spp)
+    jx = rng.random(px.size)
+    jy = rng.random(px.size)
+    x = ((px % width) + jx) / width * 2 - 1
+    y = 1 - ((px // width) + jy) / height * 2
+    d = f[None, :] + (x * tan * aspect)[:, None] * r[None, :] + (y * tan)[:, None] * u[None, :]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(px.size, RAY_DTYPE)
+    rays["o"] = np.asarray(eye, np.float32)
+    rays["d"] = d.astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
+def _onb(n):
+    """Orthonormal basis around unit normals n (Duff et al. 2017)."""
+    s = np.where(n[:, 2] >= 0, 1.0, -1.0)
+    a = -1.0 / (s + n[:, 2])
+    b = n[:, 0] * n[:, 1] * a
+    t = np.stack([1 + s * n[:, 0] ** 2 * a, s * b, -s * n[:, 0]], axis=1)
+    bt = np.stack([b, s + n[:, 1] ** 2 * a, -n[:, 1]], axis=1)
+    return t, bt
+
+
+def bounce_rays(idx, pts, count, seed, offset=1e-4):
+    """Cosine-distributed rays about the upward geometric normal at uniformly area-sampled points."""
+    rng = np.random.default_rng(seed)
+    p0, p1, p2 = (pts[idx[:, k]].astype(np.float64) for k in range(3))
+    cr = np.cross(p1 - p0, p2 - p0)
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    cdf = np.cumsum(area)
+    tri = np.searchsorted(cdf, rng.random(count) * cdf[-1], side="right")
+    tri = np.minimum(tri, idx.shape[0] - 1)
+    r1 = np.sqrt(rng.random(count))
+    r2 = rng.random(count)
+    b0 = 1 - r1
+    b1 = r1 * (1 - r2)
+    b2 = r1 * r2
+    p = b0[:, None] * p0[tri] + b1[:, None] * p1[tri] + b2[:, None] * p2[tri]
+    n = cr[tri] / np.linalg.norm(cr[tri], axis=1, keepdims=True)
+    n = np.where(n[:, 1:2] < 0, -n, n)
+    u1 = rng.random(count)
+    u2 = rng.random(count)
+    rad = np.sqrt(u1)
+    phi = 2 * np.pi * u2
+    lx, ly, lz = rad * np.cos(phi), rad * np.sin(phi), np.sqrt(np.maximum(0.0, 1 - u1))
+    t, bt = _onb(n)
+    d = lx[:, None] * t + ly[:, None] * bt + lz[:, None] * n
+    rays = np.zeros(count, RAY_DTYPE)
+    rays["o"] = (p + offset * n).astype(np.float32)
+    rays["d"] = d.astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
+def shadow_rays(rays, hits, light=(3.0, 10.0, 2.0), pull=1e-3):
+    """One segment ray per closest hit toward a point light: o = hit point pulled back along the
+    incoming ray, d = light - o (unnormalised), t in (1e-4, 1 - 1e-4). Misses get a zero-length ray."""
+    hit = np.isfinite(hits["t"])
+    t = np.where(hit, hits["t"], 0.0).astype(np.float64)
+    o = rays["o"].astype(np.float64)
+    d = rays["d"].astype(np.float64)
+    p = o + (t * (1.0 - pull))[:, None] * d
+    out = np.zeros(rays.size, RAY_DTYPE)
+    out["o"] = p.astype(np.float32)
+    out["d"] = np.where(hit[:, None], np.asarray(light)[None, :] - p, 0.0).astype(np.float32)
+    out["tmin"] = 1e-4
+    out["tmax"] = 1.0 - 1e-4
+    return out
+
+
+def traversal_workload(n_coherent=4194304, n_incoherent=4194304, width=1024, spp=8):
+    """The 1M-triangle heightfield and its coherent + incoherent ray sets (seeds 3 and 4)."""
+    idx, pts, nrm, uv = heightfield(seed=3)
+    height = max(1, n_coherent // (width * spp))
+    coh = primary_rays(width, height, spp, seed=5)[:n_coherent]
+    inc = bounce_rays(idx, pts, n_incoherent, seed=4)
+    return (idx, pts, nrm, uv), coh, inc

@@ -1,0 +1,192 @@
+"""CPU oracle (ctypes over liboracle.so). TEST INFRASTRUCTURE ONLY.
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg, as the checker or
+the timed reference baseline; the product (myrt_amd) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+AABB_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("centroid", "<f4", 3), ("has_centroid", "<u4")])
+NODE_DTYPE = np.dtype([("min", "<f4", 3), ("type_and_parent", "<u4"), ("max", "<f4", 3), ("first", "<u4"),
+                       ("second", "<u4"), ("pad", "<u4", 3)])
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("b1", "<f4"), ("b2", "<f4")])
+
+
+class _Scene(ctypes.Structure):
+    _fields_ = [("idx", ctypes.c_void_p), ("pos", ctypes.c_void_p), ("geo_idx_off", ctypes.c_void_p),
+                ("geo_tri_cnt", ctypes.c_void_p), ("geo_vtx_off", ctypes.c_void_p), ("n_geoms", ctypes.c_uint32),
+                ("inst_geom", ctypes.c_void_p), ("inst_inv", ctypes.c_void_p), ("n_inst", ctypes.c_uint32)]
+
+
+_lib = None
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("liboracle.so is not built (run python oracle/build.py)")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "orc_triangle_bounds": ([_P, _SZ, _P, _P], None),
+            "orc_bvh_build": ([_P, _SZ, ctypes.c_int, ctypes.POINTER(_P)], ctypes.c_int),
+            "orc_bvh_sizes": ([_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], None),
+            "orc_bvh_copy": ([_P, _P, _P, _P], None),
+            "orc_bvh_destroy": ([_P], None),
+            "orc_invert": ([_P, _P], ctypes.c_int),
+            "orc_transform_box": ([_P, _P, _P, _P, _P], None),
+            "orc_watertight": ([_P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+            "orc_moller_trumbore": ([_P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+            "orc_trace_brute": ([ctypes.POINTER(_Scene), _P, _SZ, ctypes.c_int, _P, ctypes.c_int], None),
+            "orc_accel_build": ([ctypes.POINTER(_Scene), ctypes.POINTER(_P)], ctypes.c_int),
+            "orc_accel_trace": ([_P, _P, _SZ, ctypes.c_int, _P, ctypes.c_int], None),
+            "orc_accel_destroy": ([_P], None),
+            "orc_reference_trace": ([_P, _P, _SZ, ctypes.c_int, _P], None),
+            "orc_reference_time": ([_P, _P, _SZ, ctypes.c_int, ctypes.c_int, _SZ, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int)], ctypes.c_double),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def triangle_bounds(indices, points):
+    idx = np.ascontiguousarray(indices, np.uint32).ravel()
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    out = np.zeros(idx.size // 3, AABB_DTYPE)
+    lib().orc_triangle_bounds(_p(idx), idx.size, _p(pts), _p(out))
+    return out
+
+
+def bvh_build(boxes, split_aabbs):
+    """Literal restatement of bvh(aabbs, split): returns (nodes, indices, final_boxes)."""
+    boxes = np.ascontiguousarray(boxes, AABB_DTYPE)
+    h = ctypes.c_void_p()
+    if lib().orc_bvh_build(_p(boxes), boxes.size, 1 if split_aabbs else 0, ctypes.byref(h)) != 0:
+        raise RuntimeError("oracle bvh build failed")
+    nn, ni = ctypes.c_size_t(), ctypes.c_size_t()
+    lib().orc_bvh_sizes(h, ctypes.byref(nn), ctypes.byref(ni))
+    nodes = np.zeros(nn.value, NODE_DTYPE)
+    idx = np.zeros(ni.value, np.uint32)
+    fb = np.zeros(ni.value, AABB_DTYPE)
+    lib().orc_bvh_copy(h, _p(nodes), _p(idx), _p(fb))
+    lib().orc_bvh_destroy(h)
+    return nodes, idx, fb
+
+
+def invert(m4):
+    """Rows 0-2 of the inverse of a 4x4 matrix m4[r][c] (float32)."""
+    col = np.ascontiguousarray(np.asarray(m4, np.float32).T).ravel()
+    out = np.zeros(12, np.float32)
+    if not lib().orc_invert(_p(col), _p(out)):
+        raise ValueError("singular matrix")
+    return out
+
+
+def transform_box(bmin, bmax, m4):
+    col = np.ascontiguousarray(np.asarray(m4, np.float32).T).ravel()
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    lib().orc_transform_box(_p(np.asarray(bmin, np.float32)), _p(np.asarray(bmax, np.float32)), _p(col), _p(lo),
+                            _p(hi))
+    return lo, hi
+
+
+def watertight(o, d, v0, v1, v2):
+    args = [np.ascontiguousarray(x, np.float32) for x in (o, d, v0, v1, v2)]
+    t, b1, b2 = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    hit = lib().orc_watertight(*[_p(a) for a in args], ctypes.byref(t), ctypes.byref(b1), ctypes.byref(b2))
+    return bool(hit), t.value, b1.value, b2.value
+
+
+def moller_trumbore(o, d, v1, v2, v3):
+    args = [np.ascontiguousarray(x, np.float32) for x in (o, d, v1, v2, v3)]
+    t, bx, by = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    hit = lib().orc_moller_trumbore(*[_p(a) for a in args], ctypes.byref(t), ctypes.byref(bx), ctypes.byref(by))
+    return bool(hit), t.value, bx.value, by.value
+
+
+class OracleScene:
+    """Flat scene arrays for the brute-force / BVH oracle queries."""
+
+    def __init__(self, geometries, instances):
+        """geometries: list of (indices, points); instances: list of (geom, 4x4 matrix)."""
+        idx, pos, io, tc, vo = [], [], [], [], []
+        n_idx = n_vtx = 0
+        for ind, pts in geometries:
+            ind = np.asarray(ind, np.uint32).ravel()
+            pts = np.asarray(pts, np.float32).reshape(-1, 3)
+            idx.append(ind)
+            pos.append(pts)
+            io.append(n_idx)
+            tc.append(ind.size // 3)
+            vo.append(n_vtx)
+            n_idx += ind.size
+            n_vtx += pts.shape[0]
+        self.idx = np.ascontiguousarray(np.concatenate(idx))
+        self.pos = np.ascontiguousarray(np.concatenate(pos))
+        self.io = np.asarray(io, np.uint32)
+        self.tc = np.asarray(tc, np.uint32)
+        self.vo = np.asarray(vo, np.uint32)
+        self.inst_geom = np.asarray([g for g, _ in instances], np.uint32)
+        self.inst_inv = np.ascontiguousarray(np.stack([invert(m) for _, m in instances])
+                                             if instances else np.zeros((1, 12), np.float32))
+        self.c = _Scene(self.idx.ctypes.data, self.pos.ctypes.data, self.io.ctypes.data, self.tc.ctypes.data,
+                        self.vo.ctypes.data, len(geometries), self.inst_geom.ctypes.data,
+                        self.inst_inv.ctypes.data, len(instances))
+        self._accel = None
+
+    def brute(self, rays, any_hit=False, threads=0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(rays.size, HIT_DTYPE)
+        lib().orc_trace_brute(ctypes.byref(self.c), _p(rays), rays.size, 1 if any_hit else 0, _p(out), threads)
+        return out
+
+    def accel(self):
+        if self._accel is None:
+            h = ctypes.c_void_p()
+            if lib().orc_accel_build(ctypes.byref(self.c), ctypes.byref(h)) != 0:
+                raise RuntimeError("oracle accel build failed")
+            self._accel = h
+        return self._accel
+
+    def bvh(self, rays, any_hit=False, threads=0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(rays.size, HIT_DTYPE)
+        lib().orc_accel_trace(self.accel(), _p(rays), rays.size, 1 if any_hit else 0, _p(out), threads)
+        return out
+
+    def reference(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(rays.size, HIT_DTYPE)
+        lib().orc_reference_trace(self.accel(), _p(rays), rays.size, 1 if any_hit else 0, _p(out))
+        return out
+
+    def reference_time(self, rays, any_hit=False, threads=0, chunk=65536, reps=5):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        used = ctypes.c_int()
+        rate = lib().orc_reference_time(self.accel(), _p(rays), rays.size, 1 if any_hit else 0, threads, chunk,
+                                        reps, ctypes.byref(used))
+        return rate, used.value
+
+    def __del__(self):
+        try:
+            if self._accel is not None:
+                lib().orc_accel_destroy(self._accel)
+        except Exception:
+            pass

@@ -1,0 +1,115 @@
+/*
+ * CPU oracle for the MI355X path-tracing core. TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may load liboracle.so,
+ * and only as the checker or the timed CPU baseline, never as part of the product path.
+ *
+ * Every function restates the reference's algorithm (johannes-braun/myrt, file:line cited at
+ * each definition) in plain single-precision C++ compiled with -ffp-contract=off.
+ */
+#ifndef MYRT_ORACLE_H
+#define MYRT_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_aabb {
+  float min[3];
+  float max[3];
+  float centroid[3];
+  uint32_t has_centroid;
+} orc_aabb;
+
+typedef struct orc_node {
+  float min[3];
+  uint32_t type_and_parent;
+  float max[3];
+  uint32_t first;
+  uint32_t second;
+  uint32_t pad[3];
+} orc_node;
+
+typedef struct orc_ray {
+  float o[3];
+  float tmin;
+  float d[3];
+  float tmax;
+} orc_ray;
+
+typedef struct orc_hit {
+  float t;
+  uint32_t prim;
+  uint32_t inst;
+  float b1, b2;
+} orc_hit;
+
+/* A scene as flat arrays: geometries (concatenated index/position arrays with per-geometry
+ * offsets) and instances (geometry id + rows 0-2 of the inverse transform). */
+typedef struct orc_scene {
+  const uint32_t* idx;        /* concatenated triangle indices (local to each geometry) */
+  const float* pos;           /* concatenated xyz positions */
+  const uint32_t* geo_idx_off; /* per geometry: first index in idx */
+  const uint32_t* geo_tri_cnt; /* per geometry: triangle count */
+  const uint32_t* geo_vtx_off; /* per geometry: first vertex in pos (in vertices) */
+  uint32_t n_geoms;
+  const uint32_t* inst_geom;   /* per instance: geometry */
+  const float* inst_inv;       /* per instance: 12 floats, rows 0-2 of M^-1 */
+  uint32_t n_inst;
+} orc_scene;
+
+/* generate_triangle_bounds (src/pathtracer/bvh.cpp:136-152). */
+void orc_triangle_bounds(const uint32_t* idx, size_t n_idx, const float* pos3, orc_aabb* out);
+/* bvh::bvh + bvh::create + bvh::split + bvh::compute_split_axis (src/pathtracer/bvh.cpp:43-116,
+ * 154-161, 356-512), literal: one global reference array with insertion and range shifting.
+ * Returns 0 on success. Sizes via orc_bvh_sizes after a build into an internal handle. */
+typedef struct orc_bvh_s* orc_bvh;
+int orc_bvh_build(const orc_aabb* boxes, size_t n, int split_aabbs, orc_bvh* out);
+void orc_bvh_sizes(orc_bvh b, size_t* n_nodes, size_t* n_indices);
+void orc_bvh_copy(orc_bvh b, orc_node* nodes, uint32_t* indices, orc_aabb* final_boxes);
+void orc_bvh_destroy(orc_bvh b);
+
+/* Inverse of a column-major 4x4 (rows 0-2 of the inverse, row-major 3x4). Returns 0 if singular. */
+int orc_invert(const float m[16], float out[12]);
+/* The instance box of scene::enqueue (src/pathtracer/scene.cpp:256-261). */
+void orc_transform_box(const float box_min[3], const float box_max[3], const float m[16], float out_min[3],
+                       float out_max[3]);
+
+/* Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013) with the FP64 fallback, in the
+ * arithmetic order of the product kernels. Returns 1 on a crossing, with t = T/det. */
+int orc_watertight(const float o[3], const float d[3], const float v0[3], const float v1[3], const float v2[3],
+                   float* t, float* b1, float* b2);
+/* ray_t::intersect(v1, v2, v3) (src/pathtracer/bvh.cpp:514-538; src/pt/intersect.glsl:3-28). */
+int orc_moller_trumbore(const float o[3], const float d[3], const float v1[3], const float v2[3],
+                        const float v3[3], float* t, float* bx, float* by);
+
+/* Closest hit (lowest (inst, prim) on equal t) or any hit by testing every triangle of every
+ * instance. any_hit != 0 writes hits[i].t = 1 for occluded rays and +inf otherwise. */
+void orc_trace_brute(const orc_scene* s, const orc_ray* rays, size_t n, int any_hit, orc_hit* hits, int threads);
+
+/* The same query over per-geometry binary BVHs from orc_bvh_build (split_aabbs) and an instance
+ * BVH (dont_split): stack traversal with a robust slab test. Same results as brute force. */
+typedef struct orc_accel_s* orc_accel;
+int orc_accel_build(const orc_scene* s, orc_accel* out);
+void orc_accel_trace(orc_accel a, const orc_ray* rays, size_t n, int any_hit, orc_hit* hits, int threads);
+void orc_accel_destroy(orc_accel a);
+
+/* The reference's own query, used as the CPU baseline: stackless parent-link traversal with
+ * 32-bit bit-stacks (src/pathtracer/bvh.cpp:237-286; src/pt/bvh.impl.glsl:8-72), the slab test
+ * of bvh.glsl:31-45, Moeller-Trumbore with eps 1e-23 / border 1e-6, first-found strict
+ * t comparisons (src/pt/pt_trace.comp:259-309). hits[i].prim/inst = 0xFFFFFFFF on a miss. */
+void orc_reference_trace(orc_accel a, const orc_ray* rays, size_t n, int any_hit, orc_hit* hits);
+/* Times orc_reference_trace under a clone of myrt::thread_pool (src/thread_pool.hpp:34-162):
+ * `threads` workers (0 = hardware_concurrency), jobs of `chunk` rays submitted with run_async and
+ * joined on their futures; one untimed warm-up pass, then the median of `reps` passes.
+ * Returns rays per second; *threads_used receives the worker count. */
+double orc_reference_time(orc_accel a, const orc_ray* rays, size_t n, int any_hit, int threads, size_t chunk,
+                          int reps, int* threads_used);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
