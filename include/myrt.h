@@ -75,8 +75,37 @@ typedef struct mrt_aabb {
   uint32_t has_centroid;
 } mrt_aabb;
 
+/* Material record, 32 B: the reference's wavefront material {albedo: vec4unorm8, ior, roughness,
+ * metallic, transmission, albedo_texture} (src/main.cpp:454-464, layout rules src/types.hpp:216-280);
+ * albedo is RGBA8 with red in the low byte (packUnorm4x8), texture 0 = none. */
+typedef struct mrt_material {
+  uint32_t albedo_rgba8;
+  float ior;
+  float roughness;
+  float metallic;
+  float transmission;
+  uint32_t texture;
+  uint32_t pad[2];
+} mrt_material;
+
+/* One-sided quad area light: corner + a*u + b*v (a, b in [0,1]) emitting `emission` (linear RGB
+ * radiance) towards normalize(cross(u, v)). Replaces the two hard-coded sphere lights of
+ * src/pt/pt_trace.comp:141-146. */
+typedef struct mrt_quad_light {
+  float corner[3];
+  float u[3];
+  float v[3];
+  float emission[3];
+} mrt_quad_light;
+
+typedef struct mrt_render_stats {
+  uint64_t samples;      /* pixel samples rendered since creation */
+  double milliseconds;   /* wall time inside mrt_render */
+} mrt_render_stats;
+
 typedef struct mrt_scene_s* mrt_scene;
 typedef struct mrt_bvh_s* mrt_bvh;
+typedef struct mrt_renderer_s* mrt_renderer;
 
 /* Counters filled by mrt_scene_prepare (scene::prepare_result_t, src/pathtracer/scene.hpp:84-91,
  * extended with the device-side sizes of the wide BVH). */
@@ -158,6 +187,17 @@ int mrt_scene_build_host(mrt_scene s, mrt_prepare_result* out);
 int mrt_scene_copy_wide(mrt_scene s, void* blas_nodes, void* tris, void* tlas_nodes, uint32_t* tlas_items,
                         void* instances);
 
+/* scene::push_material / set_parameter(material, name, value) (src/pathtracer/scene.hpp:134-140,
+ * 249-252; src/material.hpp:81-102). Parameter names: "albedo" (3 or 4 floats), "ior",
+ * "roughness", "metallic", "transmission"; a wrong count is an error (parameter.cpp:40-41). The
+ * material index is what mrt_scene_enqueue_geometry takes. */
+int mrt_scene_push_material(mrt_scene s, const mrt_material* mat, uint32_t* out_id);
+int mrt_scene_set_material(mrt_scene s, uint32_t id, const mrt_material* mat);
+int mrt_scene_set_material_param(mrt_scene s, uint32_t id, const char* name, const float* value, size_t n);
+/* Lights and the sky-gradient environment scale (src/pt/pt_color.comp:189-201). */
+int mrt_scene_set_lights(mrt_scene s, const mrt_quad_light* lights, size_t n);
+int mrt_scene_set_environment(mrt_scene s, float scale);
+
 /* ---- ray queries (GPU) ------------------------------------------------------------------- */
 /* nearest_hit / any_hit (src/pt/pt_trace.comp:311-386) over the two-level BVH
  * (src/pt/bvh.impl.glsl:8-72), with a watertight triangle test and lowest-(inst, prim) ties. */
@@ -171,6 +211,23 @@ int mrt_trace_stats_device(mrt_scene s, const mrt_ray* d_rays, size_t n, int any
 /* scene::pick (src/pathtracer/scene.cpp:273-314): closest hit of one ray; returns 1 in *hit_found
  * when something was hit. */
 int mrt_scene_pick(mrt_scene s, const mrt_ray* ray, mrt_hit* hit, int* hit_found);
+
+/* ---- wavefront renderer (sequential_pathtracer, src/pathtracer/sequential_pathtracer.hpp:26-58) */
+int mrt_renderer_create(mrt_renderer* out);
+int mrt_renderer_destroy(mrt_renderer r);
+/* set_view_matrix + set_projection_matrix (column-major like rnu::mat4); restarts accumulation. */
+int mrt_renderer_set_camera(mrt_renderer r, const float view_colmajor[16], const float proj_colmajor[16]);
+/* set_num_bounces (default 10, sequential_pathtracer.hpp:134). */
+int mrt_renderer_set_num_bounces(mrt_renderer r, int n);
+/* invalidate_counter / sample_count. */
+int mrt_renderer_invalidate(mrt_renderer r);
+int mrt_renderer_sample_count(mrt_renderer r, uint32_t* out);
+/* run(provider, scene, w, h) repeated spp times: prepares the scene, adds spp samples per pixel
+ * to the running accumulation and synchronises. Image rows start at the bottom (GL convention). */
+int mrt_render(mrt_renderer r, mrt_scene s, int width, int height, int spp);
+/* The accumulated mean radiance, width*height RGBA floats (alpha = 1). */
+int mrt_renderer_read(mrt_renderer r, float* rgba);
+int mrt_renderer_stats(mrt_renderer r, mrt_render_stats* out);
 
 #ifdef __cplusplus
 }

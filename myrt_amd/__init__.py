@@ -25,6 +25,10 @@ INSTANCE_DTYPE = np.dtype([("inv", "<f4", 12), ("blas_root", "<u4"), ("geom", "<
                            ("pad", "<u4")])
 assert WIDE_NODE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48 and INSTANCE_DTYPE.itemsize == 64
 
+MATERIAL_DTYPE = np.dtype([("albedo_rgba8", "<u4"), ("ior", "<f4"), ("roughness", "<f4"), ("metallic", "<f4"),
+                           ("transmission", "<f4"), ("texture", "<u4"), ("pad", "<u4", 2)])
+LIGHT_DTYPE = np.dtype([("corner", "<f4", 3), ("u", "<f4", 3), ("v", "<f4", 3), ("emission", "<f4", 3)])
+
 SPLIT_NONE = 0
 SPLIT_AABBS = 1
 MISS = 0xFFFFFFFF
@@ -40,6 +44,10 @@ class _PrepareResult(ctypes.Structure):
                 ("blas_wide_nodes", ctypes.c_uint32), ("blas_tri_records", ctypes.c_uint32),
                 ("tlas_wide_nodes", ctypes.c_uint32), ("max_stack_entries", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64)]
+
+
+class _RenderStats(ctypes.Structure):
+    _fields_ = [("samples", ctypes.c_uint64), ("milliseconds", ctypes.c_double)]
 
 
 class _TraceStats(ctypes.Structure):
@@ -86,6 +94,20 @@ SIGNATURES = {
     "mrt_trace_any_device": ([_P, _P, _SZ, _P, _P], ctypes.c_int),
     "mrt_trace_stats_device": ([_P, _P, _SZ, ctypes.c_int, ctypes.POINTER(_TraceStats), _P], ctypes.c_int),
     "mrt_scene_pick": ([_P, _P, _P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "mrt_scene_push_material": ([_P, _P, _U32P], ctypes.c_int),
+    "mrt_scene_set_material": ([_P, ctypes.c_uint32, _P], ctypes.c_int),
+    "mrt_scene_set_material_param": ([_P, ctypes.c_uint32, ctypes.c_char_p, _P, _SZ], ctypes.c_int),
+    "mrt_scene_set_lights": ([_P, _P, _SZ], ctypes.c_int),
+    "mrt_scene_set_environment": ([_P, ctypes.c_float], ctypes.c_int),
+    "mrt_renderer_create": ([ctypes.POINTER(_P)], ctypes.c_int),
+    "mrt_renderer_destroy": ([_P], ctypes.c_int),
+    "mrt_renderer_set_camera": ([_P, _P, _P], ctypes.c_int),
+    "mrt_renderer_set_num_bounces": ([_P, ctypes.c_int], ctypes.c_int),
+    "mrt_renderer_invalidate": ([_P], ctypes.c_int),
+    "mrt_renderer_sample_count": ([_P, _U32P], ctypes.c_int),
+    "mrt_render": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mrt_renderer_read": ([_P, _P], ctypes.c_int),
+    "mrt_renderer_stats": ([_P, ctypes.POINTER(_RenderStats)], ctypes.c_int),
 }
 
 
@@ -338,3 +360,125 @@ class Scene:
         found = ctypes.c_int()
         _check(lib().mrt_scene_pick(self.h, _ptr(r), _ptr(h), ctypes.byref(found)))
         return h[0] if found.value else None
+
+
+def pack_rgba8(rgb, a=1.0):
+    c = [int(round(min(max(float(x), 0.0), 1.0) * 255)) for x in list(rgb) + [a]][:4]
+    return c[0] | (c[1] << 8) | (c[2] << 16) | (c[3] << 24)
+
+
+def material(albedo=(0.8, 0.8, 0.8), ior=1.5, roughness=0.5, metallic=0.0, transmission=0.0):
+    m = np.zeros(1, MATERIAL_DTYPE)
+    m["albedo_rgba8"] = pack_rgba8(albedo)
+    m["ior"], m["roughness"], m["metallic"], m["transmission"] = ior, roughness, metallic, transmission
+    return m
+
+
+def quad_light(corner, u, v, emission):
+    l = np.zeros(1, LIGHT_DTYPE)
+    l["corner"], l["u"], l["v"], l["emission"] = corner, u, v, emission
+    return l
+
+
+def _scene_push_material(self, mat):
+    mat = np.ascontiguousarray(mat, MATERIAL_DTYPE)
+    out = ctypes.c_uint32()
+    _check(lib().mrt_scene_push_material(self.h, _ptr(mat), ctypes.byref(out)))
+    return out.value
+
+
+def _scene_set_material_param(self, mid, name, value):
+    v = np.atleast_1d(np.asarray(value, np.float32))
+    _check(lib().mrt_scene_set_material_param(self.h, int(mid), name.encode(), _ptr(v), v.size))
+
+
+def _scene_set_lights(self, lights):
+    lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
+    _check(lib().mrt_scene_set_lights(self.h, _ptr(lights) if lights.size else None, lights.size))
+
+
+def _scene_set_environment(self, scale):
+    _check(lib().mrt_scene_set_environment(self.h, float(scale)))
+
+
+Scene.push_material = _scene_push_material
+Scene.set_parameter = _scene_set_material_param
+Scene.set_lights = _scene_set_lights
+Scene.set_environment = _scene_set_environment
+
+
+def look_at(eye, target, up=(0.0, 1.0, 0.0)):
+    """View matrix (world -> camera, camera looks down -z), 4x4 m[r][c]."""
+    e = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - e
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = s, u, -f
+    m[:3, 3] = -m[:3, :3] @ e
+    return m.astype(np.float32)
+
+
+def perspective(fovy_deg, aspect, near=0.1, far=1000.0):
+    f = 1.0 / np.tan(np.radians(fovy_deg) / 2)
+    m = np.zeros((4, 4))
+    m[0, 0] = f / aspect
+    m[1, 1] = f
+    m[2, 2] = (far + near) / (near - far)
+    m[2, 3] = 2 * far * near / (near - far)
+    m[3, 2] = -1.0
+    return m.astype(np.float32)
+
+
+class Renderer:
+    """sequential_pathtracer (src/pathtracer/sequential_pathtracer.hpp:26-58) over the C ABI."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().mrt_renderer_create(ctypes.byref(h)))
+        self.h = h
+        self.size = (0, 0)
+
+    def close(self):
+        if self.h:
+            lib().mrt_renderer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_camera(self, view, proj):
+        v = np.ascontiguousarray(np.asarray(view, np.float32).T).ravel()
+        p = np.ascontiguousarray(np.asarray(proj, np.float32).T).ravel()
+        _check(lib().mrt_renderer_set_camera(self.h, _ptr(v), _ptr(p)))
+
+    def set_num_bounces(self, n):
+        _check(lib().mrt_renderer_set_num_bounces(self.h, int(n)))
+
+    def invalidate(self):
+        _check(lib().mrt_renderer_invalidate(self.h))
+
+    def sample_count(self):
+        out = ctypes.c_uint32()
+        _check(lib().mrt_renderer_sample_count(self.h, ctypes.byref(out)))
+        return out.value
+
+    def render(self, scene, width, height, spp=1):
+        _check(lib().mrt_render(self.h, scene.h, int(width), int(height), int(spp)))
+        self.size = (width, height)
+
+    def image(self):
+        w, h = self.size
+        out = np.zeros((h, w, 4), np.float32)
+        _check(lib().mrt_renderer_read(self.h, _ptr(out)))
+        return out
+
+    def stats(self):
+        s = _RenderStats()
+        _check(lib().mrt_renderer_stats(self.h, ctypes.byref(s)))
+        return {"samples": s.samples, "milliseconds": s.milliseconds}

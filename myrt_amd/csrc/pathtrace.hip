@@ -1,0 +1,413 @@
+// Wavefront path-tracing passes for gfx950: generate, shade and resolve. Extension and shadow
+// rays go through the traversal kernels of trace.hip between these passes.
+//
+// Replaces src/pt/pt_generate.comp:50-84, pt_color.comp:113-202, pt_ray_filter.comp:49-63 and the
+// light/shadow part of pt_trace.comp:134-226:
+//   * the sampler is counter-based: u = hash(pixel, sample, dimension), so a CPU oracle can
+//     replay exactly the same random numbers;
+//   * materials keep the reference's parameters {albedo, ior, roughness, metallic, transmission}
+//     and its Fresnel-weighted lobe choice (src/pt/pbr.glsl:164-201), with physically based
+//     lobes: GGX reflection (alpha = roughness^2, Smith G2), Lambert diffuse and a smooth
+//     dielectric transmission lobe;
+//   * quad area lights with next-event estimation and power-heuristic MIS replace the two
+//     hard-coded sphere lights;
+//   * survivors and shadow rays are appended to dense queues with one ballot and one atomic
+//     per wave instead of one atomic per ray.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pathtrace.hpp"
+
+namespace mrt {
+namespace ptk {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kDelta = 1u << 8;
+constexpr float kPi = 3.14159265358979f;
+
+struct V3 {
+  float x, y, z;
+};
+__device__ inline V3 mk(float a, float b, float c) { return {a, b, c}; }
+__device__ inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ inline V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ inline V3 normalize(V3 a) {
+  const float l = sqrtf(dot(a, a));
+  return a * (1.0f / l);
+}
+__device__ inline V3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+
+// Sampler: lowbias32 (C. Wellons' integer hash) chained over (pixel, sample, dimension).
+__device__ inline uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ inline float rnd(uint32_t pixel, uint32_t sample, uint32_t dim) {
+  const uint32_t h = mix32(mix32(mix32(pixel) + sample) + dim * 0x9E3779B9u);
+  return float(h >> 8) * (1.0f / 16777216.0f);
+}
+// Dimensions: 0-1 pixel jitter; bounce b uses 2 + 8b + {0 lobe, 1-2 direction, 3 roulette,
+// 4-5 light point, 6 light choice}.
+__device__ inline uint32_t dim_of(uint32_t bounce, uint32_t k) { return 2u + 8u * bounce + k; }
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// Appends the lanes with pred set to a queue: one ballot and one atomic per wave.
+__device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return 0;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (int(lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+  base = __shfl(base, leader);
+  return base + uint32_t(__popcll(mask & ((1ull << lane_id()) - 1ull)));
+}
+
+__device__ inline void add_pixel(float4* accum, uint32_t pixel, V3 c) {
+  float* a = reinterpret_cast<float*>(accum + pixel);
+  unsafeAtomicAdd(a + 0, c.x);
+  unsafeAtomicAdd(a + 1, c.y);
+  unsafeAtomicAdd(a + 2, c.z);
+}
+
+__device__ inline V3 unpack_rgba8(uint32_t c) {
+  return mk(float(c & 0xffu) * (1.0f / 255.0f), float((c >> 8) & 0xffu) * (1.0f / 255.0f),
+            float((c >> 16) & 0xffu) * (1.0f / 255.0f));
+}
+
+// Environment of src/pt/pt_color.comp:198 (sky gradient), scaled.
+__device__ inline V3 environment(V3 d, float scale) {
+  const float t = (d.y + 1.0f) * 0.5f;
+  return mk(0.3f + (1.0f - 0.3f) * t, 0.4f + (0.98f - 0.4f) * t, 0.5f + (0.96f - 0.5f) * t) * scale;
+}
+
+__device__ inline float ggx_d(float nh, float a2) {
+  const float den = nh * nh * (a2 - 1.0f) + 1.0f;
+  return a2 / (kPi * den * den);
+}
+__device__ inline float ggx_g1(float nx, float a2) { return 2.0f * nx / (nx + sqrtf(a2 + (1.0f - a2) * nx * nx)); }
+
+struct Lobes {
+  V3 albedo, spec_col;
+  float a2, metallic;
+  float ws, wt, wd;  // lobe weights = selection probabilities (sum to 1)
+};
+
+// Non-delta part of the BSDF and its sampling density: ws * GGX + wd * Lambert.
+__device__ inline void eval_bsdf(const Lobes& L, V3 n, V3 wo, V3 wi, V3& f, float& pdf) {
+  const float nv = dot(n, wo), nl = dot(n, wi);
+  f = mk(0, 0, 0);
+  pdf = 0.f;
+  if (nv <= 0.f || nl <= 0.f) return;
+  const V3 h = normalize(wo + wi);
+  const float nh = fmaxf(dot(n, h), 0.f), vh = fmaxf(dot(wo, h), 1e-8f);
+  const float D = ggx_d(nh, L.a2);
+  const float G = ggx_g1(nv, L.a2) * ggx_g1(nl, L.a2);
+  const float fs = D * G / (4.0f * nv * nl);
+  const float fd = (1.0f - L.metallic) / kPi;
+  f = L.spec_col * (L.ws * fs) + L.albedo * (L.wd * fd);
+  pdf = L.ws * (D * nh / (4.0f * vh)) + L.wd * (nl / kPi);
+}
+
+// Orthonormal basis around n (Duff et al. 2017).
+__device__ inline void onb(V3 n, V3& t, V3& b) {
+  const float s = n.z >= 0.f ? 1.0f : -1.0f;
+  const float a = -1.0f / (s + n.z);
+  const float c = n.x * n.y * a;
+  t = mk(1.0f + s * n.x * n.x * a, s * c, -s * n.x);
+  b = mk(c, s + n.y * n.y * a, -n.y);
+}
+
+__global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers b, uint32_t cur, uint32_t n_paths,
+                                                          uint32_t sample_base) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i == 0) b.counts[cur] = n_paths;
+  if (i >= n_paths) return;
+  const uint32_t npix = cam.width * cam.height;
+  const uint32_t pixel = i % npix;
+  const uint32_t sample = sample_base + i / npix;
+  const uint32_t x = pixel % cam.width, y = pixel / cam.width;
+  const float nx = (float(x) + rnd(pixel, sample, 0)) / float(cam.width) * 2.0f - 1.0f;
+  const float ny = (float(y) + rnd(pixel, sample, 1)) / float(cam.height) * 2.0f - 1.0f;
+  const float* m = cam.inv_view_proj;
+  const float qx = m[0] * nx + m[4] * ny + m[8] + m[12];
+  const float qy = m[1] * nx + m[5] * ny + m[9] + m[13];
+  const float qz = m[2] * nx + m[6] * ny + m[10] + m[14];
+  const float qw = m[3] * nx + m[7] * ny + m[11] + m[15];
+  V3 target = qw != 0.f ? mk(qx / qw, qy / qw, qz / qw) : mk(qx, qy, qz);
+  const V3 eye = ld3(cam.eye);
+  const V3 d = qw != 0.f ? normalize(target - eye) : normalize(target);
+  float4* r = reinterpret_cast<float4*>(b.rays[cur]) + 2 * size_t(i);
+  r[0] = make_float4(eye.x, eye.y, eye.z, 0.0f);
+  r[1] = make_float4(d.x, d.y, d.z, __int_as_float(0x7f800000));
+  PathState ps;
+  ps.thr[0] = ps.thr[1] = ps.thr[2] = 1.0f;
+  ps.pixel = pixel;
+  ps.sample = sample;
+  ps.flags = kDelta;
+  ps.last_pdf = 0.f;
+  ps.pad = 0;
+  b.paths[cur][i] = ps;
+}
+
+__global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, uint32_t cur, uint32_t max_depth) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = b.counts[cur];
+  if (blockIdx.x * kBlock >= n) return;  // whole block idle (uniform)
+  const uint32_t nxt = cur ^ 1u;
+  const bool live = i < n;
+
+  bool survive = false, shadow = false;
+  V3 new_o = mk(0, 0, 0), new_d = mk(0, 0, 1);
+  V3 sh_o = mk(0, 0, 0), sh_d = mk(0, 0, 1), sh_c = mk(0, 0, 0);
+  PathState ps{};
+  if (live) {
+    ps = b.paths[cur][i];
+    const float4* rp = reinterpret_cast<const float4*>(b.rays[cur]) + 2 * size_t(i);
+    const float4 r0 = rp[0], r1 = rp[1];
+    const V3 ro = mk(r0.x, r0.y, r0.z), rd = mk(r1.x, r1.y, r1.z);
+    const uint4 hit = reinterpret_cast<const uint4*>(b.hits)[i];
+    const bool found = hit.z != 0xFFFFFFFFu;
+    const float t_hit = found ? __uint_as_float(hit.x) : __int_as_float(0x7f800000);
+    const uint32_t bounce = ps.flags & 0xffu;
+    V3 thr = mk(ps.thr[0], ps.thr[1], ps.thr[2]);
+
+    // Emitters along the ray (lights are not in the BVH).
+    int light_hit = -1;
+    float t_light = t_hit;
+    float cos_light = 0.f;
+    for (uint32_t l = 0; l < sc.num_lights; ++l) {
+      const QuadLight& q = sc.lights[l];
+      const V3 u = ld3(q.u), v = ld3(q.v), c = ld3(q.corner);
+      const V3 nl = cross(u, v);
+      const float area = sqrtf(dot(nl, nl));
+      const V3 n1 = nl * (1.0f / area);
+      const float den = dot(n1, rd);
+      if (den >= 0.f) continue;  // one-sided: seen from behind it neither emits nor blocks
+      const float t = dot(n1, c - ro) / den;
+      if (!(t > 1e-6f && t < t_light)) continue;
+      const V3 pq = ro + rd * t - c;
+      const float a = dot(pq, u) / dot(u, u), bb = dot(pq, v) / dot(v, v);
+      if (a < 0.f || a > 1.f || bb < 0.f || bb > 1.f) continue;
+      light_hit = int(l);
+      t_light = t;
+      cos_light = -den;
+    }
+    if (light_hit >= 0) {
+      const QuadLight& q = sc.lights[light_hit];
+      float w = 1.0f;
+      if (!(ps.flags & kDelta)) {
+        const V3 nl = cross(ld3(q.u), ld3(q.v));
+        const float area = sqrtf(dot(nl, nl));
+        const float pdf_l = t_light * t_light / (cos_light * area * float(sc.num_lights));
+        w = ps.last_pdf * ps.last_pdf / (ps.last_pdf * ps.last_pdf + pdf_l * pdf_l);
+      }
+      add_pixel(b.accum, ps.pixel, thr * ld3(q.emission) * w);
+    } else if (!found) {
+      add_pixel(b.accum, ps.pixel, thr * environment(rd, sc.env_scale));
+    } else {
+      // ---- surface interaction ----
+      const uint32_t inst = hit.z, prim = hit.y;
+      const InstanceRecord& ir = sc.instances[inst];
+      const GeoInfo gi = sc.geos[ir.geom];
+      const uint32_t i0 = sc.idx[gi.idx_base + 3 * prim], i1 = sc.idx[gi.idx_base + 3 * prim + 1],
+                     i2 = sc.idx[gi.idx_base + 3 * prim + 2];
+      const float4 p0 = sc.pos[gi.vtx_base + i0], p1 = sc.pos[gi.vtx_base + i1], p2 = sc.pos[gi.vtx_base + i2];
+      const float b1 = float(hit.w & 0xffffu) * (1.0f / 65535.0f), b2 = float(hit.w >> 16) * (1.0f / 65535.0f);
+      const float b0 = 1.0f - b1 - b2;
+      const V3 e1 = mk(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z), e2 = mk(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z);
+      const V3 ngo = cross(e1, e2);
+      V3 nso = ngo;
+      if (gi.has_normals) {
+        const float4 n0 = sc.nrm[gi.vtx_base + i0], n1 = sc.nrm[gi.vtx_base + i1], n2 = sc.nrm[gi.vtx_base + i2];
+        nso = mk(b0 * n0.x + b1 * n1.x + b2 * n2.x, b0 * n0.y + b1 * n1.y + b2 * n2.y, b0 * n0.z + b1 * n1.z + b2 * n2.z);
+      }
+      const float* iv = ir.inv;  // normals transform with the transpose of M^-1
+      V3 ng = normalize(mk(iv[0] * ngo.x + iv[4] * ngo.y + iv[8] * ngo.z, iv[1] * ngo.x + iv[5] * ngo.y + iv[9] * ngo.z,
+                           iv[2] * ngo.x + iv[6] * ngo.y + iv[10] * ngo.z));
+      V3 ns = normalize(mk(iv[0] * nso.x + iv[4] * nso.y + iv[8] * nso.z, iv[1] * nso.x + iv[5] * nso.y + iv[9] * nso.z,
+                           iv[2] * nso.x + iv[6] * nso.y + iv[10] * nso.z));
+      const V3 p = ro + rd * t_hit;
+      const V3 wo = -rd;
+      const bool entering = dot(rd, ng) < 0.f;
+      if (!entering) {
+        ng = -ng;
+        ns = -ns;
+      }
+      if (dot(ns, ng) <= 0.f) ns = ng;
+      const float eps = 1e-4f * (1.0f + fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z)));
+
+      uint32_t mi = sc.inst_material[inst];
+      if (mi >= sc.num_materials) mi = 0;
+      const Material mat = sc.materials[mi];
+      Lobes L;
+      L.albedo = unpack_rgba8(mat.albedo_rgba8);
+      L.metallic = mat.metallic;
+      const float sm = sqrtf(fmaxf(mat.metallic, 0.f));
+      L.spec_col = mk(1.0f + (L.albedo.x - 1.0f) * sm, 1.0f + (L.albedo.y - 1.0f) * sm, 1.0f + (L.albedo.z - 1.0f) * sm);
+      const float alpha = fmaxf(mat.roughness * mat.roughness, 1e-3f);
+      L.a2 = alpha * alpha;
+      const float f0d = (mat.ior - 1.0f) / (mat.ior + 1.0f);
+      const float f0 = f0d * f0d + (1.0f - f0d * f0d) * mat.metallic;
+      const float cv = fminf(fmaxf(dot(ns, wo), 0.f), 1.0f);
+      const float om = 1.0f - cv;
+      const float Fv = f0 + (1.0f - f0) * (om * om * om * om * om);
+      L.ws = Fv;
+      L.wt = (1.0f - Fv) * mat.transmission;
+      L.wd = (1.0f - Fv) * (1.0f - mat.transmission);
+
+      // ---- next-event estimation toward one quad light ----
+      if (sc.num_lights > 0 && (L.ws + L.wd) > 0.f) {
+        uint32_t l = uint32_t(rnd(ps.pixel, ps.sample, dim_of(bounce, 6)) * float(sc.num_lights));
+        if (l >= sc.num_lights) l = sc.num_lights - 1;
+        const QuadLight& q = sc.lights[l];
+        const V3 u = ld3(q.u), v = ld3(q.v);
+        const float r1 = rnd(ps.pixel, ps.sample, dim_of(bounce, 4)), r2 = rnd(ps.pixel, ps.sample, dim_of(bounce, 5));
+        const V3 x = ld3(q.corner) + u * r1 + v * r2;
+        const V3 nl = cross(u, v);
+        const float area = sqrtf(dot(nl, nl));
+        V3 wi = x - p;
+        const float d2 = dot(wi, wi);
+        wi = wi * (1.0f / sqrtf(d2));
+        const float cl = -dot(nl, wi) / area;
+        if (cl > 0.f && dot(ng, wi) > 0.f) {
+          V3 f;
+          float pdf_b;
+          eval_bsdf(L, ns, wo, wi, f, pdf_b);
+          const float pdf_l = d2 / (cl * area * float(sc.num_lights));
+          const float w = pdf_l * pdf_l / (pdf_l * pdf_l + pdf_b * pdf_b);
+          const V3 c = thr * f * ld3(q.emission) * (dot(ns, wi) * w / pdf_l);
+          if (c.x > 0.f || c.y > 0.f || c.z > 0.f) {
+            shadow = true;
+            sh_o = p + ng * eps;
+            sh_d = x - sh_o;
+            sh_c = c;
+          }
+        }
+      }
+
+      // ---- sample the next direction ----
+      const float ul = rnd(ps.pixel, ps.sample, dim_of(bounce, 0));
+      const float u1 = rnd(ps.pixel, ps.sample, dim_of(bounce, 1)), u2 = rnd(ps.pixel, ps.sample, dim_of(bounce, 2));
+      bool ok = true;
+      V3 wi;
+      V3 origin = p + ng * eps;
+      if (ul >= L.ws && ul < L.ws + L.wt) {
+        // Smooth dielectric transmission (delta lobe), total internal reflection as a mirror.
+        const float eta = entering ? 1.0f / mat.ior : mat.ior;
+        const float ci = fminf(fmaxf(dot(wo, ns), 0.f), 1.0f);
+        const float s2 = eta * eta * (1.0f - ci * ci);
+        if (s2 > 1.0f) {
+          wi = ns * (2.0f * ci) - wo;
+        } else {
+          wi = normalize(-wo * eta + ns * (eta * ci - sqrtf(1.0f - s2)));
+          origin = p - ng * eps;
+        }
+        thr = thr * L.albedo;
+        ps.flags |= kDelta;
+      } else {
+        if (ul < L.ws) {
+          const float cos_t = sqrtf((1.0f - u1) / (1.0f + (L.a2 - 1.0f) * u1));
+          const float sin_t = sqrtf(fmaxf(0.f, 1.0f - cos_t * cos_t));
+          const float phi = 2.0f * kPi * u2;
+          V3 t, bt;
+          onb(ns, t, bt);
+          const V3 h = t * (sin_t * cosf(phi)) + bt * (sin_t * sinf(phi)) + ns * cos_t;
+          wi = h * (2.0f * dot(wo, h)) - wo;
+        } else {
+          const float rr = sqrtf(u1), phi = 2.0f * kPi * u2;
+          V3 t, bt;
+          onb(ns, t, bt);
+          wi = t * (rr * cosf(phi)) + bt * (rr * sinf(phi)) + ns * sqrtf(fmaxf(0.f, 1.0f - u1));
+        }
+        V3 f;
+        float pdf;
+        eval_bsdf(L, ns, wo, wi, f, pdf);
+        if (dot(ng, wi) <= 0.f || pdf <= 0.f) {
+          ok = false;
+        } else {
+          thr = thr * f * (dot(ns, wi) / pdf);
+          ps.last_pdf = pdf;
+          ps.flags &= ~kDelta;
+        }
+      }
+      if (ok && bounce >= 3) {
+        const float q = fminf(0.95f, fmaxf(fmaxf(thr.x, thr.y), thr.z));
+        if (rnd(ps.pixel, ps.sample, dim_of(bounce, 3)) >= q)
+          ok = false;
+        else
+          thr = thr * (1.0f / q);
+      }
+      if (ok && bounce + 1 < max_depth && (thr.x > 0.f || thr.y > 0.f || thr.z > 0.f)) {
+        survive = true;
+        new_o = origin;
+        new_d = wi;
+        ps.thr[0] = thr.x;
+        ps.thr[1] = thr.y;
+        ps.thr[2] = thr.z;
+        ps.flags = (ps.flags & ~0xffu) | (bounce + 1);
+      }
+    }
+  }
+  const uint32_t slot = wave_append(&b.counts[nxt], survive);
+  if (survive) {
+    float4* r = reinterpret_cast<float4*>(b.rays[nxt]) + 2 * size_t(slot);
+    r[0] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
+    r[1] = make_float4(new_d.x, new_d.y, new_d.z, __int_as_float(0x7f800000));
+    b.paths[nxt][slot] = ps;
+  }
+  const uint32_t sslot = wave_append(&b.counts[2], shadow);
+  if (shadow) {
+    float4* r = reinterpret_cast<float4*>(b.shadow_rays) + 2 * size_t(sslot);
+    r[0] = make_float4(sh_o.x, sh_o.y, sh_o.z, 0.0f);
+    r[1] = make_float4(sh_d.x, sh_d.y, sh_d.z, 1.0f - 1e-3f);
+    ShadowRecord rec;
+    rec.contrib[0] = sh_c.x;
+    rec.contrib[1] = sh_c.y;
+    rec.contrib[2] = sh_c.z;
+    rec.pixel = ps.pixel;
+    b.shadow[sslot] = rec;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void resolve_kernel(PtBuffers b) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= b.counts[2]) return;
+  if (b.occluded[i]) return;
+  const ShadowRecord r = b.shadow[i];
+  add_pixel(b.accum, r.pixel, mk(r.contrib[0], r.contrib[1], r.contrib[2]));
+}
+
+}  // namespace ptk
+
+int launch_pt_generate(const Camera& cam, const PtBuffers& b, uint32_t cur, uint32_t n_paths, uint32_t sample_base,
+                       hipStream_t stream) {
+  const uint32_t grid = (n_paths + ptk::kBlock - 1) / ptk::kBlock;
+  hipLaunchKernelGGL(ptk::generate_kernel, dim3(grid ? grid : 1), dim3(ptk::kBlock), 0, stream, cam, b, cur, n_paths,
+                     sample_base);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_pt_shade(const PtScene& sc, const PtBuffers& b, uint32_t cur, uint32_t max_depth, hipStream_t stream) {
+  const uint32_t grid = (b.capacity + ptk::kBlock - 1) / ptk::kBlock;
+  hipLaunchKernelGGL(ptk::shade_kernel, dim3(grid), dim3(ptk::kBlock), 0, stream, sc, b, cur, max_depth);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_pt_resolve(const PtBuffers& b, hipStream_t stream) {
+  const uint32_t grid = (b.capacity + ptk::kBlock - 1) / ptk::kBlock;
+  hipLaunchKernelGGL(ptk::resolve_kernel, dim3(grid), dim3(ptk::kBlock), 0, stream, b);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace mrt

@@ -1,0 +1,95 @@
+// Wavefront path tracer: records and kernel launchers shared by renderer.cpp and pathtrace.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_types.hpp"
+
+namespace mrt {
+
+// Material record, 32 B: the reference's wavefront material {albedo: vec4unorm8, ior, roughness,
+// metallic, transmission, albedo_texture} (src/main.cpp:454-464) with the texture handle kept as a
+// plain id (0 = none).
+struct Material {
+  uint32_t albedo_rgba8;
+  float ior;
+  float roughness;
+  float metallic;
+  float transmission;
+  uint32_t texture;
+  uint32_t pad[2];
+};
+static_assert(sizeof(Material) == 32, "material record is 32 bytes");
+
+// One-sided quad area light: points corner + a*u + b*v, a, b in [0, 1], emitting Le towards
+// normalize(cross(u, v)).
+struct QuadLight {
+  float corner[3];
+  float u[3];
+  float v[3];
+  float emission[3];
+};
+
+// Per-geometry attribute bases into the concatenated vertex/index arrays.
+struct GeoInfo {
+  uint32_t idx_base;   // first index (3 per triangle)
+  uint32_t vtx_base;   // first vertex
+  uint32_t has_normals;
+  uint32_t pad;
+};
+
+// Path state, 32 B per live path (ray in a separate 32-B mrt_ray array).
+struct PathState {
+  float thr[3];
+  uint32_t pixel;
+  uint32_t sample;
+  uint32_t flags;     // bits 0-7: bounce; bit 8: previous vertex was the camera or a delta lobe
+  float last_pdf;     // solid-angle pdf of the previous non-delta bounce (for MIS)
+  uint32_t pad;
+};
+static_assert(sizeof(PathState) == 32, "path state is 32 bytes");
+
+struct ShadowRecord {
+  float contrib[3];
+  uint32_t pixel;
+};
+
+struct Camera {
+  float inv_view_proj[16];  // column-major inverse(projection * view)... stored as V^-1 * P^-1
+  float eye[3];
+  uint32_t width, height;
+};
+
+struct PtScene {
+  const InstanceRecord* instances;
+  const uint32_t* inst_material;
+  const GeoInfo* geos;
+  const uint32_t* idx;
+  const float4* pos;
+  const float4* nrm;
+  const Material* materials;
+  const QuadLight* lights;
+  uint32_t num_lights;
+  uint32_t num_materials;
+  float env_scale;
+};
+
+struct PtBuffers {
+  void* rays[2];          // mrt_ray queues
+  PathState* paths[2];
+  void* hits;             // mrt_hit
+  void* shadow_rays;      // mrt_ray
+  ShadowRecord* shadow;
+  uint8_t* occluded;
+  uint32_t* counts;       // [0], [1]: path queues; [2]: shadow queue
+  float4* accum;          // per pixel running sum (rgb); w unused
+  uint32_t capacity;
+};
+
+int launch_pt_generate(const Camera& cam, const PtBuffers& b, uint32_t cur, uint32_t n_paths, uint32_t sample_base,
+                       hipStream_t stream);
+int launch_pt_shade(const PtScene& sc, const PtBuffers& b, uint32_t cur, uint32_t max_depth, hipStream_t stream);
+int launch_pt_resolve(const PtBuffers& b, hipStream_t stream);
+
+}  // namespace mrt

@@ -1,0 +1,385 @@
+// Wavefront renderer host (the sequential_pathtracer of src/pathtracer/sequential_pathtracer.*)
+// and the scene's material / light entry points.
+//
+// One render pass per sample batch: generate -> for each bounce {extend (closest hit), shade
+// (NEE + BSDF sampling + compaction into the next queue), connect (any hit on shadow rays),
+// resolve}. Queue sizes stay on the device between passes, like the reference's indirect
+// dispatch (src/pt/pt_ray_filter.comp:58-62); the host issues launches only.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/myrt.h"
+#include "pathtrace.hpp"
+#include "scene_internal.hpp"
+
+namespace mrt {
+namespace {
+
+// 4x4 column-major helpers (rnu::mat4 layout).
+void mat_mul(const double a[16], const double b[16], double out[16]) {
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      double s = 0;
+      for (int k = 0; k < 4; ++k) s += a[k * 4 + r] * b[c * 4 + k];
+      out[c * 4 + r] = s;
+    }
+}
+
+bool mat_inv(const double m[16], double out[16]) {
+  double inv[16];
+  inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] -
+           m[13] * m[7] * m[10];
+  inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+           m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] -
+           m[12] * m[7] * m[9];
+  inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+            m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+           m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] -
+           m[12] * m[3] * m[10];
+  inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+           m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+            m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] -
+           m[13] * m[3] * m[6];
+  inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] +
+           m[12] * m[3] * m[6];
+  inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] -
+            m[12] * m[3] * m[5];
+  inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+            m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] +
+           m[9] * m[3] * m[6];
+  inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] -
+           m[8] * m[3] * m[6];
+  inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] +
+            m[8] * m[3] * m[5];
+  inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] -
+            m[8] * m[2] * m[5];
+  double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+  if (det == 0.0 || !std::isfinite(det)) return false;
+  for (int i = 0; i < 16; ++i) out[i] = inv[i] / det;
+  return true;
+}
+
+template <typename T>
+int ensure(T*& p, size_t& cap, size_t count) {
+  if (p && cap >= count) return MRT_OK;
+  if (p) hipFree(p);
+  p = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc renderer buffer");
+  cap = count;
+  return MRT_OK;
+}
+
+}  // namespace
+
+struct Renderer {
+  float view[16], proj[16];
+  Camera cam{};
+  int num_bounces = 10;      // sequential_pathtracer.hpp:134
+  uint32_t sample_count = 0;
+  uint32_t width = 0, height = 0;
+  size_t cap_paths = 0, cap_accum = 0;
+  void* rays[2] = {nullptr, nullptr};
+  PathState* paths[2] = {nullptr, nullptr};
+  void *hits = nullptr, *shadow_rays = nullptr;
+  ShadowRecord* shadow = nullptr;
+  uint8_t* occluded = nullptr;
+  uint32_t* counts = nullptr;
+  float4* accum = nullptr;
+  size_t c_rays0 = 0, c_rays1 = 0, c_paths0 = 0, c_paths1 = 0, c_hits = 0, c_shr = 0, c_sh = 0, c_occ = 0, c_cnt = 0;
+  hipStream_t stream = nullptr;
+  mrt_render_stats stats{};
+
+  Renderer() {
+    std::memset(view, 0, sizeof(view));
+    std::memset(proj, 0, sizeof(proj));
+    for (int i = 0; i < 4; ++i) view[i * 5] = proj[i * 5] = 1.f;
+  }
+  ~Renderer() {
+    for (void* p : {rays[0], rays[1], (void*)paths[0], (void*)paths[1], hits, shadow_rays, (void*)shadow,
+                    (void*)occluded, (void*)counts, (void*)accum})
+      if (p) hipFree(p);
+  }
+
+  int update_camera() {
+    double v[16], pj[16], vinv[16], pinv[16], m[16];
+    for (int i = 0; i < 16; ++i) {
+      v[i] = view[i];
+      pj[i] = proj[i];
+    }
+    if (!mat_inv(v, vinv) || !mat_inv(pj, pinv)) return fail(MRT_ERR_INVALID, "camera matrix is not invertible");
+    mat_mul(vinv, pinv, m);
+    for (int i = 0; i < 16; ++i) cam.inv_view_proj[i] = float(m[i]);
+    cam.eye[0] = float(vinv[12]);
+    cam.eye[1] = float(vinv[13]);
+    cam.eye[2] = float(vinv[14]);
+    return MRT_OK;
+  }
+
+  int render(Scene& s, uint32_t w, uint32_t h, uint32_t spp) {
+    int rc = update_camera();
+    if (rc != MRT_OK) return rc;
+    if (w == 0 || h == 0) return fail(MRT_ERR_INVALID, "empty image");
+    if (uint64_t(w) * h > (1u << 26)) return fail(MRT_ERR_LIMIT, "image larger than 64M pixels");
+    // scene.prepare() at the start of every run (sequential_pathtracer.cpp:45); a change
+    // restarts the progressive accumulation (:47-53).
+    mrt_prepare_result pr{};
+    const bool changed_before = s.geoms_dirty || s.inst_dirty || !s.pending.empty() || s.shading_dirty;
+    rc = mrt_scene_prepare(reinterpret_cast<mrt_scene>(&s), &pr);
+    if (rc != MRT_OK) return rc;
+    const size_t npix = size_t(w) * h;
+    if (w != width || h != height || changed_before) {
+      width = w;
+      height = h;
+      sample_count = 0;
+    }
+    uint32_t batch = uint32_t(std::max<size_t>(1, std::min<size_t>(spp, (size_t(1) << 21) / npix)));
+    const size_t cap = npix * batch;
+    if ((rc = ensure(reinterpret_cast<char*&>(rays[0]), c_rays0, cap * 32)) ||
+        (rc = ensure(reinterpret_cast<char*&>(rays[1]), c_rays1, cap * 32)) || (rc = ensure(paths[0], c_paths0, cap)) ||
+        (rc = ensure(paths[1], c_paths1, cap)) || (rc = ensure(reinterpret_cast<char*&>(hits), c_hits, cap * 16)) ||
+        (rc = ensure(reinterpret_cast<char*&>(shadow_rays), c_shr, cap * 32)) || (rc = ensure(shadow, c_sh, cap)) ||
+        (rc = ensure(occluded, c_occ, cap)) || (rc = ensure(counts, c_cnt, 16)))
+      return rc;
+    if (!accum || cap_accum < npix) {
+      if ((rc = ensure(accum, cap_accum, npix)) != MRT_OK) return rc;
+      sample_count = 0;
+    }
+    if (sample_count == 0) {
+      hipError_t e = hipMemsetAsync(accum, 0, npix * sizeof(float4), stream);
+      if (e != hipSuccess) return hip_fail(e, "clearing the accumulator");
+    }
+    cam.width = w;
+    cam.height = h;
+    PtBuffers b{};
+    b.rays[0] = rays[0];
+    b.rays[1] = rays[1];
+    b.paths[0] = paths[0];
+    b.paths[1] = paths[1];
+    b.hits = hits;
+    b.shadow_rays = shadow_rays;
+    b.shadow = shadow;
+    b.occluded = occluded;
+    b.counts = counts;
+    b.accum = accum;
+    const PtScene sc = s.shading_view();
+    const auto t0 = std::chrono::steady_clock::now();
+    uint32_t done = 0;
+    while (done < spp) {
+      const uint32_t nb = std::min(batch, spp - done);
+      const uint32_t n_paths = uint32_t(npix * nb);
+      b.capacity = n_paths;
+      uint32_t cur = 0;
+      if (launch_pt_generate(cam, b, cur, n_paths, sample_count + done, stream)) return fail(MRT_ERR_HIP, "generate");
+      for (int bounce = 0; bounce < num_bounces; ++bounce) {
+        // counts[cur] is live; clear the next queue and the shadow queue.
+        hipError_t e = hipMemsetAsync(counts + (cur ^ 1u), 0, 4, stream);
+        if (e == hipSuccess) e = hipMemsetAsync(counts + 2, 0, 4, stream);
+        if (e != hipSuccess) return hip_fail(e, "clearing queue counters");
+        if ((rc = s.trace(rays[cur], n_paths, hits, false, false, stream, counts + cur)) != MRT_OK) return rc;
+        if (launch_pt_shade(sc, b, cur, uint32_t(num_bounces), stream)) return fail(MRT_ERR_HIP, "shade");
+        if ((rc = s.trace(shadow_rays, n_paths, occluded, true, false, stream, counts + 2)) != MRT_OK) return rc;
+        if (launch_pt_resolve(b, stream)) return fail(MRT_ERR_HIP, "resolve");
+        cur ^= 1u;
+      }
+      done += nb;
+    }
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "render");
+    uint32_t flags = 0;
+    if ((rc = s.read_ctrl(&flags, nullptr, stream)) != MRT_OK) return rc;
+    if (flags & 1u) return fail(MRT_ERR_INTERNAL, "traversal stack overflow");
+    const auto t1 = std::chrono::steady_clock::now();
+    sample_count += spp;
+    stats.samples += uint64_t(spp) * npix;
+    stats.milliseconds += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return MRT_OK;
+  }
+
+  int read(float* rgba) {
+    if (!accum || sample_count == 0) return fail(MRT_ERR_INVALID, "nothing rendered yet");
+    const size_t npix = size_t(width) * height;
+    std::vector<float4> host(npix);
+    hipError_t e = hipMemcpy(host.data(), accum, npix * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "reading the accumulator");
+    const float inv = 1.0f / float(sample_count);
+    for (size_t i = 0; i < npix; ++i) {
+      rgba[4 * i + 0] = host[i].x * inv;
+      rgba[4 * i + 1] = host[i].y * inv;
+      rgba[4 * i + 2] = host[i].z * inv;
+      rgba[4 * i + 3] = 1.0f;
+    }
+    return MRT_OK;
+  }
+};
+
+}  // namespace mrt
+
+using namespace mrt;
+
+#define MRT_GUARD_BEGIN try {
+#define MRT_GUARD_END                                 \
+  }                                                   \
+  catch (const std::exception& ex) {                  \
+    return fail(MRT_ERR_INTERNAL, ex.what());         \
+  }
+
+extern "C" {
+
+int mrt_scene_push_material(mrt_scene sh, const mrt_material* mat, uint32_t* out) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !mat) return fail(MRT_ERR_INVALID, "null argument");
+  static_assert(sizeof(mrt_material) == sizeof(Material), "material layouts differ");
+  Material m;
+  std::memcpy(&m, mat, sizeof(m));
+  s->materials.push_back(m);
+  s->shading_dirty = true;
+  if (out) *out = uint32_t(s->materials.size() - 1);
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_set_material(mrt_scene sh, uint32_t id, const mrt_material* mat) {
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !mat) return fail(MRT_ERR_INVALID, "null argument");
+  if (id >= s->materials.size()) return fail(MRT_ERR_INVALID, "unknown material");
+  std::memcpy(&s->materials[id], mat, sizeof(Material));
+  s->shading_dirty = true;
+  return MRT_OK;
+}
+
+int mrt_scene_set_material_param(mrt_scene sh, uint32_t id, const char* name, const float* value, size_t n) {
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !name || !value) return fail(MRT_ERR_INVALID, "null argument");
+  if (id >= s->materials.size()) return fail(MRT_ERR_INVALID, "unknown material");
+  Material& m = s->materials[id];
+  const std::string k(name);
+  if (k == "albedo") {
+    // vec4unorm8 parameter (src/types.hpp:66-68): 3 or 4 floats in [0, 1] packed to RGBA8.
+    if (n != 3 && n != 4) return fail(MRT_ERR_INVALID, "albedo takes 3 or 4 values");
+    uint32_t packed = 0;
+    for (size_t c = 0; c < 4; ++c) {
+      const float v = c < n ? value[c] : 1.0f;
+      const uint32_t q = uint32_t(std::lround(std::fmin(std::fmax(v, 0.f), 1.f) * 255.f));
+      packed |= q << (8 * c);
+    }
+    m.albedo_rgba8 = packed;
+  } else if (n != 1) {
+    return fail(MRT_ERR_INVALID, "parameter '" + k + "' takes one value");  // parameter.cpp:40-41
+  } else if (k == "ior") {
+    m.ior = value[0];
+  } else if (k == "roughness") {
+    m.roughness = value[0];
+  } else if (k == "metallic") {
+    m.metallic = value[0];
+  } else if (k == "transmission") {
+    m.transmission = value[0];
+  } else {
+    return fail(MRT_ERR_INVALID, "unknown material parameter '" + k + "'");
+  }
+  s->shading_dirty = true;
+  return MRT_OK;
+}
+
+int mrt_scene_set_lights(mrt_scene sh, const mrt_quad_light* lights, size_t n) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || (n && !lights)) return fail(MRT_ERR_INVALID, "null argument");
+  static_assert(sizeof(mrt_quad_light) == sizeof(QuadLight), "light layouts differ");
+  s->lights.assign(reinterpret_cast<const QuadLight*>(lights), reinterpret_cast<const QuadLight*>(lights) + n);
+  s->shading_dirty = true;
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_set_environment(mrt_scene sh, float scale) {
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s) return fail(MRT_ERR_INVALID, "null scene");
+  s->env_scale = scale;
+  s->shading_dirty = true;
+  return MRT_OK;
+}
+
+int mrt_renderer_create(mrt_renderer* out) {
+  MRT_GUARD_BEGIN
+  if (!out) return fail(MRT_ERR_INVALID, "null output");
+  *out = reinterpret_cast<mrt_renderer>(new Renderer);
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_renderer_destroy(mrt_renderer r) {
+  delete reinterpret_cast<Renderer*>(r);
+  return MRT_OK;
+}
+
+int mrt_renderer_set_camera(mrt_renderer rh, const float view_colmajor[16], const float proj_colmajor[16]) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || !view_colmajor || !proj_colmajor) return fail(MRT_ERR_INVALID, "null argument");
+  std::memcpy(r->view, view_colmajor, sizeof(r->view));
+  std::memcpy(r->proj, proj_colmajor, sizeof(r->proj));
+  r->sample_count = 0;  // every setter invalidates the counter (sequential_pathtracer.cpp:70-144)
+  return MRT_OK;
+}
+
+int mrt_renderer_set_num_bounces(mrt_renderer rh, int n) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || n < 1 || n > 64) return fail(MRT_ERR_INVALID, "bounces must be in [1, 64]");
+  r->num_bounces = n;
+  r->sample_count = 0;
+  return MRT_OK;
+}
+
+int mrt_renderer_invalidate(mrt_renderer rh) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r) return fail(MRT_ERR_INVALID, "null renderer");
+  r->sample_count = 0;
+  return MRT_OK;
+}
+
+int mrt_renderer_sample_count(mrt_renderer rh, uint32_t* out) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || !out) return fail(MRT_ERR_INVALID, "null argument");
+  *out = r->sample_count;
+  return MRT_OK;
+}
+
+int mrt_render(mrt_renderer rh, mrt_scene sh, int width, int height, int spp) {
+  MRT_GUARD_BEGIN
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!r || !s || width <= 0 || height <= 0 || spp <= 0) return fail(MRT_ERR_INVALID, "bad render arguments");
+  return r->render(*s, uint32_t(width), uint32_t(height), uint32_t(spp));
+  MRT_GUARD_END
+}
+
+int mrt_renderer_read(mrt_renderer rh, float* rgba) {
+  MRT_GUARD_BEGIN
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || !rgba) return fail(MRT_ERR_INVALID, "null argument");
+  return r->read(rgba);
+  MRT_GUARD_END
+}
+
+int mrt_renderer_stats(mrt_renderer rh, mrt_render_stats* out) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || !out) return fail(MRT_ERR_INVALID, "null argument");
+  *out = r->stats;
+  return MRT_OK;
+}
+
+}  // extern "C"

@@ -146,6 +146,26 @@ int Scene::build_host() {
         }
       }
       if (tris.size() >= kMaxItemIndex) return fail(MRT_ERR_LIMIT, "scene exceeds 16M triangle references");
+      geo_info.clear();
+      all_idx.clear();
+      all_pos.clear();
+      all_nrm.clear();
+      for (const Geometry& geo : geoms) {
+        GeoInfo gi{};
+        gi.idx_base = uint32_t(all_idx.size());
+        gi.vtx_base = uint32_t(all_pos.size());
+        gi.has_normals = geo.nrm.empty() ? 0u : 1u;
+        geo_info.push_back(gi);
+        all_idx.insert(all_idx.end(), geo.idx.begin(), geo.idx.end());
+        const size_t nv = geo.pos.size() / 3;
+        for (size_t v = 0; v < nv; ++v) {
+          all_pos.push_back(make_float4(geo.pos[3 * v], geo.pos[3 * v + 1], geo.pos[3 * v + 2], 0.f));
+          if (!geo.nrm.empty())
+            all_nrm.push_back(make_float4(geo.nrm[3 * v], geo.nrm[3 * v + 1], geo.nrm[3 * v + 2], 0.f));
+          else
+            all_nrm.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+        }
+      }
     }
     if (inst_dirty) {
       tlas_nodes.clear();
@@ -162,6 +182,8 @@ int Scene::build_host() {
         tlas_depth = w.depth;
         for (uint32_t item : w.items) tlas_items.push_back(tlas.indices[item]);
         if (tlas_items.size() >= kMaxItemIndex) return fail(MRT_ERR_LIMIT, "scene exceeds 16M instance references");
+        inst_material.clear();
+        for (const Instance& in : current) inst_material.push_back(in.material);
         for (size_t i = 0; i < current.size(); ++i) {
           InstanceRecord r{};
           std::memcpy(r.inv, current[i].inv, sizeof(r.inv));
@@ -191,6 +213,26 @@ int Scene::upload_all() {
     if ((rc = upload(d_tlas, cap_tlas, tlas_nodes)) != MRT_OK) return rc;
     if ((rc = upload(d_tlas_items, cap_items, tlas_items)) != MRT_OK) return rc;
     if ((rc = upload(d_inst, cap_inst, inst_records)) != MRT_OK) return rc;
+    if ((rc = upload(d_inst_material, cap_inst_mat, inst_material)) != MRT_OK) return rc;
+  }
+  if (geoms_dirty) {
+    if ((rc = upload(d_geo_info, cap_geo_info, geo_info)) != MRT_OK) return rc;
+    if ((rc = upload(d_idx, cap_idx, all_idx)) != MRT_OK) return rc;
+    if ((rc = upload(d_pos, cap_pos, all_pos)) != MRT_OK) return rc;
+    if ((rc = upload(d_nrm, cap_nrm, all_nrm)) != MRT_OK) return rc;
+  }
+  if (shading_dirty) {
+    std::vector<Material> mats = materials;
+    if (mats.empty()) {
+      Material def{};
+      def.albedo_rgba8 = 0xffccccccu;
+      def.ior = 1.5f;
+      def.roughness = 1.0f;
+      mats.push_back(def);
+    }
+    if ((rc = upload(d_materials, cap_mat, mats)) != MRT_OK) return rc;
+    if ((rc = upload(d_lights, cap_lights, lights)) != MRT_OK) return rc;
+    shading_dirty = false;
   }
   if (!d_ctrl) {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_ctrl), 64);
@@ -206,6 +248,9 @@ int Scene::upload_all() {
 }
 
 Scene::~Scene() {
+  for (void* p : {(void*)d_geo_info, (void*)d_idx, (void*)d_inst_material, (void*)d_pos, (void*)d_nrm,
+                  (void*)d_materials, (void*)d_lights})
+    if (p) hipFree(p);
   if (d_blas) hipFree(d_blas);
   if (d_tris) hipFree(d_tris);
   if (d_tlas) hipFree(d_tlas);
@@ -215,7 +260,24 @@ Scene::~Scene() {
   if (d_spill) hipFree(d_spill);
 }
 
-int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream) {
+PtScene Scene::shading_view() const {
+  PtScene v{};
+  v.instances = d_inst;
+  v.inst_material = d_inst_material;
+  v.geos = d_geo_info;
+  v.idx = d_idx;
+  v.pos = d_pos;
+  v.nrm = d_nrm;
+  v.materials = d_materials;
+  v.lights = d_lights;
+  v.num_lights = uint32_t(lights.size());
+  v.num_materials = uint32_t(std::max<size_t>(materials.size(), 1));
+  v.env_scale = env_scale;
+  return v;
+}
+
+int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream,
+                 const uint32_t* n_dev) {
   if (!prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
   if (n == 0) return MRT_OK;
   if (n >= 0x7fffffffu) return fail(MRT_ERR_LIMIT, "at most 2^31-1 rays per call");
@@ -245,6 +307,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.rays = d_rays;
   p.out = d_out;
   p.n = uint32_t(n);
+  p.n_dev = n_dev;
   p.has_tlas = tlas_nodes.empty() ? 0u : 1u;
   p.counter = d_ctrl;
   p.flags = d_ctrl + 1;
@@ -513,11 +576,23 @@ int mrt_scene_enqueue_geometry(mrt_scene sh, uint32_t geom, uint32_t material, c
   MRT_GUARD_END
 }
 
+static bool same_drawables(const std::vector<Instance>& a, const std::vector<Instance>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); ++i)
+    if (a[i].geom != b[i].geom || a[i].material != b[i].material || std::memcmp(a[i].m, b[i].m, sizeof(a[i].m)) != 0)
+      return false;
+  return true;
+}
+
 static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
+  // The instance BVH is rebuilt only when this frame's drawables differ from the prepared ones
+  // (the reference compares drawable hashes, scene.cpp:336-344).
   if (!s->pending.empty()) {
-    s->current = std::move(s->pending);
+    if (!s->prepared || !same_drawables(s->pending, s->current)) {
+      s->current = std::move(s->pending);
+      s->inst_dirty = true;
+    }
     s->pending.clear();
-    s->inst_dirty = true;
   }
   const bool gd = s->geoms_dirty, id = s->inst_dirty;
   int rc = s->build_host();

@@ -7,6 +7,7 @@
 
 #include "bvh_build.hpp"
 #include "device_types.hpp"
+#include "pathtrace.hpp"
 #include "wide_bvh.hpp"
 
 namespace mrt {
@@ -39,6 +40,12 @@ struct Scene {
   std::vector<Instance> current;  // the prepared frame
   bool geoms_dirty = false, inst_dirty = false, prepared = false;
 
+  // Shading data: materials (material_buffer, src/material.hpp:72-118), quad lights, environment.
+  std::vector<Material> materials;
+  std::vector<QuadLight> lights;
+  float env_scale = 1.0f;
+  bool shading_dirty = true;
+
   // Host copies of the device arrays.
   std::vector<WideNode> blas_nodes, tlas_nodes;
   std::vector<TriRecord> tris;
@@ -46,6 +53,9 @@ struct Scene {
   std::vector<InstanceRecord> inst_records;
   BinaryBvh tlas;
   uint32_t blas_depth = 0, tlas_depth = 0, max_stack = 0;
+  std::vector<GeoInfo> geo_info;
+  std::vector<uint32_t> all_idx, inst_material;
+  std::vector<float4> all_pos, all_nrm;
 
   // Device arrays.
   WideNode *d_blas = nullptr, *d_tlas = nullptr;
@@ -53,6 +63,12 @@ struct Scene {
   uint32_t* d_tlas_items = nullptr;
   InstanceRecord* d_inst = nullptr;
   size_t cap_blas = 0, cap_tris = 0, cap_tlas = 0, cap_items = 0, cap_inst = 0;
+  GeoInfo* d_geo_info = nullptr;
+  uint32_t *d_idx = nullptr, *d_inst_material = nullptr;
+  float4 *d_pos = nullptr, *d_nrm = nullptr;
+  Material* d_materials = nullptr;
+  QuadLight* d_lights = nullptr;
+  size_t cap_geo_info = 0, cap_idx = 0, cap_inst_mat = 0, cap_pos = 0, cap_nrm = 0, cap_mat = 0, cap_lights = 0;
   uint32_t* d_ctrl = nullptr;  // [0] ray counter, [1] flags, [8..15] four u64 counters
   uint32_t* d_spill = nullptr;
   size_t spill_cap = 0;
@@ -62,7 +78,9 @@ struct Scene {
   ~Scene();
   int build_host();
   int upload_all();
-  int trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream);
+  int trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream,
+            const uint32_t* n_dev = nullptr);
+  PtScene shading_view() const;
   int read_ctrl(uint32_t* flags, unsigned long long* stats, hipStream_t stream);
 };
 

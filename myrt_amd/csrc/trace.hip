@@ -106,6 +106,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
 
   int ray_id = -1;
   bool queue_empty = false;
+  const uint32_t n_rays = p.n_dev ? *p.n_dev : p.n;
 
   float wox = 0, woy = 0, woz = 0, wdx = 0, wdy = 0, wdz = 0, wix = 0, wiy = 0, wiz = 0;
   float tmin = 0, tmax = 0;
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
       if (want) {
         const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
         const uint32_t r = base + rank;
-        if (r < p.n) {
+        if (r < n_rays) {
           ray_id = int(r);
           const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(r);
           const float4 r0 = rp[0], r1 = rp[1];

@@ -17,6 +17,7 @@ struct TraceParams {
   const void* rays;                // mrt_ray[n]
   void* out;                       // mrt_hit[n] or uint8_t[n]
   uint32_t n;
+  const uint32_t* n_dev;           // if set, the ray count is read from device memory (wavefront queues)
   uint32_t has_tlas;
   uint32_t* counter;               // ray counter, zero before the launch
   uint32_t* flags;                 // bit 0: stack overflow

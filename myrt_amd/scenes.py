@@ -224,3 +224,67 @@ def traversal_workload(n_coherent=4194304, n_incoherent=4194304, width=1024, spp
     coh = primary_rays(width, height, spp, seed=5)[:n_coherent]
     inc = bounce_rays(idx, pts, n_incoherent, seed=4)
     return (idx, pts, nrm, uv), coh, inc
+
+
+# ---- path-tracing scenes ----------------------------------------------------------------------
+
+MATERIAL_DTYPE = np.dtype([("albedo_rgba8", "<u4"), ("ior", "<f4"), ("roughness", "<f4"), ("metallic", "<f4"),
+                           ("transmission", "<f4"), ("texture", "<u4"), ("pad", "<u4", 2)])
+LIGHT_DTYPE = np.dtype([("corner", "<f4", 3), ("u", "<f4", 3), ("v", "<f4", 3), ("emission", "<f4", 3)])
+
+
+def _rgba8(rgb):
+    c = [int(round(min(max(float(x), 0.0), 1.0) * 255)) for x in list(rgb) + [1.0]]
+    return c[0] | (c[1] << 8) | (c[2] << 16) | (c[3] << 24)
+
+
+def sphere_field_scene(n_spheres=200, level=2, seed=1, glass_every=0, metal_every=0, ground=True):
+    """Ground quad plus n icospheres (radius U[0.1, 0.5], centre U[-5, 5]^3, seed), each with its
+    own material: GGX roughness U[0.05, 1], albedo U[0.1, 0.9] (seed + 1); every `glass_every`-th
+    sphere is a dielectric (transmission 1) and every `metal_every`-th a metal. One quad light
+    above the field. Returns dict(geometries, instances, materials, lights, view, proj)."""
+    (fi, fp, fn), mats = sphere_field(n_spheres, level, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    materials = np.zeros(n_spheres + 1, MATERIAL_DTYPE)
+    for k in range(n_spheres):
+        albedo = rng.uniform(0.1, 0.9, 3)
+        rough = rng.uniform(0.05, 1.0)
+        materials[k] = (_rgba8(albedo), 1.5, rough, 0.0, 0.0, 0, (0, 0))
+        if glass_every and k % glass_every == 0:
+            materials[k]["transmission"] = 1.0
+            materials[k]["roughness"] = 0.05
+        elif metal_every and k % metal_every == 1:
+            materials[k]["metallic"] = 1.0
+    materials[n_spheres] = (_rgba8((0.7, 0.7, 0.7)), 1.5, 0.9, 0.0, 0.0, 0, (0, 0))
+    geometries = [(fi, fp, fn)]
+    instances = [(0, mt, k) for k, mt in enumerate(mats)]
+    if ground:
+        gi, gp, gn = ground_quad(size=8.0, y=-5.0)
+        geometries.append((gi, gp, gn))
+        instances.append((1, np.eye(4, dtype=np.float32), n_spheres))
+    lights = np.zeros(1, LIGHT_DTYPE)
+    lights[0] = ((-1.5, 7.0, -1.5), (3.0, 0.0, 0.0), (0.0, 0.0, 3.0), (12.0, 11.0, 10.0))
+    # cross(u, v) = (0, 0, 3) x ... -> u = +x, v = +z gives cross = -y: emits downward.
+    return dict(geometries=geometries, instances=instances, materials=materials, lights=lights,
+                eye=(0.0, 4.0, -14.0), target=(0.0, -1.0, 0.0), fovy=45.0)
+
+
+def camera_matrices(eye, target, fovy, aspect):
+    e = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - e
+    f /= np.linalg.norm(f)
+    s = np.cross(f, (0.0, 1.0, 0.0))
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    view = np.eye(4)
+    view[0, :3], view[1, :3], view[2, :3] = s, u, -f
+    view[:3, 3] = -view[:3, :3] @ e
+    t = 1.0 / np.tan(np.radians(fovy) / 2)
+    near, far = 0.05, 1000.0
+    proj = np.zeros((4, 4))
+    proj[0, 0] = t / aspect
+    proj[1, 1] = t
+    proj[2, 2] = (far + near) / (near - far)
+    proj[2, 3] = 2 * far * near / (near - far)
+    proj[3, 2] = -1.0
+    return view.astype(np.float32), proj.astype(np.float32)

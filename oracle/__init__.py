@@ -24,6 +24,18 @@ class _Scene(ctypes.Structure):
                 ("inst_geom", ctypes.c_void_p), ("inst_inv", ctypes.c_void_p), ("n_inst", ctypes.c_uint32)]
 
 
+class _PtScene(ctypes.Structure):
+    _fields_ = [("geo", ctypes.POINTER(_Scene)), ("nrm", ctypes.c_void_p), ("geo_has_nrm", ctypes.c_void_p),
+                ("inst_material", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32),
+                ("lights", ctypes.c_void_p), ("n_lights", ctypes.c_uint32), ("env_scale", ctypes.c_double)]
+
+
+MATERIAL_DTYPE = np.dtype([("albedo_rgba8", "<u4"), ("ior", "<f4"), ("roughness", "<f4"), ("metallic", "<f4"),
+                           ("transmission", "<f4"), ("texture", "<u4"), ("pad", "<u4", 2)])
+LIGHT_DTYPE = np.dtype([("corner", "<f4", 3), ("u", "<f4", 3), ("v", "<f4", 3), ("emission", "<f4", 3)])
+# The product's material when none was pushed.
+DEFAULT_MATERIAL = np.array([(0xffcccccc, 1.5, 1.0, 0.0, 0.0, 0, (0, 0))], MATERIAL_DTYPE)
+
 _lib = None
 _P = ctypes.c_void_p
 _SZ = ctypes.c_size_t
@@ -52,6 +64,8 @@ def lib():
             "orc_reference_trace": ([_P, _P, _SZ, ctypes.c_int, _P], None),
             "orc_reference_time": ([_P, _P, _SZ, ctypes.c_int, ctypes.c_int, _SZ, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int)], ctypes.c_double),
+            "orc_render": ([ctypes.POINTER(_PtScene), _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_int, _P, ctypes.c_int], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -190,3 +204,37 @@ class OracleScene:
                 lib().orc_accel_destroy(self._accel)
         except Exception:
             pass
+
+
+class OracleRender:
+    """FP64 render of a scene description with the product's sampler sequence."""
+
+    def __init__(self, geometries, instances, materials=None, lights=None, env_scale=1.0):
+        """geometries: list of (indices, points, normals or None); instances: (geom, 4x4, material)."""
+        self.scene = OracleScene([(g[0], g[1]) for g in geometries], [(g, mt) for g, mt, _ in instances])
+        nrm = []
+        for ind, pts, nr in geometries:
+            n = np.zeros((np.asarray(pts).reshape(-1, 3).shape[0], 3), np.float32) if nr is None else nr
+            nrm.append(np.asarray(n, np.float32).reshape(-1, 3))
+        self.nrm = np.ascontiguousarray(np.concatenate(nrm))
+        self.has_nrm = np.array([0 if g[2] is None else 1 for g in geometries], np.uint32)
+        self.inst_mat = np.array([mi for _, _, mi in instances] or [0], np.uint32)
+        mats = DEFAULT_MATERIAL if materials is None or len(materials) == 0 else np.ascontiguousarray(materials,
+                                                                                                      MATERIAL_DTYPE)
+        self.mats = np.ascontiguousarray(mats, MATERIAL_DTYPE)
+        self.lights = np.ascontiguousarray(np.zeros(0, LIGHT_DTYPE) if lights is None else lights, LIGHT_DTYPE)
+        self._lights_buf = self.lights if self.lights.size else np.zeros(1, LIGHT_DTYPE)
+        self.c = _PtScene(ctypes.pointer(self.scene.c), self.nrm.ctypes.data, self.has_nrm.ctypes.data,
+                          self.inst_mat.ctypes.data, self.mats.ctypes.data, self.mats.size,
+                          self._lights_buf.ctypes.data, self.lights.size, float(env_scale))
+
+    def render(self, view, proj, width, height, spp, max_depth, sample_base=0, threads=0):
+        """Mean radiance per pixel, shape (height, width, 3), float64 (rows from the bottom)."""
+        v = np.ascontiguousarray(np.asarray(view, np.float32).T).ravel()
+        p = np.ascontiguousarray(np.asarray(proj, np.float32).T).ravel()
+        out = np.zeros(width * height * 3, np.float64)
+        rc = lib().orc_render(ctypes.byref(self.c), self.scene.accel(), _p(v), _p(p), width, height, spp,
+                              sample_base, max_depth, _p(out), threads)
+        if rc != 0:
+            raise RuntimeError("oracle render failed")
+        return out.reshape(height, width, 3) / spp

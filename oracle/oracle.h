@@ -109,6 +109,34 @@ void orc_reference_trace(orc_accel a, const orc_ray* rays, size_t n, int any_hit
 double orc_reference_time(orc_accel a, const orc_ray* rays, size_t n, int any_hit, int threads, size_t chunk,
                           int reps, int* threads_used);
 
+/* ---- FP64 path-tracing oracle -------------------------------------------------------------- */
+typedef struct orc_material {
+  uint32_t albedo_rgba8;
+  float ior, roughness, metallic, transmission;
+  uint32_t texture;
+  uint32_t pad[2];
+} orc_material;
+
+typedef struct orc_pt_scene {
+  const orc_scene* geo;
+  const float* nrm;              /* per-vertex normals, same layout as geo->pos */
+  const uint32_t* geo_has_nrm;   /* per geometry */
+  const uint32_t* inst_material; /* per instance */
+  const orc_material* materials;
+  uint32_t n_materials;
+  const float* lights;           /* 12 floats per quad light: corner, u, v, emission */
+  uint32_t n_lights;
+  double env_scale;
+} orc_pt_scene;
+
+/* The product's wavefront estimator in binary64: camera (inverse(proj * view), with the
+ * homogeneous divide), Fresnel-weighted GGX / Lambert / smooth-dielectric lobes, quad-light NEE
+ * with power-heuristic MIS, Russian roulette from bounce 3, sky environment; the sampler is the
+ * product's counter hash. out_rgb receives per-pixel SUMS over spp samples (w*h*3 doubles).
+ * Traces through the oracle BVH `a` built over s->geo. Returns 0 on success. */
+int orc_render(const orc_pt_scene* s, orc_accel a, const float view_colmajor[16], const float proj_colmajor[16],
+               int width, int height, int spp, int sample_base, int max_depth, double* out_rgb, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,76 @@
+"""The FP64 path-tracing oracle, pinned by closed forms (no GPU):
+
+* direct lighting of a Lambert plane by a quad light equals rho/pi times the irradiance
+  integral E = int Le cos(theta) cos(theta') / r^2 dA, evaluated here by a 400x400 midpoint
+  rule (this exercises light sampling, the MIS split between light and BSDF samples, the Lambert
+  lobe and the camera);
+* a white-furnace check: a Lambert sphere under a uniform-looking environment reflects at most
+  what it receives and a black sphere reflects nothing;
+* determinism: the same sampler sequence gives the same image.
+"""
+import numpy as np
+
+import oracle as orc
+from myrt_amd import scenes
+
+
+def lambert_plane(rho=0.5, light_h=2.0, emission=10.0):
+    gi, gp, gn = scenes.ground_quad(size=20.0, y=0.0)
+    mat = np.zeros(1, orc.MATERIAL_DTYPE)
+    mat[0] = (scenes._rgba8((rho, rho, rho)), 1.0, 1.0, 0.0, 0.0, 0, (0, 0))
+    light = np.zeros(1, orc.LIGHT_DTYPE)
+    light[0] = ((-0.5, light_h, -0.5), (1, 0, 0), (0, 0, 1), (emission,) * 3)
+    return orc.OracleRender([(gi, gp, gn)], [(0, np.eye(4, dtype=np.float32), 0)], mat, light, 0.0)
+
+
+def irradiance_below_square(h, emission, n=400):
+    xs = (np.arange(n) + 0.5) / n - 0.5
+    X, Z = np.meshgrid(xs, xs)
+    r2 = X ** 2 + Z ** 2 + h * h
+    return float((emission * (h / np.sqrt(r2)) ** 2 / r2).sum() / n ** 2)
+
+
+def test_direct_light_closed_form(ensure_built):
+    for h in (1.0, 2.0, 4.0):
+        R = lambert_plane(light_h=h)
+        view, proj = scenes.camera_matrices((0, h + 3, 0.001), (0, 0, 0), 1.0, 1.0)
+        img = R.render(view, proj, 3, 3, 8192, 2)
+        expect = (128 / 255) / np.pi * irradiance_below_square(h, 10.0)
+        assert abs(img[1, 1, 0] - expect) / expect < 0.01, (h, img[1, 1, 0], expect)
+
+
+def test_light_seen_directly(ensure_built):
+    """A camera ray hitting the emitting side of the light returns its radiance exactly."""
+    R = lambert_plane(light_h=2.0, emission=7.0)
+    view, proj = scenes.camera_matrices((0.0, 0.5, 0.0), (0.0, 3.0, 0.001), 1.0, 1.0)
+    img = R.render(view, proj, 3, 3, 4, 2)
+    assert np.allclose(img[1, 1], 7.0)
+
+
+def test_furnace_sphere(ensure_built):
+    (fi, fp, fn) = scenes.icosphere(3)
+    white = np.zeros(2, orc.MATERIAL_DTYPE)
+    white[0] = (scenes._rgba8((1.0, 1.0, 1.0)), 1.0, 1.0, 0.0, 0.0, 0, (0, 0))
+    # A black metal: Fresnel weight 1 and a black specular tint, so nothing is reflected.
+    white[1] = (scenes._rgba8((0.0, 0.0, 0.0)), 1.0, 1.0, 1.0, 0.0, 0, (0, 0))
+    view, proj = scenes.camera_matrices((0, 0, -4), (0, 0, 0), 20.0, 1.0)
+    for mi, bound in ((0, 1.0), (1, 0.0)):
+        R = orc.OracleRender([(fi, fp, fn)], [(0, np.eye(4, dtype=np.float32), mi)], white, None, 1.0)
+        img = R.render(view, proj, 8, 8, 256, 6)
+        env_max = 1.0  # the sky gradient never exceeds 1 with scale 1
+        centre = img[3:5, 3:5].mean()
+        if bound == 0.0:
+            assert centre == 0.0
+        else:
+            assert 0.2 < centre <= env_max * 1.001
+
+
+def test_deterministic(ensure_built):
+    sc = scenes.sphere_field_scene(20, 1, seed=3, glass_every=5, metal_every=4)
+    R = orc.OracleRender(sc["geometries"], sc["instances"], sc["materials"], sc["lights"], 1.0)
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], 1.5)
+    a = R.render(view, proj, 24, 16, 8, 4, threads=1)
+    b = R.render(view, proj, 24, 16, 8, 4, threads=4)
+    assert np.array_equal(a, b)
+    c = R.render(view, proj, 24, 16, 8, 4, sample_base=8)
+    assert not np.array_equal(a, c)
