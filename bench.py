@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="incoherent rays timed on the host")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel 3 times on BYTES (calibrates rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
@@ -57,6 +59,15 @@ def main():
 
     import myrt_amd as m
     from myrt_amd import scenes
+
+    if args.copy_calibration:
+        n = args.copy_calibration // 16 * 16
+        a, b = m.DeviceBuffer(n), m.DeviceBuffer(n)
+        for _ in range(3):
+            m._check(m.lib().mrt_device_copy(b.ptr, a.ptr, n, None))
+        m.synchronize()
+        print(json.dumps({"copy_bytes": n, "launches": 3}), flush=True)
+        return
 
     log("generating workload")
     (idx, pts, nrm, uv), coh, inc = scenes.traversal_workload()

@@ -103,9 +103,23 @@ typedef struct mrt_render_stats {
   double milliseconds;   /* wall time inside mrt_render */
 } mrt_render_stats;
 
+/* Radiance-cache query, 64 B: shading point, outgoing direction (towards the viewer), shading
+ * normal, GGX roughness and albedo. */
+typedef struct mrt_cache_query {
+  float pos[3];
+  float roughness;
+  float dir[3];
+  float pad0;
+  float nrm[3];
+  float pad1;
+  float albedo[3];
+  float pad2;
+} mrt_cache_query;
+
 typedef struct mrt_scene_s* mrt_scene;
 typedef struct mrt_bvh_s* mrt_bvh;
 typedef struct mrt_renderer_s* mrt_renderer;
+typedef struct mrt_cache_s* mrt_cache;
 
 /* Counters filled by mrt_scene_prepare (scene::prepare_result_t, src/pathtracer/scene.hpp:84-91,
  * extended with the device-side sizes of the wide BVH). */
@@ -137,6 +151,9 @@ int mrt_device_free(void* ptr);
 int mrt_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int mrt_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int mrt_device_synchronize(void);
+/* A plain streaming copy kernel (bytes a multiple of 16) on `stream`; reads and writes exactly
+ * `bytes` once, so it calibrates hardware byte counters. No reference counterpart. */
+int mrt_device_copy(void* dst, const void* src, size_t bytes, void* stream);
 /* Stream-ordered GPU timer: records start/stop events on `stream` (NULL = default stream). */
 typedef struct mrt_timer_s* mrt_timer;
 int mrt_timer_create(mrt_timer* out);
@@ -228,6 +245,28 @@ int mrt_render(mrt_renderer r, mrt_scene s, int width, int height, int spp);
 /* The accumulated mean radiance, width*height RGBA floats (alpha = 1). */
 int mrt_renderer_read(mrt_renderer r, float* rgba);
 int mrt_renderer_stats(mrt_renderer r, mrt_render_stats* out);
+
+/* ---- neural radiance cache (no reference counterpart: the north star's learned part) ------
+ * A 64-input frequency-encoded MLP with 5 hidden layers of 64 ReLU units and 3 outputs
+ * (20,672 weights, no biases), bf16 MFMA with fp32 accumulation, Adam on fp32 master weights.
+ * Parameters are packed layer by layer, W[out][in] row-major: 5 x 64x64 then 3x64. */
+int mrt_cache_create(mrt_cache* out, uint64_t seed);
+int mrt_cache_destroy(mrt_cache c);
+/* Box the position encoding maps to [0,1]^3. */
+int mrt_cache_set_bounds(mrt_cache c, const float lo[3], const float hi[3]);
+int mrt_cache_set_adam(mrt_cache c, float lr, float beta1, float beta2, float eps);
+int mrt_cache_get_params(mrt_cache c, float* out, size_t n);
+int mrt_cache_set_params(mrt_cache c, const float* in, size_t n);
+/* Predicted radiance, 3 floats per query. */
+int mrt_cache_infer(mrt_cache c, const mrt_cache_query* q, size_t n, float* out_rgb);
+int mrt_cache_infer_device(mrt_cache c, const mrt_cache_query* d_q, size_t n, float* d_out_rgb, void* stream);
+/* One Adam step on the relative-L2 loss mean((y - t)^2 / (y^2 + 0.01)) over the batch. */
+int mrt_cache_train(mrt_cache c, const mrt_cache_query* q, const float* targets_rgb, size_t n, float* loss);
+int mrt_cache_train_device(mrt_cache c, const mrt_cache_query* d_q, const float* d_targets_rgb, size_t n,
+                           void* stream);
+/* The loss gradient (packed like the parameters) without updating anything. */
+int mrt_cache_gradient(mrt_cache c, const mrt_cache_query* q, const float* targets_rgb, size_t n, float* grad,
+                       float* loss);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,9 @@ assert WIDE_NODE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48 and INSTANCE_
 MATERIAL_DTYPE = np.dtype([("albedo_rgba8", "<u4"), ("ior", "<f4"), ("roughness", "<f4"), ("metallic", "<f4"),
                            ("transmission", "<f4"), ("texture", "<u4"), ("pad", "<u4", 2)])
 LIGHT_DTYPE = np.dtype([("corner", "<f4", 3), ("u", "<f4", 3), ("v", "<f4", 3), ("emission", "<f4", 3)])
+QUERY_DTYPE = np.dtype([("pos", "<f4", 3), ("roughness", "<f4"), ("dir", "<f4", 3), ("pad0", "<f4"),
+                        ("nrm", "<f4", 3), ("pad1", "<f4"), ("albedo", "<f4", 3), ("pad2", "<f4")])
+CACHE_PARAMS = 20672
 
 SPLIT_NONE = 0
 SPLIT_AABBS = 1
@@ -69,6 +72,7 @@ SIGNATURES = {
     "mrt_memcpy_h2d": ([_P, _P, _SZ], ctypes.c_int),
     "mrt_memcpy_d2h": ([_P, _P, _SZ], ctypes.c_int),
     "mrt_device_synchronize": ([], ctypes.c_int),
+    "mrt_device_copy": ([_P, _P, _SZ, _P], ctypes.c_int),
     "mrt_timer_create": ([ctypes.POINTER(_P)], ctypes.c_int),
     "mrt_timer_destroy": ([_P], ctypes.c_int),
     "mrt_timer_start": ([_P, _P], ctypes.c_int),
@@ -108,6 +112,17 @@ SIGNATURES = {
     "mrt_render": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_read": ([_P, _P], ctypes.c_int),
     "mrt_renderer_stats": ([_P, ctypes.POINTER(_RenderStats)], ctypes.c_int),
+    "mrt_cache_create": ([ctypes.POINTER(_P), ctypes.c_uint64], ctypes.c_int),
+    "mrt_cache_destroy": ([_P], ctypes.c_int),
+    "mrt_cache_set_bounds": ([_P, _P, _P], ctypes.c_int),
+    "mrt_cache_set_adam": ([_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float], ctypes.c_int),
+    "mrt_cache_get_params": ([_P, _P, _SZ], ctypes.c_int),
+    "mrt_cache_set_params": ([_P, _P, _SZ], ctypes.c_int),
+    "mrt_cache_infer": ([_P, _P, _SZ, _P], ctypes.c_int),
+    "mrt_cache_infer_device": ([_P, _P, _SZ, _P, _P], ctypes.c_int),
+    "mrt_cache_train": ([_P, _P, _P, _SZ, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "mrt_cache_train_device": ([_P, _P, _P, _SZ, _P], ctypes.c_int),
+    "mrt_cache_gradient": ([_P, _P, _P, _SZ, _P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
 }
 
 
@@ -482,3 +497,61 @@ class Renderer:
         s = _RenderStats()
         _check(lib().mrt_renderer_stats(self.h, ctypes.byref(s)))
         return {"samples": s.samples, "milliseconds": s.milliseconds}
+
+
+class RadianceCache:
+    """The MLP radiance cache over the C ABI (mrt_cache_*)."""
+
+    def __init__(self, seed=1):
+        h = ctypes.c_void_p()
+        _check(lib().mrt_cache_create(ctypes.byref(h), int(seed)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mrt_cache_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_bounds(self, lo, hi):
+        lo = np.asarray(lo, np.float32)
+        hi = np.asarray(hi, np.float32)
+        _check(lib().mrt_cache_set_bounds(self.h, _ptr(lo), _ptr(hi)))
+
+    def set_adam(self, lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-8):
+        _check(lib().mrt_cache_set_adam(self.h, lr, beta1, beta2, eps))
+
+    def params(self):
+        out = np.zeros(CACHE_PARAMS, np.float32)
+        _check(lib().mrt_cache_get_params(self.h, _ptr(out), out.size))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().mrt_cache_set_params(self.h, _ptr(p), p.size))
+
+    def infer(self, queries):
+        q = np.ascontiguousarray(queries, QUERY_DTYPE)
+        out = np.zeros((q.size, 3), np.float32)
+        _check(lib().mrt_cache_infer(self.h, _ptr(q), q.size, _ptr(out)))
+        return out
+
+    def train(self, queries, targets):
+        q = np.ascontiguousarray(queries, QUERY_DTYPE)
+        t = np.ascontiguousarray(targets, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().mrt_cache_train(self.h, _ptr(q), _ptr(t), q.size, ctypes.byref(loss)))
+        return loss.value
+
+    def gradient(self, queries, targets):
+        q = np.ascontiguousarray(queries, QUERY_DTYPE)
+        t = np.ascontiguousarray(targets, np.float32)
+        g = np.zeros(CACHE_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().mrt_cache_gradient(self.h, _ptr(q), _ptr(t), q.size, _ptr(g), ctypes.byref(loss)))
+        return g, loss.value

@@ -1,0 +1,45 @@
+// Neural radiance cache: records and launchers (mlp.hip) shared with cache.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mrt {
+
+// One cache query, 64 B: shading point, outgoing direction, normal, roughness and albedo.
+struct CacheQuery {
+  float pos[3];
+  float roughness;
+  float dir[3];
+  float pad0;
+  float nrm[3];
+  float pad1;
+  float albedo[3];
+  float pad2;
+};
+static_assert(sizeof(CacheQuery) == 64, "cache query is 64 bytes");
+
+struct CacheParams {
+  uint16_t* w;       // bf16 [6][64][64] W[out][in]
+  uint16_t* wt;      // bf16 [6][64][64] W^T[in][out]
+  float* master;     // fp32 [6][64][64]
+  float* m;          // Adam first moment
+  float* v;          // Adam second moment
+  float lo[3];       // position box of the encoding
+  float inv_ext[3];  // 1 / box extent
+  uint32_t batch_stride;  // queries per row of the training activation buffers
+};
+
+constexpr int kCacheLayers = 6;
+constexpr int kCacheWidth = 64;
+constexpr int kCacheParamsPadded = kCacheLayers * kCacheWidth * kCacheWidth;  // 24576
+constexpr int kCacheParams = 5 * 64 * 64 + 3 * 64;                            // 20672
+
+int launch_cache_infer(const CacheParams& P, const CacheQuery* q, uint32_t n, float* out, int grid, hipStream_t s);
+int launch_cache_train(const CacheParams& P, const CacheQuery* q, const float* targets, uint32_t n, float grad_scale,
+                       uint16_t* h_buf, uint16_t* g_buf, float* loss_tiles, float* partial, uint32_t chunk,
+                       float lr, float beta1, float beta2, float eps, float bc1, float bc2, float* grad_out,
+                       hipStream_t s);
+int launch_cache_sync(const CacheParams& P, hipStream_t s);
+
+}  // namespace mrt

@@ -24,6 +24,7 @@
 
 namespace mrt {
 
+int launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
 thread_local std::string g_last_error;
 
 int fail(int code, const std::string& msg) {
@@ -387,6 +388,12 @@ int mrt_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipMemcpy D2H");
 }
+int mrt_device_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!dst || !src || bytes % 16 != 0) return fail(MRT_ERR_INVALID, "copy needs non-null pointers and a multiple of 16 bytes");
+  return mrt::launch_copy(src, dst, bytes, static_cast<hipStream_t>(stream)) == 0 ? MRT_OK
+                                                                                  : fail(MRT_ERR_HIP, "copy kernel launch failed");
+}
+
 int mrt_device_synchronize(void) {
   hipError_t e = hipDeviceSynchronize();
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipDeviceSynchronize");
