@@ -98,6 +98,32 @@ typedef struct mrt_quad_light {
   float emission[3];
 } mrt_quad_light;
 
+/* One instruction of an SDF program, 32 B. A program is the postfix form of the reference's
+ * sdf:: tree (src/pathtracer/sdf.hpp:24-67, 399-633): primitives push a distance, operators
+ * combine the top two (lhs pushed first), PUSH_TRANSLATE / PUSH_MIRROR transform the position
+ * for the single operand that follows, up to the matching POP. */
+typedef struct mrt_sdf_node {
+  uint32_t op;
+  uint32_t material;  /* primitives: material index of the surface */
+  float p[6];
+} mrt_sdf_node;
+enum {
+  MRT_SDF_SPHERE = 1,        /* p0 radius */
+  MRT_SDF_BOX = 2,           /* p0-p2 half extents */
+  MRT_SDF_TORUS = 3,         /* p0 large radius, p1 small radius */
+  MRT_SDF_CAPSULE = 4,       /* vertical capsule: p0 height, p1 radius */
+  MRT_SDF_MENGER = 5,        /* unit Menger sponge */
+  MRT_SDF_UNION = 16,
+  MRT_SDF_SUBTRACT = 17,     /* max(-lhs, rhs) */
+  MRT_SDF_INTERSECT = 18,
+  MRT_SDF_SMOOTH_UNION = 19, /* p0 k */
+  MRT_SDF_SMOOTH_SUBTRACT = 20,
+  MRT_SDF_SMOOTH_INTERSECT = 21,
+  MRT_SDF_PUSH_TRANSLATE = 32, /* p0-p2 offset */
+  MRT_SDF_PUSH_MIRROR = 33,    /* p0 axis (0, 1, 2) */
+  MRT_SDF_POP = 34
+};
+
 typedef struct mrt_render_stats {
   uint64_t samples;      /* pixel samples rendered since creation */
   double milliseconds;   /* wall time inside mrt_render */
@@ -211,6 +237,17 @@ int mrt_scene_copy_wide(mrt_scene s, void* blas_nodes, void* tris, void* tlas_no
 int mrt_scene_push_material(mrt_scene s, const mrt_material* mat, uint32_t* out_id);
 int mrt_scene_set_material(mrt_scene s, uint32_t id, const mrt_material* mat);
 int mrt_scene_set_material_param(mrt_scene s, uint32_t id, const char* name, const float* value, size_t n);
+/* scene::push_sdf(sdf_info_t{root}) (src/pathtracer/scene.cpp:189-204) with the tree as a postfix
+ * program; the program is validated (stack discipline, depth <= 8, modifiers <= 4 deep). */
+int mrt_scene_push_sdf(mrt_scene s, const mrt_sdf_node* nodes, size_t n, uint32_t* out_sdf);
+/* scene::enqueue(sdf, mat4) (src/pathtracer/scene.hpp:147-148, scene.cpp:411-417). SDF hits come
+ * back with inst = 0x80000000 | drawable index, prim = material and an octahedral normal in bary. */
+int mrt_scene_enqueue_sdf(mrt_scene s, uint32_t sdf, const float m4x4_colmajor[16], uint32_t* out_drawable);
+/* The prepared SDF drawables, 80 B each: rows 0-2 of M^-1, world box (lo, tape offset, hi, tape
+ * length). Like mrt_scene_copy_wide, it exposes the device records (scene::get_scene_buffers). */
+int mrt_scene_sdf_drawables(mrt_scene s, void* out, size_t n);
+/* set_sdf_marching_steps / set_sdf_marching_epsilon (src/pathtracer/sequential_pathtracer.hpp:44-45). */
+int mrt_scene_set_sdf_marching(mrt_scene s, int steps, float eps);
 /* Lights and the sky-gradient environment scale (src/pt/pt_color.comp:189-201). */
 int mrt_scene_set_lights(mrt_scene s, const mrt_quad_light* lights, size_t n);
 int mrt_scene_set_environment(mrt_scene s, float scale);

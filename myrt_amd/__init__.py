@@ -28,6 +28,12 @@ assert WIDE_NODE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48 and INSTANCE_
 MATERIAL_DTYPE = np.dtype([("albedo_rgba8", "<u4"), ("ior", "<f4"), ("roughness", "<f4"), ("metallic", "<f4"),
                            ("transmission", "<f4"), ("texture", "<u4"), ("pad", "<u4", 2)])
 LIGHT_DTYPE = np.dtype([("corner", "<f4", 3), ("u", "<f4", 3), ("v", "<f4", 3), ("emission", "<f4", 3)])
+SDF_NODE_DTYPE = np.dtype([("op", "<u4"), ("material", "<u4"), ("p", "<f4", 6)])
+SDF = dict(SPHERE=1, BOX=2, TORUS=3, CAPSULE=4, MENGER=5, UNION=16, SUBTRACT=17, INTERSECT=18, SMOOTH_UNION=19,
+           SMOOTH_SUBTRACT=20, SMOOTH_INTERSECT=21, PUSH_TRANSLATE=32, PUSH_MIRROR=33, POP=34)
+SDF_HIT = 0x80000000
+SDF_DRAWABLE_DTYPE = np.dtype([("inv", "<f4", 12), ("box_lo", "<f4", 3), ("tape_offset", "<u4"),
+                               ("box_hi", "<f4", 3), ("tape_len", "<u4")])
 QUERY_DTYPE = np.dtype([("pos", "<f4", 3), ("roughness", "<f4"), ("dir", "<f4", 3), ("pad0", "<f4"),
                         ("nrm", "<f4", 3), ("pad1", "<f4"), ("albedo", "<f4", 3), ("pad2", "<f4")])
 CACHE_PARAMS = 20672
@@ -103,6 +109,10 @@ SIGNATURES = {
     "mrt_scene_set_material_param": ([_P, ctypes.c_uint32, ctypes.c_char_p, _P, _SZ], ctypes.c_int),
     "mrt_scene_set_lights": ([_P, _P, _SZ], ctypes.c_int),
     "mrt_scene_set_environment": ([_P, ctypes.c_float], ctypes.c_int),
+    "mrt_scene_push_sdf": ([_P, _P, _SZ, _U32P], ctypes.c_int),
+    "mrt_scene_enqueue_sdf": ([_P, ctypes.c_uint32, _P, _U32P], ctypes.c_int),
+    "mrt_scene_set_sdf_marching": ([_P, ctypes.c_int, ctypes.c_float], ctypes.c_int),
+    "mrt_scene_sdf_drawables": ([_P, _P, _SZ], ctypes.c_int),
     "mrt_renderer_create": ([ctypes.POINTER(_P)], ctypes.c_int),
     "mrt_renderer_destroy": ([_P], ctypes.c_int),
     "mrt_renderer_set_camera": ([_P, _P, _P], ctypes.c_int),
@@ -416,6 +426,46 @@ def _scene_set_environment(self, scale):
     _check(lib().mrt_scene_set_environment(self.h, float(scale)))
 
 
+def _scene_push_sdf(self, nodes):
+    t = np.ascontiguousarray(nodes, SDF_NODE_DTYPE)
+    out = ctypes.c_uint32()
+    _check(lib().mrt_scene_push_sdf(self.h, _ptr(t), t.size, ctypes.byref(out)))
+    return out.value
+
+
+def _scene_enqueue_sdf(self, sdf, transformation=None):
+    m = np.eye(4, dtype=np.float32) if transformation is None else np.asarray(transformation, np.float32)
+    colmajor = np.ascontiguousarray(m.T).ravel()
+    out = ctypes.c_uint32()
+    _check(lib().mrt_scene_enqueue_sdf(self.h, int(sdf), _ptr(colmajor), ctypes.byref(out)))
+    return out.value
+
+
+def _scene_set_sdf_marching(self, steps=400, eps=1e-5):
+    _check(lib().mrt_scene_set_sdf_marching(self.h, int(steps), float(eps)))
+
+
+def sdf_program(nodes):
+    """Builds a tape from (op_name, material, params...) tuples."""
+    t = np.zeros(len(nodes), SDF_NODE_DTYPE)
+    for i, nd in enumerate(nodes):
+        t[i]["op"] = SDF[nd[0]]
+        t[i]["material"] = nd[1] if len(nd) > 1 else 0
+        ps = list(nd[2:]) if len(nd) > 2 else []
+        t[i]["p"][:len(ps)] = ps
+    return t
+
+
+def _scene_sdf_drawables(self, n):
+    out = np.zeros(max(n, 1), SDF_DRAWABLE_DTYPE)
+    _check(lib().mrt_scene_sdf_drawables(self.h, _ptr(out), out.size))
+    return out[:n]
+
+
+Scene.sdf_drawables = _scene_sdf_drawables
+Scene.push_sdf = _scene_push_sdf
+Scene.enqueue_sdf = _scene_enqueue_sdf
+Scene.set_sdf_marching = _scene_set_sdf_marching
 Scene.push_material = _scene_push_material
 Scene.set_parameter = _scene_set_material_param
 Scene.set_lights = _scene_set_lights

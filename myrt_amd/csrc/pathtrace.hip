@@ -119,6 +119,20 @@ __device__ inline void eval_bsdf(const Lobes& L, V3 n, V3 wo, V3 wi, V3& f, floa
   pdf = L.ws * (D * nh / (4.0f * vh)) + L.wd * (nl / kPi);
 }
 
+// Octahedral normal of an SDF hit record (16 + 16 bits).
+__device__ inline V3 oct_decode(uint32_t w) {
+  float u = float(w & 0xffffu) * (2.0f / 65535.0f) - 1.0f;
+  float v = float(w >> 16) * (2.0f / 65535.0f) - 1.0f;
+  const float z = 1.0f - fabsf(u) - fabsf(v);
+  if (z < 0.f) {
+    const float uu = (1.0f - fabsf(v)) * (u >= 0.f ? 1.f : -1.f);
+    const float vv = (1.0f - fabsf(u)) * (v >= 0.f ? 1.f : -1.f);
+    u = uu;
+    v = vv;
+  }
+  return normalize(mk(u, v, z));
+}
+
 // Orthonormal basis around n (Duff et al. 2017).
 __device__ inline void onb(V3 n, V3& t, V3& b) {
   const float s = n.z >= 0.f ? 1.0f : -1.0f;
@@ -218,25 +232,36 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
     } else {
       // ---- surface interaction ----
       const uint32_t inst = hit.z, prim = hit.y;
-      const InstanceRecord& ir = sc.instances[inst];
-      const GeoInfo gi = sc.geos[ir.geom];
-      const uint32_t i0 = sc.idx[gi.idx_base + 3 * prim], i1 = sc.idx[gi.idx_base + 3 * prim + 1],
-                     i2 = sc.idx[gi.idx_base + 3 * prim + 2];
-      const float4 p0 = sc.pos[gi.vtx_base + i0], p1 = sc.pos[gi.vtx_base + i1], p2 = sc.pos[gi.vtx_base + i2];
-      const float b1 = float(hit.w & 0xffffu) * (1.0f / 65535.0f), b2 = float(hit.w >> 16) * (1.0f / 65535.0f);
-      const float b0 = 1.0f - b1 - b2;
-      const V3 e1 = mk(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z), e2 = mk(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z);
-      const V3 ngo = cross(e1, e2);
-      V3 nso = ngo;
-      if (gi.has_normals) {
-        const float4 n0 = sc.nrm[gi.vtx_base + i0], n1 = sc.nrm[gi.vtx_base + i1], n2 = sc.nrm[gi.vtx_base + i2];
-        nso = mk(b0 * n0.x + b1 * n1.x + b2 * n2.x, b0 * n0.y + b1 * n1.y + b2 * n2.y, b0 * n0.z + b1 * n1.z + b2 * n2.z);
+      V3 ng, ns;
+      uint32_t mi;
+      if (inst & 0x80000000u) {
+        // SDF surface: the record carries the world normal and the material.
+        ng = oct_decode(hit.w);
+        ns = ng;
+        mi = prim;
+      } else {
+        const InstanceRecord& ir = sc.instances[inst];
+        const GeoInfo gi = sc.geos[ir.geom];
+        const uint32_t i0 = sc.idx[gi.idx_base + 3 * prim], i1 = sc.idx[gi.idx_base + 3 * prim + 1],
+                       i2 = sc.idx[gi.idx_base + 3 * prim + 2];
+        const float4 p0 = sc.pos[gi.vtx_base + i0], p1 = sc.pos[gi.vtx_base + i1], p2 = sc.pos[gi.vtx_base + i2];
+        const float b1 = float(hit.w & 0xffffu) * (1.0f / 65535.0f), b2 = float(hit.w >> 16) * (1.0f / 65535.0f);
+        const float b0 = 1.0f - b1 - b2;
+        const V3 e1 = mk(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z), e2 = mk(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z);
+        const V3 ngo = cross(e1, e2);
+        V3 nso = ngo;
+        if (gi.has_normals) {
+          const float4 n0 = sc.nrm[gi.vtx_base + i0], n1 = sc.nrm[gi.vtx_base + i1], n2 = sc.nrm[gi.vtx_base + i2];
+          nso = mk(b0 * n0.x + b1 * n1.x + b2 * n2.x, b0 * n0.y + b1 * n1.y + b2 * n2.y,
+                   b0 * n0.z + b1 * n1.z + b2 * n2.z);
+        }
+        const float* iv = ir.inv;  // normals transform with the transpose of M^-1
+        ng = normalize(mk(iv[0] * ngo.x + iv[4] * ngo.y + iv[8] * ngo.z, iv[1] * ngo.x + iv[5] * ngo.y + iv[9] * ngo.z,
+                          iv[2] * ngo.x + iv[6] * ngo.y + iv[10] * ngo.z));
+        ns = normalize(mk(iv[0] * nso.x + iv[4] * nso.y + iv[8] * nso.z, iv[1] * nso.x + iv[5] * nso.y + iv[9] * nso.z,
+                          iv[2] * nso.x + iv[6] * nso.y + iv[10] * nso.z));
+        mi = sc.inst_material[inst];
       }
-      const float* iv = ir.inv;  // normals transform with the transpose of M^-1
-      V3 ng = normalize(mk(iv[0] * ngo.x + iv[4] * ngo.y + iv[8] * ngo.z, iv[1] * ngo.x + iv[5] * ngo.y + iv[9] * ngo.z,
-                           iv[2] * ngo.x + iv[6] * ngo.y + iv[10] * ngo.z));
-      V3 ns = normalize(mk(iv[0] * nso.x + iv[4] * nso.y + iv[8] * nso.z, iv[1] * nso.x + iv[5] * nso.y + iv[9] * nso.z,
-                           iv[2] * nso.x + iv[6] * nso.y + iv[10] * nso.z));
       const V3 p = ro + rd * t_hit;
       const V3 wo = -rd;
       const bool entering = dot(rd, ng) < 0.f;
@@ -247,7 +272,6 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
       if (dot(ns, ng) <= 0.f) ns = ng;
       const float eps = 1e-4f * (1.0f + fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z)));
 
-      uint32_t mi = sc.inst_material[inst];
       if (mi >= sc.num_materials) mi = 0;
       const Material mat = sc.materials[mi];
       Lobes L;

@@ -19,6 +19,7 @@
 #include "bvh_build.hpp"
 #include "device_types.hpp"
 #include "scene_internal.hpp"
+#include "sdf_kernels.hpp"
 #include "trace.hpp"
 #include "wide_bvh.hpp"
 
@@ -222,6 +223,11 @@ int Scene::upload_all() {
     if ((rc = upload(d_pos, cap_pos, all_pos)) != MRT_OK) return rc;
     if ((rc = upload(d_nrm, cap_nrm, all_nrm)) != MRT_OK) return rc;
   }
+  if (sdf_dirty) {
+    if ((rc = upload(d_sdf_tape, cap_sdf_tape, sdf_tape)) != MRT_OK) return rc;
+    if ((rc = upload(d_sdf_drawables, cap_sdf_dr, sdf_drawables)) != MRT_OK) return rc;
+    sdf_dirty = false;
+  }
   if (shading_dirty) {
     std::vector<Material> mats = materials;
     if (mats.empty()) {
@@ -249,7 +255,7 @@ int Scene::upload_all() {
 }
 
 Scene::~Scene() {
-  for (void* p : {(void*)d_geo_info, (void*)d_idx, (void*)d_inst_material, (void*)d_pos, (void*)d_nrm,
+  for (void* p : {(void*)d_sdf_drawables, (void*)d_sdf_tape, (void*)d_geo_info, (void*)d_idx, (void*)d_inst_material, (void*)d_pos, (void*)d_nrm,
                   (void*)d_materials, (void*)d_lights})
     if (p) hipFree(p);
   if (d_blas) hipFree(d_blas);
@@ -318,6 +324,19 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.total_threads = total_threads;
   p.refill_below = refill_below;
   if (launch_trace(p, any, stats, grid, stream) != 0) return fail(MRT_ERR_HIP, "trace kernel launch failed");
+  if (!sdf_drawables.empty()) {
+    SdfParams sp{};
+    sp.drawables = d_sdf_drawables;
+    sp.tape = d_sdf_tape;
+    sp.num_drawables = uint32_t(sdf_drawables.size());
+    sp.rays = d_rays;
+    sp.out = d_out;
+    sp.n = uint32_t(n);
+    sp.n_dev = n_dev;
+    sp.steps = sdf_steps;
+    sp.eps = sdf_eps;
+    if (launch_sdf(sp, any, uint32_t(n), stream) != 0) return fail(MRT_ERR_HIP, "sdf kernel launch failed");
+  }
   return MRT_OK;
 }
 
@@ -591,7 +610,34 @@ static bool same_drawables(const std::vector<Instance>& a, const std::vector<Ins
   return true;
 }
 
+// World-space box of an SDF drawable: the tape's object bounds through M, widened by a margin
+// for the marching tolerance.
+static void build_sdf_drawables(Scene* s) {
+  s->sdf_drawables.clear();
+  for (const SdfInstance& in : s->sdf_current) {
+    SdfDrawable d{};
+    std::memcpy(d.inv, in.inv, sizeof(d.inv));
+    d.tape_offset = s->sdf_offset[in.sdf];
+    d.tape_len = s->sdf_len[in.sdf];
+    const Box wb = transform_box(s->sdf_bounds[in.sdf], in.m);
+    double mag = 0;
+    for (int a = 0; a < 3; ++a) mag = std::max(mag, std::max(std::fabs(double(wb.mn[a])), std::fabs(double(wb.mx[a]))));
+    const float pad = float(1e-4 * (1.0 + mag)) + 1e-3f;
+    for (int a = 0; a < 3; ++a) {
+      d.box_lo[a] = wb.mn[a] - pad;
+      d.box_hi[a] = wb.mx[a] + pad;
+    }
+    s->sdf_drawables.push_back(d);
+  }
+  s->sdf_dirty = true;
+}
+
 static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
+  if (!s->sdf_pending.empty()) {
+    s->sdf_current = std::move(s->sdf_pending);
+    s->sdf_pending.clear();
+    build_sdf_drawables(s);
+  }
   // The instance BVH is rebuilt only when this frame's drawables differ from the prepared ones
   // (the reference compares drawable hashes, scene.cpp:336-344).
   if (!s->pending.empty()) {
@@ -601,7 +647,7 @@ static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
     }
     s->pending.clear();
   }
-  const bool gd = s->geoms_dirty, id = s->inst_dirty;
+  const bool gd = s->geoms_dirty, id = s->inst_dirty || s->sdf_dirty;
   int rc = s->build_host();
   if (rc != MRT_OK) return rc;
   if (upload_too) {
@@ -625,6 +671,172 @@ static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
                         s->tlas_nodes.size() * sizeof(WideNode) + s->tlas_items.size() * 4 +
                         s->inst_records.size() * sizeof(InstanceRecord);
   }
+  return MRT_OK;
+}
+
+// Object-space bounds of a tape by interval propagation; also checks the stack discipline.
+static bool sdf_bounds(const SdfNode* t, size_t n, Box& out, std::string& err) {
+  std::vector<Box> st;
+  std::vector<std::pair<int, float>> mods;  // (kind, param) of the open modifiers, innermost last
+  struct Frame {
+    Box b;
+  };
+  std::vector<size_t> mod_depth;  // stack size when each modifier opened
+  std::vector<SdfNode> open;
+  auto apply_mods = [&](Box b) {
+    for (size_t k = open.size(); k-- > 0;) {
+      const SdfNode& m = open[k];
+      if (m.op == kSdfPushTranslate) {
+        for (int a = 0; a < 3; ++a) {
+          b.mn[a] += m.p[a];
+          b.mx[a] += m.p[a];
+        }
+      } else {
+        const int a = int(m.p[0]);
+        const float r = std::max(std::fabs(b.mn[a]), std::fabs(b.mx[a]));
+        b.mn[a] = -r;
+        b.mx[a] = r;
+      }
+    }
+    return b;
+  };
+  for (size_t i = 0; i < n; ++i) {
+    const SdfNode& nd = t[i];
+    Box b;
+    switch (nd.op) {
+      case kSdfSphere:
+        if (!(nd.p[0] > 0.f)) return err = "sphere radius must be positive", false;
+        b.mn = {-nd.p[0], -nd.p[0], -nd.p[0]};
+        b.mx = {nd.p[0], nd.p[0], nd.p[0]};
+        st.push_back(apply_mods(b));
+        break;
+      case kSdfBox:
+        b.mn = {-std::fabs(nd.p[0]), -std::fabs(nd.p[1]), -std::fabs(nd.p[2])};
+        b.mx = {std::fabs(nd.p[0]), std::fabs(nd.p[1]), std::fabs(nd.p[2])};
+        st.push_back(apply_mods(b));
+        break;
+      case kSdfTorus: {
+        const float r = std::fabs(nd.p[0]) + std::fabs(nd.p[1]);
+        b.mn = {-r, -std::fabs(nd.p[1]), -r};
+        b.mx = {r, std::fabs(nd.p[1]), r};
+        st.push_back(apply_mods(b));
+        break;
+      }
+      case kSdfCapsule:
+        if (!(nd.p[0] >= 0.f)) return err = "capsule height must be >= 0", false;
+        b.mn = {-std::fabs(nd.p[1]), -std::fabs(nd.p[1]), -std::fabs(nd.p[1])};
+        b.mx = {std::fabs(nd.p[1]), nd.p[0] + std::fabs(nd.p[1]), std::fabs(nd.p[1])};
+        st.push_back(apply_mods(b));
+        break;
+      case kSdfMenger:
+        b.mn = {-1.f, -1.f, -1.f};
+        b.mx = {1.f, 1.f, 1.f};
+        st.push_back(apply_mods(b));
+        break;
+      case kSdfUnion:
+      case kSdfSubtract:
+      case kSdfIntersect:
+      case kSdfSmoothUnion:
+      case kSdfSmoothSubtract:
+      case kSdfSmoothIntersect: {
+        if (st.size() < 2 || (!mod_depth.empty() && st.size() < mod_depth.back() + 2))
+          return err = "operator needs two operands", false;
+        Box rhs = st.back();
+        st.pop_back();
+        Box lhs = st.back();
+        st.pop_back();
+        const bool smooth = nd.op >= kSdfSmoothUnion;
+        if (smooth && !(nd.p[0] > 0.f)) return err = "smooth operator needs k > 0", false;
+        const float k = smooth ? nd.p[0] : 0.f;
+        Box r;
+        if (nd.op == kSdfUnion || nd.op == kSdfSmoothUnion) {
+          r = lhs;
+          r.enclose(rhs);
+        } else if (nd.op == kSdfSubtract || nd.op == kSdfSmoothSubtract) {
+          r = rhs;
+        } else {
+          r = lhs;
+          r.enclose(rhs);  // conservative for intersections
+        }
+        for (int a = 0; a < 3; ++a) {
+          r.mn[a] -= k;
+          r.mx[a] += k;
+        }
+        st.push_back(r);
+        break;
+      }
+      case kSdfPushTranslate:
+      case kSdfPushMirror:
+        if (nd.op == kSdfPushMirror && !(nd.p[0] == 0.f || nd.p[0] == 1.f || nd.p[0] == 2.f))
+          return err = "mirror axis must be 0, 1 or 2", false;
+        if (open.size() >= size_t(kSdfPosStack)) return err = "more than 4 nested modifiers", false;
+        open.push_back(nd);
+        mod_depth.push_back(st.size());
+        break;
+      case kSdfPop:
+        if (open.empty()) return err = "pop without a modifier", false;
+        if (st.size() != mod_depth.back() + 1) return err = "a modifier must wrap exactly one operand", false;
+        open.pop_back();
+        mod_depth.pop_back();
+        break;
+      default:
+        return err = "unknown sdf instruction " + std::to_string(nd.op), false;
+    }
+    if (st.size() > size_t(kSdfStack)) return err = "operand stack deeper than 8", false;
+  }
+  if (!open.empty()) return err = "unclosed modifier", false;
+  if (st.size() != 1) return err = "the program must leave exactly one distance", false;
+  out = st[0];
+  return true;
+}
+
+int mrt_scene_push_sdf(mrt_scene sh, const mrt_sdf_node* nodes, size_t n, uint32_t* out) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !nodes || n == 0) return fail(MRT_ERR_INVALID, "need a non-empty sdf program");
+  static_assert(sizeof(mrt_sdf_node) == sizeof(SdfNode), "sdf node layouts differ");
+  const SdfNode* t = reinterpret_cast<const SdfNode*>(nodes);
+  Box b;
+  std::string err;
+  if (!sdf_bounds(t, n, b, err)) return fail(MRT_ERR_INVALID, "invalid sdf program: " + err);
+  s->sdf_offset.push_back(uint32_t(s->sdf_tape.size()));
+  s->sdf_len.push_back(uint32_t(n));
+  s->sdf_tape.insert(s->sdf_tape.end(), t, t + n);
+  s->sdf_bounds.push_back(b);
+  s->sdf_dirty = true;
+  if (out) *out = uint32_t(s->sdf_len.size() - 1);
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_enqueue_sdf(mrt_scene sh, uint32_t sdf, const float m4x4_colmajor[16], uint32_t* out) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !m4x4_colmajor) return fail(MRT_ERR_INVALID, "null argument");
+  if (sdf >= s->sdf_len.size()) return fail(MRT_ERR_INVALID, "unknown sdf");
+  SdfInstance in;
+  in.sdf = sdf;
+  std::memcpy(in.m, m4x4_colmajor, sizeof(in.m));
+  if (!invert_affine_rows(in.m, in.inv)) return fail(MRT_ERR_INVALID, "transformation is not invertible");
+  s->sdf_pending.push_back(in);
+  if (out) *out = uint32_t(s->sdf_pending.size() - 1);
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_sdf_drawables(mrt_scene sh, void* out, size_t n) {
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || (n && !out)) return fail(MRT_ERR_INVALID, "null argument");
+  if (n < s->sdf_drawables.size()) return fail(MRT_ERR_INVALID, "output too small");
+  if (!s->sdf_drawables.empty()) std::memcpy(out, s->sdf_drawables.data(), s->sdf_drawables.size() * sizeof(SdfDrawable));
+  return MRT_OK;
+}
+
+int mrt_scene_set_sdf_marching(mrt_scene sh, int steps, float eps) {
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || steps < 1 || !(eps > 0.f)) return fail(MRT_ERR_INVALID, "steps must be >= 1 and eps > 0");
+  s->sdf_steps = steps;
+  s->sdf_eps = eps;
   return MRT_OK;
 }
 

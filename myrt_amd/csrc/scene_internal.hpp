@@ -8,6 +8,7 @@
 #include "bvh_build.hpp"
 #include "device_types.hpp"
 #include "pathtrace.hpp"
+#include "sdf.hpp"
 #include "wide_bvh.hpp"
 
 namespace mrt {
@@ -34,8 +35,27 @@ struct Instance {
   Box world_box;
 };
 
+// An enqueued SDF (scene::enqueue(sdf, mat4), src/pathtracer/scene.cpp:411-417).
+struct SdfInstance {
+  uint32_t sdf = 0;
+  float m[16];
+  float inv[12];
+};
+
 struct Scene {
   std::vector<Geometry> geoms;
+  // SDF programs (tapes concatenated), their object-space bounds and this frame's drawables.
+  std::vector<SdfNode> sdf_tape;
+  std::vector<uint32_t> sdf_offset, sdf_len;
+  std::vector<Box> sdf_bounds;
+  std::vector<SdfInstance> sdf_pending, sdf_current;
+  std::vector<SdfDrawable> sdf_drawables;
+  bool sdf_dirty = false;
+  int sdf_steps = 400;      // sequential_pathtracer.hpp:135
+  float sdf_eps = 1e-5f;    // sequential_pathtracer.hpp:136
+  SdfDrawable* d_sdf_drawables = nullptr;
+  SdfNode* d_sdf_tape = nullptr;
+  size_t cap_sdf_dr = 0, cap_sdf_tape = 0;
   std::vector<Instance> pending;  // enqueued since the last prepare
   std::vector<Instance> current;  // the prepared frame
   bool geoms_dirty = false, inst_dirty = false, prepared = false;

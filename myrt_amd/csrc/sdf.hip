@@ -1,0 +1,315 @@
+// Sphere tracing of SDF drawables for gfx950 (src/pt/raymarch.glsl:95-171 over the tape format of
+// sdf.hpp). Runs after the BVH pass on the same rays: a closer SDF surface replaces the hit
+// record (instance id with bit 31 set, triangle id = material, barycentrics = octahedral normal);
+// for shadow rays any SDF surface inside the segment occludes.
+//
+// The interpreter keeps its distance, material and position stacks in registers and shifts them
+// on push/pop, so there is no dynamically indexed private array (no scratch memory).
+// All binary32 arithmetic is written out; the oracle repeats it operation for operation.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sdf.hpp"
+#include "sdf_kernels.hpp"
+
+namespace mrt {
+namespace sdfk {
+
+constexpr int kBlock = 256;
+
+struct Stack {
+  float d0, d1, d2, d3, d4, d5, d6, d7;
+  uint32_t m0, m1, m2, m3, m4, m5, m6, m7;
+  __device__ void push(float d, uint32_t m) {
+    d7 = d6; d6 = d5; d5 = d4; d4 = d3; d3 = d2; d2 = d1; d1 = d0; d0 = d;
+    m7 = m6; m6 = m5; m5 = m4; m4 = m3; m3 = m2; m2 = m1; m1 = m0; m0 = m;
+  }
+  // Replaces the top two entries (lhs = d1, rhs = d0) by one.
+  __device__ void reduce(float d, uint32_t m) {
+    d0 = d; d1 = d2; d2 = d3; d3 = d4; d4 = d5; d5 = d6; d6 = d7;
+    m0 = m; m1 = m2; m2 = m3; m3 = m4; m4 = m5; m5 = m6; m6 = m7;
+  }
+};
+
+struct PosStack {
+  float x0, y0, z0, x1, y1, z1, x2, y2, z2, x3, y3, z3;
+  __device__ void push(float x, float y, float z) {
+    x3 = x2; y3 = y2; z3 = z2; x2 = x1; y2 = y1; z2 = z1; x1 = x0; y1 = y0; z1 = z0;
+    x0 = x; y0 = y; z0 = z;
+  }
+  __device__ void pop(float& x, float& y, float& z) {
+    x = x0; y = y0; z = z0;
+    x0 = x1; y0 = y1; z0 = z1; x1 = x2; y1 = y2; z1 = z2; x2 = x3; y2 = y3; z2 = z3;
+  }
+};
+
+__device__ inline float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+__device__ inline float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+__device__ inline float mixf(float a, float b, float h) { return a * (1.0f - h) + b * h; }
+
+__device__ inline float menger(float px, float py, float pz) {
+  float x = px, y = py, z = pz;
+  for (int n = 0; n < 4; ++n) {
+    x = fabsf(x);
+    y = fabsf(y);
+    z = fabsf(z);
+    if (x < y) { const float t = x; x = y; y = t; }
+    if (x < z) { const float t = x; x = z; z = t; }
+    if (y < z) { const float t = y; y = z; z = t; }
+    z = z - 1.0f / 3.0f;
+    z = -fabsf(z);
+    z = z + 1.0f / 3.0f;
+    x = x * 3.0f;
+    y = y * 3.0f;
+    z = z * 3.0f;
+    x = x - 2.0f;
+    y = y - 2.0f;
+  }
+  const float dx = fabsf(x) - 1.0f, dy = fabsf(y) - 1.0f, dz = fabsf(z) - 1.0f;
+  const float inner = fminf(fmaxf(dx, fmaxf(dy, dz)), 0.0f);
+  const float dis = inner + len3(fmaxf(dx, 0.0f), fmaxf(dy, 0.0f), fmaxf(dz, 0.0f));
+  return dis * (1.0f / 81.0f);
+}
+
+// map(p): distance and material of a tape at object-space point p.
+__device__ inline float map(const SdfNode* tape, uint32_t len, float px, float py, float pz, uint32_t& mat) {
+  Stack st{};
+  PosStack ps{};
+  float x = px, y = py, z = pz;
+  for (uint32_t i = 0; i < len; ++i) {
+    const SdfNode nd = tape[i];
+    switch (nd.op) {
+      case kSdfSphere:
+        st.push(len3(x, y, z) - nd.p[0], nd.material);
+        break;
+      case kSdfBox: {
+        const float qx = fabsf(x) - nd.p[0], qy = fabsf(y) - nd.p[1], qz = fabsf(z) - nd.p[2];
+        const float d = len3(fmaxf(qx, 0.0f), fmaxf(qy, 0.0f), fmaxf(qz, 0.0f)) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+        st.push(d, nd.material);
+        break;
+      }
+      case kSdfTorus: {
+        const float qx = sqrtf(x * x + z * z) - nd.p[0];
+        st.push(sqrtf(qx * qx + y * y) - nd.p[1], nd.material);
+        break;
+      }
+      case kSdfCapsule: {
+        const float yy = y - fminf(fmaxf(y, 0.0f), nd.p[0]);
+        st.push(len3(x, yy, z) - nd.p[1], nd.material);
+        break;
+      }
+      case kSdfMenger:
+        st.push(menger(x, y, z), nd.material);
+        break;
+      case kSdfUnion: {
+        const float m = fminf(st.d1, st.d0);
+        st.reduce(m, m == st.d0 ? st.m0 : st.m1);
+        break;
+      }
+      case kSdfSubtract: {
+        const float m = fmaxf(-st.d1, st.d0);
+        st.reduce(m, m == st.d0 ? st.m0 : st.m1);
+        break;
+      }
+      case kSdfIntersect: {
+        const float m = fmaxf(st.d1, st.d0);
+        st.reduce(m, m == st.d0 ? st.m0 : st.m1);
+        break;
+      }
+      case kSdfSmoothUnion: {
+        const float d1 = st.d1, d2 = st.d0, k = nd.p[0];
+        const float h = clamp01(0.5f + 0.5f * (d2 - d1) / k);
+        st.reduce(mixf(d2, d1, h) - k * h * (1.0f - h), h >= 0.5f ? st.m1 : st.m0);
+        break;
+      }
+      case kSdfSmoothSubtract: {
+        const float d1 = st.d1, d2 = st.d0, k = nd.p[0];
+        const float h = clamp01(0.5f - 0.5f * (d2 + d1) / k);
+        st.reduce(mixf(d2, -d1, h) + k * h * (1.0f - h), h >= 0.5f ? st.m1 : st.m0);
+        break;
+      }
+      case kSdfSmoothIntersect: {
+        const float d1 = st.d1, d2 = st.d0, k = nd.p[0];
+        const float h = clamp01(0.5f - 0.5f * (d2 - d1) / k);
+        st.reduce(mixf(d2, d1, h) + k * h * (1.0f - h), h >= 0.5f ? st.m1 : st.m0);
+        break;
+      }
+      case kSdfPushTranslate:
+        ps.push(x, y, z);
+        x = x - nd.p[0];
+        y = y - nd.p[1];
+        z = z - nd.p[2];
+        break;
+      case kSdfPushMirror: {
+        ps.push(x, y, z);
+        const int axis = int(nd.p[0]);
+        if (axis == 0) x = fabsf(x);
+        else if (axis == 1) y = fabsf(y);
+        else z = fabsf(z);
+        break;
+      }
+      case kSdfPop:
+        ps.pop(x, y, z);
+        break;
+      default:
+        break;
+    }
+  }
+  mat = st.m0;
+  return st.d0;
+}
+
+__device__ inline uint32_t oct_encode(float x, float y, float z) {
+  const float s = fabsf(x) + fabsf(y) + fabsf(z);
+  float u = x / s, v = y / s;
+  if (z < 0.f) {
+    const float uu = (1.0f - fabsf(v)) * (u >= 0.f ? 1.f : -1.f);
+    const float vv = (1.0f - fabsf(u)) * (v >= 0.f ? 1.f : -1.f);
+    u = uu;
+    v = vv;
+  }
+  const uint32_t a = uint32_t(__saturatef(u * 0.5f + 0.5f) * 65535.f + 0.5f);
+  const uint32_t b = uint32_t(__saturatef(v * 0.5f + 0.5f) * 65535.f + 0.5f);
+  return a | (b << 16);
+}
+
+// Slab clip of the world ray against a drawable's box: returns false if it misses [t0, t1].
+__device__ inline bool clip(const SdfDrawable& dr, float ox, float oy, float oz, float dx, float dy, float dz, float& t0,
+                            float& t1) {
+  const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+  float a = (dr.box_lo[0] - ox) * ix, b = (dr.box_hi[0] - ox) * ix;
+  float lo = fminf(a, b), hi = fmaxf(a, b);
+  a = (dr.box_lo[1] - oy) * iy;
+  b = (dr.box_hi[1] - oy) * iy;
+  lo = fmaxf(lo, fminf(a, b));
+  hi = fminf(hi, fmaxf(a, b));
+  a = (dr.box_lo[2] - oz) * iz;
+  b = (dr.box_hi[2] - oz) * iz;
+  lo = fmaxf(lo, fminf(a, b));
+  hi = fminf(hi, fmaxf(a, b));
+  t0 = fmaxf(t0, lo);
+  t1 = fminf(t1, hi * 1.0000005f);
+  return t0 <= t1;
+}
+
+// march_sdf (raymarch.glsl:116-171): world hit distance or -1.
+__device__ inline float march(const SdfDrawable& dr, const SdfNode* tape, float ox, float oy, float oz, float dx,
+                              float dy, float dz, float t_enter, float t_exit, int steps, float eps, float& lx,
+                              float& ly, float& lz, float& len_out) {
+  const float* m = dr.inv;
+  lx = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
+  ly = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
+  lz = m[8] * ox + m[9] * oy + m[10] * oz + m[11];
+  const float rx = m[0] * dx + m[1] * dy + m[2] * dz;
+  const float ry = m[4] * dx + m[5] * dy + m[6] * dz;
+  const float rz = m[8] * dx + m[9] * dy + m[10] * dz;
+  const float len = len3(rx, ry, rz);
+  len_out = len;
+  float t = fmaxf(10.0f * eps, t_enter * len);
+  int si = 0;
+  float res = -1.0f;
+  for (int i = 0; i < steps && t / len < t_exit; ++i) {
+    const float s = t / len;
+    uint32_t mat;
+    const float d = map(tape, dr.tape_len, lx + rx * s, ly + ry * s, lz + rz * s, mat);
+    if (si == 0) si = d > 0.f ? 1 : (d < 0.f ? -1 : 0);
+    if (fabsf(d) < eps) {
+      res = t;
+      break;
+    }
+    t = t + float(si) * d;
+  }
+  // Keep the object-space ray for the normal.
+  const float tw = res < 0.f ? -1.0f : res / len;
+  lx = res < 0.f ? lx : lx + rx * tw;
+  ly = res < 0.f ? ly : ly + ry * tw;
+  lz = res < 0.f ? lz : lz + rz * tw;
+  return tw;
+}
+
+template <bool ANY>
+__global__ __launch_bounds__(kBlock) void sdf_kernel(SdfParams p) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = p.n_dev ? *p.n_dev : p.n;
+  if (i >= n) return;
+  const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(i);
+  const float4 r0 = rp[0], r1 = rp[1];
+  if (!(r1.x != 0.f || r1.y != 0.f || r1.z != 0.f)) return;
+  float limit = r1.w;
+  if (ANY) {
+    if (reinterpret_cast<const uint8_t*>(p.out)[i]) return;
+  } else {
+    const uint4 h = reinterpret_cast<const uint4*>(p.out)[i];
+    if (h.z != 0xFFFFFFFFu) limit = fminf(limit, __uint_as_float(h.x));
+  }
+  float best = limit;
+  int best_j = -1;
+  float bx = 0, by = 0, bz = 0, blen = 1;
+  for (uint32_t j = 0; j < p.num_drawables; ++j) {
+    const SdfDrawable dr = p.drawables[j];
+    float t0 = r0.w, t1 = best;
+    if (!clip(dr, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1)) continue;
+    float lx, ly, lz, len;
+    const float tw = march(dr, p.tape + dr.tape_offset, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1, p.steps, p.eps, lx,
+                           ly, lz, len);
+    if (tw >= 0.f && tw < best && tw > r0.w) {
+      best = tw;
+      best_j = int(j);
+      bx = lx;
+      by = ly;
+      bz = lz;
+      blen = len;
+      if (ANY) break;
+    }
+  }
+  if (best_j < 0) return;
+  if (ANY) {
+    reinterpret_cast<uint8_t*>(p.out)[i] = 1;
+    return;
+  }
+  // Normal: the tetrahedral gradient of raymarch.glsl:100-114, offset 0.0005 / len.
+  const SdfDrawable dr = p.drawables[best_j];
+  const SdfNode* tape = p.tape + dr.tape_offset;
+  float nx = 0, ny = 0, nz = 0;
+  for (int k = 0; k < 4; ++k) {
+    const float ex = 0.5773f * (2.0f * float(((k + 3) >> 1) & 1) - 1.0f);
+    const float ey = 0.5773f * (2.0f * float((k >> 1) & 1) - 1.0f);
+    const float ez = 0.5773f * (2.0f * float(k & 1) - 1.0f);
+    const float o = 0.0005f / blen;
+    uint32_t mm;
+    const float d = map(tape, dr.tape_len, bx + o * ex, by + o * ey, bz + o * ez, mm) * blen;
+    nx = nx + ex * d;
+    ny = ny + ey * d;
+    nz = nz + ez * d;
+  }
+  uint32_t mat;
+  map(tape, dr.tape_len, bx, by, bz, mat);
+  const float* iv = dr.inv;  // world normal = (M^-1)^T n
+  float wx = iv[0] * nx + iv[4] * ny + iv[8] * nz;
+  float wy = iv[1] * nx + iv[5] * ny + iv[9] * nz;
+  float wz = iv[2] * nx + iv[6] * ny + iv[10] * nz;
+  const float l = len3(wx, wy, wz);
+  wx = wx / l;
+  wy = wy / l;
+  wz = wz / l;
+  uint4 h;
+  h.x = __float_as_uint(best);
+  h.y = mat;
+  h.z = 0x80000000u | uint32_t(best_j);
+  h.w = oct_encode(wx, wy, wz);
+  reinterpret_cast<uint4*>(p.out)[i] = h;
+}
+
+}  // namespace sdfk
+
+int launch_sdf(const SdfParams& p, bool any, uint32_t n_max, hipStream_t stream) {
+  const uint32_t grid = (n_max + sdfk::kBlock - 1) / sdfk::kBlock;
+  if (any)
+    hipLaunchKernelGGL(sdfk::sdf_kernel<true>, dim3(grid ? grid : 1), dim3(sdfk::kBlock), 0, stream, p);
+  else
+    hipLaunchKernelGGL(sdfk::sdf_kernel<false>, dim3(grid ? grid : 1), dim3(sdfk::kBlock), 0, stream, p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace mrt

@@ -1,0 +1,50 @@
+// Signed-distance programs: the tape format shared by the host (scene.cpp) and sdf.hip.
+//
+// The reference compiles an sdf:: tree of primitives, operators and modifiers into a GLSL map()
+// function by string generation (src/pathtracer/sdf.cpp:122-407). Here the tree is a postfix
+// tape of 32-byte instructions evaluated by one device interpreter: primitives push a distance
+// (and material), operators pop two and push one, PUSH_* modifiers save the position and
+// transform it for the following subtree, POP restores it.
+#pragma once
+#include <cstdint>
+
+namespace mrt {
+
+enum SdfOp : uint32_t {
+  kSdfSphere = 1,       // p0 radius                         (sdf.hpp:399-414)
+  kSdfBox = 2,          // p0-p2 half extents                 (sdf.hpp:524-542)
+  kSdfTorus = 3,        // p0 large radius, p1 small radius   (sdf.hpp:545-565)
+  kSdfCapsule = 4,      // vertical: p0 height, p1 radius     (sdf.hpp:603-618)
+  kSdfMenger = 5,       // unit Menger sponge, 4 iterations   (sdf.hpp:567-601)
+  kSdfUnion = 16,       // min(a, b)                          (sdf.hpp:429-438)
+  kSdfSubtract = 17,    // max(-a, b)                         (sdf.hpp:440-449)
+  kSdfIntersect = 18,   // max(a, b)                          (sdf.hpp:451-460)
+  kSdfSmoothUnion = 19,      // p0 k                          (sdf.hpp:462-482)
+  kSdfSmoothSubtract = 20,   // p0 k                          (sdf.hpp:484-502)
+  kSdfSmoothIntersect = 21,  // p0 k                          (sdf.hpp:504-522)
+  kSdfPushTranslate = 32,    // p -= (p0, p1, p2)             (sdf.hpp:416-427)
+  kSdfPushMirror = 33,       // p[p0] = |p[p0]|               (sdf.hpp:620-632)
+  kSdfPop = 34,              // restore the saved position
+};
+
+struct SdfNode {
+  uint32_t op;
+  uint32_t material;
+  float p[6];
+};
+static_assert(sizeof(SdfNode) == 32, "sdf node is 32 bytes");
+
+constexpr int kSdfStack = 8;     // distance/material stack depth of the interpreter
+constexpr int kSdfPosStack = 4;  // nested modifier depth
+
+// One enqueued SDF drawable (drawable_sdf_t, src/pt/pt_trace.comp:49-54), 80 B.
+struct SdfDrawable {
+  float inv[12];      // rows 0-2 of M^-1
+  float box_lo[3];    // world-space bounds of the surface (for ray clipping)
+  uint32_t tape_offset;
+  float box_hi[3];
+  uint32_t tape_len;
+};
+static_assert(sizeof(SdfDrawable) == 80, "sdf drawable is 80 bytes");
+
+}  // namespace mrt

@@ -1,0 +1,25 @@
+// Launcher of the SDF sphere-tracing pass (sdf.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sdf.hpp"
+
+namespace mrt {
+
+struct SdfParams {
+  const SdfDrawable* drawables;
+  const SdfNode* tape;
+  uint32_t num_drawables;
+  const void* rays;        // mrt_ray[n]
+  void* out;               // mrt_hit[n] (closest) or uint8_t[n] (occluded)
+  uint32_t n;
+  const uint32_t* n_dev;   // device-side ray count, if set
+  int steps;               // sdf_marching_steps (sequential_pathtracer.hpp:135, default 400)
+  float eps;               // sdf_marching_epsilon (default 1e-5)
+};
+
+int launch_sdf(const SdfParams& p, bool any, uint32_t n_max, hipStream_t stream);
+
+}  // namespace mrt

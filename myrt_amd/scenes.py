@@ -288,3 +288,41 @@ def camera_matrices(eye, target, fovy, aspect):
     proj[2, 3] = 2 * far * near / (near - far)
     proj[3, 2] = -1.0
     return view.astype(np.float32), proj.astype(np.float32)
+
+
+# ---- SDF programs -----------------------------------------------------------------------------
+
+SDF_NODE_DTYPE = np.dtype([("op", "<u4"), ("material", "<u4"), ("p", "<f4", 6)])
+_SDF_OPS = dict(SPHERE=1, BOX=2, TORUS=3, CAPSULE=4, MENGER=5, UNION=16, SUBTRACT=17, INTERSECT=18, SMOOTH_UNION=19,
+                SMOOTH_SUBTRACT=20, SMOOTH_INTERSECT=21, PUSH_TRANSLATE=32, PUSH_MIRROR=33, POP=34)
+
+
+def sdf_tape(nodes):
+    """Tape from (op_name, material, *params) tuples."""
+    t = np.zeros(len(nodes), SDF_NODE_DTYPE)
+    for i, nd in enumerate(nodes):
+        t[i]["op"] = _SDF_OPS[nd[0]]
+        t[i]["material"] = nd[1] if len(nd) > 1 else 0
+        ps = list(nd[2:])
+        t[i]["p"][:len(ps)] = ps
+    return t
+
+
+def sdf_cluster(n=64, seed=12, k=0.1, box=1.5, n_materials=4):
+    """n spheres and boxes (alternating) at centres U[-box, box]^3, sizes U[0.15, 0.45], each
+    wrapped in a translation, joined left to right by smooth_union(k)."""
+    rng = np.random.default_rng(seed)
+    nodes = []
+    for i in range(n):
+        c = rng.uniform(-box, box, 3)
+        s = rng.uniform(0.15, 0.45)
+        mat = int(rng.integers(0, n_materials))
+        nodes.append(("PUSH_TRANSLATE", 0, *c))
+        if i % 2 == 0:
+            nodes.append(("SPHERE", mat, s))
+        else:
+            nodes.append(("BOX", mat, s, s * 0.8, s * 0.6))
+        nodes.append(("POP", 0))
+        if i > 0:
+            nodes.append(("SMOOTH_UNION", 0, k))
+    return sdf_tape(nodes)

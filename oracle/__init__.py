@@ -27,7 +27,9 @@ class _Scene(ctypes.Structure):
 class _PtScene(ctypes.Structure):
     _fields_ = [("geo", ctypes.POINTER(_Scene)), ("nrm", ctypes.c_void_p), ("geo_has_nrm", ctypes.c_void_p),
                 ("inst_material", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32),
-                ("lights", ctypes.c_void_p), ("n_lights", ctypes.c_uint32), ("env_scale", ctypes.c_double)]
+                ("lights", ctypes.c_void_p), ("n_lights", ctypes.c_uint32), ("env_scale", ctypes.c_double),
+                ("sdf_tape", ctypes.c_void_p), ("sdf_drawables", ctypes.c_void_p), ("n_sdf", ctypes.c_uint32),
+                ("sdf_steps", ctypes.c_int), ("sdf_eps", ctypes.c_float)]
 
 
 MATERIAL_DTYPE = np.dtype([("albedo_rgba8", "<u4"), ("ior", "<f4"), ("roughness", "<f4"), ("metallic", "<f4"),
@@ -64,6 +66,9 @@ def lib():
             "orc_reference_trace": ([_P, _P, _SZ, ctypes.c_int, _P], None),
             "orc_reference_time": ([_P, _P, _SZ, ctypes.c_int, ctypes.c_int, _SZ, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int)], ctypes.c_double),
+            "orc_sdf_map": ([_P, ctypes.c_uint32, _P, _P], ctypes.c_float),
+            "orc_sdf_trace": ([_P, _P, ctypes.c_uint32, _P, _SZ, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P,
+                               _P, ctypes.c_int], None),
             "orc_render": ([ctypes.POINTER(_PtScene), _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                             ctypes.c_int, ctypes.c_int, _P, ctypes.c_int], ctypes.c_int),
         }
@@ -206,11 +211,41 @@ class OracleScene:
             pass
 
 
+SDF_NODE_DTYPE = np.dtype([("op", "<u4"), ("material", "<u4"), ("p", "<f4", 6)])
+SDF_DRAWABLE_DTYPE = np.dtype([("inv", "<f4", 12), ("box_lo", "<f4", 3), ("tape_offset", "<u4"),
+                               ("box_hi", "<f4", 3), ("tape_len", "<u4")])
+
+
+def sdf_map(tape, p):
+    tape = np.ascontiguousarray(tape, SDF_NODE_DTYPE)
+    mat = ctypes.c_uint32()
+    d = lib().orc_sdf_map(_p(tape), tape.size, _p(np.ascontiguousarray(p, np.float32)), ctypes.byref(mat))
+    return d, mat.value
+
+
+def sdf_trace(tape, drawables, rays, hits=None, any_hit=False, steps=400, eps=1e-5, threads=0):
+    """SDF march on top of `hits` (from OracleScene.brute/bvh); returns (hits, world normals)."""
+    tape = np.ascontiguousarray(tape, SDF_NODE_DTYPE)
+    drawables = np.ascontiguousarray(drawables, SDF_DRAWABLE_DTYPE)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    if hits is None:
+        hits = np.zeros(rays.size, HIT_DTYPE)
+        hits["t"] = np.inf
+        hits["prim"] = hits["inst"] = 0xFFFFFFFF
+    hits = np.ascontiguousarray(hits.copy(), HIT_DTYPE)
+    normals = np.zeros((rays.size, 3), np.float32)
+    lib().orc_sdf_trace(_p(tape), _p(drawables), drawables.size, _p(rays), rays.size, 1 if any_hit else 0, steps,
+                        eps, _p(hits), _p(normals), threads)
+    return hits, normals
+
+
 class OracleRender:
     """FP64 render of a scene description with the product's sampler sequence."""
 
-    def __init__(self, geometries, instances, materials=None, lights=None, env_scale=1.0):
-        """geometries: list of (indices, points, normals or None); instances: (geom, 4x4, material)."""
+    def __init__(self, geometries, instances, materials=None, lights=None, env_scale=1.0, sdf_tape=None,
+                 sdf_drawables=None, sdf_steps=400, sdf_eps=1e-5):
+        """geometries: list of (indices, points, normals or None); instances: (geom, 4x4, material);
+        sdf_drawables: the product's prepared drawable records (mrt_scene_sdf_drawables)."""
         self.scene = OracleScene([(g[0], g[1]) for g in geometries], [(g, mt) for g, mt, _ in instances])
         nrm = []
         for ind, pts, nr in geometries:
@@ -224,9 +259,14 @@ class OracleRender:
         self.mats = np.ascontiguousarray(mats, MATERIAL_DTYPE)
         self.lights = np.ascontiguousarray(np.zeros(0, LIGHT_DTYPE) if lights is None else lights, LIGHT_DTYPE)
         self._lights_buf = self.lights if self.lights.size else np.zeros(1, LIGHT_DTYPE)
+        self.sdf_tape = None if sdf_tape is None else np.ascontiguousarray(sdf_tape, SDF_NODE_DTYPE)
+        self.sdf_dr = None if sdf_drawables is None else np.ascontiguousarray(sdf_drawables, SDF_DRAWABLE_DTYPE)
+        n_sdf = 0 if self.sdf_dr is None else self.sdf_dr.size
         self.c = _PtScene(ctypes.pointer(self.scene.c), self.nrm.ctypes.data, self.has_nrm.ctypes.data,
                           self.inst_mat.ctypes.data, self.mats.ctypes.data, self.mats.size,
-                          self._lights_buf.ctypes.data, self.lights.size, float(env_scale))
+                          self._lights_buf.ctypes.data, self.lights.size, float(env_scale),
+                          None if not n_sdf else self.sdf_tape.ctypes.data,
+                          None if not n_sdf else self.sdf_dr.ctypes.data, n_sdf, int(sdf_steps), float(sdf_eps))
 
     def render(self, view, proj, width, height, spp, max_depth, sample_base=0, threads=0):
         """Mean radiance per pixel, shape (height, width, 3), float64 (rows from the bottom)."""

@@ -127,6 +127,11 @@ typedef struct orc_pt_scene {
   const float* lights;           /* 12 floats per quad light: corner, u, v, emission */
   uint32_t n_lights;
   double env_scale;
+  const void* sdf_tape;          /* orc_sdf_node[], may be NULL */
+  const void* sdf_drawables;     /* orc_sdf_drawable[] */
+  uint32_t n_sdf;
+  int sdf_steps;
+  float sdf_eps;
 } orc_pt_scene;
 
 /* The product's wavefront estimator in binary64: camera (inverse(proj * view), with the
@@ -136,6 +141,31 @@ typedef struct orc_pt_scene {
  * Traces through the oracle BVH `a` built over s->geo. Returns 0 on success. */
 int orc_render(const orc_pt_scene* s, orc_accel a, const float view_colmajor[16], const float proj_colmajor[16],
                int width, int height, int spp, int sample_base, int max_depth, double* out_rgb, int threads);
+
+/* ---- SDF sphere tracing -------------------------------------------------------------------- */
+typedef struct orc_sdf_node {
+  uint32_t op;
+  uint32_t material;
+  float p[6];
+} orc_sdf_node;
+
+typedef struct orc_sdf_drawable {
+  float inv[12];
+  float box_lo[3];
+  uint32_t tape_offset;
+  float box_hi[3];
+  uint32_t tape_len;
+} orc_sdf_drawable;
+
+/* map(p) of a tape (the formulas of src/pathtracer/sdf.hpp:399-633 in postfix form), binary32. */
+float orc_sdf_map(const orc_sdf_node* tape, uint32_t len, const float p[3], uint32_t* material);
+/* march_sdf over every drawable (src/pt/raymarch.glsl:116-171), starting at the drawable box's
+ * entry, on top of prior hits: closest (hits[i] from a triangle query; a closer SDF surface
+ * replaces it, inst = 0x80000000 | drawable, prim = material, normals[3i..3i+2] = world normal
+ * when `normals` is given) or any (hits[i].t finite = occluded). */
+void orc_sdf_trace(const orc_sdf_node* tape, const orc_sdf_drawable* drawables, uint32_t n_drawables,
+                   const orc_ray* rays, size_t n, int any_hit, int steps, float eps, orc_hit* hits, float* normals,
+                   int threads);
 
 #ifdef __cplusplus
 }

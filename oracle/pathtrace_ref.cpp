@@ -147,6 +147,10 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
     r.tmax = INFINITY;
     orc_hit hit;
     orc_accel_trace(C.a, &r, 1, 0, &hit, 1);
+    float sdf_n[3] = {0, 0, 0};
+    if (S->n_sdf)
+      orc_sdf_trace(static_cast<const orc_sdf_node*>(S->sdf_tape), static_cast<const orc_sdf_drawable*>(S->sdf_drawables),
+                    S->n_sdf, &r, 1, 0, S->sdf_steps, S->sdf_eps, &hit, sdf_n, 1);
     const bool found = hit.inst != 0xFFFFFFFFu;
     const double t_hit = found ? double(hit.t) : INFINITY;
     // The product works on the binary32 ray it stored; continue from the same values.
@@ -188,25 +192,35 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
       break;
     }
     const uint32_t inst = hit.inst, prim = hit.prim;
-    const uint32_t g = G->inst_geom[inst];
-    const uint32_t* tri = G->idx + G->geo_idx_off[g] + 3 * prim;
-    const float* P = G->pos + 3 * size_t(G->geo_vtx_off[g]);
-    const D3 p0 = ld(P + 3 * tri[0]), p1 = ld(P + 3 * tri[1]), p2 = ld(P + 3 * tri[2]);
-    const double b1 = std::round(std::fmin(std::fmax(double(hit.b1), 0.0), 1.0) * 65535.0) / 65535.0;
-    const double b2 = std::round(std::fmin(std::fmax(double(hit.b2), 0.0), 1.0) * 65535.0) / 65535.0;
-    const double b0 = 1.0 - b1 - b2;
-    const D3 ngo = cross(p1 - p0, p2 - p0);
-    D3 nso = ngo;
-    if (S->geo_has_nrm[g]) {
-      const float* N = S->nrm + 3 * size_t(G->geo_vtx_off[g]);
-      nso = ld(N + 3 * tri[0]) * b0 + ld(N + 3 * tri[1]) * b1 + ld(N + 3 * tri[2]) * b2;
+    D3 ng, ns;
+    uint32_t mi;
+    if (inst & 0x80000000u) {
+      ng = normalize(D3{sdf_n[0], sdf_n[1], sdf_n[2]});
+      ns = ng;
+      mi = prim;
+    } else {
+      const uint32_t g = G->inst_geom[inst];
+      const uint32_t* tri = G->idx + G->geo_idx_off[g] + 3 * prim;
+      const float* P = G->pos + 3 * size_t(G->geo_vtx_off[g]);
+      const D3 p0 = ld(P + 3 * tri[0]), p1 = ld(P + 3 * tri[1]), p2 = ld(P + 3 * tri[2]);
+      const double b1 = std::round(std::fmin(std::fmax(double(hit.b1), 0.0), 1.0) * 65535.0) / 65535.0;
+      const double b2 = std::round(std::fmin(std::fmax(double(hit.b2), 0.0), 1.0) * 65535.0) / 65535.0;
+      const double b0 = 1.0 - b1 - b2;
+      const D3 ngo = cross(p1 - p0, p2 - p0);
+      D3 nso = ngo;
+      if (S->geo_has_nrm[g]) {
+        const float* N = S->nrm + 3 * size_t(G->geo_vtx_off[g]);
+        nso = ld(N + 3 * tri[0]) * b0 + ld(N + 3 * tri[1]) * b1 + ld(N + 3 * tri[2]) * b2;
+      }
+      const float* iv = G->inst_inv + 12 * inst;
+      auto xf = [&](D3 n) {
+        return normalize(D3{iv[0] * n.x + iv[4] * n.y + iv[8] * n.z, iv[1] * n.x + iv[5] * n.y + iv[9] * n.z,
+                            iv[2] * n.x + iv[6] * n.y + iv[10] * n.z});
+      };
+      ng = xf(ngo);
+      ns = xf(nso);
+      mi = S->inst_material[inst];
     }
-    const float* iv = G->inst_inv + 12 * inst;
-    auto xf = [&](D3 n) {
-      return normalize(D3{iv[0] * n.x + iv[4] * n.y + iv[8] * n.z, iv[1] * n.x + iv[5] * n.y + iv[9] * n.z,
-                          iv[2] * n.x + iv[6] * n.y + iv[10] * n.z});
-    };
-    D3 ng = xf(ngo), ns = xf(nso);
     const D3 p = ro + rd * t_hit;
     const D3 wo = -rd;
     const bool entering = dot(rd, ng) < 0;
@@ -216,7 +230,6 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
     }
     if (dot(ns, ng) <= 0) ns = ng;
     const double eps = 1e-4 * (1.0 + std::max(std::max(std::fabs(p.x), std::fabs(p.y)), std::fabs(p.z)));
-    uint32_t mi = S->inst_material[inst];
     if (mi >= S->n_materials) mi = 0;
     const orc_material& mat = S->materials[mi];
     Lobes L;
@@ -266,6 +279,10 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
           sr.tmax = 1.0f - 1e-3f;
           orc_hit sh;
           orc_accel_trace(C.a, &sr, 1, 1, &sh, 1);
+          if (S->n_sdf && !std::isfinite(sh.t))
+            orc_sdf_trace(static_cast<const orc_sdf_node*>(S->sdf_tape),
+                          static_cast<const orc_sdf_drawable*>(S->sdf_drawables), S->n_sdf, &sr, 1, 1, S->sdf_steps,
+                          S->sdf_eps, &sh, nullptr, 1);
           if (!std::isfinite(sh.t)) sum = sum + c;
         }
       }

@@ -98,3 +98,24 @@ def test_direct_light_closed_form_on_gpu(ensure_built):
     expect = (128 / 255) / np.pi * E
     got = r.image()[1, 1, 0]
     assert abs(got - expect) / expect < 0.01
+
+
+def test_render_with_sdf_matches_fp64_oracle(ensure_built):
+    sc = scenes.sphere_field_scene(20, 2, seed=4, glass_every=0, metal_every=3)
+    tape = scenes.sdf_cluster(64, seed=12, n_materials=4)
+    W, H, spp, depth = 80, 60, 32, 4
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
+    s = gpu_scene(sc)
+    sid = s.push_sdf(tape)
+    s.enqueue_sdf(sid, scenes.translate_scale((0.0, -1.0, 0.0), 1.5))
+    r = m.Renderer()
+    r.set_camera(view, proj)
+    r.set_num_bounces(depth)
+    r.render(s, W, H, spp)
+    img = r.image()[..., :3].astype(np.float64)
+    dr = s.sdf_drawables(1)
+    ref = orc.OracleRender(sc["geometries"], sc["instances"], sc["materials"], sc["lights"], 1.0, sdf_tape=tape,
+                           sdf_drawables=dr).render(view, proj, W, H, spp, depth)
+    err = rel_mse(img, ref)
+    say("sdf render relative MSE %.3e" % err)
+    assert err <= REL_MSE_TOL
