@@ -273,6 +273,11 @@ int mrt_renderer_destroy(mrt_renderer r);
 int mrt_renderer_set_camera(mrt_renderer r, const float view_colmajor[16], const float proj_colmajor[16]);
 /* set_num_bounces (default 10, sequential_pathtracer.hpp:134). */
 int mrt_renderer_set_num_bounces(mrt_renderer r, int n);
+/* Attaches a radiance cache (no reference counterpart). mode 0: off; 1: train only (every pass
+ * builds a training batch from a `train_fraction` of paths, which stay unbiased, and takes
+ * `train_steps` Adam steps); 2: train and end the other paths in the cache at their secondary
+ * vertex (roughness >= 0.25, or any later vertex; not on dielectrics). */
+int mrt_renderer_set_cache(mrt_renderer r, mrt_cache c, int mode, float train_fraction, int train_steps);
 /* invalidate_counter / sample_count. */
 int mrt_renderer_invalidate(mrt_renderer r);
 int mrt_renderer_sample_count(mrt_renderer r, uint32_t* out);

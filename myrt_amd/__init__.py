@@ -118,6 +118,7 @@ SIGNATURES = {
     "mrt_renderer_set_camera": ([_P, _P, _P], ctypes.c_int),
     "mrt_renderer_set_num_bounces": ([_P, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_invalidate": ([_P], ctypes.c_int),
+    "mrt_renderer_set_cache": ([_P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_sample_count": ([_P, _U32P], ctypes.c_int),
     "mrt_render": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_read": ([_P, _P], ctypes.c_int),
@@ -527,6 +528,10 @@ class Renderer:
 
     def invalidate(self):
         _check(lib().mrt_renderer_invalidate(self.h))
+
+    def set_cache(self, cache, mode=2, train_fraction=1.0 / 16, train_steps=4):
+        _check(lib().mrt_renderer_set_cache(self.h, cache.h if cache is not None else None, int(mode),
+                                            float(train_fraction), int(train_steps)))
 
     def sample_count(self):
         out = ctypes.c_uint32()

@@ -120,10 +120,12 @@ __device__ inline void store_tile(const f32x16& d0, const f32x16& d1, uint16_t* 
   }
 }
 
-__global__ __launch_bounds__(256) void infer_kernel(CacheParams P, const CacheQuery* qs, uint32_t n, float* out) {
+__global__ __launch_bounds__(256) void infer_kernel(CacheParams P, const CacheQuery* qs, uint32_t n_host,
+                                                    const uint32_t* n_dev, float* out) {
   __shared__ __attribute__((aligned(16))) uint16_t tiles[4][2][32 * kStride];
   const int wave = threadIdx.x >> 6;
   const int lane = __lane_id();
+  const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;  // a device count is capped by the buffer size
   const uint32_t tiles_total = (n + 31) / 32;
   for (uint32_t t = blockIdx.x * 4 + wave; t < tiles_total; t += gridDim.x * 4) {
     uint16_t* a = tiles[wave][0];
@@ -275,8 +277,9 @@ __global__ __launch_bounds__(256) void sync_bf16_kernel(CacheParams P) {
 
 }  // namespace mlpk
 
-int launch_cache_infer(const CacheParams& P, const CacheQuery* q, uint32_t n, float* out, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(mlpk::infer_kernel, dim3(grid), dim3(256), 0, s, P, q, n, out);
+int launch_cache_infer(const CacheParams& P, const CacheQuery* q, uint32_t n, const uint32_t* n_dev, float* out,
+                       int grid, hipStream_t s) {
+  hipLaunchKernelGGL(mlpk::infer_kernel, dim3(grid), dim3(256), 0, s, P, q, n, n_dev, out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -35,7 +35,8 @@ constexpr int kCacheWidth = 64;
 constexpr int kCacheParamsPadded = kCacheLayers * kCacheWidth * kCacheWidth;  // 24576
 constexpr int kCacheParams = 5 * 64 * 64 + 3 * 64;                            // 20672
 
-int launch_cache_infer(const CacheParams& P, const CacheQuery* q, uint32_t n, float* out, int grid, hipStream_t s);
+int launch_cache_infer(const CacheParams& P, const CacheQuery* q, uint32_t n, const uint32_t* n_dev, float* out,
+                       int grid, hipStream_t s);
 int launch_cache_train(const CacheParams& P, const CacheQuery* q, const float* targets, uint32_t n, float grad_scale,
                        uint16_t* h_buf, uint16_t* g_buf, float* loss_tiles, float* partial, uint32_t chunk,
                        float lr, float beta1, float beta2, float eps, float bc1, float bc2, float* grad_out,

@@ -59,6 +59,7 @@ __device__ inline float rnd(uint32_t pixel, uint32_t sample, uint32_t dim) {
 // Dimensions: 0-1 pixel jitter; bounce b uses 2 + 8b + {0 lobe, 1-2 direction, 3 roulette,
 // 4-5 light point, 6 light choice}.
 __device__ inline uint32_t dim_of(uint32_t bounce, uint32_t k) { return 2u + 8u * bounce + k; }
+constexpr uint32_t kTrainDim = 0x40000000u;  // dimension that selects training paths
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
@@ -78,6 +79,15 @@ __device__ inline void add_pixel(float4* accum, uint32_t pixel, V3 c) {
   unsafeAtomicAdd(a + 0, c.x);
   unsafeAtomicAdd(a + 1, c.y);
   unsafeAtomicAdd(a + 2, c.z);
+}
+
+// Adds a path contribution to its training header (one writer per path and pass).
+__device__ inline void add_train(const PtBuffers& b, uint32_t slot, V3 c) {
+  if (slot == 0xFFFFFFFFu) return;
+  TrainPath& tp = b.cache.tpaths[slot];
+  tp.T[0] += c.x;
+  tp.T[1] += c.y;
+  tp.T[2] += c.z;
 }
 
 __device__ inline V3 unpack_rgba8(uint32_t c) {
@@ -146,10 +156,12 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers 
                                                           uint32_t sample_base) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i == 0) b.counts[cur] = n_paths;
-  if (i >= n_paths) return;
+  if (blockIdx.x * kBlock >= n_paths) return;  // whole block idle
+  const bool live = i < n_paths;
+  const uint32_t ii = live ? i : n_paths - 1;  // idle lanes mirror a live one but write nothing
   const uint32_t npix = cam.width * cam.height;
-  const uint32_t pixel = i % npix;
-  const uint32_t sample = sample_base + i / npix;
+  const uint32_t pixel = ii % npix;
+  const uint32_t sample = sample_base + ii / npix;
   const uint32_t x = pixel % cam.width, y = pixel / cam.width;
   const float nx = (float(x) + rnd(pixel, sample, 0)) / float(cam.width) * 2.0f - 1.0f;
   const float ny = (float(y) + rnd(pixel, sample, 1)) / float(cam.height) * 2.0f - 1.0f;
@@ -161,17 +173,28 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers 
   V3 target = qw != 0.f ? mk(qx / qw, qy / qw, qz / qw) : mk(qx, qy, qz);
   const V3 eye = ld3(cam.eye);
   const V3 d = qw != 0.f ? normalize(target - eye) : normalize(target);
-  float4* r = reinterpret_cast<float4*>(b.rays[cur]) + 2 * size_t(i);
-  r[0] = make_float4(eye.x, eye.y, eye.z, 0.0f);
-  r[1] = make_float4(d.x, d.y, d.z, __int_as_float(0x7f800000));
+  if (live) {
+    float4* r = reinterpret_cast<float4*>(b.rays[cur]) + 2 * size_t(i);
+    r[0] = make_float4(eye.x, eye.y, eye.z, 0.0f);
+    r[1] = make_float4(d.x, d.y, d.z, __int_as_float(0x7f800000));
+  }
   PathState ps;
   ps.thr[0] = ps.thr[1] = ps.thr[2] = 1.0f;
   ps.pixel = pixel;
   ps.sample = sample;
   ps.flags = kDelta;
   ps.last_pdf = 0.f;
-  ps.pad = 0;
-  b.paths[cur][i] = ps;
+  ps.pad = 0xFFFFFFFFu;
+  const bool want_train = live && b.cache.train && rnd(pixel, sample, kTrainDim) < b.cache.train_fraction;
+  const uint32_t slot = wave_append(&b.counts[3], want_train);
+  if (want_train && slot < b.cache.max_train_paths) {
+    ps.pad = slot;
+    TrainPath tp;
+    tp.T[0] = tp.T[1] = tp.T[2] = 0.f;
+    tp.nv = 0;
+    b.cache.tpaths[slot] = tp;
+  }
+  if (live) b.paths[cur][i] = ps;
 }
 
 __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, uint32_t cur, uint32_t max_depth) {
@@ -181,7 +204,9 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
   const uint32_t nxt = cur ^ 1u;
   const bool live = i < n;
 
-  bool survive = false, shadow = false;
+  bool survive = false, shadow = false, want_query = false;
+  CacheQuery cache_q{};
+  V3 cache_thr = mk(0, 0, 0);
   V3 new_o = mk(0, 0, 0), new_d = mk(0, 0, 1);
   V3 sh_o = mk(0, 0, 0), sh_d = mk(0, 0, 1), sh_c = mk(0, 0, 0);
   PathState ps{};
@@ -226,9 +251,13 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
         const float pdf_l = t_light * t_light / (cos_light * area * float(sc.num_lights));
         w = ps.last_pdf * ps.last_pdf / (ps.last_pdf * ps.last_pdf + pdf_l * pdf_l);
       }
-      add_pixel(b.accum, ps.pixel, thr * ld3(q.emission) * w);
+      const V3 c = thr * ld3(q.emission) * w;
+      add_pixel(b.accum, ps.pixel, c);
+      add_train(b, ps.pad, c);
     } else if (!found) {
-      add_pixel(b.accum, ps.pixel, thr * environment(rd, sc.env_scale));
+      const V3 c = thr * environment(rd, sc.env_scale);
+      add_pixel(b.accum, ps.pixel, c);
+      add_train(b, ps.pad, c);
     } else {
       // ---- surface interaction ----
       const uint32_t inst = hit.z, prim = hit.y;
@@ -290,8 +319,52 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
       L.wt = (1.0f - Fv) * mat.transmission;
       L.wd = (1.0f - Fv) * (1.0f - mat.transmission);
 
+      // ---- radiance cache: end here (non-training paths) or record the vertex (training paths) ----
+      bool cached = false;
+      CacheQuery cq;
+      if (b.cache.terminate || ps.pad != 0xFFFFFFFFu) {
+        cq.pos[0] = p.x;
+        cq.pos[1] = p.y;
+        cq.pos[2] = p.z;
+        cq.roughness = mat.roughness;
+        cq.dir[0] = wo.x;
+        cq.dir[1] = wo.y;
+        cq.dir[2] = wo.z;
+        cq.pad0 = 0.f;
+        cq.nrm[0] = ns.x;
+        cq.nrm[1] = ns.y;
+        cq.nrm[2] = ns.z;
+        cq.pad1 = 0.f;
+        cq.albedo[0] = L.albedo.x;
+        cq.albedo[1] = L.albedo.y;
+        cq.albedo[2] = L.albedo.z;
+        cq.pad2 = 0.f;
+      }
+      if (ps.pad != 0xFFFFFFFFu) {
+        TrainPath& tp = b.cache.tpaths[ps.pad];
+        if (tp.nv < b.cache.max_vertices) {
+          TrainVertex& tv = b.cache.tverts[size_t(ps.pad) * b.cache.max_vertices + tp.nv];
+          tv.q = cq;
+          tv.thr[0] = thr.x;
+          tv.thr[1] = thr.y;
+          tv.thr[2] = thr.z;
+          tv.prefix[0] = tp.T[0];
+          tv.prefix[1] = tp.T[1];
+          tv.prefix[2] = tp.T[2];
+          tp.nv = tp.nv + 1;
+        }
+      } else if (b.cache.terminate && bounce >= 1 &&
+                 (mat.roughness >= b.cache.min_roughness || bounce >= 2) && mat.transmission < 0.5f) {
+        cached = true;
+      }
+      if (cached) {
+        cache_q = cq;
+        cache_thr = thr;
+        want_query = true;
+      }
+
       // ---- next-event estimation toward one quad light ----
-      if (sc.num_lights > 0 && (L.ws + L.wd) > 0.f) {
+      if (!cached && sc.num_lights > 0 && (L.ws + L.wd) > 0.f) {
         uint32_t l = uint32_t(rnd(ps.pixel, ps.sample, dim_of(bounce, 6)) * float(sc.num_lights));
         if (l >= sc.num_lights) l = sc.num_lights - 1;
         const QuadLight& q = sc.lights[l];
@@ -321,6 +394,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
       }
 
       // ---- sample the next direction ----
+      if (cached) goto done_shading;
+      {
       const float ul = rnd(ps.pixel, ps.sample, dim_of(bounce, 0));
       const float u1 = rnd(ps.pixel, ps.sample, dim_of(bounce, 1)), u2 = rnd(ps.pixel, ps.sample, dim_of(bounce, 2));
       bool ok = true;
@@ -381,6 +456,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
         ps.thr[2] = thr.z;
         ps.flags = (ps.flags & ~0xffu) | (bounce + 1);
       }
+      }
+    done_shading:;
     }
   }
   const uint32_t slot = wave_append(&b.counts[nxt], survive);
@@ -400,7 +477,19 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
     rec.contrib[1] = sh_c.y;
     rec.contrib[2] = sh_c.z;
     rec.pixel = ps.pixel;
+    rec.train_slot = ps.pad;
+    rec.pad[0] = rec.pad[1] = rec.pad[2] = 0;
     b.shadow[sslot] = rec;
+  }
+  const uint32_t qslot = wave_append(&b.counts[4], want_query);
+  if (want_query && qslot < b.cache.max_queries) {
+    b.cache.queries[qslot] = cache_q;
+    CacheRecord cr;
+    cr.thr[0] = cache_thr.x;
+    cr.thr[1] = cache_thr.y;
+    cr.thr[2] = cache_thr.z;
+    cr.pixel = ps.pixel;
+    b.cache.qrec[qslot] = cr;
   }
 }
 
@@ -409,7 +498,48 @@ __global__ __launch_bounds__(kBlock) void resolve_kernel(PtBuffers b) {
   if (i >= b.counts[2]) return;
   if (b.occluded[i]) return;
   const ShadowRecord r = b.shadow[i];
-  add_pixel(b.accum, r.pixel, mk(r.contrib[0], r.contrib[1], r.contrib[2]));
+  const V3 c = mk(r.contrib[0], r.contrib[1], r.contrib[2]);
+  add_pixel(b.accum, r.pixel, c);
+  add_train(b, r.train_slot, c);
+}
+
+// Paths that ended in the cache: add throughput times the predicted radiance.
+__global__ __launch_bounds__(kBlock) void cache_resolve_kernel(PtBuffers b) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = min(b.counts[4], b.cache.max_queries);
+  if (i >= n) return;
+  const CacheRecord r = b.cache.qrec[i];
+  const float* y = b.cache.qout + 3 * size_t(i);
+  add_pixel(b.accum, r.pixel, mk(r.thr[0] * fmaxf(y[0], 0.f), r.thr[1] * fmaxf(y[1], 0.f), r.thr[2] * fmaxf(y[2], 0.f)));
+}
+
+// Training samples: suffix radiance (T - prefix) / thr of every recorded vertex.
+__global__ __launch_bounds__(kBlock) void train_batch_kernel(PtBuffers b) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n_paths = min(b.counts[3], b.cache.max_train_paths);
+  const uint32_t path = i / b.cache.max_vertices, k = i % b.cache.max_vertices;
+  bool ok = false;
+  CacheQuery q{};
+  V3 t = mk(0, 0, 0);
+  if (path < n_paths) {
+    const TrainPath tp = b.cache.tpaths[path];
+    if (k < tp.nv) {
+      const TrainVertex v = b.cache.tverts[size_t(path) * b.cache.max_vertices + k];
+      q = v.q;
+      t = mk((tp.T[0] - v.prefix[0]) / fmaxf(v.thr[0], 1e-4f), (tp.T[1] - v.prefix[1]) / fmaxf(v.thr[1], 1e-4f),
+             (tp.T[2] - v.prefix[2]) / fmaxf(v.thr[2], 1e-4f));
+      ok = t.x == t.x && t.y == t.y && t.z == t.z;
+    }
+  }
+  if (blockIdx.x * kBlock >= b.cache.max_train_paths * b.cache.max_vertices) return;
+  const uint32_t slot = wave_append(&b.counts[5], ok);
+  if (ok && slot < b.cache.max_batch) {
+    b.cache.batch_q[slot] = q;
+    float* o = b.cache.batch_t + 3 * size_t(slot);
+    o[0] = fmaxf(t.x, 0.f);
+    o[1] = fmaxf(t.y, 0.f);
+    o[2] = fmaxf(t.z, 0.f);
+  }
 }
 
 }  // namespace ptk
@@ -425,6 +555,19 @@ int launch_pt_generate(const Camera& cam, const PtBuffers& b, uint32_t cur, uint
 int launch_pt_shade(const PtScene& sc, const PtBuffers& b, uint32_t cur, uint32_t max_depth, hipStream_t stream) {
   const uint32_t grid = (b.capacity + ptk::kBlock - 1) / ptk::kBlock;
   hipLaunchKernelGGL(ptk::shade_kernel, dim3(grid), dim3(ptk::kBlock), 0, stream, sc, b, cur, max_depth);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_pt_cache_resolve(const PtBuffers& b, hipStream_t stream) {
+  const uint32_t grid = (b.cache.max_queries + ptk::kBlock - 1) / ptk::kBlock;
+  hipLaunchKernelGGL(ptk::cache_resolve_kernel, dim3(grid ? grid : 1), dim3(ptk::kBlock), 0, stream, b);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_pt_train_batch(const PtBuffers& b, hipStream_t stream) {
+  const uint32_t n = b.cache.max_train_paths * b.cache.max_vertices;
+  const uint32_t grid = (n + ptk::kBlock - 1) / ptk::kBlock;
+  hipLaunchKernelGGL(ptk::train_batch_kernel, dim3(grid ? grid : 1), dim3(ptk::kBlock), 0, stream, b);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

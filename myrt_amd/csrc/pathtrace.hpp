@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "device_types.hpp"
+#include "mlp.hpp"
 
 namespace mrt {
 
@@ -53,6 +54,44 @@ static_assert(sizeof(PathState) == 32, "path state is 32 bytes");
 struct ShadowRecord {
   float contrib[3];
   uint32_t pixel;
+  uint32_t train_slot;  // training path of the contribution, or 0xFFFFFFFF
+  uint32_t pad[3];
+};
+
+// Radiance-cache training: one header per training path and up to max_vertices recorded
+// vertices; the suffix radiance of vertex k is (T_final - prefix_k) / thr_k.
+struct TrainPath {
+  float T[3];   // radiance the path has contributed so far
+  uint32_t nv;  // recorded vertices
+};
+struct TrainVertex {
+  CacheQuery q;
+  float thr[3];
+  float prefix[3];
+  float pad[2];
+};
+static_assert(sizeof(TrainVertex) == 96, "training vertex is 96 bytes");
+struct CacheRecord {
+  float thr[3];
+  uint32_t pixel;
+};
+
+struct PtCache {
+  uint32_t terminate;       // non-training paths end in the cache
+  uint32_t train;           // training paths record vertices
+  float train_fraction;
+  uint32_t max_train_paths;
+  uint32_t max_vertices;
+  uint32_t max_queries;
+  uint32_t max_batch;
+  float min_roughness;      // a secondary vertex ends in the cache if roughness >= this; later ones always
+  TrainPath* tpaths;
+  TrainVertex* tverts;
+  CacheQuery* queries;
+  CacheRecord* qrec;
+  float* qout;
+  CacheQuery* batch_q;
+  float* batch_t;
 };
 
 struct Camera {
@@ -82,14 +121,18 @@ struct PtBuffers {
   void* shadow_rays;      // mrt_ray
   ShadowRecord* shadow;
   uint8_t* occluded;
-  uint32_t* counts;       // [0], [1]: path queues; [2]: shadow queue
+  uint32_t* counts;       // [0], [1]: path queues; [2]: shadow queue; [3] training paths;
+                          // [4] cache queries; [5] training samples
   float4* accum;          // per pixel running sum (rgb); w unused
   uint32_t capacity;
+  PtCache cache;
 };
 
 int launch_pt_generate(const Camera& cam, const PtBuffers& b, uint32_t cur, uint32_t n_paths, uint32_t sample_base,
                        hipStream_t stream);
 int launch_pt_shade(const PtScene& sc, const PtBuffers& b, uint32_t cur, uint32_t max_depth, hipStream_t stream);
 int launch_pt_resolve(const PtBuffers& b, hipStream_t stream);
+int launch_pt_cache_resolve(const PtBuffers& b, hipStream_t stream);
+int launch_pt_train_batch(const PtBuffers& b, hipStream_t stream);
 
 }  // namespace mrt

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/myrt.h"
+#include "cache_internal.hpp"
 #include "pathtrace.hpp"
 #include "scene_internal.hpp"
 
@@ -102,6 +103,22 @@ struct Renderer {
   size_t c_rays0 = 0, c_rays1 = 0, c_paths0 = 0, c_paths1 = 0, c_hits = 0, c_shr = 0, c_sh = 0, c_occ = 0, c_cnt = 0;
   hipStream_t stream = nullptr;
   mrt_render_stats stats{};
+  // Radiance cache (optional).
+  Cache* cache = nullptr;
+  int cache_mode = 0;  // 0 off, 1 train only, 2 train and end paths in the cache
+  float train_fraction = 1.0f / 16.0f;
+  int train_steps = 4;
+  uint32_t max_batch = 1u << 16;
+  float min_roughness = 0.25f;
+  TrainPath* tpaths = nullptr;
+  TrainVertex* tverts = nullptr;
+  CacheQuery* queries = nullptr;
+  CacheRecord* qrec = nullptr;
+  float* qout = nullptr;
+  CacheQuery* batch_q = nullptr;
+  float* batch_t = nullptr;
+  size_t c_tp = 0, c_tv = 0, c_q = 0, c_qr = 0, c_qo = 0, c_bq = 0, c_bt = 0;
+  float last_loss = 0.f;
 
   Renderer() {
     std::memset(view, 0, sizeof(view));
@@ -110,7 +127,8 @@ struct Renderer {
   }
   ~Renderer() {
     for (void* p : {rays[0], rays[1], (void*)paths[0], (void*)paths[1], hits, shadow_rays, (void*)shadow,
-                    (void*)occluded, (void*)counts, (void*)accum})
+                    (void*)occluded, (void*)counts, (void*)accum, (void*)tpaths, (void*)tverts, (void*)queries,
+                    (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t})
       if (p) hipFree(p);
   }
 
@@ -162,6 +180,31 @@ struct Renderer {
       hipError_t e = hipMemsetAsync(accum, 0, npix * sizeof(float4), stream);
       if (e != hipSuccess) return hip_fail(e, "clearing the accumulator");
     }
+    PtCache pc{};
+    const bool use_cache = cache && cache_mode > 0;
+    if (use_cache) {
+      const uint32_t depth = uint32_t(num_bounces);
+      const size_t max_train = size_t(double(cap) * double(train_fraction) * 1.25) + 1024;
+      if ((rc = ensure(tpaths, c_tp, max_train)) || (rc = ensure(tverts, c_tv, max_train * depth)) ||
+          (rc = ensure(queries, c_q, cap)) || (rc = ensure(qrec, c_qr, cap)) || (rc = ensure(qout, c_qo, 3 * cap)) ||
+          (rc = ensure(batch_q, c_bq, max_batch)) || (rc = ensure(batch_t, c_bt, 3 * size_t(max_batch))))
+        return rc;
+      pc.terminate = cache_mode >= 2 ? 1u : 0u;
+      pc.train = 1u;
+      pc.train_fraction = train_fraction;
+      pc.max_train_paths = uint32_t(max_train);
+      pc.max_vertices = depth;
+      pc.max_queries = uint32_t(cap);
+      pc.max_batch = max_batch;
+      pc.min_roughness = min_roughness;
+      pc.tpaths = tpaths;
+      pc.tverts = tverts;
+      pc.queries = queries;
+      pc.qrec = qrec;
+      pc.qout = qout;
+      pc.batch_q = batch_q;
+      pc.batch_t = batch_t;
+    }
     cam.width = w;
     cam.height = h;
     PtBuffers b{};
@@ -175,6 +218,7 @@ struct Renderer {
     b.occluded = occluded;
     b.counts = counts;
     b.accum = accum;
+    b.cache = pc;
     const PtScene sc = s.shading_view();
     const auto t0 = std::chrono::steady_clock::now();
     uint32_t done = 0;
@@ -183,6 +227,8 @@ struct Renderer {
       const uint32_t n_paths = uint32_t(npix * nb);
       b.capacity = n_paths;
       uint32_t cur = 0;
+      hipError_t e0 = hipMemsetAsync(counts + 3, 0, 3 * 4, stream);
+      if (e0 != hipSuccess) return hip_fail(e0, "clearing cache counters");
       if (launch_pt_generate(cam, b, cur, n_paths, sample_count + done, stream)) return fail(MRT_ERR_HIP, "generate");
       for (int bounce = 0; bounce < num_bounces; ++bounce) {
         // counts[cur] is live; clear the next queue and the shadow queue.
@@ -195,6 +241,9 @@ struct Renderer {
         if (launch_pt_resolve(b, stream)) return fail(MRT_ERR_HIP, "resolve");
         cur ^= 1u;
       }
+      if (use_cache) {
+        if ((rc = cache_pass(b, stream)) != MRT_OK) return rc;
+      }
       done += nb;
     }
     hipError_t e = hipStreamSynchronize(stream);
@@ -206,6 +255,31 @@ struct Renderer {
     sample_count += spp;
     stats.samples += uint64_t(spp) * npix;
     stats.milliseconds += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return MRT_OK;
+  }
+
+  // After a pass: resolve the paths that ended in the cache with its current weights, then build
+  // this pass's training batch from the training paths and take train_steps Adam steps on it.
+  int cache_pass(const PtBuffers& b, hipStream_t st) {
+    if (b.cache.terminate) {
+      const int grid = std::max(1, std::min(cache->infer_grid, int((b.cache.max_queries + 127) / 128)));
+      if (launch_cache_infer(cache->P, queries, b.cache.max_queries, counts + 4, qout, grid, st))
+        return fail(MRT_ERR_HIP, "cache inference");
+      if (launch_pt_cache_resolve(b, st)) return fail(MRT_ERR_HIP, "cache resolve");
+    }
+    if (launch_pt_train_batch(b, st)) return fail(MRT_ERR_HIP, "training batch");
+    uint32_t n_batch = 0;
+    hipError_t e = hipMemcpyAsync(&n_batch, counts + 5, 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "reading the training batch size");
+    n_batch = std::min(n_batch, max_batch);
+    if (n_batch < 64) return MRT_OK;
+    // train_steps steps on disjoint slices of the batch.
+    const uint32_t per = std::max<uint32_t>(64, n_batch / uint32_t(std::max(train_steps, 1)));
+    for (uint32_t off = 0; off + per <= n_batch; off += per) {
+      int rc = cache->train(batch_q + off, batch_t + 3 * size_t(off), per, cache->lr, st);
+      if (rc != MRT_OK) return rc;
+    }
     return MRT_OK;
   }
 
@@ -340,6 +414,19 @@ int mrt_renderer_set_num_bounces(mrt_renderer rh, int n) {
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || n < 1 || n > 64) return fail(MRT_ERR_INVALID, "bounces must be in [1, 64]");
   r->num_bounces = n;
+  r->sample_count = 0;
+  return MRT_OK;
+}
+
+int mrt_renderer_set_cache(mrt_renderer rh, mrt_cache c, int mode, float train_fraction, int train_steps) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || mode < 0 || mode > 2 || (mode > 0 && !c) || !(train_fraction > 0.f && train_fraction <= 1.f) ||
+      train_steps < 1 || train_steps > 64)
+    return fail(MRT_ERR_INVALID, "bad cache settings");
+  r->cache = reinterpret_cast<Cache*>(c);
+  r->cache_mode = mode;
+  r->train_fraction = train_fraction;
+  r->train_steps = train_steps;
   r->sample_count = 0;
   return MRT_OK;
 }
