@@ -42,12 +42,75 @@ def timed_passes(fn, steps):
     return wall, timer.elapsed_ms() / steps
 
 
+def build_wavefront_scene(sc):
+    import myrt_amd as m
+    s = m.Scene()
+    gids = [s.push_geometry(i, p, n) for i, p, n in sc["geometries"]]
+    for mat in sc["materials"]:
+        s.push_material(mat)
+    s.set_lights(sc["lights"])
+    s.set_environment(1.0)
+    for g, mt, mi in sc["instances"]:
+        s.enqueue(gids[g], mt, mi)
+    sid = s.push_sdf(sc["sdf_tape"])
+    s.enqueue_sdf(sid, sc["sdf_transform"])
+    return s
+
+
+def wavefront_extras(args):
+    """The 4M-triangle wavefront scene (2000 instanced icospheres, displaced ground, 64 SDF
+    primitives): incoherent closest-hit rays/s and path-tracer samples/s at 1920x1080, 16 spp, depth 8."""
+    import myrt_amd as m
+    from myrt_amd import scenes
+    log("wavefront scene: generating")
+    sc = scenes.wavefront_scene()
+    tris, refs = scenes.world_triangles(sc["geometries"], sc["instances"])
+    rays = scenes.scene_bounce_rays(tris, refs, 1 << 22, seed=4)
+    n_tris = int(tris.shape[0])
+    del tris, refs
+    s = build_wavefront_scene(sc)
+    t0 = time.perf_counter()
+    prep = s.prepare()
+    prep_s = time.perf_counter() - t0
+    log("wavefront scene prepared in %.2f s: %s" % (prep_s, prep))
+    d_r = m.DeviceBuffer.from_array(rays)
+    d_h = m.DeviceBuffer(rays.size * HIT_BYTES)
+    step = lambda: s.trace_closest_device(d_r, rays.size, d_h)
+    step()
+    _, ms = timed_passes(step, max(3, args.steps // 2))
+    st = s.trace_stats(d_r, rays.size)
+    W, H, spp, depth = 1920, 1080, 16, 8
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
+    r = m.Renderer()
+    r.set_camera(view, proj)
+    r.set_num_bounces(depth)
+    r.render(s, W, H, 1)  # warm-up; allocates the queues
+    st0 = r.stats()
+    log("wavefront render %dx%d %d spp depth %d" % (W, H, spp, depth))
+    r.render(s, W, H, spp)
+    st1 = r.stats()
+    dt = (st1["milliseconds"] - st0["milliseconds"]) * 1e-3
+    n_rays = st1["rays"] - st0["rays"]
+    return {"triangles": n_tris, "sdf_primitives": 64, "prepare_s": round(prep_s, 3),
+            "wide_nodes": prep["blas_wide_nodes"] + prep["tlas_wide_nodes"], "device_bytes": prep["device_bytes"],
+            "incoherent_closest_mrays": round(rays.size / (ms * 1e-3) / 1e6, 2),
+            "incoherent_node_fetches_per_ray": round(st["node_visits"] / rays.size, 3),
+            "incoherent_tri_fetches_per_ray": round(st["tri_tests"] / rays.size, 3),
+            "incoherent_instance_visits_per_ray": round(st["instance_visits"] / rays.size, 3),
+            "render": {"width": W, "height": H, "spp": spp, "depth": depth, "seconds": round(dt, 4),
+                       "samples_per_s": round(W * H * spp / dt, 1), "rays_traced": int(n_rays),
+                       "mrays_per_s": round(n_rays / dt / 1e6, 2)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only-incoherent", action="store_true",
+                    help="time only the incoherent closest-hit passes (for profiler runs)")
+    ap.add_argument("--no-wavefront", action="store_true", help="skip the 4M-triangle wavefront scene")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="incoherent rays timed on the host")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel 3 times on BYTES (calibrates rocprof byte counters)")
@@ -95,6 +158,22 @@ def main():
     value = n_inc * args.steps / wall / 1e6
     kernel_rate = n_inc / (ev_ms * 1e-3)
 
+    # Fetch counts per ray from the instrumented traversal (same kernel with counters).
+    st_inc = scene.trace_stats(d_inc, n_inc)
+    n_node = st_inc["node_visits"] / n_inc
+    n_tri = st_inc["tri_tests"] / n_inc
+    n_ins = st_inc["instance_visits"] / n_inc
+    b_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * n_node + TRI_BYTES * n_tri + INST_BYTES * n_ins
+    achieved = kernel_rate * b_ray / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_ray": round(b_ray, 1), "node_fetches_per_ray": round(n_node, 3),
+                "tri_fetches_per_ray": round(n_tri, 3), "kernel_ms": round(ev_ms, 4)}
+    if args.only_incoherent:
+        print(json.dumps({"metric": "incoherent Mrays/s", "value": round(value, 2), "steps": args.steps,
+                          "roofline": roofline}), flush=True)
+        return
+
     # Coherent closest-hit and any-hit shadow rays (extra fields, same protocol).
     coh_step = lambda: scene.trace_closest_device(d_coh, n_coh, d_hits)
     coh_step()
@@ -107,17 +186,12 @@ def main():
     sh_step()
     sh_wall, sh_ms = timed_passes(sh_step, max(3, args.steps // 2))
 
-    # Fetch counts per ray from the instrumented traversal (same kernel with counters).
-    st_inc = scene.trace_stats(d_inc, n_inc)
     st_coh = scene.trace_stats(d_coh, n_coh)
-    n_node = st_inc["node_visits"] / n_inc
-    n_tri = st_inc["tri_tests"] / n_inc
-    n_ins = st_inc["instance_visits"] / n_inc
-    b_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * n_node + TRI_BYTES * n_tri + INST_BYTES * n_ins
-    achieved = kernel_rate * b_ray / 1e9
+
+    wave = None if (args.only_incoherent or args.no_wavefront) else wavefront_extras(args)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.only_incoherent:
         import oracle
         sample = inc[: args.cpu_sample]
         log("cpu baseline: reference traversal on %d incoherent rays" % sample.size)
@@ -144,10 +218,7 @@ def main():
         "config": {"workload": "traversal microbench: 1,000,000-triangle fBm heightfield (1000x500, 6 octaves), "
                                "4,194,304 incoherent cosine bounce rays, closest hit",
                    "triangles": int(idx.shape[0]), "rays_per_step": int(n_inc), "device": name, "cus": cus},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_ray": round(b_ray, 1), "node_fetches_per_ray": round(n_node, 3),
-                     "tri_fetches_per_ray": round(n_tri, 3), "kernel_ms": round(ev_ms, 4)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": {
             "incoherent_kernel_mrays": round(kernel_rate / 1e6, 2),
@@ -158,6 +229,7 @@ def main():
             "incoherent_hit_fraction": round(float(np.isfinite(hits_inc["t"]).mean()), 4),
             "bvh": {"wide_nodes": prep["blas_wide_nodes"], "tri_records": prep["blas_tri_records"],
                     "device_bytes": prep["device_bytes"], "host_build_s": round(build_s, 3)},
+            "wavefront_4M": wave,
         },
     }
     print(json.dumps(out), flush=True)

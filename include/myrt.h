@@ -127,6 +127,7 @@ enum {
 typedef struct mrt_render_stats {
   uint64_t samples;      /* pixel samples rendered since creation */
   double milliseconds;   /* wall time inside mrt_render */
+  uint64_t rays;         /* closest-hit plus shadow rays traced since creation */
 } mrt_render_stats;
 
 /* Radiance-cache query, 64 B: shading point, outgoing direction (towards the viewer), shading
@@ -276,7 +277,8 @@ int mrt_renderer_set_num_bounces(mrt_renderer r, int n);
 /* Attaches a radiance cache (no reference counterpart). mode 0: off; 1: train only (every pass
  * builds a training batch from a `train_fraction` of paths, which stay unbiased, and takes
  * `train_steps` Adam steps); 2: train and end the other paths in the cache at their secondary
- * vertex (roughness >= 0.25, or any later vertex; not on dielectrics). */
+ * vertex (roughness >= 0.25, or any later vertex; not on dielectrics). The accumulated image is
+ * kept; call mrt_renderer_invalidate to restart it. */
 int mrt_renderer_set_cache(mrt_renderer r, mrt_cache c, int mode, float train_fraction, int train_steps);
 /* invalidate_counter / sample_count. */
 int mrt_renderer_invalidate(mrt_renderer r);

@@ -56,7 +56,7 @@ class _PrepareResult(ctypes.Structure):
 
 
 class _RenderStats(ctypes.Structure):
-    _fields_ = [("samples", ctypes.c_uint64), ("milliseconds", ctypes.c_double)]
+    _fields_ = [("samples", ctypes.c_uint64), ("milliseconds", ctypes.c_double), ("rays", ctypes.c_uint64)]
 
 
 class _TraceStats(ctypes.Structure):
@@ -551,7 +551,7 @@ class Renderer:
     def stats(self):
         s = _RenderStats()
         _check(lib().mrt_renderer_stats(self.h, ctypes.byref(s)))
-        return {"samples": s.samples, "milliseconds": s.milliseconds}
+        return {"samples": s.samples, "milliseconds": s.milliseconds, "rays": s.rays}
 
 
 class RadianceCache:

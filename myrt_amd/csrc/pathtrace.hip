@@ -200,6 +200,7 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers 
 __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, uint32_t cur, uint32_t max_depth) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t n = b.counts[cur];
+  if (i == 0) atomicAdd(reinterpret_cast<unsigned long long*>(b.counts + 8), (unsigned long long)n);  // rays traced
   if (blockIdx.x * kBlock >= n) return;  // whole block idle (uniform)
   const uint32_t nxt = cur ^ 1u;
   const bool live = i < n;
@@ -495,6 +496,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
 
 __global__ __launch_bounds__(kBlock) void resolve_kernel(PtBuffers b) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i == 0) atomicAdd(reinterpret_cast<unsigned long long*>(b.counts + 8), (unsigned long long)b.counts[2]);
   if (i >= b.counts[2]) return;
   if (b.occluded[i]) return;
   const ShadowRecord r = b.shadow[i];
@@ -517,11 +519,15 @@ __global__ __launch_bounds__(kBlock) void cache_resolve_kernel(PtBuffers b) {
 __global__ __launch_bounds__(kBlock) void train_batch_kernel(PtBuffers b) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t n_paths = min(b.counts[3], b.cache.max_train_paths);
-  const uint32_t path = i / b.cache.max_vertices, k = i % b.cache.max_vertices;
+  // Training paths were numbered in pixel order; visit them in a scattered order (j * s mod n with
+  // s prime to n, a bijection) so that the first max_batch samples cover the whole image.
+  const uint32_t j = i / b.cache.max_vertices, k = i % b.cache.max_vertices;
+  const uint64_t stride = (n_paths % 1000003u) ? 1000003u : 999983u;
+  const uint32_t path = n_paths ? uint32_t((uint64_t(j) * stride) % n_paths) : 0u;
   bool ok = false;
   CacheQuery q{};
   V3 t = mk(0, 0, 0);
-  if (path < n_paths) {
+  if (j < n_paths) {
     const TrainPath tp = b.cache.tpaths[path];
     if (k < tp.nv) {
       const TrainVertex v = b.cache.tverts[size_t(path) * b.cache.max_vertices + k];

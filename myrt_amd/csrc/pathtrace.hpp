@@ -122,7 +122,7 @@ struct PtBuffers {
   ShadowRecord* shadow;
   uint8_t* occluded;
   uint32_t* counts;       // [0], [1]: path queues; [2]: shadow queue; [3] training paths;
-                          // [4] cache queries; [5] training samples
+                          // [4] cache queries; [5] training samples; [8..9] rays traced (u64)
   float4* accum;          // per pixel running sum (rgb); w unused
   uint32_t capacity;
   PtCache cache;

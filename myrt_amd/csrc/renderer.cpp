@@ -222,6 +222,10 @@ struct Renderer {
     const PtScene sc = s.shading_view();
     const auto t0 = std::chrono::steady_clock::now();
     uint32_t done = 0;
+    {
+      hipError_t e = hipMemsetAsync(counts + 8, 0, 8, stream);
+      if (e != hipSuccess) return hip_fail(e, "clearing the ray counter");
+    }
     while (done < spp) {
       const uint32_t nb = std::min(batch, spp - done);
       const uint32_t n_paths = uint32_t(npix * nb);
@@ -246,7 +250,9 @@ struct Renderer {
       }
       done += nb;
     }
-    hipError_t e = hipStreamSynchronize(stream);
+    uint64_t rays_traced = 0;
+    hipError_t e = hipMemcpyAsync(&rays_traced, counts + 8, 8, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "render");
     uint32_t flags = 0;
     if ((rc = s.read_ctrl(&flags, nullptr, stream)) != MRT_OK) return rc;
@@ -254,6 +260,7 @@ struct Renderer {
     const auto t1 = std::chrono::steady_clock::now();
     sample_count += spp;
     stats.samples += uint64_t(spp) * npix;
+    stats.rays += rays_traced;
     stats.milliseconds += std::chrono::duration<double, std::milli>(t1 - t0).count();
     return MRT_OK;
   }
@@ -427,7 +434,6 @@ int mrt_renderer_set_cache(mrt_renderer rh, mrt_cache c, int mode, float train_f
   r->cache_mode = mode;
   r->train_fraction = train_fraction;
   r->train_steps = train_steps;
-  r->sample_count = 0;
   return MRT_OK;
 }
 

@@ -674,13 +674,14 @@ static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
   return MRT_OK;
 }
 
-// Object-space bounds of a tape by interval propagation; also checks the stack discipline.
+// Object-space bounds of a tape; also checks the stack discipline. Each operand carries a list of
+// boxes whose union holds every point where its distance is below zero. The bounds follow from
+// the operators: smooth union r >= min(a, b) - k/4 (equality at a = b), so its zero set lies where
+// a or b is below k/4, i.e. within the operands' boxes grown by k/4 (primitive distances are at
+// least the distance to their boxes); hard union r = min(a, b); subtraction r >= max(-a, b) >= b
+// keeps the right operand's boxes; intersection r >= max(a, b) keeps the left operand's.
 static bool sdf_bounds(const SdfNode* t, size_t n, Box& out, std::string& err) {
-  std::vector<Box> st;
-  std::vector<std::pair<int, float>> mods;  // (kind, param) of the open modifiers, innermost last
-  struct Frame {
-    Box b;
-  };
+  std::vector<std::vector<Box>> st;
   std::vector<size_t> mod_depth;  // stack size when each modifier opened
   std::vector<SdfNode> open;
   auto apply_mods = [&](Box b) {
@@ -698,7 +699,7 @@ static bool sdf_bounds(const SdfNode* t, size_t n, Box& out, std::string& err) {
         b.mx[a] = r;
       }
     }
-    return b;
+    return std::vector<Box>{b};
   };
   for (size_t i = 0; i < n; ++i) {
     const SdfNode& nd = t[i];
@@ -741,28 +742,30 @@ static bool sdf_bounds(const SdfNode* t, size_t n, Box& out, std::string& err) {
       case kSdfSmoothIntersect: {
         if (st.size() < 2 || (!mod_depth.empty() && st.size() < mod_depth.back() + 2))
           return err = "operator needs two operands", false;
-        Box rhs = st.back();
+        std::vector<Box> rhs = std::move(st.back());
         st.pop_back();
-        Box lhs = st.back();
+        std::vector<Box> lhs = std::move(st.back());
         st.pop_back();
         const bool smooth = nd.op >= kSdfSmoothUnion;
         if (smooth && !(nd.p[0] > 0.f)) return err = "smooth operator needs k > 0", false;
-        const float k = smooth ? nd.p[0] : 0.f;
-        Box r;
+        std::vector<Box> r;
         if (nd.op == kSdfUnion || nd.op == kSdfSmoothUnion) {
-          r = lhs;
-          r.enclose(rhs);
+          r = std::move(lhs);
+          r.insert(r.end(), rhs.begin(), rhs.end());
+          if (smooth) {
+            const float g = 0.25f * nd.p[0];
+            for (Box& x : r)
+              for (int a = 0; a < 3; ++a) {
+                x.mn[a] -= g;
+                x.mx[a] += g;
+              }
+          }
         } else if (nd.op == kSdfSubtract || nd.op == kSdfSmoothSubtract) {
-          r = rhs;
+          r = std::move(rhs);
         } else {
-          r = lhs;
-          r.enclose(rhs);  // conservative for intersections
+          r = std::move(lhs);
         }
-        for (int a = 0; a < 3; ++a) {
-          r.mn[a] -= k;
-          r.mx[a] += k;
-        }
-        st.push_back(r);
+        st.push_back(std::move(r));
         break;
       }
       case kSdfPushTranslate:
@@ -786,7 +789,8 @@ static bool sdf_bounds(const SdfNode* t, size_t n, Box& out, std::string& err) {
   }
   if (!open.empty()) return err = "unclosed modifier", false;
   if (st.size() != 1) return err = "the program must leave exactly one distance", false;
-  out = st[0];
+  out = Box{};
+  for (const Box& x : st[0]) out.enclose(x);
   return true;
 }
 

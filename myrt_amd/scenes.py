@@ -269,6 +269,87 @@ def sphere_field_scene(n_spheres=200, level=2, seed=1, glass_every=0, metal_ever
                 eye=(0.0, 4.0, -14.0), target=(0.0, -1.0, 0.0), fovy=45.0)
 
 
+def wavefront_scene(n_spheres=2000, level=3, seed=7, ground_cells=(1000, 750), n_sdf=64):
+    """The 4M-triangle wavefront scene: n_spheres instanced icospheres (level 3: 1280 faces; radius
+    U[0.1, 0.5], centre U[-5, 5]^3) over a displaced ground (fBm heightfield of ground_cells,
+    24 x 18 units, 1.2 high, at y = -5.5) and an SDF cluster of n_sdf spheres and boxes joined by
+    smooth-min k = 0.1, scaled 1.5x at the origin. Materials: GGX (roughness U[0.05, 1], albedo
+    U[0.1, 0.9]); every 6th sphere a dielectric, every 5th a metal, every 7th Lambert (roughness 1).
+    One quad light above the field and the sky."""
+    (fi, fp, fn), mats = sphere_field(n_spheres, level, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    materials = np.zeros(n_spheres + 5, MATERIAL_DTYPE)
+    for k in range(n_spheres):
+        albedo = rng.uniform(0.1, 0.9, 3)
+        materials[k] = (_rgba8(albedo), 1.5, rng.uniform(0.05, 1.0), 0.0, 0.0, 0, (0, 0))
+        if k % 6 == 0:
+            materials[k]["transmission"] = 1.0
+            materials[k]["roughness"] = 0.05
+        elif k % 5 == 1:
+            materials[k]["metallic"] = 1.0
+        elif k % 7 == 2:
+            materials[k]["roughness"] = 1.0
+            materials[k]["ior"] = 1.0
+    ground_mat = n_spheres
+    materials[ground_mat] = (_rgba8((0.6, 0.6, 0.55)), 1.0, 1.0, 0.0, 0.0, 0, (0, 0))
+    for j, rgb in enumerate([(0.8, 0.3, 0.2), (0.2, 0.6, 0.8), (0.9, 0.9, 0.9), (0.3, 0.8, 0.3)]):
+        materials[n_spheres + 1 + j] = (_rgba8(rgb), 1.5, 0.3 + 0.2 * j, 0.0, 0.0, 0, (0, 0))
+    gi, gp, gn, _ = heightfield(ground_cells[0], ground_cells[1], seed=seed + 2, extent=(24.0, 18.0), height=1.2)
+    gp = gp + np.array([0.0, -5.5, 0.0], np.float32)
+    geometries = [(fi, fp, fn), (gi, gp, gn)]
+    instances = [(0, mt, k) for k, mt in enumerate(mats)]
+    instances.append((1, np.eye(4, dtype=np.float32), ground_mat))
+    tape = sdf_cluster(n_sdf, seed=seed + 3, k=0.1, n_materials=4)
+    tape["material"] += np.uint32(n_spheres + 1)
+    lights = np.zeros(1, LIGHT_DTYPE)
+    lights[0] = ((-2.0, 8.0, -2.0), (4.0, 0.0, 0.0), (0.0, 0.0, 4.0), (10.0, 9.5, 9.0))
+    return dict(geometries=geometries, instances=instances, materials=materials, lights=lights, sdf_tape=tape,
+                sdf_transform=translate_scale((0.0, 0.0, 0.0), 1.5), eye=(0.0, 3.0, -15.0), target=(0.0, -1.5, 0.0),
+                fovy=45.0)
+
+
+def world_triangles(geometries, instances):
+    """World-space triangles of every instance (n, 3, 3) float64 and, per triangle, the point its
+    outward normal faces away from: the instance's origin for closed meshes (geometry 0), a point far
+    below for open ones (the ground)."""
+    tris, refs = [], []
+    for g, mt, _ in instances:
+        idx, pts = geometries[g][0], geometries[g][1]
+        m = np.asarray(mt, np.float64)
+        w = pts.astype(np.float64) @ m[:3, :3].T + m[:3, 3]
+        t = w[idx]
+        tris.append(t)
+        ref = m[:3, 3] if g == 0 else np.array([0.0, -1e6, 0.0])
+        refs.append(np.broadcast_to(ref, (t.shape[0], 3)))
+    return np.concatenate(tris), np.concatenate(refs)
+
+
+def scene_bounce_rays(tris, refs, count, seed, offset=1e-4):
+    """Cosine-distributed rays about the outward normal at uniformly area-sampled points of
+    world-space triangles (see world_triangles)."""
+    rng = np.random.default_rng(seed)
+    cr = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    cdf = np.cumsum(area)
+    tri = np.minimum(np.searchsorted(cdf, rng.random(count) * cdf[-1], side="right"), tris.shape[0] - 1)
+    r1 = np.sqrt(rng.random(count))
+    r2 = rng.random(count)
+    t = tris[tri]
+    p = (1 - r1)[:, None] * t[:, 0] + (r1 * (1 - r2))[:, None] * t[:, 1] + (r1 * r2)[:, None] * t[:, 2]
+    n = cr[tri] / np.linalg.norm(cr[tri], axis=1, keepdims=True)
+    n = np.where((np.einsum("ij,ij->i", n, t.mean(axis=1) - refs[tri]) < 0)[:, None], -n, n)
+    u1, u2 = rng.random(count), rng.random(count)
+    rad, phi = np.sqrt(u1), 2 * np.pi * u2
+    tx, bt = _onb(n)
+    d = (rad * np.cos(phi))[:, None] * tx + (rad * np.sin(phi))[:, None] * bt + np.sqrt(np.maximum(0.0, 1 - u1))[:, None] * n
+    rays = np.zeros(count, RAY_DTYPE)
+    rays["o"] = (p + offset * n).astype(np.float32)
+    rays["d"] = d.astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
 def camera_matrices(eye, target, fovy, aspect):
     e = np.asarray(eye, np.float64)
     f = np.asarray(target, np.float64) - e

@@ -111,3 +111,32 @@ def test_drawable_box_contains_surface(ensure_built):
     assert f.sum() > 200
     p = rays["o"][f] + rays["d"][f] * hits["t"][f][:, None]
     assert np.all(p >= dr["box_lo"][0]) and np.all(p <= dr["box_hi"][0])
+
+
+@pytest.mark.parametrize("op", ["SMOOTH_UNION", "SMOOTH_SUBTRACT", "SMOOTH_INTERSECT", "UNION", "SUBTRACT",
+                                "INTERSECT"])
+def test_drawable_box_contains_negative_region(ensure_built, op):
+    """Every grid point where the oracle's distance is negative lies inside the product's box, for
+    each operator under a mirror and translations (a chain deep enough that loose bounds add up)."""
+    spread = 0.1 if "INTERSECT" in op else 0.8  # keep the intersection non-empty
+    nodes = [("PUSH_MIRROR", 0, 0), ("PUSH_TRANSLATE", 0, 6 * spread / 8, 0.1, 0.0), ("SPHERE", 1, 0.5), ("POP",),
+             ("POP",)]
+    rng = np.random.default_rng(3)
+    for i in range(5):
+        c = rng.uniform(-spread, spread, 3)
+        if "INTERSECT" in op:
+            prim = ("BOX", 2, 0.6, 0.5, 0.55) if i % 2 else ("SPHERE", 3, 0.7)
+        else:
+            prim = ("BOX", 2, 0.4, 0.3, 0.35) if i % 2 else ("TORUS", 3, 0.4, 0.12)
+        nodes += [("PUSH_TRANSLATE", 0, *c), prim, ("POP",), (op, 0, 0.35)]
+    tape = scenes.sdf_tape(nodes)
+    s = m.Scene()
+    s.enqueue_sdf(s.push_sdf(tape), np.eye(4, dtype=np.float32))
+    s.build_host()
+    dr = s.sdf_drawables(1)
+    xs = np.linspace(-2.5, 2.5, 41)
+    pts = np.stack(np.meshgrid(xs, xs, xs, indexing="ij"), -1).reshape(-1, 3)
+    d = np.array([orc.sdf_map(tape, p)[0] for p in pts])
+    inside = pts[d < 0]
+    assert inside.shape[0] > 20
+    assert np.all(inside >= dr["box_lo"][0]) and np.all(inside <= dr["box_hi"][0])
