@@ -42,7 +42,7 @@ def timed_passes(fn, steps):
     return wall, timer.elapsed_ms() / steps
 
 
-def build_wavefront_scene(sc):
+def build_wavefront_scene(sc, with_sdf=True):
     import myrt_amd as m
     s = m.Scene()
     gids = [s.push_geometry(i, p, n) for i, p, n in sc["geometries"]]
@@ -52,8 +52,9 @@ def build_wavefront_scene(sc):
     s.set_environment(1.0)
     for g, mt, mi in sc["instances"]:
         s.enqueue(gids[g], mt, mi)
-    sid = s.push_sdf(sc["sdf_tape"])
-    s.enqueue_sdf(sid, sc["sdf_transform"])
+    if with_sdf:
+        sid = s.push_sdf(sc["sdf_tape"])
+        s.enqueue_sdf(sid, sc["sdf_transform"])
     return s
 
 
@@ -68,17 +69,24 @@ def wavefront_extras(args):
     rays = scenes.scene_bounce_rays(tris, refs, 1 << 22, seed=4)
     n_tris = int(tris.shape[0])
     del tris, refs
+    d_r = m.DeviceBuffer.from_array(rays)
+    d_h = m.DeviceBuffer(rays.size * HIT_BYTES)
+    # Triangles only (the 4M instanced triangles), then the same scene with the SDF cluster.
+    st_tri = build_wavefront_scene(sc, with_sdf=False)
+    st_tri.prepare()
+    step = lambda: st_tri.trace_closest_device(d_r, rays.size, d_h)
+    step()
+    _, ms_tri = timed_passes(step, max(3, args.steps // 2))
+    st = st_tri.trace_stats(d_r, rays.size)
+    st_tri.close()
     s = build_wavefront_scene(sc)
     t0 = time.perf_counter()
     prep = s.prepare()
     prep_s = time.perf_counter() - t0
     log("wavefront scene prepared in %.2f s: %s" % (prep_s, prep))
-    d_r = m.DeviceBuffer.from_array(rays)
-    d_h = m.DeviceBuffer(rays.size * HIT_BYTES)
     step = lambda: s.trace_closest_device(d_r, rays.size, d_h)
     step()
     _, ms = timed_passes(step, max(3, args.steps // 2))
-    st = s.trace_stats(d_r, rays.size)
     W, H, spp, depth = 1920, 1080, 16, 8
     view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
     r = m.Renderer()
@@ -93,7 +101,8 @@ def wavefront_extras(args):
     n_rays = st1["rays"] - st0["rays"]
     return {"triangles": n_tris, "sdf_primitives": 64, "prepare_s": round(prep_s, 3),
             "wide_nodes": prep["blas_wide_nodes"] + prep["tlas_wide_nodes"], "device_bytes": prep["device_bytes"],
-            "incoherent_closest_mrays": round(rays.size / (ms * 1e-3) / 1e6, 2),
+            "incoherent_closest_mrays_triangles_only": round(rays.size / (ms_tri * 1e-3) / 1e6, 2),
+            "incoherent_closest_mrays_with_sdf": round(rays.size / (ms * 1e-3) / 1e6, 2),
             "incoherent_node_fetches_per_ray": round(st["node_visits"] / rays.size, 3),
             "incoherent_tri_fetches_per_ray": round(st["tri_tests"] / rays.size, 3),
             "incoherent_instance_visits_per_ray": round(st["instance_visits"] / rays.size, 3),
@@ -111,6 +120,8 @@ def main():
     ap.add_argument("--only-incoherent", action="store_true",
                     help="time only the incoherent closest-hit passes (for profiler runs)")
     ap.add_argument("--no-wavefront", action="store_true", help="skip the 4M-triangle wavefront scene")
+    ap.add_argument("--only-wavefront", action="store_true",
+                    help="run only the 4M-triangle wavefront scene and print its fields (for profiler runs)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="incoherent rays timed on the host")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel 3 times on BYTES (calibrates rocprof byte counters)")
@@ -130,6 +141,10 @@ def main():
             m._check(m.lib().mrt_device_copy(b.ptr, a.ptr, n, None))
         m.synchronize()
         print(json.dumps({"copy_bytes": n, "launches": 3}), flush=True)
+        return
+
+    if args.only_wavefront:
+        print(json.dumps(wavefront_extras(args)), flush=True)
         return
 
     log("generating workload")

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -224,8 +225,14 @@ int Scene::upload_all() {
     if ((rc = upload(d_nrm, cap_nrm, all_nrm)) != MRT_OK) return rc;
   }
   if (sdf_dirty) {
-    if ((rc = upload(d_sdf_tape, cap_sdf_tape, sdf_tape)) != MRT_OK) return rc;
-    if ((rc = upload(d_sdf_drawables, cap_sdf_dr, sdf_drawables)) != MRT_OK) return rc;
+    // The device drawables address the compiled programs.
+    std::vector<SdfDrawable> dev = sdf_drawables;
+    for (size_t j = 0; j < dev.size(); ++j) {
+      dev[j].tape_offset = sdf_prog_offset[sdf_current[j].sdf];
+      dev[j].tape_len = sdf_prog_len[sdf_current[j].sdf];
+    }
+    if ((rc = upload(d_sdf_prog, cap_sdf_tape, sdf_prog)) != MRT_OK) return rc;
+    if ((rc = upload(d_sdf_drawables, cap_sdf_dr, dev)) != MRT_OK) return rc;
     sdf_dirty = false;
   }
   if (shading_dirty) {
@@ -255,7 +262,7 @@ int Scene::upload_all() {
 }
 
 Scene::~Scene() {
-  for (void* p : {(void*)d_sdf_drawables, (void*)d_sdf_tape, (void*)d_geo_info, (void*)d_idx, (void*)d_inst_material, (void*)d_pos, (void*)d_nrm,
+  for (void* p : {(void*)d_sdf_drawables, (void*)d_sdf_prog, (void*)d_geo_info, (void*)d_idx, (void*)d_inst_material, (void*)d_pos, (void*)d_nrm,
                   (void*)d_materials, (void*)d_lights})
     if (p) hipFree(p);
   if (d_blas) hipFree(d_blas);
@@ -288,12 +295,12 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   if (!prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
   if (n == 0) return MRT_OK;
   if (n >= 0x7fffffffu) return fail(MRT_ERR_LIMIT, "at most 2^31-1 rays per call");
-  const int per_cu = trace_blocks_per_cu(any, stats);
+  const int per_cu = trace_blocks_per_cu(any, stats, lds_depth);
   const size_t max_grid = size_t(num_cus) * size_t(per_cu);
   const size_t want = (n + kTraceBlock - 1) / kTraceBlock;
   const int grid = int(std::max<size_t>(1, std::min(max_grid, want)));
   const uint32_t total_threads = uint32_t(grid) * kTraceBlock;
-  const uint32_t spill_depth = max_stack > kTraceStackLds ? max_stack - kTraceStackLds : 1;
+  const uint32_t spill_depth = max_stack > lds_depth ? max_stack - lds_depth : 1;
   const size_t spill_entries = size_t(spill_depth) * total_threads;
   if (spill_entries > spill_cap) {
     if (d_spill) hipFree(d_spill);
@@ -323,11 +330,12 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.spill_depth = spill_depth;
   p.total_threads = total_threads;
   p.refill_below = refill_below;
+  p.lds_depth = lds_depth;
   if (launch_trace(p, any, stats, grid, stream) != 0) return fail(MRT_ERR_HIP, "trace kernel launch failed");
   if (!sdf_drawables.empty()) {
     SdfParams sp{};
     sp.drawables = d_sdf_drawables;
-    sp.tape = d_sdf_tape;
+    sp.prog = d_sdf_prog;
     sp.num_drawables = uint32_t(sdf_drawables.size());
     sp.rays = d_rays;
     sp.out = d_out;
@@ -549,7 +557,17 @@ int mrt_bvh_destroy(mrt_bvh b) {
 int mrt_scene_create(mrt_scene* out) {
   MRT_GUARD_BEGIN
   if (!out) return fail(MRT_ERR_INVALID, "null output handle");
-  *out = reinterpret_cast<mrt_scene>(new Scene);
+  Scene* s = new Scene;
+  // Launch-shape knobs for tuning sweeps (the defaults are the measured best).
+  if (const char* v = std::getenv("MRT_TRACE_LDS_STACK")) {
+    const long d = std::strtol(v, nullptr, 10);
+    if (d >= 4 && d <= 64 && (d & (d - 1)) == 0) s->lds_depth = uint32_t(d);
+  }
+  if (const char* v = std::getenv("MRT_TRACE_REFILL")) {
+    const long r = std::strtol(v, nullptr, 10);
+    if (r >= 1 && r <= 64) s->refill_below = uint32_t(r);
+  }
+  *out = reinterpret_cast<mrt_scene>(s);
   return MRT_OK;
   MRT_GUARD_END
 }
@@ -794,6 +812,35 @@ static bool sdf_bounds(const SdfNode* t, size_t n, Box& out, std::string& err) {
   return true;
 }
 
+// Compiles a validated tape into executable instructions: PUSH_TRANSLATE, primitive, POP becomes a
+// translated primitive, and a binary operator right after a primitive is fused into it.
+static void compile_sdf(const SdfNode* t, size_t n, std::vector<SdfInsn>& out) {
+  auto is_prim = [](uint32_t op) { return op >= kSdfSphere && op <= kSdfMenger; };
+  auto is_binary = [](uint32_t op) { return op >= kSdfUnion && op <= kSdfSmoothIntersect; };
+  size_t i = 0;
+  while (i < n) {
+    SdfInsn in{};
+    if (t[i].op == kSdfPushTranslate && i + 2 < n && is_prim(t[i + 1].op) && t[i + 2].op == kSdfPop) {
+      in.op = t[i + 1].op | kSdfFuseTranslate;
+      in.material = t[i + 1].material;
+      for (int a = 0; a < 3; ++a) in.t[a] = t[i].p[a];
+      std::memcpy(in.p, t[i + 1].p, sizeof(in.p));
+      i += 3;
+    } else {
+      in.op = t[i].op;
+      in.material = t[i].material;
+      std::memcpy(in.p, t[i].p, sizeof(in.p));
+      i += 1;
+    }
+    if (is_prim(in.op & 0xffu) && i < n && is_binary(t[i].op)) {  // validation guarantees an lhs
+      in.op |= t[i].op << 8;
+      in.k = t[i].p[0];
+      i += 1;
+    }
+    out.push_back(in);
+  }
+}
+
 int mrt_scene_push_sdf(mrt_scene sh, const mrt_sdf_node* nodes, size_t n, uint32_t* out) {
   MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
@@ -807,6 +854,9 @@ int mrt_scene_push_sdf(mrt_scene sh, const mrt_sdf_node* nodes, size_t n, uint32
   s->sdf_len.push_back(uint32_t(n));
   s->sdf_tape.insert(s->sdf_tape.end(), t, t + n);
   s->sdf_bounds.push_back(b);
+  s->sdf_prog_offset.push_back(uint32_t(s->sdf_prog.size()));
+  compile_sdf(t, n, s->sdf_prog);
+  s->sdf_prog_len.push_back(uint32_t(s->sdf_prog.size()) - s->sdf_prog_offset.back());
   s->sdf_dirty = true;
   if (out) *out = uint32_t(s->sdf_len.size() - 1);
   return MRT_OK;

@@ -47,6 +47,8 @@ struct Scene {
   // SDF programs (tapes concatenated), their object-space bounds and this frame's drawables.
   std::vector<SdfNode> sdf_tape;
   std::vector<uint32_t> sdf_offset, sdf_len;
+  std::vector<SdfInsn> sdf_prog;  // compiled tapes (device form)
+  std::vector<uint32_t> sdf_prog_offset, sdf_prog_len;
   std::vector<Box> sdf_bounds;
   std::vector<SdfInstance> sdf_pending, sdf_current;
   std::vector<SdfDrawable> sdf_drawables;
@@ -54,7 +56,7 @@ struct Scene {
   int sdf_steps = 400;      // sequential_pathtracer.hpp:135
   float sdf_eps = 1e-5f;    // sequential_pathtracer.hpp:136
   SdfDrawable* d_sdf_drawables = nullptr;
-  SdfNode* d_sdf_tape = nullptr;
+  SdfInsn* d_sdf_prog = nullptr;
   size_t cap_sdf_dr = 0, cap_sdf_tape = 0;
   std::vector<Instance> pending;  // enqueued since the last prepare
   std::vector<Instance> current;  // the prepared frame
@@ -94,6 +96,7 @@ struct Scene {
   size_t spill_cap = 0;
   int num_cus = 0;
   uint32_t refill_below = 40;
+  uint32_t lds_depth = 16;  // LDS stack entries per lane; MRT_TRACE_LDS_STACK overrides (a power of two, 4-64)
 
   ~Scene();
   int build_host();

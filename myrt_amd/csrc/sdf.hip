@@ -72,88 +72,108 @@ __device__ inline float menger(float px, float py, float pz) {
   return dis * (1.0f / 81.0f);
 }
 
-// map(p): distance and material of a tape at object-space point p.
-__device__ inline float map(const SdfNode* tape, uint32_t len, float px, float py, float pz, uint32_t& mat) {
+__device__ inline float prim_distance(uint32_t kind, const SdfInsn& in, float x, float y, float z) {
+  switch (kind) {
+    case kSdfSphere:
+      return len3(x, y, z) - in.p[0];
+    case kSdfBox: {
+      const float qx = fabsf(x) - in.p[0], qy = fabsf(y) - in.p[1], qz = fabsf(z) - in.p[2];
+      return len3(fmaxf(qx, 0.0f), fmaxf(qy, 0.0f), fmaxf(qz, 0.0f)) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+    }
+    case kSdfTorus: {
+      const float qx = sqrtf(x * x + z * z) - in.p[0];
+      return sqrtf(qx * qx + y * y) - in.p[1];
+    }
+    case kSdfCapsule: {
+      const float yy = y - fminf(fmaxf(y, 0.0f), in.p[0]);
+      return len3(x, yy, z) - in.p[1];
+    }
+    default:
+      return menger(x, y, z);
+  }
+}
+
+// Binary operator on (lhs d1, m1) and (rhs d2, m2).
+__device__ inline float combine(uint32_t op, float k, float d1, uint32_t m1, float d2, uint32_t m2, uint32_t& m) {
+  switch (op) {
+    case kSdfUnion: {
+      const float r = fminf(d1, d2);
+      m = r == d2 ? m2 : m1;
+      return r;
+    }
+    case kSdfSubtract: {
+      const float r = fmaxf(-d1, d2);
+      m = r == d2 ? m2 : m1;
+      return r;
+    }
+    case kSdfIntersect: {
+      const float r = fmaxf(d1, d2);
+      m = r == d2 ? m2 : m1;
+      return r;
+    }
+    case kSdfSmoothUnion: {
+      const float h = clamp01(0.5f + 0.5f * (d2 - d1) / k);
+      m = h >= 0.5f ? m1 : m2;
+      return mixf(d2, d1, h) - k * h * (1.0f - h);
+    }
+    case kSdfSmoothSubtract: {
+      const float h = clamp01(0.5f - 0.5f * (d2 + d1) / k);
+      m = h >= 0.5f ? m1 : m2;
+      return mixf(d2, -d1, h) + k * h * (1.0f - h);
+    }
+    default: {  // kSdfSmoothIntersect
+      const float h = clamp01(0.5f - 0.5f * (d2 - d1) / k);
+      m = h >= 0.5f ? m1 : m2;
+      return mixf(d2, d1, h) + k * h * (1.0f - h);
+    }
+  }
+}
+
+// map(p): distance and material of a compiled program at object-space point p. The program is
+// uniform across the wave, so its instructions come through scalar loads; the next one is
+// fetched before the current one is evaluated.
+__device__ inline float map(const SdfInsn* prog, uint32_t len, float px, float py, float pz, uint32_t& mat) {
   Stack st{};
   PosStack ps{};
   float x = px, y = py, z = pz;
+  SdfInsn nxt = prog[0];
   for (uint32_t i = 0; i < len; ++i) {
-    const SdfNode nd = tape[i];
-    switch (nd.op) {
-      case kSdfSphere:
-        st.push(len3(x, y, z) - nd.p[0], nd.material);
-        break;
-      case kSdfBox: {
-        const float qx = fabsf(x) - nd.p[0], qy = fabsf(y) - nd.p[1], qz = fabsf(z) - nd.p[2];
-        const float d = len3(fmaxf(qx, 0.0f), fmaxf(qy, 0.0f), fmaxf(qz, 0.0f)) + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
-        st.push(d, nd.material);
-        break;
+    const SdfInsn in = nxt;
+    if (i + 1 < len) nxt = prog[i + 1];
+    const uint32_t kind = in.op & 0xffu;
+    if (kind <= kSdfMenger) {
+      float qx = x, qy = y, qz = z;
+      if (in.op & kSdfFuseTranslate) {
+        qx = x - in.t[0];
+        qy = y - in.t[1];
+        qz = z - in.t[2];
       }
-      case kSdfTorus: {
-        const float qx = sqrtf(x * x + z * z) - nd.p[0];
-        st.push(sqrtf(qx * qx + y * y) - nd.p[1], nd.material);
-        break;
+      const float d = prim_distance(kind, in, qx, qy, qz);
+      const uint32_t fused = (in.op >> 8) & 0xffu;
+      if (fused) {
+        uint32_t m;
+        st.d0 = combine(fused, in.k, st.d0, st.m0, d, in.material, m);
+        st.m0 = m;
+      } else {
+        st.push(d, in.material);
       }
-      case kSdfCapsule: {
-        const float yy = y - fminf(fmaxf(y, 0.0f), nd.p[0]);
-        st.push(len3(x, yy, z) - nd.p[1], nd.material);
-        break;
-      }
-      case kSdfMenger:
-        st.push(menger(x, y, z), nd.material);
-        break;
-      case kSdfUnion: {
-        const float m = fminf(st.d1, st.d0);
-        st.reduce(m, m == st.d0 ? st.m0 : st.m1);
-        break;
-      }
-      case kSdfSubtract: {
-        const float m = fmaxf(-st.d1, st.d0);
-        st.reduce(m, m == st.d0 ? st.m0 : st.m1);
-        break;
-      }
-      case kSdfIntersect: {
-        const float m = fmaxf(st.d1, st.d0);
-        st.reduce(m, m == st.d0 ? st.m0 : st.m1);
-        break;
-      }
-      case kSdfSmoothUnion: {
-        const float d1 = st.d1, d2 = st.d0, k = nd.p[0];
-        const float h = clamp01(0.5f + 0.5f * (d2 - d1) / k);
-        st.reduce(mixf(d2, d1, h) - k * h * (1.0f - h), h >= 0.5f ? st.m1 : st.m0);
-        break;
-      }
-      case kSdfSmoothSubtract: {
-        const float d1 = st.d1, d2 = st.d0, k = nd.p[0];
-        const float h = clamp01(0.5f - 0.5f * (d2 + d1) / k);
-        st.reduce(mixf(d2, -d1, h) + k * h * (1.0f - h), h >= 0.5f ? st.m1 : st.m0);
-        break;
-      }
-      case kSdfSmoothIntersect: {
-        const float d1 = st.d1, d2 = st.d0, k = nd.p[0];
-        const float h = clamp01(0.5f - 0.5f * (d2 - d1) / k);
-        st.reduce(mixf(d2, d1, h) + k * h * (1.0f - h), h >= 0.5f ? st.m1 : st.m0);
-        break;
-      }
-      case kSdfPushTranslate:
-        ps.push(x, y, z);
-        x = x - nd.p[0];
-        y = y - nd.p[1];
-        z = z - nd.p[2];
-        break;
-      case kSdfPushMirror: {
-        ps.push(x, y, z);
-        const int axis = int(nd.p[0]);
-        if (axis == 0) x = fabsf(x);
-        else if (axis == 1) y = fabsf(y);
-        else z = fabsf(z);
-        break;
-      }
-      case kSdfPop:
-        ps.pop(x, y, z);
-        break;
-      default:
-        break;
+    } else if (kind <= kSdfSmoothIntersect) {
+      uint32_t m;
+      const float r = combine(kind, in.p[0], st.d1, st.m1, st.d0, st.m0, m);
+      st.reduce(r, m);
+    } else if (kind == kSdfPushTranslate) {
+      ps.push(x, y, z);
+      x = x - in.p[0];
+      y = y - in.p[1];
+      z = z - in.p[2];
+    } else if (kind == kSdfPushMirror) {
+      ps.push(x, y, z);
+      const int axis = int(in.p[0]);
+      if (axis == 0) x = fabsf(x);
+      else if (axis == 1) y = fabsf(y);
+      else z = fabsf(z);
+    } else if (kind == kSdfPop) {
+      ps.pop(x, y, z);
     }
   }
   mat = st.m0;
@@ -194,7 +214,7 @@ __device__ inline bool clip(const SdfDrawable& dr, float ox, float oy, float oz,
 }
 
 // march_sdf (raymarch.glsl:116-171): world hit distance or -1.
-__device__ inline float march(const SdfDrawable& dr, const SdfNode* tape, float ox, float oy, float oz, float dx,
+__device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, float ox, float oy, float oz, float dx,
                               float dy, float dz, float t_enter, float t_exit, int steps, float eps, float& lx,
                               float& ly, float& lz, float& len_out) {
   const float* m = dr.inv;
@@ -251,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void sdf_kernel(SdfParams p) {
     float t0 = r0.w, t1 = best;
     if (!clip(dr, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1)) continue;
     float lx, ly, lz, len;
-    const float tw = march(dr, p.tape + dr.tape_offset, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1, p.steps, p.eps, lx,
+    const float tw = march(dr, p.prog + dr.tape_offset, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1, p.steps, p.eps, lx,
                            ly, lz, len);
     if (tw >= 0.f && tw < best && tw > r0.w) {
       best = tw;
@@ -270,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void sdf_kernel(SdfParams p) {
   }
   // Normal: the tetrahedral gradient of raymarch.glsl:100-114, offset 0.0005 / len.
   const SdfDrawable dr = p.drawables[best_j];
-  const SdfNode* tape = p.tape + dr.tape_offset;
+  const SdfInsn* tape = p.prog + dr.tape_offset;
   float nx = 0, ny = 0, nz = 0;
   for (int k = 0; k < 4; ++k) {
     const float ex = 0.5773f * (2.0f * float(((k + 3) >> 1) & 1) - 1.0f);

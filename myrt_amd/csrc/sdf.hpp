@@ -34,6 +34,22 @@ struct SdfNode {
 };
 static_assert(sizeof(SdfNode) == 32, "sdf node is 32 bytes");
 
+// Executable instruction, 48 B: the host compiles each tape into these before upload. Bits 0-7 of
+// `op` hold a tape op. A primitive may carry a translation (a PUSH_TRANSLATE, primitive, POP
+// triple of the tape; bit kSdfFuseTranslate) and a binary operator that directly follows it on
+// the tape (bits 8-15): the operator then combines the stack top (lhs) with the primitive (rhs)
+// in place. Fusion keeps every binary32 operation of the tape in its order, so the results are
+// those of the plain tape; it removes most stack shifts and instruction dispatches.
+struct SdfInsn {
+  uint32_t op;
+  uint32_t material;
+  float t[3];   // translation subtracted from the position before the primitive
+  float k;      // parameter of the fused operator
+  float p[6];   // parameters of the tape op
+};
+static_assert(sizeof(SdfInsn) == 48, "sdf instruction is 48 bytes");
+constexpr uint32_t kSdfFuseTranslate = 1u << 16;
+
 constexpr int kSdfStack = 8;     // distance/material stack depth of the interpreter
 constexpr int kSdfPosStack = 4;  // nested modifier depth
 

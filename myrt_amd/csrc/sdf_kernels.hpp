@@ -10,7 +10,7 @@ namespace mrt {
 
 struct SdfParams {
   const SdfDrawable* drawables;
-  const SdfNode* tape;
+  const SdfInsn* prog;     // compiled tapes; drawables address them
   uint32_t num_drawables;
   const void* rays;        // mrt_ray[n]
   void* out;               // mrt_hit[n] (closest) or uint8_t[n] (occluded)

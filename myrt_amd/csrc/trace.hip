@@ -10,8 +10,8 @@
 //     fallback for zero edge functions, so the hit set does not depend on traversal order;
 //   * closest-hit ties broken by the lowest (instance, triangle) pair;
 //   * persistent waves that refill idle lanes from a global ray counter (ballot + one atomic per
-//     wave), and a per-lane ring stack of 16 entries in LDS that spills to a per-thread slice of
-//     global memory when a path needs more.
+//     wave), and a per-lane ring stack in LDS (16 entries by default) that spills to a per-thread
+//     slice of global memory when a path needs more.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -23,7 +23,6 @@ namespace mrt {
 namespace dev {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kStackLds = 16;
 constexpr float kRobust = 1.0000005f;  // > 1 + 2*gamma(3) for binary32
 
 struct RayPre {  // per-space ray data for the watertight test
@@ -99,7 +98,8 @@ __device__ inline float byte_of(uint32_t w, int c) { return float((w >> (8 * c))
 
 template <bool ANY, bool STATS>
 __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
-  __shared__ uint32_t lds_stack[kStackLds][kBlock];
+  extern __shared__ uint32_t lds_stack[];  // [lds_depth][kBlock]
+  const uint32_t kStackLds = p.lds_depth;
   const uint32_t tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * kBlock + tid;
   const int lane = __lane_id();
@@ -124,21 +124,21 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
     if (sp >= kStackLds) {
       const uint32_t deep = sp - kStackLds;
       if (deep < p.spill_depth) {
-        p.spill[size_t(deep) * p.total_threads + gtid] = lds_stack[slot][tid];
+        p.spill[size_t(deep) * p.total_threads + gtid] = lds_stack[slot * kBlock + tid];
       } else {
         atomicOr(p.flags, 1u);  // overflow: the entry is dropped and the call reports an error
       }
     }
-    lds_stack[slot][tid] = e;
+    lds_stack[slot * kBlock + tid] = e;
     ++sp;
   };
   const auto pop = [&]() -> uint32_t {
     --sp;
     const uint32_t slot = sp & (kStackLds - 1);
-    const uint32_t e = lds_stack[slot][tid];
+    const uint32_t e = lds_stack[slot * kBlock + tid];
     if (sp >= kStackLds) {
       const uint32_t deep = sp - kStackLds;
-      if (deep < p.spill_depth) lds_stack[slot][tid] = p.spill[size_t(deep) * p.total_threads + gtid];
+      if (deep < p.spill_depth) lds_stack[slot * kBlock + tid] = p.spill[size_t(deep) * p.total_threads + gtid];
     }
     return e;
   };
@@ -331,29 +331,31 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
 
 int launch_trace(const TraceParams& p, bool any, bool stats, int grid, hipStream_t stream) {
   const dim3 block(dev::kBlock), g(grid);
+  const size_t lds = size_t(p.lds_depth) * dev::kBlock * 4;
   if (any) {
     if (stats)
-      hipLaunchKernelGGL((dev::trace_kernel<true, true>), g, block, 0, stream, p);
+      hipLaunchKernelGGL((dev::trace_kernel<true, true>), g, block, lds, stream, p);
     else
-      hipLaunchKernelGGL((dev::trace_kernel<true, false>), g, block, 0, stream, p);
+      hipLaunchKernelGGL((dev::trace_kernel<true, false>), g, block, lds, stream, p);
   } else {
     if (stats)
-      hipLaunchKernelGGL((dev::trace_kernel<false, true>), g, block, 0, stream, p);
+      hipLaunchKernelGGL((dev::trace_kernel<false, true>), g, block, lds, stream, p);
     else
-      hipLaunchKernelGGL((dev::trace_kernel<false, false>), g, block, 0, stream, p);
+      hipLaunchKernelGGL((dev::trace_kernel<false, false>), g, block, lds, stream, p);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int trace_blocks_per_cu(bool any, bool stats) {
+int trace_blocks_per_cu(bool any, bool stats, uint32_t lds_depth) {
   int blocks = 0;
+  const size_t lds = size_t(lds_depth) * dev::kBlock * 4;
   hipError_t e;
   if (any)
-    e = stats ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<true, true>, dev::kBlock, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<true, false>, dev::kBlock, 0);
+    e = stats ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<true, true>, dev::kBlock, lds)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<true, false>, dev::kBlock, lds);
   else
-    e = stats ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<false, true>, dev::kBlock, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<false, false>, dev::kBlock, 0);
+    e = stats ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<false, true>, dev::kBlock, lds)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<false, false>, dev::kBlock, lds);
   if (e != hipSuccess || blocks < 1) blocks = 1;
   return blocks;
 }
