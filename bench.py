@@ -180,8 +180,16 @@ def main():
     n_ins = st_inc["instance_visits"] / n_inc
     b_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * n_node + TRI_BYTES * n_tri + INST_BYTES * n_ins
     achieved = kernel_rate * b_ray / 1e9
+    # HBM traffic per launch from the committed rocprofv3 counter passes over this command
+    # (FETCH_SIZE and WRITE_SIZE, calibrated on the copy kernel).
+    traffic, l2_hit = None, None
+    pmc = os.path.join(ROOT, "profiles", "trace_kernel_heightfield_1M_pmc.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            rec = json.load(f)
+        traffic, l2_hit = rec["traffic_bytes_per_launch"], rec["l2_hit_rate"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "l2_hit_rate": l2_hit,
                 "bytes_per_ray": round(b_ray, 1), "node_fetches_per_ray": round(n_node, 3),
                 "tri_fetches_per_ray": round(n_tri, 3), "kernel_ms": round(ev_ms, 4)}
     if args.only_incoherent:
