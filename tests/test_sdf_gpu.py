@@ -35,7 +35,7 @@ def check(gpu, ref, ref_n, label):
     assert np.array_equal(gpu["prim"], ref["prim"]), label + ": triangle ids / materials differ"
     f = gpu["inst"] != m.MISS
     assert np.array_equal(gpu["t"][f].view(np.uint32), ref["t"][f].astype(np.float32).view(np.uint32)), label
-    s = (gpu["inst"] & m.SDF_HIT) != 0
+    s = (gpu["inst"] != m.MISS) & ((gpu["inst"] & m.SDF_HIT) != 0)
     if s.any():
         n = oct_decode(gpu["bary"][s])
         assert np.abs(n - ref_n[s]).max() < 2e-3, label + ": normals"
@@ -75,5 +75,5 @@ def test_sdf_with_triangles(ensure_built):
     gpu = s.trace_closest(rays)
     o = orc.OracleScene([(idx, pts)], [(0, np.eye(4, dtype=np.float32))])
     ref, ref_n = orc.sdf_trace(tape, dr, rays, hits=o.brute(rays))
-    assert ((gpu["inst"] & m.SDF_HIT) != 0).sum() > 500 and (gpu["inst"] == 0).sum() > 500
+    assert ((gpu["inst"] != m.MISS) & ((gpu["inst"] & m.SDF_HIT) != 0)).sum() > 500 and (gpu["inst"] == 0).sum() > 500
     check(gpu, ref, ref_n, "mixed")
