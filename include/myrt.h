@@ -147,6 +147,7 @@ typedef struct mrt_scene_s* mrt_scene;
 typedef struct mrt_bvh_s* mrt_bvh;
 typedef struct mrt_renderer_s* mrt_renderer;
 typedef struct mrt_cache_s* mrt_cache;
+typedef struct mrt_denoiser_s* mrt_denoiser;
 
 /* Counters filled by mrt_scene_prepare (scene::prepare_result_t, src/pathtracer/scene.hpp:84-91,
  * extended with the device-side sizes of the wide BVH). */
@@ -311,6 +312,38 @@ int mrt_cache_train_device(mrt_cache c, const mrt_cache_query* d_q, const float*
 /* The loss gradient (packed like the parameters) without updating anything. */
 int mrt_cache_gradient(mrt_cache c, const mrt_cache_query* q, const float* targets_rgb, size_t n, float* grad,
                        float* loss);
+
+/* ---- denoiser (no reference counterpart: the north star's convolutional denoiser, which the
+ * reference's bilateral pp_denoise.comp:24-63 stands in for) ------------------------------------
+ * A 5-level U-Net (32/48/64/96/128 channels, two 3x3 conv + ReLU per level and side, 2x2 average
+ * pooling, nearest upsampling with skip concatenation, 1x1 output) over 10 raw features per
+ * pixel: mean radiance rgb, albedo rgb, normal xyz, hit distance (inf for a miss). It predicts a
+ * log-space residual: out = expm1(max(log1p(rgb) + y, 0)). Inference in fp16 MFMA, training in
+ * fp32 MFMA, both accumulating in fp32; Adam on fp32 weights. Parameters are packed layer by
+ * layer (encoder levels 0-4 conv a, conv b; decoder levels 3-0 conv a, conv b; output), each as
+ * W[cout][tap][cin] (tap = 3*ky + kx; cin in [upsampled, skip] order) followed by b[cout]. */
+int mrt_denoiser_create(mrt_denoiser* out, uint64_t seed);
+int mrt_denoiser_destroy(mrt_denoiser d);
+int mrt_denoiser_num_params(size_t* n);
+int mrt_denoiser_get_params(mrt_denoiser d, float* out, size_t n);
+int mrt_denoiser_set_params(mrt_denoiser d, const float* in, size_t n);
+int mrt_denoiser_set_adam(mrt_denoiser d, float lr, float beta1, float beta2, float eps);
+/* The residual y (B*H*W*3) of a batch of raw feature images (B*H*W*10); H, W multiples of 16.
+ * half = 1 runs the fp16 inference kernels, 0 the fp32 training kernels. */
+int mrt_denoiser_forward(mrt_denoiser d, const float* raw, int B, int H, int W, int half, float* y);
+/* Loss mean(|log1p(rgb) + y - log1p(target)|) and its gradient (packed like the parameters). */
+int mrt_denoiser_gradient(mrt_denoiser d, const float* raw, const float* target_rgb, int B, int H, int W, float* grad,
+                          float* loss);
+/* One Adam step on a batch (host or device buffers). */
+int mrt_denoiser_train(mrt_denoiser d, const float* raw, const float* target_rgb, int B, int H, int W, float* loss);
+int mrt_denoiser_train_device(mrt_denoiser d, const float* d_raw, const float* d_target_rgb, int B, int H, int W,
+                              float* loss, void* stream);
+/* Denoised rgba (H*W*4) of one raw feature image of any size. */
+int mrt_denoiser_denoise(mrt_denoiser d, const float* raw, int H, int W, float* rgba);
+/* The renderer's per-pixel feature means (H*W*8: albedo rgb, normal xyz, hit distance, 0) of the
+ * primary hits, and its accumulated image passed through a denoiser (H*W*4). */
+int mrt_renderer_read_aux(mrt_renderer r, float* out);
+int mrt_renderer_denoise(mrt_renderer r, mrt_denoiser d, float* rgba);
 
 #ifdef __cplusplus
 }

@@ -134,6 +134,22 @@ SIGNATURES = {
     "mrt_cache_train": ([_P, _P, _P, _SZ, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "mrt_cache_train_device": ([_P, _P, _P, _SZ, _P], ctypes.c_int),
     "mrt_cache_gradient": ([_P, _P, _P, _SZ, _P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "mrt_denoiser_create": ([ctypes.POINTER(_P), ctypes.c_uint64], ctypes.c_int),
+    "mrt_denoiser_destroy": ([_P], ctypes.c_int),
+    "mrt_denoiser_num_params": ([ctypes.POINTER(_SZ)], ctypes.c_int),
+    "mrt_denoiser_get_params": ([_P, _P, _SZ], ctypes.c_int),
+    "mrt_denoiser_set_params": ([_P, _P, _SZ], ctypes.c_int),
+    "mrt_denoiser_set_adam": ([_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float], ctypes.c_int),
+    "mrt_denoiser_forward": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "mrt_denoiser_gradient": ([_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
+                               ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "mrt_denoiser_train": ([_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)],
+                           ctypes.c_int),
+    "mrt_denoiser_train_device": ([_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_float), _P], ctypes.c_int),
+    "mrt_denoiser_denoise": ([_P, _P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "mrt_renderer_read_aux": ([_P, _P], ctypes.c_int),
+    "mrt_renderer_denoise": ([_P, _P, _P], ctypes.c_int),
 }
 
 
@@ -552,6 +568,105 @@ class Renderer:
         s = _RenderStats()
         _check(lib().mrt_renderer_stats(self.h, ctypes.byref(s)))
         return {"samples": s.samples, "milliseconds": s.milliseconds, "rays": s.rays}
+
+    def aux(self):
+        """Per-pixel means of the primary hits: (albedo (H, W, 3), normal (H, W, 3), distance (H, W))."""
+        w, h = self.size
+        out = np.zeros((h, w, 8), np.float32)
+        _check(lib().mrt_renderer_read_aux(self.h, _ptr(out)))
+        return out[..., 0:3], out[..., 3:6], out[..., 6]
+
+    def raw_features(self):
+        """The denoiser's 10 raw features per pixel (H, W, 10): radiance, albedo, normal, distance."""
+        albedo, normal, depth = self.aux()
+        return np.concatenate([self.image()[..., :3], albedo, normal, depth[..., None]], axis=-1)
+
+    def denoise(self, denoiser):
+        """The accumulated image through the denoiser (H, W, 4), computed on the device."""
+        w, h = self.size
+        out = np.zeros((h, w, 4), np.float32)
+        _check(lib().mrt_renderer_denoise(self.h, denoiser.h, _ptr(out)))
+        return out
+
+
+class Denoiser:
+    """The U-Net denoiser over the C ABI (mrt_denoiser_*)."""
+
+    def __init__(self, seed=1):
+        h = ctypes.c_void_p()
+        _check(lib().mrt_denoiser_create(ctypes.byref(h), int(seed)))
+        self.h = h
+        n = _SZ()
+        _check(lib().mrt_denoiser_num_params(ctypes.byref(n)))
+        self.n_params = n.value
+
+    def close(self):
+        if self.h:
+            lib().mrt_denoiser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        _check(lib().mrt_denoiser_get_params(self.h, _ptr(out), out.size))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().mrt_denoiser_set_params(self.h, _ptr(p), p.size))
+
+    def set_adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(lib().mrt_denoiser_set_adam(self.h, lr, beta1, beta2, eps))
+
+    def forward(self, raw, half=False):
+        """Log-space residual y (B, H, W, 3) of raw features (B, H, W, 10)."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        B, H, W, _ = raw.shape
+        y = np.zeros((B, H, W, 3), np.float32)
+        _check(lib().mrt_denoiser_forward(self.h, _ptr(raw), B, H, W, int(bool(half)), _ptr(y)))
+        return y
+
+    def gradient(self, raw, target):
+        raw = np.ascontiguousarray(raw, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        B, H, W, _ = raw.shape
+        g = np.zeros(self.n_params, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().mrt_denoiser_gradient(self.h, _ptr(raw), _ptr(target), B, H, W, _ptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, raw, target):
+        raw = np.ascontiguousarray(raw, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        B, H, W, _ = raw.shape
+        loss = ctypes.c_float()
+        _check(lib().mrt_denoiser_train(self.h, _ptr(raw), _ptr(target), B, H, W, ctypes.byref(loss)))
+        return loss.value
+
+    def train_device(self, d_raw, d_target, B, H, W, want_loss=True):
+        loss = ctypes.c_float()
+        _check(lib().mrt_denoiser_train_device(self.h, d_raw.ptr, d_target.ptr, B, H, W,
+                                               ctypes.byref(loss) if want_loss else None, None))
+        return loss.value if want_loss else None
+
+    def denoise(self, raw):
+        """Denoised rgba (H, W, 4) of one raw feature image (H, W, 10), any size (fp16 kernels)."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        H, W, _ = raw.shape
+        out = np.zeros((H, W, 4), np.float32)
+        _check(lib().mrt_denoiser_denoise(self.h, _ptr(raw), H, W, _ptr(out)))
+        return out
+
+    def save(self, path):
+        np.save(path, self.params(), allow_pickle=False)
+
+    def load(self, path):
+        self.set_params(np.load(path, allow_pickle=False))
 
 
 class RadianceCache:

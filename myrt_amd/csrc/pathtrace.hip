@@ -319,6 +319,18 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
       L.ws = Fv;
       L.wt = (1.0f - Fv) * mat.transmission;
       L.wd = (1.0f - Fv) * (1.0f - mat.transmission);
+      // Primary-hit features for the denoiser (sums; the host divides by the sample count).
+      if (bounce == 0 && b.aux) {
+        float* a = reinterpret_cast<float*>(b.aux + 2 * size_t(ps.pixel));
+        unsafeAtomicAdd(a + 0, L.albedo.x);
+        unsafeAtomicAdd(a + 1, L.albedo.y);
+        unsafeAtomicAdd(a + 2, L.albedo.z);
+        unsafeAtomicAdd(a + 3, t_hit);
+        unsafeAtomicAdd(a + 4, ns.x);
+        unsafeAtomicAdd(a + 5, ns.y);
+        unsafeAtomicAdd(a + 6, ns.z);
+        unsafeAtomicAdd(a + 7, 1.0f);
+      }
 
       // ---- radiance cache: end here (non-training paths) or record the vertex (training paths) ----
       bool cached = false;
@@ -549,6 +561,36 @@ __global__ __launch_bounds__(kBlock) void train_batch_kernel(PtBuffers b) {
 }
 
 }  // namespace ptk
+
+namespace ptk {
+// Raw denoiser features of each pixel: mean radiance, albedo, normal, hit distance (inf if no
+// sample hit a surface).
+__global__ __launch_bounds__(kBlock) void denoise_input_kernel(const float4* accum, const float4* aux, float inv_spp,
+                                                               float* raw, size_t npix) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= npix) return;
+  const float4 c = accum[i], a0 = aux[2 * i], a1 = aux[2 * i + 1];
+  float* o = raw + 10 * i;
+  o[0] = c.x * inv_spp;
+  o[1] = c.y * inv_spp;
+  o[2] = c.z * inv_spp;
+  o[3] = a0.x * inv_spp;
+  o[4] = a0.y * inv_spp;
+  o[5] = a0.z * inv_spp;
+  o[6] = a1.x * inv_spp;
+  o[7] = a1.y * inv_spp;
+  o[8] = a1.z * inv_spp;
+  o[9] = a1.w > 0.f ? a0.w / a1.w : __int_as_float(0x7f800000);
+}
+}  // namespace ptk
+
+int launch_pt_denoise_input(const float4* accum, const float4* aux, uint32_t spp, float* raw, size_t npix,
+                            hipStream_t stream) {
+  const unsigned grid = unsigned((npix + ptk::kBlock - 1) / ptk::kBlock);
+  hipLaunchKernelGGL(ptk::denoise_input_kernel, dim3(grid ? grid : 1), dim3(ptk::kBlock), 0, stream, accum, aux,
+                     1.0f / float(spp ? spp : 1), raw, npix);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int launch_pt_generate(const Camera& cam, const PtBuffers& b, uint32_t cur, uint32_t n_paths, uint32_t sample_base,
                        hipStream_t stream) {

@@ -124,6 +124,7 @@ struct PtBuffers {
   uint32_t* counts;       // [0], [1]: path queues; [2]: shadow queue; [3] training paths;
                           // [4] cache queries; [5] training samples; [8..9] rays traced (u64)
   float4* accum;          // per pixel running sum (rgb); w unused
+  float4* aux;            // per pixel 2 x float4 of primary-hit sums: albedo, t; normal, hits
   uint32_t capacity;
   PtCache cache;
 };
@@ -134,5 +135,7 @@ int launch_pt_shade(const PtScene& sc, const PtBuffers& b, uint32_t cur, uint32_
 int launch_pt_resolve(const PtBuffers& b, hipStream_t stream);
 int launch_pt_cache_resolve(const PtBuffers& b, hipStream_t stream);
 int launch_pt_train_batch(const PtBuffers& b, hipStream_t stream);
+int launch_pt_denoise_input(const float4* accum, const float4* aux, uint32_t spp, float* raw, size_t npix,
+                            hipStream_t stream);
 
 }  // namespace mrt

@@ -16,6 +16,7 @@
 
 #include "../../include/myrt.h"
 #include "cache_internal.hpp"
+#include "denoiser_internal.hpp"
 #include "pathtrace.hpp"
 #include "scene_internal.hpp"
 
@@ -100,6 +101,11 @@ struct Renderer {
   uint8_t* occluded = nullptr;
   uint32_t* counts = nullptr;
   float4* accum = nullptr;
+  float4* aux = nullptr;  // primary-hit feature sums, 2 per pixel
+  size_t cap_aux = 0;
+  float* raw = nullptr;   // denoiser input (10 floats per pixel)
+  float* den = nullptr;   // denoiser output (rgba)
+  size_t cap_raw = 0, cap_den = 0;
   size_t c_rays0 = 0, c_rays1 = 0, c_paths0 = 0, c_paths1 = 0, c_hits = 0, c_shr = 0, c_sh = 0, c_occ = 0, c_cnt = 0;
   hipStream_t stream = nullptr;
   mrt_render_stats stats{};
@@ -128,7 +134,7 @@ struct Renderer {
   ~Renderer() {
     for (void* p : {rays[0], rays[1], (void*)paths[0], (void*)paths[1], hits, shadow_rays, (void*)shadow,
                     (void*)occluded, (void*)counts, (void*)accum, (void*)tpaths, (void*)tverts, (void*)queries,
-                    (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t})
+                    (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den})
       if (p) hipFree(p);
   }
 
@@ -174,10 +180,12 @@ struct Renderer {
       return rc;
     if (!accum || cap_accum < npix) {
       if ((rc = ensure(accum, cap_accum, npix)) != MRT_OK) return rc;
+      if ((rc = ensure(aux, cap_aux, 2 * npix)) != MRT_OK) return rc;
       sample_count = 0;
     }
     if (sample_count == 0) {
       hipError_t e = hipMemsetAsync(accum, 0, npix * sizeof(float4), stream);
+      if (e == hipSuccess) e = hipMemsetAsync(aux, 0, 2 * npix * sizeof(float4), stream);
       if (e != hipSuccess) return hip_fail(e, "clearing the accumulator");
     }
     PtCache pc{};
@@ -218,6 +226,7 @@ struct Renderer {
     b.occluded = occluded;
     b.counts = counts;
     b.accum = accum;
+    b.aux = aux;
     b.cache = pc;
     const PtScene sc = s.shading_view();
     const auto t0 = std::chrono::steady_clock::now();
@@ -288,6 +297,41 @@ struct Renderer {
       if (rc != MRT_OK) return rc;
     }
     return MRT_OK;
+  }
+
+  int read_aux(float* out) {
+    if (!aux || sample_count == 0) return fail(MRT_ERR_INVALID, "nothing rendered yet");
+    const size_t npix = size_t(width) * height;
+    std::vector<float4> host(2 * npix);
+    hipError_t e = hipMemcpy(host.data(), aux, 2 * npix * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "reading the features");
+    const float inv = 1.0f / float(sample_count);
+    for (size_t i = 0; i < npix; ++i) {
+      const float4 a0 = host[2 * i], a1 = host[2 * i + 1];
+      float* o = out + 8 * i;
+      o[0] = a0.x * inv;
+      o[1] = a0.y * inv;
+      o[2] = a0.z * inv;
+      o[3] = a1.x * inv;
+      o[4] = a1.y * inv;
+      o[5] = a1.z * inv;
+      o[6] = a1.w > 0.f ? a0.w / a1.w : INFINITY;
+      o[7] = 0.f;
+    }
+    return MRT_OK;
+  }
+
+  int denoise(Denoiser& d, float* rgba) {
+    if (!accum || sample_count == 0) return fail(MRT_ERR_INVALID, "nothing rendered yet");
+    const size_t npix = size_t(width) * height;
+    int rc;
+    if ((rc = ensure(raw, cap_raw, 10 * npix)) || (rc = ensure(den, cap_den, 4 * npix))) return rc;
+    if (launch_pt_denoise_input(accum, aux, sample_count, raw, npix, stream))
+      return fail(MRT_ERR_HIP, "denoiser input");
+    if ((rc = d.denoise(raw, int(height), int(width), den, stream)) != MRT_OK) return rc;
+    hipError_t e = hipMemcpyAsync(rgba, den, npix * 16, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e == hipSuccess ? MRT_OK : hip_fail(e, "reading the denoised image");
   }
 
   int read(float* rgba) {
@@ -465,6 +509,22 @@ int mrt_renderer_read(mrt_renderer rh, float* rgba) {
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || !rgba) return fail(MRT_ERR_INVALID, "null argument");
   return r->read(rgba);
+  MRT_GUARD_END
+}
+
+int mrt_renderer_read_aux(mrt_renderer rh, float* out) {
+  MRT_GUARD_BEGIN
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || !out) return fail(MRT_ERR_INVALID, "null argument");
+  return r->read_aux(out);
+  MRT_GUARD_END
+}
+
+int mrt_renderer_denoise(mrt_renderer rh, mrt_denoiser dh, float* rgba) {
+  MRT_GUARD_BEGIN
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || !dh || !rgba) return fail(MRT_ERR_INVALID, "null argument");
+  return r->denoise(*reinterpret_cast<Denoiser*>(dh), rgba);
   MRT_GUARD_END
 }
 

@@ -1,0 +1,502 @@
+// Denoiser U-Net kernels for gfx950.
+//
+// Convolutions are implicit GEMMs on MFMA: output channels are the M dimension (32 per block),
+// pixels the N dimension (32 consecutive pixels of a row per block) and input channels x taps the
+// K dimension. A wave owns 64 output channels x 64 pixels (4 accumulator blocks); operands come
+// straight from NHWC tensors with 16-byte loads (8 halves or 4 floats per lane). Inference runs in
+// fp16 (v_mfma_f32_32x32x16_f16), training in fp32 (v_mfma_f32_32x32x2_f32), both accumulating in
+// fp32. Weight gradients reduce over pixels with the fp32 MFMA (K = 2 pixels per instruction),
+// whose operand layout reads 32 consecutive channels of one pixel per half wave.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "unet.hpp"
+
+namespace mrt {
+namespace unk {
+
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 h8;
+typedef __attribute__((__vector_size__(4 * sizeof(_Float16)))) _Float16 h4;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+constexpr int kBlock = 256;
+
+template <typename T>
+struct Frag;
+
+// fp16: one 32x32x16 MFMA per K step of 16 channels; lane l holds channels 8*(l/32) .. +7.
+template <>
+struct Frag<_Float16> {
+  static constexpr int kStep = 16;
+  static constexpr int kLane = 8;
+  using V = h8;
+  __device__ static V load(const _Float16* p) { return *reinterpret_cast<const V*>(p); }
+  __device__ static void mma(const V& a, const V& b, f32x16& acc) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+  }
+  __device__ static void store4(_Float16* p, float a, float b, float c, float d) {
+    h4 v;
+    v[0] = (_Float16)a;
+    v[1] = (_Float16)b;
+    v[2] = (_Float16)c;
+    v[3] = (_Float16)d;
+    *reinterpret_cast<h4*>(p) = v;
+  }
+  __device__ static void load4(const _Float16* p, float* o) {
+    const h4 v = *reinterpret_cast<const h4*>(p);
+    o[0] = float(v[0]);
+    o[1] = float(v[1]);
+    o[2] = float(v[2]);
+    o[3] = float(v[3]);
+  }
+};
+
+// fp32: four 32x32x2 MFMAs per K step of 8 channels; lane l holds channels 4*(l/32) .. +3 and
+// MFMA j takes element j (k = 0 from the lower half wave, k = 1 from the upper).
+template <>
+struct Frag<float> {
+  static constexpr int kStep = 8;
+  static constexpr int kLane = 4;
+  using V = f4;
+  __device__ static V load(const float* p) { return *reinterpret_cast<const V*>(p); }
+  __device__ static void mma(const V& a, const V& b, f32x16& acc) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[0], b[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[1], b[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[2], b[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[3], b[3], acc, 0, 0, 0);
+  }
+  __device__ static void store4(float* p, float a, float b, float c, float d) {
+    *reinterpret_cast<f4*>(p) = f4{a, b, c, d};
+  }
+  __device__ static void load4(const float* p, float* o) {
+    const f4 v = *reinterpret_cast<const f4*>(p);
+    o[0] = v[0];
+    o[1] = v[1];
+    o[2] = v[2];
+    o[3] = v[3];
+  }
+};
+
+// One wave: output channels m0 .. m0+63 (m0 .. m0+31 when cout - m0 == 32) of pixels
+// x0 .. x0+63 of row y of image img. Requirements (checked by the launcher): cout a multiple of
+// 32, c0 and c1 multiples of the K step, H and W even when in0 is upsampled.
+template <typename T, int TAPS>
+__global__ __launch_bounds__(kBlock) void conv_kernel(ConvArgs a) {
+  using F = Frag<T>;
+  using V = typename F::V;
+  const int lane = threadIdx.x & 63;
+  const int wave = int((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const int col = lane & 31, half = lane >> 5;
+  const int nxb = (a.W + 63) >> 6;
+  const int ncb = (a.cout + 63) >> 6;
+  const int total = a.B * a.H * nxb * ncb;
+  if (wave >= total) return;
+  int t = wave;
+  const int cb = t % ncb;
+  t /= ncb;
+  const int xb = t % nxb;
+  t /= nxb;
+  const int y = t % a.H;
+  const int img = t / a.H;
+  const int m0 = cb * 64;
+  const bool two_m = m0 + 32 < a.cout;
+  const int cin = a.c0 + a.c1;
+  const int H0 = a.up0 ? (a.H >> 1) : a.H, W0 = a.up0 ? (a.W >> 1) : a.W;
+  const T* in0 = static_cast<const T*>(a.in0);
+  const T* in1 = static_cast<const T*>(a.in1);
+  const T* w = static_cast<const T*>(a.w);
+
+  f32x16 acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};
+  for (int tap = 0; tap < TAPS; ++tap) {
+    const int dy = TAPS == 9 ? tap / 3 - 1 : 0;
+    const int dx = TAPS == 9 ? tap % 3 - 1 : 0;
+    const int sy = y + dy;
+    const int px0 = xb * 64 + col, px1 = px0 + 32;
+    const int sx0 = px0 + dx, sx1 = px1 + dx;
+    const bool row_ok = sy >= 0 && sy < a.H;
+    const bool ok0 = row_ok && px0 < a.W && sx0 >= 0 && sx0 < a.W;
+    const bool ok1 = row_ok && px1 < a.W && sx1 >= 0 && sx1 < a.W;
+    // Element offsets of the two source pixels in in0 (possibly upsampled) and in1.
+    const int y0 = a.up0 ? (sy >> 1) : sy;
+    const size_t o00 = ok0 ? ((size_t(img) * H0 + y0) * W0 + (a.up0 ? (sx0 >> 1) : sx0)) * a.c0 : 0;
+    const size_t o01 = ok1 ? ((size_t(img) * H0 + y0) * W0 + (a.up0 ? (sx1 >> 1) : sx1)) * a.c0 : 0;
+    const size_t o10 = ok0 ? ((size_t(img) * a.H + sy) * a.W + sx0) * a.c1 : 0;
+    const size_t o11 = ok1 ? ((size_t(img) * a.H + sy) * a.W + sx1) * a.c1 : 0;
+    const T* wr0 = w + (size_t(m0 + col) * TAPS + tap) * cin + half * F::kLane;
+    const T* wr1 = two_m ? wr0 + size_t(32) * TAPS * cin : wr0;
+    for (int c = 0; c < cin; c += F::kStep) {
+      V b0 = {}, b1 = {};
+      if (c < a.c0) {
+        if (ok0) b0 = F::load(in0 + o00 + c + half * F::kLane);
+        if (ok1) b1 = F::load(in0 + o01 + c + half * F::kLane);
+      } else {
+        if (ok0) b0 = F::load(in1 + o10 + (c - a.c0) + half * F::kLane);
+        if (ok1) b1 = F::load(in1 + o11 + (c - a.c0) + half * F::kLane);
+      }
+      const V w0 = F::load(wr0 + c);
+      F::mma(w0, b0, acc00);
+      F::mma(w0, b1, acc01);
+      if (two_m) {
+        const V w1 = F::load(wr1 + c);
+        F::mma(w1, b0, acc10);
+        F::mma(w1, b1, acc11);
+      }
+    }
+  }
+  // Lane (col, half) of block (mt, nt) holds pixel x0 + 32 nt + col and output channels
+  // m0 + 32 mt + 8 j + 4 half + i (element 4 j + i).
+  T* out = static_cast<T*>(a.out);
+  const T* mask = static_cast<const T*>(a.mask);
+  for (int mt = 0; mt < (two_m ? 2 : 1); ++mt) {
+    for (int nt = 0; nt < 2; ++nt) {
+      const int px = xb * 64 + nt * 32 + col;
+      if (px >= a.W) continue;
+      const f32x16& acc = mt == 0 ? (nt == 0 ? acc00 : acc01) : (nt == 0 ? acc10 : acc11);
+      const size_t base = ((size_t(img) * a.H + y) * a.W + px) * a.cout;
+      for (int j = 0; j < 4; ++j) {
+        const int co = m0 + mt * 32 + 8 * j + 4 * half;
+        float v[4];
+        for (int i = 0; i < 4; ++i) {
+          v[i] = acc[4 * j + i];
+          if (a.bias) v[i] += a.bias[co + i];
+          if (a.relu) v[i] = fmaxf(v[i], 0.0f);
+        }
+        if (mask) {
+          float mv[4];
+          F::load4(mask + base + co, mv);
+          for (int i = 0; i < 4; ++i) v[i] = mv[i] > 0.0f ? v[i] : 0.0f;
+        }
+        F::store4(out + base + co, v[0], v[1], v[2], v[3]);
+      }
+    }
+  }
+}
+
+// One wave: dW block [m0, m0+32) x [n0, n0+32) of one tap over `chunk` pixels.
+__global__ __launch_bounds__(kBlock) void wgrad_kernel(WgradArgs a, int taps) {
+  const int lane = threadIdx.x & 63;
+  const int wave = int((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const int col = lane & 31, half = lane >> 5;
+  const int cin = a.c0 + a.c1;
+  const int tm = a.cout / 32, tn = (cin + 31) / 32;
+  const size_t npix = size_t(a.B) * a.H * a.W;
+  const int nchunks = int((npix + a.chunk - 1) / a.chunk);
+  const int total = tm * tn * taps * nchunks;
+  if (wave >= total) return;
+  int t = wave;
+  const int mi = t % tm;
+  t /= tm;
+  const int ni = t % tn;
+  t /= tn;
+  const int tap = t % taps;
+  const int chunk = t / taps;
+  const int dy = taps == 9 ? tap / 3 - 1 : 0, dx = taps == 9 ? tap % 3 - 1 : 0;
+  const int m0 = mi * 32, n0 = ni * 32;
+  const int co = m0 + col, ci = n0 + col;
+  const bool ci_ok = ci < cin;
+  const int H0 = a.up0 ? (a.H >> 1) : a.H, W0 = a.up0 ? (a.W >> 1) : a.W;
+  const size_t p_begin = size_t(chunk) * a.chunk;
+  const size_t p_end = p_begin + a.chunk < npix ? p_begin + a.chunk : npix;
+  f32x16 acc = {};
+  for (size_t p = p_begin; p < p_end; p += 2) {
+    const size_t pp = p + half;
+    float av = 0.0f, bv = 0.0f;
+    if (pp < p_end) {
+      av = a.g[pp * a.cout + co];
+      const int x = int(pp % a.W);
+      const size_t r = pp / a.W;
+      const int y = int(r % a.H);
+      const int img = int(r / a.H);
+      const int sy = y + dy, sx = x + dx;
+      if (ci_ok && sy >= 0 && sy < a.H && sx >= 0 && sx < a.W) {
+        if (ci < a.c0) {
+          const int yy = a.up0 ? (sy >> 1) : sy, xx = a.up0 ? (sx >> 1) : sx;
+          bv = a.in0[((size_t(img) * H0 + yy) * W0 + xx) * a.c0 + ci];
+        } else {
+          bv = a.in1[((size_t(img) * a.H + sy) * a.W + sx) * a.c1 + (ci - a.c0)];
+        }
+      }
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+  }
+  if (!ci_ok) return;
+  for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 4; ++i) {
+      const int row = m0 + 8 * j + 4 * half + i;
+      unsafeAtomicAdd(a.dw + (size_t(row) * taps + tap) * cin + ci, acc[4 * j + i]);
+    }
+}
+
+// db[co] += sum of g[p][co] over a chunk of 1024 pixels (one thread per channel).
+__global__ __launch_bounds__(kBlock) void bgrad_kernel(const float* g, float* db, int cout, size_t npix) {
+  const int co = threadIdx.x;
+  if (co >= cout) return;
+  const size_t p0 = size_t(blockIdx.x) * 1024;
+  const size_t p1 = p0 + 1024 < npix ? p0 + 1024 : npix;
+  float s = 0.0f;
+  for (size_t p = p0; p < p1; ++p) s += g[p * cout + co];
+  unsafeAtomicAdd(db + co, s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void pool_kernel(const T* in, T* out, int B, int H, int W, int C) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int Ho = H >> 1, Wo = W >> 1;
+  const size_t n = size_t(B) * Ho * Wo * C;
+  if (i >= n) return;
+  const int c = int(i % C);
+  size_t r = i / C;
+  const int x = int(r % Wo);
+  r /= Wo;
+  const int y = int(r % Ho);
+  const int b = int(r / Ho);
+  const size_t row0 = (size_t(b) * H + 2 * y) * W, row1 = row0 + W;
+  const float s = float(in[(row0 + 2 * x) * C + c]) + float(in[(row0 + 2 * x + 1) * C + c]) +
+                  float(in[(row1 + 2 * x) * C + c]) + float(in[(row1 + 2 * x + 1) * C + c]);
+  out[i] = T(s * 0.25f);
+}
+
+__global__ __launch_bounds__(kBlock) void grad_skip_pool_kernel(const float* dcat, int ctot, int off, const float* dP,
+                                                                const float* E, float* dE, int B, int H, int W,
+                                                                int C) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  const size_t n = size_t(B) * H * W * C;
+  if (i >= n) return;
+  const int c = int(i % C);
+  const size_t p = i / C;
+  float v = dcat ? dcat[p * ctot + off + c] : 0.0f;
+  if (dP) {
+    const int x = int(p % W);
+    const size_t r = p / W;
+    const int y = int(r % H);
+    const int b = int(r / H);
+    v += 0.25f * dP[((size_t(b) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1)) * C + c];
+  }
+  dE[i] = E[i] > 0.0f ? v : 0.0f;
+}
+
+// Coarse element (b, y, x, c): sum of dcat over the 2x2 fine block, channels [0, C).
+__global__ __launch_bounds__(kBlock) void grad_up_kernel(const float* dcat, int ctot, const float* D, float* dD, int B,
+                                                         int H, int W, int C) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int Hc = H >> 1, Wc = W >> 1;
+  const size_t n = size_t(B) * Hc * Wc * C;
+  if (i >= n) return;
+  const int c = int(i % C);
+  size_t r = i / C;
+  const int x = int(r % Wc);
+  r /= Wc;
+  const int y = int(r % Hc);
+  const int b = int(r / Hc);
+  const size_t row0 = (size_t(b) * H + 2 * y) * W, row1 = row0 + W;
+  const float s = dcat[(row0 + 2 * x) * ctot + c] + dcat[(row0 + 2 * x + 1) * ctot + c] +
+                  dcat[(row1 + 2 * x) * ctot + c] + dcat[(row1 + 2 * x + 1) * ctot + c];
+  dD[i] = D[i] > 0.0f ? s : 0.0f;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void prep_kernel(const float* raw, T* x, int B, int Hs, int Ws, int H, int W) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  const size_t n = size_t(B) * H * W;
+  if (i >= n) return;
+  const int px = int(i % W);
+  const size_t r = i / W;
+  const int py = int(r % H);
+  const int b = int(r / H);
+  const int sx = px < Ws ? px : Ws - 1, sy = py < Hs ? py : Hs - 1;
+  const float* s = raw + ((size_t(b) * Hs + sy) * Ws + sx) * kUnetInCh;
+  float v[kUnetInPhys];
+  for (int c = 0; c < 3; ++c) v[c] = log1pf(fmaxf(s[c], 0.0f));
+  for (int c = 3; c < 9; ++c) v[c] = s[c];
+  const float d = s[9];
+  v[9] = (d > 0.0f && d < __int_as_float(0x7f800000)) ? 1.0f / (1.0f + d) : 0.0f;
+  for (int c = 10; c < kUnetInPhys; ++c) v[c] = 0.0f;
+  T* o = x + i * kUnetInPhys;
+  for (int c = 0; c < kUnetInPhys; ++c) o[c] = T(v[c]);
+}
+
+__global__ __launch_bounds__(kBlock) void loss_kernel(const float* x, const float* y, int yc, const float* target,
+                                                      float* g, float* partial, size_t npix, float inv_n) {
+  __shared__ float red[kBlock];
+  const size_t p = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  float l = 0.0f;
+  if (p < npix) {
+    for (int c = 0; c < yc; ++c) {
+      float gv = 0.0f;
+      if (c < 3) {
+        const float z = x[p * kUnetInPhys + c] + y[p * yc + c];
+        const float tt = log1pf(fmaxf(target[p * 3 + c], 0.0f));
+        const float d = z - tt;
+        l += fabsf(d);
+        gv = d > 0.0f ? inv_n : (d < 0.0f ? -inv_n : 0.0f);
+      }
+      g[p * yc + c] = gv;
+    }
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void finish_kernel(const float* raw, const T* y, int yc, float* out, int Hs, int Ws,
+                                                        int W) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  const size_t n = size_t(Hs) * Ws;
+  if (i >= n) return;
+  const int px = int(i % Ws), py = int(i / Ws);
+  const size_t q = size_t(py) * W + px;
+  float o[3];
+  for (int c = 0; c < 3; ++c) {
+    const float z = log1pf(fmaxf(raw[i * kUnetInCh + c], 0.0f)) + float(y[q * yc + c]);
+    o[c] = expm1f(fmaxf(z, 0.0f));
+  }
+  reinterpret_cast<float4*>(out)[i] = make_float4(o[0], o[1], o[2], 1.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void transpose_w_kernel(const float* w, float* wt, int cout, int taps, int cin) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  const size_t n = size_t(cout) * taps * cin;
+  if (i >= n) return;
+  const int ci = int(i % cin);
+  const size_t r = i / cin;
+  const int tap = int(r % taps);
+  const int co = int(r / taps);
+  wt[(size_t(ci) * taps + (taps - 1 - tap)) * cout + co] = w[i];
+}
+
+__global__ __launch_bounds__(kBlock) void to_half_kernel(const float* in, _Float16* out, size_t n) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) out[i] = (_Float16)in[i];
+}
+
+__global__ __launch_bounds__(kBlock) void adam_kernel(float* w, float* m, float* v, const float* g, size_t n, float lr,
+                                                      float b1, float b2, float eps, float bc1, float bc2) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock); }
+inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
+
+}  // namespace unk
+
+template <typename T>
+int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
+  const int step = unk::Frag<T>::kStep;
+  if (a.cout % 32 || a.c0 % step || a.c1 % step || (a.up0 && (a.H % 2 || a.W % 2)) || (a.c1 && !a.in1) ||
+      (taps != 9 && taps != 1))
+    return -1;
+  const size_t waves = size_t(a.B) * a.H * ((a.W + 63) / 64) * ((a.cout + 63) / 64);
+  const unsigned grid = unsigned((waves * 64 + unk::kBlock - 1) / unk::kBlock);
+  if (taps == 9)
+    hipLaunchKernelGGL((unk::conv_kernel<T, 9>), dim3(grid), dim3(unk::kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((unk::conv_kernel<T, 1>), dim3(grid), dim3(unk::kBlock), 0, s, a);
+  return unk::ok();
+}
+template int launch_unet_conv<float>(const ConvArgs&, int, hipStream_t);
+template int launch_unet_conv<_Float16>(const ConvArgs&, int, hipStream_t);
+
+int launch_unet_wgrad(const WgradArgs& a, int taps, hipStream_t s) {
+  if (a.cout % 32 || a.chunk < 2 || a.chunk % 2 || (a.c1 && !a.in1) || (a.up0 && (a.H % 2 || a.W % 2))) return -1;
+  const size_t npix = size_t(a.B) * a.H * a.W;
+  const size_t chunks = (npix + a.chunk - 1) / a.chunk;
+  const size_t waves = size_t(a.cout / 32) * ((a.c0 + a.c1 + 31) / 32) * taps * chunks;
+  const unsigned grid = unsigned((waves * 64 + unk::kBlock - 1) / unk::kBlock);
+  hipLaunchKernelGGL(unk::wgrad_kernel, dim3(grid), dim3(unk::kBlock), 0, s, a, taps);
+  return unk::ok();
+}
+
+int launch_unet_bgrad(const float* g, float* db, int cout, size_t pixels, hipStream_t s) {
+  if (cout > unk::kBlock) return -1;
+  const unsigned grid = unsigned((pixels + 1023) / 1024);
+  hipLaunchKernelGGL(unk::bgrad_kernel, dim3(grid ? grid : 1), dim3(unk::kBlock), 0, s, g, db, cout, pixels);
+  return unk::ok();
+}
+
+template <typename T>
+int launch_unet_pool(const T* in, T* out, int B, int H, int W, int C, hipStream_t s) {
+  const size_t n = size_t(B) * (H / 2) * (W / 2) * C;
+  hipLaunchKernelGGL(unk::pool_kernel<T>, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, in, out, B, H, W, C);
+  return unk::ok();
+}
+template int launch_unet_pool<float>(const float*, float*, int, int, int, int, hipStream_t);
+template int launch_unet_pool<_Float16>(const _Float16*, _Float16*, int, int, int, int, hipStream_t);
+
+int launch_unet_grad_skip_pool(const float* dcat, int ctot, int off, const float* dP, const float* E, float* dE, int B,
+                               int H, int W, int C, hipStream_t s) {
+  const size_t n = size_t(B) * H * W * C;
+  hipLaunchKernelGGL(unk::grad_skip_pool_kernel, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, dcat, ctot, off, dP,
+                     E, dE, B, H, W, C);
+  return unk::ok();
+}
+
+int launch_unet_grad_up(const float* dcat, int ctot, const float* D, float* dD, int B, int H, int W, int C,
+                        hipStream_t s) {
+  const size_t n = size_t(B) * (H / 2) * (W / 2) * C;
+  hipLaunchKernelGGL(unk::grad_up_kernel, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, dcat, ctot, D, dD, B, H, W,
+                     C);
+  return unk::ok();
+}
+
+template <typename T>
+int launch_unet_prep(const float* raw, T* x, int B, int Hs, int Ws, int H, int W, hipStream_t s) {
+  const size_t n = size_t(B) * H * W;
+  hipLaunchKernelGGL(unk::prep_kernel<T>, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, raw, x, B, Hs, Ws, H, W);
+  return unk::ok();
+}
+template int launch_unet_prep<float>(const float*, float*, int, int, int, int, int, hipStream_t);
+template int launch_unet_prep<_Float16>(const float*, _Float16*, int, int, int, int, int, hipStream_t);
+
+int launch_unet_loss(const float* x, const float* y, int yc, const float* target, float* g, float* partial, int B, int H,
+                     int W, int* blocks, hipStream_t s) {
+  const size_t npix = size_t(B) * H * W;
+  const unsigned grid = unk::blocks_for(npix);
+  *blocks = int(grid);
+  hipLaunchKernelGGL(unk::loss_kernel, dim3(grid), dim3(unk::kBlock), 0, s, x, y, yc, target, g, partial, npix,
+                     1.0f / float(npix * 3));
+  return unk::ok();
+}
+
+template <typename T>
+int launch_unet_finish(const float* raw, const T* y, int yc, float* out_rgba, int Hs, int Ws, int W, hipStream_t s) {
+  const size_t n = size_t(Hs) * Ws;
+  hipLaunchKernelGGL(unk::finish_kernel<T>, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, raw, y, yc, out_rgba, Hs,
+                     Ws, W);
+  return unk::ok();
+}
+template int launch_unet_finish<float>(const float*, const float*, int, float*, int, int, int, hipStream_t);
+template int launch_unet_finish<_Float16>(const float*, const _Float16*, int, float*, int, int, int, hipStream_t);
+
+int launch_unet_transpose_w(const float* w, float* wt, int cout, int taps, int cin, hipStream_t s) {
+  const size_t n = size_t(cout) * taps * cin;
+  hipLaunchKernelGGL(unk::transpose_w_kernel, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, w, wt, cout, taps, cin);
+  return unk::ok();
+}
+
+int launch_unet_to_half(const float* in, _Float16* out, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(unk::to_half_kernel, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, in, out, n);
+  return unk::ok();
+}
+
+int launch_unet_adam(float* w, float* m, float* v, const float* g, size_t n, float lr, float b1, float b2, float eps,
+                     float bc1, float bc2, hipStream_t s) {
+  hipLaunchKernelGGL(unk::adam_kernel, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, w, m, v, g, n, lr, b1, b2, eps,
+                     bc1, bc2);
+  return unk::ok();
+}
+
+}  // namespace mrt
