@@ -231,8 +231,14 @@ int Scene::upload_all() {
       dev[j].tape_offset = sdf_prog_offset[sdf_current[j].sdf];
       dev[j].tape_len = sdf_prog_len[sdf_current[j].sdf];
     }
+    std::vector<SdfGrid> grids(dev.size());
+    for (size_t j = 0; j < dev.size(); ++j) grids[j] = sdf_grid[sdf_current[j].sdf];
+    std::vector<uint64_t> masks = sdf_masks;
+    if (masks.empty()) masks.push_back(~0ull);
     if ((rc = upload(d_sdf_prog, cap_sdf_tape, sdf_prog)) != MRT_OK) return rc;
     if ((rc = upload(d_sdf_drawables, cap_sdf_dr, dev)) != MRT_OK) return rc;
+    if ((rc = upload(d_sdf_grids, cap_sdf_grids, grids)) != MRT_OK) return rc;
+    if ((rc = upload(d_sdf_masks, cap_sdf_masks, masks)) != MRT_OK) return rc;
     sdf_dirty = false;
   }
   if (shading_dirty) {
@@ -262,7 +268,7 @@ int Scene::upload_all() {
 }
 
 Scene::~Scene() {
-  for (void* p : {(void*)d_sdf_drawables, (void*)d_sdf_prog, (void*)d_geo_info, (void*)d_idx, (void*)d_inst_material, (void*)d_pos, (void*)d_nrm,
+  for (void* p : {(void*)d_sdf_drawables, (void*)d_sdf_prog, (void*)d_sdf_grids, (void*)d_sdf_masks, (void*)d_geo_info, (void*)d_idx, (void*)d_inst_material, (void*)d_pos, (void*)d_nrm,
                   (void*)d_materials, (void*)d_lights})
     if (p) hipFree(p);
   if (d_blas) hipFree(d_blas);
@@ -336,6 +342,8 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     SdfParams sp{};
     sp.drawables = d_sdf_drawables;
     sp.prog = d_sdf_prog;
+    sp.grids = d_sdf_grids;
+    sp.masks = d_sdf_masks;
     sp.num_drawables = uint32_t(sdf_drawables.size());
     sp.rays = d_rays;
     sp.out = d_out;
@@ -841,6 +849,137 @@ static void compile_sdf(const SdfNode* t, size_t n, std::vector<SdfInsn>& out) {
   }
 }
 
+namespace {
+
+// Exact signed distance (double) and object-space bounding box of a chain primitive.
+double chain_prim_distance(const SdfInsn& in, double x, double y, double z) {
+  if (in.op & kSdfFuseTranslate) {
+    x -= in.t[0];
+    y -= in.t[1];
+    z -= in.t[2];
+  }
+  switch (in.op & 0xffu) {
+    case kSdfSphere:
+      return std::sqrt(x * x + y * y + z * z) - in.p[0];
+    case kSdfBox: {
+      const double qx = std::fabs(x) - in.p[0], qy = std::fabs(y) - in.p[1], qz = std::fabs(z) - in.p[2];
+      const double ox = std::max(qx, 0.0), oy = std::max(qy, 0.0), oz = std::max(qz, 0.0);
+      return std::sqrt(ox * ox + oy * oy + oz * oz) + std::min(std::max(qx, std::max(qy, qz)), 0.0);
+    }
+    case kSdfTorus: {
+      const double qx = std::sqrt(x * x + z * z) - in.p[0];
+      return std::sqrt(qx * qx + y * y) - in.p[1];
+    }
+    default: {  // capsule
+      const double yy = y - std::min(std::max(y, 0.0), double(in.p[0]));
+      return std::sqrt(x * x + yy * yy + z * z) - in.p[1];
+    }
+  }
+}
+
+void chain_prim_box(const SdfInsn& in, double lo[3], double hi[3]) {
+  double h[3], c[3] = {0, 0, 0};
+  switch (in.op & 0xffu) {
+    case kSdfSphere:
+      h[0] = h[1] = h[2] = in.p[0];
+      break;
+    case kSdfBox:
+      for (int a = 0; a < 3; ++a) h[a] = std::fabs(in.p[a]);
+      break;
+    case kSdfTorus:
+      h[0] = h[2] = double(in.p[0]) + in.p[1];
+      h[1] = in.p[1];
+      break;
+    default:  // capsule: y in [-r, height + r]
+      h[0] = h[2] = in.p[1];
+      h[1] = 0.5 * in.p[0] + in.p[1];
+      c[1] = 0.5 * in.p[0];
+      break;
+  }
+  for (int a = 0; a < 3; ++a) {
+    const double t = (in.op & kSdfFuseTranslate) ? in.t[a] : 0.0;
+    lo[a] = t + c[a] - h[a];
+    hi[a] = t + c[a] + h[a];
+  }
+}
+
+// Primitives whose distance is exact (1-Lipschitz and at least the distance to their box).
+bool chain_prim_ok(const SdfInsn& in) {
+  switch (in.op & 0xffu) {
+    case kSdfSphere:
+      return in.p[0] > 0.f;
+    case kSdfBox:
+      return true;
+    case kSdfTorus:
+      return in.p[0] >= 0.f && in.p[1] > 0.f;
+    case kSdfCapsule:
+      return in.p[0] >= 0.f && in.p[1] > 0.f;
+    default:
+      return false;
+  }
+}
+
+// Builds the culling grid of a compiled program if it is one union chain of exact primitives.
+void build_sdf_grid(const SdfInsn* prog, uint32_t n, const Box& box, SdfGrid& g, std::vector<uint64_t>& masks) {
+  g = SdfGrid{};
+  if (n < 2 || n > uint32_t(kSdfChainMax)) return;
+  if (!chain_prim_ok(prog[0]) || (prog[0].op & ~(0xffu | kSdfFuseTranslate)) != 0) return;
+  for (uint32_t j = 1; j < n; ++j) {
+    const uint32_t fused = (prog[j].op >> 8) & 0xffu;
+    if (!chain_prim_ok(prog[j]) || (prog[j].op >> 17) != 0) return;
+    if (fused != kSdfUnion && !(fused == kSdfSmoothUnion && prog[j].k > 0.f)) return;
+  }
+  double ext[3];
+  for (int a = 0; a < 3; ++a) {
+    ext[a] = double(box.mx[a]) - double(box.mn[a]);
+    if (!(ext[a] > 0.0) || !std::isfinite(ext[a])) return;
+  }
+  std::vector<double> blo(3 * n), bhi(3 * n);
+  for (uint32_t j = 0; j < n; ++j) chain_prim_box(prog[j], &blo[3 * j], &bhi[3 * j]);
+  const int D = kSdfGridDim;
+  g.dim = uint32_t(D);
+  g.mask_offset = uint32_t(masks.size());
+  for (int a = 0; a < 3; ++a) {
+    g.lo[a] = box.mn[a];
+    g.inv[a] = float(double(D) / ext[a]);
+  }
+  masks.reserve(masks.size() + size_t(D) * D * D);
+  for (int cz = 0; cz < D; ++cz)
+    for (int cy = 0; cy < D; ++cy)
+      for (int cx = 0; cx < D; ++cx) {
+        const int ci[3] = {cx, cy, cz};
+        double clo[3], chi[3], ctr[3], half2 = 0;
+        for (int a = 0; a < 3; ++a) {
+          // The cell the device computes in binary32 may reach a little past the exact one.
+          const double size = ext[a] / D, slack = 1e-4 * ext[a];
+          clo[a] = double(box.mn[a]) + size * ci[a] - slack;
+          chi[a] = double(box.mn[a]) + size * (ci[a] + 1) + slack;
+          ctr[a] = 0.5 * (clo[a] + chi[a]);
+          half2 += 0.25 * (chi[a] - clo[a]) * (chi[a] - clo[a]);
+        }
+        const double half = std::sqrt(half2);
+        uint64_t mask = 1;
+        double upper = chain_prim_distance(prog[0], ctr[0], ctr[1], ctr[2]) + half;
+        for (uint32_t j = 1; j < n; ++j) {
+          double gap2 = 0;
+          bool overlap = true;
+          for (int a = 0; a < 3; ++a) {
+            const double gap = std::max(0.0, std::max(blo[3 * j + a] - chi[a], clo[a] - bhi[3 * j + a]));
+            if (gap > 0) overlap = false;
+            gap2 += gap * gap;
+          }
+          const bool smooth = ((prog[j].op >> 8) & 0xffu) == kSdfSmoothUnion;
+          const double margin = (smooth ? double(prog[j].k) * (1.0 + 1e-3) : 0.0) + 1e-4 * (1.0 + std::fabs(upper));
+          const bool drop = !overlap && std::sqrt(gap2) > upper + margin;
+          if (!drop) mask |= uint64_t(1) << j;
+          upper = std::min(upper, chain_prim_distance(prog[j], ctr[0], ctr[1], ctr[2]) + half);
+        }
+        masks.push_back(mask);
+      }
+}
+
+}  // namespace
+
 int mrt_scene_push_sdf(mrt_scene sh, const mrt_sdf_node* nodes, size_t n, uint32_t* out) {
   MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
@@ -857,6 +996,9 @@ int mrt_scene_push_sdf(mrt_scene sh, const mrt_sdf_node* nodes, size_t n, uint32
   s->sdf_prog_offset.push_back(uint32_t(s->sdf_prog.size()));
   compile_sdf(t, n, s->sdf_prog);
   s->sdf_prog_len.push_back(uint32_t(s->sdf_prog.size()) - s->sdf_prog_offset.back());
+  SdfGrid g;
+  build_sdf_grid(s->sdf_prog.data() + s->sdf_prog_offset.back(), s->sdf_prog_len.back(), b, g, s->sdf_masks);
+  s->sdf_grid.push_back(g);
   s->sdf_dirty = true;
   if (out) *out = uint32_t(s->sdf_len.size() - 1);
   return MRT_OK;

@@ -49,6 +49,8 @@ struct Scene {
   std::vector<uint32_t> sdf_offset, sdf_len;
   std::vector<SdfInsn> sdf_prog;  // compiled tapes (device form)
   std::vector<uint32_t> sdf_prog_offset, sdf_prog_len;
+  std::vector<SdfGrid> sdf_grid;    // per program
+  std::vector<uint64_t> sdf_masks;  // culling masks of all grids
   std::vector<Box> sdf_bounds;
   std::vector<SdfInstance> sdf_pending, sdf_current;
   std::vector<SdfDrawable> sdf_drawables;
@@ -57,6 +59,9 @@ struct Scene {
   float sdf_eps = 1e-5f;    // sequential_pathtracer.hpp:136
   SdfDrawable* d_sdf_drawables = nullptr;
   SdfInsn* d_sdf_prog = nullptr;
+  SdfGrid* d_sdf_grids = nullptr;
+  uint64_t* d_sdf_masks = nullptr;
+  size_t cap_sdf_grids = 0, cap_sdf_masks = 0;
   size_t cap_sdf_dr = 0, cap_sdf_tape = 0;
   std::vector<Instance> pending;  // enqueued since the last prepare
   std::vector<Instance> current;  // the prepared frame

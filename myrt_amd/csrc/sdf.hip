@@ -180,6 +180,56 @@ __device__ inline float map(const SdfInsn* prog, uint32_t len, float px, float p
   return st.d0;
 }
 
+// map() of a union-chain program with a culling grid: the lane looks up the cell of its point and
+// evaluates the head and the chain instructions of that cell's mask, in chain order. Dropped
+// instructions leave the running distance and material unchanged bit for bit, so the result is
+// map()'s. Points outside the grid evaluate every instruction.
+__device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGrid& g, const uint64_t* masks, float px,
+                                  float py, float pz, uint32_t& mat) {
+  uint64_t mask = len >= 64 ? ~0ull : ((1ull << len) - 1ull);
+  const float fx = (px - g.lo[0]) * g.inv[0], fy = (py - g.lo[1]) * g.inv[1], fz = (pz - g.lo[2]) * g.inv[2];
+  const float dim = float(g.dim);
+  if (fx >= 0.f && fx < dim && fy >= 0.f && fy < dim && fz >= 0.f && fz < dim) {
+    const uint32_t cx = uint32_t(fx), cy = uint32_t(fy), cz = uint32_t(fz);
+    mask = masks[g.mask_offset + (cz * g.dim + cy) * g.dim + cx];
+  }
+  const SdfInsn h = prog[0];
+  float qx = px, qy = py, qz = pz;
+  if (h.op & kSdfFuseTranslate) {
+    qx = px - h.t[0];
+    qy = py - h.t[1];
+    qz = pz - h.t[2];
+  }
+  float r = prim_distance(h.op & 0xffu, h, qx, qy, qz);
+  uint32_t m = h.material;
+  mask &= ~1ull;
+  while (mask) {
+    const uint32_t j = uint32_t(__builtin_ctzll(mask));
+    mask &= mask - 1ull;
+    const SdfInsn in = prog[j];
+    qx = px;
+    qy = py;
+    qz = pz;
+    if (in.op & kSdfFuseTranslate) {
+      qx = px - in.t[0];
+      qy = py - in.t[1];
+      qz = pz - in.t[2];
+    }
+    const float d = prim_distance(in.op & 0xffu, in, qx, qy, qz);
+    uint32_t mm;
+    r = combine((in.op >> 8) & 0xffu, in.k, r, m, d, in.material, mm);
+    m = mm;
+  }
+  mat = m;
+  return r;
+}
+
+// Distance of drawable program at a point, culled when the program has a grid.
+__device__ inline float map_any(const SdfInsn* prog, uint32_t len, const SdfGrid& g, const uint64_t* masks, float px,
+                                float py, float pz, uint32_t& mat) {
+  return g.dim ? map_chain(prog, len, g, masks, px, py, pz, mat) : map(prog, len, px, py, pz, mat);
+}
+
 __device__ inline uint32_t oct_encode(float x, float y, float z) {
   const float s = fabsf(x) + fabsf(y) + fabsf(z);
   float u = x / s, v = y / s;
@@ -214,9 +264,9 @@ __device__ inline bool clip(const SdfDrawable& dr, float ox, float oy, float oz,
 }
 
 // march_sdf (raymarch.glsl:116-171): world hit distance or -1.
-__device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, float ox, float oy, float oz, float dx,
-                              float dy, float dz, float t_enter, float t_exit, int steps, float eps, float& lx,
-                              float& ly, float& lz, float& len_out) {
+__device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, const SdfGrid& grid, const uint64_t* masks,
+                              float ox, float oy, float oz, float dx, float dy, float dz, float t_enter, float t_exit,
+                              int steps, float eps, float& lx, float& ly, float& lz, float& len_out) {
   const float* m = dr.inv;
   lx = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
   ly = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
@@ -232,7 +282,7 @@ __device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, float 
   for (int i = 0; i < steps && t / len < t_exit; ++i) {
     const float s = t / len;
     uint32_t mat;
-    const float d = map(tape, dr.tape_len, lx + rx * s, ly + ry * s, lz + rz * s, mat);
+    const float d = map_any(tape, dr.tape_len, grid, masks, lx + rx * s, ly + ry * s, lz + rz * s, mat);
     if (si == 0) si = d > 0.f ? 1 : (d < 0.f ? -1 : 0);
     if (fabsf(d) < eps) {
       res = t;
@@ -271,7 +321,8 @@ __global__ __launch_bounds__(kBlock) void sdf_kernel(SdfParams p) {
     float t0 = r0.w, t1 = best;
     if (!clip(dr, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1)) continue;
     float lx, ly, lz, len;
-    const float tw = march(dr, p.prog + dr.tape_offset, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1, p.steps, p.eps, lx,
+    const SdfGrid grid = p.grids[j];
+    const float tw = march(dr, p.prog + dr.tape_offset, grid, p.masks, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1, p.steps, p.eps, lx,
                            ly, lz, len);
     if (tw >= 0.f && tw < best && tw > r0.w) {
       best = tw;
@@ -291,6 +342,7 @@ __global__ __launch_bounds__(kBlock) void sdf_kernel(SdfParams p) {
   // Normal: the tetrahedral gradient of raymarch.glsl:100-114, offset 0.0005 / len.
   const SdfDrawable dr = p.drawables[best_j];
   const SdfInsn* tape = p.prog + dr.tape_offset;
+  const SdfGrid grid = p.grids[best_j];
   float nx = 0, ny = 0, nz = 0;
   for (int k = 0; k < 4; ++k) {
     const float ex = 0.5773f * (2.0f * float(((k + 3) >> 1) & 1) - 1.0f);
@@ -298,13 +350,13 @@ __global__ __launch_bounds__(kBlock) void sdf_kernel(SdfParams p) {
     const float ez = 0.5773f * (2.0f * float(k & 1) - 1.0f);
     const float o = 0.0005f / blen;
     uint32_t mm;
-    const float d = map(tape, dr.tape_len, bx + o * ex, by + o * ey, bz + o * ez, mm) * blen;
+    const float d = map_any(tape, dr.tape_len, grid, p.masks, bx + o * ex, by + o * ey, bz + o * ez, mm) * blen;
     nx = nx + ex * d;
     ny = ny + ey * d;
     nz = nz + ez * d;
   }
   uint32_t mat;
-  map(tape, dr.tape_len, bx, by, bz, mat);
+  map_any(tape, dr.tape_len, grid, p.masks, bx, by, bz, mat);
   const float* iv = dr.inv;  // world normal = (M^-1)^T n
   float wx = iv[0] * nx + iv[4] * ny + iv[8] * nz;
   float wy = iv[1] * nx + iv[5] * ny + iv[9] * nz;

@@ -63,4 +63,18 @@ struct SdfDrawable {
 };
 static_assert(sizeof(SdfDrawable) == 80, "sdf drawable is 80 bytes");
 
+// Culling grid of a program that is one chain of primitives joined by (smooth) unions: for each
+// cell of a regular grid over the program's object-space bounds, the set of chain instructions
+// (bit j = instruction j, bit 0 = the head) whose primitive can change the distance somewhere in
+// the cell. A dropped primitive is at least k (plus a rounding margin) above the running value
+// everywhere in the cell, where the union returns its left operand bit for bit.
+struct SdfGrid {
+  float lo[3];
+  uint32_t dim;          // cells per axis; 0 = no grid (every instruction is evaluated)
+  float inv[3];          // cells per object-space unit
+  uint32_t mask_offset;  // first cell's mask in the mask array
+};
+constexpr int kSdfGridDim = 32;
+constexpr int kSdfChainMax = 64;
+
 }  // namespace mrt

@@ -11,6 +11,8 @@ namespace mrt {
 struct SdfParams {
   const SdfDrawable* drawables;
   const SdfInsn* prog;     // compiled tapes; drawables address them
+  const SdfGrid* grids;    // per drawable
+  const uint64_t* masks;   // culling masks of all grids
   uint32_t num_drawables;
   const void* rays;        // mrt_ray[n]
   void* out;               // mrt_hit[n] (closest) or uint8_t[n] (occluded)
