@@ -308,6 +308,61 @@ def wavefront_scene(n_spheres=2000, level=3, seed=7, ground_cells=(1000, 750), n
                 fovy=45.0)
 
 
+def training_scene(seed):
+    """A seeded random scene for denoiser training data: 20-1500 icospheres (level 2 or 3, radius
+    U[0.1, 0.6], centre U[-5, 5]^3) with GGX, metal, glass and Lambert materials over a flat or
+    displaced ground, optionally an SDF cluster, one quad light of random size, place and power,
+    a random sky level and a camera on a random orbit. About 10K to 2M triangles."""
+    rng = np.random.default_rng(10_000 + seed)
+    n = int(rng.integers(20, 1500))
+    level = int(rng.integers(2, 4))
+    (fi, fp, fn), mats = sphere_field(n, level, seed=int(rng.integers(1 << 30)), radius=(0.1, 0.6))
+    materials = np.zeros(n + 5, MATERIAL_DTYPE)
+    for k in range(n):
+        albedo = rng.uniform(0.05, 0.95, 3)
+        materials[k] = (_rgba8(albedo), 1.5, rng.uniform(0.02, 1.0), 0.0, 0.0, 0, (0, 0))
+        u = rng.random()
+        if u < 0.12:
+            materials[k]["transmission"] = 1.0
+            materials[k]["roughness"] = rng.uniform(0.0, 0.3)
+        elif u < 0.3:
+            materials[k]["metallic"] = 1.0
+        elif u < 0.45:
+            materials[k]["roughness"] = 1.0
+            materials[k]["ior"] = 1.0
+    ground_mat = n
+    materials[ground_mat] = (_rgba8(rng.uniform(0.2, 0.9, 3)), float(rng.choice([1.0, 1.5])),
+                             rng.uniform(0.3, 1.0), 0.0, 0.0, 0, (0, 0))
+    for j in range(4):
+        materials[n + 1 + j] = (_rgba8(rng.uniform(0.1, 0.9, 3)), 1.5, rng.uniform(0.1, 1.0), 0.0, 0.0, 0, (0, 0))
+    geometries = [(fi, fp, fn)]
+    instances = [(0, mt, k) for k, mt in enumerate(mats)]
+    if rng.random() < 0.5:
+        gi, gp, gn = ground_quad(size=12.0, y=-5.5)
+    else:
+        cells = int(rng.integers(100, 600))
+        gi, gp, gn, _ = heightfield(cells, cells, seed=int(rng.integers(1 << 30)), extent=(24.0, 24.0),
+                                    height=float(rng.uniform(0.3, 2.0)))
+        gp = gp + np.array([0.0, -5.5, 0.0], np.float32)
+    geometries.append((gi, gp, gn))
+    instances.append((1, np.eye(4, dtype=np.float32), ground_mat))
+    lights = np.zeros(1, LIGHT_DTYPE)
+    size = rng.uniform(1.0, 5.0)
+    corner = (rng.uniform(-6, 3), rng.uniform(5, 10), rng.uniform(-6, 3))
+    power = rng.uniform(4.0, 25.0) * rng.uniform(0.7, 1.0, 3)
+    lights[0] = (corner, (size, 0.0, 0.0), (0.0, 0.0, size), tuple(power))
+    az, el, dist = rng.uniform(0, 2 * np.pi), rng.uniform(0.05, 0.9), rng.uniform(9, 20)
+    eye = (dist * np.cos(el) * np.sin(az), dist * np.sin(el) - 1.0, dist * np.cos(el) * np.cos(az))
+    out = dict(geometries=geometries, instances=instances, materials=materials, lights=lights, eye=eye,
+               target=tuple(rng.uniform(-2, 2, 3)), fovy=float(rng.uniform(30, 60)), env=float(rng.uniform(0.0, 1.5)))
+    if rng.random() < 0.4:
+        tape = sdf_cluster(int(rng.integers(8, 65)), seed=int(rng.integers(1 << 30)), k=0.1, n_materials=4)
+        tape["material"] += np.uint32(n + 1)
+        out["sdf_tape"] = tape
+        out["sdf_transform"] = translate_scale(tuple(rng.uniform(-2, 2, 3)), float(rng.uniform(0.8, 2.0)))
+    return out
+
+
 def world_triangles(geometries, instances):
     """World-space triangles of every instance (n, 3, 3) float64 and, per triangle, the point its
     outward normal faces away from: the instance's origin for closed meshes (geometry 0), a point far
