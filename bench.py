@@ -111,6 +111,82 @@ def wavefront_extras(args):
                        "mrays_per_s": round(n_rays / dt / 1e6, 2)}}
 
 
+def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budget_spp=64):
+    """Equal-time MSE against a converged render of the 4M-triangle wavefront scene: the plain path
+    tracer, plain + denoiser, radiance cache, and radiance cache + denoiser, each given the time the
+    plain tracer takes for budget_spp samples per pixel."""
+    import myrt_amd as m
+    from myrt_amd import scenes
+    weights = os.path.join(ROOT, "myrt_amd", "data", "denoiser_unet.npy")
+    sc = scenes.wavefront_scene()
+    s = build_wavefront_scene(sc)
+    s.prepare()
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
+
+    def renderer():
+        r = m.Renderer()
+        r.set_camera(view, proj)
+        r.set_num_bounces(depth)
+        return r
+
+    log("equal-time study: reference %d spp at %dx%d" % (ref_spp, W, H))
+    ref_r = renderer()
+    done = 0
+    while done < ref_spp:
+        ref_r.render(s, W, H, 256)
+        done += 256
+    ref = ref_r.image()[..., :3].astype(np.float64)
+
+    def mse(img):
+        return float(((img[..., :3].astype(np.float64) - ref) ** 2).mean())
+
+    def timed(fn):
+        m.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        m.synchronize()
+        return out, time.perf_counter() - t0
+
+    warm = renderer()
+    warm.render(s, W, H, 16)  # allocations and first-launch costs stay out of the timings
+    plain = renderer()
+    _, budget = timed(lambda: plain.render(s, W, H, budget_spp))
+    res = {"width": W, "height": H, "depth": depth, "reference_spp": ref_spp, "budget_s": round(budget, 4),
+           "plain": {"spp": budget_spp, "mse": mse(plain.image())}}
+    den = None
+    if os.path.exists(weights):
+        den = m.Denoiser()
+        den.load(weights)
+        plain.denoise(den)  # warm-up
+        img, t_dn = timed(lambda: plain.denoise(den))
+        res["plain_denoised"] = {"spp": budget_spp, "mse": mse(img), "denoise_s": round(t_dn, 4)}
+    # Radiance cache: the first pass only trains (unbiased samples), later passes end paths in it.
+    lo, hi = np.array([-12.5, -7.0, -9.5], np.float32), np.array([12.5, 6.0, 9.5], np.float32)
+    cache = m.RadianceCache(seed=1)
+    cache.set_bounds(lo, hi)
+    cr = renderer()
+    cr.set_cache(cache, mode=1)
+    spent, spp = 0.0, 0
+    t_dn = 0.0 if den is None else res["plain_denoised"]["denoise_s"]
+    while True:
+        _, dt = timed(lambda: cr.render(s, W, H, 16))
+        spent += dt
+        spp += 16
+        if spp == 16:
+            cr.set_cache(cache, mode=2)
+        if spent + dt * 0.5 + t_dn > budget:
+            break
+    res["cache"] = {"spp": spp, "seconds": round(spent, 4), "mse": mse(cr.image())}
+    if den is not None:
+        img, t = timed(lambda: cr.denoise(den))
+        res["cache_denoised"] = {"spp": spp, "seconds": round(spent + t, 4), "mse": mse(img)}
+        res["mse_ratio_plain_over_cache_denoised"] = round(res["plain"]["mse"] / res["cache_denoised"]["mse"], 3)
+        res["mse_ratio_plain_over_plain_denoised"] = round(res["plain"]["mse"] / res["plain_denoised"]["mse"], 3)
+    res["mse_ratio_plain_over_cache"] = round(res["plain"]["mse"] / res["cache"]["mse"], 3)
+    s.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +196,8 @@ def main():
     ap.add_argument("--only-incoherent", action="store_true",
                     help="time only the incoherent closest-hit passes (for profiler runs)")
     ap.add_argument("--no-wavefront", action="store_true", help="skip the 4M-triangle wavefront scene")
+    ap.add_argument("--no-study", action="store_true", help="skip the equal-time MSE study")
+    ap.add_argument("--only-study", action="store_true", help="run only the equal-time MSE study")
     ap.add_argument("--only-wavefront", action="store_true",
                     help="run only the 4M-triangle wavefront scene and print its fields (for profiler runs)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="incoherent rays timed on the host")
@@ -143,6 +221,9 @@ def main():
         print(json.dumps({"copy_bytes": n, "launches": 3}), flush=True)
         return
 
+    if args.only_study:
+        print(json.dumps(equal_time_study(args)), flush=True)
+        return
     if args.only_wavefront:
         print(json.dumps(wavefront_extras(args)), flush=True)
         return
@@ -212,6 +293,7 @@ def main():
     st_coh = scene.trace_stats(d_coh, n_coh)
 
     wave = None if (args.only_incoherent or args.no_wavefront) else wavefront_extras(args)
+    study = None if (args.only_incoherent or args.no_study) else equal_time_study(args)
 
     cpu = None
     if not args.no_cpu_baseline and not args.only_incoherent:
@@ -253,6 +335,7 @@ def main():
             "bvh": {"wide_nodes": prep["blas_wide_nodes"], "tri_records": prep["blas_tri_records"],
                     "device_bytes": prep["device_bytes"], "host_build_s": round(build_s, 3)},
             "wavefront_4M": wave,
+            "equal_time": study,
         },
     }
     print(json.dumps(out), flush=True)
