@@ -111,10 +111,13 @@ def wavefront_extras(args):
                        "mrays_per_s": round(n_rays / dt / 1e6, 2)}}
 
 
-def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budget_spp=64):
-    """Equal-time MSE against a converged render of the 4M-triangle wavefront scene: the plain path
-    tracer, plain + denoiser, radiance cache, and radiance cache + denoiser, each given the time the
-    plain tracer takes for budget_spp samples per pixel."""
+def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 64), warm_frames=16):
+    """Equal-time MSE against a converged render (ref_spp samples) of the 4M-triangle wavefront
+    scene. For each budget (the time the plain path tracer takes for that many samples per pixel)
+    four methods get the same time: plain, plain + denoiser, radiance cache, radiance cache +
+    denoiser. The cache starts from warm_frames frames of 16 samples of the same view (an
+    interactive session's previous frames; not charged to the budget) and keeps training while
+    it renders."""
     import myrt_amd as m
     from myrt_amd import scenes
     weights = os.path.join(ROOT, "myrt_amd", "data", "denoiser_unet.npy")
@@ -147,44 +150,111 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budget_spp=64):
         m.synchronize()
         return out, time.perf_counter() - t0
 
-    warm = renderer()
-    warm.render(s, W, H, 16)  # allocations and first-launch costs stay out of the timings
+    def fresh(r):
+        r.invalidate()
+        return r
+
     plain = renderer()
-    _, budget = timed(lambda: plain.render(s, W, H, budget_spp))
-    res = {"width": W, "height": H, "depth": depth, "reference_spp": ref_spp, "budget_s": round(budget, 4),
-           "plain": {"spp": budget_spp, "mse": mse(plain.image())}}
+    plain.render(s, W, H, 16)  # allocations and first launches stay out of the timings
     den = None
     if os.path.exists(weights):
         den = m.Denoiser()
         den.load(weights)
-        plain.denoise(den)  # warm-up
-        img, t_dn = timed(lambda: plain.denoise(den))
-        res["plain_denoised"] = {"spp": budget_spp, "mse": mse(img), "denoise_s": round(t_dn, 4)}
-    # Radiance cache: the first pass only trains (unbiased samples), later passes end paths in it.
-    lo, hi = np.array([-12.5, -7.0, -9.5], np.float32), np.array([12.5, 6.0, 9.5], np.float32)
+        plain.denoise(den)
     cache = m.RadianceCache(seed=1)
-    cache.set_bounds(lo, hi)
+    cache.set_bounds((-12.5, -7.0, -9.5), (12.5, 6.0, 9.5))
+    warm = renderer()
+    warm.set_cache(cache, mode=2)
+    for _ in range(warm_frames):
+        warm.render(s, W, H, 16)
     cr = renderer()
-    cr.set_cache(cache, mode=1)
-    spent, spp = 0.0, 0
-    t_dn = 0.0 if den is None else res["plain_denoised"]["denoise_s"]
-    while True:
-        _, dt = timed(lambda: cr.render(s, W, H, 16))
-        spent += dt
-        spp += 16
-        if spp == 16:
-            cr.set_cache(cache, mode=2)
-        if spent + dt * 0.5 + t_dn > budget:
-            break
-    res["cache"] = {"spp": spp, "seconds": round(spent, 4), "mse": mse(cr.image())}
-    if den is not None:
-        img, t = timed(lambda: cr.denoise(den))
-        res["cache_denoised"] = {"spp": spp, "seconds": round(spent + t, 4), "mse": mse(img)}
-        res["mse_ratio_plain_over_cache_denoised"] = round(res["plain"]["mse"] / res["cache_denoised"]["mse"], 3)
-        res["mse_ratio_plain_over_plain_denoised"] = round(res["plain"]["mse"] / res["plain_denoised"]["mse"], 3)
-    res["mse_ratio_plain_over_cache"] = round(res["plain"]["mse"] / res["cache"]["mse"], 3)
+    cr.set_cache(cache, mode=2)
+    cr.render(s, W, H, 16)
+    _, t16 = timed(lambda: fresh(cr).render(s, W, H, 16))
+    res = {"width": W, "height": H, "depth": depth, "reference_spp": ref_spp, "cache_warm_frames": warm_frames,
+           "cache_seconds_per_16spp": round(t16, 4), "budgets": []}
+    for b in budgets:
+        _, T = timed(lambda: fresh(plain).render(s, W, H, b))
+        row = {"budget_spp": b, "budget_s": round(T, 4), "plain_mse": mse(plain.image())}
+        t_dn = 0.0
+        if den is not None:
+            img, t_dn = timed(lambda: plain.denoise(den))
+            row["plain_denoised_mse"] = mse(img)
+            row["denoise_s"] = round(t_dn, 4)
+        # The cache path gets as many samples as fit in the budget at its measured rate.
+        spp_c = max(1, int((T - t_dn) / t16 * 16))
+        _, tc = timed(lambda: fresh(cr).render(s, W, H, spp_c))
+        row.update({"cache_spp": spp_c, "cache_s": round(tc, 4), "cache_mse": mse(cr.image())})
+        if den is not None:
+            img, t = timed(lambda: cr.denoise(den))
+            row["cache_denoised_mse"] = mse(img)
+            row["ratio_plain_over_cache_denoised"] = round(row["plain_mse"] / row["cache_denoised_mse"], 3)
+            row["ratio_plain_over_plain_denoised"] = round(row["plain_mse"] / row["plain_denoised_mse"], 3)
+        row["ratio_plain_over_cache"] = round(row["plain_mse"] / row["cache_mse"], 3)
+        res["budgets"].append(row)
+        log("budget %d spp: %s" % (b, row))
+    fresh(cr).render(s, W, H, 256)
+    res["cache_256spp_mse"] = mse(cr.image())  # bias floor of the trained cache
     s.close()
     return res
+
+
+UNET_LEVELS = (32, 48, 64, 96, 128)
+
+
+def unet_macs_per_pixel():
+    """Multiply-adds per full-resolution pixel of the denoiser (logical channel counts)."""
+    macs, c_in = 0.0, 10
+    for l, c in enumerate(UNET_LEVELS):
+        macs += (9 * c_in * c + 9 * c * c) / 4 ** l
+        c_in = c
+    for l in range(len(UNET_LEVELS) - 2, -1, -1):
+        c = UNET_LEVELS[l]
+        macs += (9 * (UNET_LEVELS[l + 1] + c) * c + 9 * c * c) / 4 ** l
+    return macs + UNET_LEVELS[0] * 3
+
+
+def learned_kernels(args):
+    """Timings of the learned parts on the device: the denoiser at 1920x1080 (fp16 MFMA), the
+    radiance-cache MLP inference on 2^20 queries and one training step on 2^16 samples (bf16 MFMA)."""
+    import myrt_amd as m
+    out = {}
+    rng = np.random.default_rng(0)
+    W, H = 1920, 1080
+    raw = np.concatenate([rng.uniform(0, 2, (H, W, 3)), rng.uniform(0, 1, (H, W, 3)),
+                          rng.normal(size=(H, W, 3)), rng.uniform(1, 20, (H, W, 1))], -1).astype(np.float32)
+    d = m.Denoiser(seed=1)
+    d_raw = m.DeviceBuffer.from_array(raw)
+    d_out = m.DeviceBuffer(W * H * 16)
+    step = lambda: m._check(m.lib().mrt_denoiser_denoise_device(d.h, d_raw.ptr, H, W, d_out.ptr, None))
+    step()
+    _, ms = timed_passes(step, 10)
+    flops = 2.0 * unet_macs_per_pixel() * W * ((H + 15) // 16 * 16)
+    out["denoiser_1080p"] = {"ms": round(ms, 3), "tflops": round(flops / (ms * 1e-3) / 1e12, 1),
+                             "frac_of_fp16_mfma_peak": round(flops / (ms * 1e-3) / 2.5e15, 4),
+                             "flop": flops}
+    c = m.RadianceCache(seed=1)
+    nq = 1 << 20
+    q = np.zeros(nq, m.QUERY_DTYPE)
+    q["pos"] = rng.uniform(-1, 1, (nq, 3))
+    q["dir"] = rng.normal(size=(nq, 3))
+    q["nrm"] = rng.normal(size=(nq, 3))
+    q["roughness"] = rng.uniform(0, 1, nq)
+    q["albedo"] = rng.uniform(0, 1, (nq, 3))
+    d_q = m.DeviceBuffer.from_array(q)
+    d_y = m.DeviceBuffer(nq * 12)
+    inf = lambda: m._check(m.lib().mrt_cache_infer_device(c.h, d_q.ptr, nq, d_y.ptr, None))
+    inf()
+    _, ms_i = timed_passes(inf, 10)
+    nt = 1 << 16
+    tgt = m.DeviceBuffer.from_array(rng.uniform(0, 1, (nt, 3)).astype(np.float32))
+    trn = lambda: m._check(m.lib().mrt_cache_train_device(c.h, d_q.ptr, tgt.ptr, nt, None))
+    trn()
+    _, ms_t = timed_passes(trn, 10)
+    out["cache_infer_2^20"] = {"ms": round(ms_i, 3), "tflops": round(41344.0 * nq / (ms_i * 1e-3) / 1e12, 1),
+                               "frac_of_bf16_mfma_peak": round(41344.0 * nq / (ms_i * 1e-3) / 2.5e15, 4)}
+    out["cache_train_2^16"] = {"ms": round(ms_t, 3), "tflops": round(3 * 41344.0 * nt / (ms_t * 1e-3) / 1e12, 1)}
+    return out
 
 
 def main():
@@ -198,6 +268,7 @@ def main():
     ap.add_argument("--no-wavefront", action="store_true", help="skip the 4M-triangle wavefront scene")
     ap.add_argument("--no-study", action="store_true", help="skip the equal-time MSE study")
     ap.add_argument("--only-study", action="store_true", help="run only the equal-time MSE study")
+    ap.add_argument("--only-learned", action="store_true", help="time only the denoiser and cache kernels")
     ap.add_argument("--only-wavefront", action="store_true",
                     help="run only the 4M-triangle wavefront scene and print its fields (for profiler runs)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="incoherent rays timed on the host")
@@ -221,6 +292,9 @@ def main():
         print(json.dumps({"copy_bytes": n, "launches": 3}), flush=True)
         return
 
+    if args.only_learned:
+        print(json.dumps(learned_kernels(args)), flush=True)
+        return
     if args.only_study:
         print(json.dumps(equal_time_study(args)), flush=True)
         return
@@ -294,6 +368,7 @@ def main():
 
     wave = None if (args.only_incoherent or args.no_wavefront) else wavefront_extras(args)
     study = None if (args.only_incoherent or args.no_study) else equal_time_study(args)
+    learned = None if args.only_incoherent else learned_kernels(args)
 
     cpu = None
     if not args.no_cpu_baseline and not args.only_incoherent:
@@ -336,6 +411,7 @@ def main():
                     "device_bytes": prep["device_bytes"], "host_build_s": round(build_s, 3)},
             "wavefront_4M": wave,
             "equal_time": study,
+            "learned_kernels": learned,
         },
     }
     print(json.dumps(out), flush=True)

@@ -276,11 +276,13 @@ int mrt_renderer_set_camera(mrt_renderer r, const float view_colmajor[16], const
 /* set_num_bounces (default 10, sequential_pathtracer.hpp:134). */
 int mrt_renderer_set_num_bounces(mrt_renderer r, int n);
 /* Attaches a radiance cache (no reference counterpart). mode 0: off; 1: train only (every pass
- * builds a training batch from a `train_fraction` of paths, which stay unbiased, and takes
- * `train_steps` Adam steps); 2: train and end the other paths in the cache at their secondary
- * vertex (roughness >= 0.25, or any later vertex; not on dielectrics). The accumulated image is
- * kept; call mrt_renderer_invalidate to restart it. */
-int mrt_renderer_set_cache(mrt_renderer r, mrt_cache c, int mode, float train_fraction, int train_steps);
+ * builds a training batch of up to 2^18 path-vertex samples from a `train_fraction` of paths,
+ * which stay unbiased, and takes `train_steps` Adam steps on disjoint slices of it); 2: train and
+ * end the other paths in the cache at vertex `min_bounce` (0 = primary hit) if its roughness is
+ * >= 0.25, or at any later vertex; never on dielectrics. The accumulated image is kept; call
+ * mrt_renderer_invalidate to restart it. */
+int mrt_renderer_set_cache(mrt_renderer r, mrt_cache c, int mode, float train_fraction, int train_steps,
+                           int min_bounce);
 /* invalidate_counter / sample_count. */
 int mrt_renderer_invalidate(mrt_renderer r);
 int mrt_renderer_sample_count(mrt_renderer r, uint32_t* out);
@@ -340,6 +342,7 @@ int mrt_denoiser_train_device(mrt_denoiser d, const float* d_raw, const float* d
                               float* loss, void* stream);
 /* Denoised rgba (H*W*4) of one raw feature image of any size. */
 int mrt_denoiser_denoise(mrt_denoiser d, const float* raw, int H, int W, float* rgba);
+int mrt_denoiser_denoise_device(mrt_denoiser d, const float* d_raw, int H, int W, float* d_rgba, void* stream);
 /* The renderer's per-pixel feature means (H*W*8: albedo rgb, normal xyz, hit distance, 0) of the
  * primary hits, and its accumulated image passed through a denoiser (H*W*4). */
 int mrt_renderer_read_aux(mrt_renderer r, float* out);

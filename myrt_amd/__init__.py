@@ -118,7 +118,7 @@ SIGNATURES = {
     "mrt_renderer_set_camera": ([_P, _P, _P], ctypes.c_int),
     "mrt_renderer_set_num_bounces": ([_P, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_invalidate": ([_P], ctypes.c_int),
-    "mrt_renderer_set_cache": ([_P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int], ctypes.c_int),
+    "mrt_renderer_set_cache": ([_P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_sample_count": ([_P, _U32P], ctypes.c_int),
     "mrt_render": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_read": ([_P, _P], ctypes.c_int),
@@ -148,6 +148,7 @@ SIGNATURES = {
     "mrt_denoiser_train_device": ([_P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_float), _P], ctypes.c_int),
     "mrt_denoiser_denoise": ([_P, _P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
+    "mrt_denoiser_denoise_device": ([_P, _P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "mrt_renderer_read_aux": ([_P, _P], ctypes.c_int),
     "mrt_renderer_denoise": ([_P, _P, _P], ctypes.c_int),
 }
@@ -545,9 +546,9 @@ class Renderer:
     def invalidate(self):
         _check(lib().mrt_renderer_invalidate(self.h))
 
-    def set_cache(self, cache, mode=2, train_fraction=1.0 / 16, train_steps=4):
+    def set_cache(self, cache, mode=2, train_fraction=1.0 / 16, train_steps=16, min_bounce=2):
         _check(lib().mrt_renderer_set_cache(self.h, cache.h if cache is not None else None, int(mode),
-                                            float(train_fraction), int(train_steps)))
+                                            float(train_fraction), int(train_steps), int(min_bounce)))
 
     def sample_count(self):
         out = ctypes.c_uint32()

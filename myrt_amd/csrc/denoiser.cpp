@@ -521,4 +521,11 @@ int mrt_denoiser_denoise(mrt_denoiser dh, const float* raw, int H, int W, float*
   return e == hipSuccess ? MRT_OK : hip_fail(e, "reading the denoised image");
 }
 
+int mrt_denoiser_denoise_device(mrt_denoiser dh, const float* d_raw, int H, int W, float* d_rgba, void* stream) {
+  Denoiser* d = reinterpret_cast<Denoiser*>(dh);
+  if (!d || !d_raw || !d_rgba) return fail(MRT_ERR_INVALID, "null argument");
+  if (H < 1 || W < 1 || size_t(H) * W > (size_t(1) << 26)) return fail(MRT_ERR_INVALID, "bad image size");
+  return d->denoise(d_raw, H, W, d_rgba, static_cast<hipStream_t>(stream));
+}
+
 }  // extern "C"

@@ -366,8 +366,9 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
           tv.prefix[2] = tp.T[2];
           tp.nv = tp.nv + 1;
         }
-      } else if (b.cache.terminate && bounce >= 1 &&
-                 (mat.roughness >= b.cache.min_roughness || bounce >= 2) && mat.transmission < 0.5f) {
+      } else if (b.cache.terminate && bounce >= b.cache.min_bounce &&
+                 (mat.roughness >= b.cache.min_roughness || bounce > b.cache.min_bounce) &&
+                 mat.transmission < 0.5f) {
         cached = true;
       }
       if (cached) {

@@ -84,7 +84,8 @@ struct PtCache {
   uint32_t max_vertices;
   uint32_t max_queries;
   uint32_t max_batch;
-  float min_roughness;      // a secondary vertex ends in the cache if roughness >= this; later ones always
+  float min_roughness;      // vertex min_bounce ends in the cache if roughness >= this; later ones always
+  uint32_t min_bounce;      // first vertex index (0 = primary hit) that may end in the cache
   TrainPath* tpaths;
   TrainVertex* tverts;
   CacheQuery* queries;

@@ -113,9 +113,10 @@ struct Renderer {
   Cache* cache = nullptr;
   int cache_mode = 0;  // 0 off, 1 train only, 2 train and end paths in the cache
   float train_fraction = 1.0f / 16.0f;
-  int train_steps = 4;
-  uint32_t max_batch = 1u << 16;
+  int train_steps = 16;
+  uint32_t max_batch = 1u << 18;
   float min_roughness = 0.25f;
+  uint32_t min_bounce = 2;
   TrainPath* tpaths = nullptr;
   TrainVertex* tverts = nullptr;
   CacheQuery* queries = nullptr;
@@ -205,6 +206,7 @@ struct Renderer {
       pc.max_queries = uint32_t(cap);
       pc.max_batch = max_batch;
       pc.min_roughness = min_roughness;
+      pc.min_bounce = min_bounce;
       pc.tpaths = tpaths;
       pc.tverts = tverts;
       pc.queries = queries;
@@ -469,11 +471,13 @@ int mrt_renderer_set_num_bounces(mrt_renderer rh, int n) {
   return MRT_OK;
 }
 
-int mrt_renderer_set_cache(mrt_renderer rh, mrt_cache c, int mode, float train_fraction, int train_steps) {
+int mrt_renderer_set_cache(mrt_renderer rh, mrt_cache c, int mode, float train_fraction, int train_steps,
+                           int min_bounce) {
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || mode < 0 || mode > 2 || (mode > 0 && !c) || !(train_fraction > 0.f && train_fraction <= 1.f) ||
-      train_steps < 1 || train_steps > 64)
+      train_steps < 1 || train_steps > 64 || min_bounce < 1 || min_bounce > 16)
     return fail(MRT_ERR_INVALID, "bad cache settings");
+  r->min_bounce = uint32_t(min_bounce);
   r->cache = reinterpret_cast<Cache*>(c);
   r->cache_mode = mode;
   r->train_fraction = train_fraction;

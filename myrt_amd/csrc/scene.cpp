@@ -180,7 +180,7 @@ int Scene::build_host() {
         boxes.reserve(current.size());
         for (const Instance& in : current) boxes.push_back(in.world_box);
         tlas = build_binary_bvh(boxes, false);
-        WideBvh w = collapse_to_wide(tlas, kTlasPadRel);
+        WideBvh w = collapse_to_wide(tlas, kTlasPadRel, tlas_leaf);
         tlas_nodes = std::move(w.nodes);
         tlas_depth = w.depth;
         for (uint32_t item : w.items) tlas_items.push_back(tlas.indices[item]);
@@ -571,6 +571,14 @@ int mrt_scene_create(mrt_scene* out) {
     const long d = std::strtol(v, nullptr, 10);
     if (d >= 4 && d <= 64 && (d & (d - 1)) == 0) s->lds_depth = uint32_t(d);
   }
+  if (const char* v = std::getenv("MRT_BVH_LEAF")) {
+    const long l = std::strtol(v, nullptr, 10);
+    if (l >= 1 && l <= 127) s->blas_leaf = uint32_t(l);
+  }
+  if (const char* v = std::getenv("MRT_TLAS_LEAF")) {
+    const long l = std::strtol(v, nullptr, 10);
+    if (l >= 1 && l <= 127) s->tlas_leaf = uint32_t(l);
+  }
   if (const char* v = std::getenv("MRT_TRACE_REFILL")) {
     const long r = std::strtol(v, nullptr, 10);
     if (r >= 1 && r <= 64) s->refill_below = uint32_t(r);
@@ -601,7 +609,7 @@ int mrt_scene_push_geometry(mrt_scene sh, const uint32_t* idx, size_t n_idx, con
   g.bvh = build_binary_bvh(triangle_bounds(idx, n_idx, pos3), true);
   g.box.mn = {g.bvh.nodes[0].mn[0], g.bvh.nodes[0].mn[1], g.bvh.nodes[0].mn[2]};
   g.box.mx = {g.bvh.nodes[0].mx[0], g.bvh.nodes[0].mx[1], g.bvh.nodes[0].mx[2]};
-  g.wide = collapse_to_wide(g.bvh, 1.0 / (1 << 20));
+  g.wide = collapse_to_wide(g.bvh, 1.0 / (1 << 20), s->blas_leaf);
   s->geoms.push_back(std::move(g));
   s->geoms_dirty = true;
   if (out_geom) *out_geom = uint32_t(s->geoms.size() - 1);

@@ -101,6 +101,8 @@ struct Scene {
   size_t spill_cap = 0;
   int num_cus = 0;
   uint32_t refill_below = 40;
+  uint32_t blas_leaf = 127;  // largest wide leaf (triangles); MRT_BVH_LEAF overrides
+  uint32_t tlas_leaf = 127;  // largest TLAS leaf (instances); MRT_TLAS_LEAF overrides
   uint32_t lds_depth = 16;  // LDS stack entries per lane; MRT_TRACE_LDS_STACK overrides (a power of two, 4-64)
 
   ~Scene();

@@ -2,6 +2,7 @@
 // child boxes that always decode to a box strictly containing the child.
 #include "wide_bvh.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -9,54 +10,69 @@ namespace mrt {
 namespace {
 
 struct Child {
-  bool from_bin = false;  // a binary node; otherwise a range of reference entries
+  bool from_bin = false;          // a binary inner node; otherwise a list of reference entries
   uint32_t bin = 0;
-  uint32_t start = 0, count = 0;
+  std::vector<uint32_t> items;    // positions in BinaryBvh::indices (leaf lists)
   Box box;
-  bool internal = false;  // needs a wide node of its own
+  bool internal = false;          // needs a wide node of its own
 };
 
-Child bin_child(const BinaryBvh& b, uint32_t idx) {
+Child list_child(const BinaryBvh& b, std::vector<uint32_t> items, uint32_t max_leaf) {
+  Child c;
+  c.items = std::move(items);
+  for (uint32_t i : c.items) c.box.enclose(b.ref_boxes[i]);
+  c.internal = c.items.size() > max_leaf;
+  return c;
+}
+
+Child bin_child(const BinaryBvh& b, uint32_t idx, uint32_t max_leaf) {
   const BinNode& n = b.nodes[idx];
+  if (n.is_leaf()) {
+    std::vector<uint32_t> items;
+    for (uint32_t i = n.first; i <= n.second; ++i) items.push_back(i);
+    return list_child(b, std::move(items), max_leaf);
+  }
   Child c;
   c.from_bin = true;
   c.bin = idx;
   c.box.mn = {n.mn[0], n.mn[1], n.mn[2]};
   c.box.mx = {n.mx[0], n.mx[1], n.mx[2]};
-  if (n.is_leaf()) {
-    c.start = n.first;
-    c.count = n.second - n.first + 1;
-    c.internal = c.count > kMaxLeafItems;
-  } else {
-    c.internal = true;
+  c.internal = true;
+  return c;
+}
+
+// Splits a leaf list at the median centroid of its widest centroid axis (index order when all
+// centroids coincide).
+void split_list(const BinaryBvh& b, const Child& c, Child out[2], uint32_t max_leaf) {
+  std::vector<uint32_t> items = c.items;
+  Box cb;
+  for (uint32_t i : items) cb.enclose(b.ref_boxes[i].center());
+  const vec3f d = cb.dim();
+  const int axis = (d.x >= d.y && d.x >= d.z) ? 0 : (d.y >= d.z ? 1 : 2);
+  const size_t half = items.size() / 2;
+  if (d[axis] > 0.f) {
+    std::nth_element(items.begin(), items.begin() + half, items.end(), [&](uint32_t a, uint32_t bb) {
+      const float ca = b.ref_boxes[a].center()[axis], cbv = b.ref_boxes[bb].center()[axis];
+      return ca < cbv || (ca == cbv && a < bb);
+    });
   }
-  return c;
+  out[0] = list_child(b, std::vector<uint32_t>(items.begin(), items.begin() + half), max_leaf);
+  out[1] = list_child(b, std::vector<uint32_t>(items.begin() + half, items.end()), max_leaf);
 }
 
-Child range_child(const BinaryBvh& b, uint32_t start, uint32_t count) {
-  Child c;
-  c.start = start;
-  c.count = count;
-  for (uint32_t i = start; i < start + count; ++i) c.box.enclose(b.ref_boxes[i]);
-  c.internal = count > kMaxLeafItems;
-  return c;
-}
-
-void expand(const BinaryBvh& b, const Child& c, Child out[2]) {
-  if (c.from_bin && !b.nodes[c.bin].is_leaf()) {
-    out[0] = bin_child(b, b.nodes[c.bin].first);
-    out[1] = bin_child(b, b.nodes[c.bin].second);
-  } else {  // an over-full leaf: halve its range
-    const uint32_t half = c.count / 2;
-    out[0] = range_child(b, c.start, half);
-    out[1] = range_child(b, c.start + half, c.count - half);
+void expand(const BinaryBvh& b, const Child& c, Child out[2], uint32_t max_leaf) {
+  if (c.from_bin) {
+    out[0] = bin_child(b, b.nodes[c.bin].first, max_leaf);
+    out[1] = bin_child(b, b.nodes[c.bin].second, max_leaf);
+  } else {
+    split_list(b, c, out, max_leaf);
   }
 }
 
 // Opens the largest-area inner child until the node has four children or only leaves remain.
-std::vector<Child> gather_children(const BinaryBvh& b, const Child& parent) {
+std::vector<Child> gather_children(const BinaryBvh& b, const Child& parent, uint32_t max_leaf) {
   std::vector<Child> ch(2);
-  expand(b, parent, ch.data());
+  expand(b, parent, ch.data(), max_leaf);
   while (ch.size() < 4) {
     int best = -1;
     float best_area = 0.f;
@@ -70,7 +86,7 @@ std::vector<Child> gather_children(const BinaryBvh& b, const Child& parent) {
     }
     if (best < 0) break;
     Child e[2];
-    expand(b, ch[best], e);
+    expand(b, ch[best], e, max_leaf);
     ch[best] = e[0];
     ch.insert(ch.begin() + best + 1, e[1]);
   }
@@ -165,7 +181,9 @@ void decode_child_box(const WideNode& n, int c, float lo[3], float hi[3]) {
   }
 }
 
-WideBvh collapse_to_wide(const BinaryBvh& b, double pad_rel) {
+WideBvh collapse_to_wide(const BinaryBvh& b, double pad_rel, uint32_t max_leaf) {
+  if (max_leaf < 1) max_leaf = 1;
+  if (max_leaf > kMaxLeafItems) max_leaf = kMaxLeafItems;
   WideBvh w;
   if (b.nodes.empty()) return w;
   struct Pending {
@@ -174,12 +192,12 @@ WideBvh collapse_to_wide(const BinaryBvh& b, double pad_rel) {
     uint32_t depth;
   };
   w.nodes.emplace_back();
-  std::vector<Pending> stack{{0u, bin_child(b, 0), 1u}};
+  std::vector<Pending> stack{{0u, bin_child(b, 0, max_leaf), 1u}};
   while (!stack.empty()) {
     Pending p = stack.back();
     stack.pop_back();
     w.depth = std::max(w.depth, p.depth);
-    const std::vector<Child> ch = p.c.internal ? gather_children(b, p.c) : std::vector<Child>{p.c};
+    const std::vector<Child> ch = p.c.internal ? gather_children(b, p.c, max_leaf) : std::vector<Child>{p.c};
     uint32_t n_inner = 0;
     for (const Child& c : ch) n_inner += c.internal ? 1 : 0;
     const uint32_t node_base = uint32_t(w.nodes.size());
@@ -195,9 +213,9 @@ WideBvh collapse_to_wide(const BinaryBvh& b, double pad_rel) {
         n.count[k] = 0;
         n.offset[k] = uint16_t(inner++);
       } else {
-        n.count[k] = uint8_t(ch[k].count);
+        n.count[k] = uint8_t(ch[k].items.size());
         n.offset[k] = uint16_t(w.items.size() - n.item_base);
-        for (uint32_t i = 0; i < ch[k].count; ++i) w.items.push_back(ch[k].start + i);
+        w.items.insert(w.items.end(), ch[k].items.begin(), ch[k].items.end());
       }
     }
     quantize(n, ch, pad_rel);

@@ -14,8 +14,10 @@ struct WideBvh {
 };
 
 // pad_rel widens each child box by pad_rel * (largest |coordinate| of the box) on every side
-// before quantisation; decoded boxes additionally end strictly outside the padded box.
-WideBvh collapse_to_wide(const BinaryBvh& b, double pad_rel);
+// before quantisation; decoded boxes additionally end strictly outside the padded box. Binary
+// leaves with more than max_leaf items are split further at centroid medians, so wide leaves hold
+// at most max_leaf items (the binary BVH itself is left as built).
+WideBvh collapse_to_wide(const BinaryBvh& b, double pad_rel, uint32_t max_leaf = 127);
 
 // Decoded child box, exactly as the kernels compute it (fmaf(code, 2^e, origin)).
 void decode_child_box(const WideNode& n, int c, float lo[3], float hi[3]);

@@ -1,7 +1,7 @@
 """Trains the denoiser on (noisy, converged) image pairs rendered by this path tracer.
 
 Each seeded training scene (myrt_amd.scenes.training_scene) is rendered at RES x RES, depth 8:
-noisy inputs at 1, 2, 4, 8 and 16 samples per pixel (radiance plus primary-hit albedo, normal and
+noisy inputs at 1, 2, 4, 8, 16, 32 and 64 samples per pixel (radiance plus primary-hit albedo, normal and
 distance), and a converged target at TARGET_SPP. The network is trained with Adam on random
 128 x 128 crops (flips and transposes as augmentation) by the library's own fp32 forward and
 backward kernels, and the weights are saved as a float32 .npy of the packed parameters.
@@ -21,7 +21,7 @@ import numpy as np
 import myrt_amd as m
 from myrt_amd import scenes
 
-INPUT_SPP = (1, 2, 4, 8, 16)
+INPUT_SPP = (1, 2, 4, 8, 16, 32, 64)
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "data", "denoiser_unet.npy")
 
