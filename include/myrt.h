@@ -275,6 +275,17 @@ int mrt_renderer_destroy(mrt_renderer r);
 int mrt_renderer_set_camera(mrt_renderer r, const float view_colmajor[16], const float proj_colmajor[16]);
 /* set_num_bounces (default 10, sequential_pathtracer.hpp:134). */
 int mrt_renderer_set_num_bounces(mrt_renderer r, int n);
+/* set_dof_enabled / set_lens_size / set_focus / set_bokeh_mask (sequential_pathtracer.hpp:30-33;
+ * defaults off, 10 x 10, 10). A thin lens around each pinhole ray: lens offset o uniform in the
+ * disk of radius 0.49, focal point at `focus` along the pinhole ray, new direction
+ * normalize(o * lens / 2, focus) in a frame around the pinhole direction (pt_generate.comp:38-52).
+ * The mask (rgba floats, w x h, row 0 at v = 0; w < 10 means none, bokeh.glsl:24-25) is sampled
+ * bilinearly at 0.5 + o; black offsets are redrawn up to 8 times and the mask colour tints the
+ * path. Each setter restarts accumulation when the value changes. */
+int mrt_renderer_set_dof_enabled(mrt_renderer r, int enabled);
+int mrt_renderer_set_lens_size(mrt_renderer r, float w, float h);
+int mrt_renderer_set_focus(mrt_renderer r, float focus);
+int mrt_renderer_set_bokeh_mask(mrt_renderer r, const float* rgba, int w, int h);
 /* Attaches a radiance cache (no reference counterpart). mode 0: off; 1: train only (every pass
  * builds a training batch of up to 2^18 path-vertex samples from a `train_fraction` of paths,
  * which stay unbiased, and takes `train_steps` Adam steps on disjoint slices of it); 2: train and

@@ -118,6 +118,10 @@ SIGNATURES = {
     "mrt_renderer_set_camera": ([_P, _P, _P], ctypes.c_int),
     "mrt_renderer_set_num_bounces": ([_P, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_invalidate": ([_P], ctypes.c_int),
+    "mrt_renderer_set_dof_enabled": ([_P, ctypes.c_int], ctypes.c_int),
+    "mrt_renderer_set_lens_size": ([_P, ctypes.c_float, ctypes.c_float], ctypes.c_int),
+    "mrt_renderer_set_focus": ([_P, ctypes.c_float], ctypes.c_int),
+    "mrt_renderer_set_bokeh_mask": ([_P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_set_cache": ([_P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_sample_count": ([_P, _U32P], ctypes.c_int),
     "mrt_render": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -545,6 +549,20 @@ class Renderer:
 
     def invalidate(self):
         _check(lib().mrt_renderer_invalidate(self.h))
+
+    def set_dof(self, enabled=True, lens_size=(10.0, 10.0), focus=10.0, bokeh_mask=None):
+        """set_dof_enabled / set_lens_size / set_focus / set_bokeh_mask; mask: (h, w, 3 or 4) floats."""
+        _check(lib().mrt_renderer_set_dof_enabled(self.h, int(bool(enabled))))
+        _check(lib().mrt_renderer_set_lens_size(self.h, float(lens_size[0]), float(lens_size[1])))
+        _check(lib().mrt_renderer_set_focus(self.h, float(focus)))
+        if bokeh_mask is None:
+            _check(lib().mrt_renderer_set_bokeh_mask(self.h, None, 0, 0))
+        else:
+            mk = np.asarray(bokeh_mask, np.float32)
+            rgba = np.ones(mk.shape[:2] + (4,), np.float32)
+            rgba[..., :mk.shape[2]] = mk
+            rgba = np.ascontiguousarray(rgba)
+            _check(lib().mrt_renderer_set_bokeh_mask(self.h, _ptr(rgba), rgba.shape[1], rgba.shape[0]))
 
     def set_cache(self, cache, mode=2, train_fraction=1.0 / 16, train_steps=16, min_bounce=2):
         _check(lib().mrt_renderer_set_cache(self.h, cache.h if cache is not None else None, int(mode),

@@ -60,6 +60,7 @@ __device__ inline float rnd(uint32_t pixel, uint32_t sample, uint32_t dim) {
 // 4-5 light point, 6 light choice}.
 __device__ inline uint32_t dim_of(uint32_t bounce, uint32_t k) { return 2u + 8u * bounce + k; }
 constexpr uint32_t kTrainDim = 0x40000000u;  // dimension that selects training paths
+constexpr uint32_t kLensDim = 0x30000000u;   // lens samples: kLensDim + 2 * try + {0, 1}
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
@@ -152,6 +153,21 @@ __device__ inline void onb(V3 n, V3& t, V3& b) {
   b = mk(c, s + n.y * n.y * a, -n.y);
 }
 
+// Bilinear lookup of the bokeh mask at (u, v) in [0, 1]^2, texel centres at (i + 0.5) / size,
+// coordinates clamped to the edge texels.
+__device__ inline V3 bokeh_sample(const Camera& cam, float u, float v) {
+  const float fx = fminf(fmaxf(u * float(cam.bokeh_w) - 0.5f, 0.f), float(cam.bokeh_w - 1));
+  const float fy = fminf(fmaxf(v * float(cam.bokeh_h) - 0.5f, 0.f), float(cam.bokeh_h - 1));
+  const uint32_t x0 = uint32_t(fx), y0 = uint32_t(fy);
+  const uint32_t x1 = min(x0 + 1, cam.bokeh_w - 1), y1 = min(y0 + 1, cam.bokeh_h - 1);
+  const float tx = fx - float(x0), ty = fy - float(y0);
+  const float4 a = cam.bokeh[y0 * cam.bokeh_w + x0], b = cam.bokeh[y0 * cam.bokeh_w + x1];
+  const float4 c = cam.bokeh[y1 * cam.bokeh_w + x0], e = cam.bokeh[y1 * cam.bokeh_w + x1];
+  const float w00 = (1.f - tx) * (1.f - ty), w10 = tx * (1.f - ty), w01 = (1.f - tx) * ty, w11 = tx * ty;
+  return mk(a.x * w00 + b.x * w10 + c.x * w01 + e.x * w11, a.y * w00 + b.y * w10 + c.y * w01 + e.y * w11,
+            a.z * w00 + b.z * w10 + c.z * w01 + e.z * w11);
+}
+
 __global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers b, uint32_t cur, uint32_t n_paths,
                                                           uint32_t sample_base) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -171,15 +187,47 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers 
   const float qz = m[2] * nx + m[6] * ny + m[10] + m[14];
   const float qw = m[3] * nx + m[7] * ny + m[11] + m[15];
   V3 target = qw != 0.f ? mk(qx / qw, qy / qw, qz / qw) : mk(qx, qy, qz);
-  const V3 eye = ld3(cam.eye);
-  const V3 d = qw != 0.f ? normalize(target - eye) : normalize(target);
+  V3 eye = ld3(cam.eye);
+  V3 d = qw != 0.f ? normalize(target - eye) : normalize(target);
+  V3 tint = mk(1.0f, 1.0f, 1.0f);
+  if (cam.dof) {
+    // A lens offset uniform in the disk of radius 0.49; with a mask, offsets where the mask is
+    // black are redrawn (up to 8 tries, then the sample carries no weight) and the mask colour
+    // tints the path. The focal point at `focus` along the pinhole ray stays fixed.
+    float ox = 0.f, oy = 0.f;
+    bool ok = false;
+    for (uint32_t k = 0; k < 8 && !ok; ++k) {
+      const float a = rnd(pixel, sample, kLensDim + 2 * k) * 6.2831853f;
+      const float rr = sqrtf(rnd(pixel, sample, kLensDim + 2 * k + 1)) * 0.49f;
+      ox = rr * sinf(a);
+      oy = rr * cosf(a);
+      if (!cam.bokeh) {
+        ok = true;
+      } else {
+        tint = bokeh_sample(cam, 0.5f + ox, 0.5f + oy);
+        ok = fmaxf(fmaxf(tint.x, tint.y), tint.z) >= 1e-5f;
+      }
+    }
+    if (!ok) tint = mk(0.f, 0.f, 0.f);
+    const V3 center = eye + d * cam.focus;
+    const V3 local = normalize(mk(ox * cam.lens[0] * 0.5f, oy * cam.lens[1] * 0.5f, cam.focus));
+    V3 u = cross(d, mk(1.f, 0.f, 0.f));
+    if (dot(u, u) == 0.f) u = cross(d, mk(0.f, 1.f, 0.f));
+    u = normalize(u);
+    const V3 v = cross(d, u);
+    const V3 nd = normalize(u * local.x + v * local.y + d * local.z);
+    eye = center - nd * cam.focus;
+    d = nd;
+  }
   if (live) {
     float4* r = reinterpret_cast<float4*>(b.rays[cur]) + 2 * size_t(i);
     r[0] = make_float4(eye.x, eye.y, eye.z, 0.0f);
     r[1] = make_float4(d.x, d.y, d.z, __int_as_float(0x7f800000));
   }
   PathState ps;
-  ps.thr[0] = ps.thr[1] = ps.thr[2] = 1.0f;
+  ps.thr[0] = tint.x;
+  ps.thr[1] = tint.y;
+  ps.thr[2] = tint.z;
   ps.pixel = pixel;
   ps.sample = sample;
   ps.flags = kDelta;

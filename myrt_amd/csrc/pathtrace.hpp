@@ -99,6 +99,13 @@ struct Camera {
   float inv_view_proj[16];  // column-major inverse(projection * view)... stored as V^-1 * P^-1
   float eye[3];
   uint32_t width, height;
+  // Depth of field (pt_generate.comp:38-52, bokeh.glsl:15-34): lens size, focus distance and an
+  // optional bokeh mask (rgba, bilinear, clamped) sampled at 0.5 + lens offset.
+  uint32_t dof;
+  float lens[2];
+  float focus;
+  const float4* bokeh;
+  uint32_t bokeh_w, bokeh_h;
 };
 
 struct PtScene {

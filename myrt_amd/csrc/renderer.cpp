@@ -110,6 +110,12 @@ struct Renderer {
   hipStream_t stream = nullptr;
   mrt_render_stats stats{};
   // Radiance cache (optional).
+  // Depth of field (sequential_pathtracer.hpp:131-139 defaults).
+  bool dof = false;
+  float lens[2] = {10.f, 10.f};
+  float focus = 10.f;
+  float4* bokeh = nullptr;
+  uint32_t bokeh_w = 0, bokeh_h = 0;
   Cache* cache = nullptr;
   int cache_mode = 0;  // 0 off, 1 train only, 2 train and end paths in the cache
   float train_fraction = 1.0f / 16.0f;
@@ -135,7 +141,7 @@ struct Renderer {
   ~Renderer() {
     for (void* p : {rays[0], rays[1], (void*)paths[0], (void*)paths[1], hits, shadow_rays, (void*)shadow,
                     (void*)occluded, (void*)counts, (void*)accum, (void*)tpaths, (void*)tverts, (void*)queries,
-                    (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den})
+                    (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den, (void*)bokeh})
       if (p) hipFree(p);
   }
 
@@ -151,6 +157,13 @@ struct Renderer {
     cam.eye[0] = float(vinv[12]);
     cam.eye[1] = float(vinv[13]);
     cam.eye[2] = float(vinv[14]);
+    cam.dof = dof ? 1u : 0u;
+    cam.lens[0] = lens[0];
+    cam.lens[1] = lens[1];
+    cam.focus = focus;
+    cam.bokeh = bokeh;
+    cam.bokeh_w = bokeh_w;
+    cam.bokeh_h = bokeh_h;
     return MRT_OK;
   }
 
@@ -460,6 +473,48 @@ int mrt_renderer_set_camera(mrt_renderer rh, const float view_colmajor[16], cons
   std::memcpy(r->view, view_colmajor, sizeof(r->view));
   std::memcpy(r->proj, proj_colmajor, sizeof(r->proj));
   r->sample_count = 0;  // every setter invalidates the counter (sequential_pathtracer.cpp:70-144)
+  return MRT_OK;
+}
+
+int mrt_renderer_set_dof_enabled(mrt_renderer rh, int enabled) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r) return fail(MRT_ERR_INVALID, "null renderer");
+  if (r->dof != (enabled != 0)) r->sample_count = 0;
+  r->dof = enabled != 0;
+  return MRT_OK;
+}
+
+int mrt_renderer_set_lens_size(mrt_renderer rh, float w, float h) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || !std::isfinite(w) || !std::isfinite(h)) return fail(MRT_ERR_INVALID, "bad lens size");
+  if (w != r->lens[0] || h != r->lens[1]) r->sample_count = 0;
+  r->lens[0] = w;
+  r->lens[1] = h;
+  return MRT_OK;
+}
+
+int mrt_renderer_set_focus(mrt_renderer rh, float focus) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || !(focus > 0.f) || !std::isfinite(focus)) return fail(MRT_ERR_INVALID, "focus must be positive");
+  if (focus != r->focus) r->sample_count = 0;
+  r->focus = focus;
+  return MRT_OK;
+}
+
+int mrt_renderer_set_bokeh_mask(mrt_renderer rh, const float* rgba, int w, int h) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || w < 0 || h < 0 || (w > 0 && (!rgba || h == 0))) return fail(MRT_ERR_INVALID, "bad bokeh mask");
+  if (r->bokeh) hipFree(r->bokeh);
+  r->bokeh = nullptr;
+  r->bokeh_w = r->bokeh_h = 0;
+  r->sample_count = 0;
+  // Like bokeh_get (bokeh.glsl:24-25), masks narrower than 10 texels mean "no mask".
+  if (w < 10) return MRT_OK;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->bokeh), size_t(w) * h * sizeof(float4));
+  if (e == hipSuccess) e = hipMemcpy(r->bokeh, rgba, size_t(w) * h * sizeof(float4), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "uploading the bokeh mask");
+  r->bokeh_w = uint32_t(w);
+  r->bokeh_h = uint32_t(h);
   return MRT_OK;
 }
 

@@ -29,7 +29,9 @@ class _PtScene(ctypes.Structure):
                 ("inst_material", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32),
                 ("lights", ctypes.c_void_p), ("n_lights", ctypes.c_uint32), ("env_scale", ctypes.c_double),
                 ("sdf_tape", ctypes.c_void_p), ("sdf_drawables", ctypes.c_void_p), ("n_sdf", ctypes.c_uint32),
-                ("sdf_steps", ctypes.c_int), ("sdf_eps", ctypes.c_float)]
+                ("sdf_steps", ctypes.c_int), ("sdf_eps", ctypes.c_float), ("dof", ctypes.c_int),
+                ("lens", ctypes.c_float * 2), ("focus", ctypes.c_float), ("bokeh", ctypes.c_void_p),
+                ("bokeh_w", ctypes.c_int), ("bokeh_h", ctypes.c_int)]
 
 
 MATERIAL_DTYPE = np.dtype([("albedo_rgba8", "<u4"), ("ior", "<f4"), ("roughness", "<f4"), ("metallic", "<f4"),
@@ -267,6 +269,20 @@ class OracleRender:
                           self._lights_buf.ctypes.data, self.lights.size, float(env_scale),
                           None if not n_sdf else self.sdf_tape.ctypes.data,
                           None if not n_sdf else self.sdf_dr.ctypes.data, n_sdf, int(sdf_steps), float(sdf_eps))
+
+    def set_dof(self, enabled=True, lens_size=(10.0, 10.0), focus=10.0, bokeh_mask=None):
+        """The product's thin-lens camera; mask (h, w, 3 or 4) floats, narrower than 10 = none."""
+        self.c.dof = int(bool(enabled))
+        self.c.lens[0], self.c.lens[1] = float(lens_size[0]), float(lens_size[1])
+        self.c.focus = float(focus)
+        self._bokeh = None
+        self.c.bokeh, self.c.bokeh_w, self.c.bokeh_h = None, 0, 0
+        if bokeh_mask is not None and np.asarray(bokeh_mask).shape[1] >= 10:
+            mk = np.asarray(bokeh_mask, np.float32)
+            rgba = np.ones(mk.shape[:2] + (4,), np.float32)
+            rgba[..., :mk.shape[2]] = mk
+            self._bokeh = np.ascontiguousarray(rgba)
+            self.c.bokeh, self.c.bokeh_w, self.c.bokeh_h = self._bokeh.ctypes.data, rgba.shape[1], rgba.shape[0]
 
     def render(self, view, proj, width, height, spp, max_depth, sample_base=0, threads=0):
         """Mean radiance per pixel, shape (height, width, 3), float64 (rows from the bottom)."""

@@ -132,6 +132,12 @@ typedef struct orc_pt_scene {
   uint32_t n_sdf;
   int sdf_steps;
   float sdf_eps;
+  /* thin-lens camera (the product's generate pass): disabled when dof == 0 */
+  int dof;
+  float lens[2];
+  float focus;
+  const float* bokeh;            /* rgba floats, bokeh_w x bokeh_h, or NULL */
+  int bokeh_w, bokeh_h;
 } orc_pt_scene;
 
 /* The product's wavefront estimator in binary64: camera (inverse(proj * view), with the

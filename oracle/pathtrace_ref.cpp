@@ -113,6 +113,25 @@ bool mat_inv4(const double m[16], double out[16]) {
   return true;
 }
 
+constexpr uint32_t kLensDim = 0x30000000u;
+
+// Bilinear bokeh mask lookup, texel centres at (i + 0.5) / size, clamped to the edge texels.
+D3 bokeh_sample(const orc_pt_scene* S, double u, double v) {
+  const int w = S->bokeh_w, h = S->bokeh_h;
+  const double fx = std::min(std::max(u * w - 0.5, 0.0), double(w - 1));
+  const double fy = std::min(std::max(v * h - 0.5, 0.0), double(h - 1));
+  const int x0 = int(fx), y0 = int(fy), x1 = std::min(x0 + 1, w - 1), y1 = std::min(y0 + 1, h - 1);
+  const double tx = fx - x0, ty = fy - y0;
+  D3 out{0, 0, 0};
+  const int xs[4] = {x0, x1, x0, x1}, ys[4] = {y0, y0, y1, y1};
+  const double ws[4] = {(1 - tx) * (1 - ty), tx * (1 - ty), (1 - tx) * ty, tx * ty};
+  for (int k = 0; k < 4; ++k) {
+    const float* t = S->bokeh + 4 * (size_t(ys[k]) * w + xs[k]);
+    out = out + D3{t[0], t[1], t[2]} * ws[k];
+  }
+  return out;
+}
+
 struct Ctx {
   const orc_pt_scene* s;
   orc_accel a;
@@ -135,6 +154,36 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
   D3 ro = C.eye;
   D3 rd = qw != 0 ? normalize(D3{qx / qw, qy / qw, qz / qw} - C.eye) : normalize(D3{qx, qy, qz});
   D3 thr{1, 1, 1}, sum{0, 0, 0};
+  if (S->dof) {
+    // Thin lens: offset uniform in the disk of radius 0.49 (redrawn where the mask is black, up
+    // to 8 times), focal point fixed at `focus` along the pinhole ray, mask colour as the tint.
+    double ox = 0, oy = 0;
+    bool ok = false;
+    D3 tint{1, 1, 1};
+    for (uint32_t k = 0; k < 8 && !ok; ++k) {
+      const double a = rnd(pixel, sample, kLensDim + 2 * k) * 6.2831853;
+      const double rr = std::sqrt(rnd(pixel, sample, kLensDim + 2 * k + 1)) * 0.49;
+      ox = rr * std::sin(a);
+      oy = rr * std::cos(a);
+      if (!S->bokeh) {
+        ok = true;
+      } else {
+        tint = bokeh_sample(S, 0.5 + ox, 0.5 + oy);
+        ok = std::max(std::max(tint.x, tint.y), tint.z) >= 1e-5;
+      }
+    }
+    if (!ok) tint = D3{0, 0, 0};
+    const D3 center = ro + rd * double(S->focus);
+    const D3 local = normalize(D3{ox * S->lens[0] * 0.5, oy * S->lens[1] * 0.5, double(S->focus)});
+    D3 u = cross(rd, D3{1, 0, 0});
+    if (dot(u, u) == 0.0) u = cross(rd, D3{0, 1, 0});
+    u = normalize(u);
+    const D3 v = cross(rd, u);
+    const D3 nd = normalize(u * local.x + v * local.y + rd * local.z);
+    ro = center - nd * double(S->focus);
+    rd = nd;
+    thr = tint;
+  }
   bool delta = true;
   double last_pdf = 0;
   for (int bounce = 0; bounce < C.max_depth; ++bounce) {

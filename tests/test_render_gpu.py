@@ -119,3 +119,38 @@ def test_render_with_sdf_matches_fp64_oracle(ensure_built):
     err = rel_mse(img, ref)
     say("sdf render relative MSE %.3e" % err)
     assert err <= REL_MSE_TOL
+
+
+def hexagon_mask(n=32):
+    """A coloured hexagonal aperture: inside rgb (1, 0.8, 0.6), outside black."""
+    ys, xs = np.mgrid[0:n, 0:n]
+    u, v = (xs + 0.5) / n - 0.5, (ys + 0.5) / n - 0.5
+    inside = np.ones((n, n), bool)
+    for k in range(6):
+        a = np.pi / 3 * k
+        inside &= u * np.cos(a) + v * np.sin(a) <= 0.4
+    mask = np.zeros((n, n, 3), np.float32)
+    mask[inside] = (1.0, 0.8, 0.6)
+    return mask
+
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_depth_of_field_matches_fp64_oracle(ensure_built, masked):
+    sc = scenes.sphere_field_scene(40, 2, seed=1, glass_every=6, metal_every=5)
+    W, H, spp, depth = 80, 60, 32, 4
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
+    mask = hexagon_mask() if masked else None
+    say("dof render %dx%d %d spp masked=%s" % (W, H, spp, masked))
+    s = gpu_scene(sc)
+    r = m.Renderer()
+    r.set_camera(view, proj)
+    r.set_num_bounces(depth)
+    r.set_dof(True, (1.2, 1.2), 12.0, bokeh_mask=mask)
+    r.render(s, W, H, spp)
+    img = r.image()[..., :3].astype(np.float64)
+    o = orc.OracleRender(sc["geometries"], sc["instances"], sc["materials"], sc["lights"], 1.0)
+    o.set_dof(True, (1.2, 1.2), 12.0, bokeh_mask=mask)
+    ref = o.render(view, proj, W, H, spp, depth)
+    err = rel_mse(img, ref)
+    say("dof relative MSE %.3e" % err)
+    assert err <= REL_MSE_TOL

@@ -74,3 +74,28 @@ def test_deterministic(ensure_built):
     assert np.array_equal(a, b)
     c = R.render(view, proj, 24, 16, 8, 4, sample_base=8)
     assert not np.array_equal(a, c)
+
+
+def small_dof_scene():
+    sc = scenes.sphere_field_scene(12, 1, seed=9)
+    r = orc.OracleRender(sc["geometries"], sc["instances"], sc["materials"], sc["lights"], 1.0)
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], 1.0)
+    return r, view, proj
+
+
+def test_dof_closed_forms(ensure_built):
+    """Thin lens of size 0 is the pinhole camera; an all-black mask yields black; a constant
+    mask c scales the lens image by c (the first lens sample is accepted, so paths coincide)."""
+    r, view, proj = small_dof_scene()
+    pin = r.render(view, proj, 12, 10, 4, 3)
+    r.set_dof(True, (0.0, 0.0), 9.0)
+    zero_lens = r.render(view, proj, 12, 10, 4, 3)
+    assert np.allclose(zero_lens, pin, rtol=1e-6, atol=1e-9)
+    r.set_dof(True, (1.5, 1.5), 9.0)
+    lens = r.render(view, proj, 12, 10, 4, 3)
+    assert not np.allclose(lens, pin, rtol=1e-3)
+    r.set_dof(True, (1.5, 1.5), 9.0, bokeh_mask=np.full((16, 16, 3), 0.5))
+    half = r.render(view, proj, 12, 10, 4, 3)
+    assert np.allclose(half, 0.5 * lens, rtol=1e-9, atol=1e-12)
+    r.set_dof(True, (1.5, 1.5), 9.0, bokeh_mask=np.zeros((16, 16, 3)))
+    assert not r.render(view, proj, 12, 10, 4, 3).any()
