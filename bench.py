@@ -181,9 +181,14 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
             img, t_dn = timed(lambda: plain.denoise(den))
             row["plain_denoised_mse"] = mse(img)
             row["denoise_s"] = round(t_dn, 4)
-        # The cache path gets as many samples as fit in the budget at its measured rate.
+        # The cache path gets as many samples as fit in the budget at its measured rate; the
+        # estimate is refined until its render plus the denoiser stays within the budget.
         spp_c = max(1, int((T - t_dn) / t16 * 16))
-        _, tc = timed(lambda: fresh(cr).render(s, W, H, spp_c))
+        for _ in range(4):
+            _, tc = timed(lambda: fresh(cr).render(s, W, H, spp_c))
+            if tc + t_dn <= T * 1.01 or spp_c == 1:
+                break
+            spp_c = max(1, min(spp_c - 1, int(spp_c * (T - t_dn) / tc)))
         row.update({"cache_spp": spp_c, "cache_s": round(tc, 4), "cache_mse": mse(cr.image())})
         if den is not None:
             img, t = timed(lambda: cr.denoise(den))

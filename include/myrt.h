@@ -232,6 +232,10 @@ int mrt_scene_build_host(mrt_scene s, mrt_prepare_result* out);
 int mrt_scene_copy_wide(mrt_scene s, void* blas_nodes, void* tris, void* tlas_nodes, uint32_t* tlas_items,
                         void* instances);
 
+/* Albedo texture (the bindless albedo_texture of the material layout, main.cpp:454-464): w x h
+ * rgba floats, row 0 at v = 0. Returns its id (>= 1) for mrt_material.texture; a textured hit
+ * takes albedo = texture(u, 1 - v)^2.2 with bilinear filtering and wrap (material_new.glsl:29-31). */
+int mrt_scene_push_texture(mrt_scene s, const float* rgba, int w, int h, uint32_t* out_id);
 /* scene::push_material / set_parameter(material, name, value) (src/pathtracer/scene.hpp:134-140,
  * 249-252; src/material.hpp:81-102). Parameter names: "albedo" (3 or 4 floats), "ior",
  * "roughness", "metallic", "transmission"; a wrong count is an error (parameter.cpp:40-41). The
@@ -286,6 +290,11 @@ int mrt_renderer_set_dof_enabled(mrt_renderer r, int enabled);
 int mrt_renderer_set_lens_size(mrt_renderer r, float w, float h);
 int mrt_renderer_set_focus(mrt_renderer r, float focus);
 int mrt_renderer_set_bokeh_mask(mrt_renderer r, const float* rgba, int w, int h);
+/* set_cubemap (sequential_pathtracer.hpp:34): environment for rays that leave the scene, six
+ * faces (+x, -x, +y, -y, +z, -z; OpenGL face orientation) of size x size rgba floats, bilinear
+ * within a face and scaled by mrt_scene_set_environment; size <= 10 restores the sky gradient
+ * (pt_color.comp:191-199). */
+int mrt_renderer_set_cubemap(mrt_renderer r, const float* faces, int size);
 /* Attaches a radiance cache (no reference counterpart). mode 0: off; 1: train only (every pass
  * builds a training batch of up to 2^18 path-vertex samples from a `train_fraction` of paths,
  * which stay unbiased, and takes `train_steps` Adam steps on disjoint slices of it); 2: train and

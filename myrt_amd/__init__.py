@@ -122,6 +122,8 @@ SIGNATURES = {
     "mrt_renderer_set_lens_size": ([_P, ctypes.c_float, ctypes.c_float], ctypes.c_int),
     "mrt_renderer_set_focus": ([_P, ctypes.c_float], ctypes.c_int),
     "mrt_renderer_set_bokeh_mask": ([_P, _P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mrt_renderer_set_cubemap": ([_P, _P, ctypes.c_int], ctypes.c_int),
+    "mrt_scene_push_texture": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
     "mrt_renderer_set_cache": ([_P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_sample_count": ([_P, _U32P], ctypes.c_int),
     "mrt_render": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -484,6 +486,18 @@ def _scene_sdf_drawables(self, n):
     return out[:n]
 
 
+def _scene_push_texture(self, image):
+    """image: (h, w, 3 or 4) floats, row 0 at v = 0. Returns the texture id (>= 1)."""
+    im = np.asarray(image, np.float32)
+    rgba = np.ones(im.shape[:2] + (4,), np.float32)
+    rgba[..., :im.shape[2]] = im
+    rgba = np.ascontiguousarray(rgba)
+    out = ctypes.c_uint32()
+    _check(lib().mrt_scene_push_texture(self.h, _ptr(rgba), rgba.shape[1], rgba.shape[0], ctypes.byref(out)))
+    return out.value
+
+
+Scene.push_texture = _scene_push_texture
 Scene.sdf_drawables = _scene_sdf_drawables
 Scene.push_sdf = _scene_push_sdf
 Scene.enqueue_sdf = _scene_enqueue_sdf
@@ -549,6 +563,17 @@ class Renderer:
 
     def invalidate(self):
         _check(lib().mrt_renderer_invalidate(self.h))
+
+    def set_cubemap(self, faces):
+        """faces: (6, n, n, 3 or 4) floats in +x, -x, +y, -y, +z, -z order, or None for the sky."""
+        if faces is None:
+            _check(lib().mrt_renderer_set_cubemap(self.h, None, 0))
+            return
+        f = np.asarray(faces, np.float32)
+        rgba = np.ones(f.shape[:3] + (4,), np.float32)
+        rgba[..., :f.shape[3]] = f
+        rgba = np.ascontiguousarray(rgba)
+        _check(lib().mrt_renderer_set_cubemap(self.h, _ptr(rgba), rgba.shape[1]))
 
     def set_dof(self, enabled=True, lens_size=(10.0, 10.0), focus=10.0, bokeh_mask=None):
         """set_dof_enabled / set_lens_size / set_focus / set_bokeh_mask; mask: (h, w, 3 or 4) floats."""

@@ -96,10 +96,69 @@ __device__ inline V3 unpack_rgba8(uint32_t c) {
             float((c >> 16) & 0xffu) * (1.0f / 255.0f));
 }
 
-// Environment of src/pt/pt_color.comp:198 (sky gradient), scaled.
-__device__ inline V3 environment(V3 d, float scale) {
+// Bilinear lookup with wrap (repeat) of an albedo texture at (u, v), texel centres at
+// (i + 0.5) / size.
+__device__ inline V3 texture_lookup(const PtScene& sc, const TexInfo& t, float u, float v) {
+  float fx = u * float(t.w) - 0.5f, fy = v * float(t.h) - 0.5f;
+  fx = fx - floorf(fx / float(t.w)) * float(t.w);
+  fy = fy - floorf(fy / float(t.h)) * float(t.h);
+  uint32_t x0 = min(uint32_t(fx), t.w - 1), y0 = min(uint32_t(fy), t.h - 1);
+  const float tx = fminf(fmaxf(fx - float(x0), 0.f), 1.f), ty = fminf(fmaxf(fy - float(y0), 0.f), 1.f);
+  const uint32_t x1 = x0 + 1 == t.w ? 0 : x0 + 1, y1 = y0 + 1 == t.h ? 0 : y0 + 1;
+  const float4* T = sc.texels + t.offset;
+  const float4 a = T[y0 * t.w + x0], b = T[y0 * t.w + x1], c = T[y1 * t.w + x0], e = T[y1 * t.w + x1];
+  const float w00 = (1.f - tx) * (1.f - ty), w10 = tx * (1.f - ty), w01 = (1.f - tx) * ty, w11 = tx * ty;
+  return mk(a.x * w00 + b.x * w10 + c.x * w01 + e.x * w11, a.y * w00 + b.y * w10 + c.y * w01 + e.y * w11,
+            a.z * w00 + b.z * w10 + c.z * w01 + e.z * w11);
+}
+
+// Cube-map lookup (face selection and face coordinates as in the OpenGL specification's cube
+// map table), bilinear within the face, clamped to its edge texels.
+__device__ inline V3 cubemap_lookup(const PtScene& sc, V3 d) {
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  uint32_t face;
+  float scc, tcc, ma;
+  if (ax >= ay && ax >= az) {
+    face = d.x >= 0.f ? 0u : 1u;
+    scc = d.x >= 0.f ? -d.z : d.z;
+    tcc = -d.y;
+    ma = ax;
+  } else if (ay >= az) {
+    face = d.y >= 0.f ? 2u : 3u;
+    scc = d.x;
+    tcc = d.y >= 0.f ? d.z : -d.z;
+    ma = ay;
+  } else {
+    face = d.z >= 0.f ? 4u : 5u;
+    scc = d.z >= 0.f ? d.x : -d.x;
+    tcc = -d.y;
+    ma = az;
+  }
+  const float n = float(sc.cube_size);
+  const float u = 0.5f * (scc / ma + 1.0f), v = 0.5f * (tcc / ma + 1.0f);
+  const float fx = fminf(fmaxf(u * n - 0.5f, 0.f), n - 1.f), fy = fminf(fmaxf(v * n - 0.5f, 0.f), n - 1.f);
+  const uint32_t x0 = uint32_t(fx), y0 = uint32_t(fy);
+  const uint32_t x1 = min(x0 + 1, sc.cube_size - 1), y1 = min(y0 + 1, sc.cube_size - 1);
+  const float tx = fx - float(x0), ty = fy - float(y0);
+  const float4* F = sc.cubemap + size_t(face) * sc.cube_size * sc.cube_size;
+  const float4 a = F[y0 * sc.cube_size + x0], b = F[y0 * sc.cube_size + x1];
+  const float4 c = F[y1 * sc.cube_size + x0], e = F[y1 * sc.cube_size + x1];
+  const float w00 = (1.f - tx) * (1.f - ty), w10 = tx * (1.f - ty), w01 = (1.f - tx) * ty, w11 = tx * ty;
+  return mk(a.x * w00 + b.x * w10 + c.x * w01 + e.x * w11, a.y * w00 + b.y * w10 + c.y * w01 + e.y * w11,
+            a.z * w00 + b.z * w10 + c.z * w01 + e.z * w11);
+}
+
+// Sky gradient of src/pt/pt_color.comp:198, scaled.
+__device__ inline V3 sky_gradient(V3 d, float scale) {
   const float t = (d.y + 1.0f) * 0.5f;
   return mk(0.3f + (1.0f - 0.3f) * t, 0.4f + (0.98f - 0.4f) * t, 0.5f + (0.96f - 0.5f) * t) * scale;
+}
+
+// Environment of src/pt/pt_color.comp:191-199: the cube map if one is set, else the sky
+// gradient; scaled.
+__device__ inline V3 environment(const PtScene& sc, V3 d) {
+  if (sc.cubemap) return cubemap_lookup(sc, d) * sc.env_scale;
+  return sky_gradient(d, sc.env_scale);
 }
 
 __device__ inline float ggx_d(float nh, float a2) {
@@ -304,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
       add_pixel(b.accum, ps.pixel, c);
       add_train(b, ps.pad, c);
     } else if (!found) {
-      const V3 c = thr * environment(rd, sc.env_scale);
+      const V3 c = thr * environment(sc, rd);
       add_pixel(b.accum, ps.pixel, c);
       add_train(b, ps.pad, c);
     } else {
@@ -312,6 +371,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
       const uint32_t inst = hit.z, prim = hit.y;
       V3 ng, ns;
       uint32_t mi;
+      float hu = 0.f, hv = 0.f;
+      bool has_uv = false;
       if (inst & 0x80000000u) {
         // SDF surface: the record carries the world normal and the material.
         ng = oct_decode(hit.w);
@@ -325,6 +386,12 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
         const float4 p0 = sc.pos[gi.vtx_base + i0], p1 = sc.pos[gi.vtx_base + i1], p2 = sc.pos[gi.vtx_base + i2];
         const float b1 = float(hit.w & 0xffffu) * (1.0f / 65535.0f), b2 = float(hit.w >> 16) * (1.0f / 65535.0f);
         const float b0 = 1.0f - b1 - b2;
+        if (sc.uv) {
+          const float2 t0 = sc.uv[gi.vtx_base + i0], t1 = sc.uv[gi.vtx_base + i1], t2 = sc.uv[gi.vtx_base + i2];
+          hu = b0 * t0.x + b1 * t1.x + b2 * t2.x;
+          hv = b0 * t0.y + b1 * t1.y + b2 * t2.y;
+          has_uv = true;
+        }
         const V3 e1 = mk(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z), e2 = mk(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z);
         const V3 ngo = cross(e1, e2);
         V3 nso = ngo;
@@ -354,6 +421,11 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
       const Material mat = sc.materials[mi];
       Lobes L;
       L.albedo = unpack_rgba8(mat.albedo_rgba8);
+      if (has_uv && mat.texture >= 1u && mat.texture <= sc.num_textures) {
+        // The albedo texture replaces the colour, decoded with gamma 2.2 (material_new.glsl:29-31).
+        const V3 t = texture_lookup(sc, sc.textures[mat.texture - 1], hu, 1.0f - hv);
+        L.albedo = mk(powf(fmaxf(t.x, 0.f), 2.2f), powf(fmaxf(t.y, 0.f), 2.2f), powf(fmaxf(t.z, 0.f), 2.2f));
+      }
       L.metallic = mat.metallic;
       const float sm = sqrtf(fmaxf(mat.metallic, 0.f));
       L.spec_col = mk(1.0f + (L.albedo.x - 1.0f) * sm, 1.0f + (L.albedo.y - 1.0f) * sm, 1.0f + (L.albedo.z - 1.0f) * sm);

@@ -108,6 +108,13 @@ struct Camera {
   uint32_t bokeh_w, bokeh_h;
 };
 
+// An albedo texture in the scene's texel array (rgba floats, row-major, row 0 at v = 0).
+struct TexInfo {
+  uint32_t offset;
+  uint32_t w, h;
+  uint32_t pad;
+};
+
 struct PtScene {
   const InstanceRecord* instances;
   const uint32_t* inst_material;
@@ -115,10 +122,16 @@ struct PtScene {
   const uint32_t* idx;
   const float4* pos;
   const float4* nrm;
+  const float2* uv;
   const Material* materials;
   const QuadLight* lights;
+  const TexInfo* textures;
+  const float4* texels;
+  const float4* cubemap;   // 6 faces (+x, -x, +y, -y, +z, -z) of cube_size^2 texels, or null
+  uint32_t cube_size;
   uint32_t num_lights;
   uint32_t num_materials;
+  uint32_t num_textures;
   float env_scale;
 };
 

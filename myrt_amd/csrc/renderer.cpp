@@ -116,6 +116,8 @@ struct Renderer {
   float focus = 10.f;
   float4* bokeh = nullptr;
   uint32_t bokeh_w = 0, bokeh_h = 0;
+  float4* cubemap = nullptr;  // set_cubemap: 6 faces of cube_size^2 rgba texels
+  uint32_t cube_size = 0;
   Cache* cache = nullptr;
   int cache_mode = 0;  // 0 off, 1 train only, 2 train and end paths in the cache
   float train_fraction = 1.0f / 16.0f;
@@ -141,7 +143,7 @@ struct Renderer {
   ~Renderer() {
     for (void* p : {rays[0], rays[1], (void*)paths[0], (void*)paths[1], hits, shadow_rays, (void*)shadow,
                     (void*)occluded, (void*)counts, (void*)accum, (void*)tpaths, (void*)tverts, (void*)queries,
-                    (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den, (void*)bokeh})
+                    (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den, (void*)bokeh, (void*)cubemap})
       if (p) hipFree(p);
   }
 
@@ -243,7 +245,9 @@ struct Renderer {
     b.accum = accum;
     b.aux = aux;
     b.cache = pc;
-    const PtScene sc = s.shading_view();
+    PtScene sc = s.shading_view();
+    sc.cubemap = cubemap;
+    sc.cube_size = cube_size;
     const auto t0 = std::chrono::steady_clock::now();
     uint32_t done = 0;
     {
@@ -378,6 +382,21 @@ using namespace mrt;
   }
 
 extern "C" {
+
+int mrt_scene_push_texture(mrt_scene sh, const float* rgba, int w, int h, uint32_t* out) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !rgba || w < 1 || h < 1 || size_t(w) * h > (size_t(1) << 26)) return fail(MRT_ERR_INVALID, "bad texture");
+  if (s->texels.size() + size_t(w) * h >= (size_t(1) << 31)) return fail(MRT_ERR_LIMIT, "texture memory exceeds 2^31 texels");
+  TexInfo t{uint32_t(s->texels.size()), uint32_t(w), uint32_t(h), 0};
+  const float4* px = reinterpret_cast<const float4*>(rgba);
+  s->texels.insert(s->texels.end(), px, px + size_t(w) * h);
+  s->textures.push_back(t);
+  s->shading_dirty = true;
+  if (out) *out = uint32_t(s->textures.size());  // 0 means "no texture"
+  return MRT_OK;
+  MRT_GUARD_END
+}
 
 int mrt_scene_push_material(mrt_scene sh, const mrt_material* mat, uint32_t* out) {
   MRT_GUARD_BEGIN
@@ -515,6 +534,23 @@ int mrt_renderer_set_bokeh_mask(mrt_renderer rh, const float* rgba, int w, int h
   if (e != hipSuccess) return hip_fail(e, "uploading the bokeh mask");
   r->bokeh_w = uint32_t(w);
   r->bokeh_h = uint32_t(h);
+  return MRT_OK;
+}
+
+int mrt_renderer_set_cubemap(mrt_renderer rh, const float* faces, int size) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || size < 0 || (size > 0 && !faces) || size > 16384) return fail(MRT_ERR_INVALID, "bad cube map");
+  if (r->cubemap) hipFree(r->cubemap);
+  r->cubemap = nullptr;
+  r->cube_size = 0;
+  r->sample_count = 0;
+  // Like pt_color.comp:192, maps of at most 10 texels a side mean "use the sky gradient".
+  if (size <= 10) return MRT_OK;
+  const size_t n = size_t(6) * size * size;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->cubemap), n * sizeof(float4));
+  if (e == hipSuccess) e = hipMemcpy(r->cubemap, faces, n * sizeof(float4), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "uploading the cube map");
+  r->cube_size = uint32_t(size);
   return MRT_OK;
 }
 

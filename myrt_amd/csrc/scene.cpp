@@ -153,6 +153,7 @@ int Scene::build_host() {
       all_idx.clear();
       all_pos.clear();
       all_nrm.clear();
+      all_uv.clear();
       for (const Geometry& geo : geoms) {
         GeoInfo gi{};
         gi.idx_base = uint32_t(all_idx.size());
@@ -167,6 +168,7 @@ int Scene::build_host() {
             all_nrm.push_back(make_float4(geo.nrm[3 * v], geo.nrm[3 * v + 1], geo.nrm[3 * v + 2], 0.f));
           else
             all_nrm.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+          all_uv.push_back(make_float2(geo.uv[2 * v], geo.uv[2 * v + 1]));
         }
       }
     }
@@ -223,6 +225,7 @@ int Scene::upload_all() {
     if ((rc = upload(d_idx, cap_idx, all_idx)) != MRT_OK) return rc;
     if ((rc = upload(d_pos, cap_pos, all_pos)) != MRT_OK) return rc;
     if ((rc = upload(d_nrm, cap_nrm, all_nrm)) != MRT_OK) return rc;
+    if ((rc = upload(d_uv, cap_uv, all_uv)) != MRT_OK) return rc;
   }
   if (sdf_dirty) {
     // The device drawables address the compiled programs.
@@ -252,6 +255,12 @@ int Scene::upload_all() {
     }
     if ((rc = upload(d_materials, cap_mat, mats)) != MRT_OK) return rc;
     if ((rc = upload(d_lights, cap_lights, lights)) != MRT_OK) return rc;
+    std::vector<TexInfo> tex = textures;
+    std::vector<float4> tx = texels;
+    if (tex.empty()) tex.push_back(TexInfo{0, 1, 1, 0});
+    if (tx.empty()) tx.push_back(make_float4(1.f, 1.f, 1.f, 1.f));
+    if ((rc = upload(d_textures, cap_textures, tex)) != MRT_OK) return rc;
+    if ((rc = upload(d_texels, cap_texels, tx)) != MRT_OK) return rc;
     shading_dirty = false;
   }
   if (!d_ctrl) {
@@ -269,7 +278,7 @@ int Scene::upload_all() {
 
 Scene::~Scene() {
   for (void* p : {(void*)d_sdf_drawables, (void*)d_sdf_prog, (void*)d_sdf_grids, (void*)d_sdf_masks, (void*)d_geo_info, (void*)d_idx, (void*)d_inst_material, (void*)d_pos, (void*)d_nrm,
-                  (void*)d_materials, (void*)d_lights})
+                  (void*)d_materials, (void*)d_lights, (void*)d_uv, (void*)d_textures, (void*)d_texels})
     if (p) hipFree(p);
   if (d_blas) hipFree(d_blas);
   if (d_tris) hipFree(d_tris);
@@ -288,6 +297,10 @@ PtScene Scene::shading_view() const {
   v.idx = d_idx;
   v.pos = d_pos;
   v.nrm = d_nrm;
+  v.uv = d_uv;
+  v.textures = d_textures;
+  v.texels = d_texels;
+  v.num_textures = uint32_t(textures.size());
   v.materials = d_materials;
   v.lights = d_lights;
   v.num_lights = uint32_t(lights.size());

@@ -69,6 +69,8 @@ struct Scene {
 
   // Shading data: materials (material_buffer, src/material.hpp:72-118), quad lights, environment.
   std::vector<Material> materials;
+  std::vector<TexInfo> textures;   // albedo textures (mrt_scene_push_texture), ids from 1
+  std::vector<float4> texels;
   std::vector<QuadLight> lights;
   float env_scale = 1.0f;
   bool shading_dirty = true;
@@ -83,6 +85,7 @@ struct Scene {
   std::vector<GeoInfo> geo_info;
   std::vector<uint32_t> all_idx, inst_material;
   std::vector<float4> all_pos, all_nrm;
+  std::vector<float2> all_uv;
 
   // Device arrays.
   WideNode *d_blas = nullptr, *d_tlas = nullptr;
@@ -95,14 +98,21 @@ struct Scene {
   float4 *d_pos = nullptr, *d_nrm = nullptr;
   Material* d_materials = nullptr;
   QuadLight* d_lights = nullptr;
+  float2* d_uv = nullptr;
+  TexInfo* d_textures = nullptr;
+  float4* d_texels = nullptr;
+  size_t cap_uv = 0, cap_textures = 0, cap_texels = 0;
   size_t cap_geo_info = 0, cap_idx = 0, cap_inst_mat = 0, cap_pos = 0, cap_nrm = 0, cap_mat = 0, cap_lights = 0;
   uint32_t* d_ctrl = nullptr;  // [0] ray counter, [1] flags, [8..15] four u64 counters
   uint32_t* d_spill = nullptr;
   size_t spill_cap = 0;
   int num_cus = 0;
   uint32_t refill_below = 40;
-  uint32_t blas_leaf = 127;  // largest wide leaf (triangles); MRT_BVH_LEAF overrides
-  uint32_t tlas_leaf = 127;  // largest TLAS leaf (instances); MRT_TLAS_LEAF overrides
+  // Largest wide leaf: binary leaves are split at centroid medians down to this many triangles
+  // (instances for the TLAS). Measured best: 2 triangles, 1 instance. MRT_BVH_LEAF and
+  // MRT_TLAS_LEAF override.
+  uint32_t blas_leaf = 2;
+  uint32_t tlas_leaf = 1;
   uint32_t lds_depth = 16;  // LDS stack entries per lane; MRT_TRACE_LDS_STACK overrides (a power of two, 4-64)
 
   ~Scene();
