@@ -124,6 +124,29 @@ def ground_quad(size=6.0, y=-5.0):
     return idx, pts, nrm
 
 
+def textured_ground(size=8.0, y=-5.0, tiles=4.0):
+    """ground_quad with uvs running 0..tiles across (for textures that wrap)."""
+    idx, pts, nrm = ground_quad(size, y)
+    uv = np.array([[0, 0], [tiles, 0], [tiles, tiles], [0, tiles]], np.float32)
+    return idx, pts, nrm, uv
+
+
+def checker_texture(n=64, cells=8, a=(0.9, 0.85, 0.8), b=(0.15, 0.2, 0.3)):
+    """An n x n rgb checkerboard of cells x cells squares."""
+    ys, xs = np.mgrid[0:n, 0:n]
+    on = ((xs * cells // n) + (ys * cells // n)) % 2 == 0
+    img = np.where(on[..., None], np.asarray(a, np.float32), np.asarray(b, np.float32))
+    return img.astype(np.float32)
+
+
+def gradient_cubemap(n=32):
+    """A six-face cube map: each face a distinct tint times a vertical ramp (row index)."""
+    tints = np.array([[1.0, 0.3, 0.2], [0.2, 1.0, 0.3], [0.3, 0.2, 1.0], [1.0, 1.0, 0.2], [0.2, 1.0, 1.0],
+                      [1.0, 0.2, 1.0]], np.float32)
+    ramp = (np.arange(n, dtype=np.float32) + 0.5) / n
+    return (tints[:, None, None, :] * (0.5 + ramp)[None, :, None, None]).repeat(n, axis=2).astype(np.float32)
+
+
 # ---- rays -------------------------------------------------------------------------------------
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])

@@ -31,7 +31,9 @@ class _PtScene(ctypes.Structure):
                 ("sdf_tape", ctypes.c_void_p), ("sdf_drawables", ctypes.c_void_p), ("n_sdf", ctypes.c_uint32),
                 ("sdf_steps", ctypes.c_int), ("sdf_eps", ctypes.c_float), ("dof", ctypes.c_int),
                 ("lens", ctypes.c_float * 2), ("focus", ctypes.c_float), ("bokeh", ctypes.c_void_p),
-                ("bokeh_w", ctypes.c_int), ("bokeh_h", ctypes.c_int)]
+                ("bokeh_w", ctypes.c_int), ("bokeh_h", ctypes.c_int), ("uv", ctypes.c_void_p),
+                ("texels", ctypes.c_void_p), ("tex_info", ctypes.c_void_p), ("n_textures", ctypes.c_uint32),
+                ("cubemap", ctypes.c_void_p), ("cube_size", ctypes.c_int)]
 
 
 MATERIAL_DTYPE = np.dtype([("albedo_rgba8", "<u4"), ("ior", "<f4"), ("roughness", "<f4"), ("metallic", "<f4"),
@@ -245,16 +247,35 @@ class OracleRender:
     """FP64 render of a scene description with the product's sampler sequence."""
 
     def __init__(self, geometries, instances, materials=None, lights=None, env_scale=1.0, sdf_tape=None,
-                 sdf_drawables=None, sdf_steps=400, sdf_eps=1e-5):
-        """geometries: list of (indices, points, normals or None); instances: (geom, 4x4, material);
-        sdf_drawables: the product's prepared drawable records (mrt_scene_sdf_drawables)."""
+                 sdf_drawables=None, sdf_steps=400, sdf_eps=1e-5, textures=None):
+        """geometries: list of (indices, points, normals or None[, uvs]); instances: (geom, 4x4,
+        material); sdf_drawables: the product's prepared drawable records (mrt_scene_sdf_drawables);
+        textures: list of (h, w, 3 or 4) float images (ids 1, 2, ... in order)."""
         self.scene = OracleScene([(g[0], g[1]) for g in geometries], [(g, mt) for g, mt, _ in instances])
         nrm = []
-        for ind, pts, nr in geometries:
+        for g in geometries:
+            pts, nr = g[1], g[2]
             n = np.zeros((np.asarray(pts).reshape(-1, 3).shape[0], 3), np.float32) if nr is None else nr
             nrm.append(np.asarray(n, np.float32).reshape(-1, 3))
         self.nrm = np.ascontiguousarray(np.concatenate(nrm))
         self.has_nrm = np.array([0 if g[2] is None else 1 for g in geometries], np.uint32)
+        uvs = []
+        for g in geometries:
+            npts = np.asarray(g[1]).reshape(-1, 3).shape[0]
+            uv = g[3] if len(g) > 3 and g[3] is not None else np.zeros((npts, 2), np.float32)
+            uvs.append(np.asarray(uv, np.float32).reshape(-1, 2))
+        self.uv = np.ascontiguousarray(np.concatenate(uvs))
+        texels, info, off = [], [], 0
+        for im in textures or []:
+            im = np.asarray(im, np.float32)
+            rgba = np.ones(im.shape[:2] + (4,), np.float32)
+            rgba[..., :im.shape[2]] = im
+            texels.append(rgba.reshape(-1, 4))
+            info.append((off, im.shape[1], im.shape[0], 0))
+            off += im.shape[0] * im.shape[1]
+        self.texels = np.ascontiguousarray(np.concatenate(texels) if texels else np.ones((1, 4), np.float32))
+        self.tex_info = np.ascontiguousarray(np.array(info or [(0, 1, 1, 0)], np.uint32))
+        self.n_textures = len(info)
         self.inst_mat = np.array([mi for _, _, mi in instances] or [0], np.uint32)
         mats = DEFAULT_MATERIAL if materials is None or len(materials) == 0 else np.ascontiguousarray(materials,
                                                                                                       MATERIAL_DTYPE)
@@ -269,6 +290,22 @@ class OracleRender:
                           self._lights_buf.ctypes.data, self.lights.size, float(env_scale),
                           None if not n_sdf else self.sdf_tape.ctypes.data,
                           None if not n_sdf else self.sdf_dr.ctypes.data, n_sdf, int(sdf_steps), float(sdf_eps))
+        self.c.uv = self.uv.ctypes.data
+        self.c.texels = self.texels.ctypes.data
+        self.c.tex_info = self.tex_info.ctypes.data
+        self.c.n_textures = self.n_textures
+        self._cube = None
+
+    def set_cubemap(self, faces):
+        """faces: (6, n, n, 3 or 4) floats (+x, -x, +y, -y, +z, -z), or None for the sky gradient."""
+        self._cube = None
+        self.c.cubemap, self.c.cube_size = None, 0
+        if faces is not None and np.asarray(faces).shape[1] > 10:
+            f = np.asarray(faces, np.float32)
+            rgba = np.ones(f.shape[:3] + (4,), np.float32)
+            rgba[..., :f.shape[3]] = f
+            self._cube = np.ascontiguousarray(rgba)
+            self.c.cubemap, self.c.cube_size = self._cube.ctypes.data, rgba.shape[1]
 
     def set_dof(self, enabled=True, lens_size=(10.0, 10.0), focus=10.0, bokeh_mask=None):
         """The product's thin-lens camera; mask (h, w, 3 or 4) floats, narrower than 10 = none."""

@@ -138,6 +138,13 @@ typedef struct orc_pt_scene {
   float focus;
   const float* bokeh;            /* rgba floats, bokeh_w x bokeh_h, or NULL */
   int bokeh_w, bokeh_h;
+  /* albedo textures and environment cube map */
+  const float* uv;               /* per-vertex uv, same layout as nrm (2 floats), or NULL */
+  const float* texels;           /* rgba floats of all textures */
+  const uint32_t* tex_info;      /* per texture: offset (texels), w, h, 0 */
+  uint32_t n_textures;
+  const float* cubemap;          /* 6 faces of cube_size^2 rgba floats, or NULL */
+  int cube_size;
 } orc_pt_scene;
 
 /* The product's wavefront estimator in binary64: camera (inverse(proj * view), with the

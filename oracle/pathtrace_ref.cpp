@@ -81,9 +81,68 @@ void onb(D3 n, D3& t, D3& b) {
   b = {c, s + n.y * n.y * a, -n.y};
 }
 
-D3 environment(D3 d, double scale) {
+D3 sky(D3 d, double scale) {
   const double t = (d.y + 1.0) * 0.5;
   return D3{0.3 + 0.7 * t, 0.4 + 0.58 * t, 0.5 + 0.46 * t} * scale;
+}
+
+// Bilinear rgb of a w x h rgba image at continuous texel coordinates (fx, fy) (texel centres at
+// integers), neighbours given by the caller's wrap or clamp.
+D3 bilinear(const float* img, int w, double fx, double fy, int x0, int y0, int x1, int y1) {
+  const double tx = fx - std::floor(fx), ty = fy - std::floor(fy);
+  const int xs[4] = {x0, x1, x0, x1}, ys[4] = {y0, y0, y1, y1};
+  const double ws[4] = {(1 - tx) * (1 - ty), tx * (1 - ty), (1 - tx) * ty, tx * ty};
+  D3 out{0, 0, 0};
+  for (int k = 0; k < 4; ++k) {
+    const float* t = img + 4 * (size_t(ys[k]) * w + xs[k]);
+    out = out + D3{t[0], t[1], t[2]} * ws[k];
+  }
+  return out;
+}
+
+// Albedo texture: bilinear with wrap at (u, v), texel centres at (i + 0.5) / size.
+D3 texture_lookup(const orc_pt_scene* S, uint32_t id, double u, double v) {
+  const uint32_t* ti = S->tex_info + 4 * (id - 1);
+  const int w = int(ti[1]), h = int(ti[2]);
+  double fx = u * w - 0.5, fy = v * h - 0.5;
+  fx -= std::floor(fx / w) * w;
+  fy -= std::floor(fy / h) * h;
+  const int x0 = std::min(int(fx), w - 1), y0 = std::min(int(fy), h - 1);
+  return bilinear(S->texels + 4 * size_t(ti[0]), w, fx, fy, x0, y0, (x0 + 1) % w, (y0 + 1) % h);
+}
+
+// Cube map per the OpenGL face table, bilinear within the face, clamped to its edges.
+D3 cube_lookup(const orc_pt_scene* S, D3 d) {
+  const double ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+  int face;
+  double sc, tc, ma;
+  if (ax >= ay && ax >= az) {
+    face = d.x >= 0 ? 0 : 1;
+    sc = d.x >= 0 ? -d.z : d.z;
+    tc = -d.y;
+    ma = ax;
+  } else if (ay >= az) {
+    face = d.y >= 0 ? 2 : 3;
+    sc = d.x;
+    tc = d.y >= 0 ? d.z : -d.z;
+    ma = ay;
+  } else {
+    face = d.z >= 0 ? 4 : 5;
+    sc = d.z >= 0 ? d.x : -d.x;
+    tc = -d.y;
+    ma = az;
+  }
+  const int n = S->cube_size;
+  const double fx = std::min(std::max(0.5 * (sc / ma + 1) * n - 0.5, 0.0), double(n - 1));
+  const double fy = std::min(std::max(0.5 * (tc / ma + 1) * n - 0.5, 0.0), double(n - 1));
+  const int x0 = int(fx), y0 = int(fy);
+  return bilinear(S->cubemap + 4 * size_t(face) * n * n, n, fx, fy, x0, y0, std::min(x0 + 1, n - 1),
+                  std::min(y0 + 1, n - 1));
+}
+
+D3 environment(const orc_pt_scene* S, D3 d) {
+  if (S->cubemap) return cube_lookup(S, d) * S->env_scale;
+  return sky(d, S->env_scale);
 }
 
 bool mat_inv4(const double m[16], double out[16]) {
@@ -237,12 +296,14 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
       break;
     }
     if (!found) {
-      sum = sum + thr * environment(rd, S->env_scale);
+      sum = sum + thr * environment(S, rd);
       break;
     }
     const uint32_t inst = hit.inst, prim = hit.prim;
     D3 ng, ns;
     uint32_t mi;
+    double hu = 0, hv = 0;
+    bool has_uv = false;
     if (inst & 0x80000000u) {
       ng = normalize(D3{sdf_n[0], sdf_n[1], sdf_n[2]});
       ns = ng;
@@ -255,6 +316,12 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
       const double b1 = std::round(std::fmin(std::fmax(double(hit.b1), 0.0), 1.0) * 65535.0) / 65535.0;
       const double b2 = std::round(std::fmin(std::fmax(double(hit.b2), 0.0), 1.0) * 65535.0) / 65535.0;
       const double b0 = 1.0 - b1 - b2;
+      if (S->uv) {
+        const float* T = S->uv + 2 * size_t(G->geo_vtx_off[g]);
+        hu = T[2 * tri[0]] * b0 + T[2 * tri[1]] * b1 + T[2 * tri[2]] * b2;
+        hv = T[2 * tri[0] + 1] * b0 + T[2 * tri[1] + 1] * b1 + T[2 * tri[2] + 1] * b2;
+        has_uv = true;
+      }
       const D3 ngo = cross(p1 - p0, p2 - p0);
       D3 nso = ngo;
       if (S->geo_has_nrm[g]) {
@@ -283,6 +350,11 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
     const orc_material& mat = S->materials[mi];
     Lobes L;
     L.albedo = rgba8(mat.albedo_rgba8);
+    if (has_uv && mat.texture >= 1 && mat.texture <= S->n_textures) {
+      const D3 t = texture_lookup(S, mat.texture, hu, 1.0 - hv);
+      L.albedo = D3{std::pow(std::max(t.x, 0.0), 2.2), std::pow(std::max(t.y, 0.0), 2.2),
+                    std::pow(std::max(t.z, 0.0), 2.2)};
+    }
     L.metallic = mat.metallic;
     const double sm = std::sqrt(std::max(double(mat.metallic), 0.0));
     L.spec_col = D3{1 + (L.albedo.x - 1) * sm, 1 + (L.albedo.y - 1) * sm, 1 + (L.albedo.z - 1) * sm};

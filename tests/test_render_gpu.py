@@ -154,3 +154,42 @@ def test_depth_of_field_matches_fp64_oracle(ensure_built, masked):
     err = rel_mse(img, ref)
     say("dof relative MSE %.3e" % err)
     assert err <= REL_MSE_TOL
+
+
+def test_textures_and_cubemap_match_fp64_oracle(ensure_built):
+    """A checkerboard albedo texture on the ground and on some spheres (icosphere uvs are zero,
+    so those read one texel), lit by the quad light and a six-face cube map."""
+    sc = scenes.sphere_field_scene(30, 2, seed=5, metal_every=4, ground=False)
+    gi, gp, gn, guv = scenes.textured_ground(size=8.0, y=-5.0, tiles=3.0)
+    geoms = list(sc["geometries"]) + [(gi, gp, gn, guv)]
+    mats = sc["materials"].copy()
+    mats[-1]["texture"] = 1
+    mats[3]["texture"] = 2
+    insts = list(sc["instances"]) + [(1, np.eye(4, dtype=np.float32), len(mats) - 1)]
+    tex = [scenes.checker_texture(64, 8), scenes.checker_texture(16, 2, a=(0.9, 0.2, 0.2), b=(0.2, 0.9, 0.2))]
+    cube = scenes.gradient_cubemap(32)
+    W, H, spp, depth = 80, 60, 32, 4
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
+    s = m.Scene()
+    gids = [s.push_geometry(*g) for g in geoms]
+    for t in tex:
+        s.push_texture(t)
+    for mat in mats:
+        s.push_material(mat)
+    s.set_lights(sc["lights"])
+    s.set_environment(0.8)
+    for g, mt, mi in insts:
+        s.enqueue(gids[g], mt, mi)
+    r = m.Renderer()
+    r.set_camera(view, proj)
+    r.set_num_bounces(depth)
+    r.set_cubemap(cube)
+    say("textured render %dx%d %d spp" % (W, H, spp))
+    r.render(s, W, H, spp)
+    img = r.image()[..., :3].astype(np.float64)
+    o = orc.OracleRender(geoms, insts, mats, sc["lights"], 0.8, textures=tex)
+    o.set_cubemap(cube)
+    ref = o.render(view, proj, W, H, spp, depth)
+    err = rel_mse(img, ref)
+    say("textured relative MSE %.3e" % err)
+    assert err <= REL_MSE_TOL

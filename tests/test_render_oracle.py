@@ -99,3 +99,43 @@ def test_dof_closed_forms(ensure_built):
     assert np.allclose(half, 0.5 * lens, rtol=1e-9, atol=1e-12)
     r.set_dof(True, (1.5, 1.5), 9.0, bokeh_mask=np.zeros((16, 16, 3)))
     assert not r.render(view, proj, 12, 10, 4, 3).any()
+
+
+def test_texture_and_cubemap_closed_forms(ensure_built):
+    """A white albedo texture equals a white material; a texture of value c gives albedo c^2.2; a
+    camera ray that leaves the scene along +x reads face 0 of the cube map at the texel row the
+    OpenGL face table gives."""
+    gi, gp, gn, guv = scenes.textured_ground(size=3.0, y=0.0, tiles=2.0)
+    geo = [(gi, gp, gn, guv)]
+    inst = [(0, np.eye(4, dtype=np.float32), 0)]
+    lights = scenes.LIGHT_DTYPE and np.zeros(1, scenes.LIGHT_DTYPE)
+    lights[0] = ((-0.5, 2.0, -0.5), (1, 0, 0), (0, 0, 1), (10, 10, 10))
+    view, proj = scenes.camera_matrices((0, 4, 0.01), (0, 0, 0), 40.0, 1.0)
+
+    def mat(rgb, tex):
+        m = np.zeros(1, scenes.MATERIAL_DTYPE)
+        m[0] = (scenes._rgba8(rgb), 1.0, 1.0, 0.0, 0.0, tex, (0, 0))
+        return m
+
+    white = orc.OracleRender(geo, inst, mat((1, 1, 1), 0), lights, 0.5).render(view, proj, 8, 8, 4, 3)
+    tex_white = orc.OracleRender(geo, inst, mat((0.2, 0.3, 0.4), 1), lights, 0.5,
+                                 textures=[np.ones((4, 4, 3))]).render(view, proj, 8, 8, 4, 3)
+    assert np.allclose(tex_white, white, rtol=1e-12, atol=1e-15)
+    k = np.array([51, 102, 204])
+    c = ((k / 255.0) ** (1 / 2.2)).astype(np.float32)
+    plain = orc.OracleRender(geo, inst, mat(k / 255.0, 0), lights, 0.5).render(view, proj, 8, 8, 4, 3)
+    tex_c = orc.OracleRender(geo, inst, mat((1, 1, 1), 1), lights, 0.5,
+                             textures=[np.broadcast_to(c, (4, 4, 3))]).render(view, proj, 8, 8, 4, 3)
+    assert np.allclose(tex_c, plain, rtol=1e-5)
+    # Cube map: one pixel looking along +x, above the ground plane's extent.
+    far = [(gi, gp - np.array([0, 100, 0], np.float32), gn, guv)]
+    o = orc.OracleRender(far, inst, mat((1, 1, 1), 0), None, 2.0)
+    cube = scenes.gradient_cubemap(32)
+    o.set_cubemap(cube)
+    v2, p2 = scenes.camera_matrices((0, 0, 0), (1, 0.3, 0), 1e-3, 1.0)
+    got = o.render(v2, p2, 1, 1, 2, 1)[0, 0]
+    d = np.array([1.0, 0.3, 0.0]) / np.linalg.norm([1.0, 0.3, 0.0])
+    t = 0.5 * (-d[1] / d[0] + 1) * 32 - 0.5  # face +x: tc = -y / |x|
+    row = (np.arange(32) + 0.5) / 32
+    expect = 2.0 * np.array([1.0, 0.3, 0.2]) * (0.5 + np.interp(t, np.arange(32), row))
+    assert np.allclose(got, expect, rtol=1e-3)
