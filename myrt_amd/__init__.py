@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmyrt_amd.so")
+# MYRT_AMD_LIBRARY selects another build of the library (launch-shape tuning builds).
+LIB_PATH = os.environ.get("MYRT_AMD_LIBRARY") or os.path.join(HERE, "libmyrt_amd.so")
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("bary", "<u4")])

@@ -23,6 +23,10 @@ namespace mrt {
 namespace dev {
 
 constexpr int kBlock = 256;
+// Waves per SIMD the register allocation targets (512 VGPRs / waves per lane).
+#ifndef MRT_TRACE_WAVES
+#define MRT_TRACE_WAVES 5
+#endif
 constexpr float kRobust = 1.0000005f;  // > 1 + 2*gamma(3) for binary32
 
 struct RayPre {  // per-space ray data for the watertight test
@@ -97,7 +101,8 @@ __device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 __device__ inline float byte_of(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
 
 template <bool ANY, bool STATS>
-__global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRT_TRACE_WAVES, MRT_TRACE_WAVES))) void
+trace_kernel(TraceParams p) {
   extern __shared__ uint32_t lds_stack[];  // [lds_depth][kBlock]
   const uint32_t kStackLds = p.lds_depth;
   const uint32_t tid = threadIdx.x;
