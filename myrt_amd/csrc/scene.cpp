@@ -19,6 +19,7 @@
 #include "../../include/myrt.h"
 #include "bvh_build.hpp"
 #include "device_types.hpp"
+#include "raysort.hpp"
 #include "scene_internal.hpp"
 #include "sdf_kernels.hpp"
 #include "trace.hpp"
@@ -181,6 +182,13 @@ int Scene::build_host() {
         std::vector<Box> boxes;
         boxes.reserve(current.size());
         for (const Instance& in : current) boxes.push_back(in.world_box);
+        Box all;
+        for (const Box& b : boxes) all.enclose(b);
+        for (int a = 0; a < 3; ++a) {
+          const float ext = all.mx[a] - all.mn[a];
+          scene_lo[a] = all.mn[a];
+          scene_inv_ext[a] = ext > 0.f && std::isfinite(ext) ? 1.0f / ext : 1.0f;
+        }
         tlas = build_binary_bvh(boxes, false);
         WideBvh w = collapse_to_wide(tlas, kTlasPadRel, tlas_leaf);
         tlas_nodes = std::move(w.nodes);
@@ -287,6 +295,8 @@ Scene::~Scene() {
   if (d_inst) hipFree(d_inst);
   if (d_ctrl) hipFree(d_ctrl);
   if (d_spill) hipFree(d_spill);
+  for (uint32_t* q : {d_sort_keys, d_sort_perm, d_sort_counts})
+    if (q) hipFree(q);
 }
 
 PtScene Scene::shading_view() const {
@@ -350,6 +360,33 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.total_threads = total_threads;
   p.refill_below = refill_below;
   p.lds_depth = lds_depth;
+  p.perm = nullptr;
+  if (ray_sort && n >= 4096) {
+    if (cap_sort < n || !d_sort_counts) {
+      for (uint32_t* q : {d_sort_keys, d_sort_perm, d_sort_counts})
+        if (q) hipFree(q);
+      d_sort_keys = d_sort_perm = d_sort_counts = nullptr;
+      cap_sort = 0;
+      hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&d_sort_keys), n * 4);
+      if (e1 == hipSuccess) e1 = hipMalloc(reinterpret_cast<void**>(&d_sort_perm), n * 4);
+      if (e1 == hipSuccess) e1 = hipMalloc(reinterpret_cast<void**>(&d_sort_counts), kRaySortBins * 4);
+      if (e1 != hipSuccess) return hip_fail(e1, "hipMalloc ray binning");
+      cap_sort = n;
+    }
+    RaySortArgs ra{};
+    ra.rays = d_rays;
+    ra.n = uint32_t(n);
+    ra.n_dev = n_dev;
+    for (int a = 0; a < 3; ++a) {
+      ra.lo[a] = scene_lo[a];
+      ra.inv_ext[a] = scene_inv_ext[a];
+    }
+    ra.keys = d_sort_keys;
+    ra.counts = d_sort_counts;
+    ra.perm = d_sort_perm;
+    if (launch_ray_sort(ra, stream) != 0) return fail(MRT_ERR_HIP, "ray binning launch failed");
+    p.perm = d_sort_perm;
+  }
   if (launch_trace(p, any, stats, grid, stream) != 0) return fail(MRT_ERR_HIP, "trace kernel launch failed");
   if (!sdf_drawables.empty()) {
     SdfParams sp{};
@@ -592,6 +629,7 @@ int mrt_scene_create(mrt_scene* out) {
     const long l = std::strtol(v, nullptr, 10);
     if (l >= 1 && l <= 127) s->tlas_leaf = uint32_t(l);
   }
+  if (const char* v = std::getenv("MRT_RAY_SORT")) s->ray_sort = std::strtol(v, nullptr, 10) != 0;
   if (const char* v = std::getenv("MRT_TRACE_REFILL")) {
     const long r = std::strtol(v, nullptr, 10);
     if (r >= 1 && r <= 64) s->refill_below = uint32_t(r);

@@ -111,6 +111,11 @@ struct Scene {
   // Largest wide leaf: binary leaves are split at centroid medians down to this many triangles
   // (instances for the TLAS). Measured best: 2 triangles, 1 instance. MRT_BVH_LEAF and
   // MRT_TLAS_LEAF override.
+  // Ray binning before traversal (MRT_RAY_SORT=1 switches it on) and its buffers.
+  bool ray_sort = false;
+  uint32_t *d_sort_keys = nullptr, *d_sort_counts = nullptr, *d_sort_perm = nullptr;
+  size_t cap_sort = 0;
+  float scene_lo[3] = {0, 0, 0}, scene_inv_ext[3] = {1, 1, 1};
   uint32_t blas_leaf = 2;
   uint32_t tlas_leaf = 1;
   uint32_t lds_depth = 16;  // LDS stack entries per lane; MRT_TRACE_LDS_STACK overrides (a power of two, 4-64)

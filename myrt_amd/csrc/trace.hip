@@ -161,8 +161,9 @@ trace_kernel(TraceParams p) {
         const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
         const uint32_t r = base + rank;
         if (r < n_rays) {
-          ray_id = int(r);
-          const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(r);
+          const uint32_t rid = p.perm ? p.perm[r] : r;
+          ray_id = int(rid);
+          const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(rid);
           const float4 r0 = rp[0], r1 = rp[1];
           wox = r0.x; woy = r0.y; woz = r0.z; tmin = r0.w;
           wdx = r1.x; wdy = r1.y; wdz = r1.z; tmax = r1.w;

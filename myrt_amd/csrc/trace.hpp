@@ -26,6 +26,7 @@ struct TraceParams {
   uint32_t total_threads;
   uint32_t refill_below;           // refill a wave when fewer lanes than this are busy
   uint32_t lds_depth;              // LDS stack entries per lane (a power of two)
+  const uint32_t* perm;            // if set, queue position -> ray index (binned order)
   unsigned long long* stats;       // rays, node visits, triangle tests, instance visits
 };
 

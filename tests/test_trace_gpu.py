@@ -150,3 +150,22 @@ def test_heightfield_full_size(ensure_built):
         occ = s.trace_any(sh)
         ref_occ = np.isfinite(o.bvh(sh, any_hit=True)["t"]).astype(np.uint8)
         assert np.array_equal(occ, ref_occ), name + ": shadow occlusion differs"
+
+
+def test_ray_binning_gives_identical_hits(ensure_built, monkeypatch):
+    """Binning rays by octant and origin cell before traversal changes the order rays are taken
+    from the queue, never a result: hits equal the unbinned ones bit for bit."""
+    (idx, pts, nrm, uv), coh, inc = scenes.traversal_workload(n_coherent=1 << 16, n_incoherent=1 << 18)
+    plain = m.Scene()
+    plain.enqueue(plain.push_geometry(idx, pts, nrm, uv))
+    plain.prepare()
+    monkeypatch.setenv("MRT_RAY_SORT", "1")
+    binned = m.Scene()
+    binned.enqueue(binned.push_geometry(idx, pts, nrm, uv))
+    binned.prepare()
+    for name, rays in (("coherent", coh), ("incoherent", inc)):
+        say("binning: %s %d rays" % (name, rays.size))
+        a, b = plain.trace_closest(rays), binned.trace_closest(rays)
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
+        sh = scenes.shadow_rays(rays, a)
+        assert np.array_equal(plain.trace_any(sh), binned.trace_any(sh)), name + " shadow"
