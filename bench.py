@@ -176,6 +176,10 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
     for b in budgets:
         _, T = timed(lambda: fresh(plain).render(s, W, H, b))
         row = {"budget_spp": b, "budget_s": round(T, 4), "plain_mse": mse(plain.image())}
+        # The reference's own post filter (bilateral, the app's sigmas) as the classical baseline.
+        img, t_bl = timed(lambda: m.post_process(plain.image(), [m.POST_BILATERAL], 10.0, 0.05))
+        row["plain_bilateral_mse"] = mse(img)
+        row["bilateral_s"] = round(t_bl, 4)
         t_dn = 0.0
         if den is not None:
             img, t_dn = timed(lambda: plain.denoise(den))

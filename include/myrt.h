@@ -368,6 +368,19 @@ int mrt_denoiser_denoise_device(mrt_denoiser d, const float* d_raw, int H, int W
 int mrt_renderer_read_aux(mrt_renderer r, float* out);
 int mrt_renderer_denoise(mrt_renderer r, mrt_denoiser d, float* rgba);
 
+/* ---- post-process chain (src/pathtracer/post_process.hpp:30-60 and src/pt/pp_*.comp) --------
+ * rgba float images, row-major. MRT_POST_BILATERAL is pp_denoise.comp:24-63 (9x9 window, spatial
+ * sigma `exponent`, range sigma `strength`, neighbours clamped to [0, 1]; the app's defaults are
+ * 10 and 0.05, post_process.hpp:33-34); MRT_POST_TONEMAP_FILMIC is pp_tonemap_filmic.cl:1-5;
+ * MRT_POST_FXAA is pp_fxaa.comp:31-105 (bilinear taps clamped to the edge). */
+enum { MRT_POST_BILATERAL = 1, MRT_POST_TONEMAP_FILMIC = 2, MRT_POST_FXAA = 3 };
+/* One pass on device images (in and out distinct), asynchronous on `stream`. */
+int mrt_post_pass_device(int op, const float* d_in, float* d_out, int w, int h, float exponent, float strength,
+                         void* stream);
+/* A sequence of passes on host images. */
+int mrt_post_process(const float* rgba_in, float* rgba_out, int w, int h, const int* ops, int n_ops, float exponent,
+                     float strength);
+
 #ifdef __cplusplus
 }
 #endif

@@ -157,6 +157,10 @@ SIGNATURES = {
     "mrt_denoiser_denoise": ([_P, _P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "mrt_denoiser_denoise_device": ([_P, _P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
     "mrt_renderer_read_aux": ([_P, _P], ctypes.c_int),
+    "mrt_post_pass_device": ([ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _P],
+                             ctypes.c_int),
+    "mrt_post_process": ([_P, _P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float],
+                         ctypes.c_int),
     "mrt_renderer_denoise": ([_P, _P, _P], ctypes.c_int),
 }
 
@@ -632,6 +636,23 @@ class Renderer:
         out = np.zeros((h, w, 4), np.float32)
         _check(lib().mrt_renderer_denoise(self.h, denoiser.h, _ptr(out)))
         return out
+
+
+POST_BILATERAL, POST_TONEMAP_FILMIC, POST_FXAA = 1, 2, 3
+
+
+def post_process(rgba, ops, exponent=10.0, strength=0.05):
+    """Runs the post passes `ops` (sequence of POST_*) over an (h, w, 3 or 4) float image on the GPU;
+    returns (h, w, 4)."""
+    im = np.asarray(rgba, np.float32)
+    x = np.ones(im.shape[:2] + (4,), np.float32)
+    x[..., :im.shape[2]] = im
+    x = np.ascontiguousarray(x)
+    out = np.zeros_like(x)
+    o = np.ascontiguousarray(np.asarray(list(ops), np.int32))
+    _check(lib().mrt_post_process(_ptr(x), _ptr(out), x.shape[1], x.shape[0], _ptr(o) if o.size else None, o.size,
+                                  float(exponent), float(strength)))
+    return out
 
 
 class Denoiser:
