@@ -148,6 +148,20 @@ typedef struct mrt_bvh_s* mrt_bvh;
 typedef struct mrt_renderer_s* mrt_renderer;
 typedef struct mrt_cache_s* mrt_cache;
 typedef struct mrt_denoiser_s* mrt_denoiser;
+typedef struct mrt_obj_s* mrt_obj;
+
+/* One triangulated OBJ group with the MTL fields of its material (obj.hpp material_t). */
+typedef struct mrt_obj_group_info {
+  uint64_t n_idx;
+  uint64_t n_vtx;
+  float diffuse[3];  /* Kd */
+  float specularity; /* Ns */
+  float ior;         /* Ni */
+  float dissolve;    /* d */
+  char name[128];    /* "object/group", truncated */
+  char material_name[64];
+  char albedo_texture[256];
+} mrt_obj_group_info;
 
 /* Counters filled by mrt_scene_prepare (scene::prepare_result_t, src/pathtracer/scene.hpp:84-91,
  * extended with the device-side sizes of the wide BVH). */
@@ -242,6 +256,8 @@ int mrt_scene_push_texture(mrt_scene s, const float* rgba, int w, int h, uint32_
  * material index is what mrt_scene_enqueue_geometry takes. */
 int mrt_scene_push_material(mrt_scene s, const mrt_material* mat, uint32_t* out_id);
 int mrt_scene_set_material(mrt_scene s, uint32_t id, const mrt_material* mat);
+/* A material record as the shaders read it (scene::get_material_buffer, src/pathtracer/scene.hpp:178). */
+int mrt_scene_get_material(mrt_scene s, uint32_t id, mrt_material* out);
 int mrt_scene_set_material_param(mrt_scene s, uint32_t id, const char* name, const float* value, size_t n);
 /* scene::push_sdf(sdf_info_t{root}) (src/pathtracer/scene.cpp:189-204) with the tree as a postfix
  * program; the program is validated (stack discipline, depth <= 8, modifiers <= 4 deep). */
@@ -257,6 +273,26 @@ int mrt_scene_set_sdf_marching(mrt_scene s, int steps, float eps);
 /* Lights and the sky-gradient environment scale (src/pt/pt_color.comp:189-201). */
 int mrt_scene_set_lights(mrt_scene s, const mrt_quad_light* lights, size_t n);
 int mrt_scene_set_environment(mrt_scene s, float scale);
+
+/* ---- Wavefront OBJ/MTL ingest (src/obj.hpp, src/obj.cpp) ----------------------------------- */
+/* myrt::obj::load_obj followed by triangulate for every object (src/obj.cpp:96-361): one group
+ * per `g` / `usemtl` run, named "object/group", fan-triangulated with vertices shared per
+ * distinct (position, normal, uv). Host only. An index outside its list is an error. */
+int mrt_obj_load(const char* path, mrt_obj* out);
+int mrt_obj_destroy(mrt_obj o);
+int mrt_obj_num_groups(mrt_obj o, size_t* n);
+/* A group's sizes, name and MTL entry (material_t, src/obj.hpp); albedo_texture is map_Kd
+ * joined to the OBJ's folder, "" when there is none. */
+int mrt_obj_group(mrt_obj o, size_t i, mrt_obj_group_info* info);
+/* Copies a group's triangle indices (n_idx) and n_vtx positions, normals and uvs; any pointer
+ * may be NULL. */
+int mrt_obj_group_copy(mrt_obj o, size_t i, uint32_t* idx, float* pos3, float* nrm3, float* uv2);
+/* load_object_file (src/main.cpp:902-949): pushes every non-empty group as a geometry and a
+ * material {albedo = unorm8(Kd, 1), ior = Ni, roughness = max(0.054, (1/Ns)^(1/pi)),
+ * transmission = 1 - d}. n = the number of groups; an empty group's ids are 0xFFFFFFFF. The
+ * caller loads map_Kd images and sets the materials' albedo_texture, and enqueues the pairs
+ * with its import scale, as the reference's caller does. */
+int mrt_scene_push_obj(mrt_scene s, mrt_obj o, uint32_t* geometries, uint32_t* materials, size_t n);
 
 /* ---- ray queries (GPU) ------------------------------------------------------------------- */
 /* nearest_hit / any_hit (src/pt/pt_trace.comp:311-386) over the two-level BVH

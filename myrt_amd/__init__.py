@@ -107,6 +107,7 @@ SIGNATURES = {
     "mrt_scene_pick": ([_P, _P, _P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "mrt_scene_push_material": ([_P, _P, _U32P], ctypes.c_int),
     "mrt_scene_set_material": ([_P, ctypes.c_uint32, _P], ctypes.c_int),
+    "mrt_scene_get_material": ([_P, ctypes.c_uint32, _P], ctypes.c_int),
     "mrt_scene_set_material_param": ([_P, ctypes.c_uint32, ctypes.c_char_p, _P, _SZ], ctypes.c_int),
     "mrt_scene_set_lights": ([_P, _P, _SZ], ctypes.c_int),
     "mrt_scene_set_environment": ([_P, ctypes.c_float], ctypes.c_int),
@@ -162,6 +163,12 @@ SIGNATURES = {
     "mrt_post_process": ([_P, _P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float],
                          ctypes.c_int),
     "mrt_renderer_denoise": ([_P, _P, _P], ctypes.c_int),
+    "mrt_obj_load": ([ctypes.c_char_p, _P], ctypes.c_int),
+    "mrt_obj_destroy": ([_P], ctypes.c_int),
+    "mrt_obj_num_groups": ([_P, _P], ctypes.c_int),
+    "mrt_obj_group": ([_P, _SZ, _P], ctypes.c_int),
+    "mrt_obj_group_copy": ([_P, _SZ, _P, _P, _P, _P], ctypes.c_int),
+    "mrt_scene_push_obj": ([_P, _P, _P, _P, _SZ], ctypes.c_int),
 }
 
 
@@ -441,6 +448,12 @@ def _scene_push_material(self, mat):
     return out.value
 
 
+def _scene_get_material(self, mid):
+    out = np.zeros(1, MATERIAL_DTYPE)
+    _check(lib().mrt_scene_get_material(self.h, int(mid), _ptr(out)))
+    return out
+
+
 def _scene_set_material_param(self, mid, name, value):
     v = np.atleast_1d(np.asarray(value, np.float32))
     _check(lib().mrt_scene_set_material_param(self.h, int(mid), name.encode(), _ptr(v), v.size))
@@ -508,6 +521,7 @@ Scene.push_sdf = _scene_push_sdf
 Scene.enqueue_sdf = _scene_enqueue_sdf
 Scene.set_sdf_marching = _scene_set_sdf_marching
 Scene.push_material = _scene_push_material
+Scene.get_material = _scene_get_material
 Scene.set_parameter = _scene_set_material_param
 Scene.set_lights = _scene_set_lights
 Scene.set_environment = _scene_set_environment

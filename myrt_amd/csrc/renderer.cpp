@@ -421,6 +421,14 @@ int mrt_scene_set_material(mrt_scene sh, uint32_t id, const mrt_material* mat) {
   return MRT_OK;
 }
 
+int mrt_scene_get_material(mrt_scene sh, uint32_t id, mrt_material* out) {
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !out) return fail(MRT_ERR_INVALID, "null argument");
+  if (id >= s->materials.size()) return fail(MRT_ERR_INVALID, "unknown material");
+  std::memcpy(out, &s->materials[id], sizeof(Material));
+  return MRT_OK;
+}
+
 int mrt_scene_set_material_param(mrt_scene sh, uint32_t id, const char* name, const float* value, size_t n) {
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s || !name || !value) return fail(MRT_ERR_INVALID, "null argument");
@@ -447,6 +455,12 @@ int mrt_scene_set_material_param(mrt_scene sh, uint32_t id, const char* name, co
     m.metallic = value[0];
   } else if (k == "transmission") {
     m.transmission = value[0];
+  } else if (k == "albedo_texture") {
+    // A texture id from mrt_scene_push_texture, or 0 for none.
+    const float v = value[0];
+    if (!(v >= 0.f) || v != std::floor(v) || v > float(s->textures.size()))
+      return fail(MRT_ERR_INVALID, "albedo_texture is not a texture id");
+    m.texture = uint32_t(v);
   } else {
     return fail(MRT_ERR_INVALID, "unknown material parameter '" + k + "'");
   }
