@@ -19,7 +19,11 @@ constexpr int kBlock = 256;
 __device__ inline float normpdf(float x, float sigma) { return 0.39894f * expf(-0.5f * x * x / (sigma * sigma)) / sigma; }
 
 // pp_denoise.comp:24-63: 9x9 window, spatial sigma `exponent`, range sigma `strength`; the
-// neighbours are clamped to [0, 1], the centre is not.
+// neighbours are clamped to [0, 1], the centre is not. The range weights are taken relative to
+// the window's smallest colour distance q_min, exp(-(q - q_min) / (2 sigma_r^2)): the common
+// factor cancels in the normalised sum, so the result is the reference's wherever its fp32
+// weights do not all underflow, and stays defined where they do (a centre far above every
+// clamped neighbour, where the reference divides 0 by 0).
 __global__ __launch_bounds__(kBlock) void bilateral_kernel(const float4* in, float4* out, int w, int h, float sigma,
                                                            float bsigma) {
   const int i = int(blockIdx.x * kBlock + threadIdx.x);
@@ -29,20 +33,28 @@ __global__ __launch_bounds__(kBlock) void bilateral_kernel(const float4* in, flo
   float kernel[2 * kSize + 1];
   for (int j = 0; j <= kSize; ++j) kernel[kSize + j] = kernel[kSize - j] = normpdf(float(j), sigma);
   const float4 c = in[i];
-  const float bz = 1.0f / normpdf(0.0f, bsigma);
+  auto neighbour = [&](int di, int dj) {
+    const int sx = min(max(x + di, 0), w - 1), sy = min(max(y + dj, 0), h - 1);
+    const float4 s = in[sy * w + sx];
+    return make_float3(fminf(fmaxf(s.x, 0.f), 1.f), fminf(fmaxf(s.y, 0.f), 1.f), fminf(fmaxf(s.z, 0.f), 1.f));
+  };
+  auto dist2 = [&](float3 s) {
+    const float dr = s.x - c.x, dg = s.y - c.y, db = s.z - c.z;
+    return dr * dr + dg * dg + db * db;
+  };
+  float qmin = INFINITY;
+  for (int di = -kSize; di <= kSize; ++di)
+    for (int dj = -kSize; dj <= kSize; ++dj) qmin = fminf(qmin, dist2(neighbour(di, dj)));
+  const float inv2s2 = 0.5f / (bsigma * bsigma);
   float z = 0.f, fr = 0.f, fg = 0.f, fb = 0.f;
   for (int di = -kSize; di <= kSize; ++di) {
     for (int dj = -kSize; dj <= kSize; ++dj) {
-      const int sx = min(max(x + di, 0), w - 1), sy = min(max(y + dj, 0), h - 1);
-      const float4 s = in[sy * w + sx];
-      const float r = fminf(fmaxf(s.x, 0.f), 1.f), g = fminf(fmaxf(s.y, 0.f), 1.f), b = fminf(fmaxf(s.z, 0.f), 1.f);
-      const float dr = r - c.x, dg = g - c.y, db = b - c.z;
-      const float f = 0.39894f * expf(-0.5f * (dr * dr + dg * dg + db * db) / (bsigma * bsigma)) / bsigma * bz *
-                      kernel[kSize + dj] * kernel[kSize + di];
+      const float3 s = neighbour(di, dj);
+      const float f = expf(-(dist2(s) - qmin) * inv2s2) * kernel[kSize + dj] * kernel[kSize + di];
       z += f;
-      fr += f * r;
-      fg += f * g;
-      fb += f * b;
+      fr += f * s.x;
+      fg += f * s.y;
+      fb += f * s.z;
     }
   }
   out[i] = make_float4(fr / z, fg / z, fb / z, 1.0f);

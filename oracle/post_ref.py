@@ -9,22 +9,27 @@ def normpdf(x, sigma):
 
 
 def bilateral(img, exponent=10.0, strength=0.05):
+    """pp_denoise.comp:24-63 with the range weights relative to the window's smallest colour
+    distance q_min: exp(-(q - q_min) / (2 sigma_r^2)) times the spatial kernel. The reference's
+    factors 0.39894 / sigma_r * bZ and exp(-q_min / (2 sigma_r^2)) are common to the window and
+    cancel in the normalised sum; the shift keeps the sum non-zero where every unshifted weight
+    underflows."""
     img = np.asarray(img, np.float64)
     h, w = img.shape[:2]
     k = 4
     kern = normpdf(np.arange(-k, k + 1, dtype=np.float64), exponent)
     c = img[..., :3]
-    bz = 1.0 / normpdf(0.0, strength)
+    ys, xs = np.mgrid[0:h, 0:w]
+    offs = [(di, dj) for di in range(-k, k + 1) for dj in range(-k, k + 1)]
+    samples = [np.clip(img[np.clip(ys + dj, 0, h - 1), np.clip(xs + di, 0, w - 1), :3], 0.0, 1.0) for di, dj in offs]
+    q = [((s - c) ** 2).sum(-1) for s in samples]
+    qmin = np.min(q, axis=0)
     z = np.zeros((h, w))
     acc = np.zeros((h, w, 3))
-    ys, xs = np.mgrid[0:h, 0:w]
-    for di in range(-k, k + 1):      # x offset
-        for dj in range(-k, k + 1):  # y offset
-            s = np.clip(img[np.clip(ys + dj, 0, h - 1), np.clip(xs + di, 0, w - 1), :3], 0.0, 1.0)
-            d = s - c
-            f = 0.39894 * np.exp(-0.5 * (d * d).sum(-1) / (strength * strength)) / strength * bz * kern[dj + k] * kern[di + k]
-            z += f
-            acc += f[..., None] * s
+    for (di, dj), s, qq in zip(offs, samples, q):
+        f = np.exp(-0.5 * (qq - qmin) / (strength * strength)) * kern[dj + k] * kern[di + k]
+        z += f
+        acc += f[..., None] * s
     out = np.ones((h, w, 4))
     out[..., :3] = acc / z[..., None]
     return out
