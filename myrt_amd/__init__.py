@@ -119,6 +119,7 @@ SIGNATURES = {
     "mrt_renderer_destroy": ([_P], ctypes.c_int),
     "mrt_renderer_set_camera": ([_P, _P, _P], ctypes.c_int),
     "mrt_renderer_set_num_bounces": ([_P, ctypes.c_int], ctypes.c_int),
+    "mrt_renderer_set_material_sort": ([_P, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_invalidate": ([_P], ctypes.c_int),
     "mrt_renderer_set_dof_enabled": ([_P, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_set_lens_size": ([_P, ctypes.c_float, ctypes.c_float], ctypes.c_int),
@@ -579,6 +580,10 @@ class Renderer:
 
     def set_num_bounces(self, n):
         _check(lib().mrt_renderer_set_num_bounces(self.h, int(n)))
+
+    def set_material_sort(self, enabled=True):
+        """Shade each bounce's paths in material order; the image does not change."""
+        _check(lib().mrt_renderer_set_material_sort(self.h, 1 if enabled else 0))
 
     def invalidate(self):
         _check(lib().mrt_renderer_invalidate(self.h))

@@ -319,11 +319,12 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
   V3 sh_o = mk(0, 0, 0), sh_d = mk(0, 0, 1), sh_c = mk(0, 0, 0);
   PathState ps{};
   if (live) {
-    ps = b.paths[cur][i];
-    const float4* rp = reinterpret_cast<const float4*>(b.rays[cur]) + 2 * size_t(i);
+    const uint32_t src = b.perm ? b.perm[i] : i;  // shading order by material, when sorted
+    ps = b.paths[cur][src];
+    const float4* rp = reinterpret_cast<const float4*>(b.rays[cur]) + 2 * size_t(src);
     const float4 r0 = rp[0], r1 = rp[1];
     const V3 ro = mk(r0.x, r0.y, r0.z), rd = mk(r1.x, r1.y, r1.z);
-    const uint4 hit = reinterpret_cast<const uint4*>(b.hits)[i];
+    const uint4 hit = reinterpret_cast<const uint4*>(b.hits)[src];
     const bool found = hit.z != 0xFFFFFFFFu;
     const float t_hit = found ? __uint_as_float(hit.x) : __int_as_float(0x7f800000);
     const uint32_t bounce = ps.flags & 0xffu;

@@ -146,6 +146,7 @@ struct PtBuffers {
                           // [4] cache queries; [5] training samples; [8..9] rays traced (u64)
   float4* accum;          // per pixel running sum (rgb); w unused
   float4* aux;            // per pixel 2 x float4 of primary-hit sums: albedo, t; normal, hits
+  const uint32_t* perm;   // shading position -> queue entry (material order), or null
   uint32_t capacity;
   PtCache cache;
 };

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -17,6 +18,7 @@
 #include "../../include/myrt.h"
 #include "cache_internal.hpp"
 #include "denoiser_internal.hpp"
+#include "matsort.hpp"
 #include "pathtrace.hpp"
 #include "scene_internal.hpp"
 
@@ -134,8 +136,13 @@ struct Renderer {
   float* batch_t = nullptr;
   size_t c_tp = 0, c_tv = 0, c_q = 0, c_qr = 0, c_qo = 0, c_bq = 0, c_bt = 0;
   float last_loss = 0.f;
+  // Shading in material order (MRT_MATERIAL_SORT or mrt_renderer_set_material_sort).
+  bool material_sort = false;
+  uint32_t *ms_keys = nullptr, *ms_perm = nullptr, *ms_counts = nullptr;
+  size_t c_msk = 0, c_msp = 0, c_msc = 0;
 
   Renderer() {
+    if (const char* v = std::getenv("MRT_MATERIAL_SORT")) material_sort = std::strtol(v, nullptr, 10) != 0;
     std::memset(view, 0, sizeof(view));
     std::memset(proj, 0, sizeof(proj));
     for (int i = 0; i < 4; ++i) view[i * 5] = proj[i * 5] = 1.f;
@@ -143,7 +150,8 @@ struct Renderer {
   ~Renderer() {
     for (void* p : {rays[0], rays[1], (void*)paths[0], (void*)paths[1], hits, shadow_rays, (void*)shadow,
                     (void*)occluded, (void*)counts, (void*)accum, (void*)tpaths, (void*)tverts, (void*)queries,
-                    (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den, (void*)bokeh, (void*)cubemap})
+                    (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den, (void*)bokeh, (void*)cubemap, (void*)ms_keys, (void*)ms_perm,
+                    (void*)ms_counts})
       if (p) hipFree(p);
   }
 
@@ -204,6 +212,9 @@ struct Renderer {
       if (e == hipSuccess) e = hipMemsetAsync(aux, 0, 2 * npix * sizeof(float4), stream);
       if (e != hipSuccess) return hip_fail(e, "clearing the accumulator");
     }
+    if (material_sort && ((rc = ensure(ms_keys, c_msk, cap)) || (rc = ensure(ms_perm, c_msp, cap)) ||
+                          (rc = ensure(ms_counts, c_msc, kMatSortBins))))
+      return rc;
     PtCache pc{};
     const bool use_cache = cache && cache_mode > 0;
     if (use_cache) {
@@ -268,6 +279,11 @@ struct Renderer {
         if (e == hipSuccess) e = hipMemsetAsync(counts + 2, 0, 4, stream);
         if (e != hipSuccess) return hip_fail(e, "clearing queue counters");
         if ((rc = s.trace(rays[cur], n_paths, hits, false, false, stream, counts + cur)) != MRT_OK) return rc;
+        if (material_sort) {
+          MatSortArgs ma{hits, counts + cur, n_paths, sc.inst_material, ms_keys, ms_counts, ms_perm};
+          if (launch_material_sort(ma, stream)) return fail(MRT_ERR_HIP, "material sort");
+          b.perm = ms_perm;
+        }
         if (launch_pt_shade(sc, b, cur, uint32_t(num_bounces), stream)) return fail(MRT_ERR_HIP, "shade");
         if ((rc = s.trace(shadow_rays, n_paths, occluded, true, false, stream, counts + 2)) != MRT_OK) return rc;
         if (launch_pt_resolve(b, stream)) return fail(MRT_ERR_HIP, "resolve");
@@ -573,6 +589,13 @@ int mrt_renderer_set_num_bounces(mrt_renderer rh, int n) {
   if (!r || n < 1 || n > 64) return fail(MRT_ERR_INVALID, "bounces must be in [1, 64]");
   r->num_bounces = n;
   r->sample_count = 0;
+  return MRT_OK;
+}
+
+int mrt_renderer_set_material_sort(mrt_renderer rh, int enabled) {
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r) return fail(MRT_ERR_INVALID, "null renderer");
+  r->material_sort = enabled != 0;
   return MRT_OK;
 }
 

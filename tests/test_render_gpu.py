@@ -193,3 +193,26 @@ def test_textures_and_cubemap_match_fp64_oracle(ensure_built):
     err = rel_mse(img, ref)
     say("textured relative MSE %.3e" % err)
     assert err <= REL_MSE_TOL
+
+
+def test_material_sorted_shading_gives_the_same_image(ensure_built):
+    """Shading each bounce in material order reorders only the atomic adds into the accumulator:
+    the image equals the unsorted one up to fp32 summation order."""
+    sc = scenes.sphere_field_scene(40, 2, seed=3, glass_every=4, metal_every=3)
+    W, H, spp, depth = 96, 64, 16, 6
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
+    imgs = []
+    for sort in (False, True):
+        s = gpu_scene(sc)
+        r = m.Renderer()
+        r.set_camera(view, proj)
+        r.set_num_bounces(depth)
+        r.set_material_sort(sort)
+        say("render %dx%d %d spp, material sort %s" % (W, H, spp, sort))
+        r.render(s, W, H, spp)
+        imgs.append(r.image()[..., :3].astype(np.float64))
+        s.close()
+    diff = np.abs(imgs[0] - imgs[1])
+    say("max difference %.3e, image mean %.3f" % (diff.max(), imgs[0].mean()))
+    assert imgs[0].mean() > 0.01
+    assert np.all(diff <= 1e-5 * np.maximum(1.0, np.abs(imgs[0])))
