@@ -324,7 +324,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   if (!prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
   if (n == 0) return MRT_OK;
   if (n >= 0x7fffffffu) return fail(MRT_ERR_LIMIT, "at most 2^31-1 rays per call");
-  const int per_cu = trace_blocks_per_cu(any, stats, lds_depth);
+  const int per_cu = trace_blocks_per_cu(any, stats, postpone != 0, lds_depth);
   const size_t max_grid = size_t(num_cus) * size_t(per_cu);
   const size_t want = (n + kTraceBlock - 1) / kTraceBlock;
   const int grid = int(std::max<size_t>(1, std::min(max_grid, want)));
@@ -361,6 +361,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.refill_below = refill_below;
   p.lds_depth = lds_depth;
   p.perm = nullptr;
+  p.postpone = postpone;
   if (ray_sort && n >= 4096) {
     if (cap_sort < n || !d_sort_counts) {
       for (uint32_t* q : {d_sort_keys, d_sort_perm, d_sort_counts})
@@ -633,6 +634,10 @@ int mrt_scene_create(mrt_scene* out) {
   if (const char* v = std::getenv("MRT_TRACE_REFILL")) {
     const long r = std::strtol(v, nullptr, 10);
     if (r >= 1 && r <= 64) s->refill_below = uint32_t(r);
+  }
+  if (const char* v = std::getenv("MRT_TRACE_POSTPONE")) {
+    const long k = std::strtol(v, nullptr, 10);
+    if (k >= 0 && k <= 64) s->postpone = uint32_t(k);
   }
   *out = reinterpret_cast<mrt_scene>(s);
   return MRT_OK;

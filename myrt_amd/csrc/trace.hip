@@ -100,7 +100,7 @@ __device__ inline uint32_t pack_bary(float b1, float b2) {
 __device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 __device__ inline float byte_of(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
 
-template <bool ANY, bool STATS>
+template <bool ANY, bool STATS, bool POSTPONE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRT_TRACE_WAVES, MRT_TRACE_WAVES))) void
 trace_kernel(TraceParams p) {
   extern __shared__ uint32_t lds_stack[];  // [lds_depth][kBlock]
@@ -120,6 +120,8 @@ trace_kernel(TraceParams p) {
   float t_best = 0, b1_best = 0, b2_best = 0;
   uint32_t inst_best = 0, prim_best = 0, cur_inst = 0;
   uint32_t entry = 0, sp = 0;
+  uint32_t leaf = 0;                        // parked triangle leaf (leaf postponing)
+  bool have_leaf = false, have_entry = false;
   int level = 0;
   bool found = false;
   uint32_t n_nodes = 0, n_tris = 0, n_inst = 0;
@@ -178,6 +180,8 @@ trace_kernel(TraceParams p) {
           entry = 0;  // TLAS root
           if (STATS) n_nodes = n_tris = n_inst = 0;
           if (!p.has_tlas || !(wdx != 0.f || wdy != 0.f || wdz != 0.f)) entry = kStackReturn;  // finishes at once
+          have_entry = true;
+          have_leaf = false;
         }
       }
     }
@@ -185,150 +189,217 @@ trace_kernel(TraceParams p) {
     if (__ballot(ray_id >= 0) == 0) break;
 
     // ---- traverse until too many lanes are idle ----
-    for (;;) {
-      if (ray_id >= 0) {
-        bool have_next = false;
-        uint32_t next = 0;
-        bool done = false;
-        if (entry == kStackReturn) {
-          if (level == 0) {
-            done = true;  // no scene or degenerate ray
-          } else {
-            level = 0;
-            ox = wox; oy = woy; oz = woz; ix = wix; iy = wiy; iz = wiz;
-          }
-        } else if (entry & 0x80000000u) {
-          const uint32_t cnt = (entry >> 24) & 0x7fu;
-          const uint32_t start = entry & 0xffffffu;
-          if (level == 0) {
-            // Enter an instance: the remaining instances of this leaf stay on the stack.
-            const uint32_t inst = p.tlas_items[start];
-            if (cnt > 1) push(0x80000000u | ((cnt - 1) << 24) | (start + 1));
-            push(kStackReturn);
-            const float4* ip = reinterpret_cast<const float4*>(p.instances + inst);
-            const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
-            const uint4 meta = reinterpret_cast<const uint4*>(ip)[3];
-            ox = m0.x * wox + m0.y * woy + m0.z * woz + m0.w;
-            oy = m1.x * wox + m1.y * woy + m1.z * woz + m1.w;
-            oz = m2.x * wox + m2.y * woy + m2.z * woz + m2.w;
-            const float dx = m0.x * wdx + m0.y * wdy + m0.z * wdz;
-            const float dy = m1.x * wdx + m1.y * wdy + m1.z * wdz;
-            const float dz = m2.x * wdx + m2.y * wdy + m2.z * wdz;
-            ix = 1.0f / dx; iy = 1.0f / dy; iz = 1.0f / dz;
-            make_pre(ox, oy, oz, dx, dy, dz, pre);
-            cur_inst = meta.z;
-            level = 1;
-            next = meta.x;  // BLAS root node
-            have_next = true;
-            if (STATS) ++n_inst;
-          } else {
-            const float4* tp = reinterpret_cast<const float4*>(p.tris) + 3 * size_t(start);
-            for (uint32_t k = 0; k < cnt; ++k) {
-              const float4 a = tp[3 * k], b = tp[3 * k + 1], c = tp[3 * k + 2];
-              if (STATS) ++n_tris;
-              float t, b1, b2;
-              if (!watertight(pre, a, b, c, t, b1, b2)) continue;
-              if (!(t > tmin && t < tmax)) continue;
-              const uint32_t prim = __float_as_uint(a.w);
-              if (ANY) {
-                found = true;
-                done = true;
-                break;
-              }
-              if (t < t_best || (t == t_best && (cur_inst < inst_best || (cur_inst == inst_best && prim < prim_best)))) {
-                t_best = t;
-                inst_best = cur_inst;
-                prim_best = prim;
-                b1_best = b1;
-                b2_best = b2;
-                found = true;
-              }
-            }
-          }
+    // One step on an entry that is not a triangle leaf: an inner node (test the four children,
+    // push the far ones, continue with the nearest), an instance leaf of the TLAS (move the ray
+    // into object space and continue at the BLAS root) or a RETURN marker (back to world space).
+    const auto node_step = [&](uint32_t e, bool& have_next, uint32_t& next, bool& done) {
+      if (e == kStackReturn) {
+        if (level == 0) {
+          done = true;  // no scene or degenerate ray
         } else {
-          // ---- inner node: test the four quantised child boxes ----
-          const WideNode* nodes = level ? p.blas_nodes : p.tlas_nodes;
-          const uint4* np = reinterpret_cast<const uint4*>(nodes + entry);
-          const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
-          if (STATS) ++n_nodes;
-          const float orx = __uint_as_float(q0.x), ory = __uint_as_float(q0.y), orz = __uint_as_float(q0.z);
-          const float scx = exp2i(int(int8_t(q0.w & 0xffu)));
-          const float scy = exp2i(int(int8_t((q0.w >> 8) & 0xffu)));
-          const float scz = exp2i(int(int8_t((q0.w >> 16) & 0xffu)));
-          const uint32_t nch = q0.w >> 24;
-          float key[4];
-          uint32_t ent[4];
+          level = 0;
+          ox = wox; oy = woy; oz = woz; ix = wix; iy = wiy; iz = wiz;
+        }
+      } else if (e & 0x80000000u) {
+        // Enter an instance: the remaining instances of this leaf stay on the stack.
+        const uint32_t cnt = (e >> 24) & 0x7fu;
+        const uint32_t start = e & 0xffffffu;
+        const uint32_t inst = p.tlas_items[start];
+        if (cnt > 1) push(0x80000000u | ((cnt - 1) << 24) | (start + 1));
+        push(kStackReturn);
+        const float4* ip = reinterpret_cast<const float4*>(p.instances + inst);
+        const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
+        const uint4 meta = reinterpret_cast<const uint4*>(ip)[3];
+        ox = m0.x * wox + m0.y * woy + m0.z * woz + m0.w;
+        oy = m1.x * wox + m1.y * woy + m1.z * woz + m1.w;
+        oz = m2.x * wox + m2.y * woy + m2.z * woz + m2.w;
+        const float dx = m0.x * wdx + m0.y * wdy + m0.z * wdz;
+        const float dy = m1.x * wdx + m1.y * wdy + m1.z * wdz;
+        const float dz = m2.x * wdx + m2.y * wdy + m2.z * wdz;
+        ix = 1.0f / dx; iy = 1.0f / dy; iz = 1.0f / dz;
+        make_pre(ox, oy, oz, dx, dy, dz, pre);
+        cur_inst = meta.z;
+        level = 1;
+        next = meta.x;  // BLAS root node
+        have_next = true;
+        if (STATS) ++n_inst;
+      } else {
+        // ---- inner node: test the four quantised child boxes ----
+        const WideNode* nodes = level ? p.blas_nodes : p.tlas_nodes;
+        const uint4* np = reinterpret_cast<const uint4*>(nodes + e);
+        const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
+        if (STATS) ++n_nodes;
+        const float orx = __uint_as_float(q0.x), ory = __uint_as_float(q0.y), orz = __uint_as_float(q0.z);
+        const float scx = exp2i(int(int8_t(q0.w & 0xffu)));
+        const float scy = exp2i(int(int8_t((q0.w >> 8) & 0xffu)));
+        const float scz = exp2i(int(int8_t((q0.w >> 16) & 0xffu)));
+        const uint32_t nch = q0.w >> 24;
+        float key[4];
+        uint32_t ent[4];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const float lx = fmaf(byte_of(q1.z, c), scx, orx), hx = fmaf(byte_of(q2.y, c), scx, orx);
-            const float ly = fmaf(byte_of(q1.w, c), scy, ory), hy = fmaf(byte_of(q2.z, c), scy, ory);
-            const float lz = fmaf(byte_of(q2.x, c), scz, orz), hz = fmaf(byte_of(q2.w, c), scz, orz);
-            const float tx0 = (lx - ox) * ix, tx1 = (hx - ox) * ix;
-            const float ty0 = (ly - oy) * iy, ty1 = (hy - oy) * iy;
-            const float tz0 = (lz - oz) * iz, tz1 = (hz - oz) * iz;
-            const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-            const float tf = fminf(fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)), t_best) * kRobust;
-            const bool hit = (uint32_t(c) < nch) && tn <= tf;
-            const uint32_t cnt = (q3.x >> (8 * c)) & 0xffu;
-            const uint32_t off = (c < 2 ? (q3.y >> (16 * c)) : (q3.z >> (16 * (c - 2)))) & 0xffffu;
-            ent[c] = cnt == 0 ? (q1.x + off) : (0x80000000u | (cnt << 24) | (q1.y + off));
-            key[c] = hit ? tn : __int_as_float(0x7f800000);
-          }
-          // Sort the four (distance, entry) pairs ascending; misses carry +inf and sort last.
+        for (int c = 0; c < 4; ++c) {
+          const float lx = fmaf(byte_of(q1.z, c), scx, orx), hx = fmaf(byte_of(q2.y, c), scx, orx);
+          const float ly = fmaf(byte_of(q1.w, c), scy, ory), hy = fmaf(byte_of(q2.z, c), scy, ory);
+          const float lz = fmaf(byte_of(q2.x, c), scz, orz), hz = fmaf(byte_of(q2.w, c), scz, orz);
+          const float tx0 = (lx - ox) * ix, tx1 = (hx - ox) * ix;
+          const float ty0 = (ly - oy) * iy, ty1 = (hy - oy) * iy;
+          const float tz0 = (lz - oz) * iz, tz1 = (hz - oz) * iz;
+          const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+          const float tf = fminf(fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)), t_best) * kRobust;
+          const bool hit = (uint32_t(c) < nch) && tn <= tf;
+          const uint32_t cnt = (q3.x >> (8 * c)) & 0xffu;
+          const uint32_t off = (c < 2 ? (q3.y >> (16 * c)) : (q3.z >> (16 * (c - 2)))) & 0xffffu;
+          ent[c] = cnt == 0 ? (q1.x + off) : (0x80000000u | (cnt << 24) | (q1.y + off));
+          key[c] = hit ? tn : __int_as_float(0x7f800000);
+        }
+        // Sort the four (distance, entry) pairs ascending; misses carry +inf and sort last.
 #define MRT_CSWAP(i, j)                                  \
   if (key[j] < key[i]) {                                 \
     const float tk = key[i]; key[i] = key[j]; key[j] = tk; \
     const uint32_t te = ent[i]; ent[i] = ent[j]; ent[j] = te; \
   }
-          MRT_CSWAP(0, 1) MRT_CSWAP(2, 3) MRT_CSWAP(0, 2) MRT_CSWAP(1, 3) MRT_CSWAP(1, 2)
+        MRT_CSWAP(0, 1) MRT_CSWAP(2, 3) MRT_CSWAP(0, 2) MRT_CSWAP(1, 3) MRT_CSWAP(1, 2)
 #undef MRT_CSWAP
-          const float inf = __int_as_float(0x7f800000);
-          // A box whose near distance is +inf is never reached (a zero direction component with
-          // the origin outside that slab), so it counts as missed.
-          uint32_t nhit = 0;
+        const float inf = __int_as_float(0x7f800000);
+        // A box whose near distance is +inf is never reached (a zero direction component with
+        // the origin outside that slab), so it counts as missed.
+        uint32_t nhit = 0;
 #pragma unroll
-          for (int c = 0; c < 4; ++c) nhit += (key[c] < inf) ? 1u : 0u;
-          if (nhit > 3) push(ent[3]);
-          if (nhit > 2) push(ent[2]);
-          if (nhit > 1) push(ent[1]);
-          if (nhit > 0) {
-            next = ent[0];
-            have_next = true;
-          }
-        }
-        if (!done && !have_next) {
-          if (sp == 0) {
-            done = true;
-          } else {
-            next = pop();
-            have_next = true;
-          }
-        }
-        if (done) {
-          if (ANY) {
-            reinterpret_cast<uint8_t*>(p.out)[ray_id] = found ? 1 : 0;
-          } else {
-            uint4 h;
-            h.x = found ? __float_as_uint(t_best) : 0x7f800000u;
-            h.y = found ? prim_best : 0xFFFFFFFFu;
-            h.z = found ? inst_best : 0xFFFFFFFFu;
-            h.w = found ? pack_bary(b1_best, b2_best) : 0u;
-            reinterpret_cast<uint4*>(p.out)[ray_id] = h;
-          }
-          if (STATS) {
-            atomicAdd(p.stats + 0, 1ull);
-            atomicAdd(p.stats + 1, (unsigned long long)n_nodes);
-            atomicAdd(p.stats + 2, (unsigned long long)n_tris);
-            atomicAdd(p.stats + 3, (unsigned long long)n_inst);
-          }
-          ray_id = -1;
-        } else {
-          entry = next;
+        for (int c = 0; c < 4; ++c) nhit += (key[c] < inf) ? 1u : 0u;
+        if (nhit > 3) push(ent[3]);
+        if (nhit > 2) push(ent[2]);
+        if (nhit > 1) push(ent[1]);
+        if (nhit > 0) {
+          next = ent[0];
+          have_next = true;
         }
       }
-      const uint64_t active = __ballot(ray_id >= 0);
-      if (active == 0) break;
-      if (!queue_empty && __popcll(active) < p.refill_below) break;
+    };
+    // The triangles of a BLAS leaf; returns true when an any-hit query is answered.
+    const auto leaf_step = [&](uint32_t e) -> bool {
+      const uint32_t cnt = (e >> 24) & 0x7fu;
+      const uint32_t start = e & 0xffffffu;
+      const float4* tp = reinterpret_cast<const float4*>(p.tris) + 3 * size_t(start);
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const float4 a = tp[3 * k], b = tp[3 * k + 1], c = tp[3 * k + 2];
+        if (STATS) ++n_tris;
+        float t, b1, b2;
+        if (!watertight(pre, a, b, c, t, b1, b2)) continue;
+        if (!(t > tmin && t < tmax)) continue;
+        const uint32_t prim = __float_as_uint(a.w);
+        if (ANY) {
+          found = true;
+          return true;
+        }
+        if (t < t_best || (t == t_best && (cur_inst < inst_best || (cur_inst == inst_best && prim < prim_best)))) {
+          t_best = t;
+          inst_best = cur_inst;
+          prim_best = prim;
+          b1_best = b1;
+          b2_best = b2;
+          found = true;
+        }
+      }
+      return false;
+    };
+    const auto finish = [&]() {
+      if (ANY) {
+        reinterpret_cast<uint8_t*>(p.out)[ray_id] = found ? 1 : 0;
+      } else {
+        uint4 h;
+        h.x = found ? __float_as_uint(t_best) : 0x7f800000u;
+        h.y = found ? prim_best : 0xFFFFFFFFu;
+        h.z = found ? inst_best : 0xFFFFFFFFu;
+        h.w = found ? pack_bary(b1_best, b2_best) : 0u;
+        reinterpret_cast<uint4*>(p.out)[ray_id] = h;
+      }
+      if (STATS) {
+        atomicAdd(p.stats + 0, 1ull);
+        atomicAdd(p.stats + 1, (unsigned long long)n_nodes);
+        atomicAdd(p.stats + 2, (unsigned long long)n_tris);
+        atomicAdd(p.stats + 3, (unsigned long long)n_inst);
+      }
+      ray_id = -1;
+    };
+    const auto is_tri_leaf = [&](uint32_t e) { return level == 1 && e != kStackReturn && (e & 0x80000000u); };
+
+    if (!POSTPONE) {
+      // Every lane processes its current entry each iteration, whatever its kind.
+      for (;;) {
+        if (ray_id >= 0) {
+          bool have_next = false, done = false;
+          uint32_t next = 0;
+          if (is_tri_leaf(entry)) {
+            done = leaf_step(entry);
+          } else {
+            node_step(entry, have_next, next, done);
+          }
+          if (!done && !have_next) {
+            if (sp == 0) {
+              done = true;
+            } else {
+              next = pop();
+              have_next = true;
+            }
+          }
+          if (done) {
+            finish();
+          } else {
+            entry = next;
+          }
+        }
+        const uint64_t active = __ballot(ray_id >= 0);
+        if (active == 0) break;
+        if (!queue_empty && __popcll(active) < p.refill_below) break;
+      }
+    } else {
+      // Leaf postponing (Aila and Laine, HPG 2009): a lane that reaches a triangle leaf parks it
+      // and keeps traversing inner nodes of the same instance; the wave tests parked leaves
+      // together once no busy lane is still looking for one, or once p.postpone lanes cannot
+      // move without theirs. A lane never leaves its instance with a leaf parked.
+      for (;;) {
+        bool stuck = false;
+        if (ray_id >= 0) {
+          if (have_entry && !have_leaf && is_tri_leaf(entry)) {
+            leaf = entry;
+            have_leaf = true;
+            have_entry = sp > 0;
+            if (have_entry) entry = pop();
+          }
+          const bool blocked = !have_entry || is_tri_leaf(entry) || (have_leaf && entry == kStackReturn);
+          if (!blocked) {
+            bool have_next = false, done = false;
+            uint32_t next = 0;
+            node_step(entry, have_next, next, done);
+            if (done) {
+              have_entry = false;  // level 0 never holds a parked leaf
+            } else {
+              if (!have_next && sp > 0) {
+                next = pop();
+                have_next = true;
+              }
+              have_entry = have_next;
+              entry = next;
+            }
+          }
+          if (!have_entry && !have_leaf) {
+            finish();
+          } else {
+            stuck = have_leaf && (!have_entry || is_tri_leaf(entry) || entry == kStackReturn);
+          }
+        }
+        const uint64_t searching = __ballot(ray_id >= 0 && !have_leaf);
+        const uint64_t stuck_mask = __ballot(stuck);
+        if (searching == 0 || uint32_t(__popcll(stuck_mask)) >= p.postpone) {
+          if (ray_id >= 0 && have_leaf) {
+            have_leaf = false;
+            if (leaf_step(leaf) || !have_entry) finish();
+          }
+        }
+        const uint64_t active = __ballot(ray_id >= 0);
+        if (active == 0) break;
+        if (!queue_empty && __popcll(active) < p.refill_below) break;
+      }
     }
   }
 }
@@ -338,30 +409,36 @@ trace_kernel(TraceParams p) {
 int launch_trace(const TraceParams& p, bool any, bool stats, int grid, hipStream_t stream) {
   const dim3 block(dev::kBlock), g(grid);
   const size_t lds = size_t(p.lds_depth) * dev::kBlock * 4;
+  const bool pp = p.postpone != 0;
+#define MRT_LAUNCH(A, S, P) hipLaunchKernelGGL((dev::trace_kernel<A, S, P>), g, block, lds, stream, p)
   if (any) {
-    if (stats)
-      hipLaunchKernelGGL((dev::trace_kernel<true, true>), g, block, lds, stream, p);
-    else
-      hipLaunchKernelGGL((dev::trace_kernel<true, false>), g, block, lds, stream, p);
+    if (stats) {
+      if (pp) MRT_LAUNCH(true, true, true); else MRT_LAUNCH(true, true, false);
+    } else {
+      if (pp) MRT_LAUNCH(true, false, true); else MRT_LAUNCH(true, false, false);
+    }
   } else {
-    if (stats)
-      hipLaunchKernelGGL((dev::trace_kernel<false, true>), g, block, lds, stream, p);
-    else
-      hipLaunchKernelGGL((dev::trace_kernel<false, false>), g, block, lds, stream, p);
+    if (stats) {
+      if (pp) MRT_LAUNCH(false, true, true); else MRT_LAUNCH(false, true, false);
+    } else {
+      if (pp) MRT_LAUNCH(false, false, true); else MRT_LAUNCH(false, false, false);
+    }
   }
+#undef MRT_LAUNCH
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int trace_blocks_per_cu(bool any, bool stats, uint32_t lds_depth) {
+int trace_blocks_per_cu(bool any, bool stats, bool postpone, uint32_t lds_depth) {
   int blocks = 0;
   const size_t lds = size_t(lds_depth) * dev::kBlock * 4;
-  hipError_t e;
+  const void* k;
   if (any)
-    e = stats ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<true, true>, dev::kBlock, lds)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<true, false>, dev::kBlock, lds);
+    k = stats ? (postpone ? (const void*)dev::trace_kernel<true, true, true> : (const void*)dev::trace_kernel<true, true, false>)
+              : (postpone ? (const void*)dev::trace_kernel<true, false, true> : (const void*)dev::trace_kernel<true, false, false>);
   else
-    e = stats ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<false, true>, dev::kBlock, lds)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dev::trace_kernel<false, false>, dev::kBlock, lds);
+    k = stats ? (postpone ? (const void*)dev::trace_kernel<false, true, true> : (const void*)dev::trace_kernel<false, true, false>)
+              : (postpone ? (const void*)dev::trace_kernel<false, false, true> : (const void*)dev::trace_kernel<false, false, false>);
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, dev::kBlock, lds);
   if (e != hipSuccess || blocks < 1) blocks = 1;
   return blocks;
 }
