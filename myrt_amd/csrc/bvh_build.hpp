@@ -68,4 +68,8 @@ std::vector<Box> triangle_bounds(const uint32_t* idx, size_t n_idx, const float*
 // bvh::bvh(aabbs, split) (src/pathtracer/bvh.cpp:154-161). split_aabbs: box-clip wide references.
 BinaryBvh build_binary_bvh(const std::vector<Box>& boxes, bool split_aabbs, unsigned threads = 0);
 
+// Traversal-quality BVH (sah_build.cpp): binned/sweep SAH with leaves of at most max_leaf
+// references; no reference duplication. Used for the trees the GPU walks.
+BinaryBvh build_sah_bvh(const std::vector<Box>& boxes, uint32_t max_leaf, unsigned threads = 0);
+
 }  // namespace mrt

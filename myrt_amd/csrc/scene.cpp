@@ -189,7 +189,7 @@ int Scene::build_host() {
           scene_lo[a] = all.mn[a];
           scene_inv_ext[a] = ext > 0.f && std::isfinite(ext) ? 1.0f / ext : 1.0f;
         }
-        tlas = build_binary_bvh(boxes, false);
+        tlas = sah_builder ? build_sah_bvh(boxes, tlas_leaf) : build_binary_bvh(boxes, false);
         WideBvh w = collapse_to_wide(tlas, kTlasPadRel, tlas_leaf);
         tlas_nodes = std::move(w.nodes);
         tlas_depth = w.depth;
@@ -635,6 +635,7 @@ int mrt_scene_create(mrt_scene* out) {
     const long r = std::strtol(v, nullptr, 10);
     if (r >= 1 && r <= 64) s->refill_below = uint32_t(r);
   }
+  if (const char* v = std::getenv("MRT_BVH_BUILDER")) s->sah_builder = std::strtol(v, nullptr, 10) != 0;
   if (const char* v = std::getenv("MRT_TRACE_POSTPONE")) {
     const long k = std::strtol(v, nullptr, 10);
     if (k >= 0 && k <= 64) s->postpone = uint32_t(k);
@@ -662,7 +663,8 @@ int mrt_scene_push_geometry(mrt_scene sh, const uint32_t* idx, size_t n_idx, con
   if (nrm3) g.nrm.assign(nrm3, nrm3 + 3 * n_vtx);
   if (uv2) g.uv.assign(uv2, uv2 + 2 * n_vtx);
   else g.uv.assign(2 * n_vtx, 0.f);  // scene.cpp:55-58 zero-fills missing uvs
-  g.bvh = build_binary_bvh(triangle_bounds(idx, n_idx, pos3), true);
+  g.bvh = s->sah_builder ? build_sah_bvh(triangle_bounds(idx, n_idx, pos3), s->blas_leaf)
+                         : build_binary_bvh(triangle_bounds(idx, n_idx, pos3), true);
   g.box.mn = {g.bvh.nodes[0].mn[0], g.bvh.nodes[0].mn[1], g.bvh.nodes[0].mn[2]};
   g.box.mx = {g.bvh.nodes[0].mx[0], g.bvh.nodes[0].mx[1], g.bvh.nodes[0].mx[2]};
   g.wide = collapse_to_wide(g.bvh, 1.0 / (1 << 20), s->blas_leaf);

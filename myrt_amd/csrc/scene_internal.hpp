@@ -108,6 +108,7 @@ struct Scene {
   size_t spill_cap = 0;
   int num_cus = 0;
   uint32_t refill_below = 40;
+  bool sah_builder = true;  // traversal trees from build_sah_bvh (MRT_BVH_BUILDER=0: the reference builder's)
   uint32_t postpone = 0;  // leaf postponing: 0 off, else the stuck-lane count that runs the leaf phase
   // Largest wide leaf: binary leaves are split at centroid medians down to this many triangles
   // (instances for the TLAS). Measured best: 2 triangles, 1 instance. MRT_BVH_LEAF and

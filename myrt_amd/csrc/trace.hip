@@ -27,7 +27,14 @@ constexpr int kBlock = 256;
 #ifndef MRT_TRACE_WAVES
 #define MRT_TRACE_WAVES 5
 #endif
-constexpr float kRobust = 1.0000005f;  // > 1 + 2*gamma(3) for binary32
+constexpr float kRobust = 1.0000005f;  // 1 + 8 ulp: covers the relative rounding of the slab test
+constexpr float kSlack = 1.0f / 1048576.0f;  // 2^-20 of |b|: covers the absolute rounding of b
+constexpr float kSlackFloor = 7.8886091e-31f;  // 2^-100: covers a plane term that underflows
+constexpr float kInvMax = 18446744073709551616.0f;  // 2^64: |1/d| is clamped here for box tests
+
+// 1/d for box tests, clamped to [-2^64, 2^64]: a zero component gives +-2^64 instead of +-inf (a
+// NaN component gives -2^64), so every plane distance below stays finite.
+__device__ inline float box_inv(float d) { return fminf(fmaxf(1.0f / d, -kInvMax), kInvMax); }
 
 struct RayPre {  // per-space ray data for the watertight test
   float ox, oy, oz;   // origin permuted to (kx, ky, kz)
@@ -169,7 +176,7 @@ trace_kernel(TraceParams p) {
           const float4 r0 = rp[0], r1 = rp[1];
           wox = r0.x; woy = r0.y; woz = r0.z; tmin = r0.w;
           wdx = r1.x; wdy = r1.y; wdz = r1.z; tmax = r1.w;
-          wix = 1.0f / wdx; wiy = 1.0f / wdy; wiz = 1.0f / wdz;
+          wix = box_inv(wdx); wiy = box_inv(wdy); wiz = box_inv(wdz);
           ox = wox; oy = woy; oz = woz; ix = wix; iy = wiy; iz = wiz;
           t_best = tmax;
           inst_best = prim_best = 0xFFFFFFFFu;
@@ -216,7 +223,7 @@ trace_kernel(TraceParams p) {
         const float dx = m0.x * wdx + m0.y * wdy + m0.z * wdz;
         const float dy = m1.x * wdx + m1.y * wdy + m1.z * wdz;
         const float dz = m2.x * wdx + m2.y * wdy + m2.z * wdz;
-        ix = 1.0f / dx; iy = 1.0f / dy; iz = 1.0f / dz;
+        ix = box_inv(dx); iy = box_inv(dy); iz = box_inv(dz);
         make_pre(ox, oy, oz, dx, dy, dz, pre);
         cur_inst = meta.z;
         level = 1;
@@ -234,18 +241,29 @@ trace_kernel(TraceParams p) {
         const float scy = exp2i(int(int8_t((q0.w >> 8) & 0xffu)));
         const float scz = exp2i(int(int8_t((q0.w >> 16) & 0xffu)));
         const uint32_t nch = q0.w >> 24;
+        // A child plane at origin + q * scale is crossed at t = q * a + b, with a = scale * i
+        // (exact: scale is a power of two) and b = (origin - o) * i. Computing b rounds twice,
+        // an absolute error of at most 2.2 ulp of |b|; the near planes use b - S and the far
+        // planes b + S with S = 2^-20 |b| + 2^-100, so no box that the exact slab test accepts
+        // is culled. The remaining relative rounding is covered by kRobust on the far side. The
+        // direction signs pick which quantised plane is near on each axis.
+        const float ax = scx * ix, ay = scy * iy, az = scz * iz;
+        const float bx = (orx - ox) * ix, by = (ory - oy) * iy, bz = (orz - oz) * iz;
+        const float sxx = fmaf(fabsf(bx), kSlack, kSlackFloor), syy = fmaf(fabsf(by), kSlack, kSlackFloor),
+                    szz = fmaf(fabsf(bz), kSlack, kSlackFloor);
+        const float bnx = bx - sxx, bfx = bx + sxx, bny = by - syy, bfy = by + syy, bnz = bz - szz, bfz = bz + szz;
+        const bool nx = ix < 0.f, ny = iy < 0.f, nz = iz < 0.f;
+        const uint32_t wnx = nx ? q2.y : q1.z, wfx = nx ? q1.z : q2.y;
+        const uint32_t wny = ny ? q2.z : q1.w, wfy = ny ? q1.w : q2.z;
+        const uint32_t wnz = nz ? q2.w : q2.x, wfz = nz ? q2.x : q2.w;
         float key[4];
         uint32_t ent[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const float lx = fmaf(byte_of(q1.z, c), scx, orx), hx = fmaf(byte_of(q2.y, c), scx, orx);
-          const float ly = fmaf(byte_of(q1.w, c), scy, ory), hy = fmaf(byte_of(q2.z, c), scy, ory);
-          const float lz = fmaf(byte_of(q2.x, c), scz, orz), hz = fmaf(byte_of(q2.w, c), scz, orz);
-          const float tx0 = (lx - ox) * ix, tx1 = (hx - ox) * ix;
-          const float ty0 = (ly - oy) * iy, ty1 = (hy - oy) * iy;
-          const float tz0 = (lz - oz) * iz, tz1 = (hz - oz) * iz;
-          const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-          const float tf = fminf(fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)), t_best) * kRobust;
+          const float tn = fmaxf(fmaxf(fmaf(byte_of(wnx, c), ax, bnx), fmaf(byte_of(wny, c), ay, bny)),
+                                 fmaxf(fmaf(byte_of(wnz, c), az, bnz), tmin));
+          const float tf = fminf(fminf(fmaf(byte_of(wfx, c), ax, bfx), fmaf(byte_of(wfy, c), ay, bfy)),
+                                 fminf(fmaf(byte_of(wfz, c), az, bfz), t_best)) * kRobust;
           const bool hit = (uint32_t(c) < nch) && tn <= tf;
           const uint32_t cnt = (q3.x >> (8 * c)) & 0xffu;
           const uint32_t off = (c < 2 ? (q3.y >> (16 * c)) : (q3.z >> (16 * (c - 2)))) & 0xffffu;
@@ -261,9 +279,7 @@ trace_kernel(TraceParams p) {
         MRT_CSWAP(0, 1) MRT_CSWAP(2, 3) MRT_CSWAP(0, 2) MRT_CSWAP(1, 3) MRT_CSWAP(1, 2)
 #undef MRT_CSWAP
         const float inf = __int_as_float(0x7f800000);
-        // A box whose near distance is +inf is never reached (a zero direction component with
-        // the origin outside that slab), so it counts as missed.
-        uint32_t nhit = 0;
+        uint32_t nhit = 0;  // plane distances are finite, so +inf marks exactly the missed children
 #pragma unroll
         for (int c = 0; c < 4; ++c) nhit += (key[c] < inf) ? 1u : 0u;
         if (nhit > 3) push(ent[3]);

@@ -5,6 +5,7 @@ triangle referenced once (not box-split), each decoded child box contains the tr
 strictly; (b) leaf slots hold exactly the reordered reference array of the binary BVH; (c) the
 records match the mesh."""
 import numpy as np
+import pytest
 
 import myrt_amd as m
 import oracle as orc
@@ -34,7 +35,11 @@ def walk(nodes):
                 yield start, int(n["count"][c]), path + [box]
 
 
-def test_heightfield_wide_bvh_contains_every_triangle(ensure_built):
+@pytest.mark.parametrize("builder", ["sah", "reference"])
+def test_heightfield_wide_bvh_contains_every_triangle(ensure_built, builder, monkeypatch):
+    # The scene's traversal trees come from the SAH builder by default; MRT_BVH_BUILDER=0 (read
+    # when a scene is created) selects the reference builder's trees.
+    monkeypatch.setenv("MRT_BVH_BUILDER", "1" if builder == "sah" else "0")
     idx, pts, _, _ = scenes.heightfield(120, 60, seed=3)
     s = m.Scene()
     g = s.push_geometry(idx, pts)
@@ -55,9 +60,13 @@ def test_heightfield_wide_bvh_contains_every_triangle(ensure_built):
             assert np.array_equal(t["v0"], pts[idx[t["prim"], 0]])
             assert np.array_equal(t["v2"], pts[idx[t["prim"], 2]])
     assert seen.all()
-    # The leaf slots are the binary BVH's reordered references, grouped per leaf.
-    _, ref_idx, _ = orc.bvh_build(orc.triangle_bounds(idx, pts), True)
-    assert sorted(tris["prim"].tolist()) == sorted(ref_idx.tolist())
+    if builder == "sah":
+        # No reference duplication: every triangle sits in exactly one leaf.
+        assert np.array_equal(np.sort(tris["prim"]), np.arange(idx.shape[0]))
+    else:
+        # The leaf slots are the reference builder's reordered references, grouped per leaf.
+        _, ref_idx, _ = orc.bvh_build(orc.triangle_bounds(idx, pts), True)
+        assert sorted(tris["prim"].tolist()) == sorted(ref_idx.tolist())
     # TLAS: one instance, identity.
     assert items.tolist() == [0] and inst["blas_root"][0] == 0
 
