@@ -27,14 +27,15 @@ constexpr int kBlock = 256;
 #ifndef MRT_TRACE_WAVES
 #define MRT_TRACE_WAVES 5
 #endif
-constexpr float kRobust = 1.0000005f;  // 1 + 8 ulp: covers the relative rounding of the slab test
+constexpr float kRobust = 1.000001f;  // 1 + 16 ulp: covers the relative rounding of the slab test
 constexpr float kSlack = 1.0f / 1048576.0f;  // 2^-20 of |b|: covers the absolute rounding of b
 constexpr float kSlackFloor = 7.8886091e-31f;  // 2^-100: covers a plane term that underflows
 constexpr float kInvMax = 18446744073709551616.0f;  // 2^64: |1/d| is clamped here for box tests
 
-// 1/d for box tests, clamped to [-2^64, 2^64]: a zero component gives +-2^64 instead of +-inf (a
-// NaN component gives -2^64), so every plane distance below stays finite.
-__device__ inline float box_inv(float d) { return fminf(fmaxf(1.0f / d, -kInvMax), kInvMax); }
+// 1/d for box tests, from the hardware reciprocal (1 ulp), clamped to [-2^64, 2^64]: a zero
+// component gives +-2^64 instead of +-inf (a NaN component gives -2^64), so every plane distance
+// below stays finite. Its relative error is part of what kRobust covers.
+__device__ inline float box_inv(float d) { return fminf(fmaxf(__builtin_amdgcn_rcpf(d), -kInvMax), kInvMax); }
 
 struct RayPre {  // per-space ray data for the watertight test
   float ox, oy, oz;   // origin permuted to (kx, ky, kz)
@@ -58,9 +59,12 @@ __device__ inline void make_pre(float ox, float oy, float oz, float dx, float dy
   r.kx = kx;
   r.ky = ky;
   r.kz = kz;
-  r.sx = sel3(kx, dx, dy, dz) / dkz;
-  r.sy = sel3(ky, dx, dy, dz) / dkz;
-  r.sz = 1.0f / dkz;
+  // One division per ray and space; the shear is the same for every triangle of the ray,
+  // which is all the watertight property needs.
+  const float inv = 1.0f / dkz;
+  r.sx = sel3(kx, dx, dy, dz) * inv;
+  r.sy = sel3(ky, dx, dy, dz) * inv;
+  r.sz = inv;
   r.ox = sel3(kx, ox, oy, oz);
   r.oy = sel3(ky, ox, oy, oz);
   r.oz = sel3(kz, ox, oy, oz);

@@ -29,9 +29,10 @@ void make_pre(const float o[3], const float d[3], Pre& p) {
   p.kx = kx;
   p.ky = ky;
   p.kz = kz;
-  p.sx = d[kx] / d[kz];
-  p.sy = d[ky] / d[kz];
-  p.sz = 1.0f / d[kz];
+  const float inv = 1.0f / d[kz];
+  p.sx = d[kx] * inv;
+  p.sy = d[ky] * inv;
+  p.sz = inv;
   p.o[0] = o[kx];
   p.o[1] = o[ky];
   p.o[2] = o[kz];
