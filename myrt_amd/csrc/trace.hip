@@ -25,7 +25,7 @@ namespace dev {
 constexpr int kBlock = 256;
 // Waves per SIMD the register allocation targets (512 VGPRs / waves per lane).
 #ifndef MRT_TRACE_WAVES
-#define MRT_TRACE_WAVES 5
+#define MRT_TRACE_WAVES 6
 #endif
 constexpr float kRobust = 1.000001f;  // 1 + 16 ulp: covers the relative rounding of the slab test
 constexpr float kSlack = 1.0f / 1048576.0f;  // 2^-20 of |b|: covers the absolute rounding of b
@@ -221,12 +221,12 @@ trace_kernel(TraceParams p) {
         const float4* ip = reinterpret_cast<const float4*>(p.instances + inst);
         const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
         const uint4 meta = reinterpret_cast<const uint4*>(ip)[3];
-        ox = m0.x * wox + m0.y * woy + m0.z * woz + m0.w;
-        oy = m1.x * wox + m1.y * woy + m1.z * woz + m1.w;
-        oz = m2.x * wox + m2.y * woy + m2.z * woz + m2.w;
-        const float dx = m0.x * wdx + m0.y * wdy + m0.z * wdz;
-        const float dy = m1.x * wdx + m1.y * wdy + m1.z * wdz;
-        const float dz = m2.x * wdx + m2.y * wdy + m2.z * wdz;
+        ox = fmaf(m0.x, wox, fmaf(m0.y, woy, fmaf(m0.z, woz, m0.w)));
+        oy = fmaf(m1.x, wox, fmaf(m1.y, woy, fmaf(m1.z, woz, m1.w)));
+        oz = fmaf(m2.x, wox, fmaf(m2.y, woy, fmaf(m2.z, woz, m2.w)));
+        const float dx = fmaf(m0.x, wdx, fmaf(m0.y, wdy, m0.z * wdz));
+        const float dy = fmaf(m1.x, wdx, fmaf(m1.y, wdy, m1.z * wdz));
+        const float dz = fmaf(m2.x, wdx, fmaf(m2.y, wdy, m2.z * wdz));
         ix = box_inv(dx); iy = box_inv(dy); iz = box_inv(dz);
         make_pre(ox, oy, oz, dx, dy, dz, pre);
         cur_inst = meta.z;

@@ -65,12 +65,13 @@ bool watertight(const Pre& p, const float* a, const float* b, const float* c, fl
   return true;
 }
 
-// Object-space ray of an instance: rows 0-2 of M^-1 applied to (o, 1) and (d, 0), left to right.
+// Object-space ray of an instance: rows 0-2 of M^-1 applied to (o, 1) and (d, 0) as nested fused
+// multiply-adds, innermost term last (the product kernel's order).
 void to_object(const float* inv, const float o[3], const float d[3], float oo[3], float od[3]) {
   for (int r = 0; r < 3; ++r) {
     const float* m = inv + 4 * r;
-    oo[r] = m[0] * o[0] + m[1] * o[1] + m[2] * o[2] + m[3];
-    od[r] = m[0] * d[0] + m[1] * d[1] + m[2] * d[2];
+    oo[r] = std::fmaf(m[0], o[0], std::fmaf(m[1], o[1], std::fmaf(m[2], o[2], m[3])));
+    od[r] = std::fmaf(m[0], d[0], std::fmaf(m[1], d[1], m[2] * d[2]));
   }
 }
 
