@@ -324,7 +324,10 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   if (!prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
   if (n == 0) return MRT_OK;
   if (n >= 0x7fffffffu) return fail(MRT_ERR_LIMIT, "at most 2^31-1 rays per call");
-  const int per_cu = trace_blocks_per_cu(any, stats, postpone != 0, lds_depth);
+  // Seven waves per SIMD pay off where rays change instances often (the per-iteration work is
+  // larger and more divergent); a single mesh runs best at six. MRT_TRACE_WAVES overrides.
+  const uint32_t waves = trace_waves ? trace_waves : (current.size() > 1 ? 7u : 6u);
+  const int per_cu = trace_blocks_per_cu(any, stats, waves, lds_depth);
   const size_t max_grid = size_t(num_cus) * size_t(per_cu);
   const size_t want = (n + kTraceBlock - 1) / kTraceBlock;
   const int grid = int(std::max<size_t>(1, std::min(max_grid, want)));
@@ -361,7 +364,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.refill_below = refill_below;
   p.lds_depth = lds_depth;
   p.perm = nullptr;
-  p.postpone = postpone;
+  p.waves = waves;
   if (ray_sort && n >= 4096) {
     if (cap_sort < n || !d_sort_counts) {
       for (uint32_t* q : {d_sort_keys, d_sort_perm, d_sort_counts})
@@ -636,9 +639,9 @@ int mrt_scene_create(mrt_scene* out) {
     if (r >= 1 && r <= 64) s->refill_below = uint32_t(r);
   }
   if (const char* v = std::getenv("MRT_BVH_BUILDER")) s->sah_builder = std::strtol(v, nullptr, 10) != 0;
-  if (const char* v = std::getenv("MRT_TRACE_POSTPONE")) {
-    const long k = std::strtol(v, nullptr, 10);
-    if (k >= 0 && k <= 64) s->postpone = uint32_t(k);
+  if (const char* v = std::getenv("MRT_TRACE_WAVES")) {
+    const long w = std::strtol(v, nullptr, 10);
+    if (w == 6 || w == 7) s->trace_waves = uint32_t(w);
   }
   *out = reinterpret_cast<mrt_scene>(s);
   return MRT_OK;

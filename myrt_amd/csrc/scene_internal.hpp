@@ -109,7 +109,7 @@ struct Scene {
   int num_cus = 0;
   uint32_t refill_below = 40;
   bool sah_builder = true;  // traversal trees from build_sah_bvh (MRT_BVH_BUILDER=0: the reference builder's)
-  uint32_t postpone = 0;  // leaf postponing: 0 off, else the stuck-lane count that runs the leaf phase
+  uint32_t trace_waves = 0;  // 0: by scene (7 with several instances, else 6); MRT_TRACE_WAVES=6|7
   // Largest wide leaf: binary leaves are split at centroid medians down to this many triangles
   // (instances for the TLAS). Measured best: 2 triangles, 1 instance. MRT_BVH_LEAF and
   // MRT_TLAS_LEAF override.

@@ -23,10 +23,6 @@ namespace mrt {
 namespace dev {
 
 constexpr int kBlock = 256;
-// Waves per SIMD the register allocation targets (512 VGPRs / waves per lane).
-#ifndef MRT_TRACE_WAVES
-#define MRT_TRACE_WAVES 6
-#endif
 constexpr float kRobust = 1.000001f;  // 1 + 16 ulp: covers the relative rounding of the slab test
 constexpr float kSlack = 1.0f / 1048576.0f;  // 2^-20 of |b|: covers the absolute rounding of b
 constexpr float kSlackFloor = 7.8886091e-31f;  // 2^-100: covers a plane term that underflows
@@ -111,8 +107,8 @@ __device__ inline uint32_t pack_bary(float b1, float b2) {
 __device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 __device__ inline float byte_of(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
 
-template <bool ANY, bool STATS, bool POSTPONE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRT_TRACE_WAVES, MRT_TRACE_WAVES))) void
+template <bool ANY, bool STATS, int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void
 trace_kernel(TraceParams p) {
   extern __shared__ uint32_t lds_stack[];  // [lds_depth][kBlock]
   const uint32_t kStackLds = p.lds_depth;
@@ -131,8 +127,6 @@ trace_kernel(TraceParams p) {
   float t_best = 0, b1_best = 0, b2_best = 0;
   uint32_t inst_best = 0, prim_best = 0, cur_inst = 0;
   uint32_t entry = 0, sp = 0;
-  uint32_t leaf = 0;                        // parked triangle leaf (leaf postponing)
-  bool have_leaf = false, have_entry = false;
   int level = 0;
   bool found = false;
   uint32_t n_nodes = 0, n_tris = 0, n_inst = 0;
@@ -191,8 +185,6 @@ trace_kernel(TraceParams p) {
           entry = 0;  // TLAS root
           if (STATS) n_nodes = n_tris = n_inst = 0;
           if (!p.has_tlas || !(wdx != 0.f || wdy != 0.f || wdz != 0.f)) entry = kStackReturn;  // finishes at once
-          have_entry = true;
-          have_leaf = false;
         }
       }
     }
@@ -343,8 +335,8 @@ trace_kernel(TraceParams p) {
     };
     const auto is_tri_leaf = [&](uint32_t e) { return level == 1 && e != kStackReturn && (e & 0x80000000u); };
 
-    if (!POSTPONE) {
-      // Every lane processes its current entry each iteration, whatever its kind.
+    // Every lane processes its current entry each iteration, whatever its kind.
+    {
       for (;;) {
         if (ray_id >= 0) {
           bool have_next = false, done = false;
@@ -372,54 +364,6 @@ trace_kernel(TraceParams p) {
         if (active == 0) break;
         if (!queue_empty && __popcll(active) < p.refill_below) break;
       }
-    } else {
-      // Leaf postponing (Aila and Laine, HPG 2009): a lane that reaches a triangle leaf parks it
-      // and keeps traversing inner nodes of the same instance; the wave tests parked leaves
-      // together once no busy lane is still looking for one, or once p.postpone lanes cannot
-      // move without theirs. A lane never leaves its instance with a leaf parked.
-      for (;;) {
-        bool stuck = false;
-        if (ray_id >= 0) {
-          if (have_entry && !have_leaf && is_tri_leaf(entry)) {
-            leaf = entry;
-            have_leaf = true;
-            have_entry = sp > 0;
-            if (have_entry) entry = pop();
-          }
-          const bool blocked = !have_entry || is_tri_leaf(entry) || (have_leaf && entry == kStackReturn);
-          if (!blocked) {
-            bool have_next = false, done = false;
-            uint32_t next = 0;
-            node_step(entry, have_next, next, done);
-            if (done) {
-              have_entry = false;  // level 0 never holds a parked leaf
-            } else {
-              if (!have_next && sp > 0) {
-                next = pop();
-                have_next = true;
-              }
-              have_entry = have_next;
-              entry = next;
-            }
-          }
-          if (!have_entry && !have_leaf) {
-            finish();
-          } else {
-            stuck = have_leaf && (!have_entry || is_tri_leaf(entry) || entry == kStackReturn);
-          }
-        }
-        const uint64_t searching = __ballot(ray_id >= 0 && !have_leaf);
-        const uint64_t stuck_mask = __ballot(stuck);
-        if (searching == 0 || uint32_t(__popcll(stuck_mask)) >= p.postpone) {
-          if (ray_id >= 0 && have_leaf) {
-            have_leaf = false;
-            if (leaf_step(leaf) || !have_entry) finish();
-          }
-        }
-        const uint64_t active = __ballot(ray_id >= 0);
-        if (active == 0) break;
-        if (!queue_empty && __popcll(active) < p.refill_below) break;
-      }
     }
   }
 }
@@ -429,35 +373,36 @@ trace_kernel(TraceParams p) {
 int launch_trace(const TraceParams& p, bool any, bool stats, int grid, hipStream_t stream) {
   const dim3 block(dev::kBlock), g(grid);
   const size_t lds = size_t(p.lds_depth) * dev::kBlock * 4;
-  const bool pp = p.postpone != 0;
-#define MRT_LAUNCH(A, S, P) hipLaunchKernelGGL((dev::trace_kernel<A, S, P>), g, block, lds, stream, p)
+  const bool w7 = p.waves == 7;
+#define MRT_LAUNCH(A, S, W) hipLaunchKernelGGL((dev::trace_kernel<A, S, W>), g, block, lds, stream, p)
   if (any) {
     if (stats) {
-      if (pp) MRT_LAUNCH(true, true, true); else MRT_LAUNCH(true, true, false);
+      if (w7) MRT_LAUNCH(true, true, 7); else MRT_LAUNCH(true, true, 6);
     } else {
-      if (pp) MRT_LAUNCH(true, false, true); else MRT_LAUNCH(true, false, false);
+      if (w7) MRT_LAUNCH(true, false, 7); else MRT_LAUNCH(true, false, 6);
     }
   } else {
     if (stats) {
-      if (pp) MRT_LAUNCH(false, true, true); else MRT_LAUNCH(false, true, false);
+      if (w7) MRT_LAUNCH(false, true, 7); else MRT_LAUNCH(false, true, 6);
     } else {
-      if (pp) MRT_LAUNCH(false, false, true); else MRT_LAUNCH(false, false, false);
+      if (w7) MRT_LAUNCH(false, false, 7); else MRT_LAUNCH(false, false, 6);
     }
   }
 #undef MRT_LAUNCH
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int trace_blocks_per_cu(bool any, bool stats, bool postpone, uint32_t lds_depth) {
+int trace_blocks_per_cu(bool any, bool stats, uint32_t waves, uint32_t lds_depth) {
   int blocks = 0;
   const size_t lds = size_t(lds_depth) * dev::kBlock * 4;
+  const bool w7 = waves == 7;
   const void* k;
   if (any)
-    k = stats ? (postpone ? (const void*)dev::trace_kernel<true, true, true> : (const void*)dev::trace_kernel<true, true, false>)
-              : (postpone ? (const void*)dev::trace_kernel<true, false, true> : (const void*)dev::trace_kernel<true, false, false>);
+    k = stats ? (w7 ? (const void*)dev::trace_kernel<true, true, 7> : (const void*)dev::trace_kernel<true, true, 6>)
+              : (w7 ? (const void*)dev::trace_kernel<true, false, 7> : (const void*)dev::trace_kernel<true, false, 6>);
   else
-    k = stats ? (postpone ? (const void*)dev::trace_kernel<false, true, true> : (const void*)dev::trace_kernel<false, true, false>)
-              : (postpone ? (const void*)dev::trace_kernel<false, false, true> : (const void*)dev::trace_kernel<false, false, false>);
+    k = stats ? (w7 ? (const void*)dev::trace_kernel<false, true, 7> : (const void*)dev::trace_kernel<false, true, 6>)
+              : (w7 ? (const void*)dev::trace_kernel<false, false, 7> : (const void*)dev::trace_kernel<false, false, 6>);
   const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, dev::kBlock, lds);
   if (e != hipSuccess || blocks < 1) blocks = 1;
   return blocks;

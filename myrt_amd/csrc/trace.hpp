@@ -27,13 +27,12 @@ struct TraceParams {
   uint32_t refill_below;           // refill a wave when fewer lanes than this are busy
   uint32_t lds_depth;              // LDS stack entries per lane (a power of two)
   const uint32_t* perm;            // if set, queue position -> ray index (binned order)
-  uint32_t postpone;               // 0: no leaf postponing; else lanes stuck on a parked leaf
-                                   // that trigger the wave's leaf phase
+  uint32_t waves;                  // waves per SIMD the kernel is compiled for: 6 or 7
   unsigned long long* stats;       // rays, node visits, triangle tests, instance visits
 };
 
 int launch_trace(const TraceParams& p, bool any, bool stats, int grid, hipStream_t stream);
-int trace_blocks_per_cu(bool any, bool stats, bool postpone, uint32_t lds_depth);
+int trace_blocks_per_cu(bool any, bool stats, uint32_t waves, uint32_t lds_depth);
 constexpr int kTraceBlock = 256;
 constexpr uint32_t kTraceStackLds = 16;  // default LDS stack entries per lane
 
