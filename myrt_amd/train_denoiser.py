@@ -3,8 +3,9 @@
 Each seeded training scene (myrt_amd.scenes.training_scene) is rendered at RES x RES, depth 8:
 noisy inputs at 1, 2, 4, 8, 16, 32 and 64 samples per pixel (radiance plus primary-hit albedo, normal and
 distance), and a converged target at TARGET_SPP. The network is trained with Adam on random
-128 x 128 crops (flips and transposes as augmentation) by the library's own fp32 forward and
-backward kernels, and the weights are saved as a float32 .npy of the packed parameters.
+128 x 128 crops (flips and transposes as augmentation; input levels drawn in proportion to
+sqrt(spp) by default) by the library's own fp32 forward and backward kernels, and the weights are
+saved as a float32 .npy of the packed parameters.
 
 Usage:
   python -m myrt_amd.train_denoiser [--scenes 48] [--val 4] [--res 256] [--target-spp 1024]
@@ -96,6 +97,10 @@ def main(argv=None):
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--level-weight", choices=("uniform", "sqrt"), default="sqrt",
+                    help="how often each input spp level is drawn: uniformly, or in proportion to "
+                         "sqrt(spp) so that the smaller errors of the cleaner inputs weigh as much "
+                         "in the L1 loss as the noisy ones")
     ap.add_argument("--out", default=DEFAULT_OUT)
     args = ap.parse_args(argv)
 
@@ -111,6 +116,8 @@ def main(argv=None):
 
     d = m.Denoiser(seed=args.seed)
     rng = np.random.default_rng(args.seed)
+    level_p = np.sqrt(np.asarray(INPUT_SPP, np.float64)) if args.level_weight == "sqrt" else np.ones(len(INPUT_SPP))
+    level_p /= level_p.sum()
     B, C, R = args.batch, args.crop, args.res
     losses = []
     t1 = time.time()
@@ -122,7 +129,7 @@ def main(argv=None):
         tgt = np.empty((B, C, C, 3), np.float32)
         for b in range(B):
             inputs, target = train[int(rng.integers(len(train)))]
-            lvl = int(rng.integers(len(INPUT_SPP)))
+            lvl = int(rng.choice(len(INPUT_SPP), p=level_p))
             y, x = int(rng.integers(R - C + 1)), int(rng.integers(R - C + 1))
             rr, tt = augment(inputs[lvl, y:y + C, x:x + C], target[y:y + C, x:x + C], rng)
             raw[b], tgt[b] = rr, tt

@@ -170,3 +170,46 @@ def test_denoise_rendered_image(ensure_built):
     # The untrained network's residual is small: the output stays near its input.
     img = r.image()[..., :3]
     assert np.abs(np.log1p(out[..., :3]) - np.log1p(img)).mean() < 0.2
+
+
+def test_trained_network_gains_on_a_held_out_scene(ensure_built):
+    """The shipped weights (myrt_amd/data/denoiser_unet.npy, trained on training scenes 0-131) on
+    training_scene(500), which they never saw: denoising a 1-spp render cuts the MSE against a
+    1024-spp render of the same scene at least 4x, a 16-spp render at least 2x."""
+    import os
+    weights = os.path.join(os.path.dirname(m.__file__), "data", "denoiser_unet.npy")
+    sc = scenes.training_scene(500)
+    s = m.Scene()
+    gids = [s.push_geometry(i, p, n) for i, p, n in sc["geometries"]]
+    for mat in sc["materials"]:
+        s.push_material(mat)
+    s.set_lights(sc["lights"])
+    s.set_environment(sc.get("env", 1.0))
+    for g, mt, mi in sc["instances"]:
+        s.enqueue(gids[g], mt, mi)
+    if "sdf_tape" in sc:
+        s.enqueue_sdf(s.push_sdf(sc["sdf_tape"]), sc["sdf_transform"])
+    res = 192
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], 1.0)
+
+    def renderer():
+        r = m.Renderer()
+        r.set_camera(view, proj)
+        r.set_num_bounces(8)
+        return r
+
+    say("held-out scene: 1024-spp reference at %d^2" % res)
+    ref_r = renderer()
+    for _ in range(4):
+        ref_r.render(s, res, res, 256)
+    ref = ref_r.image()[..., :3].astype(np.float64)
+    d = m.Denoiser()
+    d.load(weights)
+    for spp, need in ((1, 4.0), (16, 2.0)):
+        r = renderer()
+        r.render(s, res, res, spp)
+        noisy = ((r.image()[..., :3] - ref) ** 2).mean()
+        den = ((r.denoise(d)[..., :3] - ref) ** 2).mean()
+        say("%d spp: noisy MSE %.3e, denoised %.3e (x%.1f)" % (spp, noisy, den, noisy / den))
+        assert noisy / den >= need
+    s.close()
