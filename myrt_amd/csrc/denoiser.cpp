@@ -226,7 +226,8 @@ int Denoiser::ensure_half(hipStream_t s) {
 }
 
 template <typename T>
-int Denoiser::forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, int H, int W, hipStream_t s) {
+int Denoiser::forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, int H, int W, hipStream_t s,
+                      bool with_out) {
   constexpr bool half = sizeof(T) == 2;
   if (H % 16 || W % 16 || B < 1 || H < 16 || W < 16) return fail(MRT_ERR_INVALID, "denoiser tiles are 16-pixel multiples");
   int rc = A.ensure(B, H, W);
@@ -261,10 +262,11 @@ int Denoiser::forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, i
                         nullptr, 1, B, Hl, Wl, half, s)) != MRT_OK)
       return rc;
   }
+  if (!with_out) return MRT_OK;
   return conv_call(L[kOut], A.db[0].p, 0, nullptr, wl(kOut), b.p + L[kOut].b_phys, A.y.p, nullptr, 0, B, H, W, half, s);
 }
-template int Denoiser::forward<float>(UnetActs<float>&, const float*, int, int, int, int, int, hipStream_t);
-template int Denoiser::forward<_Float16>(UnetActs<_Float16>&, const float*, int, int, int, int, int, hipStream_t);
+template int Denoiser::forward<float>(UnetActs<float>&, const float*, int, int, int, int, int, hipStream_t, bool);
+template int Denoiser::forward<_Float16>(UnetActs<_Float16>&, const float*, int, int, int, int, int, hipStream_t, bool);
 
 int Denoiser::train(const float* d_raw, const float* d_target, int B, int H, int W, float lr_used, float* loss,
                     hipStream_t s) {
@@ -372,10 +374,12 @@ int Denoiser::train(const float* d_raw, const float* d_target, int B, int H, int
 
 int Denoiser::denoise(const float* d_raw, int Hs, int Ws, float* d_out, hipStream_t s) {
   const int H = (Hs + 15) / 16 * 16, W = (Ws + 15) / 16 * 16;
-  int rc = forward<_Float16>(hact, d_raw, 1, Hs, Ws, H, W, s);
+  int rc = forward<_Float16>(hact, d_raw, 1, Hs, Ws, H, W, s, false);
   if (rc != MRT_OK) return rc;
-  return launch_unet_finish<_Float16>(d_raw, hact.y.p, kYc, d_out, Hs, Ws, W, s) ? fail(MRT_ERR_HIP, "denoiser output")
-                                                                                 : int(MRT_OK);
+  return launch_unet_finish_out(d_raw, hact.db[0].p, L[kOut].cin_phys, wh.p + L[kOut].w_phys, b.p + L[kOut].b_phys,
+                                d_out, Hs, Ws, W, s)
+             ? fail(MRT_ERR_HIP, "denoiser output")
+             : int(MRT_OK);
 }
 
 }  // namespace mrt

@@ -60,7 +60,9 @@ struct Denoiser {
   int get_logical(float* p, bool grads);
   // Forward pass in T over raw features (device [B][Hs][Ws][10]) padded to H x W.
   template <typename T>
-  int forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, int H, int W, hipStream_t s);
+  // with_out = false stops before the 1x1 output layer (the fused finish computes it).
+  int forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, int H, int W, hipStream_t s,
+              bool with_out = true);
   // Loss, gradients into gw/gb (zeroed first) and, if lr > 0, one Adam step.
   int train(const float* d_raw, const float* d_target, int B, int H, int W, float lr_used, float* loss,
             hipStream_t s);

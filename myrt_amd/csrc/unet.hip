@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "unet.hpp"
 
@@ -169,6 +170,151 @@ __global__ __launch_bounds__(kBlock) void conv_kernel(ConvArgs a) {
           for (int i = 0; i < 4; ++i) v[i] = mv[i] > 0.0f ? v[i] : 0.0f;
         }
         F::store4(out + base + co, v[0], v[1], v[2], v[3]);
+      }
+    }
+  }
+}
+
+// Tiled fp16 3x3 convolution for inference. A workgroup of 4 waves computes a tile of kTileRows
+// output rows x 64 pixels x NM blocks of 32 output channels (output-channel group g of the
+// layer's cout / (32 NM)); wave w owns row w of the tile. For each chunk of 16 input channels the
+// workgroup stages in LDS the input halo of the tile (kTileRows + 2 rows x 66 pixels) and the
+// chunk's weights for all 9 taps, then every wave runs 9 taps x NM x 2 MFMAs (32x32x16) out of LDS:
+// an input element is read from L2 once per tile and channel group instead of once per tap. The
+// next chunk's global loads are issued into registers before the current chunk's MFMAs, so their
+// latency overlaps the math. LDS layouts put the two 8-channel halves of a fragment in separate
+// planes, so a half wave reads 32 consecutive 16-byte records (no bank conflicts): input
+// [half][row][col][8], weights [half][tap][cout][8].
+constexpr int kTileRows = 4;
+constexpr int kTileCols = 64;
+constexpr int kHaloRows = kTileRows + 2, kHaloCols = kTileCols + 2;
+constexpr int kHaloItems = 2 * kHaloRows * kHaloCols;                 // 16-byte records
+constexpr int kHaloPer = (kHaloItems + kBlock - 1) / kBlock;
+
+template <int NM>
+__global__ __launch_bounds__(kBlock) void conv_tile_kernel(ConvArgs a, int groups) {
+  constexpr int kCout = NM * 32;
+  constexpr int kWItems = 2 * 9 * kCout;
+  constexpr int kWPer = (kWItems + kBlock - 1) / kBlock;
+  __shared__ h8 s_in[kHaloItems];
+  __shared__ h8 s_w[kWItems];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int col = lane & 31, half = lane >> 5;
+  const int ntx = (a.W + kTileCols - 1) / kTileCols, nty = (a.H + kTileRows - 1) / kTileRows;
+  int t = blockIdx.x;
+  const int g = t % groups;
+  t /= groups;
+  const int tx = t % ntx;
+  t /= ntx;
+  const int ty = t % nty;
+  const int img = t / nty;
+  const int x0 = tx * kTileCols, y0 = ty * kTileRows, m0 = g * kCout;
+  const int cin = a.c0 + a.c1;
+  const int H0 = a.up0 ? (a.H >> 1) : a.H, W0 = a.up0 ? (a.W >> 1) : a.W;
+  const _Float16* in0 = static_cast<const _Float16*>(a.in0);
+  const _Float16* in1 = static_cast<const _Float16*>(a.in1);
+  const _Float16* w = static_cast<const _Float16*>(a.w);
+
+  h8 r_in[kHaloPer], r_w[kWPer];
+  const auto fetch = [&](int c) {
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) {
+      const int i = tid + k * kBlock;
+      h8 v = {};
+      if (i < kHaloItems) {
+        const int hf = i & 1, pix = i >> 1;
+        const int r = pix / kHaloCols, cc = pix - r * kHaloCols;
+        const int sy = y0 - 1 + r, sx = x0 - 1 + cc;
+        if (sy >= 0 && sy < a.H && sx >= 0 && sx < a.W) {
+          if (c < a.c0) {
+            const int yy = a.up0 ? (sy >> 1) : sy, xx = a.up0 ? (sx >> 1) : sx;
+            v = *reinterpret_cast<const h8*>(in0 + ((size_t(img) * H0 + yy) * W0 + xx) * a.c0 + c + 8 * hf);
+          } else {
+            v = *reinterpret_cast<const h8*>(in1 + ((size_t(img) * a.H + sy) * a.W + sx) * a.c1 + (c - a.c0) +
+                                             8 * hf);
+          }
+        }
+      }
+      r_in[k] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < kWPer; ++k) {
+      const int i = tid + k * kBlock;
+      if (i < kWItems) {
+        const int hf = i & 1, rest = i >> 1;
+        const int co = rest / 9, tap = rest - co * 9;
+        r_w[k] = *reinterpret_cast<const h8*>(w + (size_t(m0 + co) * 9 + tap) * cin + c + 8 * hf);
+      }
+    }
+  };
+  const auto stage = [&]() {
+#pragma unroll
+    for (int k = 0; k < kHaloPer; ++k) {
+      const int i = tid + k * kBlock;
+      if (i < kHaloItems) {
+        const int hf = i & 1, pix = i >> 1;
+        const int r = pix / kHaloCols, cc = pix - r * kHaloCols;
+        s_in[(hf * kHaloRows + r) * kHaloCols + cc] = r_in[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kWPer; ++k) {
+      const int i = tid + k * kBlock;
+      if (i < kWItems) {
+        const int hf = i & 1, rest = i >> 1;
+        const int co = rest / 9, tap = rest - co * 9;
+        s_w[(hf * 9 + tap) * kCout + co] = r_w[k];
+      }
+    }
+  };
+
+  f32x16 acc[NM][2];
+#pragma unroll
+  for (int mb = 0; mb < NM; ++mb) acc[mb][0] = acc[mb][1] = f32x16{};
+
+  fetch(0);
+  for (int c = 0; c < cin; c += 16) {
+    __syncthreads();  // the previous chunk's LDS reads are done
+    stage();
+    __syncthreads();
+    if (c + 16 < cin) fetch(c + 16);
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap % 3;
+      const int row = wave + dy;
+      const h8 b0 = s_in[(half * kHaloRows + row) * kHaloCols + col + dx];
+      const h8 b1 = s_in[(half * kHaloRows + row) * kHaloCols + 32 + col + dx];
+#pragma unroll
+      for (int mb = 0; mb < NM; ++mb) {
+        const h8 av = s_w[(half * 9 + tap) * kCout + mb * 32 + col];
+        acc[mb][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, b0, acc[mb][0], 0, 0, 0);
+        acc[mb][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, b1, acc[mb][1], 0, 0, 0);
+      }
+    }
+  }
+  // Lane (col, half) of block (mb, nb) holds pixel x0 + 32 nb + col of row y0 + wave and output
+  // channels m0 + 32 mb + 8 j + 4 half + i (element 4 j + i).
+  const int y = y0 + wave;
+  if (y >= a.H) return;
+  _Float16* out = static_cast<_Float16*>(a.out);
+#pragma unroll
+  for (int mb = 0; mb < NM; ++mb) {
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      const int px = x0 + nb * 32 + col;
+      if (px >= a.W) continue;
+      const size_t base = ((size_t(img) * a.H + y) * a.W + px) * a.cout;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int co = m0 + mb * 32 + 8 * j + 4 * half;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          v[i] = acc[mb][nb][4 * j + i];
+          if (a.bias) v[i] += a.bias[co + i];
+          if (a.relu) v[i] = fmaxf(v[i], 0.0f);
+        }
+        Frag<_Float16>::store4(out + base + co, v[0], v[1], v[2], v[3]);
       }
     }
   }
@@ -360,6 +506,34 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(const float* raw, const 
   reinterpret_cast<float4*>(out)[i] = make_float4(o[0], o[1], o[2], 1.0f);
 }
 
+// The 1x1 output convolution (cin channels -> 3) fused with the finish step: per pixel the
+// residual y = W x + b is computed in fp32 from the fp16 decoder output and never stored.
+__global__ __launch_bounds__(kBlock) void finish_out_kernel(const float* raw, const _Float16* x, int cin,
+                                                            const _Float16* w, const float* bias, float* out, int Hs,
+                                                            int Ws, int W) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  const size_t n = size_t(Hs) * Ws;
+  if (i >= n) return;
+  const int px = int(i % Ws), py = int(i / Ws);
+  const _Float16* xp = x + (size_t(py) * W + px) * cin;
+  float y[3] = {bias[0], bias[1], bias[2]};
+  for (int k = 0; k < cin; k += 8) {
+    const h8 v = *reinterpret_cast<const h8*>(xp + k);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const h8 wv = *reinterpret_cast<const h8*>(w + size_t(c) * cin + k);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) y[c] = fmaf(float(wv[e]), float(v[e]), y[c]);
+    }
+  }
+  float o[3];
+  for (int c = 0; c < 3; ++c) {
+    const float z = log1pf(fmaxf(raw[i * kUnetInCh + c], 0.0f)) + y[c];
+    o[c] = expm1f(fmaxf(z, 0.0f));
+  }
+  reinterpret_cast<float4*>(out)[i] = make_float4(o[0], o[1], o[2], 1.0f);
+}
+
 __global__ __launch_bounds__(kBlock) void transpose_w_kernel(const float* w, float* wt, int cout, int taps, int cin) {
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
   const size_t n = size_t(cout) * taps * cin;
@@ -399,6 +573,30 @@ int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
   if (a.cout % 32 || a.c0 % step || a.c1 % step || (a.up0 && (a.H % 2 || a.W % 2)) || (a.c1 && !a.in1) ||
       (taps != 9 && taps != 1))
     return -1;
+  if (std::is_same<T, _Float16>::value && taps == 9 && !a.mask && a.cout <= 128 && a.c0 % 16 == 0 &&
+      a.c1 % 16 == 0) {
+    // Inference 3x3: the LDS-tiled kernel, one workgroup per 4 x 64 output tile and group of
+    // 32 NM output channels. NM is the largest divisor of cout / 32 that still gives at least
+    // 1024 workgroups (4 per CU), so the coarse levels split their channels across workgroups.
+    const unsigned tiles = unsigned(a.B) * unsigned((a.H + unk::kTileRows - 1) / unk::kTileRows) *
+                           unsigned((a.W + unk::kTileCols - 1) / unk::kTileCols);
+    const int mt = a.cout / 32;
+    int nm = 1;
+    for (int d = mt; d >= 1; --d)
+      if (mt % d == 0 && tiles * unsigned(mt / d) >= 1024u) {
+        nm = d;
+        break;
+      }
+    const int groups = mt / nm;
+    const dim3 grid(tiles * unsigned(groups));
+    switch (nm) {
+      case 1: hipLaunchKernelGGL((unk::conv_tile_kernel<1>), grid, dim3(unk::kBlock), 0, s, a, groups); break;
+      case 2: hipLaunchKernelGGL((unk::conv_tile_kernel<2>), grid, dim3(unk::kBlock), 0, s, a, groups); break;
+      case 3: hipLaunchKernelGGL((unk::conv_tile_kernel<3>), grid, dim3(unk::kBlock), 0, s, a, groups); break;
+      default: hipLaunchKernelGGL((unk::conv_tile_kernel<4>), grid, dim3(unk::kBlock), 0, s, a, groups); break;
+    }
+    return unk::ok();
+  }
   const size_t waves = size_t(a.B) * a.H * ((a.W + 63) / 64) * ((a.cout + 63) / 64);
   const unsigned grid = unsigned((waves * 64 + unk::kBlock - 1) / unk::kBlock);
   if (taps == 9)
@@ -480,6 +678,15 @@ int launch_unet_finish(const float* raw, const T* y, int yc, float* out_rgba, in
 }
 template int launch_unet_finish<float>(const float*, const float*, int, float*, int, int, int, hipStream_t);
 template int launch_unet_finish<_Float16>(const float*, const _Float16*, int, float*, int, int, int, hipStream_t);
+
+int launch_unet_finish_out(const float* raw, const _Float16* x, int cin, const _Float16* w, const float* bias,
+                           float* out_rgba, int Hs, int Ws, int W, hipStream_t s) {
+  if (cin % 8) return -1;
+  const size_t n = size_t(Hs) * Ws;
+  hipLaunchKernelGGL(unk::finish_out_kernel, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, raw, x, cin, w, bias,
+                     out_rgba, Hs, Ws, W);
+  return unk::ok();
+}
 
 int launch_unet_transpose_w(const float* w, float* wt, int cout, int taps, int cin, hipStream_t s) {
   const size_t n = size_t(cout) * taps * cin;

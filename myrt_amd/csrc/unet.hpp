@@ -90,6 +90,9 @@ int launch_unet_loss(const float* x, const float* y, int yc, const float* target
 // Denoised radiance expm1(max(log1p(rgb) + y, 0)) for the Hs x Ws image region.
 template <typename T>
 int launch_unet_finish(const float* raw, const T* y, int yc, float* out_rgba, int Hs, int Ws, int W, hipStream_t s);
+// Fused 1x1 output convolution (fp16 weights [3+][cin], fp32 bias) and finish, for inference.
+int launch_unet_finish_out(const float* raw, const _Float16* x, int cin, const _Float16* w, const float* bias,
+                           float* out_rgba, int Hs, int Ws, int W, hipStream_t s);
 int launch_unet_transpose_w(const float* w, float* wt, int cout, int taps, int cin, hipStream_t s);
 int launch_unet_to_half(const float* in, _Float16* out, size_t n, hipStream_t s);
 int launch_unet_adam(float* w, float* m, float* v, const float* g, size_t n, float lr, float b1, float b2, float eps,
