@@ -290,6 +290,7 @@ Scene::~Scene() {
     if (p) hipFree(p);
   if (d_blas) hipFree(d_blas);
   if (d_tris) hipFree(d_tris);
+  if (d_sdf_side) hipFree(d_sdf_side);
   if (d_tlas) hipFree(d_tlas);
   if (d_tlas_items) hipFree(d_tlas_items);
   if (d_inst) hipFree(d_inst);
@@ -405,7 +406,21 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     sp.n_dev = n_dev;
     sp.steps = sdf_steps;
     sp.eps = sdf_eps;
-    if (launch_sdf(sp, any, uint32_t(n), stream) != 0) return fail(MRT_ERR_HIP, "sdf kernel launch failed");
+    sp.counter = d_ctrl + 2;  // zeroed with the traversal counters above
+    if (!any && sdf_side_cap < n) {
+      if (d_sdf_side) hipFree(d_sdf_side);
+      d_sdf_side = nullptr;
+      sdf_side_cap = 0;
+      hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&d_sdf_side), n * sizeof(float4));
+      if (e2 != hipSuccess) return hip_fail(e2, "hipMalloc sdf hit points");
+      sdf_side_cap = n;
+    }
+    sp.side = d_sdf_side;
+    sp.refill_below = 48;
+    sp.prog_len = uint32_t(sdf_prog.size());
+    sp.persistent = (sdf_persistent == 1 || (sdf_persistent == 2 && !any)) ? 1u : 0u;
+    if (launch_sdf(sp, any, num_cus, stream) != 0)
+      return fail(MRT_ERR_HIP, "sdf kernel launch failed");
   }
   return MRT_OK;
 }
@@ -639,6 +654,10 @@ int mrt_scene_create(mrt_scene* out) {
     if (r >= 1 && r <= 64) s->refill_below = uint32_t(r);
   }
   if (const char* v = std::getenv("MRT_BVH_BUILDER")) s->sah_builder = std::strtol(v, nullptr, 10) != 0;
+  if (const char* v = std::getenv("MRT_SDF_PERSISTENT")) {
+    const long k = std::strtol(v, nullptr, 10);
+    if (k >= 0 && k <= 2) s->sdf_persistent = int(k);
+  }
   if (const char* v = std::getenv("MRT_TRACE_WAVES")) {
     const long w = std::strtol(v, nullptr, 10);
     if (w == 6 || w == 7) s->trace_waves = uint32_t(w);

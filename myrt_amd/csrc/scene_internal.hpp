@@ -106,6 +106,9 @@ struct Scene {
   uint32_t* d_ctrl = nullptr;  // [0] ray counter, [1] flags, [8..15] four u64 counters
   uint32_t* d_spill = nullptr;
   size_t spill_cap = 0;
+  float4* d_sdf_side = nullptr;  // per-ray SDF hit points between the march and the normal pass
+  int sdf_persistent = 0;  // MRT_SDF_PERSISTENT: 0 one thread per ray, 1 persistent lanes, 2 persistent for closest hits
+  size_t sdf_side_cap = 0;
   int num_cus = 0;
   uint32_t refill_below = 40;
   bool sah_builder = true;  // traversal trees from build_sah_bvh (MRT_BVH_BUILDER=0: the reference builder's)

@@ -8,6 +8,7 @@
 // All binary32 arithmetic is written out; the oracle repeats it operation for operation.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "sdf.hpp"
@@ -298,48 +299,19 @@ __device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, const 
   return tw;
 }
 
-template <bool ANY>
-__global__ __launch_bounds__(kBlock) void sdf_kernel(SdfParams p) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t n = p.n_dev ? *p.n_dev : p.n;
-  if (i >= n) return;
-  const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(i);
-  const float4 r0 = rp[0], r1 = rp[1];
-  if (!(r1.x != 0.f || r1.y != 0.f || r1.z != 0.f)) return;
-  float limit = r1.w;
-  if (ANY) {
-    if (reinterpret_cast<const uint8_t*>(p.out)[i]) return;
-  } else {
-    const uint4 h = reinterpret_cast<const uint4*>(p.out)[i];
-    if (h.z != 0xFFFFFFFFu) limit = fminf(limit, __uint_as_float(h.x));
-  }
-  float best = limit;
-  int best_j = -1;
-  float bx = 0, by = 0, bz = 0, blen = 1;
-  for (uint32_t j = 0; j < p.num_drawables; ++j) {
-    const SdfDrawable dr = p.drawables[j];
-    float t0 = r0.w, t1 = best;
-    if (!clip(dr, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1)) continue;
-    float lx, ly, lz, len;
-    const SdfGrid grid = p.grids[j];
-    const float tw = march(dr, p.prog + dr.tape_offset, grid, p.masks, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1, p.steps, p.eps, lx,
-                           ly, lz, len);
-    if (tw >= 0.f && tw < best && tw > r0.w) {
-      best = tw;
-      best_j = int(j);
-      bx = lx;
-      by = ly;
-      bz = lz;
-      blen = len;
-      if (ANY) break;
-    }
-  }
-  if (best_j < 0) return;
-  if (ANY) {
-    reinterpret_cast<uint8_t*>(p.out)[i] = 1;
-    return;
-  }
-  // Normal: the tetrahedral gradient of raymarch.glsl:100-114, offset 0.0005 / len.
+// Second pass of a closest-hit query: every record the march marked as an SDF hit (instance bit
+// 31) gets its material and normal, the tetrahedral gradient of raymarch.glsl:100-114 (offset
+// 0.0005 / len) at the object-space hit point the march left in `side`, turned to world space.
+// Kept out of the marching kernel so that its registers do not lower that kernel's occupancy.
+__global__ __launch_bounds__(kBlock) void sdf_normal_kernel(SdfParams p) {
+  const uint32_t ray = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = p.n_dev ? min(*p.n_dev, p.n) : p.n;
+  if (ray >= n) return;
+  uint4 h = reinterpret_cast<const uint4*>(p.out)[ray];
+  if (h.z == 0xFFFFFFFFu || !(h.z & 0x80000000u)) return;
+  const int best_j = int(h.z & 0x7fffffffu);
+  const float4 sp = p.side[ray];
+  const float bx = sp.x, by = sp.y, bz = sp.z, blen = sp.w;
   const SdfDrawable dr = p.drawables[best_j];
   const SdfInsn* tape = p.prog + dr.tape_offset;
   const SdfGrid grid = p.grids[best_j];
@@ -365,22 +337,252 @@ __global__ __launch_bounds__(kBlock) void sdf_kernel(SdfParams p) {
   wx = wx / l;
   wy = wy / l;
   wz = wz / l;
-  uint4 h;
-  h.x = __float_as_uint(best);
   h.y = mat;
-  h.z = 0x80000000u | uint32_t(best_j);
   h.w = oct_encode(wx, wy, wz);
-  reinterpret_cast<uint4*>(p.out)[i] = h;
+  reinterpret_cast<uint4*>(p.out)[ray] = h;
+}
+
+// Persistent sphere tracing. Every lane runs a small state machine over its ray: pick the next
+// drawable whose box the segment [tmin, best] crosses, march it one step per iteration
+// (march_sdf of raymarch.glsl:116-171: t += sign * d until |d| < eps, steps run out or t leaves
+// the box), keep the closest hit, and finish (normal + record, or the occlusion flag). A lane
+// whose ray is done takes the next ray from a global counter (one ballot and one atomic per
+// wave) once fewer than p.refill_below lanes are busy, so rays that march hundreds of steps do
+// not hold whole waves idle. Each ray's arithmetic is the sequential algorithm's, in its order.
+// A closest hit is written as (t, drawable) with the object-space hit point in p.side;
+// sdf_normal_kernel completes the record.
+template <bool ANY>
+__device__ __attribute__((always_inline)) inline void march_persistent(const SdfParams& p, const SdfInsn* prog) {
+  const int lane = __lane_id();
+  const uint32_t n = p.n_dev ? min(*p.n_dev, p.n) : p.n;
+  int ray = -1;
+  bool queue_empty = false;
+  // Ray and search state.
+  float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, tmin = 0, best = 0;
+  int best_j = -1;
+  uint32_t j = 0;  // next drawable to clip
+  float bx = 0, by = 0, bz = 0, blen = 1;
+  // March state of drawable j - 1 while marching.
+  bool marching = false;
+  float lx = 0, ly = 0, lz = 0, rx = 0, ry = 0, rz = 0, len = 1, t = 0, t_exit = 0;
+  int si = 0, step = 0;
+
+  for (;;) {
+    const bool want = ray < 0 && !queue_empty;
+    const uint64_t want_mask = __ballot(want);
+    if (want_mask) {
+      const int leader = __ffsll((unsigned long long)want_mask) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(p.counter, uint32_t(__popcll(want_mask)));
+      base = __shfl(base, leader);
+      if (want) {
+        const uint32_t r = base + uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
+        if (r < n) {
+          const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(r);
+          const float4 r0 = rp[0], r1 = rp[1];
+          bool skip = !(r1.x != 0.f || r1.y != 0.f || r1.z != 0.f);
+          float limit = r1.w;
+          if (ANY) {
+            skip = skip || reinterpret_cast<const uint8_t*>(p.out)[r] != 0;
+          } else {
+            const uint4 h = reinterpret_cast<const uint4*>(p.out)[r];
+            if (h.z != 0xFFFFFFFFu) limit = fminf(limit, __uint_as_float(h.x));
+          }
+          if (!skip) {
+            ray = int(r);
+            ox = r0.x; oy = r0.y; oz = r0.z; tmin = r0.w;
+            dx = r1.x; dy = r1.y; dz = r1.z;
+            best = limit;
+            best_j = -1;
+            j = 0;
+            marching = false;
+          }
+        } else {
+          queue_empty = true;
+        }
+      }
+    }
+    if (__ballot(want && ray < 0 && !queue_empty)) continue;  // skipped rays: fetch again
+    if (__ballot(want && queue_empty)) queue_empty = true;
+    if (__ballot(ray >= 0) == 0) break;
+
+    for (;;) {
+      if (ray >= 0) {
+        // Start the next drawable whose box the current segment crosses.
+        while (!marching && j < p.num_drawables) {
+          const SdfDrawable dr = p.drawables[j++];
+          float t0 = tmin, t1 = best;
+          if (!clip(dr, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
+          const float* m = dr.inv;
+          lx = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
+          ly = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
+          lz = m[8] * ox + m[9] * oy + m[10] * oz + m[11];
+          rx = m[0] * dx + m[1] * dy + m[2] * dz;
+          ry = m[4] * dx + m[5] * dy + m[6] * dz;
+          rz = m[8] * dx + m[9] * dy + m[10] * dz;
+          len = len3(rx, ry, rz);
+          t = fmaxf(10.0f * p.eps, t0 * len);
+          t_exit = t1;
+          si = 0;
+          step = 0;
+          marching = true;
+        }
+        bool done = !marching;  // no drawable left
+        if (marching) {
+          float res = -1.0f;
+          bool ended = !(step < p.steps && t / len < t_exit);
+          if (!ended) {
+            const SdfDrawable& dr = p.drawables[j - 1];
+            const float s = t / len;
+            uint32_t mat;
+            const float d = map_any(prog + dr.tape_offset, dr.tape_len, p.grids[j - 1], p.masks, lx + rx * s,
+                                    ly + ry * s, lz + rz * s, mat);
+            if (si == 0) si = d > 0.f ? 1 : (d < 0.f ? -1 : 0);
+            if (fabsf(d) < p.eps) {
+              res = t;
+              ended = true;
+            } else {
+              t = t + float(si) * d;
+              ++step;
+            }
+          }
+          if (ended) {
+            marching = false;
+            const float tw = res < 0.f ? -1.0f : res / len;
+            if (tw >= 0.f && tw < best && tw > tmin) {
+              best = tw;
+              best_j = int(j - 1);
+              bx = lx + rx * tw;
+              by = ly + ry * tw;
+              bz = lz + rz * tw;
+              blen = len;
+              if (ANY) done = true;
+            }
+            if (j >= p.num_drawables) done = true;
+          }
+        }
+        if (done) {
+          if (best_j >= 0) {
+            if (ANY) {
+              reinterpret_cast<uint8_t*>(p.out)[ray] = 1;
+            } else {
+              // Distance and drawable now; material and normal in sdf_normal_kernel.
+              reinterpret_cast<uint4*>(p.out)[ray] = make_uint4(__float_as_uint(best), 0u, 0x80000000u | uint32_t(best_j), 0u);
+              p.side[ray] = make_float4(bx, by, bz, blen);
+            }
+          }
+          ray = -1;
+        }
+      }
+      const uint64_t active = __ballot(ray >= 0);
+      if (active == 0) break;
+      if (!queue_empty && __popcll(active) < p.refill_below) break;
+    }
+  }
+}
+
+// One thread per ray (the sequential algorithm as it stands): clip every drawable, march the
+// ones the segment [tmin, best] crosses, keep the closest hit. Occupancy is higher than the
+// persistent form's; a wave waits for its longest march.
+template <bool ANY>
+__device__ __attribute__((always_inline)) inline void march_per_ray(const SdfParams& p, const SdfInsn* prog) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = p.n_dev ? min(*p.n_dev, p.n) : p.n;
+  if (i >= n) return;
+  const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(i);
+  const float4 r0 = rp[0], r1 = rp[1];
+  if (!(r1.x != 0.f || r1.y != 0.f || r1.z != 0.f)) return;
+  float limit = r1.w;
+  if (ANY) {
+    if (reinterpret_cast<const uint8_t*>(p.out)[i]) return;
+  } else {
+    const uint4 h = reinterpret_cast<const uint4*>(p.out)[i];
+    if (h.z != 0xFFFFFFFFu) limit = fminf(limit, __uint_as_float(h.x));
+  }
+  float best = limit;
+  int best_j = -1;
+  float bx = 0, by = 0, bz = 0, blen = 1;
+  for (uint32_t j = 0; j < p.num_drawables; ++j) {
+    const SdfDrawable dr = p.drawables[j];
+    float t0 = r0.w, t1 = best;
+    if (!clip(dr, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1)) continue;
+    float lx, ly, lz, len;
+    const SdfGrid grid = p.grids[j];
+    const float tw = march(dr, prog + dr.tape_offset, grid, p.masks, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1,
+                           p.steps, p.eps, lx, ly, lz, len);
+    if (tw >= 0.f && tw < best && tw > r0.w) {
+      best = tw;
+      best_j = int(j);
+      bx = lx;
+      by = ly;
+      bz = lz;
+      blen = len;
+      if (ANY) break;
+    }
+  }
+  if (best_j < 0) return;
+  if (ANY) {
+    reinterpret_cast<uint8_t*>(p.out)[i] = 1;
+  } else {
+    reinterpret_cast<uint4*>(p.out)[i] = make_uint4(__float_as_uint(best), 0u, 0x80000000u | uint32_t(best_j), 0u);
+    p.side[i] = make_float4(bx, by, bz, blen);
+  }
+}
+
+// The compiled programs of all drawables are copied to LDS when they fit (kProgCache
+// instructions), so the per-lane instruction fetches of the culled chains are LDS reads rather
+// than L1/L2 hits.
+constexpr uint32_t kProgCache = 256;
+
+#ifndef MRT_SDF_WAVES
+#define MRT_SDF_WAVES 5
+#endif
+template <bool ANY, bool PERSIST>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PERSIST ? MRT_SDF_WAVES : 4, 8))) void sdf_kernel(
+    SdfParams p) {
+  __shared__ uint4 s_prog[kProgCache * (sizeof(SdfInsn) / 16)];
+  static_assert(sizeof(SdfInsn) % 16 == 0, "instruction records are whole 16-byte words");
+  const bool cached = p.prog_len <= kProgCache;
+  if (cached) {
+    const uint4* src = reinterpret_cast<const uint4*>(p.prog);
+    const uint32_t words = p.prog_len * uint32_t(sizeof(SdfInsn) / 16);
+    for (uint32_t k = threadIdx.x; k < words; k += kBlock) s_prog[k] = src[k];
+  }
+  __syncthreads();
+  // One code path: the pointer is generic, so its loads go to LDS or to global memory.
+  const SdfInsn* prog = cached ? reinterpret_cast<const SdfInsn*>(s_prog) : p.prog;
+  if (PERSIST)
+    march_persistent<ANY>(p, prog);
+  else
+    march_per_ray<ANY>(p, prog);
 }
 
 }  // namespace sdfk
 
-int launch_sdf(const SdfParams& p, bool any, uint32_t n_max, hipStream_t stream) {
-  const uint32_t grid = (n_max + sdfk::kBlock - 1) / sdfk::kBlock;
-  if (any)
-    hipLaunchKernelGGL(sdfk::sdf_kernel<true>, dim3(grid ? grid : 1), dim3(sdfk::kBlock), 0, stream, p);
-  else
-    hipLaunchKernelGGL(sdfk::sdf_kernel<false>, dim3(grid ? grid : 1), dim3(sdfk::kBlock), 0, stream, p);
+int launch_sdf(const SdfParams& p, bool any, int num_cus, hipStream_t stream) {
+  int grid = int((p.n + sdfk::kBlock - 1) / sdfk::kBlock);
+  if (p.persistent) {
+    int blocks = 0;
+    const hipError_t e =
+        any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sdfk::sdf_kernel<true, true>, sdfk::kBlock, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sdfk::sdf_kernel<false, true>, sdfk::kBlock, 0);
+    if (e != hipSuccess || blocks < 1) blocks = 1;
+    grid = std::min(grid, num_cus * blocks);
+  }
+  grid = std::max(grid, 1);
+  if (any) {
+    if (p.persistent)
+      hipLaunchKernelGGL((sdfk::sdf_kernel<true, true>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+    else
+      hipLaunchKernelGGL((sdfk::sdf_kernel<true, false>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+  } else {
+    if (p.persistent)
+      hipLaunchKernelGGL((sdfk::sdf_kernel<false, true>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+    else
+      hipLaunchKernelGGL((sdfk::sdf_kernel<false, false>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+    const unsigned nb = unsigned((p.n + sdfk::kBlock - 1) / sdfk::kBlock);
+    hipLaunchKernelGGL(sdfk::sdf_normal_kernel, dim3(nb ? nb : 1), dim3(sdfk::kBlock), 0, stream, p);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
