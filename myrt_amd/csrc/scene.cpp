@@ -366,6 +366,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.lds_depth = lds_depth;
   p.perm = nullptr;
   p.waves = waves;
+  p.single_instance = current.size() == 1 ? 0u : 0xFFFFFFFFu;
   if (ray_sort && n >= 4096) {
     if (cap_sort < n || !d_sort_counts) {
       for (uint32_t* q : {d_sort_keys, d_sort_perm, d_sort_counts})

@@ -155,6 +155,25 @@ trace_kernel(TraceParams p) {
     return e;
   };
 
+  // Moves the ray into instance `inst`'s object space; returns the instance's BLAS root.
+  const auto enter = [&](uint32_t inst) -> uint32_t {
+    const float4* ip = reinterpret_cast<const float4*>(p.instances + inst);
+    const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
+    const uint4 meta = reinterpret_cast<const uint4*>(ip)[3];
+    ox = fmaf(m0.x, wox, fmaf(m0.y, woy, fmaf(m0.z, woz, m0.w)));
+    oy = fmaf(m1.x, wox, fmaf(m1.y, woy, fmaf(m1.z, woz, m1.w)));
+    oz = fmaf(m2.x, wox, fmaf(m2.y, woy, fmaf(m2.z, woz, m2.w)));
+    const float dx = fmaf(m0.x, wdx, fmaf(m0.y, wdy, m0.z * wdz));
+    const float dy = fmaf(m1.x, wdx, fmaf(m1.y, wdy, m1.z * wdz));
+    const float dz = fmaf(m2.x, wdx, fmaf(m2.y, wdy, m2.z * wdz));
+    ix = box_inv(dx); iy = box_inv(dy); iz = box_inv(dz);
+    make_pre(ox, oy, oz, dx, dy, dz, pre);
+    cur_inst = meta.z;
+    level = 1;
+    if (STATS) ++n_inst;
+    return meta.x;
+  };
+
   for (;;) {
     // ---- refill idle lanes from the global ray counter ----
     const bool want = ray_id < 0 && !queue_empty;
@@ -184,7 +203,13 @@ trace_kernel(TraceParams p) {
           sp = 0;
           entry = 0;  // TLAS root
           if (STATS) n_nodes = n_tris = n_inst = 0;
-          if (!p.has_tlas || !(wdx != 0.f || wdy != 0.f || wdz != 0.f)) entry = kStackReturn;  // finishes at once
+          if (!p.has_tlas || !(wdx != 0.f || wdy != 0.f || wdz != 0.f)) {
+            entry = kStackReturn;  // finishes at once
+          } else if (p.single_instance != 0xFFFFFFFFu) {
+            // A frame of one instance: enter it here, with the other fresh rays of the wave,
+            // instead of through the TLAS root; no RETURN marker, the ray ends with its BLAS.
+            entry = enter(p.single_instance);
+          }
         }
       }
     }
@@ -210,22 +235,8 @@ trace_kernel(TraceParams p) {
         const uint32_t inst = p.tlas_items[start];
         if (cnt > 1) push(0x80000000u | ((cnt - 1) << 24) | (start + 1));
         push(kStackReturn);
-        const float4* ip = reinterpret_cast<const float4*>(p.instances + inst);
-        const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
-        const uint4 meta = reinterpret_cast<const uint4*>(ip)[3];
-        ox = fmaf(m0.x, wox, fmaf(m0.y, woy, fmaf(m0.z, woz, m0.w)));
-        oy = fmaf(m1.x, wox, fmaf(m1.y, woy, fmaf(m1.z, woz, m1.w)));
-        oz = fmaf(m2.x, wox, fmaf(m2.y, woy, fmaf(m2.z, woz, m2.w)));
-        const float dx = fmaf(m0.x, wdx, fmaf(m0.y, wdy, m0.z * wdz));
-        const float dy = fmaf(m1.x, wdx, fmaf(m1.y, wdy, m1.z * wdz));
-        const float dz = fmaf(m2.x, wdx, fmaf(m2.y, wdy, m2.z * wdz));
-        ix = box_inv(dx); iy = box_inv(dy); iz = box_inv(dz);
-        make_pre(ox, oy, oz, dx, dy, dz, pre);
-        cur_inst = meta.z;
-        level = 1;
-        next = meta.x;  // BLAS root node
+        next = enter(inst);  // BLAS root node
         have_next = true;
-        if (STATS) ++n_inst;
       } else {
         // ---- inner node: test the four quantised child boxes ----
         const WideNode* nodes = level ? p.blas_nodes : p.tlas_nodes;

@@ -28,6 +28,7 @@ struct TraceParams {
   uint32_t lds_depth;              // LDS stack entries per lane (a power of two)
   const uint32_t* perm;            // if set, queue position -> ray index (binned order)
   uint32_t waves;                  // waves per SIMD the kernel is compiled for: 6 or 7
+  uint32_t single_instance;        // the frame's only instance record, or 0xFFFFFFFF
   unsigned long long* stats;       // rays, node visits, triangle tests, instance visits
 };
 
