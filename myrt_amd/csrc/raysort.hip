@@ -74,7 +74,79 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(RaySortArgs a) {
   a.perm[slot] = i;
 }
 
+// ---- XCD groups: 8 slabs, block-local histograms (one global atomic per block and group) ----
+constexpr int kGroupPer = 16;  // rays per thread
+
+__device__ inline uint32_t group_key(const RayGroupArgs& a, uint32_t i) {
+  const float4 o = reinterpret_cast<const float4*>(a.rays)[2 * size_t(i)];
+  const float v = a.axis == 0 ? o.x : (a.axis == 1 ? o.y : o.z);
+  const float f = (v - a.lo) * a.inv_ext * float(kXcdGroups);
+  return f > 0.f ? (f < float(kXcdGroups - 1) ? uint32_t(f) : kXcdGroups - 1) : 0u;  // NaN -> 0
+}
+
+__global__ __launch_bounds__(kBlock) void group_histogram_kernel(RayGroupArgs a) {
+  __shared__ uint32_t hist[kXcdGroups];
+  if (threadIdx.x < kXcdGroups) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;
+  const uint32_t base = blockIdx.x * kBlock * kGroupPer;
+  for (int j = 0; j < kGroupPer; ++j) {
+    const uint32_t i = base + uint32_t(j * kBlock) + threadIdx.x;
+    if (i < n) {
+      const uint32_t k = group_key(a, i);
+      a.keys[i] = uint8_t(k);
+      atomicAdd(&hist[k], 1u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kXcdGroups && hist[threadIdx.x]) atomicAdd(a.counts + threadIdx.x, hist[threadIdx.x]);
+}
+
+__global__ void group_scan_kernel(RayGroupArgs a) {
+  if (threadIdx.x != 0) return;
+  uint32_t run = 0;
+  for (uint32_t g = 0; g < kXcdGroups; ++g) {
+    const uint32_t c = a.counts[g];
+    a.counts[g] = run;
+    a.group_start[g] = run;
+    run += c;
+  }
+  a.group_start[kXcdGroups] = run;
+}
+
+__global__ __launch_bounds__(kBlock) void group_scatter_kernel(RayGroupArgs a) {
+  __shared__ uint32_t hist[kXcdGroups];
+  if (threadIdx.x < kXcdGroups) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t n = a.n_dev ? min(*a.n_dev, a.n) : a.n;
+  const uint32_t base = blockIdx.x * kBlock * kGroupPer;
+  uint32_t key[kGroupPer];
+  for (int j = 0; j < kGroupPer; ++j) {
+    const uint32_t i = base + uint32_t(j * kBlock) + threadIdx.x;
+    key[j] = i < n ? uint32_t(a.keys[i]) : kXcdGroups;
+    if (key[j] < kXcdGroups) atomicAdd(&hist[key[j]], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kXcdGroups && hist[threadIdx.x]) hist[threadIdx.x] = atomicAdd(a.counts + threadIdx.x, hist[threadIdx.x]);
+  __syncthreads();
+  for (int j = 0; j < kGroupPer; ++j) {
+    if (key[j] >= kXcdGroups) continue;
+    const uint32_t i = base + uint32_t(j * kBlock) + threadIdx.x;
+    a.perm[atomicAdd(&hist[key[j]], 1u)] = i;
+  }
+}
+
 }  // namespace rsk
+
+int launch_ray_groups(const RayGroupArgs& a, hipStream_t s) {
+  if (a.n == 0) return 0;
+  if (hipMemsetAsync(a.counts, 0, (kXcdGroups + 1) * sizeof(uint32_t), s) != hipSuccess) return -1;
+  const unsigned grid = unsigned((a.n + rsk::kBlock * rsk::kGroupPer - 1) / (rsk::kBlock * rsk::kGroupPer));
+  hipLaunchKernelGGL(rsk::group_histogram_kernel, dim3(grid), dim3(rsk::kBlock), 0, s, a);
+  hipLaunchKernelGGL(rsk::group_scan_kernel, dim3(1), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(rsk::group_scatter_kernel, dim3(grid), dim3(rsk::kBlock), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int launch_ray_sort(const RaySortArgs& a, hipStream_t s) {
   if (a.n == 0) return 0;

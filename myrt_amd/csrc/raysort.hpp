@@ -21,4 +21,21 @@ struct RaySortArgs {
 
 int launch_ray_sort(const RaySortArgs& a, hipStream_t s);
 
+// Ray grouping for XCD-local traversal: rays are split into kXcdGroups slabs of their origin
+// along the scene box's widest axis, and perm lists each group's rays contiguously
+// (group_start[g] .. group_start[g + 1]).
+constexpr uint32_t kXcdGroups = 8;
+struct RayGroupArgs {
+  const void* rays;        // mrt_ray[n]
+  uint32_t n;              // buffer size
+  const uint32_t* n_dev;   // device-side ray count, if set
+  float lo, inv_ext;       // the widest axis of the scene box
+  int axis;
+  uint8_t* keys;           // [n]
+  uint32_t* counts;        // [kXcdGroups + 1]: histogram, then cursors; group_start follows
+  uint32_t* group_start;   // [kXcdGroups + 1]
+  uint32_t* perm;          // [n]
+};
+int launch_ray_groups(const RayGroupArgs& a, hipStream_t s);
+
 }  // namespace mrt

@@ -272,7 +272,7 @@ int Scene::upload_all() {
     shading_dirty = false;
   }
   if (!d_ctrl) {
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_ctrl), 64);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_ctrl), 256);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
     int dev = 0;
     hipGetDevice(&dev);
@@ -291,6 +291,8 @@ Scene::~Scene() {
   if (d_blas) hipFree(d_blas);
   if (d_tris) hipFree(d_tris);
   if (d_sdf_side) hipFree(d_sdf_side);
+  for (void* q : {(void*)d_group_keys, (void*)d_group_perm, (void*)d_group_counts})
+    if (q) hipFree(q);
   if (d_tlas) hipFree(d_tlas);
   if (d_tlas_items) hipFree(d_tlas_items);
   if (d_inst) hipFree(d_inst);
@@ -343,7 +345,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     if (e != hipSuccess) return hip_fail(e, "hipMalloc spill");
     spill_cap = spill_entries;
   }
-  hipError_t e = hipMemsetAsync(d_ctrl, 0, 64, stream);
+  hipError_t e = hipMemsetAsync(d_ctrl, 0, 256, stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
   TraceParams p{};
   p.tlas_nodes = d_tlas;
@@ -392,6 +394,40 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     ra.perm = d_sort_perm;
     if (launch_ray_sort(ra, stream) != 0) return fail(MRT_ERR_HIP, "ray binning launch failed");
     p.perm = d_sort_perm;
+  }
+  if (xcd_groups && !p.perm && n >= 65536) {
+    // Group rays by origin slab so that each XCD's L2 serves mostly its own part of the BVH.
+    if (cap_groups < n) {
+      for (void* q : {(void*)d_group_keys, (void*)d_group_perm})
+        if (q) hipFree(q);
+      d_group_keys = nullptr;
+      d_group_perm = nullptr;
+      cap_groups = 0;
+      hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&d_group_keys), n);
+      if (e1 == hipSuccess) e1 = hipMalloc(reinterpret_cast<void**>(&d_group_perm), n * 4);
+      if (e1 == hipSuccess && !d_group_counts)
+        e1 = hipMalloc(reinterpret_cast<void**>(&d_group_counts), 2 * (kXcdGroups + 1) * 4);
+      if (e1 != hipSuccess) return hip_fail(e1, "hipMalloc ray groups");
+      cap_groups = n;
+    }
+    RayGroupArgs ga{};
+    ga.rays = d_rays;
+    ga.n = uint32_t(n);
+    ga.n_dev = n_dev;
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+      if (scene_inv_ext[k] < scene_inv_ext[axis]) axis = k;
+    ga.axis = axis;
+    ga.lo = scene_lo[axis];
+    ga.inv_ext = scene_inv_ext[axis];
+    ga.keys = d_group_keys;
+    ga.counts = d_group_counts;
+    ga.group_start = d_group_counts + kXcdGroups + 1;
+    ga.perm = d_group_perm;
+    if (launch_ray_groups(ga, stream) != 0) return fail(MRT_ERR_HIP, "ray grouping launch failed");
+    p.perm = d_group_perm;
+    p.group_start = ga.group_start;
+    p.group_counter = d_ctrl + 32;  // zeroed with the counters above
   }
   if (launch_trace(p, any, stats, grid, stream) != 0) return fail(MRT_ERR_HIP, "trace kernel launch failed");
   if (!sdf_drawables.empty()) {
@@ -659,6 +695,7 @@ int mrt_scene_create(mrt_scene* out) {
     const long k = std::strtol(v, nullptr, 10);
     if (k >= 0 && k <= 2) s->sdf_persistent = int(k);
   }
+  if (const char* v = std::getenv("MRT_TRACE_XCD")) s->xcd_groups = std::strtol(v, nullptr, 10) != 0;
   if (const char* v = std::getenv("MRT_TRACE_WAVES")) {
     const long w = std::strtol(v, nullptr, 10);
     if (w == 6 || w == 7) s->trace_waves = uint32_t(w);

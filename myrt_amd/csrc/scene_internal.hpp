@@ -107,6 +107,10 @@ struct Scene {
   uint32_t* d_spill = nullptr;
   size_t spill_cap = 0;
   float4* d_sdf_side = nullptr;  // per-ray SDF hit points between the march and the normal pass
+  bool xcd_groups = false;  // MRT_TRACE_XCD=1: rays grouped by origin slab, one group per XCD
+  uint8_t* d_group_keys = nullptr;
+  uint32_t *d_group_perm = nullptr, *d_group_counts = nullptr;
+  size_t cap_groups = 0;
   int sdf_persistent = 0;  // MRT_SDF_PERSISTENT: 0 one thread per ray, 1 persistent lanes, 2 persistent for closest hits
   size_t sdf_side_cap = 0;
   int num_cus = 0;

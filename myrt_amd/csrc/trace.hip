@@ -118,6 +118,8 @@ trace_kernel(TraceParams p) {
 
   int ray_id = -1;
   bool queue_empty = false;
+  // Ray groups: XCD x of workgroup b (round-robin dispatch: x = b % 8) starts on group x.
+  uint32_t group = blockIdx.x & 7u, groups_left = 8;
   const uint32_t n_rays = p.n_dev ? *p.n_dev : p.n;
 
   float wox = 0, woy = 0, woz = 0, wdx = 0, wdy = 0, wdz = 0, wix = 0, wiy = 0, wiz = 0;
@@ -180,14 +182,26 @@ trace_kernel(TraceParams p) {
     const uint64_t want_mask = __ballot(want);
     if (want_mask) {
       const int leader = __ffsll((unsigned long long)want_mask) - 1;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(p.counter, uint32_t(__popcll(want_mask)));
+      const uint32_t cnt = uint32_t(__popcll(want_mask));
+      uint32_t base = 0, limit = n_rays, first = 0;
+      if (p.group_start) {
+        // Ray groups: take from the current group; move on when it runs dry.
+        if (lane == leader) base = atomicAdd(p.group_counter + group, cnt);
+        first = p.group_start[group];
+        limit = p.group_start[group + 1] - first;
+      } else {
+        if (lane == leader) base = atomicAdd(p.counter, cnt);
+      }
       base = __shfl(base, leader);
+      if (p.group_start && base + cnt >= limit) {
+        group = (group + 1) & 7u;
+        --groups_left;
+      }
       if (want) {
         const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
         const uint32_t r = base + rank;
-        if (r < n_rays) {
-          const uint32_t rid = p.perm ? p.perm[r] : r;
+        if (r < limit) {
+          const uint32_t rid = p.perm ? p.perm[first + r] : r;
           ray_id = int(rid);
           const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(rid);
           const float4 r0 = rp[0], r1 = rp[1];
@@ -213,8 +227,15 @@ trace_kernel(TraceParams p) {
         }
       }
     }
-    if (__ballot(want && ray_id < 0)) queue_empty = true;
-    if (__ballot(ray_id >= 0) == 0) break;
+    if (p.group_start) {
+      if (groups_left == 0) queue_empty = true;
+    } else if (__ballot(want && ray_id < 0)) {
+      queue_empty = true;
+    }
+    if (__ballot(ray_id >= 0) == 0) {
+      if (queue_empty) break;
+      continue;  // this group ran dry; the next refill takes from the next one
+    }
 
     // ---- traverse until too many lanes are idle ----
     // One step on an entry that is not a triangle leaf: an inner node (test the four children,

@@ -152,14 +152,16 @@ def test_heightfield_full_size(ensure_built):
         assert np.array_equal(occ, ref_occ), name + ": shadow occlusion differs"
 
 
-def test_ray_binning_gives_identical_hits(ensure_built, monkeypatch):
-    """Binning rays by octant and origin cell before traversal changes the order rays are taken
-    from the queue, never a result: hits equal the unbinned ones bit for bit."""
+@pytest.mark.parametrize("knob", ["MRT_RAY_SORT", "MRT_TRACE_XCD"])
+def test_ray_binning_gives_identical_hits(ensure_built, monkeypatch, knob):
+    """Binning rays (by octant and origin cell, or into per-XCD origin slabs) before traversal
+    changes the order rays are taken from the queue, never a result: hits equal the unbinned ones
+    bit for bit."""
     (idx, pts, nrm, uv), coh, inc = scenes.traversal_workload(n_coherent=1 << 16, n_incoherent=1 << 18)
     plain = m.Scene()
     plain.enqueue(plain.push_geometry(idx, pts, nrm, uv))
     plain.prepare()
-    monkeypatch.setenv("MRT_RAY_SORT", "1")
+    monkeypatch.setenv(knob, "1")
     binned = m.Scene()
     binned.enqueue(binned.push_geometry(idx, pts, nrm, uv))
     binned.prepare()
