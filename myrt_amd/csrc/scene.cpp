@@ -15,6 +15,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <thread>
+#include <atomic>
+#include <functional>
 
 #include "../../include/myrt.h"
 #include "bvh_build.hpp"
@@ -1045,7 +1048,59 @@ bool chain_prim_ok(const SdfInsn& in) {
   }
 }
 
+// Mask of one cell [clo, chi] (slack included): bit j is set unless primitive j provably leaves
+// the running value unchanged everywhere in the cell. Only the primitives in `candidates` are
+// considered; the others are known to drop over the cell. A dropped primitive never lowers the
+// bound on the running value (its distance anywhere in the cell exceeds the bound), so leaving the
+// non-candidates out of the bound changes no mask.
+uint64_t cell_mask(const SdfInsn* prog, uint32_t n, const double* blo, const double* bhi, const double clo[3],
+                   const double chi[3], uint64_t candidates) {
+  double ctr[3], half2 = 0;
+  for (int a = 0; a < 3; ++a) {
+    ctr[a] = 0.5 * (clo[a] + chi[a]);
+    half2 += 0.25 * (chi[a] - clo[a]) * (chi[a] - clo[a]);
+  }
+  const double half = std::sqrt(half2);
+  uint64_t mask = 1;
+  double upper = chain_prim_distance(prog[0], ctr[0], ctr[1], ctr[2]) + half;
+  for (uint32_t j = 1; j < n; ++j) {
+    if (!((candidates >> j) & 1u)) continue;
+    {
+      double gap2 = 0;
+      bool overlap = true;
+      for (int a = 0; a < 3; ++a) {
+        const double gap = std::max(0.0, std::max(blo[3 * j + a] - chi[a], clo[a] - bhi[3 * j + a]));
+        if (gap > 0) overlap = false;
+        gap2 += gap * gap;
+      }
+      const bool smooth = ((prog[j].op >> 8) & 0xffu) == kSdfSmoothUnion;
+      const double margin = (smooth ? double(prog[j].k) * (1.0 + 1e-3) : 0.0) + 1e-4 * (1.0 + std::fabs(upper));
+      bool drop = !overlap && std::sqrt(gap2) > upper + margin;
+      if ((prog[j].op & 0xffu) == kSdfSphere) {
+        // Tighter and valid everywhere: |p - c| - r >= (distance from the cell to c) - r.
+        double c2 = 0;
+        for (int a = 0; a < 3; ++a) {
+          const double c = (prog[j].op & kSdfFuseTranslate) ? double(prog[j].t[a]) : 0.0;
+          const double gap = std::max(0.0, std::max(c - chi[a], clo[a] - c));
+          c2 += gap * gap;
+        }
+        drop = std::sqrt(c2) - double(prog[j].p[0]) > upper + margin;
+      }
+      if (!drop) mask |= uint64_t(1) << j;
+    }
+    upper = std::min(upper, chain_prim_distance(prog[j], ctr[0], ctr[1], ctr[2]) + half);
+  }
+  return mask;
+}
+
+// Cells per axis of a chain's grid: finer for longer chains (more primitives to cull), 2 MiB of
+// masks at 64 and 16 MiB at 128.
+int sdf_grid_dim(uint32_t n) { return n <= 8 ? 32 : (n <= 24 ? 64 : 128); }
+
 // Builds the culling grid of a compiled program if it is one union chain of exact primitives.
+// A coarse grid of 32 cells per axis comes first; each fine cell then tests only the bits of the
+// coarse cell that contains it, since a primitive that provably drops over a cell drops over every
+// part of it. Slabs of cells are built in parallel.
 void build_sdf_grid(const SdfInsn* prog, uint32_t n, const Box& box, SdfGrid& g, std::vector<uint64_t>& masks) {
   g = SdfGrid{};
   if (n < 2 || n > uint32_t(kSdfChainMax)) return;
@@ -1062,46 +1117,54 @@ void build_sdf_grid(const SdfInsn* prog, uint32_t n, const Box& box, SdfGrid& g,
   }
   std::vector<double> blo(3 * n), bhi(3 * n);
   for (uint32_t j = 0; j < n; ++j) chain_prim_box(prog[j], &blo[3 * j], &bhi[3 * j]);
-  const int D = kSdfGridDim;
+  const int D = sdf_grid_dim(n), C = kSdfGridCoarse, R = D / C;
+  // Cell [lo, hi] of a grid with `dim` cells per axis, widened by a slack: the cell the device
+  // computes in binary32 may reach a little past the exact one.
+  auto cell_box = [&](int dim, const int ci[3], double clo[3], double chi[3]) {
+    for (int a = 0; a < 3; ++a) {
+      const double size = ext[a] / dim, slack = 1e-4 * ext[a];
+      clo[a] = double(box.mn[a]) + size * ci[a] - slack;
+      chi[a] = double(box.mn[a]) + size * (ci[a] + 1) + slack;
+    }
+  };
+  const uint64_t all = n >= 64 ? ~0ull : ((uint64_t(1) << n) - 1);
+  std::vector<uint64_t> coarse(size_t(C) * C * C), fine(size_t(D) * D * D);
+  auto run_slabs = [](int slabs, const std::function<void(int)>& body) {
+    const int threads = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+      pool.emplace_back([&] {
+        for (int z; (z = next.fetch_add(1)) < slabs;) body(z);
+      });
+    for (auto& th : pool) th.join();
+  };
+  run_slabs(C, [&](int cz) {
+    for (int cy = 0; cy < C; ++cy)
+      for (int cx = 0; cx < C; ++cx) {
+        const int ci[3] = {cx, cy, cz};
+        double clo[3], chi[3];
+        cell_box(C, ci, clo, chi);
+        coarse[(size_t(cz) * C + cy) * C + cx] = cell_mask(prog, n, blo.data(), bhi.data(), clo, chi, all);
+      }
+  });
+  run_slabs(D, [&](int cz) {
+    for (int cy = 0; cy < D; ++cy)
+      for (int cx = 0; cx < D; ++cx) {
+        const int ci[3] = {cx, cy, cz};
+        double clo[3], chi[3];
+        cell_box(D, ci, clo, chi);
+        const uint64_t parent = coarse[(size_t(cz / R) * C + cy / R) * C + cx / R];
+        fine[(size_t(cz) * D + cy) * D + cx] = R == 1 ? parent : cell_mask(prog, n, blo.data(), bhi.data(), clo, chi, parent);
+      }
+  });
   g.dim = uint32_t(D);
   g.mask_offset = uint32_t(masks.size());
   for (int a = 0; a < 3; ++a) {
     g.lo[a] = box.mn[a];
     g.inv[a] = float(double(D) / ext[a]);
   }
-  masks.reserve(masks.size() + size_t(D) * D * D);
-  for (int cz = 0; cz < D; ++cz)
-    for (int cy = 0; cy < D; ++cy)
-      for (int cx = 0; cx < D; ++cx) {
-        const int ci[3] = {cx, cy, cz};
-        double clo[3], chi[3], ctr[3], half2 = 0;
-        for (int a = 0; a < 3; ++a) {
-          // The cell the device computes in binary32 may reach a little past the exact one.
-          const double size = ext[a] / D, slack = 1e-4 * ext[a];
-          clo[a] = double(box.mn[a]) + size * ci[a] - slack;
-          chi[a] = double(box.mn[a]) + size * (ci[a] + 1) + slack;
-          ctr[a] = 0.5 * (clo[a] + chi[a]);
-          half2 += 0.25 * (chi[a] - clo[a]) * (chi[a] - clo[a]);
-        }
-        const double half = std::sqrt(half2);
-        uint64_t mask = 1;
-        double upper = chain_prim_distance(prog[0], ctr[0], ctr[1], ctr[2]) + half;
-        for (uint32_t j = 1; j < n; ++j) {
-          double gap2 = 0;
-          bool overlap = true;
-          for (int a = 0; a < 3; ++a) {
-            const double gap = std::max(0.0, std::max(blo[3 * j + a] - chi[a], clo[a] - bhi[3 * j + a]));
-            if (gap > 0) overlap = false;
-            gap2 += gap * gap;
-          }
-          const bool smooth = ((prog[j].op >> 8) & 0xffu) == kSdfSmoothUnion;
-          const double margin = (smooth ? double(prog[j].k) * (1.0 + 1e-3) : 0.0) + 1e-4 * (1.0 + std::fabs(upper));
-          const bool drop = !overlap && std::sqrt(gap2) > upper + margin;
-          if (!drop) mask |= uint64_t(1) << j;
-          upper = std::min(upper, chain_prim_distance(prog[j], ctr[0], ctr[1], ctr[2]) + half);
-        }
-        masks.push_back(mask);
-      }
+  masks.insert(masks.end(), fine.begin(), fine.end());
 }
 
 }  // namespace

@@ -181,18 +181,52 @@ __device__ inline float map(const SdfInsn* prog, uint32_t len, float px, float p
   return st.d0;
 }
 
+// Distance of a chain primitive (sphere, box, capsule or torus: the kinds a culling grid admits).
+// Lanes of one wave usually sit at different chain positions, so the kinds differ across the wave;
+// sphere, box and capsule therefore share one code path: each lane selects the vector whose length
+// its kind takes and the term added to that length, so a single square root serves all three and
+// nothing diverges. Every lane performs exactly its own kind's binary32 operations, in order.
+__device__ inline float chain_distance(const SdfInsn& in, float x, float y, float z) {
+  const uint32_t kind = in.op & 0xffu;
+  if (kind == kSdfTorus) {
+    const float qx = sqrtf(x * x + z * z) - in.p[0];
+    return sqrtf(qx * qx + y * y) - in.p[1];
+  }
+  const bool box = kind == kSdfBox, capsule = kind == kSdfCapsule;
+  const float qx = fabsf(x) - in.p[0], qy = fabsf(y) - in.p[1], qz = fabsf(z) - in.p[2];
+  const float yy = y - fminf(fmaxf(y, 0.0f), in.p[0]);
+  const float ax = box ? fmaxf(qx, 0.0f) : x;
+  const float ay = box ? fmaxf(qy, 0.0f) : (capsule ? yy : y);
+  const float az = box ? fmaxf(qz, 0.0f) : z;
+  const float l = len3(ax, ay, az);
+  return box ? l + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f) : l - (capsule ? in.p[1] : in.p[0]);
+}
+
+// The culling mask of the cell a march last looked up: a ray that stays in one cell (grazing
+// rays take many short steps) reads the mask once.
+struct CellCache {
+  uint32_t cell = 0xFFFFFFFFu;
+  uint64_t mask = 0;
+};
+
 // map() of a union-chain program with a culling grid: the lane looks up the cell of its point and
 // evaluates the head and the chain instructions of that cell's mask, in chain order. Dropped
 // instructions leave the running distance and material unchanged bit for bit, so the result is
-// map()'s. Points outside the grid evaluate every instruction.
+// map()'s. Points outside the grid evaluate every instruction. The next instruction of the mask is
+// fetched before the current one is evaluated.
 __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGrid& g, const uint64_t* masks, float px,
-                                  float py, float pz, uint32_t& mat) {
+                                  float py, float pz, uint32_t& mat, CellCache& cc) {
   uint64_t mask = len >= 64 ? ~0ull : ((1ull << len) - 1ull);
   const float fx = (px - g.lo[0]) * g.inv[0], fy = (py - g.lo[1]) * g.inv[1], fz = (pz - g.lo[2]) * g.inv[2];
   const float dim = float(g.dim);
   if (fx >= 0.f && fx < dim && fy >= 0.f && fy < dim && fz >= 0.f && fz < dim) {
     const uint32_t cx = uint32_t(fx), cy = uint32_t(fy), cz = uint32_t(fz);
-    mask = masks[g.mask_offset + (cz * g.dim + cy) * g.dim + cx];
+    const uint32_t cell = (cz * g.dim + cy) * g.dim + cx;
+    if (cell != cc.cell) {
+      cc.mask = masks[g.mask_offset + cell];
+      cc.cell = cell;
+    }
+    mask = cc.mask;
   }
   const SdfInsn h = prog[0];
   float qx = px, qy = py, qz = pz;
@@ -204,10 +238,12 @@ __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGr
   float r = prim_distance(h.op & 0xffu, h, qx, qy, qz);
   uint32_t m = h.material;
   mask &= ~1ull;
+  SdfInsn nxt;
+  if (mask) nxt = prog[__builtin_ctzll(mask)];
   while (mask) {
-    const uint32_t j = uint32_t(__builtin_ctzll(mask));
     mask &= mask - 1ull;
-    const SdfInsn in = prog[j];
+    const SdfInsn in = nxt;
+    if (mask) nxt = prog[__builtin_ctzll(mask)];
     qx = px;
     qy = py;
     qz = pz;
@@ -216,7 +252,7 @@ __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGr
       qy = py - in.t[1];
       qz = pz - in.t[2];
     }
-    const float d = prim_distance(in.op & 0xffu, in, qx, qy, qz);
+    const float d = chain_distance(in, qx, qy, qz);
     uint32_t mm;
     r = combine((in.op >> 8) & 0xffu, in.k, r, m, d, in.material, mm);
     m = mm;
@@ -227,8 +263,8 @@ __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGr
 
 // Distance of drawable program at a point, culled when the program has a grid.
 __device__ inline float map_any(const SdfInsn* prog, uint32_t len, const SdfGrid& g, const uint64_t* masks, float px,
-                                float py, float pz, uint32_t& mat) {
-  return g.dim ? map_chain(prog, len, g, masks, px, py, pz, mat) : map(prog, len, px, py, pz, mat);
+                                float py, float pz, uint32_t& mat, CellCache& cc) {
+  return g.dim ? map_chain(prog, len, g, masks, px, py, pz, mat, cc) : map(prog, len, px, py, pz, mat);
 }
 
 __device__ inline uint32_t oct_encode(float x, float y, float z) {
@@ -280,10 +316,11 @@ __device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, const 
   float t = fmaxf(10.0f * eps, t_enter * len);
   int si = 0;
   float res = -1.0f;
+  CellCache cc;
   for (int i = 0; i < steps && t / len < t_exit; ++i) {
     const float s = t / len;
     uint32_t mat;
-    const float d = map_any(tape, dr.tape_len, grid, masks, lx + rx * s, ly + ry * s, lz + rz * s, mat);
+    const float d = map_any(tape, dr.tape_len, grid, masks, lx + rx * s, ly + ry * s, lz + rz * s, mat, cc);
     if (si == 0) si = d > 0.f ? 1 : (d < 0.f ? -1 : 0);
     if (fabsf(d) < eps) {
       res = t;
@@ -316,19 +353,20 @@ __global__ __launch_bounds__(kBlock) void sdf_normal_kernel(SdfParams p) {
   const SdfInsn* tape = p.prog + dr.tape_offset;
   const SdfGrid grid = p.grids[best_j];
   float nx = 0, ny = 0, nz = 0;
+  CellCache cc;
   for (int k = 0; k < 4; ++k) {
     const float ex = 0.5773f * (2.0f * float(((k + 3) >> 1) & 1) - 1.0f);
     const float ey = 0.5773f * (2.0f * float((k >> 1) & 1) - 1.0f);
     const float ez = 0.5773f * (2.0f * float(k & 1) - 1.0f);
     const float o = 0.0005f / blen;
     uint32_t mm;
-    const float d = map_any(tape, dr.tape_len, grid, p.masks, bx + o * ex, by + o * ey, bz + o * ez, mm) * blen;
+    const float d = map_any(tape, dr.tape_len, grid, p.masks, bx + o * ex, by + o * ey, bz + o * ez, mm, cc) * blen;
     nx = nx + ex * d;
     ny = ny + ey * d;
     nz = nz + ez * d;
   }
   uint32_t mat;
-  map_any(tape, dr.tape_len, grid, p.masks, bx, by, bz, mat);
+  map_any(tape, dr.tape_len, grid, p.masks, bx, by, bz, mat, cc);
   const float* iv = dr.inv;  // world normal = (M^-1)^T n
   float wx = iv[0] * nx + iv[4] * ny + iv[8] * nz;
   float wy = iv[1] * nx + iv[5] * ny + iv[9] * nz;
@@ -366,6 +404,7 @@ __device__ __attribute__((always_inline)) inline void march_persistent(const Sdf
   bool marching = false;
   float lx = 0, ly = 0, lz = 0, rx = 0, ry = 0, rz = 0, len = 1, t = 0, t_exit = 0;
   int si = 0, step = 0;
+  CellCache cc;
 
   for (;;) {
     const bool want = ray < 0 && !queue_empty;
@@ -425,6 +464,7 @@ __device__ __attribute__((always_inline)) inline void march_persistent(const Sdf
           t_exit = t1;
           si = 0;
           step = 0;
+          cc = CellCache{};
           marching = true;
         }
         bool done = !marching;  // no drawable left
@@ -436,7 +476,7 @@ __device__ __attribute__((always_inline)) inline void march_persistent(const Sdf
             const float s = t / len;
             uint32_t mat;
             const float d = map_any(prog + dr.tape_offset, dr.tape_len, p.grids[j - 1], p.masks, lx + rx * s,
-                                    ly + ry * s, lz + rz * s, mat);
+                                    ly + ry * s, lz + rz * s, mat, cc);
             if (si == 0) si = d > 0.f ? 1 : (d < 0.f ? -1 : 0);
             if (fabsf(d) < p.eps) {
               res = t;

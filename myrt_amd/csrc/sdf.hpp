@@ -74,7 +74,7 @@ struct SdfGrid {
   float inv[3];          // cells per object-space unit
   uint32_t mask_offset;  // first cell's mask in the mask array
 };
-constexpr int kSdfGridDim = 32;
+constexpr int kSdfGridCoarse = 32;  // cells per axis of the first pass of the grid build
 constexpr int kSdfChainMax = 64;
 
 }  // namespace mrt
