@@ -209,11 +209,21 @@ struct CellCache {
   uint64_t mask = 0;
 };
 
+// chain_distance() at p less the instruction's fused translation, if it has one.
+__device__ inline float chain_distance_at(const SdfInsn& in, float px, float py, float pz) {
+  float qx = px, qy = py, qz = pz;
+  if (in.op & kSdfFuseTranslate) {
+    qx = px - in.t[0];
+    qy = py - in.t[1];
+    qz = pz - in.t[2];
+  }
+  return chain_distance(in, qx, qy, qz);
+}
+
 // map() of a union-chain program with a culling grid: the lane looks up the cell of its point and
 // evaluates the head and the chain instructions of that cell's mask, in chain order. Dropped
 // instructions leave the running distance and material unchanged bit for bit, so the result is
-// map()'s. Points outside the grid evaluate every instruction. The next instruction of the mask is
-// fetched before the current one is evaluated.
+// map()'s. Points outside the grid evaluate every instruction.
 __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGrid& g, const uint64_t* masks, float px,
                                   float py, float pz, uint32_t& mat, CellCache& cc) {
   uint64_t mask = len >= 64 ? ~0ull : ((1ull << len) - 1ull);
@@ -238,24 +248,22 @@ __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGr
   float r = prim_distance(h.op & 0xffu, h, qx, qy, qz);
   uint32_t m = h.material;
   mask &= ~1ull;
-  SdfInsn nxt;
-  if (mask) nxt = prog[__builtin_ctzll(mask)];
+  // Two instructions per iteration: their primitive distances do not depend on the running value,
+  // so both are in flight before the two combines, which stay in chain order. A lane with one
+  // instruction left evaluates it twice and keeps the first result.
   while (mask) {
+    const uint32_t j0 = uint32_t(__builtin_ctzll(mask));
     mask &= mask - 1ull;
-    const SdfInsn in = nxt;
-    if (mask) nxt = prog[__builtin_ctzll(mask)];
-    qx = px;
-    qy = py;
-    qz = pz;
-    if (in.op & kSdfFuseTranslate) {
-      qx = px - in.t[0];
-      qy = py - in.t[1];
-      qz = pz - in.t[2];
-    }
-    const float d = chain_distance(in, qx, qy, qz);
-    uint32_t mm;
-    r = combine((in.op >> 8) & 0xffu, in.k, r, m, d, in.material, mm);
-    m = mm;
+    const bool two = mask != 0ull;
+    const uint32_t j1 = two ? uint32_t(__builtin_ctzll(mask)) : j0;
+    mask &= mask - 1ull;
+    const SdfInsn a = prog[j0], b = prog[j1];
+    const float da = chain_distance_at(a, px, py, pz), db = chain_distance_at(b, px, py, pz);
+    uint32_t ma, mb;
+    const float ra = combine((a.op >> 8) & 0xffu, a.k, r, m, da, a.material, ma);
+    const float rb = combine((b.op >> 8) & 0xffu, b.k, ra, ma, db, b.material, mb);
+    r = two ? rb : ra;
+    m = two ? mb : ma;
   }
   mat = m;
   return r;
@@ -589,12 +597,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PERSIST 
     for (uint32_t k = threadIdx.x; k < words; k += kBlock) s_prog[k] = src[k];
   }
   __syncthreads();
-  // One code path: the pointer is generic, so its loads go to LDS or to global memory.
-  const SdfInsn* prog = cached ? reinterpret_cast<const SdfInsn*>(s_prog) : p.prog;
-  if (PERSIST)
-    march_persistent<ANY>(p, prog);
-  else
-    march_per_ray<ANY>(p, prog);
+  // Two code paths, so that each knows where its program lives: LDS reads for the cached copy
+  // (a pointer that may point to either would make every fetch a flat load, which takes the
+  // latency of the vector memory path).
+  if (cached) {
+    const SdfInsn* prog = reinterpret_cast<const SdfInsn*>(s_prog);
+    if (PERSIST)
+      march_persistent<ANY>(p, prog);
+    else
+      march_per_ray<ANY>(p, prog);
+  } else {
+    if (PERSIST)
+      march_persistent<ANY>(p, p.prog);
+    else
+      march_per_ray<ANY>(p, p.prog);
+  }
 }
 
 }  // namespace sdfk
