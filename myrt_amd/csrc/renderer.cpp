@@ -141,8 +141,25 @@ struct Renderer {
   uint32_t *ms_keys = nullptr, *ms_perm = nullptr, *ms_counts = nullptr;
   size_t c_msk = 0, c_msp = 0, c_msc = 0;
 
+  // Shadow rays of each bounce are traced and resolved on a second stream, concurrently with the
+  // next bounce's closest-hit passes (MRT_SHADOW_STREAM=0: everything on one stream). Events keep
+  // every data dependency: the shadow pass starts after the shade that wrote its rays, and the next
+  // shade (which rewrites the shadow queue and adds to the same accumulators) waits for the resolve.
+  bool shadow_overlap = true;
+  hipStream_t shadow_stream = nullptr;
+  hipEvent_t ev_shaded = nullptr, ev_resolved = nullptr;
+
+  int ensure_shadow_stream() {
+    if (shadow_stream) return MRT_OK;
+    hipError_t e = hipStreamCreateWithFlags(&shadow_stream, hipStreamNonBlocking);  // default priority
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_shaded, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_resolved, hipEventDisableTiming);
+    return e == hipSuccess ? MRT_OK : hip_fail(e, "creating the shadow stream");
+  }
+
   Renderer() {
     if (const char* v = std::getenv("MRT_MATERIAL_SORT")) material_sort = std::strtol(v, nullptr, 10) != 0;
+    if (const char* v = std::getenv("MRT_SHADOW_STREAM")) shadow_overlap = std::strtol(v, nullptr, 10) != 0;
     std::memset(view, 0, sizeof(view));
     std::memset(proj, 0, sizeof(proj));
     for (int i = 0; i < 4; ++i) view[i * 5] = proj[i * 5] = 1.f;
@@ -153,6 +170,9 @@ struct Renderer {
                     (void*)qrec, (void*)qout, (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den, (void*)bokeh, (void*)cubemap, (void*)ms_keys, (void*)ms_perm,
                     (void*)ms_counts})
       if (p) hipFree(p);
+    if (ev_shaded) hipEventDestroy(ev_shaded);
+    if (ev_resolved) hipEventDestroy(ev_resolved);
+    if (shadow_stream) hipStreamDestroy(shadow_stream);
   }
 
   int update_camera() {
@@ -265,6 +285,9 @@ struct Renderer {
       hipError_t e = hipMemsetAsync(counts + 8, 0, 8, stream);
       if (e != hipSuccess) return hip_fail(e, "clearing the ray counter");
     }
+    const bool overlap = shadow_overlap;
+    if (overlap && (rc = ensure_shadow_stream()) != MRT_OK) return rc;
+    hipStream_t sh_stream = overlap ? shadow_stream : stream;
     while (done < spp) {
       const uint32_t nb = std::min(batch, spp - done);
       const uint32_t n_paths = uint32_t(npix * nb);
@@ -273,10 +296,10 @@ struct Renderer {
       hipError_t e0 = hipMemsetAsync(counts + 3, 0, 3 * 4, stream);
       if (e0 != hipSuccess) return hip_fail(e0, "clearing cache counters");
       if (launch_pt_generate(cam, b, cur, n_paths, sample_count + done, stream)) return fail(MRT_ERR_HIP, "generate");
+      bool shadow_pending = false;
       for (int bounce = 0; bounce < num_bounces; ++bounce) {
-        // counts[cur] is live; clear the next queue and the shadow queue.
+        // counts[cur] is live; clear the next queue.
         hipError_t e = hipMemsetAsync(counts + (cur ^ 1u), 0, 4, stream);
-        if (e == hipSuccess) e = hipMemsetAsync(counts + 2, 0, 4, stream);
         if (e != hipSuccess) return hip_fail(e, "clearing queue counters");
         if ((rc = s.trace(rays[cur], n_paths, hits, false, false, stream, counts + cur)) != MRT_OK) return rc;
         if (material_sort) {
@@ -284,10 +307,29 @@ struct Renderer {
           if (launch_material_sort(ma, stream)) return fail(MRT_ERR_HIP, "material sort");
           b.perm = ms_perm;
         }
+        // The previous bounce's shadow pass has to be done with the shadow queue and the
+        // accumulators before this shade rewrites them.
+        if (shadow_pending && (e = hipStreamWaitEvent(stream, ev_resolved, 0)) != hipSuccess)
+          return hip_fail(e, "waiting for the shadow pass");
+        if ((e = hipMemsetAsync(counts + 2, 0, 4, stream)) != hipSuccess) return hip_fail(e, "clearing the shadow queue");
         if (launch_pt_shade(sc, b, cur, uint32_t(num_bounces), stream)) return fail(MRT_ERR_HIP, "shade");
-        if ((rc = s.trace(shadow_rays, n_paths, occluded, true, false, stream, counts + 2)) != MRT_OK) return rc;
-        if (launch_pt_resolve(b, stream)) return fail(MRT_ERR_HIP, "resolve");
+        if (overlap) {
+          if ((e = hipEventRecord(ev_shaded, stream)) != hipSuccess ||
+              (e = hipStreamWaitEvent(sh_stream, ev_shaded, 0)) != hipSuccess)
+            return hip_fail(e, "ordering the shadow pass");
+        }
+        if ((rc = s.trace(shadow_rays, n_paths, occluded, true, false, sh_stream, counts + 2, overlap ? 1 : 0)) != MRT_OK)
+          return rc;
+        if (launch_pt_resolve(b, sh_stream)) return fail(MRT_ERR_HIP, "resolve");
+        if (overlap) {
+          if ((e = hipEventRecord(ev_resolved, sh_stream)) != hipSuccess) return hip_fail(e, "ordering the shadow pass");
+          shadow_pending = true;
+        }
         cur ^= 1u;
+      }
+      if (shadow_pending) {
+        hipError_t e = hipStreamWaitEvent(stream, ev_resolved, 0);
+        if (e != hipSuccess) return hip_fail(e, "waiting for the shadow pass");
       }
       if (use_cache) {
         if ((rc = cache_pass(b, stream)) != MRT_OK) return rc;

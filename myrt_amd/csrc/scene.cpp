@@ -274,8 +274,8 @@ int Scene::upload_all() {
     if ((rc = upload(d_texels, cap_texels, tx)) != MRT_OK) return rc;
     shading_dirty = false;
   }
-  if (!d_ctrl) {
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_ctrl), 256);
+  if (!ctx[0].ctrl) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&ctx[0].ctrl), 256);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
     int dev = 0;
     hipGetDevice(&dev);
@@ -293,14 +293,14 @@ Scene::~Scene() {
     if (p) hipFree(p);
   if (d_blas) hipFree(d_blas);
   if (d_tris) hipFree(d_tris);
-  if (d_sdf_side) hipFree(d_sdf_side);
+  for (TraceScratch& c : ctx)
+    for (void* q : {(void*)c.ctrl, (void*)c.spill, (void*)c.sdf_side})
+      if (q) hipFree(q);
   for (void* q : {(void*)d_group_keys, (void*)d_group_perm, (void*)d_group_counts})
     if (q) hipFree(q);
   if (d_tlas) hipFree(d_tlas);
   if (d_tlas_items) hipFree(d_tlas_items);
   if (d_inst) hipFree(d_inst);
-  if (d_ctrl) hipFree(d_ctrl);
-  if (d_spill) hipFree(d_spill);
   for (uint32_t* q : {d_sort_keys, d_sort_perm, d_sort_counts})
     if (q) hipFree(q);
 }
@@ -326,10 +326,17 @@ PtScene Scene::shading_view() const {
 }
 
 int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream,
-                 const uint32_t* n_dev) {
+                 const uint32_t* n_dev, int context) {
   if (!prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
   if (n == 0) return MRT_OK;
   if (n >= 0x7fffffffu) return fail(MRT_ERR_LIMIT, "at most 2^31-1 rays per call");
+  if (context < 0 || context > 1) return fail(MRT_ERR_INVALID, "trace context must be 0 or 1");
+  TraceScratch& cx = ctx[context];
+  if (!cx.ctrl) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&cx.ctrl), 256);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc trace counters");
+  }
+  uint32_t* const d_ctrl = cx.ctrl;
   // Seven waves per SIMD pay off where rays change instances often (the per-iteration work is
   // larger and more divergent); a single mesh runs best at six. MRT_TRACE_WAVES overrides.
   const uint32_t waves = trace_waves ? trace_waves : (current.size() > 1 ? 7u : 6u);
@@ -340,13 +347,13 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   const uint32_t total_threads = uint32_t(grid) * kTraceBlock;
   const uint32_t spill_depth = max_stack > lds_depth ? max_stack - lds_depth : 1;
   const size_t spill_entries = size_t(spill_depth) * total_threads;
-  if (spill_entries > spill_cap) {
-    if (d_spill) hipFree(d_spill);
-    d_spill = nullptr;
-    spill_cap = 0;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_spill), spill_entries * 4);
+  if (spill_entries > cx.spill_cap) {
+    if (cx.spill) hipFree(cx.spill);
+    cx.spill = nullptr;
+    cx.spill_cap = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&cx.spill), spill_entries * 4);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc spill");
-    spill_cap = spill_entries;
+    cx.spill_cap = spill_entries;
   }
   hipError_t e = hipMemsetAsync(d_ctrl, 0, 256, stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
@@ -364,7 +371,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.counter = d_ctrl;
   p.flags = d_ctrl + 1;
   p.stats = reinterpret_cast<unsigned long long*>(d_ctrl + 8);
-  p.spill = d_spill;
+  p.spill = cx.spill;
   p.spill_depth = spill_depth;
   p.total_threads = total_threads;
   p.refill_below = refill_below;
@@ -372,7 +379,9 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.perm = nullptr;
   p.waves = waves;
   p.single_instance = current.size() == 1 ? 0u : 0xFFFFFFFFu;
-  if (ray_sort && n >= 4096) {
+  // The binning options keep one set of buffers and serve context 0 only (they change the order
+  // rays are taken in, never a result).
+  if (ray_sort && context == 0 && n >= 4096) {
     if (cap_sort < n || !d_sort_counts) {
       for (uint32_t* q : {d_sort_keys, d_sort_perm, d_sort_counts})
         if (q) hipFree(q);
@@ -398,7 +407,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     if (launch_ray_sort(ra, stream) != 0) return fail(MRT_ERR_HIP, "ray binning launch failed");
     p.perm = d_sort_perm;
   }
-  if (xcd_groups && !p.perm && n >= 65536) {
+  if (xcd_groups && context == 0 && !p.perm && n >= 65536) {
     // Group rays by origin slab so that each XCD's L2 serves mostly its own part of the BVH.
     if (cap_groups < n) {
       for (void* q : {(void*)d_group_keys, (void*)d_group_perm})
@@ -447,15 +456,15 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     sp.steps = sdf_steps;
     sp.eps = sdf_eps;
     sp.counter = d_ctrl + 2;  // zeroed with the traversal counters above
-    if (!any && sdf_side_cap < n) {
-      if (d_sdf_side) hipFree(d_sdf_side);
-      d_sdf_side = nullptr;
-      sdf_side_cap = 0;
-      hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&d_sdf_side), n * sizeof(float4));
+    if (!any && cx.sdf_side_cap < n) {
+      if (cx.sdf_side) hipFree(cx.sdf_side);
+      cx.sdf_side = nullptr;
+      cx.sdf_side_cap = 0;
+      hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&cx.sdf_side), n * sizeof(float4));
       if (e2 != hipSuccess) return hip_fail(e2, "hipMalloc sdf hit points");
-      sdf_side_cap = n;
+      cx.sdf_side_cap = n;
     }
-    sp.side = d_sdf_side;
+    sp.side = cx.sdf_side;
     sp.refill_below = 48;
     sp.prog_len = uint32_t(sdf_prog.size());
     sp.persistent = (sdf_persistent == 1 || (sdf_persistent == 2 && !any)) ? 1u : 0u;
@@ -466,11 +475,12 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
 }
 
 int Scene::read_ctrl(uint32_t* flags, unsigned long long* stats, hipStream_t stream) {
-  uint32_t host[16];
-  hipError_t e = hipMemcpyAsync(host, d_ctrl, 64, hipMemcpyDeviceToHost, stream);
+  uint32_t host[16], host1[16] = {};
+  hipError_t e = hipMemcpyAsync(host, ctx[0].ctrl, 64, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && ctx[1].ctrl) e = hipMemcpyAsync(host1, ctx[1].ctrl, 64, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) return hip_fail(e, "reading trace counters");
-  *flags = host[1];
+  *flags = host[1] | host1[1];
   if (stats) std::memcpy(stats, host + 8, 4 * sizeof(unsigned long long));
   return MRT_OK;
 }

@@ -103,16 +103,21 @@ struct Scene {
   float4* d_texels = nullptr;
   size_t cap_uv = 0, cap_textures = 0, cap_texels = 0;
   size_t cap_geo_info = 0, cap_idx = 0, cap_inst_mat = 0, cap_pos = 0, cap_nrm = 0, cap_mat = 0, cap_lights = 0;
-  uint32_t* d_ctrl = nullptr;  // [0] ray counter, [1] flags, [8..15] four u64 counters
-  uint32_t* d_spill = nullptr;
-  size_t spill_cap = 0;
-  float4* d_sdf_side = nullptr;  // per-ray SDF hit points between the march and the normal pass
+  // Per-trace scratch. Context 0 serves every trace by default; context 1 lets a second trace run
+  // at the same time on another stream (the renderer's shadow rays).
+  struct TraceScratch {
+    uint32_t* ctrl = nullptr;  // 256 B: [0] ray counter, [1] flags, [2] SDF ray counter, [8..15] four u64 counters
+    uint32_t* spill = nullptr;  // traversal stack entries beyond the LDS part
+    size_t spill_cap = 0;
+    float4* sdf_side = nullptr;  // per-ray SDF hit points between the march and the normal pass
+    size_t sdf_side_cap = 0;
+  };
+  TraceScratch ctx[2];
   bool xcd_groups = false;  // MRT_TRACE_XCD=1: rays grouped by origin slab, one group per XCD
   uint8_t* d_group_keys = nullptr;
   uint32_t *d_group_perm = nullptr, *d_group_counts = nullptr;
   size_t cap_groups = 0;
   int sdf_persistent = 0;  // MRT_SDF_PERSISTENT: 0 one thread per ray, 1 persistent lanes, 2 persistent for closest hits
-  size_t sdf_side_cap = 0;
   int num_cus = 0;
   uint32_t refill_below = 40;
   bool sah_builder = true;  // traversal trees from build_sah_bvh (MRT_BVH_BUILDER=0: the reference builder's)
@@ -133,7 +138,7 @@ struct Scene {
   int build_host();
   int upload_all();
   int trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream,
-            const uint32_t* n_dev = nullptr);
+            const uint32_t* n_dev = nullptr, int context = 0);
   PtScene shading_view() const;
   int read_ctrl(uint32_t* flags, unsigned long long* stats, hipStream_t stream);
 };

@@ -216,3 +216,31 @@ def test_material_sorted_shading_gives_the_same_image(ensure_built):
     say("max difference %.3e, image mean %.3f" % (diff.max(), imgs[0].mean()))
     assert imgs[0].mean() > 0.01
     assert np.all(diff <= 1e-5 * np.maximum(1.0, np.abs(imgs[0])))
+
+
+def test_shadow_stream_gives_the_same_image(ensure_built, monkeypatch):
+    """Tracing each bounce's shadow rays on a second stream, concurrently with the next bounce's
+    closest hits, keeps every data dependency: the image equals the one-stream render up to fp32
+    summation order of the atomic adds. The scene has an SDF cluster, so both the triangle and the
+    SDF shadow passes run in the second trace context."""
+    sc = scenes.sphere_field_scene(40, 2, seed=5, glass_every=4, metal_every=3)
+    tape = scenes.sdf_cluster(16, seed=2, k=0.1, box=0.8, n_materials=2)
+    W, H, spp, depth = 96, 64, 16, 6
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
+    imgs = []
+    for overlap in ("0", "1"):
+        monkeypatch.setenv("MRT_SHADOW_STREAM", overlap)
+        s = gpu_scene(sc)
+        s.enqueue_sdf(s.push_sdf(tape), scenes.translate_scale(tuple(sc["target"]), 1.0))
+        r = m.Renderer()
+        r.set_camera(view, proj)
+        r.set_num_bounces(depth)
+        say("render %dx%d %d spp, shadow stream %s" % (W, H, spp, overlap))
+        r.render(s, W, H, spp)
+        r.render(s, W, H, spp)  # a second call reuses the stream, events and trace contexts
+        imgs.append(r.image()[..., :3].astype(np.float64))
+        s.close()
+    diff = np.abs(imgs[0] - imgs[1])
+    say("max difference %.3e, image mean %.3f" % (diff.max(), imgs[0].mean()))
+    assert imgs[0].mean() > 0.01
+    assert np.all(diff <= 1e-5 * np.maximum(1.0, np.abs(imgs[0])))
