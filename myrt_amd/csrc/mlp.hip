@@ -40,23 +40,30 @@ __device__ inline bf16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const
 
 __device__ inline float sel3(int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); }
 
+// sin(2 pi r) and cos(2 pi r) of r revolutions: the hardware's transcendental units take their
+// argument in revolutions; the fractional part keeps it in their domain. The features are rounded
+// to bf16 next, far coarser than these units' error.
+__device__ inline float sin_rev(float r) { return __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(r)); }
+__device__ inline float cos_rev(float r) { return __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(r)); }
+
 // The 64 network inputs of one query (input encoding): position in the cache's unit box with
-// sin/cos at 6 octaves (36), direction and normal with sin/cos at 2 octaves (12 + 12),
-// roughness (1) and albedo (3). q0..q3 are the query's four float4 rows.
+// sin/cos of 2^k pi x at 6 octaves (36), direction and normal with sin/cos at 2 octaves (12 + 12),
+// roughness (1) and albedo (3). q0..q3 are the query's four float4 rows. 2^k pi x radians are
+// 2^(k-1) x revolutions (an exact scaling).
 __device__ inline float encode_feature(float4 q0, float4 q1, float4 q2, float4 q3, const CacheParams& P, int f) {
   if (f < 36) {
     const int axis = f / 12, r = f % 12, k = r >> 1;
     const float x = (sel3(axis, q0.x, q0.y, q0.z) - sel3(axis, P.lo[0], P.lo[1], P.lo[2])) *
                     sel3(axis, P.inv_ext[0], P.inv_ext[1], P.inv_ext[2]);
-    const float a = x * float(1 << k) * kPi;
-    return (r & 1) ? cosf(a) : sinf(a);
+    const float rev = x * float(1 << k) * 0.5f;
+    return (r & 1) ? cos_rev(rev) : sin_rev(rev);
   }
   if (f < 60) {
     const int g = f - 36;
     const float4 v = g < 12 ? q1 : q2;
     const int r = g % 12, axis = r / 4, k = (r >> 1) & 1;
-    const float a = sel3(axis, v.x, v.y, v.z) * float(1 << k) * kPi;
-    return (r & 1) ? cosf(a) : sinf(a);
+    const float rev = sel3(axis, v.x, v.y, v.z) * float(1 << k) * 0.5f;
+    return (r & 1) ? cos_rev(rev) : sin_rev(rev);
   }
   if (f == 60) return q0.w;
   return sel3(f - 61, q3.x, q3.y, q3.z);
@@ -98,26 +105,60 @@ __device__ inline void product(const uint16_t* a, const uint16_t* act, f32x16& d
   }
 }
 
-// Writes D (neurons x queries) back as [query][neuron] bf16, optionally through ReLU, optionally
-// masked by (mask[query][neuron] > 0).
-__device__ inline void store_tile(const f32x16& d0, const f32x16& d1, uint16_t* out, bool relu, const uint16_t* mask) {
+__device__ inline uint32_t pack_bf16(float lo, float hi) { return uint32_t(to_bf16(lo)) | (uint32_t(to_bf16(hi)) << 16); }
+
+// Writes D (neurons x queries) back as [query][neuron] bf16, optionally through ReLU. Lane (s, h)
+// holds neurons 8j + 4h + (0..3) of query s in d0[4j..4j+3] (and 32 more in d1): four consecutive
+// neurons, one 8-byte LDS write. Bit v of the returned mask is (stored value > 0) of d0[v], bit
+// 16 + v that of d1[v]; with use_mask, values whose bit in `mask` is clear are stored as zero
+// (the ReLU derivative of the backward pass).
+__device__ inline uint32_t store_tile(const f32x16& d0, const f32x16& d1, uint16_t* out, bool relu, bool use_mask,
+                                      uint32_t mask) {
   const int lane = __lane_id();
   const int s = lane & 31, h = lane >> 5;
+  uint32_t pos = 0;
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int n0 = 8 * (v >> 2) + 4 * h + (v & 3);
-    float x0 = d0[v], x1 = d1[v];
-    if (relu) {
-      x0 = fmaxf(x0, 0.f);
-      x1 = fmaxf(x1, 0.f);
+  for (int j = 0; j < 4; ++j) {
+    float x[2][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int v = 4 * j + i;
+      x[0][i] = d0[v];
+      x[1][i] = d1[v];
+      if (relu) {
+        x[0][i] = fmaxf(x[0][i], 0.f);
+        x[1][i] = fmaxf(x[1][i], 0.f);
+      }
+      if (use_mask) {
+        if (!((mask >> v) & 1u)) x[0][i] = 0.f;
+        if (!((mask >> (16 + v)) & 1u)) x[1][i] = 0.f;
+      }
     }
-    if (mask) {
-      if (!(from_bf16(mask[s * kStride + n0]) > 0.f)) x0 = 0.f;
-      if (!(from_bf16(mask[s * kStride + 32 + n0]) > 0.f)) x1 = 0.f;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const uint2 w = make_uint2(pack_bf16(x[m][0], x[m][1]), pack_bf16(x[m][2], x[m][3]));
+      *reinterpret_cast<uint2*>(out + s * kStride + 32 * m + 8 * j + 4 * h) = w;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint16_t bits = uint16_t(((i < 2 ? w.x : w.y) >> (16 * (i & 1))) & 0xffffu);
+        if (from_bf16(bits) > 0.f) pos |= 1u << (16 * m + 4 * j + i);
+      }
     }
-    out[s * kStride + n0] = to_bf16(x0);
-    out[s * kStride + 32 + n0] = to_bf16(x1);
   }
+  return pos;
+}
+
+// Tile [query][neuron] (row stride kStride) to dst[neuron][32 queries]: lane = neuron gathers its
+// column and writes it as 64 contiguous bytes; the wave writes 4 KB in one run.
+__device__ inline void save_columns(const uint16_t* tile, uint16_t* dst) {
+  const int neuron = __lane_id();
+  uint32_t w[16];
+#pragma unroll
+  for (int q = 0; q < 32; q += 2)
+    w[q / 2] = uint32_t(tile[q * kStride + neuron]) | (uint32_t(tile[(q + 1) * kStride + neuron]) << 16);
+  uint4* o = reinterpret_cast<uint4*>(dst + size_t(neuron) * 32);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) o[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
 __global__ __launch_bounds__(256) void infer_kernel(CacheParams P, const CacheQuery* qs, uint32_t n_host,
@@ -136,7 +177,7 @@ __global__ __launch_bounds__(256) void infer_kernel(CacheParams P, const CacheQu
     for (int l = 0; l < kLayers; ++l) {
       product(P.w + size_t(l) * kW * kW, a, d0, d1);
       if (l < kLayers - 1) {
-        store_tile(d0, d1, b, true, nullptr);
+        store_tile(d0, d1, b, true, false, 0u);
         __builtin_amdgcn_wave_barrier();
         uint16_t* tmp = a;
         a = b;
@@ -153,23 +194,33 @@ __global__ __launch_bounds__(256) void infer_kernel(CacheParams P, const CacheQu
   }
 }
 
-// Forward + backward of one tile per wave; writes H_l and G_l ([neuron][query], bf16) for the
-// weight-gradient reduction, and the tile's loss sum.
+// Forward + backward of one tile per wave; writes H_l and G_l for the weight-gradient reduction
+// and the tile's loss sum. Activation and gradient buffers are [layer][tile][neuron][32 queries]
+// (layer stride 64 * batch_stride). The ReLU pattern of every hidden layer stays in registers
+// (one bit per value a lane stores), so LDS holds only two activation and two gradient tiles.
 __global__ __launch_bounds__(64) void train_kernel(CacheParams P, const CacheQuery* qs, const float* targets,
                                                    uint32_t n, float grad_scale, uint16_t* h_out, uint16_t* g_out,
                                                    float* loss_out) {
-  __shared__ __attribute__((aligned(16))) uint16_t H[kLayers][32 * kStride];
+  __shared__ __attribute__((aligned(16))) uint16_t H[2][32 * kStride];
   __shared__ __attribute__((aligned(16))) uint16_t G[2][32 * kStride];
   const int lane = __lane_id();
   const int s = lane & 31, h = lane >> 5;
   const uint32_t t = blockIdx.x;
   const uint32_t qi = t * 32 + uint32_t(s);
+  const size_t layer_stride = size_t(kW) * P.batch_stride;
+  const size_t tile_off = size_t(t) * kW * 32;
   encode_tile(qs, t * 32, n, P, H[0]);
   __syncthreads();
+  uint32_t relu_mask[kLayers];  // [l]: ReLU pattern of H_l, l = 1..5
   f32x16 d0, d1;
+  int cur = 0;
   for (int l = 0; l < kLayers; ++l) {
-    product(P.w + size_t(l) * kW * kW, H[l], d0, d1);
-    if (l < kLayers - 1) store_tile(d0, d1, H[l + 1], true, nullptr);
+    save_columns(H[cur], h_out + l * layer_stride + tile_off);
+    product(P.w + size_t(l) * kW * kW, H[cur], d0, d1);
+    if (l < kLayers - 1) {
+      relu_mask[l + 1] = store_tile(d0, d1, H[cur ^ 1], true, false, 0u);
+      cur ^= 1;
+    }
     __syncthreads();
   }
   // Loss: relative L2 (y - t)^2 / (sg(y)^2 + 0.01) per channel; dL/dy = 2 (y - t) / (y^2 + 0.01).
@@ -188,29 +239,20 @@ __global__ __launch_bounds__(64) void train_kernel(CacheParams P, const CacheQue
   __syncthreads();
   for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
   if (lane == 0) loss_out[t] = loss;
-  int cur = 0;
-  const uint32_t ld = P.batch_stride;  // queries per row of the [neuron][query] buffers
+  int gc = 0;
   for (int l = kLayers - 1; l >= 0; --l) {
-    // Save G_l and H_l transposed for dW_l = G_l^T H_l.
-    {
-      const int neuron = lane;
-      uint16_t* go = g_out + (size_t(l) * kW + neuron) * ld + size_t(t) * 32;
-      uint16_t* ho = h_out + (size_t(l) * kW + neuron) * ld + size_t(t) * 32;
-      for (int q = 0; q < 32; ++q) {
-        go[q] = G[cur][q * kStride + neuron];
-        ho[q] = H[l][q * kStride + neuron];
-      }
-    }
+    save_columns(G[gc], g_out + l * layer_stride + tile_off);
     if (l > 0) {
-      product(P.wt + size_t(l) * kW * kW, G[cur], d0, d1);
-      store_tile(d0, d1, G[cur ^ 1], false, H[l]);
-      cur ^= 1;
+      product(P.wt + size_t(l) * kW * kW, G[gc], d0, d1);
+      store_tile(d0, d1, G[gc ^ 1], false, true, relu_mask[l]);
+      gc ^= 1;
       __syncthreads();
     }
   }
 }
 
-// dW_l partial over a range of queries: D(64 out x 64 in) = G_l[out][q] * H_l[in][q]^T.
+// dW_l partial over a range of queries: D(64 out x 64 in) = G_l[out][q] * H_l[in][q]^T, with G_l and
+// H_l in the [tile][neuron][32 queries] layout of train_kernel.
 __global__ __launch_bounds__(64) void wgrad_kernel(const uint16_t* h_in, const uint16_t* g_in, uint32_t batch_stride,
                                                    uint32_t n_pad, uint32_t chunk, float* partial) {
   const int lane = __lane_id();
@@ -222,11 +264,13 @@ __global__ __launch_bounds__(64) void wgrad_kernel(const uint16_t* h_in, const u
   const uint16_t* Hl = h_in + size_t(l) * kW * batch_stride;
   f32x16 acc[2][2] = {};
   for (uint32_t q = q0; q < q1; q += 16) {
+    // [tile][neuron][32 queries]: the 8 queries of a fragment lie in one tile row.
     const uint32_t k = q + 8 * uint32_t(hh);
-    const bf16x8 a0 = ld8(Gl + size_t(r) * batch_stride + k);
-    const bf16x8 a1 = ld8(Gl + size_t(32 + r) * batch_stride + k);
-    const bf16x8 b0 = ld8(Hl + size_t(r) * batch_stride + k);
-    const bf16x8 b1 = ld8(Hl + size_t(32 + r) * batch_stride + k);
+    const size_t base = size_t(k >> 5) * kW * 32 + (k & 31u);
+    const bf16x8 a0 = ld8(Gl + base + size_t(r) * 32);
+    const bf16x8 a1 = ld8(Gl + base + size_t(32 + r) * 32);
+    const bf16x8 b0 = ld8(Hl + base + size_t(r) * 32);
+    const bf16x8 b1 = ld8(Hl + base + size_t(32 + r) * 32);
     acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
     acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
     acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
