@@ -330,7 +330,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   if (!prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
   if (n == 0) return MRT_OK;
   if (n >= 0x7fffffffu) return fail(MRT_ERR_LIMIT, "at most 2^31-1 rays per call");
-  if (context < 0 || context > 1) return fail(MRT_ERR_INVALID, "trace context must be 0 or 1");
+  if (context < 0 || context >= kTraceContexts) return fail(MRT_ERR_INVALID, "trace context out of range");
   TraceScratch& cx = ctx[context];
   if (!cx.ctrl) {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&cx.ctrl), 256);
@@ -475,13 +475,15 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
 }
 
 int Scene::read_ctrl(uint32_t* flags, unsigned long long* stats, hipStream_t stream) {
-  uint32_t host[16], host1[16] = {};
-  hipError_t e = hipMemcpyAsync(host, ctx[0].ctrl, 64, hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess && ctx[1].ctrl) e = hipMemcpyAsync(host1, ctx[1].ctrl, 64, hipMemcpyDeviceToHost, stream);
+  uint32_t host[kTraceContexts][16] = {};
+  hipError_t e = hipSuccess;
+  for (int c = 0; c < kTraceContexts && e == hipSuccess; ++c)
+    if (ctx[c].ctrl) e = hipMemcpyAsync(host[c], ctx[c].ctrl, 64, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) return hip_fail(e, "reading trace counters");
-  *flags = host[1] | host1[1];
-  if (stats) std::memcpy(stats, host + 8, 4 * sizeof(unsigned long long));
+  *flags = 0;
+  for (int c = 0; c < kTraceContexts; ++c) *flags |= host[c][1];
+  if (stats) std::memcpy(stats, host[0] + 8, 4 * sizeof(unsigned long long));
   return MRT_OK;
 }
 

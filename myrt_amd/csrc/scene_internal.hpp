@@ -103,8 +103,8 @@ struct Scene {
   float4* d_texels = nullptr;
   size_t cap_uv = 0, cap_textures = 0, cap_texels = 0;
   size_t cap_geo_info = 0, cap_idx = 0, cap_inst_mat = 0, cap_pos = 0, cap_nrm = 0, cap_mat = 0, cap_lights = 0;
-  // Per-trace scratch. Context 0 serves every trace by default; context 1 lets a second trace run
-  // at the same time on another stream (the renderer's shadow rays).
+  // Per-trace scratch. Context 0 serves every trace by default; the others let traces run at the
+  // same time on other streams (the renderer's shadow rays and its second sample pipeline).
   struct TraceScratch {
     uint32_t* ctrl = nullptr;  // 256 B: [0] ray counter, [1] flags, [2] SDF ray counter, [8..15] four u64 counters
     uint32_t* spill = nullptr;  // traversal stack entries beyond the LDS part
@@ -112,7 +112,8 @@ struct Scene {
     float4* sdf_side = nullptr;  // per-ray SDF hit points between the march and the normal pass
     size_t sdf_side_cap = 0;
   };
-  TraceScratch ctx[2];
+  static constexpr int kTraceContexts = 4;
+  TraceScratch ctx[kTraceContexts];
   bool xcd_groups = false;  // MRT_TRACE_XCD=1: rays grouped by origin slab, one group per XCD
   uint8_t* d_group_keys = nullptr;
   uint32_t *d_group_perm = nullptr, *d_group_counts = nullptr;
