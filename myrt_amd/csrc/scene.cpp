@@ -246,7 +246,11 @@ int Scene::upload_all() {
       dev[j].tape_len = sdf_prog_len[sdf_current[j].sdf];
     }
     std::vector<SdfGrid> grids(dev.size());
-    for (size_t j = 0; j < dev.size(); ++j) grids[j] = sdf_grid[sdf_current[j].sdf];
+    sdf_chain_only = true;
+    for (size_t j = 0; j < dev.size(); ++j) {
+      grids[j] = sdf_grid[sdf_current[j].sdf];
+      if (grids[j].dim == 0) sdf_chain_only = false;
+    }
     std::vector<uint64_t> masks = sdf_masks;
     if (masks.empty()) masks.push_back(~0ull);
     if ((rc = upload(d_sdf_prog, cap_sdf_tape, sdf_prog)) != MRT_OK) return rc;
@@ -468,6 +472,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     sp.refill_below = 48;
     sp.prog_len = uint32_t(sdf_prog.size());
     sp.persistent = (sdf_persistent == 1 || (sdf_persistent == 2 && !any)) ? 1u : 0u;
+    sp.chain_only = sdf_chain_only ? 1u : 0u;
     if (launch_sdf(sp, any, num_cus, stream) != 0)
       return fail(MRT_ERR_HIP, "sdf kernel launch failed");
   }

@@ -118,6 +118,7 @@ struct Scene {
   uint8_t* d_group_keys = nullptr;
   uint32_t *d_group_perm = nullptr, *d_group_counts = nullptr;
   size_t cap_groups = 0;
+  bool sdf_chain_only = false;  // every enqueued SDF program has a culling grid
   int sdf_persistent = 0;  // MRT_SDF_PERSISTENT: 0 one thread per ray, 1 persistent lanes, 2 persistent for closest hits
   int num_cus = 0;
   uint32_t refill_below = 40;

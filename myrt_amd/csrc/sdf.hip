@@ -269,9 +269,13 @@ __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGr
   return r;
 }
 
-// Distance of drawable program at a point, culled when the program has a grid.
+// Distance of drawable program at a point, culled when the program has a grid. With CHAIN every
+// drawable of the launch has a grid, and the general interpreter (whose stacks hold 28 registers)
+// is left out of the kernel.
+template <bool CHAIN = false>
 __device__ inline float map_any(const SdfInsn* prog, uint32_t len, const SdfGrid& g, const uint64_t* masks, float px,
                                 float py, float pz, uint32_t& mat, CellCache& cc) {
+  if (CHAIN) return map_chain(prog, len, g, masks, px, py, pz, mat, cc);
   return g.dim ? map_chain(prog, len, g, masks, px, py, pz, mat, cc) : map(prog, len, px, py, pz, mat);
 }
 
@@ -309,6 +313,7 @@ __device__ inline bool clip(const SdfDrawable& dr, float ox, float oy, float oz,
 }
 
 // march_sdf (raymarch.glsl:116-171): world hit distance or -1.
+template <bool CHAIN>
 __device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, const SdfGrid& grid, const uint64_t* masks,
                               float ox, float oy, float oz, float dx, float dy, float dz, float t_enter, float t_exit,
                               int steps, float eps, float& lx, float& ly, float& lz, float& len_out) {
@@ -328,7 +333,7 @@ __device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, const 
   for (int i = 0; i < steps && t / len < t_exit; ++i) {
     const float s = t / len;
     uint32_t mat;
-    const float d = map_any(tape, dr.tape_len, grid, masks, lx + rx * s, ly + ry * s, lz + rz * s, mat, cc);
+    const float d = map_any<CHAIN>(tape, dr.tape_len, grid, masks, lx + rx * s, ly + ry * s, lz + rz * s, mat, cc);
     if (si == 0) si = d > 0.f ? 1 : (d < 0.f ? -1 : 0);
     if (fabsf(d) < eps) {
       res = t;
@@ -397,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void sdf_normal_kernel(SdfParams p) {
 // not hold whole waves idle. Each ray's arithmetic is the sequential algorithm's, in its order.
 // A closest hit is written as (t, drawable) with the object-space hit point in p.side;
 // sdf_normal_kernel completes the record.
-template <bool ANY>
+template <bool ANY, bool CHAIN>
 __device__ __attribute__((always_inline)) inline void march_persistent(const SdfParams& p, const SdfInsn* prog) {
   const int lane = __lane_id();
   const uint32_t n = p.n_dev ? min(*p.n_dev, p.n) : p.n;
@@ -483,7 +488,7 @@ __device__ __attribute__((always_inline)) inline void march_persistent(const Sdf
             const SdfDrawable& dr = p.drawables[j - 1];
             const float s = t / len;
             uint32_t mat;
-            const float d = map_any(prog + dr.tape_offset, dr.tape_len, p.grids[j - 1], p.masks, lx + rx * s,
+            const float d = map_any<CHAIN>(prog + dr.tape_offset, dr.tape_len, p.grids[j - 1], p.masks, lx + rx * s,
                                     ly + ry * s, lz + rz * s, mat, cc);
             if (si == 0) si = d > 0.f ? 1 : (d < 0.f ? -1 : 0);
             if (fabsf(d) < p.eps) {
@@ -532,7 +537,7 @@ __device__ __attribute__((always_inline)) inline void march_persistent(const Sdf
 // One thread per ray (the sequential algorithm as it stands): clip every drawable, march the
 // ones the segment [tmin, best] crosses, keep the closest hit. Occupancy is higher than the
 // persistent form's; a wave waits for its longest march.
-template <bool ANY>
+template <bool ANY, bool CHAIN>
 __device__ __attribute__((always_inline)) inline void march_per_ray(const SdfParams& p, const SdfInsn* prog) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t n = p.n_dev ? min(*p.n_dev, p.n) : p.n;
@@ -556,7 +561,7 @@ __device__ __attribute__((always_inline)) inline void march_per_ray(const SdfPar
     if (!clip(dr, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1)) continue;
     float lx, ly, lz, len;
     const SdfGrid grid = p.grids[j];
-    const float tw = march(dr, prog + dr.tape_offset, grid, p.masks, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1,
+    const float tw = march<CHAIN>(dr, prog + dr.tape_offset, grid, p.masks, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, t0, t1,
                            p.steps, p.eps, lx, ly, lz, len);
     if (tw >= 0.f && tw < best && tw > r0.w) {
       best = tw;
@@ -585,8 +590,19 @@ constexpr uint32_t kProgCache = 256;
 #ifndef MRT_SDF_WAVES
 #define MRT_SDF_WAVES 5
 #endif
-template <bool ANY, bool PERSIST>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PERSIST ? MRT_SDF_WAVES : 4, 8))) void sdf_kernel(
+// Waves per SIMD the marching kernels are compiled for. The march is latency bound (dependent
+// square-root and division chains, mask and program fetches), so occupancy pays even where the
+// register budget of 8 waves spills a few values to scratch: render of the 4M-triangle + SDF
+// scene 0.130 s at 4 waves, 0.124 s at 8; with the chain-only kernel 0.125 s at 6, 0.123 s at 8.
+#ifndef MRT_SDF_RAY_WAVES
+#define MRT_SDF_RAY_WAVES 8
+#endif
+#ifndef MRT_SDF_CHAIN_WAVES
+#define MRT_SDF_CHAIN_WAVES 8
+#endif
+template <bool ANY, bool PERSIST, bool CHAIN>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
+    PERSIST ? MRT_SDF_WAVES : (CHAIN ? MRT_SDF_CHAIN_WAVES : MRT_SDF_RAY_WAVES), 8))) void sdf_kernel(
     SdfParams p) {
   __shared__ uint4 s_prog[kProgCache * (sizeof(SdfInsn) / 16)];
   static_assert(sizeof(SdfInsn) % 16 == 0, "instruction records are whole 16-byte words");
@@ -603,40 +619,54 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PERSIST 
   if (cached) {
     const SdfInsn* prog = reinterpret_cast<const SdfInsn*>(s_prog);
     if (PERSIST)
-      march_persistent<ANY>(p, prog);
+      march_persistent<ANY, CHAIN>(p, prog);
     else
-      march_per_ray<ANY>(p, prog);
+      march_per_ray<ANY, CHAIN>(p, prog);
   } else {
     if (PERSIST)
-      march_persistent<ANY>(p, p.prog);
+      march_persistent<ANY, CHAIN>(p, p.prog);
     else
-      march_per_ray<ANY>(p, p.prog);
+      march_per_ray<ANY, CHAIN>(p, p.prog);
   }
 }
 
 }  // namespace sdfk
 
-int launch_sdf(const SdfParams& p, bool any, int num_cus, hipStream_t stream) {
+namespace {
+template <bool ANY, bool PERSIST, bool CHAIN>
+void launch_march(const SdfParams& p, int num_cus, hipStream_t stream) {
   int grid = int((p.n + sdfk::kBlock - 1) / sdfk::kBlock);
-  if (p.persistent) {
+  if (PERSIST) {
     int blocks = 0;
     const hipError_t e =
-        any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sdfk::sdf_kernel<true, true>, sdfk::kBlock, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sdfk::sdf_kernel<false, true>, sdfk::kBlock, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sdfk::sdf_kernel<ANY, PERSIST, CHAIN>, sdfk::kBlock, 0);
     if (e != hipSuccess || blocks < 1) blocks = 1;
     grid = std::min(grid, num_cus * blocks);
   }
   grid = std::max(grid, 1);
-  if (any) {
-    if (p.persistent)
-      hipLaunchKernelGGL((sdfk::sdf_kernel<true, true>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+  hipLaunchKernelGGL((sdfk::sdf_kernel<ANY, PERSIST, CHAIN>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+}
+template <bool ANY>
+void launch_march_any(const SdfParams& p, int num_cus, hipStream_t stream) {
+  if (p.persistent) {
+    if (p.chain_only)
+      launch_march<ANY, true, true>(p, num_cus, stream);
     else
-      hipLaunchKernelGGL((sdfk::sdf_kernel<true, false>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+      launch_march<ANY, true, false>(p, num_cus, stream);
   } else {
-    if (p.persistent)
-      hipLaunchKernelGGL((sdfk::sdf_kernel<false, true>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+    if (p.chain_only)
+      launch_march<ANY, false, true>(p, num_cus, stream);
     else
-      hipLaunchKernelGGL((sdfk::sdf_kernel<false, false>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+      launch_march<ANY, false, false>(p, num_cus, stream);
+  }
+}
+}  // namespace
+
+int launch_sdf(const SdfParams& p, bool any, int num_cus, hipStream_t stream) {
+  if (any) {
+    launch_march_any<true>(p, num_cus, stream);
+  } else {
+    launch_march_any<false>(p, num_cus, stream);
     const unsigned nb = unsigned((p.n + sdfk::kBlock - 1) / sdfk::kBlock);
     hipLaunchKernelGGL(sdfk::sdf_normal_kernel, dim3(nb ? nb : 1), dim3(sdfk::kBlock), 0, stream, p);
   }

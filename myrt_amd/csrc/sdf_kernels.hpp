@@ -25,6 +25,7 @@ struct SdfParams {
   uint32_t refill_below;   // refill a wave when fewer lanes than this are busy
   uint32_t prog_len;       // instructions in prog (all drawables)
   uint32_t persistent;     // 1: persistent lanes with refill; 0: one thread per ray
+  uint32_t chain_only;     // 1: every drawable's program has a culling grid (a leaner kernel serves)
 };
 
 int launch_sdf(const SdfParams& p, bool any, int num_cus, hipStream_t stream);

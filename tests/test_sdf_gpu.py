@@ -106,3 +106,41 @@ def test_sdf_culled_chain_is_exact(ensure_built, op):
     ref, ref_n = orc.sdf_trace(tape, dr, rays)
     assert (gpu["inst"] != m.MISS).mean() > 0.2
     check(gpu, ref, ref_n, "chain " + op)
+
+
+def general_program():
+    """A program the culling grid does not cover: mirror and translate modifiers, smooth and plain
+    subtraction, a torus, a Menger sponge, unions. It runs through the general tape interpreter."""
+    return scenes.sdf_tape([
+        ("CAPSULE", 2, 0.6, 0.15),
+        ("PUSH_MIRROR", 0, 0), ("PUSH_TRANSLATE", 0, 0.6, 0.0, 0.0), ("SPHERE", 1, 0.45), ("POP", 0), ("POP", 0),
+        ("BOX", 2, 0.5, 0.3, 0.4), ("SMOOTH_SUBTRACT", 0, 0.05), ("TORUS", 3, 0.7, 0.1), ("UNION", 0),
+        ("PUSH_TRANSLATE", 0, 0.0, 0.8, 0.0), ("MENGER", 1), ("POP", 0), ("SMOOTH_UNION", 0, 0.1),
+        ("SUBTRACT", 0)])
+
+
+@pytest.mark.parametrize("with_chain", [False, True])
+def test_sdf_general_programs(ensure_built, with_chain):
+    """Programs without a culling grid, alone (the general kernel) and next to a culled chain (both
+    drawables then march in the general kernel): hits, materials and t bit-exact, normals close."""
+    tape = general_program()
+    s = m.Scene()
+    s.enqueue_sdf(s.push_sdf(tape), scenes.translate_scale((0.0, 0.2, 0.0), 1.2))
+    tapes = [tape]
+    if with_chain:
+        chain = scenes.sdf_cluster(16, seed=9, box=0.6)
+        s.enqueue_sdf(s.push_sdf(chain), scenes.translate_scale((1.8, 0.0, 0.3), 0.8))
+        tapes.append(chain)
+    s.prepare()
+    dr = s.sdf_drawables(len(tapes))
+    rays = rays_at((0.5, 0.2, 0.0), 20000, 21, spread=3.0)
+    gpu = s.trace_closest(rays)
+    # The drawable records address the scene's tapes back to back, in push order.
+    joint = np.concatenate(tapes)
+    ref, ref_n = orc.sdf_trace(joint, dr, rays)
+    assert (gpu["inst"] != m.MISS).mean() > 0.15
+    check(gpu, ref, ref_n, "general" + (" + chain" if with_chain else ""))
+    rays["tmax"] = np.random.default_rng(22).uniform(1, 10, rays.size).astype(np.float32)
+    occ = s.trace_any(rays)
+    ref_any, _ = orc.sdf_trace(joint, dr, rays, any_hit=True)
+    assert np.array_equal(occ.astype(bool), np.isfinite(ref_any["t"]))
