@@ -196,8 +196,12 @@ __global__ __launch_bounds__(kBlock) void conv_tile_kernel(ConvArgs a, int group
   constexpr int kCout = NM * 32;
   constexpr int kWItems = 2 * 9 * kCout;
   constexpr int kWPer = (kWItems + kBlock - 1) / kBlock;
-  __shared__ h8 s_in[kHaloItems];
-  __shared__ h8 s_w[kWItems];
+  // One LDS block: the input halo and the weights while computing, then the output tile.
+  constexpr int kOutPitch = 5;  // 16-byte records per staged pixel: 4 of data, 1 of padding
+  static_assert(kTileRows * kTileCols * kOutPitch <= kHaloItems + kWItems, "output tile fits the LDS block");
+  __shared__ h8 s_buf[kHaloItems + kWItems];
+  h8* const s_in = s_buf;
+  h8* const s_w = s_buf + kHaloItems;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int col = lane & 31, half = lane >> 5;
   const int ntx = (a.W + kTileCols - 1) / kTileCols, nty = (a.H + kTileRows - 1) / kTileRows;
@@ -293,30 +297,44 @@ __global__ __launch_bounds__(kBlock) void conv_tile_kernel(ConvArgs a, int group
     }
   }
   // Lane (col, half) of block (mb, nb) holds pixel x0 + 32 nb + col of row y0 + wave and output
-  // channels m0 + 32 mb + 8 j + 4 half + i (element 4 j + i).
-  const int y = y0 + wave;
-  if (y >= a.H) return;
+  // channels m0 + 32 mb + 8 j + 4 half + i (element 4 j + i). Each block of 32 channels goes
+  // through LDS as [row][pixel][4 records of 8 channels] (pixel pitch 5 records, which spreads a
+  // half wave's 8-byte writes over the banks), so that the global stores write whole 16-byte
+  // records of contiguous 64-byte pixel runs: a row of the tile is one 4 KB run when the layer
+  // has 32 output channels, instead of 8-byte pieces 64 bytes apart.
   _Float16* out = static_cast<_Float16*>(a.out);
+  __syncthreads();  // every wave is done with the halo and the weights
 #pragma unroll
   for (int mb = 0; mb < NM; ++mb) {
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
-      const int px = x0 + nb * 32 + col;
-      if (px >= a.W) continue;
-      const size_t base = ((size_t(img) * a.H + y) * a.W + px) * a.cout;
+      const int px = nb * 32 + col;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int co = m0 + mb * 32 + 8 * j + 4 * half;
-        float v[4];
+        h4 v;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          v[i] = acc[mb][nb][4 * j + i];
-          if (a.bias) v[i] += a.bias[co + i];
-          if (a.relu) v[i] = fmaxf(v[i], 0.0f);
+          float x = acc[mb][nb][4 * j + i];
+          if (a.bias) x += a.bias[co + i];
+          if (a.relu) x = fmaxf(x, 0.0f);
+          v[i] = (_Float16)x;
         }
-        Frag<_Float16>::store4(out + base + co, v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<h4*>(reinterpret_cast<char*>(s_buf + (wave * kTileCols + px) * kOutPitch + j) + 8 * half) = v;
       }
     }
+    __syncthreads();
+    constexpr int kRecords = kTileRows * kTileCols * 4;
+#pragma unroll
+    for (int it = 0; it < kRecords / kBlock; ++it) {
+      const int k = tid + it * kBlock;
+      const int row = k / (kTileCols * 4), rem = k % (kTileCols * 4), pix = rem >> 2, q = rem & 3;
+      const int y = y0 + row, px = x0 + pix;
+      if (y < a.H && px < a.W)
+        *reinterpret_cast<h8*>(out + ((size_t(img) * a.H + y) * a.W + px) * a.cout + m0 + mb * 32 + 8 * q) =
+            s_buf[(row * kTileCols + pix) * kOutPitch + q];
+    }
+    if (mb + 1 < NM) __syncthreads();  // the next block's staging overwrites this one
   }
 }
 
