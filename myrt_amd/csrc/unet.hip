@@ -175,36 +175,37 @@ __global__ __launch_bounds__(kBlock) void conv_kernel(ConvArgs a) {
   }
 }
 
-// Tiled fp16 3x3 convolution for inference. A workgroup of 4 waves computes a tile of kTileRows
+// Tiled fp16 3x3 convolution for inference. A workgroup of ROWS waves computes a tile of ROWS
 // output rows x 64 pixels x NM blocks of 32 output channels (output-channel group g of the
 // layer's cout / (32 NM)); wave w owns row w of the tile. For each chunk of 16 input channels the
-// workgroup stages in LDS the input halo of the tile (kTileRows + 2 rows x 66 pixels) and the
+// workgroup stages in LDS the input halo of the tile (ROWS + 2 rows x 66 pixels) and the
 // chunk's weights for all 9 taps, then every wave runs 9 taps x NM x 2 MFMAs (32x32x16) out of LDS:
 // an input element is read from L2 once per tile and channel group instead of once per tap. The
 // next chunk's global loads are issued into registers before the current chunk's MFMAs, so their
 // latency overlaps the math. LDS layouts put the two 8-channel halves of a fragment in separate
 // planes, so a half wave reads 32 consecutive 16-byte records (no bank conflicts): input
 // [half][row][col][8], weights [half][tap][cout][8].
-constexpr int kTileRows = 4;
 constexpr int kTileCols = 64;
-constexpr int kHaloRows = kTileRows + 2, kHaloCols = kTileCols + 2;
-constexpr int kHaloItems = 2 * kHaloRows * kHaloCols;                 // 16-byte records
-constexpr int kHaloPer = (kHaloItems + kBlock - 1) / kBlock;
+constexpr int kHaloCols = kTileCols + 2;
 
-template <int NM>
-__global__ __launch_bounds__(kBlock) void conv_tile_kernel(ConvArgs a, int groups) {
+template <int NM, int ROWS>
+__global__ __launch_bounds__(ROWS * 64) void conv_tile_kernel(ConvArgs a, int groups) {
+  constexpr int kThreads = ROWS * 64;  // one wave per output row of the tile
+  constexpr int kHRows = ROWS + 2;
+  constexpr int kHItems = 2 * kHRows * kHaloCols;  // 16-byte records of the input halo
+  constexpr int kHPer = (kHItems + kThreads - 1) / kThreads;
   constexpr int kCout = NM * 32;
   constexpr int kWItems = 2 * 9 * kCout;
-  constexpr int kWPer = (kWItems + kBlock - 1) / kBlock;
+  constexpr int kWPer = (kWItems + kThreads - 1) / kThreads;
   // One LDS block: the input halo and the weights while computing, then the output tile.
   constexpr int kOutPitch = 5;  // 16-byte records per staged pixel: 4 of data, 1 of padding
-  static_assert(kTileRows * kTileCols * kOutPitch <= kHaloItems + kWItems, "output tile fits the LDS block");
-  __shared__ h8 s_buf[kHaloItems + kWItems];
+  constexpr int kBuf = kHItems + kWItems > ROWS * kTileCols * kOutPitch ? kHItems + kWItems : ROWS * kTileCols * kOutPitch;
+  __shared__ h8 s_buf[kBuf];
   h8* const s_in = s_buf;
-  h8* const s_w = s_buf + kHaloItems;
+  h8* const s_w = s_buf + kHItems;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int col = lane & 31, half = lane >> 5;
-  const int ntx = (a.W + kTileCols - 1) / kTileCols, nty = (a.H + kTileRows - 1) / kTileRows;
+  const int ntx = (a.W + kTileCols - 1) / kTileCols, nty = (a.H + ROWS - 1) / ROWS;
   int t = blockIdx.x;
   const int g = t % groups;
   t /= groups;
@@ -212,20 +213,20 @@ __global__ __launch_bounds__(kBlock) void conv_tile_kernel(ConvArgs a, int group
   t /= ntx;
   const int ty = t % nty;
   const int img = t / nty;
-  const int x0 = tx * kTileCols, y0 = ty * kTileRows, m0 = g * kCout;
+  const int x0 = tx * kTileCols, y0 = ty * ROWS, m0 = g * kCout;
   const int cin = a.c0 + a.c1;
   const int H0 = a.up0 ? (a.H >> 1) : a.H, W0 = a.up0 ? (a.W >> 1) : a.W;
   const _Float16* in0 = static_cast<const _Float16*>(a.in0);
   const _Float16* in1 = static_cast<const _Float16*>(a.in1);
   const _Float16* w = static_cast<const _Float16*>(a.w);
 
-  h8 r_in[kHaloPer], r_w[kWPer];
+  h8 r_in[kHPer], r_w[kWPer];
   const auto fetch = [&](int c) {
 #pragma unroll
-    for (int k = 0; k < kHaloPer; ++k) {
-      const int i = tid + k * kBlock;
+    for (int k = 0; k < kHPer; ++k) {
+      const int i = tid + k * kThreads;
       h8 v = {};
-      if (i < kHaloItems) {
+      if (i < kHItems) {
         const int hf = i & 1, pix = i >> 1;
         const int r = pix / kHaloCols, cc = pix - r * kHaloCols;
         const int sy = y0 - 1 + r, sx = x0 - 1 + cc;
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void conv_tile_kernel(ConvArgs a, int group
     }
 #pragma unroll
     for (int k = 0; k < kWPer; ++k) {
-      const int i = tid + k * kBlock;
+      const int i = tid + k * kThreads;
       if (i < kWItems) {
         const int hf = i & 1, rest = i >> 1;
         const int co = rest / 9, tap = rest - co * 9;
@@ -253,17 +254,17 @@ __global__ __launch_bounds__(kBlock) void conv_tile_kernel(ConvArgs a, int group
   };
   const auto stage = [&]() {
 #pragma unroll
-    for (int k = 0; k < kHaloPer; ++k) {
-      const int i = tid + k * kBlock;
-      if (i < kHaloItems) {
+    for (int k = 0; k < kHPer; ++k) {
+      const int i = tid + k * kThreads;
+      if (i < kHItems) {
         const int hf = i & 1, pix = i >> 1;
         const int r = pix / kHaloCols, cc = pix - r * kHaloCols;
-        s_in[(hf * kHaloRows + r) * kHaloCols + cc] = r_in[k];
+        s_in[(hf * kHRows + r) * kHaloCols + cc] = r_in[k];
       }
     }
 #pragma unroll
     for (int k = 0; k < kWPer; ++k) {
-      const int i = tid + k * kBlock;
+      const int i = tid + k * kThreads;
       if (i < kWItems) {
         const int hf = i & 1, rest = i >> 1;
         const int co = rest / 9, tap = rest - co * 9;
@@ -286,8 +287,8 @@ __global__ __launch_bounds__(kBlock) void conv_tile_kernel(ConvArgs a, int group
     for (int tap = 0; tap < 9; ++tap) {
       const int dy = tap / 3, dx = tap % 3;
       const int row = wave + dy;
-      const h8 b0 = s_in[(half * kHaloRows + row) * kHaloCols + col + dx];
-      const h8 b1 = s_in[(half * kHaloRows + row) * kHaloCols + 32 + col + dx];
+      const h8 b0 = s_in[(half * kHRows + row) * kHaloCols + col + dx];
+      const h8 b1 = s_in[(half * kHRows + row) * kHaloCols + 32 + col + dx];
 #pragma unroll
       for (int mb = 0; mb < NM; ++mb) {
         const h8 av = s_w[(half * 9 + tap) * kCout + mb * 32 + col];
@@ -324,10 +325,10 @@ __global__ __launch_bounds__(kBlock) void conv_tile_kernel(ConvArgs a, int group
       }
     }
     __syncthreads();
-    constexpr int kRecords = kTileRows * kTileCols * 4;
+    constexpr int kRecords = ROWS * kTileCols * 4;
 #pragma unroll
-    for (int it = 0; it < kRecords / kBlock; ++it) {
-      const int k = tid + it * kBlock;
+    for (int it = 0; it < kRecords / kThreads; ++it) {
+      const int k = tid + it * kThreads;
       const int row = k / (kTileCols * 4), rem = k % (kTileCols * 4), pix = rem >> 2, q = rem & 3;
       const int y = y0 + row, px = x0 + pix;
       if (y < a.H && px < a.W)
@@ -596,8 +597,13 @@ int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
     // Inference 3x3: the LDS-tiled kernel, one workgroup per 4 x 64 output tile and group of
     // 32 NM output channels. NM is the largest divisor of cout / 32 that still gives at least
     // 1024 workgroups (4 per CU), so the coarse levels split their channels across workgroups.
-    const unsigned tiles = unsigned(a.B) * unsigned((a.H + unk::kTileRows - 1) / unk::kTileRows) *
-                           unsigned((a.W + unk::kTileCols - 1) / unk::kTileCols);
+    // Where 8-row tiles still give 1024 workgroups (the full-resolution level), they take the
+    // place of 4-row ones: the halo re-read falls from 1.5x to 1.25x and each chunk's weights
+    // serve twice the pixels.
+    const auto tiles_of = [&](int rows) {
+      return unsigned(a.B) * unsigned((a.H + rows - 1) / rows) * unsigned((a.W + unk::kTileCols - 1) / unk::kTileCols);
+    };
+    const unsigned tiles = tiles_of(4);
     const int mt = a.cout / 32;
     int nm = 1;
     for (int d = mt; d >= 1; --d)
@@ -606,12 +612,20 @@ int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
         break;
       }
     const int groups = mt / nm;
-    const dim3 grid(tiles * unsigned(groups));
-    switch (nm) {
-      case 1: hipLaunchKernelGGL((unk::conv_tile_kernel<1>), grid, dim3(unk::kBlock), 0, s, a, groups); break;
-      case 2: hipLaunchKernelGGL((unk::conv_tile_kernel<2>), grid, dim3(unk::kBlock), 0, s, a, groups); break;
-      case 3: hipLaunchKernelGGL((unk::conv_tile_kernel<3>), grid, dim3(unk::kBlock), 0, s, a, groups); break;
-      default: hipLaunchKernelGGL((unk::conv_tile_kernel<4>), grid, dim3(unk::kBlock), 0, s, a, groups); break;
+    const bool tall = nm <= 2 && tiles_of(8) * unsigned(groups) >= 1024u;
+    const dim3 grid((tall ? tiles_of(8) : tiles) * unsigned(groups));
+    if (tall) {
+      if (nm == 1)
+        hipLaunchKernelGGL((unk::conv_tile_kernel<1, 8>), grid, dim3(8 * 64), 0, s, a, groups);
+      else
+        hipLaunchKernelGGL((unk::conv_tile_kernel<2, 8>), grid, dim3(8 * 64), 0, s, a, groups);
+    } else {
+      switch (nm) {
+        case 1: hipLaunchKernelGGL((unk::conv_tile_kernel<1, 4>), grid, dim3(4 * 64), 0, s, a, groups); break;
+        case 2: hipLaunchKernelGGL((unk::conv_tile_kernel<2, 4>), grid, dim3(4 * 64), 0, s, a, groups); break;
+        case 3: hipLaunchKernelGGL((unk::conv_tile_kernel<3, 4>), grid, dim3(4 * 64), 0, s, a, groups); break;
+        default: hipLaunchKernelGGL((unk::conv_tile_kernel<4, 4>), grid, dim3(4 * 64), 0, s, a, groups); break;
+      }
     }
     return unk::ok();
   }

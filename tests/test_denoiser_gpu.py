@@ -73,6 +73,29 @@ def test_forward_fp32_and_fp16_match_oracle(ensure_built):
     assert err16 < 3e-2
 
 
+def test_fp16_tiled_forward_matches_fp32_at_full_size(ensure_built):
+    """At 1024x2048 the fp16 inference takes the 8-row tiled convolutions (levels 0 and 1, one
+    and two 32-channel blocks) and the 4-row ones below; it must stay within the fp16 bar of the
+    fp32 forward (untiled kernels, pinned to the oracle at small size above)."""
+    d = m.Denoiser(seed=3)
+    p = d.params()
+    rng = np.random.default_rng(6)
+    L = U.unpack(p.astype(np.float64))
+    for name in L:
+        W, b = L[name]
+        L[name] = (W * (5.0 if name == "out" else 1.0), rng.normal(scale=0.05, size=b.shape))
+    d.set_params(U.pack(L).astype(np.float32))
+    raw = raw_batch(1, 1024, 2048, seed=7)
+    say("denoiser forward 1x1024x2048 fp32 and fp16")
+    y32 = d.forward(raw, half=False)
+    y16 = d.forward(raw, half=True)
+    scale = np.abs(y32).max()
+    err16 = np.abs(y16 - y32).max() / scale
+    say("max relative difference fp16 vs fp32 %.2e (scale %.3f)" % (err16, scale))
+    assert scale > 0.05 and np.isfinite(y16).all()
+    assert err16 < 3e-2
+
+
 def test_gradient_and_adam_match_oracle(ensure_built):
     d = m.Denoiser(seed=8)
     p = d.params().astype(np.float64)
