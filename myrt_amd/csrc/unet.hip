@@ -597,9 +597,9 @@ int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
     // Inference 3x3: the LDS-tiled kernel, one workgroup per 4 x 64 output tile and group of
     // 32 NM output channels. NM is the largest divisor of cout / 32 that still gives at least
     // 1024 workgroups (4 per CU), so the coarse levels split their channels across workgroups.
-    // Where 8-row tiles still give 1024 workgroups (the full-resolution level), they take the
-    // place of 4-row ones: the halo re-read falls from 1.5x to 1.25x and each chunk's weights
-    // serve twice the pixels.
+    // Where 8-row tiles of one 32-channel block still give 1024 workgroups (the two finest
+    // levels at 1080p), they take the place of 4-row ones: the halo re-read falls from 1.5x to
+    // 1.25x and each chunk's weights serve twice the pixels.
     const auto tiles_of = [&](int rows) {
       return unsigned(a.B) * unsigned((a.H + rows - 1) / rows) * unsigned((a.W + unk::kTileCols - 1) / unk::kTileCols);
     };
@@ -611,14 +611,17 @@ int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
         nm = d;
         break;
       }
-    const int groups = mt / nm;
-    const bool tall = nm <= 2 && tiles_of(8) * unsigned(groups) >= 1024u;
+    int groups = mt / nm;
+    // One 32-channel block per workgroup keeps the 8-row kernel at 90 VGPRs (5 waves per SIMD;
+    // two blocks take 150), so it is preferred wherever it alone gives the 1024 workgroups.
+    const bool tall = tiles_of(8) * unsigned(mt) >= 1024u;
+    if (tall) {
+      nm = 1;
+      groups = mt;
+    }
     const dim3 grid((tall ? tiles_of(8) : tiles) * unsigned(groups));
     if (tall) {
-      if (nm == 1)
-        hipLaunchKernelGGL((unk::conv_tile_kernel<1, 8>), grid, dim3(8 * 64), 0, s, a, groups);
-      else
-        hipLaunchKernelGGL((unk::conv_tile_kernel<2, 8>), grid, dim3(8 * 64), 0, s, a, groups);
+      hipLaunchKernelGGL((unk::conv_tile_kernel<1, 8>), grid, dim3(8 * 64), 0, s, a, groups);
     } else {
       switch (nm) {
         case 1: hipLaunchKernelGGL((unk::conv_tile_kernel<1, 4>), grid, dim3(4 * 64), 0, s, a, groups); break;

@@ -74,9 +74,9 @@ def test_forward_fp32_and_fp16_match_oracle(ensure_built):
 
 
 def test_fp16_tiled_forward_matches_fp32_at_full_size(ensure_built):
-    """At 1024x2048 the fp16 inference takes the 8-row tiled convolutions (levels 0 and 1, one
-    and two 32-channel blocks) and the 4-row ones below; it must stay within the fp16 bar of the
-    fp32 forward (untiled kernels, pinned to the oracle at small size above)."""
+    """At 1024x2048 the fp16 inference takes the 8-row tiled convolutions (levels 0 and 1, the
+    latter as two 32-channel groups) and the 4-row ones below; it must stay within the fp16 bar of
+    the fp32 forward (untiled kernels, pinned to the oracle at small size above)."""
     d = m.Denoiser(seed=3)
     p = d.params()
     rng = np.random.default_rng(6)
