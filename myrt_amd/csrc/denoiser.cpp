@@ -85,8 +85,10 @@ uint64_t splitmix_next(uint64_t& x) {
 }
 
 int conv_call(const UnetLayer& u, const void* in0, int up0, const void* in1, const void* w, const float* bias,
-              void* out, const void* mask, int relu, int B, int H, int W, bool half, hipStream_t s) {
+              void* out, const void* mask, int relu, int B, int H, int W, bool half, hipStream_t s,
+              void* pool_out = nullptr) {
   ConvArgs a{};
+  a.pool_out = pool_out;
   a.in0 = in0;
   a.in1 = in1;
   a.w = w;
@@ -241,16 +243,13 @@ int Denoiser::forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, i
   if (launch_unet_prep<T>(raw, A.x.p, B, Hs, Ws, H, W, s)) return fail(MRT_ERR_HIP, "denoiser input");
   for (int l = 0; l < kUnetLevels; ++l) {
     const int Hl = H >> l, Wl = W >> l;
-    const T* src = A.x.p;
-    if (l > 0) {
-      if (launch_unet_pool<T>(A.eb[l - 1].p, A.p[l].p, B, Hl * 2, Wl * 2, phys_level(l - 1), s))
-        return fail(MRT_ERR_HIP, "denoiser pooling");
-      src = A.p[l].p;
-    }
+    // The level's second convolution also writes the 2x2 average that feeds the next level.
+    const T* src = l > 0 ? A.p[l].p : A.x.p;
+    void* pooled = l + 1 < kUnetLevels ? static_cast<void*>(A.p[l + 1].p) : nullptr;
     if ((rc = conv_call(L[kEnc(l, 0)], src, 0, nullptr, wl(kEnc(l, 0)), b.p + L[kEnc(l, 0)].b_phys, A.ea[l].p, nullptr,
                         1, B, Hl, Wl, half, s)) != MRT_OK ||
         (rc = conv_call(L[kEnc(l, 1)], A.ea[l].p, 0, nullptr, wl(kEnc(l, 1)), b.p + L[kEnc(l, 1)].b_phys, A.eb[l].p,
-                        nullptr, 1, B, Hl, Wl, half, s)) != MRT_OK)
+                        nullptr, 1, B, Hl, Wl, half, s, pooled)) != MRT_OK)
       return rc;
   }
   for (int l = kUnetLevels - 2; l >= 0; --l) {

@@ -335,6 +335,24 @@ __global__ __launch_bounds__(ROWS * 64) void conv_tile_kernel(ConvArgs a, int gr
         *reinterpret_cast<h8*>(out + ((size_t(img) * a.H + y) * a.W + px) * a.cout + m0 + mb * 32 + 8 * q) =
             s_buf[(row * kTileCols + pix) * kOutPitch + q];
     }
+    if (a.pool_out) {
+      // The 2x2 average of the staged block (tiles start on even rows and columns): one 16-byte
+      // record per thread, summed in pool_kernel's order.
+      const int prow = tid / (kTileCols * 2), rem = tid % (kTileCols * 2), ppx = rem >> 2, q = rem & 3;
+      const int yo = (y0 >> 1) + prow, xo = (x0 >> 1) + ppx;
+      if (yo < (a.H >> 1) && xo < (a.W >> 1)) {
+        const h8 v00 = s_buf[((2 * prow) * kTileCols + 2 * ppx) * kOutPitch + q];
+        const h8 v01 = s_buf[((2 * prow) * kTileCols + 2 * ppx + 1) * kOutPitch + q];
+        const h8 v10 = s_buf[((2 * prow + 1) * kTileCols + 2 * ppx) * kOutPitch + q];
+        const h8 v11 = s_buf[((2 * prow + 1) * kTileCols + 2 * ppx + 1) * kOutPitch + q];
+        h8 o;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          o[e] = (_Float16)((float(v00[e]) + float(v01[e]) + float(v10[e]) + float(v11[e])) * 0.25f);
+        *reinterpret_cast<h8*>(static_cast<_Float16*>(a.pool_out) +
+                               ((size_t(img) * (a.H >> 1) + yo) * (a.W >> 1) + xo) * a.cout + m0 + mb * 32 + 8 * q) = o;
+      }
+    }
     if (mb + 1 < NM) __syncthreads();  // the next block's staging overwrites this one
   }
 }
@@ -638,7 +656,11 @@ int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
     hipLaunchKernelGGL((unk::conv_kernel<T, 9>), dim3(grid), dim3(unk::kBlock), 0, s, a);
   else
     hipLaunchKernelGGL((unk::conv_kernel<T, 1>), dim3(grid), dim3(unk::kBlock), 0, s, a);
-  return unk::ok();
+  if (unk::ok() != 0) return -1;
+  // This kernel does not pool: the pooled copy comes from the separate pass.
+  if (a.pool_out)
+    return launch_unet_pool<T>(static_cast<const T*>(a.out), static_cast<T*>(a.pool_out), a.B, a.H, a.W, a.cout, s);
+  return 0;
 }
 template int launch_unet_conv<float>(const ConvArgs&, int, hipStream_t);
 template int launch_unet_conv<_Float16>(const ConvArgs&, int, hipStream_t);

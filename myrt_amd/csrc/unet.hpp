@@ -48,6 +48,7 @@ struct ConvArgs {
   const float* bias;    // [cout] or null
   void* out;            // [B][H][W][cout]
   const void* mask;     // if set, [B][H][W][cout]: the result is zeroed where mask <= 0
+  void* pool_out;       // if set, also the 2x2 average of the result, [B][H/2][W/2][cout]
   int c0, c1, up0;
   int cout;
   int relu;
