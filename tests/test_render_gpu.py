@@ -273,3 +273,28 @@ def test_two_sample_pipelines_give_the_same_image(ensure_built, monkeypatch):
     say("max difference %.3e, image mean %.3f" % (diff.max(), imgs[0].mean()))
     assert imgs[0].mean() > 0.01
     assert np.all(diff <= 1e-5 * np.maximum(1.0, np.abs(imgs[0])))
+
+
+def test_reference_small_config_matches_fp64_oracle(ensure_built):
+    """The reference's CPU-runnable configuration: ground quad + 200 icospheres (64K triangles,
+    radius U[0.1, 0.5], centres U[-5, 5]^3), GGX roughness U[0.05, 1], albedo U[0.1, 0.9], one
+    quad light, 512x512 pinhole camera, 1 spp, depth 4. The oracle draws the same sampler
+    sequence, so even one sample per pixel agrees within the 0.5 % bar."""
+    sc = scenes.sphere_field_scene(200, 2, seed=1)
+    n_tris = sum(sc["geometries"][g][0].size // 3 for g, _, _ in sc["instances"])
+    assert n_tris == 200 * 320 + 2
+    W, H, spp, depth = 512, 512, 1, 4
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
+    s = gpu_scene(sc)
+    r = m.Renderer()
+    r.set_camera(view, proj)
+    r.set_num_bounces(depth)
+    say("render %dx%d %d spp depth %d, 200 icospheres" % (W, H, spp, depth))
+    r.render(s, W, H, spp)
+    img = r.image()[..., :3].astype(np.float64)
+    ref = orc.OracleRender(sc["geometries"], sc["instances"], sc["materials"], sc["lights"], 1.0).render(
+        view, proj, W, H, spp, depth)
+    err = rel_mse(img, ref)
+    say("relative MSE %.3e" % err)
+    assert np.isfinite(img).all() and img.mean() > 0.01
+    assert err <= REL_MSE_TOL
