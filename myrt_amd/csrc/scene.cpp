@@ -298,7 +298,7 @@ Scene::~Scene() {
   if (d_blas) hipFree(d_blas);
   if (d_tris) hipFree(d_tris);
   for (TraceScratch& c : ctx)
-    for (void* q : {(void*)c.ctrl, (void*)c.spill, (void*)c.sdf_side})
+    for (void* q : {(void*)c.ctrl, (void*)c.spill, (void*)c.sdf_side, (void*)c.sdf_hits})
       if (q) hipFree(q);
   for (void* q : {(void*)d_group_keys, (void*)d_group_perm, (void*)d_group_counts})
     if (q) hipFree(q);
@@ -462,13 +462,18 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     sp.counter = d_ctrl + 2;  // zeroed with the traversal counters above
     if (!any && cx.sdf_side_cap < n) {
       if (cx.sdf_side) hipFree(cx.sdf_side);
+      if (cx.sdf_hits) hipFree(cx.sdf_hits);
       cx.sdf_side = nullptr;
+      cx.sdf_hits = nullptr;
       cx.sdf_side_cap = 0;
       hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&cx.sdf_side), n * sizeof(float4));
+      if (e2 == hipSuccess) e2 = hipMalloc(reinterpret_cast<void**>(&cx.sdf_hits), n * sizeof(uint32_t));
       if (e2 != hipSuccess) return hip_fail(e2, "hipMalloc sdf hit points");
       cx.sdf_side_cap = n;
     }
     sp.side = cx.sdf_side;
+    sp.hit_list = any ? nullptr : cx.sdf_hits;
+    sp.hit_count = d_ctrl + 3;  // zeroed with the traversal counters above
     sp.refill_below = 48;
     sp.prog_len = uint32_t(sdf_prog.size());
     sp.persistent = (sdf_persistent == 1 || (sdf_persistent == 2 && !any)) ? 1u : 0u;

@@ -110,6 +110,7 @@ struct Scene {
     uint32_t* spill = nullptr;  // traversal stack entries beyond the LDS part
     size_t spill_cap = 0;
     float4* sdf_side = nullptr;  // per-ray SDF hit points between the march and the normal pass
+    uint32_t* sdf_hits = nullptr;  // rays that hit an SDF, for the normal pass
     size_t sdf_side_cap = 0;
   };
   static constexpr int kTraceContexts = 4;

@@ -349,14 +349,22 @@ __device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, const 
   return tw;
 }
 
+// Appends `ray` to the list of SDF hits (called by the lanes that found one; one atomic per wave).
+__device__ inline void append_hit(const SdfParams& p, uint32_t ray) {
+  const uint64_t lanes = __ballot(1);
+  const int lane = __lane_id(), leader = __ffsll((unsigned long long)lanes) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(p.hit_count, uint32_t(__popcll(lanes)));
+  base = __shfl(base, leader);
+  p.hit_list[base + uint32_t(__popcll(lanes & ((1ull << lane) - 1ull)))] = ray;
+}
+
 // Second pass of a closest-hit query: every record the march marked as an SDF hit (instance bit
 // 31) gets its material and normal, the tetrahedral gradient of raymarch.glsl:100-114 (offset
 // 0.0005 / len) at the object-space hit point the march left in `side`, turned to world space.
 // Kept out of the marching kernel so that its registers do not lower that kernel's occupancy.
-__global__ __launch_bounds__(kBlock) void sdf_normal_kernel(SdfParams p) {
-  const uint32_t ray = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t n = p.n_dev ? min(*p.n_dev, p.n) : p.n;
-  if (ray >= n) return;
+// Material and normal of one ray whose record the march marked as an SDF hit.
+__device__ inline void sdf_normal(const SdfParams& p, uint32_t ray) {
   uint4 h = reinterpret_cast<const uint4*>(p.out)[ray];
   if (h.z == 0xFFFFFFFFu || !(h.z & 0x80000000u)) return;
   const int best_j = int(h.z & 0x7fffffffu);
@@ -392,6 +400,14 @@ __global__ __launch_bounds__(kBlock) void sdf_normal_kernel(SdfParams p) {
   h.w = oct_encode(wx, wy, wz);
   reinterpret_cast<uint4*>(p.out)[ray] = h;
 }
+
+// With a hit list the pass walks only the listed rays (grid-stride), else every ray.
+__global__ __launch_bounds__(kBlock) void sdf_normal_kernel(SdfParams p) {
+  const uint32_t count = p.hit_list ? min(*p.hit_count, p.n) : (p.n_dev ? min(*p.n_dev, p.n) : p.n);
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count; k += gridDim.x * kBlock)
+    sdf_normal(p, p.hit_list ? p.hit_list[k] : k);
+}
+
 
 // Persistent sphere tracing. Every lane runs a small state machine over its ray: pick the next
 // drawable whose box the segment [tmin, best] crosses, march it one step per iteration
@@ -522,6 +538,7 @@ __device__ __attribute__((always_inline)) inline void march_persistent(const Sdf
               // Distance and drawable now; material and normal in sdf_normal_kernel.
               reinterpret_cast<uint4*>(p.out)[ray] = make_uint4(__float_as_uint(best), 0u, 0x80000000u | uint32_t(best_j), 0u);
               p.side[ray] = make_float4(bx, by, bz, blen);
+              if (p.hit_list) append_hit(p, uint32_t(ray));
             }
           }
           ray = -1;
@@ -579,6 +596,7 @@ __device__ __attribute__((always_inline)) inline void march_per_ray(const SdfPar
   } else {
     reinterpret_cast<uint4*>(p.out)[i] = make_uint4(__float_as_uint(best), 0u, 0x80000000u | uint32_t(best_j), 0u);
     p.side[i] = make_float4(bx, by, bz, blen);
+    if (p.hit_list) append_hit(p, i);
   }
 }
 
@@ -667,7 +685,8 @@ int launch_sdf(const SdfParams& p, bool any, int num_cus, hipStream_t stream) {
     launch_march_any<true>(p, num_cus, stream);
   } else {
     launch_march_any<false>(p, num_cus, stream);
-    const unsigned nb = unsigned((p.n + sdfk::kBlock - 1) / sdfk::kBlock);
+    unsigned nb = unsigned((p.n + sdfk::kBlock - 1) / sdfk::kBlock);
+    if (p.hit_list) nb = std::min(nb, unsigned(std::max(num_cus, 1)) * 16u);  // grid-stride over the hits
     hipLaunchKernelGGL(sdfk::sdf_normal_kernel, dim3(nb ? nb : 1), dim3(sdfk::kBlock), 0, stream, p);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -26,6 +26,8 @@ struct SdfParams {
   uint32_t prog_len;       // instructions in prog (all drawables)
   uint32_t persistent;     // 1: persistent lanes with refill; 0: one thread per ray
   uint32_t chain_only;     // 1: every drawable's program has a culling grid (a leaner kernel serves)
+  uint32_t* hit_list;      // closest hits: rays that hit an SDF, appended by the march (n entries), or null
+  uint32_t* hit_count;     // entries in hit_list, zero before the launch
 };
 
 int launch_sdf(const SdfParams& p, bool any, int num_cus, hipStream_t stream);
