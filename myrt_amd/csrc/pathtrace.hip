@@ -75,11 +75,41 @@ __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
   return base + uint32_t(__popcll(mask & ((1ull << lane_id()) - 1ull)));
 }
 
-__device__ inline void add_pixel(float4* accum, uint32_t pixel, V3 c) {
-  float* a = reinterpret_cast<float*>(accum + pixel);
-  unsafeAtomicAdd(a + 0, c.x);
-  unsafeAtomicAdd(a + 1, c.y);
-  unsafeAtomicAdd(a + 2, c.z);
+// Adds v[0..N) to dst[0..N) for the lanes with `on` set, one atomic per float and per run of
+// neighbouring lanes with the same key: each run is summed in registers first (a suffix scan that
+// stops at the run's last lane), and the run's first lane adds the total. Paths are laid out
+// pixel-major, so the samples of one pixel sit in neighbouring lanes. Lanes without a contribution
+// carry the key 0xFFFFFFFF and add nothing. Every lane of the wave must call it.
+template <int N>
+__device__ inline void wave_run_add(float* dst, uint32_t key, float (&v)[N], bool on) {
+  const int lane = int(lane_id());
+  if (!on) key = 0xFFFFFFFFu;
+  const uint32_t next_key = __shfl_down(key, 1u);
+  const uint32_t prev_key = __shfl_up(key, 1u);
+  const uint64_t ends = __ballot(lane == 63 || next_key != key);  // last lane of each run
+  const int run_end = lane + __builtin_ctzll(ends >> lane);
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+    if (!on) v[k] = 0.f;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const bool take = lane + int(off) <= run_end;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const float o = __shfl_down(v[k], off);
+      if (take) v[k] += o;
+    }
+  }
+  if (on && (lane == 0 || prev_key != key)) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) unsafeAtomicAdd(dst + k, v[k]);
+  }
+}
+
+// add_pixel for a whole wave (uniform control flow): lanes of one pixel add once.
+__device__ inline void wave_add_pixel(float4* accum, uint32_t pixel, V3 c, bool on) {
+  float v[3] = {c.x, c.y, c.z};
+  wave_run_add<3>(reinterpret_cast<float*>(accum + pixel), pixel, v, on);
 }
 
 // Adds a path contribution to its training header (one writer per path and pass).
@@ -234,9 +264,12 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers 
   if (blockIdx.x * kBlock >= n_paths) return;  // whole block idle
   const bool live = i < n_paths;
   const uint32_t ii = live ? i : n_paths - 1;  // idle lanes mirror a live one but write nothing
+  // Pixel-major: the batch's samples of one pixel are neighbouring paths (n_paths is a whole number
+  // of samples per pixel), so a wave's primary rays leave a few pixels and its pixel adds combine.
   const uint32_t npix = cam.width * cam.height;
-  const uint32_t pixel = ii % npix;
-  const uint32_t sample = sample_base + ii / npix;
+  const uint32_t nspp = max(1u, n_paths / npix);
+  const uint32_t pixel = ii / nspp;
+  const uint32_t sample = sample_base + ii % nspp;
   const uint32_t x = pixel % cam.width, y = pixel / cam.width;
   const float nx = (float(x) + rnd(pixel, sample, 0)) / float(cam.width) * 2.0f - 1.0f;
   const float ny = (float(y) + rnd(pixel, sample, 1)) / float(cam.height) * 2.0f - 1.0f;
@@ -317,6 +350,9 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
   V3 cache_thr = mk(0, 0, 0);
   V3 new_o = mk(0, 0, 0), new_d = mk(0, 0, 1);
   V3 sh_o = mk(0, 0, 0), sh_d = mk(0, 0, 1), sh_c = mk(0, 0, 0);
+  V3 pix_c = mk(0, 0, 0);  // emitter or environment seen by the ray
+  bool pix_on = false, aux_on = false;
+  float aux_v[8] = {};  // primary-hit features (albedo, depth, shading normal, count)
   PathState ps{};
   if (live) {
     const uint32_t src = b.perm ? b.perm[i] : i;  // shading order by material, when sorted
@@ -361,11 +397,13 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
         w = ps.last_pdf * ps.last_pdf / (ps.last_pdf * ps.last_pdf + pdf_l * pdf_l);
       }
       const V3 c = thr * ld3(q.emission) * w;
-      add_pixel(b.accum, ps.pixel, c);
+      pix_c = c;
+      pix_on = true;
       add_train(b, ps.pad, c);
     } else if (!found) {
       const V3 c = thr * environment(sc, rd);
-      add_pixel(b.accum, ps.pixel, c);
+      pix_c = c;
+      pix_on = true;
       add_train(b, ps.pad, c);
     } else {
       // ---- surface interaction ----
@@ -442,15 +480,15 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
       L.wd = (1.0f - Fv) * (1.0f - mat.transmission);
       // Primary-hit features for the denoiser (sums; the host divides by the sample count).
       if (bounce == 0 && b.aux) {
-        float* a = reinterpret_cast<float*>(b.aux + 2 * size_t(ps.pixel));
-        unsafeAtomicAdd(a + 0, L.albedo.x);
-        unsafeAtomicAdd(a + 1, L.albedo.y);
-        unsafeAtomicAdd(a + 2, L.albedo.z);
-        unsafeAtomicAdd(a + 3, t_hit);
-        unsafeAtomicAdd(a + 4, ns.x);
-        unsafeAtomicAdd(a + 5, ns.y);
-        unsafeAtomicAdd(a + 6, ns.z);
-        unsafeAtomicAdd(a + 7, 1.0f);
+        aux_on = true;
+        aux_v[0] = L.albedo.x;
+        aux_v[1] = L.albedo.y;
+        aux_v[2] = L.albedo.z;
+        aux_v[3] = t_hit;
+        aux_v[4] = ns.x;
+        aux_v[5] = ns.y;
+        aux_v[6] = ns.z;
+        aux_v[7] = 1.0f;
       }
 
       // ---- radiance cache: end here (non-training paths) or record the vertex (training paths) ----
@@ -595,6 +633,8 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
     done_shading:;
     }
   }
+  wave_add_pixel(b.accum, ps.pixel, pix_c, pix_on);
+  if (b.aux) wave_run_add<8>(reinterpret_cast<float*>(b.aux + 2 * size_t(ps.pixel)), ps.pixel, aux_v, aux_on);
   const uint32_t slot = wave_append(&b.counts[nxt], survive);
   if (survive) {
     float4* r = reinterpret_cast<float4*>(b.rays[nxt]) + 2 * size_t(slot);
@@ -631,22 +671,30 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
 __global__ __launch_bounds__(kBlock) void resolve_kernel(PtBuffers b) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i == 0) atomicAdd(reinterpret_cast<unsigned long long*>(b.counts + 8), (unsigned long long)b.counts[2]);
-  if (i >= b.counts[2]) return;
-  if (b.occluded[i]) return;
-  const ShadowRecord r = b.shadow[i];
+  const uint32_t n = b.counts[2];
+  if (blockIdx.x * kBlock >= n) return;  // whole block idle (uniform)
+  const bool on = i < n && !b.occluded[i];
+  ShadowRecord r{};
+  if (on) r = b.shadow[i];
   const V3 c = mk(r.contrib[0], r.contrib[1], r.contrib[2]);
-  add_pixel(b.accum, r.pixel, c);
-  add_train(b, r.train_slot, c);
+  wave_add_pixel(b.accum, r.pixel, c, on);
+  if (on) add_train(b, r.train_slot, c);
 }
 
 // Paths that ended in the cache: add throughput times the predicted radiance.
 __global__ __launch_bounds__(kBlock) void cache_resolve_kernel(PtBuffers b) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t n = min(b.counts[4], b.cache.max_queries);
-  if (i >= n) return;
-  const CacheRecord r = b.cache.qrec[i];
-  const float* y = b.cache.qout + 3 * size_t(i);
-  add_pixel(b.accum, r.pixel, mk(r.thr[0] * fmaxf(y[0], 0.f), r.thr[1] * fmaxf(y[1], 0.f), r.thr[2] * fmaxf(y[2], 0.f)));
+  if (blockIdx.x * kBlock >= n) return;  // whole block idle (uniform)
+  const bool on = i < n;
+  CacheRecord r{};
+  V3 c = mk(0, 0, 0);
+  if (on) {
+    r = b.cache.qrec[i];
+    const float* y = b.cache.qout + 3 * size_t(i);
+    c = mk(r.thr[0] * fmaxf(y[0], 0.f), r.thr[1] * fmaxf(y[1], 0.f), r.thr[2] * fmaxf(y[2], 0.f));
+  }
+  wave_add_pixel(b.accum, r.pixel, c, on);
 }
 
 // Training samples: suffix radiance (T - prefix) / thr of every recorded vertex.
