@@ -400,7 +400,10 @@ struct Renderer {
       hipError_t e = hipMemsetAsync(counts + (cur ^ 1u), 0, 4, st);
       if (e != hipSuccess) return hip_fail(e, "clearing queue counters");
       if ((rc = s.trace(p.rays[cur], n_paths, p.hits, false, false, st, counts + cur, ctx_closest)) != MRT_OK) return rc;
-      if (material_sort) {
+      // Primary hits arrive grouped by screen position (paths are pixel-major), hence by surface:
+      // sorting pays from the first scattered bounce on.
+      b.perm = nullptr;
+      if (material_sort && bounce > 0) {
         MatSortArgs ma{p.hits, counts + cur, n_paths, sc.inst_material, p.ms_keys, p.ms_counts, p.ms_perm};
         if (launch_material_sort(ma, st)) return fail(MRT_ERR_HIP, "material sort");
         b.perm = p.ms_perm;
