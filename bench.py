@@ -99,14 +99,14 @@ def wavefront_extras(args):
     st1 = r.stats()
     dt = (st1["milliseconds"] - st0["milliseconds"]) * 1e-3
     n_rays = st1["rays"] - st0["rays"]
-    # The same render with shading in queue order instead of material order (the default).
-    r.set_material_sort(False)
+    # The same render with shading in material order instead of queue order (the default).
+    r.set_material_sort(True)
     r.invalidate()
     r.render(s, W, H, 1)
     st0 = r.stats()
-    log("wavefront render, shading in queue order")
+    log("wavefront render, shading in material order")
     r.render(s, W, H, spp)
-    dt_unsorted = (r.stats()["milliseconds"] - st0["milliseconds"]) * 1e-3
+    dt_sorted = (r.stats()["milliseconds"] - st0["milliseconds"]) * 1e-3
     return {"triangles": n_tris, "sdf_primitives": 64, "prepare_s": round(prep_s, 3),
             "wide_nodes": prep["blas_wide_nodes"] + prep["tlas_wide_nodes"], "device_bytes": prep["device_bytes"],
             "incoherent_closest_mrays_triangles_only": round(rays.size / (ms_tri * 1e-3) / 1e6, 2),
@@ -116,7 +116,7 @@ def wavefront_extras(args):
             "incoherent_instance_visits_per_ray": round(st["instance_visits"] / rays.size, 3),
             "render": {"width": W, "height": H, "spp": spp, "depth": depth, "seconds": round(dt, 4),
                        "samples_per_s": round(W * H * spp / dt, 1), "rays_traced": int(n_rays),
-                       "mrays_per_s": round(n_rays / dt / 1e6, 2), "seconds_unsorted_shading": round(dt_unsorted, 4)}}
+                       "mrays_per_s": round(n_rays / dt / 1e6, 2), "seconds_sorted_shading": round(dt_sorted, 4)}}
 
 
 def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 64), warm_frames=16):
@@ -193,14 +193,22 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
             img, t_dn = timed(lambda: plain.denoise(den))
             row["plain_denoised_mse"] = mse(img)
             row["denoise_s"] = round(t_dn, 4)
-        # The cache path gets as many samples as fit in the budget at its measured rate; the
-        # estimate is refined until its render plus the denoiser stays within the budget.
+        # The cache path gets the most samples whose render plus the denoiser fits the budget. The
+        # first guess scales its 16-spp time; each render then rescales by its own time, kept
+        # between the largest count that fitted and the smallest that did not (at most 6 renders).
         spp_c = max(1, int((T - t_dn) / t16 * 16))
-        for _ in range(4):
-            _, tc = timed(lambda: fresh(cr).render(s, W, H, spp_c))
-            if tc + t_dn <= T * 1.01 or spp_c == 1:
-                break
-            spp_c = max(1, min(spp_c - 1, int(spp_c * (T - t_dn) / tc)))
+        times, last = {}, None
+        while spp_c not in times and len(times) < 6:
+            _, times[spp_c] = timed(lambda: fresh(cr).render(s, W, H, spp_c))
+            last = spp_c
+            lo = max([k for k, t in times.items() if t + t_dn <= T * 1.01], default=0)
+            hi = min([k for k, t in times.items() if t + t_dn > T * 1.01], default=1 << 30)
+            guess = int(spp_c * (T - t_dn) / times[spp_c])
+            spp_c = min(max(guess, lo + 1), hi - 1) if hi - lo > 1 else max(lo, 1)
+        spp_c = max([k for k, t in times.items() if t + t_dn <= T * 1.01], default=1)
+        if spp_c != last:
+            _, times[spp_c] = timed(lambda: fresh(cr).render(s, W, H, spp_c))
+        tc = times[spp_c]
         row.update({"cache_spp": spp_c, "cache_s": round(tc, 4), "cache_mse": mse(cr.image())})
         if den is not None:
             img, t = timed(lambda: cr.denoise(den))

@@ -317,7 +317,7 @@ int mrt_renderer_set_camera(mrt_renderer r, const float view_colmajor[16], const
 int mrt_renderer_set_num_bounces(mrt_renderer r, int n);
 /* Shade each bounce's paths in material order (a counting sort of the live queue by the hit's
  * material, pt_ray_filter.comp:49-63 leaves them in atomic order). Results do not change beyond
- * fp32 summation order; the default is on (MRT_MATERIAL_SORT=0 at creation switches it off). */
+ * fp32 summation order; the default is off (MRT_MATERIAL_SORT=1 at creation switches it on). */
 int mrt_renderer_set_material_sort(mrt_renderer r, int enabled);
 /* set_dof_enabled / set_lens_size / set_focus / set_bokeh_mask (sequential_pathtracer.hpp:30-33;
  * defaults off, 10 x 10, 10). A thin lens around each pinhole ray: lens offset o uniform in the

@@ -11,8 +11,8 @@
 //     dielectric transmission lobe;
 //   * quad area lights with next-event estimation and power-heuristic MIS replace the two
 //     hard-coded sphere lights;
-//   * survivors and shadow rays are appended to dense queues with one ballot and one atomic
-//     per wave instead of one atomic per ray.
+//   * survivors and shadow rays are appended to dense queues with ballots and one atomic per
+//     block and queue instead of one atomic per ray.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -23,6 +23,7 @@ namespace mrt {
 namespace ptk {
 
 constexpr int kBlock = 256;
+constexpr int kShadeBlock = 512;  // shade_kernel: 8 waves share one queue atomic
 constexpr uint32_t kDelta = 1u << 8;
 constexpr float kPi = 3.14159265358979f;
 
@@ -64,15 +65,38 @@ constexpr uint32_t kLensDim = 0x30000000u;   // lens samples: kLensDim + 2 * try
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
-// Appends the lanes with pred set to a queue: one ballot and one atomic per wave.
-__device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
-  const uint64_t mask = __ballot(pred);
-  if (mask == 0) return 0;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (int(lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-  base = __shfl(base, leader);
-  return base + uint32_t(__popcll(mask & ((1ull << lane_id()) - 1ull)));
+// Appends the threads with pred[q] set to queue q (Q queues at once) with one atomic per block and
+// queue. A queue counter is a single address and atomics on one address are serialised (about
+// 11 ns each here), so one atomic per wave and queue set the pace of a pass over tens of millions
+// of paths. Slots follow thread order within the block. Every thread of the block must call it.
+template <int BLOCK, int Q>
+__device__ inline void block_append(uint32_t* const (&counter)[Q], const bool (&pred)[Q], uint32_t (&slot)[Q]) {
+  constexpr int kWaves = BLOCK / 64;
+  __shared__ uint32_t cnt[Q][kWaves];
+  __shared__ uint32_t base[Q];
+  const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+  uint64_t mask[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    mask[q] = __ballot(pred[q]);
+    if (lane == 0) cnt[q][wave] = uint32_t(__popcll(mask[q]));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+    if (threadIdx.x == uint32_t(q)) {
+      uint32_t tot = 0;
+      for (int w = 0; w < kWaves; ++w) tot += cnt[q][w];
+      base[q] = tot ? atomicAdd(counter[q], tot) : 0u;
+    }
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    uint32_t s = base[q];
+    for (uint32_t w = 0; w < wave; ++w) s += cnt[q][w];
+    slot[q] = s + uint32_t(__popcll(mask[q] & below));
+  }
 }
 
 // Adds v[0..N) to dst[0..N) for the lanes with `on` set, one atomic per float and per run of
@@ -326,7 +350,11 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers 
   ps.last_pdf = 0.f;
   ps.pad = 0xFFFFFFFFu;
   const bool want_train = live && b.cache.train && rnd(pixel, sample, kTrainDim) < b.cache.train_fraction;
-  const uint32_t slot = wave_append(&b.counts[3], want_train);
+  uint32_t* const queues[1] = {&b.counts[3]};
+  const bool append[1] = {want_train};
+  uint32_t slots[1];
+  block_append<kBlock, 1>(queues, append, slots);
+  const uint32_t slot = slots[0];
   if (want_train && slot < b.cache.max_train_paths) {
     ps.pad = slot;
     TrainPath tp;
@@ -337,11 +365,11 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers 
   if (live) b.paths[cur][i] = ps;
 }
 
-__global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, uint32_t cur, uint32_t max_depth) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffers b, uint32_t cur, uint32_t max_depth) {
+  const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
   const uint32_t n = b.counts[cur];
   if (i == 0) atomicAdd(reinterpret_cast<unsigned long long*>(b.counts + 8), (unsigned long long)n);  // rays traced
-  if (blockIdx.x * kBlock >= n) return;  // whole block idle (uniform)
+  if (blockIdx.x * kShadeBlock >= n) return;  // whole block idle (uniform)
   const uint32_t nxt = cur ^ 1u;
   const bool live = i < n;
 
@@ -635,14 +663,17 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
   }
   wave_add_pixel(b.accum, ps.pixel, pix_c, pix_on);
   if (b.aux) wave_run_add<8>(reinterpret_cast<float*>(b.aux + 2 * size_t(ps.pixel)), ps.pixel, aux_v, aux_on);
-  const uint32_t slot = wave_append(&b.counts[nxt], survive);
+  uint32_t* const queues[3] = {&b.counts[nxt], &b.counts[2], &b.counts[4]};
+  const bool append[3] = {survive, shadow, want_query};
+  uint32_t slots[3];
+  block_append<kShadeBlock, 3>(queues, append, slots);
+  const uint32_t slot = slots[0], sslot = slots[1], qslot = slots[2];
   if (survive) {
     float4* r = reinterpret_cast<float4*>(b.rays[nxt]) + 2 * size_t(slot);
     r[0] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
     r[1] = make_float4(new_d.x, new_d.y, new_d.z, __int_as_float(0x7f800000));
     b.paths[nxt][slot] = ps;
   }
-  const uint32_t sslot = wave_append(&b.counts[2], shadow);
   if (shadow) {
     float4* r = reinterpret_cast<float4*>(b.shadow_rays) + 2 * size_t(sslot);
     r[0] = make_float4(sh_o.x, sh_o.y, sh_o.z, 0.0f);
@@ -656,7 +687,6 @@ __global__ __launch_bounds__(kBlock) void shade_kernel(PtScene sc, PtBuffers b, 
     rec.pad[0] = rec.pad[1] = rec.pad[2] = 0;
     b.shadow[sslot] = rec;
   }
-  const uint32_t qslot = wave_append(&b.counts[4], want_query);
   if (want_query && qslot < b.cache.max_queries) {
     b.cache.queries[qslot] = cache_q;
     CacheRecord cr;
@@ -720,7 +750,11 @@ __global__ __launch_bounds__(kBlock) void train_batch_kernel(PtBuffers b) {
     }
   }
   if (blockIdx.x * kBlock >= b.cache.max_train_paths * b.cache.max_vertices) return;
-  const uint32_t slot = wave_append(&b.counts[5], ok);
+  uint32_t* const queues[1] = {&b.counts[5]};
+  const bool append[1] = {ok};
+  uint32_t slots[1];
+  block_append<kBlock, 1>(queues, append, slots);
+  const uint32_t slot = slots[0];
   if (ok && slot < b.cache.max_batch) {
     b.cache.batch_q[slot] = q;
     float* o = b.cache.batch_t + 3 * size_t(slot);
@@ -771,8 +805,8 @@ int launch_pt_generate(const Camera& cam, const PtBuffers& b, uint32_t cur, uint
 }
 
 int launch_pt_shade(const PtScene& sc, const PtBuffers& b, uint32_t cur, uint32_t max_depth, hipStream_t stream) {
-  const uint32_t grid = (b.capacity + ptk::kBlock - 1) / ptk::kBlock;
-  hipLaunchKernelGGL(ptk::shade_kernel, dim3(grid), dim3(ptk::kBlock), 0, stream, sc, b, cur, max_depth);
+  const uint32_t grid = (b.capacity + ptk::kShadeBlock - 1) / ptk::kShadeBlock;
+  hipLaunchKernelGGL(ptk::shade_kernel, dim3(grid), dim3(ptk::kShadeBlock), 0, stream, sc, b, cur, max_depth);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -178,10 +178,10 @@ struct Renderer {
   float* batch_t = nullptr;
   size_t c_tp = 0, c_tv = 0, c_q = 0, c_qr = 0, c_qo = 0, c_bq = 0, c_bt = 0;
   float last_loss = 0.f;
-  // Shading in material order (on by default: with batches of millions of paths shading is about
-  // a sixth of the frame and sorted lanes diverge less; MRT_MATERIAL_SORT=0 or
-  // mrt_renderer_set_material_sort switches it off).
-  bool material_sort = true;
+  // Shading in material order (off by default: with block-wide queue appends and pixel-major
+  // paths, shading is about 4 % of the frame and the sort no longer pays for itself;
+  // MRT_MATERIAL_SORT=1 or mrt_renderer_set_material_sort switches it on).
+  bool material_sort = false;
 
   // Shadow rays of each bounce are traced and resolved on a second stream, concurrently with the
   // next bounce's closest-hit passes (MRT_SHADOW_STREAM=0: everything on one stream). Events keep

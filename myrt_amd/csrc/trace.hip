@@ -23,6 +23,10 @@ namespace mrt {
 namespace dev {
 
 constexpr int kBlock = 256;
+// Rays a wave reserves per atomic on the global ray counter. Atomics on one address are
+// serialised (about 11 ns each), and one per refill (about 30 rays) kept that counter near its
+// limit; later refills take from the wave's reserve first.
+constexpr uint32_t kGrab = 64;
 constexpr float kRobust = 1.000001f;  // 1 + 16 ulp: covers the relative rounding of the slab test
 constexpr float kSlack = 1.0f / 1048576.0f;  // 2^-20 of |b|: covers the absolute rounding of b
 constexpr float kSlackFloor = 7.8886091e-31f;  // 2^-100: covers a plane term that underflows
@@ -118,6 +122,7 @@ trace_kernel(TraceParams p) {
 
   int ray_id = -1;
   bool queue_empty = false;
+  uint32_t res_next = 0, res_end = 0;  // rays reserved by this wave (wave-uniform): [res_next, res_end)
   // Ray groups: XCD x of workgroup b (round-robin dispatch: x = b % 8) starts on group x.
   uint32_t group = blockIdx.x & 7u, groups_left = 8;
   const uint32_t n_rays = p.n_dev ? *p.n_dev : p.n;
@@ -190,9 +195,30 @@ trace_kernel(TraceParams p) {
         first = p.group_start[group];
         limit = p.group_start[group + 1] - first;
       } else {
-        if (lane == leader) base = atomicAdd(p.counter, cnt);
+        // Lanes of rank < have take the reserve; the rest take a fresh reservation of at least
+        // kGrab rays, whose remainder becomes the new reserve.
+        const uint32_t have = res_end - res_next;
+        const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
+        uint32_t fresh = 0;
+        const uint32_t need = cnt > have ? cnt - have : 0u;
+        // Within one resident grid of the end (as far as this wave's last reservation tells), take
+        // only what is needed, so that no wave sits on reserved rays while others run dry.
+        const bool near_end = res_end + p.total_threads >= n_rays;
+        const uint32_t grab = (need > kGrab || near_end) ? need : kGrab;
+        if (need) {
+          if (lane == leader) fresh = atomicAdd(p.counter, grab);
+          fresh = __shfl(fresh, leader);
+        }
+        // base + rank is this lane's ray in both cases.
+        base = rank < have ? res_next : fresh - have;
+        if (need) {
+          res_next = fresh + need;
+          res_end = fresh + grab;
+        } else {
+          res_next += cnt;
+        }
       }
-      base = __shfl(base, leader);
+      if (p.group_start) base = __shfl(base, leader);
       if (p.group_start && base + cnt >= limit) {
         group = (group + 1) & 7u;
         --groups_left;
