@@ -349,6 +349,9 @@ __device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, const 
   return tw;
 }
 
+// Rays a persistent wave reserves per atomic on the ray counter.
+constexpr uint32_t kSdfGrab = 64;
+
 // Appends `ray` to the list of SDF hits (called by the lanes that found one; one atomic per wave).
 __device__ inline void append_hit(const SdfParams& p, uint32_t ray) {
   const uint64_t lanes = __ballot(1);
@@ -424,6 +427,7 @@ __device__ __attribute__((always_inline)) inline void march_persistent(const Sdf
   const uint32_t n = p.n_dev ? min(*p.n_dev, p.n) : p.n;
   int ray = -1;
   bool queue_empty = false;
+  uint32_t res_next = 0, res_end = 0;  // rays reserved by this wave (wave-uniform): [res_next, res_end)
   // Ray and search state.
   float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, tmin = 0, best = 0;
   int best_j = -1;
@@ -440,11 +444,29 @@ __device__ __attribute__((always_inline)) inline void march_persistent(const Sdf
     const uint64_t want_mask = __ballot(want);
     if (want_mask) {
       const int leader = __ffsll((unsigned long long)want_mask) - 1;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(p.counter, uint32_t(__popcll(want_mask)));
-      base = __shfl(base, leader);
+      // Lanes of rank < have take the wave's reserve, the rest a fresh reservation of at least
+      // kSdfGrab rays (one atomic on the ray counter; atomics on one address are serialised).
+      // Within one resident grid of the end a wave reserves only what it needs.
+      const uint32_t cnt = uint32_t(__popcll(want_mask));
+      const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
+      const uint32_t have = res_end - res_next;
+      const uint32_t need = cnt > have ? cnt - have : 0u;
+      const bool near_end = res_end + gridDim.x * blockDim.x >= n;
+      const uint32_t grab = (need > kSdfGrab || near_end) ? need : kSdfGrab;
+      uint32_t fresh = 0;
+      if (need) {
+        if (lane == leader) fresh = atomicAdd(p.counter, grab);
+        fresh = __shfl(fresh, leader);
+      }
+      const uint32_t base = rank < have ? res_next : fresh - have;
+      if (need) {
+        res_next = fresh + need;
+        res_end = fresh + grab;
+      } else {
+        res_next += cnt;
+      }
       if (want) {
-        const uint32_t r = base + uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
+        const uint32_t r = base + rank;
         if (r < n) {
           const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(r);
           const float4 r0 = rp[0], r1 = rp[1];
