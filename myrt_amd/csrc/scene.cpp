@@ -379,6 +379,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.spill_depth = spill_depth;
   p.total_threads = total_threads;
   p.refill_below = refill_below;
+  p.grab = trace_grab;
   p.lds_depth = lds_depth;
   p.perm = nullptr;
   p.waves = waves;
@@ -714,6 +715,10 @@ int mrt_scene_create(mrt_scene* out) {
   if (const char* v = std::getenv("MRT_TRACE_REFILL")) {
     const long r = std::strtol(v, nullptr, 10);
     if (r >= 1 && r <= 64) s->refill_below = uint32_t(r);
+  }
+  if (const char* v = std::getenv("MRT_TRACE_GRAB")) {
+    const long g = std::strtol(v, nullptr, 10);
+    if (g >= 1 && g <= 4096) s->trace_grab = uint32_t(g);
   }
   if (const char* v = std::getenv("MRT_BVH_BUILDER")) s->sah_builder = std::strtol(v, nullptr, 10) != 0;
   if (const char* v = std::getenv("MRT_SDF_PERSISTENT")) {

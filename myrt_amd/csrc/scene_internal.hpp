@@ -123,6 +123,7 @@ struct Scene {
   int sdf_persistent = 0;  // MRT_SDF_PERSISTENT: 0 one thread per ray, 1 persistent lanes, 2 persistent for closest hits
   int num_cus = 0;
   uint32_t refill_below = 48;
+  uint32_t trace_grab = 128;  // MRT_TRACE_GRAB: rays a traversal wave reserves per counter atomic
   bool sah_builder = true;  // traversal trees from build_sah_bvh (MRT_BVH_BUILDER=0: the reference builder's)
   uint32_t trace_waves = 0;  // 0: by scene (7 with several instances, else 6); MRT_TRACE_WAVES=6|7
   // Largest wide leaf: binary leaves are split at centroid medians down to this many triangles

@@ -23,10 +23,6 @@ namespace mrt {
 namespace dev {
 
 constexpr int kBlock = 256;
-// Rays a wave reserves per atomic on the global ray counter. Atomics on one address are
-// serialised (about 11 ns each), and one per refill (about 30 rays) kept that counter near its
-// limit; later refills take from the wave's reserve first.
-constexpr uint32_t kGrab = 64;
 constexpr float kRobust = 1.000001f;  // 1 + 16 ulp: covers the relative rounding of the slab test
 constexpr float kSlack = 1.0f / 1048576.0f;  // 2^-20 of |b|: covers the absolute rounding of b
 constexpr float kSlackFloor = 7.8886091e-31f;  // 2^-100: covers a plane term that underflows
@@ -196,7 +192,9 @@ trace_kernel(TraceParams p) {
         limit = p.group_start[group + 1] - first;
       } else {
         // Lanes of rank < have take the reserve; the rest take a fresh reservation of at least
-        // kGrab rays, whose remainder becomes the new reserve.
+        // p.grab rays, whose remainder becomes the new reserve. Atomics on one address are
+        // serialised (about 11 ns each), and one per refill (about 30 rays) kept the ray counter
+        // near that limit.
         const uint32_t have = res_end - res_next;
         const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
         uint32_t fresh = 0;
@@ -204,7 +202,7 @@ trace_kernel(TraceParams p) {
         // Within one resident grid of the end (as far as this wave's last reservation tells), take
         // only what is needed, so that no wave sits on reserved rays while others run dry.
         const bool near_end = res_end + p.total_threads >= n_rays;
-        const uint32_t grab = (need > kGrab || near_end) ? need : kGrab;
+        const uint32_t grab = (need > p.grab || near_end) ? need : p.grab;
         if (need) {
           if (lane == leader) fresh = atomicAdd(p.counter, grab);
           fresh = __shfl(fresh, leader);

@@ -25,6 +25,7 @@ struct TraceParams {
   uint32_t spill_depth;
   uint32_t total_threads;
   uint32_t refill_below;           // refill a wave when fewer lanes than this are busy
+  uint32_t grab;                   // rays a wave reserves per atomic on the ray counter (>= 1)
   uint32_t lds_depth;              // LDS stack entries per lane (a power of two)
   const uint32_t* perm;            // if set, queue position -> ray index (binned order)
   uint32_t waves;                  // waves per SIMD the kernel is compiled for: 6 or 7
