@@ -128,6 +128,11 @@ typedef struct mrt_render_stats {
   uint64_t samples;      /* pixel samples rendered since creation */
   double milliseconds;   /* wall time inside mrt_render */
   uint64_t rays;         /* closest-hit plus shadow rays traced since creation */
+  uint64_t nonfinite;    /* non-finite values rejected since creation (pt_color.comp:84-98, 156-166;
+                            pt_trace.comp:186-189): a non-finite geometric normal makes the hit a
+                            miss, a non-finite shading normal falls back to the geometric one,
+                            non-finite contributions are dropped and a non-finite throughput ends
+                            the path */
 } mrt_render_stats;
 
 /* Radiance-cache query, 64 B: shading point, outgoing direction (towards the viewer), shading

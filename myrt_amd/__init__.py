@@ -57,7 +57,8 @@ class _PrepareResult(ctypes.Structure):
 
 
 class _RenderStats(ctypes.Structure):
-    _fields_ = [("samples", ctypes.c_uint64), ("milliseconds", ctypes.c_double), ("rays", ctypes.c_uint64)]
+    _fields_ = [("samples", ctypes.c_uint64), ("milliseconds", ctypes.c_double), ("rays", ctypes.c_uint64),
+                ("nonfinite", ctypes.c_uint64)]
 
 
 class _TraceStats(ctypes.Structure):
@@ -635,7 +636,7 @@ class Renderer:
     def stats(self):
         s = _RenderStats()
         _check(lib().mrt_renderer_stats(self.h, ctypes.byref(s)))
-        return {"samples": s.samples, "milliseconds": s.milliseconds, "rays": s.rays}
+        return {"samples": s.samples, "milliseconds": s.milliseconds, "rays": s.rays, "nonfinite": s.nonfinite}
 
     def aux(self):
         """Per-pixel means of the primary hits: (albedo (H, W, 3), normal (H, W, 3), distance (H, W))."""

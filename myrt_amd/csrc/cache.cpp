@@ -10,6 +10,7 @@
 #include "cache_internal.hpp"
 #include "mlp.hpp"
 #include "scene_internal.hpp"
+#include "abi_guard.hpp"
 
 using namespace mrt;
 
@@ -34,6 +35,7 @@ struct DevCopy {
 extern "C" {
 
 int mrt_cache_create(mrt_cache* out, uint64_t seed) {
+  MRT_GUARD_BEGIN
   if (!out) return fail(MRT_ERR_INVALID, "null output");
   auto* c = new Cache;
   const int rc = c->init(seed);
@@ -43,14 +45,18 @@ int mrt_cache_create(mrt_cache* out, uint64_t seed) {
   }
   *out = reinterpret_cast<mrt_cache>(c);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_cache_destroy(mrt_cache c) {
+  MRT_GUARD_BEGIN
   delete reinterpret_cast<Cache*>(c);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_cache_set_bounds(mrt_cache ch, const float lo[3], const float hi[3]) {
+  MRT_GUARD_BEGIN
   Cache* c = reinterpret_cast<Cache*>(ch);
   if (!c || !lo || !hi) return fail(MRT_ERR_INVALID, "null argument");
   for (int a = 0; a < 3; ++a) {
@@ -59,9 +65,11 @@ int mrt_cache_set_bounds(mrt_cache ch, const float lo[3], const float hi[3]) {
     c->P.inv_ext[a] = 1.0f / (hi[a] - lo[a]);
   }
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_cache_set_adam(mrt_cache ch, float lr, float beta1, float beta2, float eps) {
+  MRT_GUARD_BEGIN
   Cache* c = reinterpret_cast<Cache*>(ch);
   if (!c || !(lr > 0.f) || !(beta1 >= 0.f && beta1 < 1.f) || !(beta2 >= 0.f && beta2 < 1.f) || !(eps > 0.f))
     return fail(MRT_ERR_INVALID, "bad Adam parameters");
@@ -70,21 +78,27 @@ int mrt_cache_set_adam(mrt_cache ch, float lr, float beta1, float beta2, float e
   c->beta2 = beta2;
   c->eps = eps;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_cache_get_params(mrt_cache ch, float* out, size_t n) {
+  MRT_GUARD_BEGIN
   Cache* c = reinterpret_cast<Cache*>(ch);
   if (!c || !out || n != size_t(kCacheParams)) return fail(MRT_ERR_INVALID, "need 20672 floats");
   return c->get(c->P.master, out);
+  MRT_GUARD_END
 }
 
 int mrt_cache_set_params(mrt_cache ch, const float* in, size_t n) {
+  MRT_GUARD_BEGIN
   Cache* c = reinterpret_cast<Cache*>(ch);
   if (!c || !in || n != size_t(kCacheParams)) return fail(MRT_ERR_INVALID, "need 20672 floats");
   return c->set_params(in);
+  MRT_GUARD_END
 }
 
 int mrt_cache_infer_device(mrt_cache ch, const mrt_cache_query* d_q, size_t n, float* d_out, void* stream) {
+  MRT_GUARD_BEGIN
   Cache* c = reinterpret_cast<Cache*>(ch);
   if (!c || (n && (!d_q || !d_out))) return fail(MRT_ERR_INVALID, "null argument");
   if (n == 0) return MRT_OK;
@@ -95,9 +109,11 @@ int mrt_cache_infer_device(mrt_cache ch, const mrt_cache_query* d_q, size_t n, f
                             static_cast<hipStream_t>(stream)) == 0
              ? MRT_OK
              : fail(MRT_ERR_HIP, "cache inference launch failed");
+  MRT_GUARD_END
 }
 
 int mrt_cache_infer(mrt_cache ch, const mrt_cache_query* q, size_t n, float* out) {
+  MRT_GUARD_BEGIN
   Cache* c = reinterpret_cast<Cache*>(ch);
   if (!c || (n && (!q || !out))) return fail(MRT_ERR_INVALID, "null argument");
   if (n == 0) return MRT_OK;
@@ -108,15 +124,19 @@ int mrt_cache_infer(mrt_cache ch, const mrt_cache_query* q, size_t n, float* out
   if (rc != MRT_OK) return rc;
   hipError_t e = hipMemcpy(out, dout.p, 3 * n * 4, hipMemcpyDeviceToHost);
   return e == hipSuccess ? MRT_OK : hip_fail(e, "reading cache outputs");
+  MRT_GUARD_END
 }
 
 int mrt_cache_train_device(mrt_cache ch, const mrt_cache_query* d_q, const float* d_targets, size_t n, void* stream) {
+  MRT_GUARD_BEGIN
   Cache* c = reinterpret_cast<Cache*>(ch);
   if (!c || (n && (!d_q || !d_targets))) return fail(MRT_ERR_INVALID, "null argument");
   return c->train(reinterpret_cast<const CacheQuery*>(d_q), d_targets, n, c->lr, static_cast<hipStream_t>(stream));
+  MRT_GUARD_END
 }
 
 int mrt_cache_train(mrt_cache ch, const mrt_cache_query* q, const float* targets, size_t n, float* loss) {
+  MRT_GUARD_BEGIN
   Cache* c = reinterpret_cast<Cache*>(ch);
   if (!c || (n && (!q || !targets))) return fail(MRT_ERR_INVALID, "null argument");
   if (n == 0) return MRT_OK;
@@ -130,10 +150,12 @@ int mrt_cache_train(mrt_cache ch, const mrt_cache_query* q, const float* targets
     if (e != hipSuccess) rc = hip_fail(e, "cache training");
   }
   return rc;
+  MRT_GUARD_END
 }
 
 int mrt_cache_gradient(mrt_cache ch, const mrt_cache_query* q, const float* targets, size_t n, float* grad,
                        float* loss) {
+  MRT_GUARD_BEGIN
   Cache* c = reinterpret_cast<Cache*>(ch);
   if (!c || !grad || (n && (!q || !targets))) return fail(MRT_ERR_INVALID, "null argument");
   if (n == 0) return fail(MRT_ERR_INVALID, "empty batch");
@@ -148,6 +170,7 @@ int mrt_cache_gradient(mrt_cache ch, const mrt_cache_query* q, const float* targ
     rc = c->get(c->grad, grad);
   }
   return rc;
+  MRT_GUARD_END
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 #include "../../include/myrt.h"
 #include "denoiser_internal.hpp"
 #include "scene_internal.hpp"
+#include "abi_guard.hpp"
 
 namespace mrt {
 
@@ -403,6 +404,7 @@ bool bad_batch(int B, int H, int W) {
 extern "C" {
 
 int mrt_denoiser_create(mrt_denoiser* out, uint64_t seed) {
+  MRT_GUARD_BEGIN
   if (!out) return fail(MRT_ERR_INVALID, "null output");
   auto* d = new Denoiser;
   const int rc = d->init(seed);
@@ -412,33 +414,43 @@ int mrt_denoiser_create(mrt_denoiser* out, uint64_t seed) {
   }
   *out = reinterpret_cast<mrt_denoiser>(d);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_destroy(mrt_denoiser d) {
+  MRT_GUARD_BEGIN
   delete reinterpret_cast<Denoiser*>(d);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_num_params(size_t* n) {
+  MRT_GUARD_BEGIN
   if (!n) return fail(MRT_ERR_INVALID, "null output");
   Denoiser d0;  // the layer table only; no device memory
   *n = d0.n_logical;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_get_params(mrt_denoiser dh, float* out, size_t n) {
+  MRT_GUARD_BEGIN
   Denoiser* d = reinterpret_cast<Denoiser*>(dh);
   if (!d || !out || n != d->n_logical) return fail(MRT_ERR_INVALID, "parameter count mismatch");
   return d->get_logical(out, false);
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_set_params(mrt_denoiser dh, const float* in, size_t n) {
+  MRT_GUARD_BEGIN
   Denoiser* d = reinterpret_cast<Denoiser*>(dh);
   if (!d || !in || n != d->n_logical) return fail(MRT_ERR_INVALID, "parameter count mismatch");
   return d->set_logical(in);
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_set_adam(mrt_denoiser dh, float lr, float beta1, float beta2, float eps) {
+  MRT_GUARD_BEGIN
   Denoiser* d = reinterpret_cast<Denoiser*>(dh);
   if (!d || !(lr > 0.f) || !(beta1 >= 0.f && beta1 < 1.f) || !(beta2 >= 0.f && beta2 < 1.f) || !(eps > 0.f))
     return fail(MRT_ERR_INVALID, "bad Adam parameters");
@@ -447,9 +459,11 @@ int mrt_denoiser_set_adam(mrt_denoiser dh, float lr, float beta1, float beta2, f
   d->beta2 = beta2;
   d->eps = eps;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_forward(mrt_denoiser dh, const float* raw, int B, int H, int W, int half, float* y) {
+  MRT_GUARD_BEGIN
   Denoiser* d = reinterpret_cast<Denoiser*>(dh);
   if (!d || !raw || !y) return fail(MRT_ERR_INVALID, "null argument");
   if (bad_batch(B, H, W)) return fail(MRT_ERR_INVALID, "batch dimensions must be multiples of 16");
@@ -473,10 +487,12 @@ int mrt_denoiser_forward(mrt_denoiser dh, const float* raw, int B, int H, int W,
   for (size_t p = 0; p < px; ++p)
     for (int c = 0; c < 3; ++c) y[p * 3 + c] = yp[p * kYc + c];
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_gradient(mrt_denoiser dh, const float* raw, const float* target, int B, int H, int W, float* grad,
                           float* loss) {
+  MRT_GUARD_BEGIN
   Denoiser* d = reinterpret_cast<Denoiser*>(dh);
   if (!d || !raw || !target || !grad) return fail(MRT_ERR_INVALID, "null argument");
   if (bad_batch(B, H, W)) return fail(MRT_ERR_INVALID, "batch dimensions must be multiples of 16");
@@ -486,17 +502,21 @@ int mrt_denoiser_gradient(mrt_denoiser dh, const float* raw, const float* target
   if (rc == MRT_OK) rc = d->train(d->raw_dev.p, d->g_tgt.p, B, H, W, 0.f, loss, nullptr);
   if (rc == MRT_OK) rc = d->get_logical(grad, true);
   return rc;
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_train_device(mrt_denoiser dh, const float* d_raw, const float* d_target, int B, int H, int W,
                               float* loss, void* stream) {
+  MRT_GUARD_BEGIN
   Denoiser* d = reinterpret_cast<Denoiser*>(dh);
   if (!d || !d_raw || !d_target) return fail(MRT_ERR_INVALID, "null argument");
   if (bad_batch(B, H, W)) return fail(MRT_ERR_INVALID, "batch dimensions must be multiples of 16");
   return d->train(d_raw, d_target, B, H, W, d->lr, loss, static_cast<hipStream_t>(stream));
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_train(mrt_denoiser dh, const float* raw, const float* target, int B, int H, int W, float* loss) {
+  MRT_GUARD_BEGIN
   Denoiser* d = reinterpret_cast<Denoiser*>(dh);
   if (!d || !raw || !target) return fail(MRT_ERR_INVALID, "null argument");
   if (bad_batch(B, H, W)) return fail(MRT_ERR_INVALID, "batch dimensions must be multiples of 16");
@@ -509,9 +529,11 @@ int mrt_denoiser_train(mrt_denoiser dh, const float* raw, const float* target, i
     if (e != hipSuccess) rc = hip_fail(e, "denoiser training");
   }
   return rc;
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_denoise(mrt_denoiser dh, const float* raw, int H, int W, float* rgba) {
+  MRT_GUARD_BEGIN
   Denoiser* d = reinterpret_cast<Denoiser*>(dh);
   if (!d || !raw || !rgba) return fail(MRT_ERR_INVALID, "null argument");
   if (H < 1 || W < 1 || size_t(H) * W > (size_t(1) << 26)) return fail(MRT_ERR_INVALID, "bad image size");
@@ -522,13 +544,16 @@ int mrt_denoiser_denoise(mrt_denoiser dh, const float* raw, int H, int W, float*
   if (rc != MRT_OK) return rc;
   hipError_t e = hipMemcpy(rgba, d->out_dev.p, px * 16, hipMemcpyDeviceToHost);
   return e == hipSuccess ? MRT_OK : hip_fail(e, "reading the denoised image");
+  MRT_GUARD_END
 }
 
 int mrt_denoiser_denoise_device(mrt_denoiser dh, const float* d_raw, int H, int W, float* d_rgba, void* stream) {
+  MRT_GUARD_BEGIN
   Denoiser* d = reinterpret_cast<Denoiser*>(dh);
   if (!d || !d_raw || !d_rgba) return fail(MRT_ERR_INVALID, "null argument");
   if (H < 1 || W < 1 || size_t(H) * W > (size_t(1) << 26)) return fail(MRT_ERR_INVALID, "bad image size");
   return d->denoise(d_raw, H, W, d_rgba, static_cast<hipStream_t>(stream));
+  MRT_GUARD_END
 }
 
 }  // extern "C"

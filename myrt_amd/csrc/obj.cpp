@@ -20,6 +20,7 @@
 
 #include "../../include/myrt.h"
 #include "scene_internal.hpp"
+#include "abi_guard.hpp"
 
 struct mrt_obj_s;
 
@@ -281,15 +282,6 @@ struct mrt_obj_s {
 
 using namespace mrt;
 
-#define MRT_GUARD_BEGIN try {
-#define MRT_GUARD_END                                 \
-  }                                                   \
-  catch (const std::bad_alloc&) {                     \
-    return fail(MRT_ERR_LIMIT, "out of host memory"); \
-  }                                                   \
-  catch (const std::exception& ex) {                  \
-    return fail(MRT_ERR_INTERNAL, ex.what());         \
-  }
 
 extern "C" {
 
@@ -312,17 +304,22 @@ int mrt_obj_load(const char* path, mrt_obj* out) {
 }
 
 int mrt_obj_destroy(mrt_obj o) {
+  MRT_GUARD_BEGIN
   delete o;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_obj_num_groups(mrt_obj o, size_t* n) {
+  MRT_GUARD_BEGIN
   if (!o || !n) return fail(MRT_ERR_INVALID, "null argument");
   *n = o->meshes.size();
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_obj_group(mrt_obj o, size_t i, mrt_obj_group_info* info) {
+  MRT_GUARD_BEGIN
   if (!o || !info) return fail(MRT_ERR_INVALID, "null argument");
   if (i >= o->meshes.size()) return fail(MRT_ERR_INVALID, "group index out of range");
   const ObjMesh& m = o->meshes[i];
@@ -342,9 +339,11 @@ int mrt_obj_group(mrt_obj o, size_t i, mrt_obj_group_info* info) {
     copy_name(info->albedo_texture, sizeof(info->albedo_texture), tex);
   }
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_obj_group_copy(mrt_obj o, size_t i, uint32_t* idx, float* pos3, float* nrm3, float* uv2) {
+  MRT_GUARD_BEGIN
   if (!o) return fail(MRT_ERR_INVALID, "null argument");
   if (i >= o->meshes.size()) return fail(MRT_ERR_INVALID, "group index out of range");
   const ObjMesh& m = o->meshes[i];
@@ -353,6 +352,7 @@ int mrt_obj_group_copy(mrt_obj o, size_t i, uint32_t* idx, float* pos3, float* n
   if (nrm3) std::memcpy(nrm3, m.normals.data(), m.normals.size() * 4);
   if (uv2) std::memcpy(uv2, m.texcoords.data(), m.texcoords.size() * 4);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_scene_push_obj(mrt_scene sh, mrt_obj o, uint32_t* geometries, uint32_t* materials, size_t n) {

@@ -43,6 +43,17 @@ __device__ inline V3 normalize(V3 a) {
   return a * (1.0f / l);
 }
 __device__ inline V3 ld3(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ inline bool finite3(V3 a) {
+  return __builtin_isfinite(a.x) && __builtin_isfinite(a.y) && __builtin_isfinite(a.z);
+}
+
+// Adds the wave's count of lanes with `bad` set to the 64-bit counter at counts + 10 (non-finite
+// values rejected). Every lane of the wave must call it.
+__device__ inline void count_nonfinite(uint32_t* counts, bool bad) {
+  const uint64_t m = __ballot(bad);
+  if (m && __lane_id() == uint32_t(__ffsll((unsigned long long)m) - 1))
+    atomicAdd(reinterpret_cast<unsigned long long*>(counts + 10), (unsigned long long)__popcll(m));
+}
 
 // Sampler: lowbias32 (C. Wellons' integer hash) chained over (pixel, sample, dimension).
 __device__ inline uint32_t mix32(uint32_t x) {
@@ -380,6 +391,7 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffer
   V3 sh_o = mk(0, 0, 0), sh_d = mk(0, 0, 1), sh_c = mk(0, 0, 0);
   V3 pix_c = mk(0, 0, 0);  // emitter or environment seen by the ray
   bool pix_on = false, aux_on = false;
+  bool bad = false;  // a non-finite value was rejected (pt_color.comp:84-98, 156-166)
   float aux_v[8] = {};  // primary-hit features (albedo, depth, shading normal, count)
   PathState ps{};
   if (live) {
@@ -389,57 +401,20 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffer
     const float4 r0 = rp[0], r1 = rp[1];
     const V3 ro = mk(r0.x, r0.y, r0.z), rd = mk(r1.x, r1.y, r1.z);
     const uint4 hit = reinterpret_cast<const uint4*>(b.hits)[src];
-    const bool found = hit.z != 0xFFFFFFFFu;
+    bool found = hit.z != 0xFFFFFFFFu;
     const float t_hit = found ? __uint_as_float(hit.x) : __int_as_float(0x7f800000);
     const uint32_t bounce = ps.flags & 0xffu;
     V3 thr = mk(ps.thr[0], ps.thr[1], ps.thr[2]);
 
-    // Emitters along the ray (lights are not in the BVH).
-    int light_hit = -1;
-    float t_light = t_hit;
-    float cos_light = 0.f;
-    for (uint32_t l = 0; l < sc.num_lights; ++l) {
-      const QuadLight& q = sc.lights[l];
-      const V3 u = ld3(q.u), v = ld3(q.v), c = ld3(q.corner);
-      const V3 nl = cross(u, v);
-      const float area = sqrtf(dot(nl, nl));
-      const V3 n1 = nl * (1.0f / area);
-      const float den = dot(n1, rd);
-      if (den >= 0.f) continue;  // one-sided: seen from behind it neither emits nor blocks
-      const float t = dot(n1, c - ro) / den;
-      if (!(t > 1e-6f && t < t_light)) continue;
-      const V3 pq = ro + rd * t - c;
-      const float a = dot(pq, u) / dot(u, u), bb = dot(pq, v) / dot(v, v);
-      if (a < 0.f || a > 1.f || bb < 0.f || bb > 1.f) continue;
-      light_hit = int(l);
-      t_light = t;
-      cos_light = -den;
-    }
-    if (light_hit >= 0) {
-      const QuadLight& q = sc.lights[light_hit];
-      float w = 1.0f;
-      if (!(ps.flags & kDelta)) {
-        const V3 nl = cross(ld3(q.u), ld3(q.v));
-        const float area = sqrtf(dot(nl, nl));
-        const float pdf_l = t_light * t_light / (cos_light * area * float(sc.num_lights));
-        w = ps.last_pdf * ps.last_pdf / (ps.last_pdf * ps.last_pdf + pdf_l * pdf_l);
-      }
-      const V3 c = thr * ld3(q.emission) * w;
-      pix_c = c;
-      pix_on = true;
-      add_train(b, ps.pad, c);
-    } else if (!found) {
-      const V3 c = thr * environment(sc, rd);
-      pix_c = c;
-      pix_on = true;
-      add_train(b, ps.pad, c);
-    } else {
-      // ---- surface interaction ----
+    // Normals of the surface hit. A non-finite geometric normal (a degenerate triangle) makes the
+    // hit a miss, as pt_trace.comp:186-189 does; a non-finite shading normal (say NaN vertex
+    // normals) falls back to the geometric one.
+    V3 ng = mk(0, 0, 1), ns = ng;
+    uint32_t mi = 0;
+    float hu = 0.f, hv = 0.f;
+    bool has_uv = false;
+    if (found) {
       const uint32_t inst = hit.z, prim = hit.y;
-      V3 ng, ns;
-      uint32_t mi;
-      float hu = 0.f, hv = 0.f;
-      bool has_uv = false;
       if (inst & 0x80000000u) {
         // SDF surface: the record carries the world normal and the material.
         ng = oct_decode(hit.w);
@@ -474,6 +449,64 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffer
                           iv[2] * nso.x + iv[6] * nso.y + iv[10] * nso.z));
         mi = sc.inst_material[inst];
       }
+      if (!finite3(ng)) {
+        found = false;
+        bad = true;
+      } else if (!finite3(ns)) {
+        ns = ng;
+        bad = true;
+      }
+    }
+
+    // Emitters along the ray (lights are not in the BVH).
+    int light_hit = -1;
+    float t_light = t_hit;
+    float cos_light = 0.f;
+    for (uint32_t l = 0; l < sc.num_lights; ++l) {
+      const QuadLight& q = sc.lights[l];
+      const V3 u = ld3(q.u), v = ld3(q.v), c = ld3(q.corner);
+      const V3 nl = cross(u, v);
+      const float area = sqrtf(dot(nl, nl));
+      const V3 n1 = nl * (1.0f / area);
+      const float den = dot(n1, rd);
+      if (den >= 0.f) continue;  // one-sided: seen from behind it neither emits nor blocks
+      const float t = dot(n1, c - ro) / den;
+      if (!(t > 1e-6f && t < t_light)) continue;
+      const V3 pq = ro + rd * t - c;
+      const float a = dot(pq, u) / dot(u, u), bb = dot(pq, v) / dot(v, v);
+      if (a < 0.f || a > 1.f || bb < 0.f || bb > 1.f) continue;
+      light_hit = int(l);
+      t_light = t;
+      cos_light = -den;
+    }
+    if (light_hit >= 0) {
+      const QuadLight& q = sc.lights[light_hit];
+      float w = 1.0f;
+      if (!(ps.flags & kDelta)) {
+        const V3 nl = cross(ld3(q.u), ld3(q.v));
+        const float area = sqrtf(dot(nl, nl));
+        const float pdf_l = t_light * t_light / (cos_light * area * float(sc.num_lights));
+        w = ps.last_pdf * ps.last_pdf / (ps.last_pdf * ps.last_pdf + pdf_l * pdf_l);
+      }
+      const V3 c = thr * ld3(q.emission) * w;
+      if (finite3(c)) {
+        pix_c = c;
+        pix_on = true;
+        add_train(b, ps.pad, c);
+      } else {
+        bad = true;
+      }
+    } else if (!found) {
+      const V3 c = thr * environment(sc, rd);
+      if (finite3(c)) {
+        pix_c = c;
+        pix_on = true;
+        add_train(b, ps.pad, c);
+      } else {
+        bad = true;
+      }
+    } else {
+      // ---- surface interaction ----
       const V3 p = ro + rd * t_hit;
       const V3 wo = -rd;
       const bool entering = dot(rd, ng) < 0.f;
@@ -585,7 +618,9 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffer
           const float pdf_l = d2 / (cl * area * float(sc.num_lights));
           const float w = pdf_l * pdf_l / (pdf_l * pdf_l + pdf_b * pdf_b);
           const V3 c = thr * f * ld3(q.emission) * (dot(ns, wi) * w / pdf_l);
-          if (c.x > 0.f || c.y > 0.f || c.z > 0.f) {
+          if (!finite3(c)) {
+            bad = true;
+          } else if (c.x > 0.f || c.y > 0.f || c.z > 0.f) {
             shadow = true;
             sh_o = p + ng * eps;
             sh_d = x - sh_o;
@@ -641,6 +676,10 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffer
           ps.flags &= ~kDelta;
         }
       }
+      if (ok && !finite3(thr)) {
+        ok = false;  // the path ends (pt_color.comp:156-166)
+        bad = true;
+      }
       if (ok && bounce >= 3) {
         const float q = fminf(0.95f, fmaxf(fmaxf(thr.x, thr.y), thr.z));
         if (rnd(ps.pixel, ps.sample, dim_of(bounce, 3)) >= q)
@@ -662,6 +701,7 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffer
     }
   }
   wave_add_pixel(b.accum, ps.pixel, pix_c, pix_on);
+  count_nonfinite(b.counts, bad);
   if (b.aux) wave_run_add<8>(reinterpret_cast<float*>(b.aux + 2 * size_t(ps.pixel)), ps.pixel, aux_v, aux_on);
   uint32_t* const queues[3] = {&b.counts[nxt], &b.counts[2], &b.counts[4]};
   const bool append[3] = {survive, shadow, want_query};
@@ -719,12 +759,15 @@ __global__ __launch_bounds__(kBlock) void cache_resolve_kernel(PtBuffers b) {
   const bool on = i < n;
   CacheRecord r{};
   V3 c = mk(0, 0, 0);
+  bool bad = false;
   if (on) {
     r = b.cache.qrec[i];
     const float* y = b.cache.qout + 3 * size_t(i);
     c = mk(r.thr[0] * fmaxf(y[0], 0.f), r.thr[1] * fmaxf(y[1], 0.f), r.thr[2] * fmaxf(y[2], 0.f));
+    bad = !finite3(c);
   }
-  wave_add_pixel(b.accum, r.pixel, c, on);
+  wave_add_pixel(b.accum, r.pixel, c, on && !bad);
+  count_nonfinite(b.counts, bad);
 }
 
 // Training samples: suffix radiance (T - prefix) / thr of every recorded vertex.

@@ -143,7 +143,8 @@ struct PtBuffers {
   ShadowRecord* shadow;
   uint8_t* occluded;
   uint32_t* counts;       // [0], [1]: path queues; [2]: shadow queue; [3] training paths;
-                          // [4] cache queries; [5] training samples; [8..9] rays traced (u64)
+                          // [4] cache queries; [5] training samples; [8..9] rays traced (u64);
+                          // [10..11] non-finite values rejected (u64)
   float4* accum;          // per pixel running sum (rgb); w unused
   float4* aux;            // per pixel 2 x float4 of primary-hit sums: albedo, t; normal, hits
   const uint32_t* perm;   // shading position -> queue entry (material order), or null

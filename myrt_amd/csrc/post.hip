@@ -10,6 +10,7 @@
 
 #include "../../include/myrt.h"
 #include "scene_internal.hpp"
+#include "abi_guard.hpp"
 
 namespace mrt {
 namespace ppk {
@@ -157,14 +158,17 @@ extern "C" {
 
 int mrt_post_pass_device(int op, const float* d_in, float* d_out, int w, int h, float exponent, float strength,
                          void* stream) {
+  MRT_GUARD_BEGIN
   if (!d_in || !d_out || d_in == d_out) return fail(MRT_ERR_INVALID, "need distinct input and output images");
   if (bad_image(w, h)) return fail(MRT_ERR_INVALID, "bad image size");
   if (op == MRT_POST_BILATERAL && !(exponent > 0.f && strength > 0.f)) return fail(MRT_ERR_INVALID, "sigmas must be > 0");
   return launch_post(op, d_in, d_out, w, h, exponent, strength, static_cast<hipStream_t>(stream));
+  MRT_GUARD_END
 }
 
 int mrt_post_process(const float* rgba_in, float* rgba_out, int w, int h, const int* ops, int n_ops, float exponent,
                      float strength) {
+  MRT_GUARD_BEGIN
   if (!rgba_in || !rgba_out || (n_ops > 0 && !ops) || n_ops < 0) return fail(MRT_ERR_INVALID, "null argument");
   if (bad_image(w, h)) return fail(MRT_ERR_INVALID, "bad image size");
   const size_t bytes = size_t(w) * h * 16;
@@ -184,6 +188,7 @@ int mrt_post_process(const float* rgba_in, float* rgba_out, int w, int h, const 
   if (a) hipFree(a);
   if (b) hipFree(b);
   return rc;
+  MRT_GUARD_END
 }
 
 }  // extern "C"

@@ -21,6 +21,7 @@
 #include "matsort.hpp"
 #include "pathtrace.hpp"
 #include "scene_internal.hpp"
+#include "abi_guard.hpp"
 
 namespace mrt {
 namespace {
@@ -96,26 +97,27 @@ struct Renderer {
   uint32_t sample_count = 0;
   uint32_t width = 0, height = 0;
   size_t cap_paths = 0, cap_accum = 0;
-  // Queues of one sample pipeline: ray and path queues (current and next bounce), hits, shadow
-  // rays and records, occlusion flags, queue counters, material-sort scratch, and the streams and
-  // events its passes run on.
+  // The sample pipeline's queues: ray and path queues (current and next bounce), hits, shadow
+  // rays and records, occlusion flags, queue counters, material-sort scratch, and the shadow
+  // stream and events its passes run on.
   struct Pipe {
     void* rays[2] = {nullptr, nullptr};
     PathState* paths[2] = {nullptr, nullptr};
     void *hits = nullptr, *shadow_rays = nullptr;
     ShadowRecord* shadow = nullptr;
     uint8_t* occluded = nullptr;
-    uint32_t* counts = nullptr;  // [0..1] queues, [2] shadow queue, [3..5] cache, [8..9] rays traced (u64)
+    uint32_t* counts = nullptr;  // [0..1] queues, [2] shadow queue, [3..5] cache, [8..9] rays traced (u64),
+                                 // [10..11] non-finite values rejected (u64)
     uint32_t *ms_keys = nullptr, *ms_perm = nullptr, *ms_counts = nullptr;
     size_t c_rays0 = 0, c_rays1 = 0, c_paths0 = 0, c_paths1 = 0, c_hits = 0, c_shr = 0, c_sh = 0, c_occ = 0, c_cnt = 0;
     size_t c_msk = 0, c_msp = 0, c_msc = 0;
-    hipStream_t main = nullptr, shadow_stream = nullptr;  // pipe 0's main stream is the renderer's
-    hipEvent_t ev_shaded = nullptr, ev_resolved = nullptr, ev_done = nullptr;
+    hipStream_t main = nullptr, shadow_stream = nullptr;  // main is the renderer's stream
+    hipEvent_t ev_shaded = nullptr, ev_resolved = nullptr;
     ~Pipe() {
       for (void* p : {rays[0], rays[1], (void*)paths[0], (void*)paths[1], hits, shadow_rays, (void*)shadow,
                       (void*)occluded, (void*)counts, (void*)ms_keys, (void*)ms_perm, (void*)ms_counts})
         if (p) hipFree(p);
-      for (hipEvent_t ev : {ev_shaded, ev_resolved, ev_done})
+      for (hipEvent_t ev : {ev_shaded, ev_resolved})
         if (ev) hipEventDestroy(ev);
       if (shadow_stream) hipStreamDestroy(shadow_stream);
     }
@@ -132,19 +134,15 @@ struct Renderer {
         return rc;
       return MRT_OK;
     }
-    int ensure_streams(bool own_main) {
+    int ensure_streams() {
       hipError_t e = hipSuccess;
-      if (own_main && !main) e = hipStreamCreateWithFlags(&main, hipStreamNonBlocking);  // default priority
-      if (e == hipSuccess && !shadow_stream) e = hipStreamCreateWithFlags(&shadow_stream, hipStreamNonBlocking);
+      if (!shadow_stream) e = hipStreamCreateWithFlags(&shadow_stream, hipStreamNonBlocking);
       if (e == hipSuccess && !ev_shaded) e = hipEventCreateWithFlags(&ev_shaded, hipEventDisableTiming);
       if (e == hipSuccess && !ev_resolved) e = hipEventCreateWithFlags(&ev_resolved, hipEventDisableTiming);
-      if (e == hipSuccess && !ev_done) e = hipEventCreateWithFlags(&ev_done, hipEventDisableTiming);
       return e == hipSuccess ? MRT_OK : hip_fail(e, "creating render streams");
     }
   };
-  static constexpr int kPipes = 2;
-  Pipe pipes[kPipes];
-  hipEvent_t ev_start = nullptr;
+  Pipe pipe;
   float4* accum = nullptr;
   float4* aux = nullptr;  // primary-hit feature sums, 2 per pixel
   size_t cap_aux = 0;
@@ -195,18 +193,10 @@ struct Renderer {
   static constexpr size_t kCacheBatchPaths = size_t(1) << 21;
   uint32_t train_scale = 1;  // cache training of the current batch, in units of kCacheBatchPaths
   size_t max_batch_paths = size_t(1) << 25;
-  // Sample pipelines (MRT_PIPELINES=2 for two). With two, the samples of a render are split
-  // between them; each runs its own batches on its own streams and trace contexts, and both add
-  // into the same accumulators with atomics, so the image is the one-pipeline image up to fp32
-  // summation order. With batches of 2^25 paths and the shadow stream the device is already full:
-  // two pipelines rendered 1920x1080 at 16 spp in 0.134 s against 0.132 s with one, so one is the
-  // default. The radiance cache trains on per-batch buffers and always uses one.
-  int n_pipelines = 1;
 
   Renderer() {
     if (const char* v = std::getenv("MRT_MATERIAL_SORT")) material_sort = std::strtol(v, nullptr, 10) != 0;
     if (const char* v = std::getenv("MRT_SHADOW_STREAM")) shadow_overlap = std::strtol(v, nullptr, 10) != 0;
-    if (const char* v = std::getenv("MRT_PIPELINES")) n_pipelines = std::strtol(v, nullptr, 10) >= 2 ? 2 : 1;
     if (const char* v = std::getenv("MRT_BATCH_PATHS")) {
       const long long n = std::strtoll(v, nullptr, 10);
       if (n >= 1024 && n <= (1ll << 26)) max_batch_paths = size_t(n);
@@ -219,8 +209,6 @@ struct Renderer {
     for (void* p : {(void*)accum, (void*)tpaths, (void*)tverts, (void*)queries, (void*)qrec, (void*)qout,
                     (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den, (void*)bokeh, (void*)cubemap})
       if (p) hipFree(p);
-    if (ev_start) hipEventDestroy(ev_start);
-    if (pipes[1].main) hipStreamDestroy(pipes[1].main);
   }
 
   int update_camera() {
@@ -263,19 +251,15 @@ struct Renderer {
       sample_count = 0;
     }
     const bool use_cache = cache && cache_mode > 0;
-    const int n_pipes = (!use_cache && n_pipelines > 1 && spp >= 2) ? 2 : 1;
-    // Samples per batch; all pipelines together keep at most max_batch_paths paths in flight.
-    const uint32_t per_pipe = (spp + uint32_t(n_pipes) - 1) / uint32_t(n_pipes);
-    const uint32_t batch =
-        uint32_t(std::max<size_t>(1, std::min<size_t>(per_pipe, max_batch_paths / (npix * size_t(n_pipes)))));
+    // Samples per batch: at most max_batch_paths paths in flight.
+    const uint32_t batch = uint32_t(std::max<size_t>(1, std::min<size_t>(spp, max_batch_paths / npix)));
     const size_t cap = npix * batch;
     // The radiance cache trains after every batch; its cadence was set for batches of
     // kCacheBatchPaths paths. A larger batch trains as that many of them would have: proportionally
     // more samples and steps, each step the same size.
     const size_t cadence_spp = std::max<size_t>(1, kCacheBatchPaths / npix);
     const uint32_t batch_cap = max_batch * uint32_t((batch + cadence_spp - 1) / cadence_spp);
-    for (int pi = 0; pi < n_pipes; ++pi)
-      if ((rc = pipes[pi].ensure_buffers(cap, material_sort)) != MRT_OK) return rc;
+    if ((rc = pipe.ensure_buffers(cap, material_sort)) != MRT_OK) return rc;
     if (!accum || cap_accum < npix) {
       if ((rc = ensure(accum, cap_accum, npix)) != MRT_OK) return rc;
       if ((rc = ensure(aux, cap_aux, 2 * npix)) != MRT_OK) return rc;
@@ -317,40 +301,20 @@ struct Renderer {
     sc.cubemap = cubemap;
     sc.cube_size = cube_size;
     const auto t0 = std::chrono::steady_clock::now();
-    pipes[0].main = stream;
-    for (int pi = 0; pi < n_pipes; ++pi) {
-      if ((shadow_overlap || pi > 0) && (rc = pipes[pi].ensure_streams(pi > 0)) != MRT_OK) return rc;
-      hipError_t e = hipSuccess;
-      if (pi > 0) {
-        // The second pipeline starts after the accumulator clear on the renderer's stream.
-        if (!ev_start) e = hipEventCreateWithFlags(&ev_start, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventRecord(ev_start, stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(pipes[pi].main, ev_start, 0);
-      }
-      if (e == hipSuccess) e = hipMemsetAsync(pipes[pi].counts + 8, 0, 8, pipes[pi].main);
-      if (e != hipSuccess) return hip_fail(e, "starting the render pipelines");
-    }
-    uint32_t done = 0;
-    for (int k = 0; done < spp; ++k) {
+    pipe.main = stream;
+    if (shadow_overlap && (rc = pipe.ensure_streams()) != MRT_OK) return rc;
+    hipError_t e = hipMemsetAsync(pipe.counts + 8, 0, 16, stream);
+    if (e != hipSuccess) return hip_fail(e, "clearing the render counters");
+    for (uint32_t done = 0; done < spp;) {
       const uint32_t nb = std::min(batch, spp - done);
-      if ((rc = run_batch(s, sc, pipes[k % n_pipes], k % n_pipes, nb, npix, sample_count + done, pc, cadence_spp)) !=
-          MRT_OK)
-        return rc;
+      if ((rc = run_batch(s, sc, pipe, nb, npix, sample_count + done, pc, cadence_spp)) != MRT_OK) return rc;
       done += nb;
     }
-    for (int pi = 1; pi < n_pipes; ++pi) {
-      hipError_t e = hipEventRecord(pipes[pi].ev_done, pipes[pi].main);
-      if (e == hipSuccess) e = hipStreamWaitEvent(stream, pipes[pi].ev_done, 0);
-      if (e != hipSuccess) return hip_fail(e, "joining the render pipelines");
-    }
-    uint64_t rays_traced = 0;
-    for (int pi = 0; pi < n_pipes; ++pi) {
-      uint64_t r = 0;
-      hipError_t e = hipMemcpyAsync(&r, pipes[pi].counts + 8, 8, hipMemcpyDeviceToHost, stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(stream);
-      if (e != hipSuccess) return hip_fail(e, "render");
-      rays_traced += r;
-    }
+    uint64_t ctr[2] = {0, 0};  // rays traced, non-finite values rejected
+    e = hipMemcpyAsync(ctr, pipe.counts + 8, 16, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "render");
+    const uint64_t rays_traced = ctr[0];
     uint32_t flags = 0;
     if ((rc = s.read_ctrl(&flags, nullptr, stream)) != MRT_OK) return rc;
     if (flags & 1u) return fail(MRT_ERR_INTERNAL, "traversal stack overflow");
@@ -358,14 +322,15 @@ struct Renderer {
     sample_count += spp;
     stats.samples += uint64_t(spp) * npix;
     stats.rays += rays_traced;
+    stats.nonfinite += ctr[1];
     stats.milliseconds += std::chrono::duration<double, std::milli>(t1 - t0).count();
     return MRT_OK;
   }
 
-  // One batch of nb samples per pixel on pipeline `pi`: generate, then per bounce closest hits,
-  // material sort, shade, and the shadow pass (on the pipeline's shadow stream when overlapped),
-  // then the cache pass. Trace contexts 2 pi (closest) and 2 pi + 1 (shadow).
-  int run_batch(Scene& s, const PtScene& sc, Pipe& p, int pi, uint32_t nb, size_t npix, uint32_t sample_base,
+  // One batch of nb samples per pixel: generate, then per bounce closest hits, material sort,
+  // shade, and the shadow pass (on the shadow stream when overlapped), then the cache pass. Trace
+  // contexts 0 (closest) and 1 (shadow).
+  int run_batch(Scene& s, const PtScene& sc, Pipe& p, uint32_t nb, size_t npix, uint32_t sample_base,
                 const PtCache& pc, size_t cadence_spp) {
     int rc;
     const uint32_t n_paths = uint32_t(npix * nb);
@@ -388,7 +353,7 @@ struct Renderer {
     hipStream_t st = p.main;
     const bool overlap = shadow_overlap;
     hipStream_t sh_stream = overlap ? p.shadow_stream : st;
-    const int ctx_closest = 2 * pi, ctx_shadow = overlap ? 2 * pi + 1 : 2 * pi;
+    const int ctx_closest = 0, ctx_shadow = overlap ? 1 : 0;
     uint32_t* counts = p.counts;
     uint32_t cur = 0;
     hipError_t e0 = hipMemsetAsync(counts + 3, 0, 3 * 4, st);
@@ -519,12 +484,6 @@ struct Renderer {
 
 using namespace mrt;
 
-#define MRT_GUARD_BEGIN try {
-#define MRT_GUARD_END                                 \
-  }                                                   \
-  catch (const std::exception& ex) {                  \
-    return fail(MRT_ERR_INTERNAL, ex.what());         \
-  }
 
 extern "C" {
 
@@ -558,23 +517,28 @@ int mrt_scene_push_material(mrt_scene sh, const mrt_material* mat, uint32_t* out
 }
 
 int mrt_scene_set_material(mrt_scene sh, uint32_t id, const mrt_material* mat) {
+  MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s || !mat) return fail(MRT_ERR_INVALID, "null argument");
   if (id >= s->materials.size()) return fail(MRT_ERR_INVALID, "unknown material");
   std::memcpy(&s->materials[id], mat, sizeof(Material));
   s->shading_dirty = true;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_scene_get_material(mrt_scene sh, uint32_t id, mrt_material* out) {
+  MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s || !out) return fail(MRT_ERR_INVALID, "null argument");
   if (id >= s->materials.size()) return fail(MRT_ERR_INVALID, "unknown material");
   std::memcpy(out, &s->materials[id], sizeof(Material));
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_scene_set_material_param(mrt_scene sh, uint32_t id, const char* name, const float* value, size_t n) {
+  MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s || !name || !value) return fail(MRT_ERR_INVALID, "null argument");
   if (id >= s->materials.size()) return fail(MRT_ERR_INVALID, "unknown material");
@@ -611,6 +575,7 @@ int mrt_scene_set_material_param(mrt_scene sh, uint32_t id, const char* name, co
   }
   s->shading_dirty = true;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_scene_set_lights(mrt_scene sh, const mrt_quad_light* lights, size_t n) {
@@ -625,11 +590,13 @@ int mrt_scene_set_lights(mrt_scene sh, const mrt_quad_light* lights, size_t n) {
 }
 
 int mrt_scene_set_environment(mrt_scene sh, float scale) {
+  MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s) return fail(MRT_ERR_INVALID, "null scene");
   s->env_scale = scale;
   s->shading_dirty = true;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_create(mrt_renderer* out) {
@@ -641,45 +608,56 @@ int mrt_renderer_create(mrt_renderer* out) {
 }
 
 int mrt_renderer_destroy(mrt_renderer r) {
+  MRT_GUARD_BEGIN
   delete reinterpret_cast<Renderer*>(r);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_set_camera(mrt_renderer rh, const float view_colmajor[16], const float proj_colmajor[16]) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || !view_colmajor || !proj_colmajor) return fail(MRT_ERR_INVALID, "null argument");
   std::memcpy(r->view, view_colmajor, sizeof(r->view));
   std::memcpy(r->proj, proj_colmajor, sizeof(r->proj));
   r->sample_count = 0;  // every setter invalidates the counter (sequential_pathtracer.cpp:70-144)
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_set_dof_enabled(mrt_renderer rh, int enabled) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r) return fail(MRT_ERR_INVALID, "null renderer");
   if (r->dof != (enabled != 0)) r->sample_count = 0;
   r->dof = enabled != 0;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_set_lens_size(mrt_renderer rh, float w, float h) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || !std::isfinite(w) || !std::isfinite(h)) return fail(MRT_ERR_INVALID, "bad lens size");
   if (w != r->lens[0] || h != r->lens[1]) r->sample_count = 0;
   r->lens[0] = w;
   r->lens[1] = h;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_set_focus(mrt_renderer rh, float focus) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || !(focus > 0.f) || !std::isfinite(focus)) return fail(MRT_ERR_INVALID, "focus must be positive");
   if (focus != r->focus) r->sample_count = 0;
   r->focus = focus;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_set_bokeh_mask(mrt_renderer rh, const float* rgba, int w, int h) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || w < 0 || h < 0 || (w > 0 && (!rgba || h == 0))) return fail(MRT_ERR_INVALID, "bad bokeh mask");
   if (r->bokeh) hipFree(r->bokeh);
@@ -694,9 +672,11 @@ int mrt_renderer_set_bokeh_mask(mrt_renderer rh, const float* rgba, int w, int h
   r->bokeh_w = uint32_t(w);
   r->bokeh_h = uint32_t(h);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_set_cubemap(mrt_renderer rh, const float* faces, int size) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || size < 0 || (size > 0 && !faces) || size > 16384) return fail(MRT_ERR_INVALID, "bad cube map");
   if (r->cubemap) hipFree(r->cubemap);
@@ -711,25 +691,31 @@ int mrt_renderer_set_cubemap(mrt_renderer rh, const float* faces, int size) {
   if (e != hipSuccess) return hip_fail(e, "uploading the cube map");
   r->cube_size = uint32_t(size);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_set_num_bounces(mrt_renderer rh, int n) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || n < 1 || n > 64) return fail(MRT_ERR_INVALID, "bounces must be in [1, 64]");
   r->num_bounces = n;
   r->sample_count = 0;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_set_material_sort(mrt_renderer rh, int enabled) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r) return fail(MRT_ERR_INVALID, "null renderer");
   r->material_sort = enabled != 0;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_set_cache(mrt_renderer rh, mrt_cache c, int mode, float train_fraction, int train_steps,
                            int min_bounce) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || mode < 0 || mode > 2 || (mode > 0 && !c) || !(train_fraction > 0.f && train_fraction <= 1.f) ||
       train_steps < 1 || train_steps > 64 || min_bounce < 1 || min_bounce > 16)
@@ -740,20 +726,25 @@ int mrt_renderer_set_cache(mrt_renderer rh, mrt_cache c, int mode, float train_f
   r->train_fraction = train_fraction;
   r->train_steps = train_steps;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_invalidate(mrt_renderer rh) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r) return fail(MRT_ERR_INVALID, "null renderer");
   r->sample_count = 0;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_renderer_sample_count(mrt_renderer rh, uint32_t* out) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || !out) return fail(MRT_ERR_INVALID, "null argument");
   *out = r->sample_count;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_render(mrt_renderer rh, mrt_scene sh, int width, int height, int spp) {
@@ -790,10 +781,12 @@ int mrt_renderer_denoise(mrt_renderer rh, mrt_denoiser dh, float* rgba) {
 }
 
 int mrt_renderer_stats(mrt_renderer rh, mrt_render_stats* out) {
+  MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || !out) return fail(MRT_ERR_INVALID, "null argument");
   *out = r->stats;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 }  // extern "C"

@@ -22,11 +22,11 @@
 #include "../../include/myrt.h"
 #include "bvh_build.hpp"
 #include "device_types.hpp"
-#include "raysort.hpp"
 #include "scene_internal.hpp"
 #include "sdf_kernels.hpp"
 #include "trace.hpp"
 #include "wide_bvh.hpp"
+#include "abi_guard.hpp"
 
 namespace mrt {
 
@@ -185,13 +185,6 @@ int Scene::build_host() {
         std::vector<Box> boxes;
         boxes.reserve(current.size());
         for (const Instance& in : current) boxes.push_back(in.world_box);
-        Box all;
-        for (const Box& b : boxes) all.enclose(b);
-        for (int a = 0; a < 3; ++a) {
-          const float ext = all.mx[a] - all.mn[a];
-          scene_lo[a] = all.mn[a];
-          scene_inv_ext[a] = ext > 0.f && std::isfinite(ext) ? 1.0f / ext : 1.0f;
-        }
         tlas = sah_builder ? build_sah_bvh(boxes, tlas_leaf) : build_binary_bvh(boxes, false);
         WideBvh w = collapse_to_wide(tlas, kTlasPadRel, tlas_leaf);
         tlas_nodes = std::move(w.nodes);
@@ -300,13 +293,9 @@ Scene::~Scene() {
   for (TraceScratch& c : ctx)
     for (void* q : {(void*)c.ctrl, (void*)c.spill, (void*)c.sdf_side, (void*)c.sdf_hits})
       if (q) hipFree(q);
-  for (void* q : {(void*)d_group_keys, (void*)d_group_perm, (void*)d_group_counts})
-    if (q) hipFree(q);
   if (d_tlas) hipFree(d_tlas);
   if (d_tlas_items) hipFree(d_tlas_items);
   if (d_inst) hipFree(d_inst);
-  for (uint32_t* q : {d_sort_keys, d_sort_perm, d_sort_counts})
-    if (q) hipFree(q);
 }
 
 PtScene Scene::shading_view() const {
@@ -381,71 +370,8 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.refill_below = refill_below;
   p.grab = trace_grab;
   p.lds_depth = lds_depth;
-  p.perm = nullptr;
   p.waves = waves;
   p.single_instance = current.size() == 1 ? 0u : 0xFFFFFFFFu;
-  // The binning options keep one set of buffers and serve context 0 only (they change the order
-  // rays are taken in, never a result).
-  if (ray_sort && context == 0 && n >= 4096) {
-    if (cap_sort < n || !d_sort_counts) {
-      for (uint32_t* q : {d_sort_keys, d_sort_perm, d_sort_counts})
-        if (q) hipFree(q);
-      d_sort_keys = d_sort_perm = d_sort_counts = nullptr;
-      cap_sort = 0;
-      hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&d_sort_keys), n * 4);
-      if (e1 == hipSuccess) e1 = hipMalloc(reinterpret_cast<void**>(&d_sort_perm), n * 4);
-      if (e1 == hipSuccess) e1 = hipMalloc(reinterpret_cast<void**>(&d_sort_counts), kRaySortBins * 4);
-      if (e1 != hipSuccess) return hip_fail(e1, "hipMalloc ray binning");
-      cap_sort = n;
-    }
-    RaySortArgs ra{};
-    ra.rays = d_rays;
-    ra.n = uint32_t(n);
-    ra.n_dev = n_dev;
-    for (int a = 0; a < 3; ++a) {
-      ra.lo[a] = scene_lo[a];
-      ra.inv_ext[a] = scene_inv_ext[a];
-    }
-    ra.keys = d_sort_keys;
-    ra.counts = d_sort_counts;
-    ra.perm = d_sort_perm;
-    if (launch_ray_sort(ra, stream) != 0) return fail(MRT_ERR_HIP, "ray binning launch failed");
-    p.perm = d_sort_perm;
-  }
-  if (xcd_groups && context == 0 && !p.perm && n >= 65536) {
-    // Group rays by origin slab so that each XCD's L2 serves mostly its own part of the BVH.
-    if (cap_groups < n) {
-      for (void* q : {(void*)d_group_keys, (void*)d_group_perm})
-        if (q) hipFree(q);
-      d_group_keys = nullptr;
-      d_group_perm = nullptr;
-      cap_groups = 0;
-      hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&d_group_keys), n);
-      if (e1 == hipSuccess) e1 = hipMalloc(reinterpret_cast<void**>(&d_group_perm), n * 4);
-      if (e1 == hipSuccess && !d_group_counts)
-        e1 = hipMalloc(reinterpret_cast<void**>(&d_group_counts), 2 * (kXcdGroups + 1) * 4);
-      if (e1 != hipSuccess) return hip_fail(e1, "hipMalloc ray groups");
-      cap_groups = n;
-    }
-    RayGroupArgs ga{};
-    ga.rays = d_rays;
-    ga.n = uint32_t(n);
-    ga.n_dev = n_dev;
-    int axis = 0;
-    for (int k = 1; k < 3; ++k)
-      if (scene_inv_ext[k] < scene_inv_ext[axis]) axis = k;
-    ga.axis = axis;
-    ga.lo = scene_lo[axis];
-    ga.inv_ext = scene_inv_ext[axis];
-    ga.keys = d_group_keys;
-    ga.counts = d_group_counts;
-    ga.group_start = d_group_counts + kXcdGroups + 1;
-    ga.perm = d_group_perm;
-    if (launch_ray_groups(ga, stream) != 0) return fail(MRT_ERR_HIP, "ray grouping launch failed");
-    p.perm = d_group_perm;
-    p.group_start = ga.group_start;
-    p.group_counter = d_ctrl + 32;  // zeroed with the counters above
-  }
   if (launch_trace(p, any, stats, grid, stream) != 0) return fail(MRT_ERR_HIP, "trace kernel launch failed");
   if (!sdf_drawables.empty()) {
     SdfParams sp{};
@@ -460,7 +386,6 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     sp.n_dev = n_dev;
     sp.steps = sdf_steps;
     sp.eps = sdf_eps;
-    sp.counter = d_ctrl + 2;  // zeroed with the traversal counters above
     if (!any && cx.sdf_side_cap < n) {
       if (cx.sdf_side) hipFree(cx.sdf_side);
       if (cx.sdf_hits) hipFree(cx.sdf_hits);
@@ -475,9 +400,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     sp.side = cx.sdf_side;
     sp.hit_list = any ? nullptr : cx.sdf_hits;
     sp.hit_count = d_ctrl + 3;  // zeroed with the traversal counters above
-    sp.refill_below = 48;
     sp.prog_len = uint32_t(sdf_prog.size());
-    sp.persistent = (sdf_persistent == 1 || (sdf_persistent == 2 && !any)) ? 1u : 0u;
     sp.chain_only = sdf_chain_only ? 1u : 0u;
     if (launch_sdf(sp, any, num_cus, stream) != 0)
       return fail(MRT_ERR_HIP, "sdf kernel launch failed");
@@ -510,21 +433,13 @@ struct mrt_timer_s {
   hipEvent_t start = nullptr, stop = nullptr;
 };
 
-#define MRT_GUARD_BEGIN try {
-#define MRT_GUARD_END                                              \
-  }                                                                \
-  catch (const std::bad_alloc&) {                                  \
-    return fail(MRT_ERR_LIMIT, "out of host memory");              \
-  }                                                                \
-  catch (const std::exception& ex) {                               \
-    return fail(MRT_ERR_INTERNAL, ex.what());                      \
-  }
 
 extern "C" {
 
 const char* mrt_last_error(void) { return g_last_error.c_str(); }
 
 int mrt_device_info(char* name_out, size_t name_cap, int* num_cus) {
+  MRT_GUARD_BEGIN
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
@@ -536,37 +451,51 @@ int mrt_device_info(char* name_out, size_t name_cap, int* num_cus) {
   }
   if (num_cus) *num_cus = prop.multiProcessorCount;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_device_alloc(size_t bytes, void** out) {
+  MRT_GUARD_BEGIN
   if (!out) return fail(MRT_ERR_INVALID, "null output pointer");
   hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 1));
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipMalloc");
+  MRT_GUARD_END
 }
 int mrt_device_free(void* ptr) {
+  MRT_GUARD_BEGIN
   hipError_t e = hipFree(ptr);
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipFree");
+  MRT_GUARD_END
 }
 int mrt_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  MRT_GUARD_BEGIN
   hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipMemcpy H2D");
+  MRT_GUARD_END
 }
 int mrt_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  MRT_GUARD_BEGIN
   hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipMemcpy D2H");
+  MRT_GUARD_END
 }
 int mrt_device_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  MRT_GUARD_BEGIN
   if (!dst || !src || bytes % 16 != 0) return fail(MRT_ERR_INVALID, "copy needs non-null pointers and a multiple of 16 bytes");
   return mrt::launch_copy(src, dst, bytes, static_cast<hipStream_t>(stream)) == 0 ? MRT_OK
                                                                                   : fail(MRT_ERR_HIP, "copy kernel launch failed");
+  MRT_GUARD_END
 }
 
 int mrt_device_synchronize(void) {
+  MRT_GUARD_BEGIN
   hipError_t e = hipDeviceSynchronize();
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipDeviceSynchronize");
+  MRT_GUARD_END
 }
 
 int mrt_timer_create(mrt_timer* out) {
+  MRT_GUARD_BEGIN
   if (!out) return fail(MRT_ERR_INVALID, "null output pointer");
   auto* t = new mrt_timer_s;
   hipError_t e = hipEventCreate(&t->start);
@@ -577,29 +506,38 @@ int mrt_timer_create(mrt_timer* out) {
   }
   *out = t;
   return MRT_OK;
+  MRT_GUARD_END
 }
 int mrt_timer_destroy(mrt_timer t) {
+  MRT_GUARD_BEGIN
   if (!t) return MRT_OK;
   hipEventDestroy(t->start);
   hipEventDestroy(t->stop);
   delete t;
   return MRT_OK;
+  MRT_GUARD_END
 }
 int mrt_timer_start(mrt_timer t, void* stream) {
+  MRT_GUARD_BEGIN
   if (!t) return fail(MRT_ERR_INVALID, "null timer");
   hipError_t e = hipEventRecord(t->start, static_cast<hipStream_t>(stream));
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipEventRecord");
+  MRT_GUARD_END
 }
 int mrt_timer_stop(mrt_timer t, void* stream) {
+  MRT_GUARD_BEGIN
   if (!t) return fail(MRT_ERR_INVALID, "null timer");
   hipError_t e = hipEventRecord(t->stop, static_cast<hipStream_t>(stream));
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipEventRecord");
+  MRT_GUARD_END
 }
 int mrt_timer_elapsed_ms(mrt_timer t, float* ms) {
+  MRT_GUARD_BEGIN
   if (!t || !ms) return fail(MRT_ERR_INVALID, "null argument");
   hipError_t e = hipEventSynchronize(t->stop);
   if (e == hipSuccess) e = hipEventElapsedTime(ms, t->start, t->stop);
   return e == hipSuccess ? MRT_OK : hip_fail(e, "hipEventElapsedTime");
+  MRT_GUARD_END
 }
 
 static void box_to_c(const Box& b, mrt_aabb* o) {
@@ -675,23 +613,29 @@ int mrt_bvh_build_triangles(const uint32_t* idx, size_t n_idx, const float* pos3
 }
 
 int mrt_bvh_sizes(mrt_bvh b, size_t* n_nodes, size_t* n_indices) {
+  MRT_GUARD_BEGIN
   if (!b) return fail(MRT_ERR_INVALID, "null bvh");
   if (n_nodes) *n_nodes = b->b.nodes.size();
   if (n_indices) *n_indices = b->b.indices.size();
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_bvh_copy(mrt_bvh b, mrt_bvh_node* nodes, uint32_t* indices) {
+  MRT_GUARD_BEGIN
   if (!b) return fail(MRT_ERR_INVALID, "null bvh");
   static_assert(sizeof(mrt_bvh_node) == sizeof(BinNode), "node layouts differ");
   if (nodes) std::memcpy(nodes, b->b.nodes.data(), b->b.nodes.size() * sizeof(BinNode));
   if (indices) std::memcpy(indices, b->b.indices.data(), b->b.indices.size() * 4);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_bvh_destroy(mrt_bvh b) {
+  MRT_GUARD_BEGIN
   delete b;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_scene_create(mrt_scene* out) {
@@ -711,7 +655,6 @@ int mrt_scene_create(mrt_scene* out) {
     const long l = std::strtol(v, nullptr, 10);
     if (l >= 1 && l <= 127) s->tlas_leaf = uint32_t(l);
   }
-  if (const char* v = std::getenv("MRT_RAY_SORT")) s->ray_sort = std::strtol(v, nullptr, 10) != 0;
   if (const char* v = std::getenv("MRT_TRACE_REFILL")) {
     const long r = std::strtol(v, nullptr, 10);
     if (r >= 1 && r <= 64) s->refill_below = uint32_t(r);
@@ -721,11 +664,6 @@ int mrt_scene_create(mrt_scene* out) {
     if (g >= 1 && g <= 4096) s->trace_grab = uint32_t(g);
   }
   if (const char* v = std::getenv("MRT_BVH_BUILDER")) s->sah_builder = std::strtol(v, nullptr, 10) != 0;
-  if (const char* v = std::getenv("MRT_SDF_PERSISTENT")) {
-    const long k = std::strtol(v, nullptr, 10);
-    if (k >= 0 && k <= 2) s->sdf_persistent = int(k);
-  }
-  if (const char* v = std::getenv("MRT_TRACE_XCD")) s->xcd_groups = std::strtol(v, nullptr, 10) != 0;
   if (const char* v = std::getenv("MRT_TRACE_WAVES")) {
     const long w = std::strtol(v, nullptr, 10);
     if (w == 6 || w == 7) s->trace_waves = uint32_t(w);
@@ -736,8 +674,10 @@ int mrt_scene_create(mrt_scene* out) {
 }
 
 int mrt_scene_destroy(mrt_scene s) {
+  MRT_GUARD_BEGIN
   delete reinterpret_cast<Scene*>(s);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_scene_push_geometry(mrt_scene sh, const uint32_t* idx, size_t n_idx, const float* pos3, const float* nrm3,
@@ -1237,19 +1177,23 @@ int mrt_scene_enqueue_sdf(mrt_scene sh, uint32_t sdf, const float m4x4_colmajor[
 }
 
 int mrt_scene_sdf_drawables(mrt_scene sh, void* out, size_t n) {
+  MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s || (n && !out)) return fail(MRT_ERR_INVALID, "null argument");
   if (n < s->sdf_drawables.size()) return fail(MRT_ERR_INVALID, "output too small");
   if (!s->sdf_drawables.empty()) std::memcpy(out, s->sdf_drawables.data(), s->sdf_drawables.size() * sizeof(SdfDrawable));
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_scene_set_sdf_marching(mrt_scene sh, int steps, float eps) {
+  MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s || steps < 1 || !(eps > 0.f)) return fail(MRT_ERR_INVALID, "steps must be >= 1 and eps > 0");
   s->sdf_steps = steps;
   s->sdf_eps = eps;
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_scene_prepare(mrt_scene sh, mrt_prepare_result* out) {
@@ -1261,11 +1205,13 @@ int mrt_scene_prepare(mrt_scene sh, mrt_prepare_result* out) {
 }
 
 int mrt_scene_instance_inverses(mrt_scene sh, float* out, size_t n_inst) {
+  MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s || !out) return fail(MRT_ERR_INVALID, "null argument");
   if (n_inst < s->current.size()) return fail(MRT_ERR_INVALID, "output too small");
   for (size_t i = 0; i < s->current.size(); ++i) std::memcpy(out + 12 * i, s->current[i].inv, 48);
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 int mrt_scene_build_host(mrt_scene sh, mrt_prepare_result* out) {
@@ -1278,6 +1224,7 @@ int mrt_scene_build_host(mrt_scene sh, mrt_prepare_result* out) {
 
 int mrt_scene_copy_wide(mrt_scene sh, void* blas_nodes, void* tris, void* tlas_nodes, uint32_t* tlas_items,
                         void* instances) {
+  MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s) return fail(MRT_ERR_INVALID, "null scene");
   if (blas_nodes) std::memcpy(blas_nodes, s->blas_nodes.data(), s->blas_nodes.size() * sizeof(WideNode));
@@ -1286,6 +1233,7 @@ int mrt_scene_copy_wide(mrt_scene sh, void* blas_nodes, void* tris, void* tlas_n
   if (tlas_items) std::memcpy(tlas_items, s->tlas_items.data(), s->tlas_items.size() * 4);
   if (instances) std::memcpy(instances, s->inst_records.data(), s->inst_records.size() * sizeof(InstanceRecord));
   return MRT_OK;
+  MRT_GUARD_END
 }
 
 static int trace_host(mrt_scene sh, const mrt_ray* rays, size_t n, void* out, size_t out_elem, bool any) {

@@ -115,12 +115,7 @@ struct Scene {
   };
   static constexpr int kTraceContexts = 4;
   TraceScratch ctx[kTraceContexts];
-  bool xcd_groups = false;  // MRT_TRACE_XCD=1: rays grouped by origin slab, one group per XCD
-  uint8_t* d_group_keys = nullptr;
-  uint32_t *d_group_perm = nullptr, *d_group_counts = nullptr;
-  size_t cap_groups = 0;
   bool sdf_chain_only = false;  // every enqueued SDF program has a culling grid
-  int sdf_persistent = 0;  // MRT_SDF_PERSISTENT: 0 one thread per ray, 1 persistent lanes, 2 persistent for closest hits
   int num_cus = 0;
   uint32_t refill_below = 48;
   uint32_t trace_grab = 128;  // MRT_TRACE_GRAB: rays a traversal wave reserves per counter atomic
@@ -129,11 +124,6 @@ struct Scene {
   // Largest wide leaf: binary leaves are split at centroid medians down to this many triangles
   // (instances for the TLAS). Measured best: 2 triangles, 1 instance. MRT_BVH_LEAF and
   // MRT_TLAS_LEAF override.
-  // Ray binning before traversal (MRT_RAY_SORT=1 switches it on) and its buffers.
-  bool ray_sort = false;
-  uint32_t *d_sort_keys = nullptr, *d_sort_counts = nullptr, *d_sort_perm = nullptr;
-  size_t cap_sort = 0;
-  float scene_lo[3] = {0, 0, 0}, scene_inv_ext[3] = {1, 1, 1};
   uint32_t blas_leaf = 2;
   uint32_t tlas_leaf = 1;
   uint32_t lds_depth = 16;  // LDS stack entries per lane; MRT_TRACE_LDS_STACK overrides (a power of two, 4-64)

@@ -349,9 +349,6 @@ __device__ inline float march(const SdfDrawable& dr, const SdfInsn* tape, const 
   return tw;
 }
 
-// Rays a persistent wave reserves per atomic on the ray counter.
-constexpr uint32_t kSdfGrab = 64;
-
 // Appends `ray` to the list of SDF hits (called by the lanes that found one; one atomic per wave).
 __device__ inline void append_hit(const SdfParams& p, uint32_t ray) {
   const uint64_t lanes = __ballot(1);
@@ -412,170 +409,9 @@ __global__ __launch_bounds__(kBlock) void sdf_normal_kernel(SdfParams p) {
 }
 
 
-// Persistent sphere tracing. Every lane runs a small state machine over its ray: pick the next
-// drawable whose box the segment [tmin, best] crosses, march it one step per iteration
-// (march_sdf of raymarch.glsl:116-171: t += sign * d until |d| < eps, steps run out or t leaves
-// the box), keep the closest hit, and finish (normal + record, or the occlusion flag). A lane
-// whose ray is done takes the next ray from a global counter (one ballot and one atomic per
-// wave) once fewer than p.refill_below lanes are busy, so rays that march hundreds of steps do
-// not hold whole waves idle. Each ray's arithmetic is the sequential algorithm's, in its order.
-// A closest hit is written as (t, drawable) with the object-space hit point in p.side;
-// sdf_normal_kernel completes the record.
-template <bool ANY, bool CHAIN>
-__device__ __attribute__((always_inline)) inline void march_persistent(const SdfParams& p, const SdfInsn* prog) {
-  const int lane = __lane_id();
-  const uint32_t n = p.n_dev ? min(*p.n_dev, p.n) : p.n;
-  int ray = -1;
-  bool queue_empty = false;
-  uint32_t res_next = 0, res_end = 0;  // rays reserved by this wave (wave-uniform): [res_next, res_end)
-  // Ray and search state.
-  float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, tmin = 0, best = 0;
-  int best_j = -1;
-  uint32_t j = 0;  // next drawable to clip
-  float bx = 0, by = 0, bz = 0, blen = 1;
-  // March state of drawable j - 1 while marching.
-  bool marching = false;
-  float lx = 0, ly = 0, lz = 0, rx = 0, ry = 0, rz = 0, len = 1, t = 0, t_exit = 0;
-  int si = 0, step = 0;
-  CellCache cc;
-
-  for (;;) {
-    const bool want = ray < 0 && !queue_empty;
-    const uint64_t want_mask = __ballot(want);
-    if (want_mask) {
-      const int leader = __ffsll((unsigned long long)want_mask) - 1;
-      // Lanes of rank < have take the wave's reserve, the rest a fresh reservation of at least
-      // kSdfGrab rays (one atomic on the ray counter; atomics on one address are serialised).
-      // Within one resident grid of the end a wave reserves only what it needs.
-      const uint32_t cnt = uint32_t(__popcll(want_mask));
-      const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
-      const uint32_t have = res_end - res_next;
-      const uint32_t need = cnt > have ? cnt - have : 0u;
-      const bool near_end = res_end + gridDim.x * blockDim.x >= n;
-      const uint32_t grab = (need > kSdfGrab || near_end) ? need : kSdfGrab;
-      uint32_t fresh = 0;
-      if (need) {
-        if (lane == leader) fresh = atomicAdd(p.counter, grab);
-        fresh = __shfl(fresh, leader);
-      }
-      const uint32_t base = rank < have ? res_next : fresh - have;
-      if (need) {
-        res_next = fresh + need;
-        res_end = fresh + grab;
-      } else {
-        res_next += cnt;
-      }
-      if (want) {
-        const uint32_t r = base + rank;
-        if (r < n) {
-          const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(r);
-          const float4 r0 = rp[0], r1 = rp[1];
-          bool skip = !(r1.x != 0.f || r1.y != 0.f || r1.z != 0.f);
-          float limit = r1.w;
-          if (ANY) {
-            skip = skip || reinterpret_cast<const uint8_t*>(p.out)[r] != 0;
-          } else {
-            const uint4 h = reinterpret_cast<const uint4*>(p.out)[r];
-            if (h.z != 0xFFFFFFFFu) limit = fminf(limit, __uint_as_float(h.x));
-          }
-          if (!skip) {
-            ray = int(r);
-            ox = r0.x; oy = r0.y; oz = r0.z; tmin = r0.w;
-            dx = r1.x; dy = r1.y; dz = r1.z;
-            best = limit;
-            best_j = -1;
-            j = 0;
-            marching = false;
-          }
-        } else {
-          queue_empty = true;
-        }
-      }
-    }
-    if (__ballot(want && ray < 0 && !queue_empty)) continue;  // skipped rays: fetch again
-    if (__ballot(want && queue_empty)) queue_empty = true;
-    if (__ballot(ray >= 0) == 0) break;
-
-    for (;;) {
-      if (ray >= 0) {
-        // Start the next drawable whose box the current segment crosses.
-        while (!marching && j < p.num_drawables) {
-          const SdfDrawable dr = p.drawables[j++];
-          float t0 = tmin, t1 = best;
-          if (!clip(dr, ox, oy, oz, dx, dy, dz, t0, t1)) continue;
-          const float* m = dr.inv;
-          lx = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
-          ly = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
-          lz = m[8] * ox + m[9] * oy + m[10] * oz + m[11];
-          rx = m[0] * dx + m[1] * dy + m[2] * dz;
-          ry = m[4] * dx + m[5] * dy + m[6] * dz;
-          rz = m[8] * dx + m[9] * dy + m[10] * dz;
-          len = len3(rx, ry, rz);
-          t = fmaxf(10.0f * p.eps, t0 * len);
-          t_exit = t1;
-          si = 0;
-          step = 0;
-          cc = CellCache{};
-          marching = true;
-        }
-        bool done = !marching;  // no drawable left
-        if (marching) {
-          float res = -1.0f;
-          bool ended = !(step < p.steps && t / len < t_exit);
-          if (!ended) {
-            const SdfDrawable& dr = p.drawables[j - 1];
-            const float s = t / len;
-            uint32_t mat;
-            const float d = map_any<CHAIN>(prog + dr.tape_offset, dr.tape_len, p.grids[j - 1], p.masks, lx + rx * s,
-                                    ly + ry * s, lz + rz * s, mat, cc);
-            if (si == 0) si = d > 0.f ? 1 : (d < 0.f ? -1 : 0);
-            if (fabsf(d) < p.eps) {
-              res = t;
-              ended = true;
-            } else {
-              t = t + float(si) * d;
-              ++step;
-            }
-          }
-          if (ended) {
-            marching = false;
-            const float tw = res < 0.f ? -1.0f : res / len;
-            if (tw >= 0.f && tw < best && tw > tmin) {
-              best = tw;
-              best_j = int(j - 1);
-              bx = lx + rx * tw;
-              by = ly + ry * tw;
-              bz = lz + rz * tw;
-              blen = len;
-              if (ANY) done = true;
-            }
-            if (j >= p.num_drawables) done = true;
-          }
-        }
-        if (done) {
-          if (best_j >= 0) {
-            if (ANY) {
-              reinterpret_cast<uint8_t*>(p.out)[ray] = 1;
-            } else {
-              // Distance and drawable now; material and normal in sdf_normal_kernel.
-              reinterpret_cast<uint4*>(p.out)[ray] = make_uint4(__float_as_uint(best), 0u, 0x80000000u | uint32_t(best_j), 0u);
-              p.side[ray] = make_float4(bx, by, bz, blen);
-              if (p.hit_list) append_hit(p, uint32_t(ray));
-            }
-          }
-          ray = -1;
-        }
-      }
-      const uint64_t active = __ballot(ray >= 0);
-      if (active == 0) break;
-      if (!queue_empty && __popcll(active) < p.refill_below) break;
-    }
-  }
-}
-
 // One thread per ray (the sequential algorithm as it stands): clip every drawable, march the
-// ones the segment [tmin, best] crosses, keep the closest hit. Occupancy is higher than the
-// persistent form's; a wave waits for its longest march.
+// ones the segment [tmin, best] crosses, keep the closest hit. A wave waits for its longest march;
+// a persistent form that refilled finished lanes measured slower in the render.
 template <bool ANY, bool CHAIN>
 __device__ __attribute__((always_inline)) inline void march_per_ray(const SdfParams& p, const SdfInsn* prog) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -627,9 +463,6 @@ __device__ __attribute__((always_inline)) inline void march_per_ray(const SdfPar
 // than L1/L2 hits.
 constexpr uint32_t kProgCache = 256;
 
-#ifndef MRT_SDF_WAVES
-#define MRT_SDF_WAVES 5
-#endif
 // Waves per SIMD the marching kernels are compiled for. The march is latency bound (dependent
 // square-root and division chains, mask and program fetches), so occupancy pays even where the
 // register budget of 8 waves spills a few values to scratch: render of the 4M-triangle + SDF
@@ -640,10 +473,9 @@ constexpr uint32_t kProgCache = 256;
 #ifndef MRT_SDF_CHAIN_WAVES
 #define MRT_SDF_CHAIN_WAVES 8
 #endif
-template <bool ANY, bool PERSIST, bool CHAIN>
+template <bool ANY, bool CHAIN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
-    PERSIST ? MRT_SDF_WAVES : (CHAIN ? MRT_SDF_CHAIN_WAVES : MRT_SDF_RAY_WAVES), 8))) void sdf_kernel(
-    SdfParams p) {
+    CHAIN ? MRT_SDF_CHAIN_WAVES : MRT_SDF_RAY_WAVES, 8))) void sdf_kernel(SdfParams p) {
   __shared__ uint4 s_prog[kProgCache * (sizeof(SdfInsn) / 16)];
   static_assert(sizeof(SdfInsn) % 16 == 0, "instruction records are whole 16-byte words");
   const bool cached = p.prog_len <= kProgCache;
@@ -656,57 +488,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
   // Two code paths, so that each knows where its program lives: LDS reads for the cached copy
   // (a pointer that may point to either would make every fetch a flat load, which takes the
   // latency of the vector memory path).
-  if (cached) {
-    const SdfInsn* prog = reinterpret_cast<const SdfInsn*>(s_prog);
-    if (PERSIST)
-      march_persistent<ANY, CHAIN>(p, prog);
-    else
-      march_per_ray<ANY, CHAIN>(p, prog);
-  } else {
-    if (PERSIST)
-      march_persistent<ANY, CHAIN>(p, p.prog);
-    else
-      march_per_ray<ANY, CHAIN>(p, p.prog);
-  }
+  if (cached)
+    march_per_ray<ANY, CHAIN>(p, reinterpret_cast<const SdfInsn*>(s_prog));
+  else
+    march_per_ray<ANY, CHAIN>(p, p.prog);
 }
 
 }  // namespace sdfk
 
 namespace {
-template <bool ANY, bool PERSIST, bool CHAIN>
-void launch_march(const SdfParams& p, int num_cus, hipStream_t stream) {
-  int grid = int((p.n + sdfk::kBlock - 1) / sdfk::kBlock);
-  if (PERSIST) {
-    int blocks = 0;
-    const hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sdfk::sdf_kernel<ANY, PERSIST, CHAIN>, sdfk::kBlock, 0);
-    if (e != hipSuccess || blocks < 1) blocks = 1;
-    grid = std::min(grid, num_cus * blocks);
-  }
-  grid = std::max(grid, 1);
-  hipLaunchKernelGGL((sdfk::sdf_kernel<ANY, PERSIST, CHAIN>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
+template <bool ANY, bool CHAIN>
+void launch_march(const SdfParams& p, hipStream_t stream) {
+  const int grid = std::max(int((p.n + sdfk::kBlock - 1) / sdfk::kBlock), 1);
+  hipLaunchKernelGGL((sdfk::sdf_kernel<ANY, CHAIN>), dim3(grid), dim3(sdfk::kBlock), 0, stream, p);
 }
 template <bool ANY>
-void launch_march_any(const SdfParams& p, int num_cus, hipStream_t stream) {
-  if (p.persistent) {
-    if (p.chain_only)
-      launch_march<ANY, true, true>(p, num_cus, stream);
-    else
-      launch_march<ANY, true, false>(p, num_cus, stream);
-  } else {
-    if (p.chain_only)
-      launch_march<ANY, false, true>(p, num_cus, stream);
-    else
-      launch_march<ANY, false, false>(p, num_cus, stream);
-  }
+void launch_march_any(const SdfParams& p, hipStream_t stream) {
+  if (p.chain_only)
+    launch_march<ANY, true>(p, stream);
+  else
+    launch_march<ANY, false>(p, stream);
 }
 }  // namespace
 
 int launch_sdf(const SdfParams& p, bool any, int num_cus, hipStream_t stream) {
   if (any) {
-    launch_march_any<true>(p, num_cus, stream);
+    launch_march_any<true>(p, stream);
   } else {
-    launch_march_any<false>(p, num_cus, stream);
+    launch_march_any<false>(p, stream);
     unsigned nb = unsigned((p.n + sdfk::kBlock - 1) / sdfk::kBlock);
     if (p.hit_list) nb = std::min(nb, unsigned(std::max(num_cus, 1)) * 16u);  // grid-stride over the hits
     hipLaunchKernelGGL(sdfk::sdf_normal_kernel, dim3(nb ? nb : 1), dim3(sdfk::kBlock), 0, stream, p);

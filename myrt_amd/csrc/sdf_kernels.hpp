@@ -20,11 +20,8 @@ struct SdfParams {
   const uint32_t* n_dev;   // device-side ray count, if set
   int steps;               // sdf_marching_steps (sequential_pathtracer.hpp:135, default 400)
   float eps;               // sdf_marching_epsilon (default 1e-5)
-  uint32_t* counter;       // ray counter of the persistent kernel, zero before the launch
   float4* side;            // [n]: object-space hit point and ray-length scale of SDF hits
-  uint32_t refill_below;   // refill a wave when fewer lanes than this are busy
   uint32_t prog_len;       // instructions in prog (all drawables)
-  uint32_t persistent;     // 1: persistent lanes with refill; 0: one thread per ray
   uint32_t chain_only;     // 1: every drawable's program has a culling grid (a leaner kernel serves)
   uint32_t* hit_list;      // closest hits: rays that hit an SDF, appended by the march (n entries), or null
   uint32_t* hit_count;     // entries in hit_list, zero before the launch

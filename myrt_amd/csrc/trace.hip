@@ -9,9 +9,11 @@
 //   * the watertight ray/triangle test of Woop, Benthin and Wald (JCGT 2013) with the FP64
 //     fallback for zero edge functions, so the hit set does not depend on traversal order;
 //   * closest-hit ties broken by the lowest (instance, triangle) pair;
-//   * persistent waves that refill idle lanes from a global ray counter (ballot + one atomic per
-//     wave), and a per-lane ring stack in LDS (16 entries by default) that spills to a per-thread
-//     slice of global memory when a path needs more.
+//   * persistent waves that refill idle lanes when fewer than refill_below of them are busy. A
+//     wave takes its rays from a reserve of its own, which it refills with one atomic on the
+//     global ray counter for at least `grab` rays (128 by default, MRT_TRACE_GRAB overrides);
+//   * a per-lane ring stack in LDS (16 entries by default) that spills to a per-thread slice of
+//     global memory when a path needs more.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -23,7 +25,7 @@ namespace mrt {
 namespace dev {
 
 constexpr int kBlock = 256;
-constexpr float kRobust = 1.000001f;  // 1 + 16 ulp: covers the relative rounding of the slab test
+constexpr float kRobust = 1.000001f;  // 1 + 2^-20 (8 ulp): covers the relative rounding of the slab test
 constexpr float kSlack = 1.0f / 1048576.0f;  // 2^-20 of |b|: covers the absolute rounding of b
 constexpr float kSlackFloor = 7.8886091e-31f;  // 2^-100: covers a plane term that underflows
 constexpr float kInvMax = 18446744073709551616.0f;  // 2^64: |1/d| is clamped here for box tests
@@ -119,9 +121,8 @@ trace_kernel(TraceParams p) {
   int ray_id = -1;
   bool queue_empty = false;
   uint32_t res_next = 0, res_end = 0;  // rays reserved by this wave (wave-uniform): [res_next, res_end)
-  // Ray groups: XCD x of workgroup b (round-robin dispatch: x = b % 8) starts on group x.
-  uint32_t group = blockIdx.x & 7u, groups_left = 8;
-  const uint32_t n_rays = p.n_dev ? *p.n_dev : p.n;
+  // A device-side count (a wavefront queue) never exceeds the launch's capacity.
+  const uint32_t n_rays = p.n_dev ? min(*p.n_dev, p.n) : p.n;
 
   float wox = 0, woy = 0, woz = 0, wdx = 0, wdy = 0, wdz = 0, wix = 0, wiy = 0, wiz = 0;
   float tmin = 0, tmax = 0;
@@ -184,48 +185,34 @@ trace_kernel(TraceParams p) {
     if (want_mask) {
       const int leader = __ffsll((unsigned long long)want_mask) - 1;
       const uint32_t cnt = uint32_t(__popcll(want_mask));
-      uint32_t base = 0, limit = n_rays, first = 0;
-      if (p.group_start) {
-        // Ray groups: take from the current group; move on when it runs dry.
-        if (lane == leader) base = atomicAdd(p.group_counter + group, cnt);
-        first = p.group_start[group];
-        limit = p.group_start[group + 1] - first;
-      } else {
-        // Lanes of rank < have take the reserve; the rest take a fresh reservation of at least
-        // p.grab rays, whose remainder becomes the new reserve. Atomics on one address are
-        // serialised (about 11 ns each), and one per refill (about 30 rays) kept the ray counter
-        // near that limit.
-        const uint32_t have = res_end - res_next;
-        const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
-        uint32_t fresh = 0;
-        const uint32_t need = cnt > have ? cnt - have : 0u;
-        // Within one resident grid of the end (as far as this wave's last reservation tells), take
-        // only what is needed, so that no wave sits on reserved rays while others run dry.
-        const bool near_end = res_end + p.total_threads >= n_rays;
-        const uint32_t grab = (need > p.grab || near_end) ? need : p.grab;
-        if (need) {
-          if (lane == leader) fresh = atomicAdd(p.counter, grab);
-          fresh = __shfl(fresh, leader);
-        }
-        // base + rank is this lane's ray in both cases.
-        base = rank < have ? res_next : fresh - have;
-        if (need) {
-          res_next = fresh + need;
-          res_end = fresh + grab;
-        } else {
-          res_next += cnt;
-        }
+      // Lanes of rank < have take the reserve; the rest take a fresh reservation of at least
+      // p.grab rays, whose remainder becomes the new reserve. Atomics on one address are
+      // serialised (about 11 ns each), and one per refill (about 30 rays) kept the ray counter
+      // near that limit.
+      const uint32_t have = res_end - res_next;
+      const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
+      uint32_t fresh = 0;
+      const uint32_t need = cnt > have ? cnt - have : 0u;
+      // Within one resident grid of the end (as far as this wave's last reservation tells), take
+      // only what is needed, so that no wave sits on reserved rays while others run dry.
+      const bool near_end = res_end + p.total_threads >= n_rays;
+      const uint32_t grab = (need > p.grab || near_end) ? need : p.grab;
+      if (need) {
+        if (lane == leader) fresh = atomicAdd(p.counter, grab);
+        fresh = __shfl(fresh, leader);
       }
-      if (p.group_start) base = __shfl(base, leader);
-      if (p.group_start && base + cnt >= limit) {
-        group = (group + 1) & 7u;
-        --groups_left;
+      // base + rank is this lane's ray in both cases.
+      const uint32_t base = rank < have ? res_next : fresh - have;
+      if (need) {
+        res_next = fresh + need;
+        res_end = fresh + grab;
+      } else {
+        res_next += cnt;
       }
       if (want) {
-        const uint32_t rank = uint32_t(__popcll(want_mask & ((1ull << lane) - 1ull)));
         const uint32_t r = base + rank;
-        if (r < limit) {
-          const uint32_t rid = p.perm ? p.perm[first + r] : r;
+        if (r < n_rays) {
+          const uint32_t rid = r;
           ray_id = int(rid);
           const float4* rp = reinterpret_cast<const float4*>(p.rays) + 2 * size_t(rid);
           const float4 r0 = rp[0], r1 = rp[1];
@@ -251,15 +238,8 @@ trace_kernel(TraceParams p) {
         }
       }
     }
-    if (p.group_start) {
-      if (groups_left == 0) queue_empty = true;
-    } else if (__ballot(want && ray_id < 0)) {
-      queue_empty = true;
-    }
-    if (__ballot(ray_id >= 0) == 0) {
-      if (queue_empty) break;
-      continue;  // this group ran dry; the next refill takes from the next one
-    }
+    if (__ballot(want && ray_id < 0)) queue_empty = true;
+    if (__ballot(ray_id >= 0) == 0) break;  // every lane idle: the queue is drained
 
     // ---- traverse until too many lanes are idle ----
     // One step on an entry that is not a triangle leaf: an inner node (test the four children,
@@ -314,8 +294,11 @@ trace_kernel(TraceParams p) {
         for (int c = 0; c < 4; ++c) {
           const float tn = fmaxf(fmaxf(fmaf(byte_of(wnx, c), ax, bnx), fmaf(byte_of(wny, c), ay, bny)),
                                  fmaxf(fmaf(byte_of(wnz, c), az, bnz), tmin));
-          const float tf = fminf(fminf(fmaf(byte_of(wfx, c), ax, bfx), fmaf(byte_of(wfy, c), ay, bfy)),
-                                 fminf(fmaf(byte_of(wfz, c), az, bfz), t_best)) * kRobust;
+          const float tf0 = fminf(fminf(fmaf(byte_of(wfx, c), ax, bfx), fmaf(byte_of(wfy, c), ay, bfy)),
+                                  fminf(fmaf(byte_of(wfz, c), az, bfz), t_best));
+          // Widened by 2^-20 of its magnitude, so also a far plane behind the origin (a negative
+          // t_min) moves outwards; for tf0 >= 0 this equals tf0 * kRobust bit for bit.
+          const float tf = fmaf(fabsf(tf0), kRobust - 1.0f, tf0);
           const bool hit = (uint32_t(c) < nch) && tn <= tf;
           const uint32_t cnt = (q3.x >> (8 * c)) & 0xffu;
           const uint32_t off = (c < 2 ? (q3.y >> (16 * c)) : (q3.z >> (16 * (c - 2)))) & 0xffffu;

@@ -27,12 +27,8 @@ struct TraceParams {
   uint32_t refill_below;           // refill a wave when fewer lanes than this are busy
   uint32_t grab;                   // rays a wave reserves per atomic on the ray counter (>= 1)
   uint32_t lds_depth;              // LDS stack entries per lane (a power of two)
-  const uint32_t* perm;            // if set, queue position -> ray index (binned order)
   uint32_t waves;                  // waves per SIMD the kernel is compiled for: 6 or 7
   uint32_t single_instance;        // the frame's only instance record, or 0xFFFFFFFF
-  const uint32_t* group_start;     // if set, perm holds 8 ray groups; workgroup b starts on group
-                                   // b % 8 (its XCD) and takes the next groups when it runs dry
-  uint32_t* group_counter;         // [8] per-group ray counters, zero before the launch
   unsigned long long* stats;       // rays, node visits, triangle tests, instance visits
 };
 

@@ -328,6 +328,7 @@ class OracleRender:
         out = np.zeros(width * height * 3, np.float64)
         rc = lib().orc_render(ctypes.byref(self.c), self.scene.accel(), _p(v), _p(p), width, height, spp,
                               sample_base, max_depth, _p(out), threads)
-        if rc != 0:
+        if rc < 0:
             raise RuntimeError("oracle render failed")
+        self.last_nonfinite = rc  # non-finite values rejected by this render
         return out.reshape(height, width, 3) / spp

@@ -151,7 +151,10 @@ typedef struct orc_pt_scene {
  * homogeneous divide), Fresnel-weighted GGX / Lambert / smooth-dielectric lobes, quad-light NEE
  * with power-heuristic MIS, Russian roulette from bounce 3, sky environment; the sampler is the
  * product's counter hash. out_rgb receives per-pixel SUMS over spp samples (w*h*3 doubles).
- * Traces through the oracle BVH `a` built over s->geo. Returns 0 on success. */
+ * Traces through the oracle BVH `a` built over s->geo. Non-finite values are rejected as the
+ * product rejects them (a non-finite geometric normal makes the hit a miss, a non-finite shading
+ * normal falls back to the geometric one, non-finite contributions are dropped and a non-finite
+ * throughput ends the path). Returns the number of rejections (>= 0), or -1 on failure. */
 int orc_render(const orc_pt_scene* s, orc_accel a, const float view_colmajor[16], const float proj_colmajor[16],
                int width, int height, int spp, int sample_base, int max_depth, double* out_rgb, int threads);
 

@@ -26,6 +26,7 @@ inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline D3 cross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 inline D3 normalize(D3 a) { return a * (1.0 / std::sqrt(dot(a, a))); }
 inline D3 ld(const float* p) { return {p[0], p[1], p[2]}; }
+inline bool finite3(D3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
 const double kPi = 3.14159265358979323846;
 
 // lowbias32 chained over (pixel, sample, dimension): the product's sampler, bit for bit.
@@ -199,7 +200,7 @@ struct Ctx {
   int w, h, max_depth;
 };
 
-void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
+void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out, uint64_t* nonfinite) {
   const orc_pt_scene* S = C.s;
   const orc_scene* G = S->geo;
   const uint32_t x = pixel % uint32_t(C.w), y = pixel / uint32_t(C.w);
@@ -259,51 +260,19 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
     if (S->n_sdf)
       orc_sdf_trace(static_cast<const orc_sdf_node*>(S->sdf_tape), static_cast<const orc_sdf_drawable*>(S->sdf_drawables),
                     S->n_sdf, &r, 1, 0, S->sdf_steps, S->sdf_eps, &hit, sdf_n, 1);
-    const bool found = hit.inst != 0xFFFFFFFFu;
+    bool found = hit.inst != 0xFFFFFFFFu;
     const double t_hit = found ? double(hit.t) : INFINITY;
     // The product works on the binary32 ray it stored; continue from the same values.
     ro = ld(r.o);
     rd = ld(r.d);
-    int light_hit = -1;
-    double t_light = t_hit, cos_light = 0;
-    for (uint32_t l = 0; l < S->n_lights; ++l) {
-      const float* q = S->lights + 12 * l;
-      const D3 c = ld(q), u = ld(q + 3), v = ld(q + 6);
-      const D3 nl = cross(u, v);
-      const double area = std::sqrt(dot(nl, nl));
-      const D3 n1 = nl * (1.0 / area);
-      const double den = dot(n1, rd);
-      if (den >= 0) continue;
-      const double t = dot(n1, c - ro) / den;
-      if (!(t > 1e-6 && t < t_light)) continue;
-      const D3 pq = ro + rd * t - c;
-      const double a = dot(pq, u) / dot(u, u), b = dot(pq, v) / dot(v, v);
-      if (a < 0 || a > 1 || b < 0 || b > 1) continue;
-      light_hit = int(l);
-      t_light = t;
-      cos_light = -den;
-    }
-    if (light_hit >= 0) {
-      const float* q = S->lights + 12 * light_hit;
-      double w = 1.0;
-      if (!delta) {
-        const D3 nl = cross(ld(q + 3), ld(q + 6));
-        const double area = std::sqrt(dot(nl, nl));
-        const double pdf_l = t_light * t_light / (cos_light * area * double(S->n_lights));
-        w = last_pdf * last_pdf / (last_pdf * last_pdf + pdf_l * pdf_l);
-      }
-      sum = sum + thr * ld(q + 9) * w;
-      break;
-    }
-    if (!found) {
-      sum = sum + thr * environment(S, rd);
-      break;
-    }
-    const uint32_t inst = hit.inst, prim = hit.prim;
-    D3 ng, ns;
-    uint32_t mi;
+    // Non-finite geometric normal: the hit is a miss (pt_trace.comp:186-189); non-finite shading
+    // normal: the geometric one. The product's rules, in binary64.
+    D3 ng{0, 0, 1}, ns = ng;
+    uint32_t mi = 0;
     double hu = 0, hv = 0;
     bool has_uv = false;
+    if (found) {
+    const uint32_t inst = hit.inst, prim = hit.prim;
     if (inst & 0x80000000u) {
       ng = normalize(D3{sdf_n[0], sdf_n[1], sdf_n[2]});
       ns = ng;
@@ -336,6 +305,51 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
       ng = xf(ngo);
       ns = xf(nso);
       mi = S->inst_material[inst];
+    }
+    if (!finite3(ng)) {
+      found = false;
+      ++*nonfinite;
+    } else if (!finite3(ns)) {
+      ns = ng;
+      ++*nonfinite;
+    }
+    }
+    int light_hit = -1;
+    double t_light = t_hit, cos_light = 0;
+    for (uint32_t l = 0; l < S->n_lights; ++l) {
+      const float* q = S->lights + 12 * l;
+      const D3 c = ld(q), u = ld(q + 3), v = ld(q + 6);
+      const D3 nl = cross(u, v);
+      const double area = std::sqrt(dot(nl, nl));
+      const D3 n1 = nl * (1.0 / area);
+      const double den = dot(n1, rd);
+      if (den >= 0) continue;
+      const double t = dot(n1, c - ro) / den;
+      if (!(t > 1e-6 && t < t_light)) continue;
+      const D3 pq = ro + rd * t - c;
+      const double a = dot(pq, u) / dot(u, u), b = dot(pq, v) / dot(v, v);
+      if (a < 0 || a > 1 || b < 0 || b > 1) continue;
+      light_hit = int(l);
+      t_light = t;
+      cos_light = -den;
+    }
+    if (light_hit >= 0) {
+      const float* q = S->lights + 12 * light_hit;
+      double w = 1.0;
+      if (!delta) {
+        const D3 nl = cross(ld(q + 3), ld(q + 6));
+        const double area = std::sqrt(dot(nl, nl));
+        const double pdf_l = t_light * t_light / (cos_light * area * double(S->n_lights));
+        w = last_pdf * last_pdf / (last_pdf * last_pdf + pdf_l * pdf_l);
+      }
+      const D3 c = thr * ld(q + 9) * w;
+      if (finite3(c)) sum = sum + c; else ++*nonfinite;
+      break;
+    }
+    if (!found) {
+      const D3 c = thr * environment(S, rd);
+      if (finite3(c)) sum = sum + c; else ++*nonfinite;
+      break;
     }
     const D3 p = ro + rd * t_hit;
     const D3 wo = -rd;
@@ -389,7 +403,9 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
         const double pdf_l = d2 / (cl * area * double(S->n_lights));
         const double w = pdf_l * pdf_l / (pdf_l * pdf_l + pdf_b * pdf_b);
         const D3 c = thr * f * ld(q + 9) * (dot(ns, wi) * w / pdf_l);
-        if (c.x > 0 || c.y > 0 || c.z > 0) {
+        if (!finite3(c)) {
+          ++*nonfinite;
+        } else if (c.x > 0 || c.y > 0 || c.z > 0) {
           const D3 so = p + ng * eps;
           orc_ray sr;
           for (int k = 0; k < 3; ++k) {
@@ -450,6 +466,10 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out) {
         delta = false;
       }
     }
+    if (ok && !finite3(thr)) {
+      ok = false;
+      ++*nonfinite;
+    }
     if (ok && bounce >= 3) {
       const double q = std::min(0.95, std::max(std::max(thr.x, thr.y), thr.z));
       if (rnd(pixel, sample, dim_of(bounce, 3)) >= q)
@@ -493,12 +513,16 @@ extern "C" int orc_render(const orc_pt_scene* s, orc_accel a, const float view[1
   std::fill(out_rgb, out_rgb + 3 * npix, 0.0);
   if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
   std::vector<std::thread> pool;
+  std::vector<uint64_t> bad(size_t(threads), 0);
   for (int t = 0; t < threads; ++t) {
     pool.emplace_back([&, t] {
       for (size_t px = size_t(t); px < npix; px += size_t(threads))
-        for (int sidx = 0; sidx < spp; ++sidx) trace_path(C, uint32_t(px), uint32_t(sample_base + sidx), out_rgb + 3 * px);
+        for (int sidx = 0; sidx < spp; ++sidx)
+          trace_path(C, uint32_t(px), uint32_t(sample_base + sidx), out_rgb + 3 * px, &bad[size_t(t)]);
     });
   }
   for (auto& th : pool) th.join();
-  return 0;
+  uint64_t total = 0;
+  for (uint64_t b : bad) total += b;
+  return int(std::min<uint64_t>(total, 0x7fffffff));
 }

@@ -246,25 +246,25 @@ def test_shadow_stream_gives_the_same_image(ensure_built, monkeypatch):
     assert np.all(diff <= 1e-5 * np.maximum(1.0, np.abs(imgs[0])))
 
 
-def test_two_sample_pipelines_give_the_same_image(ensure_built, monkeypatch):
-    """The samples of a render split over two pipelines (own queues, streams and trace contexts,
-    shared accumulators) give the one-pipeline image up to fp32 summation order, over several
-    progressive calls with odd sample counts."""
+def test_batched_samples_give_the_same_image(ensure_built, monkeypatch):
+    """A render whose samples run in several batches (MRT_BATCH_PATHS below one call's paths)
+    gives the one-batch image up to fp32 summation order, over several progressive calls with
+    odd sample counts."""
     sc = scenes.sphere_field_scene(30, 2, seed=7, glass_every=5, metal_every=3)
     tape = scenes.sdf_cluster(12, seed=4, k=0.1, box=0.8, n_materials=2)
     W, H, depth = 80, 48, 6
     view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
     imgs = []
-    for pipelines in ("1", "2"):
-        monkeypatch.setenv("MRT_PIPELINES", pipelines)
-        monkeypatch.setenv("MRT_BATCH_PATHS", str(W * H * 3))  # several batches per pipeline
+    for batch in (None, str(W * H * 3)):
+        if batch:
+            monkeypatch.setenv("MRT_BATCH_PATHS", batch)  # three samples per batch
         s = gpu_scene(sc)
         s.enqueue_sdf(s.push_sdf(tape), scenes.translate_scale(tuple(sc["target"]), 1.0))
         r = m.Renderer()
         r.set_camera(view, proj)
         r.set_num_bounces(depth)
         for spp in (5, 1, 10):
-            say("render %dx%d %d spp, %s pipeline(s)" % (W, H, spp, pipelines))
+            say("render %dx%d %d spp, batch %s" % (W, H, spp, batch))
             r.render(s, W, H, spp)
         assert r.sample_count() == 16
         imgs.append(r.image()[..., :3].astype(np.float64))

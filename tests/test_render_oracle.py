@@ -139,3 +139,25 @@ def test_texture_and_cubemap_closed_forms(ensure_built):
     row = (np.arange(32) + 0.5) / 32
     expect = 2.0 * np.array([1.0, 0.3, 0.2]) * (0.5 + np.interp(t, np.arange(32), row))
     assert np.allclose(got, expect, rtol=1e-3)
+
+
+def test_nonfinite_rules(ensure_built):
+    """The oracle's rejection rules: NaN vertex normals fall back to the geometric normal (the
+    image equals the one rendered without vertex normals, where every shading normal is the
+    geometric one), and each fallback is counted; a scene without non-finite input counts none."""
+    gi, gp, gn = scenes.ground_quad(size=20.0, y=0.0)
+    mat = np.zeros(1, orc.MATERIAL_DTYPE)
+    mat[0] = (scenes._rgba8((0.5, 0.5, 0.5)), 1.5, 0.4, 0.0, 0.0, 0, (0, 0))
+    light = np.zeros(1, orc.LIGHT_DTYPE)
+    light[0] = ((-0.5, 2.0, -0.5), (1, 0, 0), (0, 0, 1), (10.0,) * 3)
+    view, proj = scenes.camera_matrices((0, 5, 0.001), (0, 0, 0), 1.0, 1.0)
+    inst = [(0, np.eye(4, dtype=np.float32), 0)]
+    clean = orc.OracleRender([(gi, gp, None)], inst, mat, light, 0.5)
+    a = clean.render(view, proj, 8, 8, 16, 3)
+    assert clean.last_nonfinite == 0
+    nan_n = np.full_like(np.asarray(gn, np.float32), np.nan)
+    poisoned = orc.OracleRender([(gi, gp, nan_n)], inst, mat, light, 0.5)
+    b = poisoned.render(view, proj, 8, 8, 16, 3)
+    assert np.isfinite(b).all()
+    assert poisoned.last_nonfinite >= 8 * 8 * 16  # every primary hit falls back
+    assert np.array_equal(a, b)

@@ -152,22 +152,22 @@ def test_heightfield_full_size(ensure_built):
         assert np.array_equal(occ, ref_occ), name + ": shadow occlusion differs"
 
 
-@pytest.mark.parametrize("knob", ["MRT_RAY_SORT", "MRT_TRACE_XCD"])
-def test_ray_binning_gives_identical_hits(ensure_built, monkeypatch, knob):
-    """Binning rays (by octant and origin cell, or into per-XCD origin slabs) before traversal
-    changes the order rays are taken from the queue, never a result: hits equal the unbinned ones
-    bit for bit."""
+@pytest.mark.parametrize("grab", ["1", "4096"])
+def test_ray_reserve_size_gives_identical_hits(ensure_built, monkeypatch, grab):
+    """The number of rays a traversal wave reserves per atomic on the ray counter (MRT_TRACE_GRAB,
+    128 by default) changes which wave takes which ray, never a result: hits and occlusion flags
+    equal the default's bit for bit."""
     (idx, pts, nrm, uv), coh, inc = scenes.traversal_workload(n_coherent=1 << 16, n_incoherent=1 << 18)
     plain = m.Scene()
     plain.enqueue(plain.push_geometry(idx, pts, nrm, uv))
     plain.prepare()
-    monkeypatch.setenv(knob, "1")
-    binned = m.Scene()
-    binned.enqueue(binned.push_geometry(idx, pts, nrm, uv))
-    binned.prepare()
+    monkeypatch.setenv("MRT_TRACE_GRAB", grab)
+    other = m.Scene()
+    other.enqueue(other.push_geometry(idx, pts, nrm, uv))
+    other.prepare()
     for name, rays in (("coherent", coh), ("incoherent", inc)):
-        say("binning: %s %d rays" % (name, rays.size))
-        a, b = plain.trace_closest(rays), binned.trace_closest(rays)
+        say("reserve %s: %s %d rays" % (grab, name, rays.size))
+        a, b = plain.trace_closest(rays), other.trace_closest(rays)
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
         sh = scenes.shadow_rays(rays, a)
-        assert np.array_equal(plain.trace_any(sh), binned.trace_any(sh)), name + " shadow"
+        assert np.array_equal(plain.trace_any(sh), other.trace_any(sh)), name + " shadow"
