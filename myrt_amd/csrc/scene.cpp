@@ -31,6 +31,7 @@
 namespace mrt {
 
 int launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
+int launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t stream);
 thread_local std::string g_last_error;
 
 int fail(int code, const std::string& msg) {
@@ -293,6 +294,8 @@ Scene::~Scene() {
   for (TraceScratch& c : ctx)
     for (void* q : {(void*)c.ctrl, (void*)c.spill, (void*)c.sdf_side, (void*)c.sdf_hits})
       if (q) hipFree(q);
+  for (void* q : {d_host_rays, d_host_out})
+    if (q) hipFree(q);
   if (d_tlas) hipFree(d_tlas);
   if (d_tlas_items) hipFree(d_tlas_items);
   if (d_inst) hipFree(d_inst);
@@ -318,6 +321,18 @@ PtScene Scene::shading_view() const {
   return v;
 }
 
+uint32_t Scene::spill_depth() const { return max_stack > lds_depth ? max_stack - lds_depth : 1; }
+
+int Scene::full_grid(bool any, bool stats, size_t n) const {
+  // Seven waves per SIMD pay off where rays change instances often (the per-iteration work is
+  // larger and more divergent); a single mesh runs best at six. MRT_TRACE_WAVES overrides.
+  const uint32_t waves = trace_waves ? trace_waves : (current.size() > 1 ? 7u : 6u);
+  const int per_cu = trace_blocks_per_cu(any, stats, waves, lds_depth);
+  const size_t max_grid = size_t(num_cus) * size_t(per_cu);
+  const size_t want = (n + kTraceBlock - 1) / kTraceBlock;
+  return int(std::max<size_t>(1, std::min(max_grid, want)));
+}
+
 int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream,
                  const uint32_t* n_dev, int context) {
   if (!prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
@@ -329,17 +344,7 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&cx.ctrl), 256);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc trace counters");
   }
-  uint32_t* const d_ctrl = cx.ctrl;
-  // Seven waves per SIMD pay off where rays change instances often (the per-iteration work is
-  // larger and more divergent); a single mesh runs best at six. MRT_TRACE_WAVES overrides.
-  const uint32_t waves = trace_waves ? trace_waves : (current.size() > 1 ? 7u : 6u);
-  const int per_cu = trace_blocks_per_cu(any, stats, waves, lds_depth);
-  const size_t max_grid = size_t(num_cus) * size_t(per_cu);
-  const size_t want = (n + kTraceBlock - 1) / kTraceBlock;
-  const int grid = int(std::max<size_t>(1, std::min(max_grid, want)));
-  const uint32_t total_threads = uint32_t(grid) * kTraceBlock;
-  const uint32_t spill_depth = max_stack > lds_depth ? max_stack - lds_depth : 1;
-  const size_t spill_entries = size_t(spill_depth) * total_threads;
+  const size_t spill_entries = size_t(spill_depth()) * size_t(full_grid(any, stats, n)) * kTraceBlock;
   if (spill_entries > cx.spill_cap) {
     if (cx.spill) hipFree(cx.spill);
     cx.spill = nullptr;
@@ -348,8 +353,35 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     if (e != hipSuccess) return hip_fail(e, "hipMalloc spill");
     cx.spill_cap = spill_entries;
   }
-  hipError_t e = hipMemsetAsync(d_ctrl, 0, 256, stream);
-  if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  if (!sdf_drawables.empty() && !any && cx.sdf_side_cap < n) {
+    if (cx.sdf_side) hipFree(cx.sdf_side);
+    if (cx.sdf_hits) hipFree(cx.sdf_hits);
+    cx.sdf_side = nullptr;
+    cx.sdf_hits = nullptr;
+    cx.sdf_side_cap = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&cx.sdf_side), n * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&cx.sdf_hits), n * sizeof(uint32_t));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc sdf hit points");
+    cx.sdf_side_cap = n;
+  }
+  const TraceArena a{cx.ctrl, cx.spill, cx.spill_cap, cx.sdf_side, cx.sdf_hits, cx.sdf_side_cap};
+  return trace_in(a, d_rays, n, d_out, any, stats, stream, n_dev);
+}
+
+int Scene::trace_in(const TraceArena& a, const void* d_rays, size_t n, void* d_out, bool any, bool stats,
+                    hipStream_t stream, const uint32_t* n_dev) {
+  if (!prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
+  if (n == 0) return MRT_OK;
+  if (n >= 0x7fffffffu) return fail(MRT_ERR_LIMIT, "at most 2^31-1 rays per call");
+  const bool sdf = !sdf_drawables.empty();
+  if (sdf && !any && a.sdf_cap < n) return fail(MRT_ERR_LIMIT, "SDF hit buffers smaller than the ray count");
+  const uint32_t depth = spill_depth();
+  // The persistent grid is cut to the spill stack the arena holds; each thread needs `depth` entries.
+  const size_t fit = a.spill_entries / (size_t(depth) * kTraceBlock);
+  if (fit == 0) return fail(MRT_ERR_LIMIT, "spill stack too small for one traversal workgroup");
+  const int grid = int(std::min<size_t>(size_t(full_grid(any, stats, n)), fit));
+  const uint32_t total_threads = uint32_t(grid) * kTraceBlock;
+  if (launch_fill_u32(a.ctrl, 64, 0u, stream) != 0) return fail(MRT_ERR_HIP, "clearing trace counters");
   TraceParams p{};
   p.tlas_nodes = d_tlas;
   p.tlas_items = d_tlas_items;
@@ -361,19 +393,19 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
   p.n = uint32_t(n);
   p.n_dev = n_dev;
   p.has_tlas = tlas_nodes.empty() ? 0u : 1u;
-  p.counter = d_ctrl;
-  p.flags = d_ctrl + 1;
-  p.stats = reinterpret_cast<unsigned long long*>(d_ctrl + 8);
-  p.spill = cx.spill;
-  p.spill_depth = spill_depth;
+  p.counter = a.ctrl;
+  p.flags = a.ctrl + 1;
+  p.stats = reinterpret_cast<unsigned long long*>(a.ctrl + 8);
+  p.spill = a.spill;
+  p.spill_depth = depth;
   p.total_threads = total_threads;
   p.refill_below = refill_below;
   p.grab = trace_grab;
   p.lds_depth = lds_depth;
-  p.waves = waves;
+  p.waves = trace_waves ? trace_waves : (current.size() > 1 ? 7u : 6u);
   p.single_instance = current.size() == 1 ? 0u : 0xFFFFFFFFu;
   if (launch_trace(p, any, stats, grid, stream) != 0) return fail(MRT_ERR_HIP, "trace kernel launch failed");
-  if (!sdf_drawables.empty()) {
+  if (sdf) {
     SdfParams sp{};
     sp.drawables = d_sdf_drawables;
     sp.prog = d_sdf_prog;
@@ -386,20 +418,9 @@ int Scene::trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats
     sp.n_dev = n_dev;
     sp.steps = sdf_steps;
     sp.eps = sdf_eps;
-    if (!any && cx.sdf_side_cap < n) {
-      if (cx.sdf_side) hipFree(cx.sdf_side);
-      if (cx.sdf_hits) hipFree(cx.sdf_hits);
-      cx.sdf_side = nullptr;
-      cx.sdf_hits = nullptr;
-      cx.sdf_side_cap = 0;
-      hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&cx.sdf_side), n * sizeof(float4));
-      if (e2 == hipSuccess) e2 = hipMalloc(reinterpret_cast<void**>(&cx.sdf_hits), n * sizeof(uint32_t));
-      if (e2 != hipSuccess) return hip_fail(e2, "hipMalloc sdf hit points");
-      cx.sdf_side_cap = n;
-    }
-    sp.side = cx.sdf_side;
-    sp.hit_list = any ? nullptr : cx.sdf_hits;
-    sp.hit_count = d_ctrl + 3;  // zeroed with the traversal counters above
+    sp.side = any ? nullptr : a.sdf_side;
+    sp.hit_list = any ? nullptr : a.sdf_hits;
+    sp.hit_count = a.ctrl + 3;  // zeroed with the traversal counters above
     sp.prog_len = uint32_t(sdf_prog.size());
     sp.chain_only = sdf_chain_only ? 1u : 0u;
     if (launch_sdf(sp, any, num_cus, stream) != 0)
@@ -1236,26 +1257,33 @@ int mrt_scene_copy_wide(mrt_scene sh, void* blas_nodes, void* tris, void* tlas_n
   MRT_GUARD_END
 }
 
+// Host-buffer queries stage through the scene's grow-only device buffers (no allocation once
+// they are large enough, which matters for per-click picks).
 static int trace_host(mrt_scene sh, const mrt_ray* rays, size_t n, void* out, size_t out_elem, bool any) {
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s || (n && (!rays || !out))) return fail(MRT_ERR_INVALID, "null argument");
   if (!s->prepared) return fail(MRT_ERR_NOT_PREPARED, "trace before mrt_scene_prepare");
   if (n == 0) return MRT_OK;
-  void *d_r = nullptr, *d_o = nullptr;
-  hipError_t e = hipMalloc(&d_r, n * sizeof(mrt_ray));
-  if (e == hipSuccess) e = hipMalloc(&d_o, n * out_elem);
-  if (e == hipSuccess) e = hipMemcpy(d_r, rays, n * sizeof(mrt_ray), hipMemcpyHostToDevice);
-  int rc = MRT_OK;
-  if (e != hipSuccess) rc = hip_fail(e, "staging rays");
-  if (rc == MRT_OK) rc = s->trace(d_r, n, d_o, any, false, nullptr);
+  const auto grow = [](void*& p, size_t& cap, size_t bytes) -> hipError_t {
+    if (p && cap >= bytes) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  };
+  hipError_t e = grow(s->d_host_rays, s->cap_host_rays, n * sizeof(mrt_ray));
+  if (e == hipSuccess) e = grow(s->d_host_out, s->cap_host_out, n * sizeof(mrt_hit));
+  if (e == hipSuccess) e = hipMemcpy(s->d_host_rays, rays, n * sizeof(mrt_ray), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "staging rays");
+  int rc = s->trace(s->d_host_rays, n, s->d_host_out, any, false, nullptr);
   uint32_t flags = 0;
   if (rc == MRT_OK) rc = s->read_ctrl(&flags, nullptr, nullptr);
   if (rc == MRT_OK) {
-    e = hipMemcpy(out, d_o, n * out_elem, hipMemcpyDeviceToHost);
+    e = hipMemcpy(out, s->d_host_out, n * out_elem, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = hip_fail(e, "reading results");
   }
-  if (d_r) hipFree(d_r);
-  if (d_o) hipFree(d_o);
   if (rc == MRT_OK && (flags & 1u)) rc = fail(MRT_ERR_INTERNAL, "traversal stack overflow");
   return rc;
 }

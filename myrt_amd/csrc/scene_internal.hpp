@@ -115,6 +115,10 @@ struct Scene {
   };
   static constexpr int kTraceContexts = 4;
   TraceScratch ctx[kTraceContexts];
+  // Device staging of the host-buffer queries (mrt_trace_closest/any, mrt_scene_pick); they
+  // grow to the largest call and are reused.
+  void *d_host_rays = nullptr, *d_host_out = nullptr;
+  size_t cap_host_rays = 0, cap_host_out = 0;
   bool sdf_chain_only = false;  // every enqueued SDF program has a culling grid
   int num_cus = 0;
   uint32_t refill_below = 48;
@@ -133,6 +137,24 @@ struct Scene {
   int upload_all();
   int trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream,
             const uint32_t* n_dev = nullptr, int context = 0);
+  // The device scratch of one trace call: 256 B of counters (zeroed by the call), the traversal
+  // spill stack (spill_entries 4-byte entries; the persistent grid is cut down to what fits) and,
+  // for closest hits with SDF drawables, per-ray SDF hit points and the SDF hit list (sdf_cap rays).
+  struct TraceArena {
+    uint32_t* ctrl;
+    uint32_t* spill;
+    size_t spill_entries;
+    float4* sdf_side;
+    uint32_t* sdf_hits;
+    size_t sdf_cap;
+  };
+  // Entries of spill stack per traversal thread, and the full persistent grid of a launch.
+  uint32_t spill_depth() const;
+  int full_grid(bool any, bool stats, size_t n) const;
+  // The launches of a trace into caller-provided scratch; allocates nothing. Counters are cleared
+  // with a fill kernel, so the launches may be captured into a HIP graph.
+  int trace_in(const TraceArena& a, const void* d_rays, size_t n, void* d_out, bool any, bool stats,
+               hipStream_t stream, const uint32_t* n_dev);
   PtScene shading_view() const;
   int read_ctrl(uint32_t* flags, unsigned long long* stats, hipStream_t stream);
 };
