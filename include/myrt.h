@@ -306,6 +306,15 @@ int mrt_trace_closest(mrt_scene s, const mrt_ray* rays, size_t n, mrt_hit* hits)
 int mrt_trace_any(mrt_scene s, const mrt_ray* rays, size_t n, uint8_t* occluded);      /* host buffers */
 int mrt_trace_closest_device(mrt_scene s, const mrt_ray* d_rays, size_t n, mrt_hit* d_hits, void* stream);
 int mrt_trace_any_device(mrt_scene s, const mrt_ray* d_rays, size_t n, uint8_t* d_occluded, void* stream);
+/* mrt_trace_closest_device without any allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same kernels and results; their scratch (counters, traversal
+ * spill stack, SDF hit points) lives in one 32 MiB device array per GPU that the library holds
+ * until the process ends, and the persistent grid shrinks to the spill stack that fits (deep
+ * trees then run on fewer waves). The scene must be prepared. Limits: n < 2^31, and n <= 2^20
+ * when the scene has SDF drawables (MRT_ERR_LIMIT beyond). Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (two streams or threads) race; serialise them. */
+int mrt_trace_closest_device_noalloc(mrt_scene s, const mrt_ray* d_rays, size_t n, mrt_hit* d_hits, void* stream);
 /* Same traversal with per-launch fetch counters (slower; used to price bytes per ray). */
 int mrt_trace_stats_device(mrt_scene s, const mrt_ray* d_rays, size_t n, int any_hit, mrt_trace_stats* out,
                            void* stream);

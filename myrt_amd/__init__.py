@@ -104,6 +104,7 @@ SIGNATURES = {
     "mrt_trace_any": ([_P, _P, _SZ, _P], ctypes.c_int),
     "mrt_trace_closest_device": ([_P, _P, _SZ, _P, _P], ctypes.c_int),
     "mrt_trace_any_device": ([_P, _P, _SZ, _P, _P], ctypes.c_int),
+    "mrt_trace_closest_device_noalloc": ([_P, _P, _SZ, _P, _P], ctypes.c_int),
     "mrt_trace_stats_device": ([_P, _P, _SZ, ctypes.c_int, ctypes.POINTER(_TraceStats), _P], ctypes.c_int),
     "mrt_scene_pick": ([_P, _P, _P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "mrt_scene_push_material": ([_P, _P, _U32P], ctypes.c_int),
@@ -407,6 +408,9 @@ class Scene:
 
     def trace_closest_device(self, d_rays, n, d_hits, stream=None):
         _check(lib().mrt_trace_closest_device(self.h, d_rays.ptr, n, d_hits.ptr, stream))
+
+    def trace_closest_device_noalloc(self, d_rays, n, d_hits, stream=None):
+        _check(lib().mrt_trace_closest_device_noalloc(self.h, d_rays.ptr, n, d_hits.ptr, stream))
 
     def trace_any_device(self, d_rays, n, d_occ, stream=None):
         _check(lib().mrt_trace_any_device(self.h, d_rays.ptr, n, d_occ.ptr, stream))
