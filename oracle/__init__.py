@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_PATH = os.environ.get("MYRT_ORACLE_LIBRARY") or os.path.join(HERE, "liboracle.so")
 
 AABB_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("centroid", "<f4", 3), ("has_centroid", "<u4")])
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("type_and_parent", "<u4"), ("max", "<f4", 3), ("first", "<u4"),
@@ -75,6 +75,8 @@ def lib():
                                _P, ctypes.c_int], None),
             "orc_render": ([ctypes.POINTER(_PtScene), _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                             ctypes.c_int, ctypes.c_int, _P, ctypes.c_int], ctypes.c_int),
+            "orc_bsdf_eval": ([_P, _P, _P, _P, _SZ, _P, _P], ctypes.c_int),
+            "orc_bsdf_sample": ([_P, _P, _P, ctypes.c_int, _P, _SZ, _P, _P, _P, _P, _P], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -332,3 +334,33 @@ class OracleRender:
             raise RuntimeError("oracle render failed")
         self.last_nonfinite = rc  # non-finite values rejected by this render
         return out.reshape(height, width, 3) / spp
+
+
+def bsdf_eval(material, n, wo, wi):
+    """The renderer's material model: BSDF value f (N, 3) and sampling density pdf (N,) of the
+    non-delta lobes for directions wi (N, 3), shading normal n on wo's side."""
+    mat = np.ascontiguousarray(material, MATERIAL_DTYPE).reshape(1)
+    n = np.ascontiguousarray(n, np.float64)
+    wo = np.ascontiguousarray(wo, np.float64)
+    wi = np.ascontiguousarray(wi, np.float64).reshape(-1, 3)
+    f = np.zeros_like(wi)
+    pdf = np.zeros(wi.shape[0])
+    lib().orc_bsdf_eval(_p(mat), _p(n), _p(wo), _p(wi), wi.shape[0], _p(f), _p(pdf))
+    return f, pdf
+
+
+def bsdf_sample(material, n, wo, u, entering=True):
+    """One renderer BSDF sample per uniform triple u (N, 3): dict(wi, weight, pdf, side, delta);
+    side 0 = path ends, 1 = reflected, 2 = refracted."""
+    mat = np.ascontiguousarray(material, MATERIAL_DTYPE).reshape(1)
+    n = np.ascontiguousarray(n, np.float64)
+    wo = np.ascontiguousarray(wo, np.float64)
+    u = np.ascontiguousarray(u, np.float64).reshape(-1, 3)
+    k = u.shape[0]
+    wi, w = np.zeros((k, 3)), np.zeros((k, 3))
+    pdf = np.zeros(k)
+    side = np.zeros(k, np.int32)
+    delta = np.zeros(k, np.int32)
+    lib().orc_bsdf_sample(_p(mat), _p(n), _p(wo), 1 if entering else 0, _p(u), k, _p(wi), _p(w), _p(pdf), _p(side),
+                          _p(delta))
+    return dict(wi=wi, weight=w, pdf=pdf, side=side, delta=delta)

@@ -158,6 +158,17 @@ typedef struct orc_pt_scene {
 int orc_render(const orc_pt_scene* s, orc_accel a, const float view_colmajor[16], const float proj_colmajor[16],
                int width, int height, int spp, int sample_base, int max_depth, double* out_rgb, int threads);
 
+/* The estimator's material model alone (n = shading normal on wo's side, all vectors unit),
+ * batched: orc_bsdf_eval gives, for `count` directions wi (3 doubles each), the non-delta BSDF
+ * value f (rgb) and the solid-angle density of the non-delta lobes as the renderer samples them;
+ * orc_bsdf_sample draws one direction per uniform triple u (ul, u1, u2) as the renderer does, with
+ * side 0 (path ends), 1 (reflected) or 2 (refracted), the throughput weight, the density (0 for
+ * the delta transmission lobe) and a delta flag. */
+int orc_bsdf_eval(const orc_material* m, const double n[3], const double wo[3], const double* wi, size_t count,
+                  double* f, double* pdf);
+int orc_bsdf_sample(const orc_material* m, const double n[3], const double wo[3], int entering, const double* u,
+                    size_t count, double* wi, double* weight, double* pdf, int32_t* side, int32_t* is_delta);
+
 /* ---- SDF sphere tracing -------------------------------------------------------------------- */
 typedef struct orc_sdf_node {
   uint32_t op;

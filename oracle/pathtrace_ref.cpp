@@ -82,6 +82,74 @@ void onb(D3 n, D3& t, D3& b) {
   b = {c, s + n.y * n.y * a, -n.y};
 }
 
+// Lobe weights and parameters of a material at a surface seen from wo (shading normal ns): Schlick
+// Fresnel of n.v with f0 = mix(((ior-1)/(ior+1))^2, 1, metallic) (pbr.glsl:29-34) weighs the GGX
+// reflection (alpha = roughness^2, floored at 1e-3), the smooth-dielectric transmission and the
+// Lambert lobe; the weights are the selection probabilities and sum to 1.
+Lobes make_lobes(const orc_material& mat, D3 albedo, D3 ns, D3 wo) {
+  Lobes L;
+  L.albedo = albedo;
+  L.metallic = mat.metallic;
+  const double sm = std::sqrt(std::max(double(mat.metallic), 0.0));
+  L.spec_col = D3{1 + (L.albedo.x - 1) * sm, 1 + (L.albedo.y - 1) * sm, 1 + (L.albedo.z - 1) * sm};
+  const double alpha = std::max(double(mat.roughness) * mat.roughness, 1e-3);
+  L.a2 = alpha * alpha;
+  const double f0d = (mat.ior - 1.0) / (mat.ior + 1.0);
+  const double f0 = f0d * f0d + (1 - f0d * f0d) * mat.metallic;
+  const double cv = std::min(std::max(dot(ns, wo), 0.0), 1.0);
+  const double om = 1 - cv;
+  const double Fv = f0 + (1 - f0) * (om * om * om * om * om);
+  L.ws = Fv;
+  L.wt = (1 - Fv) * mat.transmission;
+  L.wd = (1 - Fv) * (1 - mat.transmission);
+  return L;
+}
+
+// One BSDF sample from (ul, u1, u2): ul picks the lobe by weight. Transmission is a smooth
+// dielectric (Snell with eta = 1/ior entering, ior leaving; total internal reflection as a
+// mirror) and scales the throughput by the albedo; GGX reflection samples the half vector with
+// cos(theta_h) = sqrt((1-u1)/(1+(a2-1)u1)) and reflects wo about it; Lambert samples cosine-
+// weighted. Returns 0 when the path ends (direction below the geometric surface or pdf 0),
+// 1 for a reflected direction, 2 for a refracted one; `weight` multiplies the throughput and pdf
+// is the solid-angle density of the non-delta lobes (0 for delta lobes).
+int sample_bsdf(const Lobes& L, double ior, bool entering, D3 ns, D3 ng, D3 wo, double ul, double u1, double u2,
+                D3& wi, D3& weight, double& pdf, bool& delta) {
+  if (ul >= L.ws && ul < L.ws + L.wt) {
+    const double eta = entering ? 1.0 / ior : ior;
+    const double ci = std::min(std::max(dot(wo, ns), 0.0), 1.0);
+    const double s2 = eta * eta * (1 - ci * ci);
+    int side = 1;
+    if (s2 > 1.0) {
+      wi = ns * (2 * ci) - wo;
+    } else {
+      wi = normalize(-wo * eta + ns * (eta * ci - std::sqrt(1 - s2)));
+      side = 2;
+    }
+    weight = L.albedo;
+    pdf = 0;
+    delta = true;
+    return side;
+  }
+  D3 t, bt;
+  onb(ns, t, bt);
+  if (ul < L.ws) {
+    const double cos_t = std::sqrt((1 - u1) / (1 + (L.a2 - 1) * u1));
+    const double sin_t = std::sqrt(std::max(0.0, 1 - cos_t * cos_t));
+    const double phi = 2 * kPi * u2;
+    const D3 hh = t * (sin_t * std::cos(phi)) + bt * (sin_t * std::sin(phi)) + ns * cos_t;
+    wi = hh * (2 * dot(wo, hh)) - wo;
+  } else {
+    const double rr = std::sqrt(u1), phi = 2 * kPi * u2;
+    wi = t * (rr * std::cos(phi)) + bt * (rr * std::sin(phi)) + ns * std::sqrt(std::max(0.0, 1 - u1));
+  }
+  D3 f;
+  eval_bsdf(L, ns, wo, wi, f, pdf);
+  delta = false;
+  if (dot(ng, wi) <= 0 || pdf <= 0) return 0;
+  weight = f * (dot(ns, wi) / pdf);
+  return 1;
+}
+
 D3 sky(D3 d, double scale) {
   const double t = (d.y + 1.0) * 0.5;
   return D3{0.3 + 0.7 * t, 0.4 + 0.58 * t, 0.5 + 0.46 * t} * scale;
@@ -362,26 +430,12 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out, uint
     const double eps = 1e-4 * (1.0 + std::max(std::max(std::fabs(p.x), std::fabs(p.y)), std::fabs(p.z)));
     if (mi >= S->n_materials) mi = 0;
     const orc_material& mat = S->materials[mi];
-    Lobes L;
-    L.albedo = rgba8(mat.albedo_rgba8);
+    D3 albedo = rgba8(mat.albedo_rgba8);
     if (has_uv && mat.texture >= 1 && mat.texture <= S->n_textures) {
       const D3 t = texture_lookup(S, mat.texture, hu, 1.0 - hv);
-      L.albedo = D3{std::pow(std::max(t.x, 0.0), 2.2), std::pow(std::max(t.y, 0.0), 2.2),
-                    std::pow(std::max(t.z, 0.0), 2.2)};
+      albedo = D3{std::pow(std::max(t.x, 0.0), 2.2), std::pow(std::max(t.y, 0.0), 2.2), std::pow(std::max(t.z, 0.0), 2.2)};
     }
-    L.metallic = mat.metallic;
-    const double sm = std::sqrt(std::max(double(mat.metallic), 0.0));
-    L.spec_col = D3{1 + (L.albedo.x - 1) * sm, 1 + (L.albedo.y - 1) * sm, 1 + (L.albedo.z - 1) * sm};
-    const double alpha = std::max(double(mat.roughness) * mat.roughness, 1e-3);
-    L.a2 = alpha * alpha;
-    const double f0d = (mat.ior - 1.0) / (mat.ior + 1.0);
-    const double f0 = f0d * f0d + (1 - f0d * f0d) * mat.metallic;
-    const double cv = std::min(std::max(dot(ns, wo), 0.0), 1.0);
-    const double om = 1 - cv;
-    const double Fv = f0 + (1 - f0) * (om * om * om * om * om);
-    L.ws = Fv;
-    L.wt = (1 - Fv) * mat.transmission;
-    L.wd = (1 - Fv) * (1 - mat.transmission);
+    const Lobes L = make_lobes(mat, albedo, ns, wo);
 
     if (S->n_lights > 0 && (L.ws + L.wd) > 0) {
       uint32_t l = uint32_t(rnd(pixel, sample, dim_of(bounce, 6)) * double(S->n_lights));
@@ -427,44 +481,16 @@ void trace_path(const Ctx& C, uint32_t pixel, uint32_t sample, double* out, uint
 
     const double ul = rnd(pixel, sample, dim_of(bounce, 0));
     const double u1 = rnd(pixel, sample, dim_of(bounce, 1)), u2 = rnd(pixel, sample, dim_of(bounce, 2));
-    bool ok = true;
-    D3 wi;
-    D3 origin = p + ng * eps;
-    if (ul >= L.ws && ul < L.ws + L.wt) {
-      const double eta = entering ? 1.0 / mat.ior : double(mat.ior);
-      const double ci = std::min(std::max(dot(wo, ns), 0.0), 1.0);
-      const double s2 = eta * eta * (1 - ci * ci);
-      if (s2 > 1.0) {
-        wi = ns * (2 * ci) - wo;
-      } else {
-        wi = normalize(-wo * eta + ns * (eta * ci - std::sqrt(1 - s2)));
-        origin = p - ng * eps;
-      }
-      thr = thr * L.albedo;
-      delta = true;
-    } else {
-      D3 t, bt;
-      onb(ns, t, bt);
-      if (ul < L.ws) {
-        const double cos_t = std::sqrt((1 - u1) / (1 + (L.a2 - 1) * u1));
-        const double sin_t = std::sqrt(std::max(0.0, 1 - cos_t * cos_t));
-        const double phi = 2 * kPi * u2;
-        const D3 hh = t * (sin_t * std::cos(phi)) + bt * (sin_t * std::sin(phi)) + ns * cos_t;
-        wi = hh * (2 * dot(wo, hh)) - wo;
-      } else {
-        const double rr = std::sqrt(u1), phi = 2 * kPi * u2;
-        wi = t * (rr * std::cos(phi)) + bt * (rr * std::sin(phi)) + ns * std::sqrt(std::max(0.0, 1 - u1));
-      }
-      D3 f;
-      double pdf;
-      eval_bsdf(L, ns, wo, wi, f, pdf);
-      if (dot(ng, wi) <= 0 || pdf <= 0) {
-        ok = false;
-      } else {
-        thr = thr * f * (dot(ns, wi) / pdf);
-        last_pdf = pdf;
-        delta = false;
-      }
+    D3 wi, weight;
+    double pdf = 0;
+    bool was_delta = false;
+    const int side = sample_bsdf(L, mat.ior, entering, ns, ng, wo, ul, u1, u2, wi, weight, pdf, was_delta);
+    bool ok = side != 0;
+    const D3 origin = side == 2 ? p - ng * eps : p + ng * eps;
+    if (ok) {
+      thr = thr * weight;
+      delta = was_delta;
+      if (!was_delta) last_pdf = pdf;
     }
     if (ok && !finite3(thr)) {
       ok = false;
@@ -525,4 +551,43 @@ extern "C" int orc_render(const orc_pt_scene* s, orc_accel a, const float view[1
   uint64_t total = 0;
   for (uint64_t b : bad) total += b;
   return int(std::min<uint64_t>(total, 0x7fffffff));
+}
+
+// The material model on its own, for tests that pin it against the mathematics (sampling
+// densities, reciprocity, albedo, Snell's law). n is the shading normal on wo's side; albedo comes
+// from the material record. Batched over directions (eval) or uniform triples (sample).
+extern "C" int orc_bsdf_eval(const orc_material* m, const double n[3], const double wo[3], const double* wi, size_t count,
+                             double* f, double* pdf) {
+  const D3 N{n[0], n[1], n[2]}, WO{wo[0], wo[1], wo[2]};
+  const Lobes L = make_lobes(*m, rgba8(m->albedo_rgba8), N, WO);
+  for (size_t k = 0; k < count; ++k) {
+    D3 F;
+    eval_bsdf(L, N, WO, D3{wi[3 * k], wi[3 * k + 1], wi[3 * k + 2]}, F, pdf[k]);
+    f[3 * k] = F.x;
+    f[3 * k + 1] = F.y;
+    f[3 * k + 2] = F.z;
+  }
+  return 0;
+}
+
+extern "C" int orc_bsdf_sample(const orc_material* m, const double n[3], const double wo[3], int entering,
+                               const double* u, size_t count, double* wi, double* weight, double* pdf, int32_t* side,
+                               int32_t* is_delta) {
+  const D3 N{n[0], n[1], n[2]}, WO{wo[0], wo[1], wo[2]};
+  const Lobes L = make_lobes(*m, rgba8(m->albedo_rgba8), N, WO);
+  for (size_t k = 0; k < count; ++k) {
+    D3 WI{0, 0, 0}, W{0, 0, 0};
+    bool delta = false;
+    pdf[k] = 0;
+    side[k] = sample_bsdf(L, m->ior, entering != 0, N, N, WO, u[3 * k], u[3 * k + 1], u[3 * k + 2], WI, W, pdf[k],
+                          delta);
+    wi[3 * k] = WI.x;
+    wi[3 * k + 1] = WI.y;
+    wi[3 * k + 2] = WI.z;
+    weight[3 * k] = W.x;
+    weight[3 * k + 1] = W.y;
+    weight[3 * k + 2] = W.z;
+    is_delta[k] = delta ? 1 : 0;
+  }
+  return 0;
 }
