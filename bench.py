@@ -358,6 +358,9 @@ def main():
     n_node = st_inc["node_visits"] / n_inc
     n_tri = st_inc["tri_tests"] / n_inc
     n_ins = st_inc["instance_visits"] / n_inc
+    # Algorithmic bytes per ray: 32 (ray) + 16 (hit) + 64 per wide node + 48 per triangle record,
+    # plus the 64-B instance record read when the ray enters the mesh's instance (once per ray
+    # here), a named extra term.
     b_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * n_node + TRI_BYTES * n_tri + INST_BYTES * n_ins
     achieved = kernel_rate * b_ray / 1e9
     # HBM traffic per launch from the committed rocprofv3 counter passes over this command
@@ -370,8 +373,13 @@ def main():
         traffic, l2_hit = rec["traffic_bytes_per_launch"], rec["l2_hit_rate"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "l2_hit_rate": l2_hit,
-                "bytes_per_ray": round(b_ray, 1), "node_fetches_per_ray": round(n_node, 3),
-                "tri_fetches_per_ray": round(n_tri, 3), "kernel_ms": round(ev_ms, 4)}
+                "bytes_per_ray": round(b_ray, 1),
+                "bytes_per_ray_terms": {"ray_and_hit": RAY_BYTES + HIT_BYTES, "wide_nodes": round(NODE_BYTES * n_node, 1),
+                                        "triangle_records": round(TRI_BYTES * n_tri, 1),
+                                        "instance_record": round(INST_BYTES * n_ins, 1)},
+                "node_fetches_per_ray": round(n_node, 3), "tri_fetches_per_ray": round(n_tri, 3),
+                "instance_fetches_per_ray": round(n_ins, 3), "kernel_ms": round(ev_ms, 4),
+                "kernel_ms_source": "HIP events on the launch stream around the timed passes"}
     if args.only_incoherent:
         print(json.dumps({"metric": "incoherent Mrays/s", "value": round(value, 2), "steps": args.steps,
                           "roofline": roofline}), flush=True)

@@ -75,6 +75,8 @@ def lib():
                                _P, ctypes.c_int], None),
             "orc_render": ([ctypes.POINTER(_PtScene), _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                             ctypes.c_int, ctypes.c_int, _P, ctypes.c_int], ctypes.c_int),
+            "orc_fetch_counts": ([_P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_uint32, _P, _SZ, ctypes.c_int, _P, _P,
+                                  ctypes.c_int], ctypes.c_int),
             "orc_bsdf_eval": ([_P, _P, _P, _P, _SZ, _P, _P], ctypes.c_int),
             "orc_bsdf_sample": ([_P, _P, _P, ctypes.c_int, _P, _SZ, _P, _P, _P, _P, _P], ctypes.c_int),
         }
@@ -364,3 +366,20 @@ def bsdf_sample(material, n, wo, u, entering=True):
     lib().orc_bsdf_sample(_p(mat), _p(n), _p(wo), 1 if entering else 0, _p(u), k, _p(wi), _p(w), _p(pdf), _p(side),
                           _p(delta))
     return dict(wi=wi, weight=w, pdf=pdf, side=side, delta=delta)
+
+
+def fetch_counts(wide, rays, any_hit=False, threads=0):
+    """Per-ray fetches of the product traversal kernel, replayed on the CPU over the product's
+    device-format arrays `wide` = (blas, tris, tlas, items, instances) from Scene.wide_arrays():
+    returns (counts (N, 3) uint32: wide nodes, triangle records, instance records; hits)."""
+    blas, tris, tlas, items, inst = wide
+    rays = np.ascontiguousarray(rays)
+    n = rays.size
+    counts = np.zeros((n, 3), np.uint32)
+    hits = np.zeros(n, HIT_DTYPE)
+    keep = [np.ascontiguousarray(a) for a in (blas, tris, tlas, items if items.size else np.zeros(1, np.uint32),
+                                              inst)]
+    single = 0 if inst.size == 1 else 0xFFFFFFFF
+    lib().orc_fetch_counts(_p(keep[0]), _p(keep[1]), _p(keep[2]), _p(keep[3]), _p(keep[4]), 1 if tlas.size else 0,
+                           single, _p(rays), n, 1 if any_hit else 0, _p(counts), _p(hits), threads)
+    return counts, hits

@@ -10,7 +10,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-SOURCES = ["bvh_ref.cpp", "trace_ref.cpp", "baseline.cpp", "pathtrace_ref.cpp", "sdf_ref.cpp"]
+SOURCES = ["bvh_ref.cpp", "trace_ref.cpp", "baseline.cpp", "pathtrace_ref.cpp", "sdf_ref.cpp", "fetch_count.cpp"]
 FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-pthread"]
 
 

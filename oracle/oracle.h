@@ -169,6 +169,15 @@ int orc_bsdf_eval(const orc_material* m, const double n[3], const double wo[3], 
 int orc_bsdf_sample(const orc_material* m, const double n[3], const double wo[3], int entering, const double* u,
                     size_t count, double* wi, double* weight, double* pdf, int32_t* side, int32_t* is_delta);
 
+/* A CPU replica of the product traversal kernel over the product's device-format arrays
+ * (mrt_scene_copy_wide: 64-B wide nodes, 48-B triangle records, TLAS nodes and leaf slots, 64-B
+ * instance records): per ray, counts[3i..3i+2] = wide nodes, triangle records and instance
+ * records the kernel fetches; hits (may be NULL) get the closest hit (t, prim, inst) or, for
+ * any_hit, t = 0 when occluded. single_instance = 0xFFFFFFFF unless the frame has one instance. */
+int orc_fetch_counts(const void* blas_nodes, const void* tris, const void* tlas_nodes, const uint32_t* tlas_items,
+                     const void* instances, int has_tlas, uint32_t single_instance, const orc_ray* rays, size_t n,
+                     int any_hit, uint32_t* counts, orc_hit* hits, int threads);
+
 /* ---- SDF sphere tracing -------------------------------------------------------------------- */
 typedef struct orc_sdf_node {
   uint32_t op;

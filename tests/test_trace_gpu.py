@@ -171,3 +171,39 @@ def test_ray_reserve_size_gives_identical_hits(ensure_built, monkeypatch, grab):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
         sh = scenes.shadow_rays(rays, a)
         assert np.array_equal(plain.trace_any(sh), other.trace_any(sh)), name + " shadow"
+
+
+@pytest.mark.parametrize("case", ["heightfield 1M", "wavefront 4M"])
+def test_kernel_fetch_counts_match_the_cpu_replica(ensure_built, case):
+    """The roofline's fetch counts per ray come from the kernel's counting variant; the CPU
+    replica (oracle/fetch_count.cpp) walks the same device arrays the same way and reproduces
+    them. The kernel's 1/d is the hardware reciprocal (1 ulp) and the replica's the correctly
+    rounded one, so box tests on the edge of a slab may differ: totals within 0.1 %."""
+    if case == "heightfield 1M":
+        (idx, pts, nrm, uv), coh, inc = scenes.traversal_workload()
+        s = m.Scene()
+        s.enqueue(s.push_geometry(idx, pts, nrm, uv))
+        sets = (("incoherent", inc), ("coherent", coh))
+    else:
+        sc = scenes.wavefront_scene()
+        s = m.Scene()
+        gids = [s.push_geometry(i, p, n) for i, p, n in sc["geometries"]]
+        for g, mt, mi in sc["instances"]:
+            s.enqueue(gids[g], mt, mi)
+        tris, refs = scenes.world_triangles(sc["geometries"], sc["instances"])
+        sets = (("incoherent", scenes.scene_bounce_rays(tris, refs, 1 << 20, seed=4)),)
+        del tris, refs
+    s.prepare()
+    wide = s.wide_arrays()
+    for name, rays in sets:
+        d_r = m.DeviceBuffer.from_array(rays)
+        for any_hit in (False, True):
+            say("fetch counts, %s, %s %d rays, any=%d" % (case, name, rays.size, any_hit))
+            gpu = s.trace_stats(d_r, rays.size, any_hit=any_hit)
+            cpu, _ = orc.fetch_counts(wide, rays, any_hit=any_hit)
+            tot = cpu.sum(0, dtype=np.int64)
+            say("  kernel nodes %d tris %d inst %d | replica %d %d %d" % (
+                gpu["node_visits"], gpu["tri_tests"], gpu["instance_visits"], tot[0], tot[1], tot[2]))
+            assert gpu["rays"] == rays.size
+            for key, k in (("node_visits", 0), ("tri_tests", 1), ("instance_visits", 2)):
+                assert abs(gpu[key] - int(tot[k])) <= 1e-3 * max(int(tot[k]), 1), (name, key)
