@@ -104,7 +104,7 @@ int mrt_cache_infer_device(mrt_cache ch, const mrt_cache_query* d_q, size_t n, f
   if (n == 0) return MRT_OK;
   if (n >= (1u << 31)) return fail(MRT_ERR_LIMIT, "too many queries");
   static_assert(sizeof(mrt_cache_query) == sizeof(CacheQuery), "query layouts differ");
-  const int grid = int(std::min<size_t>(size_t(c->infer_grid), (n + 127) / 128));
+  const int grid = int(std::min<size_t>(size_t(c->infer_grid), (n + 255) / 256));
   return launch_cache_infer(c->P, reinterpret_cast<const CacheQuery*>(d_q), uint32_t(n), nullptr, d_out, std::max(grid, 1),
                             static_cast<hipStream_t>(stream)) == 0
              ? MRT_OK

@@ -19,7 +19,7 @@ struct Cache {
   uint16_t *h_buf = nullptr, *g_buf = nullptr;
   float *loss_tiles = nullptr, *partial = nullptr, *grad = nullptr;
   size_t cap_batch = 0;
-  uint32_t chunk = 2048;
+  uint32_t chunk = 512;  // queries per weight-gradient partial
   int infer_grid = 1024;
 
   ~Cache() {
@@ -39,7 +39,8 @@ struct Cache {
     int dev = 0;
     hipGetDevice(&dev);
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) infer_grid = prop.multiProcessorCount * 8;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      infer_grid = prop.multiProcessorCount * cache_infer_blocks_per_cu();
     for (int a = 0; a < 3; ++a) {
       P.lo[a] = -1.f;
       P.inv_ext[a] = 0.5f;

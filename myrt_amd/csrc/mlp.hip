@@ -13,6 +13,7 @@
 // 16-byte fragment reads of 32 rows spread over the banks).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "mlp.hpp"
@@ -38,54 +39,98 @@ __device__ inline float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(
 
 __device__ inline bf16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
-__device__ inline float sel3(int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); }
-
-// sin(2 pi r) and cos(2 pi r) of r revolutions: the hardware's transcendental units take their
-// argument in revolutions; the fractional part keeps it in their domain. The features are rounded
-// to bf16 next, far coarser than these units' error.
+// sin(2 pi r) of r revolutions: the hardware's transcendental unit takes its argument in
+// revolutions; the fractional part keeps it in its domain. The features are rounded to bf16
+// next, far coarser than the unit's error.
 __device__ inline float sin_rev(float r) { return __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(r)); }
-__device__ inline float cos_rev(float r) { return __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(r)); }
 
-// The 64 network inputs of one query (input encoding): position in the cache's unit box with
-// sin/cos of 2^k pi x at 6 octaves (36), direction and normal with sin/cos at 2 octaves (12 + 12),
-// roughness (1) and albedo (3). q0..q3 are the query's four float4 rows. 2^k pi x radians are
-// 2^(k-1) x revolutions (an exact scaling).
-__device__ inline float encode_feature(float4 q0, float4 q1, float4 q2, float4 q3, const CacheParams& P, int f) {
-  if (f < 36) {
-    const int axis = f / 12, r = f % 12, k = r >> 1;
-    const float x = (sel3(axis, q0.x, q0.y, q0.z) - sel3(axis, P.lo[0], P.lo[1], P.lo[2])) *
-                    sel3(axis, P.inv_ext[0], P.inv_ext[1], P.inv_ext[2]);
-    const float rev = x * float(1 << k) * 0.5f;
-    return (r & 1) ? cos_rev(rev) : sin_rev(rev);
-  }
-  if (f < 60) {
-    const int g = f - 36;
-    const float4 v = g < 12 ? q1 : q2;
-    const int r = g % 12, axis = r / 4, k = (r >> 1) & 1;
-    const float rev = sel3(axis, v.x, v.y, v.z) * float(1 << k) * 0.5f;
-    return (r & 1) ? cos_rev(rev) : sin_rev(rev);
-  }
-  if (f == 60) return q0.w;
-  return sel3(f - 61, q3.x, q3.y, q3.z);
+// The 64 network inputs of a query, by input slot. The query's 13 components are
+//   U = (position in the cache box (3), direction (3), normal (3), roughness, albedo (3)),
+// and an input is sin(2^k pi U[c]) or cos(2^k pi U[c]) (2^(k-1) U[c] revolutions, cos a quarter
+// revolution later), or U[c] itself. Position takes octaves 0-5, direction and normal 0-1. The
+// slots are ordered so that lane halves h = 0 and h = 1 of a B fragment (slots 16 kb + 8 h + i)
+// evaluate the same octave and phase on different components in six of the eight slot rows:
+//   slots  0- 7 position x, octaves 2-5      slots  8-15 position y, octaves 2-5
+//          16-23 position x, y, octaves 0-1        24-31 position z, direction x, octaves 0-1
+//          32-39 direction y, z, octaves 0-1       40-47 normal x, y, octaves 0-1
+//          48-55 position z, octaves 2-5           56-63 normal z, octaves 0-1; roughness; albedo
+struct Slot {
+  int c;       // component of U
+  int k;       // octave
+  bool cos;
+  bool ident;  // U[c] itself
+};
+__host__ __device__ constexpr Slot lo_pair(int c0, int c1, int i) { return Slot{i < 4 ? c0 : c1, (i >> 1) & 1, (i & 1) != 0, false}; }
+__host__ __device__ constexpr Slot slot_of(int s) {
+  return s < 8    ? Slot{0, 2 + (s >> 1), (s & 1) != 0, false}
+         : s < 16 ? Slot{1, 2 + ((s - 8) >> 1), (s & 1) != 0, false}
+         : s < 24 ? lo_pair(0, 1, s - 16)
+         : s < 32 ? lo_pair(2, 3, s - 24)
+         : s < 40 ? lo_pair(4, 5, s - 32)
+         : s < 48 ? lo_pair(6, 7, s - 40)
+         : s < 56 ? Slot{2, 2 + ((s - 48) >> 1), (s & 1) != 0, false}
+         : s < 60 ? lo_pair(8, 8, s - 56)
+                  : Slot{9 + (s - 60), 0, false, true};
 }
 
-__device__ inline void encode_tile(const CacheQuery* qs, uint32_t base, uint32_t n, const CacheParams& P,
-                                   uint16_t* act) {
-  const int lane = __lane_id();
-  const int s = lane & 31, half = lane >> 5;
-  const uint32_t qi = base + uint32_t(s);
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
+
+__device__ inline uint32_t pk_bf16(float lo, float hi) {
+  typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
+  const bf16x2 v = {(__bf16)lo, (__bf16)hi};  // v_cvt_pk_bf16_f32: round to nearest even
+  return __builtin_bit_cast(uint32_t, v);
+}
+
+// Encodes the query of lane (s, h) (query index qi) into its four B fragments: lane h holds
+// input slots 16 kb + 8 h + (0..7) of fragment kb. Queries at or past n encode as zeros.
+__device__ inline void encode_fragments(const CacheQuery* qs, uint32_t qi, uint32_t n, const CacheParams& P,
+                                        bf16x8 (&b)[4]) {
+  const bool hi = (__lane_id() >> 5) != 0;
   float4 q0 = make_float4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
-  if (qi < n) {
+  const bool live = qi < n;
+  if (live) {
     const float4* qp = reinterpret_cast<const float4*>(qs + qi);
     q0 = qp[0];
     q1 = qp[1];
     q2 = qp[2];
     q3 = qp[3];
   }
-  for (int f = half * 32; f < half * 32 + 32; ++f) {
-    const float v = qi < n ? encode_feature(q0, q1, q2, q3, P, f) : 0.0f;
-    act[s * kStride + f] = to_bf16(v);
+  const float U[13] = {(q0.x - P.lo[0]) * P.inv_ext[0], (q0.y - P.lo[1]) * P.inv_ext[1],
+                       (q0.z - P.lo[2]) * P.inv_ext[2], q1.x, q1.y, q1.z, q2.x, q2.y, q2.z, q0.w, q3.x, q3.y, q3.z};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+      float y[2];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const Slot a = slot_of(16 * kb + i + e), b8 = slot_of(16 * kb + 8 + i + e);
+        const float x = hi ? U[b8.c] : U[a.c];
+        const float ma = float(1 << a.k) * 0.5f, mb = float(1 << b8.k) * 0.5f;
+        const float pa = a.cos ? 0.25f : 0.0f, pb = b8.cos ? 0.25f : 0.0f;
+        const float t = (ma == mb && pa == pb) ? fmaf(x, ma, pa) : fmaf(x, hi ? mb : ma, hi ? pb : pa);
+        float v = sin_rev(t);
+        if (a.ident || b8.ident) v = (hi ? b8.ident : a.ident) ? x : v;
+        y[e] = live ? v : 0.0f;
+      }
+      w[i >> 1] = pk_bf16(y[0], y[1]);
+    }
+    const u32x4 v = {w[0], w[1], w[2], w[3]};
+    b[kb] = __builtin_bit_cast(bf16x8, v);
   }
+}
+
+// The encoding of a tile into LDS as [query][slot] rows (row stride kStride), for the training
+// kernel's LDS-operand products.
+__device__ inline void encode_tile(const CacheQuery* qs, uint32_t base, uint32_t n, const CacheParams& P,
+                                   uint16_t* act) {
+  const int lane = __lane_id();
+  const int s = lane & 31, h = lane >> 5;
+  bf16x8 b[4];
+  encode_fragments(qs, base + uint32_t(s), n, P, b);
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) *reinterpret_cast<bf16x8*>(act + s * kStride + 16 * kb + 8 * h) = b[kb];
 }
 
 // D(64 x 32) = A(64 x 64) * act^T: A rows from `a` (row stride 64, global), act [32][kStride] in LDS.
@@ -161,143 +206,320 @@ __device__ inline void save_columns(const uint16_t* tile, uint16_t* dst) {
   for (int i = 0; i < 4; ++i) o[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
-__global__ __launch_bounds__(256) void infer_kernel(CacheParams P, const CacheQuery* qs, uint32_t n_host,
+// ---- inference: weights in LDS, activations in registers -----------------------------------
+//
+// A wave runs two tiles of 32 queries at once. The B operand of a product (8 input neurons of one
+// query per lane) is built in registers: the previous layer's D holds, in lane (s, h), neurons
+// 8j + 4h + (0..3) of query s, so lane h = 0 has rows 0-3 and 8-11 of every 16-row block and
+// lane h = 1 rows 4-7 and 12-15. One v_permlane32_swap per register pair gives lane h = 0 rows
+// 0-7 and lane h = 1 rows 8-15 of the block: exactly the next layer's B fragment. Weights sit in
+// LDS in fragment order (one 16-byte record per lane, consecutive lanes consecutive records), 44 KB
+// per workgroup: five 64 x 64 layers and the output layer's first 32 rows.
+constexpr int kFragRecords = (5 * 8 + 4) * 64;  // 16-byte A-fragment records of all layers
+
+// Record index of the A fragment (layer l, row block mb, k block kb) of lane `lane`.
+__device__ inline int frag_record(int l, int mb, int kb, int lane) { return ((l * 2 + mb) * 4 + kb) * 64 + lane; }
+
+// ReLU on two packed bf16: a bf16 with its sign bit set is a negative int16, so a signed 16-bit
+// max with 0 zeroes exactly the negative values (and -0).
+__device__ inline uint32_t relu_pk(uint32_t x) {
+  typedef short s16x2 __attribute__((ext_vector_type(2)));
+  const s16x2 v = __builtin_bit_cast(s16x2, x);
+  const s16x2 z = {0, 0};
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, z));
+}
+
+// Keeps the bf16 halves of a packed pair whose bits (bit, bit + 1) of `mask` are set.
+__device__ inline uint32_t mask_pk(uint32_t x, uint32_t mask, int bit) {
+  const uint32_t t = (mask >> bit) & 3u;
+  return x & ((0u - (t & 1u)) & 0xffffu | ((0u - (t >> 1)) << 16));
+}
+
+// D (64 neurons x 32 queries, in d0 = rows 0-31 and d1 = rows 32-63) to the next layer's four B
+// fragments: MODE 0 as is, 1 through ReLU, 2 keeping the values whose bit in `mask` is set (bit v
+// of mask is d0[v], bit 16 + v is d1[v]: the ReLU derivative in the backward pass).
+template <int MODE = 1>
+__device__ inline void to_fragments(const f32x16& d0, const f32x16& d1, bf16x8 (&b)[4], uint32_t mask = 0) {
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    const f32x16& d = kb < 2 ? d0 : d1;
+    const int v = 8 * (kb & 1);  // d[v..v+3]: rows 16kb' + 4h + (0..3); d[v+4..v+7]: rows 16kb' + 8 + 4h + (0..3)
+    const int mb = 16 * (kb >> 1);
+    uint32_t x0 = pk_bf16(d[v + 0], d[v + 1]), x1 = pk_bf16(d[v + 2], d[v + 3]);
+    uint32_t y0 = pk_bf16(d[v + 4], d[v + 5]), y1 = pk_bf16(d[v + 6], d[v + 7]);
+    if (MODE == 1) {
+      x0 = relu_pk(x0);
+      x1 = relu_pk(x1);
+      y0 = relu_pk(y0);
+      y1 = relu_pk(y1);
+    } else if (MODE == 2) {
+      x0 = mask_pk(x0, mask, mb + v);
+      x1 = mask_pk(x1, mask, mb + v + 2);
+      y0 = mask_pk(y0, mask, mb + v + 4);
+      y1 = mask_pk(y1, mask, mb + v + 6);
+    }
+    // Lanes 32-63 of x trade with lanes 0-31 of y: h = 0 keeps x (rows +0..3) and gets the other
+    // half's x (rows +4..7); h = 1 gets the other half's y (rows +8..11) and keeps its y (+12..15).
+    const auto s0 = __builtin_amdgcn_permlane32_swap(x0, y0, false, false);
+    const auto s1 = __builtin_amdgcn_permlane32_swap(x1, y1, false, false);
+    const u32x4 w = {s0[0], s1[0], s0[1], s1[1]};
+    b[kb] = __builtin_bit_cast(bf16x8, w);
+  }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void infer_kernel(CacheParams P, const CacheQuery* qs, uint32_t n_host,
                                                     const uint32_t* n_dev, float* out) {
-  __shared__ __attribute__((aligned(16))) uint16_t tiles[4][2][32 * kStride];
-  const int wave = threadIdx.x >> 6;
+  __shared__ u32x4 wlds[kFragRecords];
+  // Weights to LDS in fragment order: record (l, mb, kb, lane) = W[l][32 mb + lane % 32][16 kb + 8 (lane / 32) ..+8].
+  for (int rec = threadIdx.x; rec < kFragRecords; rec += 256) {
+    const int lane = rec & 63, kb = (rec >> 6) & 3, lm = rec >> 8, l = lm >> 1, mb = lm & 1;
+    const uint16_t* src = P.w + (size_t(l) * kW + 32 * mb + (lane & 31)) * kW + 16 * kb + 8 * (lane >> 5);
+    wlds[rec] = *reinterpret_cast<const u32x4*>(src);
+  }
+  __syncthreads();
   const int lane = __lane_id();
   const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;  // a device count is capped by the buffer size
-  const uint32_t tiles_total = (n + 31) / 32;
-  for (uint32_t t = blockIdx.x * 4 + wave; t < tiles_total; t += gridDim.x * 4) {
-    uint16_t* a = tiles[wave][0];
-    uint16_t* b = tiles[wave][1];
-    encode_tile(qs, t * 32, n, P, a);
-    __builtin_amdgcn_wave_barrier();
-    f32x16 d0, d1;
+  const uint32_t pairs = (n + 63) / 64;
+  const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), waves = gridDim.x * 4;
+  for (uint32_t pr = wave; pr < pairs; pr += waves) {
+    // The weight records do not change inside the loop; an opaque zero per layer keeps the
+    // compiler from hoisting all 44 fragment reads (176 registers) out of it.
+    int wofs = 0;
+    bf16x8 b[2][4];
+    encode_fragments(qs, pr * 64 + uint32_t(lane & 31), n, P, b[0]);
+    encode_fragments(qs, pr * 64 + 32 + uint32_t(lane & 31), n, P, b[1]);
+    f32x16 d[2][2];
+#pragma unroll
     for (int l = 0; l < kLayers; ++l) {
-      product(P.w + size_t(l) * kW * kW, a, d0, d1);
+      asm volatile("" : "+v"(wofs));
+      const u32x4* wl = wlds + wofs;
+      const int mbs = l < kLayers - 1 ? 2 : 1;  // the output layer needs rows 0-2 only
+#pragma unroll
+      for (int t = 0; t < 2; ++t) d[t][0] = d[t][1] = f32x16{};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+        for (int mb = 0; mb < mbs; ++mb) {
+          const bf16x8 a = __builtin_bit_cast(bf16x8, wl[frag_record(l, mb, kb, lane)]);
+#pragma unroll
+          for (int t = 0; t < 2; ++t) d[t][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[t][kb], d[t][mb], 0, 0, 0);
+        }
       if (l < kLayers - 1) {
-        store_tile(d0, d1, b, true, false, 0u);
-        __builtin_amdgcn_wave_barrier();
-        uint16_t* tmp = a;
-        a = b;
-        b = tmp;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) to_fragments(d[t][0], d[t][1], b[t]);
       }
     }
-    // Outputs are neurons 0-2: lanes 0-31 hold them in d0[0..2] for query lane.
-    const uint32_t qi = t * 32 + uint32_t(lane & 31);
-    if (lane < 32 && qi < n) {
-      out[3 * size_t(qi) + 0] = d0[0];
-      out[3 * size_t(qi) + 1] = d0[1];
-      out[3 * size_t(qi) + 2] = d0[2];
+    // Outputs are neurons 0-2: lanes 0-31 hold them in d[t][0][0..2] for query lane.
+    if (lane < 32) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const uint32_t qi = pr * 64 + 32 * t + uint32_t(lane);
+        if (qi < n) {
+          out[3 * size_t(qi) + 0] = d[t][0][0];
+          out[3 * size_t(qi) + 1] = d[t][0][1];
+          out[3 * size_t(qi) + 2] = d[t][0][2];
+        }
+      }
     }
   }
 }
 
-// Forward + backward of one tile per wave; writes H_l and G_l for the weight-gradient reduction
-// and the tile's loss sum. Activation and gradient buffers are [layer][tile][neuron][32 queries]
-// (layer stride 64 * batch_stride). The ReLU pattern of every hidden layer stays in registers
-// (one bit per value a lane stores), so LDS holds only two activation and two gradient tiles.
-__global__ __launch_bounds__(64) void train_kernel(CacheParams P, const CacheQuery* qs, const float* targets,
-                                                   uint32_t n, float grad_scale, uint16_t* h_out, uint16_t* g_out,
-                                                   float* loss_out) {
-  __shared__ __attribute__((aligned(16))) uint16_t H[2][32 * kStride];
-  __shared__ __attribute__((aligned(16))) uint16_t G[2][32 * kStride];
+// ---- training: forward and backward of one tile per wave, activations in registers ------------
+//
+// Weights W (forward, 44 KB) and W^T (backward: layers 1-5, 34 KB) sit in LDS in fragment order.
+// The forward and backward chains keep their operands in registers (to_fragments); each layer's
+// input H_l and output gradient G_l also go to HBM as [layer][tile][neuron][32 queries] columns for
+// the weight-gradient reduction, through a per-wave 4.5 KB LDS tile that transposes them. The
+// ReLU pattern of every hidden layer stays in registers (one bit per value a lane holds).
+constexpr int kTransRecords = (4 * 8 + 2) * 64;  // W^T fragments: layers 1-4 full, layer 5 k block 0
+constexpr int kTrainWaves = 8;
+
+__device__ inline int trans_record(int l, int mb, int kb, int lane) {
+  return l < 5 ? (((l - 1) * 2 + mb) * 4 + kb) * 64 + lane : (32 + mb) * 64 + lane;
+}
+
+// Positive-value pattern of D (the ReLU derivative): bit v of d0, 16 + v of d1. Rounding to bf16
+// keeps the sign and maps no positive value above the denormal range to zero.
+__device__ inline uint32_t positive_mask(const f32x16& d0, const f32x16& d1) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    if (d0[v] > 0.f) m |= 1u << v;
+    if (d1[v] > 0.f) m |= 1u << (16 + v);
+  }
+  return m;
+}
+
+// Writes a tile's four B fragments (lane (s, h): 8 neurons 16 kb + 8 h of query s) to HBM as
+// [neuron][32 queries] columns through the wave's LDS tile.
+__device__ inline void save_fragments(const bf16x8 (&b)[4], uint16_t* tile, uint16_t* dst) {
   const int lane = __lane_id();
   const int s = lane & 31, h = lane >> 5;
-  const uint32_t t = blockIdx.x;
-  const uint32_t qi = t * 32 + uint32_t(s);
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) *reinterpret_cast<bf16x8*>(tile + s * kStride + 16 * kb + 8 * h) = b[kb];
+  __builtin_amdgcn_wave_barrier();
+  save_columns(tile, dst);
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(64 * kTrainWaves) void train_kernel(CacheParams P, const CacheQuery* qs,
+                                                                 const float* targets, uint32_t n, float grad_scale,
+                                                                 uint16_t* h_out, uint16_t* g_out, float* loss_out) {
+  __shared__ u32x4 wf[kFragRecords];
+  __shared__ u32x4 wb[kTransRecords];
+  __shared__ __attribute__((aligned(16))) uint16_t trans[kTrainWaves][32 * kStride];
+  for (int rec = threadIdx.x; rec < kFragRecords; rec += 64 * kTrainWaves) {
+    const int lane = rec & 63, kb = (rec >> 6) & 3, lm = rec >> 8, l = lm >> 1, mb = lm & 1;
+    wf[rec] = *reinterpret_cast<const u32x4*>(P.w + (size_t(l) * kW + 32 * mb + (lane & 31)) * kW + 16 * kb +
+                                               8 * (lane >> 5));
+  }
+  for (int rec = threadIdx.x; rec < kTransRecords; rec += 64 * kTrainWaves) {
+    const int lane = rec & 63;
+    int l, mb, kb;
+    if (rec < 32 * 64) {
+      kb = (rec >> 6) & 3;
+      const int lm = rec >> 8;
+      l = 1 + (lm >> 1);
+      mb = lm & 1;
+    } else {
+      l = 5;
+      kb = 0;
+      mb = (rec >> 6) - 32;
+    }
+    wb[rec] = *reinterpret_cast<const u32x4*>(P.wt + (size_t(l) * kW + 32 * mb + (lane & 31)) * kW + 16 * kb +
+                                               8 * (lane >> 5));
+  }
+  __syncthreads();
+  const int lane = __lane_id();
+  const int s = lane & 31, h = lane >> 5;
+  const int wave = threadIdx.x >> 6;
+  uint16_t* tile = trans[wave];
   const size_t layer_stride = size_t(kW) * P.batch_stride;
-  const size_t tile_off = size_t(t) * kW * 32;
-  encode_tile(qs, t * 32, n, P, H[0]);
-  __syncthreads();
-  uint32_t relu_mask[kLayers];  // [l]: ReLU pattern of H_l, l = 1..5
-  f32x16 d0, d1;
-  int cur = 0;
-  for (int l = 0; l < kLayers; ++l) {
-    save_columns(H[cur], h_out + l * layer_stride + tile_off);
-    product(P.w + size_t(l) * kW * kW, H[cur], d0, d1);
-    if (l < kLayers - 1) {
-      relu_mask[l + 1] = store_tile(d0, d1, H[cur ^ 1], true, false, 0u);
-      cur ^= 1;
+  const uint32_t tiles = (n + 31) / 32;
+  for (uint32_t t = blockIdx.x * kTrainWaves + wave; t < tiles; t += gridDim.x * kTrainWaves) {
+    int wofs = 0;
+    const uint32_t qi = t * 32 + uint32_t(s);
+    const size_t tile_off = size_t(t) * kW * 32;
+    bf16x8 b[4];
+    encode_fragments(qs, qi, n, P, b);
+    uint32_t relu_mask[kLayers];  // [l]: positive pattern of z_(l-1) = input H_l of layer l, l = 1..5
+    f32x16 d0, d1;
+#pragma unroll
+    for (int l = 0; l < kLayers; ++l) {
+      save_fragments(b, tile, h_out + l * layer_stride + tile_off);
+      // An opaque zero per layer keeps each layer's fragment reads in place instead of hoisted to
+      // the top of the loop (176 registers for all of them).
+      asm volatile("" : "+v"(wofs));
+      const u32x4* wl = wf + wofs;
+      const int mbs = l < kLayers - 1 ? 2 : 1;
+      d0 = d1 = f32x16{};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) {
+        d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wl[frag_record(l, 0, kb, lane)]), b[kb],
+                                                      d0, 0, 0, 0);
+        if (mbs > 1)
+          d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wl[frag_record(l, 1, kb, lane)]),
+                                                        b[kb], d1, 0, 0, 0);
+      }
+      if (l < kLayers - 1) {
+        relu_mask[l + 1] = positive_mask(d0, d1);
+        to_fragments(d0, d1, b);
+      }
     }
-    __syncthreads();
-  }
-  // Loss: relative L2 (y - t)^2 / (sg(y)^2 + 0.01) per channel; dL/dy = 2 (y - t) / (y^2 + 0.01).
-  for (int i = lane; i < 32 * kStride; i += 64) G[0][i] = 0;
-  __syncthreads();
-  float loss = 0.f;
-  if (h == 0 && qi < n) {
-    for (int c = 0; c < 3; ++c) {
-      const float y = d0[c], tg = targets[3 * size_t(qi) + c];
-      const float den = y * y + 0.01f;
-      const float e = y - tg;
-      loss += e * e / den;
-      G[0][s * kStride + c] = to_bf16(2.0f * e / den * grad_scale);
+    // Loss: relative L2 (y - t)^2 / (sg(y)^2 + 0.01) per channel; dL/dy = 2 (y - t) / (y^2 + 0.01).
+    // Outputs y (rows 0-2) are d0[0..2] of lanes h = 0; G_5 as fragments: kb 0, lane h = 0 slots 0-2.
+    float loss = 0.f;
+    float g[3] = {0.f, 0.f, 0.f};
+    if (h == 0 && qi < n) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float y = d0[c], tg = targets[3 * size_t(qi) + c];
+        const float den = y * y + 0.01f;
+        const float e = y - tg;
+        loss += e * e / den;
+        g[c] = 2.0f * e / den * grad_scale;
+      }
     }
-  }
-  __syncthreads();
-  for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-  if (lane == 0) loss_out[t] = loss;
-  int gc = 0;
-  for (int l = kLayers - 1; l >= 0; --l) {
-    save_columns(G[gc], g_out + l * layer_stride + tile_off);
-    if (l > 0) {
-      product(P.wt + size_t(l) * kW * kW, G[gc], d0, d1);
-      store_tile(d0, d1, G[gc ^ 1], false, true, relu_mask[l]);
-      gc ^= 1;
-      __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    if (lane == 0) loss_out[t] = loss;
+    {
+      const u32x4 z = {0u, 0u, 0u, 0u};
+      const u32x4 g0 = {pk_bf16(g[0], g[1]), pk_bf16(g[2], 0.f), 0u, 0u};
+      b[0] = __builtin_bit_cast(bf16x8, h == 0 ? g0 : z);
+      b[1] = b[2] = b[3] = __builtin_bit_cast(bf16x8, z);
+    }
+    // Backward: G_(l-1) = (W_l^T G_l) masked by the ReLU pattern of H_l, l = 5..1.
+#pragma unroll
+    for (int l = kLayers - 1; l >= 0; --l) {
+      save_fragments(b, tile, g_out + l * layer_stride + tile_off);
+      if (l == 0) break;
+      asm volatile("" : "+v"(wofs));
+      const u32x4* wt = wb + wofs;
+      const int kbs = l == kLayers - 1 ? 1 : 4;  // G_5 has rows 0-2 only
+      d0 = d1 = f32x16{};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) {
+        if (kb >= kbs) break;
+        d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 0, kb, lane)]),
+                                                      b[kb], d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 1, kb, lane)]),
+                                                      b[kb], d1, 0, 0, 0);
+      }
+      to_fragments<2>(d0, d1, b, relu_mask[l]);
     }
   }
 }
 
-// dW_l partial over a range of queries: D(64 out x 64 in) = G_l[out][q] * H_l[in][q]^T, with G_l and
-// H_l in the [tile][neuron][32 queries] layout of train_kernel.
-__global__ __launch_bounds__(64) void wgrad_kernel(const uint16_t* h_in, const uint16_t* g_in, uint32_t batch_stride,
-                                                   uint32_t n_pad, uint32_t chunk, float* partial) {
+// dW_l partial over a chunk of queries: D(64 out x 64 in) = G_l[out][q] * H_l[in][q]^T, with G_l and
+// H_l in the [tile][neuron][32 queries] layout of train_kernel. Wave w of the workgroup computes the
+// 32 x 32 quadrant (w / 2, w % 2); the output layer's G has rows 0-2 only, so its second row block
+// is zero and skipped.
+__global__ __launch_bounds__(256) void wgrad_kernel(const uint16_t* h_in, const uint16_t* g_in, uint32_t batch_stride,
+                                                    uint32_t n_pad, uint32_t chunk, float* partial) {
   const int lane = __lane_id();
   const int r = lane & 31, hh = lane >> 5;
+  const int wave = threadIdx.x >> 6, mb = wave >> 1, nb = wave & 1;
   const int l = blockIdx.y;
   const uint32_t c = blockIdx.x;
-  const uint32_t q0 = c * chunk, q1 = min(n_pad, q0 + chunk);
-  const uint16_t* Gl = g_in + size_t(l) * kW * batch_stride;
-  const uint16_t* Hl = h_in + size_t(l) * kW * batch_stride;
-  f32x16 acc[2][2] = {};
-  for (uint32_t q = q0; q < q1; q += 16) {
-    // [tile][neuron][32 queries]: the 8 queries of a fragment lie in one tile row.
-    const uint32_t k = q + 8 * uint32_t(hh);
-    const size_t base = size_t(k >> 5) * kW * 32 + (k & 31u);
-    const bf16x8 a0 = ld8(Gl + base + size_t(r) * 32);
-    const bf16x8 a1 = ld8(Gl + base + size_t(32 + r) * 32);
-    const bf16x8 b0 = ld8(Hl + base + size_t(r) * 32);
-    const bf16x8 b1 = ld8(Hl + base + size_t(32 + r) * 32);
-    acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[1][1], 0, 0, 0);
-  }
   float* out = partial + (size_t(c) * kLayers + l) * kW * kW;
+  f32x16 acc = f32x16{};
+  if (!(l == kLayers - 1 && mb == 1)) {
+    const uint32_t q0 = c * chunk, q1 = min(n_pad, q0 + chunk);
+    const uint16_t* Gl = g_in + size_t(l) * kW * batch_stride;
+    const uint16_t* Hl = h_in + size_t(l) * kW * batch_stride;
+    for (uint32_t q = q0; q < q1; q += 16) {
+      // [tile][neuron][32 queries]: the 8 queries of a fragment lie in one tile row.
+      const uint32_t k = q + 8 * uint32_t(hh);
+      const size_t base = size_t(k >> 5) * kW * 32 + (k & 31u);
+      const bf16x8 a = ld8(Gl + base + size_t(32 * mb + r) * 32);
+      const bf16x8 b = ld8(Hl + base + size_t(32 * nb + r) * 32);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+  }
 #pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int row = mb * 32 + 8 * (v >> 2) + 4 * hh + (v & 3);
-        const int col = nb * 32 + r;
-        out[row * kW + col] = acc[mb][nb][v];
-      }
+  for (int v = 0; v < 16; ++v) {
+    const int row = mb * 32 + 8 * (v >> 2) + 4 * hh + (v & 3);
+    out[row * kW + nb * 32 + r] = acc[v];
+  }
 }
 
+// Sums the chunk partials of 64 parameters per workgroup (4 slices of the chunks per parameter,
+// combined in LDS) and takes the Adam step.
 __global__ __launch_bounds__(256) void adam_kernel(CacheParams P, const float* partial, uint32_t chunks, float lr,
                                                    float beta1, float beta2, float eps, float bc1, float bc2,
                                                    float* grad_out) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= uint32_t(kLayers * kW * kW)) return;
+  __shared__ float part[4][64];
+  const uint32_t p = blockIdx.x * 64 + (threadIdx.x & 63), slice = threadIdx.x >> 6;
+  float g = 0.f;
+#pragma unroll 8
+  for (uint32_t c = slice; c < chunks; c += 4) g += partial[size_t(c) * kLayers * kW * kW + p];
+  part[slice][threadIdx.x & 63] = g;
+  __syncthreads();
+  if (slice != 0) return;
+  const uint32_t i = p;
   const uint32_t l = i / (kW * kW), rem = i % (kW * kW), row = rem / kW, col = rem % kW;
   if (l == kLayers - 1 && row >= 3) return;  // padding rows of the output layer stay zero
-  float g = 0.f;
-  for (uint32_t c = 0; c < chunks; ++c) g += partial[size_t(c) * kLayers * kW * kW + i];
+  g = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
   if (grad_out) grad_out[i] = g;
   if (lr == 0.f) return;
   const float m = beta1 * P.m[i] + (1.0f - beta1) * g;
@@ -321,6 +543,13 @@ __global__ __launch_bounds__(256) void sync_bf16_kernel(CacheParams P) {
 
 }  // namespace mlpk
 
+int cache_infer_blocks_per_cu() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, mlpk::infer_kernel, 256, 0) != hipSuccess || blocks < 1)
+    blocks = 1;
+  return blocks;
+}
+
 int launch_cache_infer(const CacheParams& P, const CacheQuery* q, uint32_t n, const uint32_t* n_dev, float* out,
                        int grid, hipStream_t s) {
   hipLaunchKernelGGL(mlpk::infer_kernel, dim3(grid), dim3(256), 0, s, P, q, n, n_dev, out);
@@ -333,13 +562,25 @@ int launch_cache_train(const CacheParams& P, const CacheQuery* q, const float* t
                        hipStream_t s) {
   const uint32_t tiles = (n + 31) / 32;
   const uint32_t n_pad = tiles * 32;
-  hipLaunchKernelGGL(mlpk::train_kernel, dim3(tiles), dim3(64), 0, s, P, q, targets, n, grad_scale, h_buf, g_buf,
-                     loss_tiles);
+  static int max_groups = 0;  // one resident workgroup per CU (LDS-limited): a persistent grid
+  if (!max_groups) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mlpk::train_kernel, 64 * mlpk::kTrainWaves, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    max_groups = std::max(1, cus * per_cu);
+  }
+  const uint32_t groups = std::min<uint32_t>(uint32_t(max_groups), (tiles + mlpk::kTrainWaves - 1) / mlpk::kTrainWaves);
+  hipLaunchKernelGGL(mlpk::train_kernel, dim3(groups), dim3(64 * mlpk::kTrainWaves), 0, s, P, q, targets, n,
+                     grad_scale, h_buf, g_buf, loss_tiles);
   const uint32_t chunks = (n_pad + chunk - 1) / chunk;
-  hipLaunchKernelGGL(mlpk::wgrad_kernel, dim3(chunks, 6), dim3(64), 0, s, h_buf, g_buf, P.batch_stride, n_pad, chunk,
+  hipLaunchKernelGGL(mlpk::wgrad_kernel, dim3(chunks, 6), dim3(256), 0, s, h_buf, g_buf, P.batch_stride, n_pad, chunk,
                      partial);
-  hipLaunchKernelGGL(mlpk::adam_kernel, dim3((6 * 64 * 64 + 255) / 256), dim3(256), 0, s, P, partial, chunks, lr,
-                     beta1, beta2, eps, bc1, bc2, grad_out);
+  hipLaunchKernelGGL(mlpk::adam_kernel, dim3(6 * 64 * 64 / 64), dim3(256), 0, s, P, partial, chunks, lr, beta1, beta2,
+                     eps, bc1, bc2, grad_out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

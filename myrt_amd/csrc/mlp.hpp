@@ -35,6 +35,9 @@ constexpr int kCacheWidth = 64;
 constexpr int kCacheParamsPadded = kCacheLayers * kCacheWidth * kCacheWidth;  // 24576
 constexpr int kCacheParams = 5 * 64 * 64 + 3 * 64;                            // 20672
 
+// Inference workgroups that fit one CU at once: a grid of that many per CU is persistent (each
+// workgroup loads the weights to LDS once and loops over its tiles).
+int cache_infer_blocks_per_cu();
 int launch_cache_infer(const CacheParams& P, const CacheQuery* q, uint32_t n, const uint32_t* n_dev, float* out,
                        int grid, hipStream_t s);
 int launch_cache_train(const CacheParams& P, const CacheQuery* q, const float* targets, uint32_t n, float grad_scale,

@@ -406,7 +406,7 @@ struct Renderer {
   // this pass's training batch from the training paths and take train_steps Adam steps on it.
   int cache_pass(const PtBuffers& b, hipStream_t st) {
     if (b.cache.terminate) {
-      const int grid = std::max(1, std::min(cache->infer_grid, int((b.cache.max_queries + 127) / 128)));
+      const int grid = std::max(1, std::min(cache->infer_grid, int((b.cache.max_queries + 255) / 256)));
       if (launch_cache_infer(cache->P, queries, b.cache.max_queries, b.counts + 4, qout, grid, st))
         return fail(MRT_ERR_HIP, "cache inference");
       if (launch_pt_cache_resolve(b, st)) return fail(MRT_ERR_HIP, "cache resolve");

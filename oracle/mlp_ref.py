@@ -39,29 +39,44 @@ def pack(mats):
     return np.concatenate([m.ravel() for m in mats])
 
 
+def _slot(s):
+    """Input slot s -> (component, octave, cos, identity); components U = position in the cache
+    box (3), direction (3), normal (3), roughness, albedo (3). Slots 0-7: position x octaves 2-5;
+    8-15: position y 2-5; 16-23: position x, y octaves 0-1; 24-31: position z, direction x 0-1;
+    32-39: direction y, z 0-1; 40-47: normal x, y 0-1; 48-55: position z 2-5; 56-59: normal z 0-1;
+    60: roughness; 61-63: albedo (the kernel's order, mlp.hip slot_of)."""
+    def lo_pair(c0, c1, i):
+        return (c0 if i < 4 else c1, (i >> 1) & 1, i & 1, False)
+    if s < 8:
+        return (0, 2 + (s >> 1), s & 1, False)
+    if s < 16:
+        return (1, 2 + ((s - 8) >> 1), s & 1, False)
+    if s < 48:
+        c0, c1 = {16: (0, 1), 24: (2, 3), 32: (4, 5), 40: (6, 7)}[s - s % 8]
+        return lo_pair(c0, c1, s % 8)
+    if s < 56:
+        return (2, 2 + ((s - 48) >> 1), s & 1, False)
+    if s < 60:
+        return lo_pair(8, 8, s - 56)
+    return (9 + (s - 60), 0, 0, True)
+
+
+SLOTS = [_slot(s) for s in range(64)]
+
+
 def encode(q, lo, hi):
     """The 64 inputs of each query: position in [lo, hi] with sin/cos of 2^k*pi*x, k = 0..5 (36),
-    direction and normal with k = 0, 1 (12 + 12), roughness, albedo (4)."""
+    direction and normal with k = 0, 1 (12 + 12), roughness, albedo (4), in the slot order SLOTS."""
     q = np.asarray(q)
     lo = np.asarray(lo, np.float64)
     ext = np.asarray(hi, np.float64) - lo
-    n = q.size
-    x = np.zeros((n, 64))
     p = (q["pos"].astype(np.float64) - lo) / ext
-    for axis in range(3):
-        for k in range(6):
-            a = p[:, axis] * (2 ** k) * np.pi
-            x[:, axis * 12 + 2 * k] = np.sin(a)
-            x[:, axis * 12 + 2 * k + 1] = np.cos(a)
-    for g, field in enumerate(("dir", "nrm")):
-        v = q[field].astype(np.float64)
-        for axis in range(3):
-            for k in range(2):
-                a = v[:, axis] * (2 ** k) * np.pi
-                x[:, 36 + g * 12 + axis * 4 + 2 * k] = np.sin(a)
-                x[:, 36 + g * 12 + axis * 4 + 2 * k + 1] = np.cos(a)
-    x[:, 60] = q["roughness"]
-    x[:, 61:64] = q["albedo"]
+    U = np.concatenate([p, q["dir"].astype(np.float64), q["nrm"].astype(np.float64),
+                        q["roughness"].astype(np.float64)[:, None], q["albedo"].astype(np.float64)], 1)
+    x = np.zeros((q.size, 64))
+    for s, (c, k, cos, ident) in enumerate(SLOTS):
+        a = U[:, c] * (2 ** k) * np.pi
+        x[:, s] = U[:, c] if ident else (np.cos(a) if cos else np.sin(a))
     return x
 
 

@@ -39,11 +39,14 @@ def test_encoding_layout():
     q["pos"][0] = (-1, -1, -1)
     x = M.encode(q, (-1, -1, -1), (1, 1, 1))
     assert x.shape == (5, 64)
-    # At the box corner every position sine is 0 and every cosine 1.
-    assert np.allclose(x[0, 0:36:2], 0) and np.allclose(x[0, 1:36:2], 1)
+    # At the box corner every position sine is 0 and every cosine 1 (position slots: 0-23 and
+    # 24-27, 48-55).
+    pos = [s for s, (c, k, cos, ident) in enumerate(M.SLOTS) if c < 3]
+    assert len(pos) == 36
+    assert all(np.isclose(x[0, s], 1.0 if M.SLOTS[s][2] else 0.0) for s in pos)
     assert np.allclose(x[:, 60], q["roughness"]) and np.allclose(x[:, 61:], q["albedo"])
-    assert np.allclose(x[:, 36], np.sin(np.pi * q["dir"][:, 0]))
-    assert np.allclose(x[:, 48 + 4 * 2 + 3], np.cos(2 * np.pi * q["nrm"][:, 2]))
+    assert np.allclose(x[:, 28], np.sin(np.pi * q["dir"][:, 0]))
+    assert np.allclose(x[:, 59], np.cos(2 * np.pi * q["nrm"][:, 2]))
 
 
 def test_bf16_rounding():
@@ -51,3 +54,22 @@ def test_bf16_rounding():
     assert M.to_bf16(1.0 + 2 ** -8) == 1.0          # tie -> even
     assert M.to_bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6
     assert M.to_bf16(-3.0) == -3.0
+
+
+def test_encoding_covers_every_feature_once():
+    """The 64 input slots hold position octaves 0-5, direction and normal octaves 0-1 (sin and cos
+    each), roughness and albedo, each exactly once; a slot's value is the closed form of its
+    feature."""
+    feats = [(c, k, cos) for c, k, cos, ident in M.SLOTS if not ident]
+    want = [(c, k, cos) for c in range(3) for k in range(6) for cos in (0, 1)]
+    want += [(c, k, cos) for c in range(3, 9) for k in range(2) for cos in (0, 1)]
+    assert sorted(feats) == sorted(want) and len(feats) == len(set(feats)) == 60
+    assert sorted(c for c, _, _, ident in M.SLOTS if ident) == [9, 10, 11, 12]
+    q = np.zeros(1, M.QUERY_DTYPE)
+    q["pos"], q["dir"], q["nrm"], q["roughness"], q["albedo"] = (0.25, -0.5, 0.75), (0.1, 0.2, 0.3), (0.4, 0.5, 0.6), 0.7, (0.8, 0.9, 1.0)
+    x = M.encode(q, (-1, -1, -1), (1, 1, 1))[0]
+    u = (np.array([0.25, -0.5, 0.75]) + 1) / 2
+    assert np.isclose(x[0], np.sin(4 * np.pi * u[0])) and np.isclose(x[7], np.cos(32 * np.pi * u[0]))
+    assert np.isclose(x[9], np.cos(4 * np.pi * u[1])) and np.isclose(x[17], np.cos(np.pi * u[0]))
+    assert np.isclose(x[28], np.sin(np.pi * 0.1)) and np.isclose(x[55], np.cos(32 * np.pi * u[2]))
+    assert np.isclose(x[59], np.cos(2 * np.pi * 0.6)) and np.allclose(x[60:], [0.7, 0.8, 0.9, 1.0])
