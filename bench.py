@@ -120,12 +120,14 @@ def wavefront_extras(args):
 
 
 def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 64), warm_frames=16):
-    """Equal-time MSE against a converged render (ref_spp samples) of the 4M-triangle wavefront
-    scene. For each budget (the time the plain path tracer takes for that many samples per pixel)
-    four methods get the same time: plain, plain + denoiser, radiance cache, radiance cache +
-    denoiser. The cache starts from warm_frames frames of 16 samples of the same view (an
-    interactive session's previous frames; not charged to the budget) and keeps training while
-    it renders."""
+    """Equal-time MSE against a converged render of the 4M-triangle wavefront scene: ref_spp
+    samples of the plain tracer in 256-sample chunks over disjoint sample ranges, each chunk's mean
+    accumulated in binary64 on the host. For each budget (the time the plain tracer takes for that
+    many samples per pixel) every method gets the same time, its denoiser included:
+    plain; plain + denoiser (as many samples as fit beside the denoiser); the radiance cache and
+    cache + denoiser from a warm cache (warm_frames frames of 16 samples of the same view, an
+    interactive session's history, not charged) and from a cold cache (a fresh network trained only
+    inside the budget). The cache keeps training while it renders."""
     import myrt_amd as m
     from myrt_amd import scenes
     weights = os.path.join(ROOT, "myrt_amd", "data", "denoiser_unet.npy")
@@ -140,13 +142,15 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
         r.set_num_bounces(depth)
         return r
 
-    log("equal-time study: reference %d spp at %dx%d" % (ref_spp, W, H))
+    log("equal-time study: reference %d spp at %dx%d (binary64 over 256-sample chunks)" % (ref_spp, W, H))
     ref_r = renderer()
-    done = 0
-    while done < ref_spp:
+    acc = np.zeros((H, W, 3), np.float64)
+    chunks = ref_spp // 256
+    for k in range(chunks):
+        ref_r.set_sample_base(1 << 20 | (256 * k))  # sample indices no timed render uses
         ref_r.render(s, W, H, 256)
-        done += 256
-    ref = ref_r.image()[..., :3].astype(np.float64)
+        acc += ref_r.image()[..., :3].astype(np.float64)
+    ref = acc / chunks
 
     def mse(img):
         return float(((img[..., :3].astype(np.float64) - ref) ** 2).mean())
@@ -162,6 +166,22 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
         r.invalidate()
         return r
 
+    def fit(render_spp, T, t_extra, first):
+        """The largest sample count whose render plus t_extra fits T (1 % slack), by rescaling
+        each render's time (at most 6 renders); returns (spp, seconds)."""
+        spp, times, last = max(1, first), {}, None
+        while spp not in times and len(times) < 6:
+            times[spp] = render_spp(spp)
+            last = spp
+            lo = max([k for k, t in times.items() if t + t_extra <= T * 1.01], default=0)
+            hi = min([k for k, t in times.items() if t + t_extra > T * 1.01], default=1 << 30)
+            guess = int(spp * (T - t_extra) / times[spp])
+            spp = min(max(guess, lo + 1), hi - 1) if hi - lo > 1 else max(lo, 1)
+        best = max([k for k, t in times.items() if t + t_extra <= T * 1.01], default=1)
+        if best != last:  # the renderer must hold the chosen render's image
+            times[best] = render_spp(best)
+        return best, times[best]
+
     plain = renderer()
     plain.render(s, W, H, 16)  # allocations and first launches stay out of the timings
     den = None
@@ -169,18 +189,20 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
         den = m.Denoiser()
         den.load(weights)
         plain.denoise(den)
-    cache = m.RadianceCache(seed=1)
-    cache.set_bounds((-12.5, -7.0, -9.5), (12.5, 6.0, 9.5))
+    warm_cache = m.RadianceCache(seed=1)
+    warm_cache.set_bounds((-12.5, -7.0, -9.5), (12.5, 6.0, 9.5))
     warm = renderer()
-    warm.set_cache(cache, mode=2)
-    for _ in range(warm_frames):
-        warm.render(s, W, H, 16)
+    warm.set_cache(warm_cache, mode=2)
+    _, warm_s = timed(lambda: [warm.render(s, W, H, 16) for _ in range(warm_frames)])
     cr = renderer()
-    cr.set_cache(cache, mode=2)
+    cr.set_cache(warm_cache, mode=2)
     cr.render(s, W, H, 16)
     _, t16 = timed(lambda: fresh(cr).render(s, W, H, 16))
-    res = {"width": W, "height": H, "depth": depth, "reference_spp": ref_spp, "cache_warm_frames": warm_frames,
+    res = {"width": W, "height": H, "depth": depth, "reference_spp": ref_spp, "reference_accumulation": "binary64 over "
+           "256-sample fp32 chunks", "cache_warm_frames": warm_frames, "cache_warm_s": round(warm_s, 4),
            "cache_seconds_per_16spp": round(t16, 4), "budgets": []}
+    cold_r = renderer()
+    cold_r.render(s, W, H, 16)  # allocations stay out of the timings
     for b in budgets:
         _, T = timed(lambda: fresh(plain).render(s, W, H, b))
         row = {"budget_spp": b, "budget_s": round(T, 4), "plain_mse": mse(plain.image())}
@@ -190,36 +212,41 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
         row["bilateral_s"] = round(t_bl, 4)
         t_dn = 0.0
         if den is not None:
-            img, t_dn = timed(lambda: plain.denoise(den))
-            row["plain_denoised_mse"] = mse(img)
+            _, t_dn = timed(lambda: plain.denoise(den))
             row["denoise_s"] = round(t_dn, 4)
-        # The cache path gets the most samples whose render plus the denoiser fits the budget. The
-        # first guess scales its 16-spp time; each render then rescales by its own time, kept
-        # between the largest count that fitted and the smallest that did not (at most 6 renders).
-        spp_c = max(1, int((T - t_dn) / t16 * 16))
-        times, last = {}, None
-        while spp_c not in times and len(times) < 6:
-            _, times[spp_c] = timed(lambda: fresh(cr).render(s, W, H, spp_c))
-            last = spp_c
-            lo = max([k for k, t in times.items() if t + t_dn <= T * 1.01], default=0)
-            hi = min([k for k, t in times.items() if t + t_dn > T * 1.01], default=1 << 30)
-            guess = int(spp_c * (T - t_dn) / times[spp_c])
-            spp_c = min(max(guess, lo + 1), hi - 1) if hi - lo > 1 else max(lo, 1)
-        spp_c = max([k for k, t in times.items() if t + t_dn <= T * 1.01], default=1)
-        if spp_c != last:
-            _, times[spp_c] = timed(lambda: fresh(cr).render(s, W, H, spp_c))
-        tc = times[spp_c]
+            spp_pd, t_pd = fit(lambda k: timed(lambda: fresh(plain).render(s, W, H, k))[1], T, t_dn,
+                               int(b * (T - t_dn) / T))
+            img, t_dn2 = timed(lambda: plain.denoise(den))
+            row.update({"plain_denoised_spp": spp_pd, "plain_denoised_s": round(t_pd + t_dn2, 4),
+                        "plain_denoised_mse": mse(img)})
+        # Warm cache: as many samples as fit (with the denoiser when there is one).
+        spp_c, tc = fit(lambda k: timed(lambda: fresh(cr).render(s, W, H, k))[1], T, t_dn,
+                        int((T - t_dn) / t16 * 16))
         row.update({"cache_spp": spp_c, "cache_s": round(tc, 4), "cache_mse": mse(cr.image())})
         if den is not None:
             img, t = timed(lambda: cr.denoise(den))
             row["cache_denoised_mse"] = mse(img)
             row["ratio_plain_over_cache_denoised"] = round(row["plain_mse"] / row["cache_denoised_mse"], 3)
             row["ratio_plain_over_plain_denoised"] = round(row["plain_mse"] / row["plain_denoised_mse"], 3)
+            # Cold cache: a fresh network for every trial, trained only inside the budget.
+            def cold_render(k):
+                cold = m.RadianceCache(seed=2)
+                cold.set_bounds((-12.5, -7.0, -9.5), (12.5, 6.0, 9.5))
+                cold_r.set_cache(cold, mode=2)
+                t = timed(lambda: fresh(cold_r).render(s, W, H, k))[1]
+                cold_render.keep = cold
+                return t
+            spp_k, tk = fit(cold_render, T, t_dn, spp_c)
+            img, t = timed(lambda: cold_r.denoise(den))
+            row.update({"cold_cache_spp": spp_k, "cold_cache_s": round(tk + t, 4), "cold_cache_denoised_mse": mse(img),
+                        "ratio_plain_over_cold_cache_denoised": round(row["plain_mse"] / mse(img), 3)})
         row["ratio_plain_over_cache"] = round(row["plain_mse"] / row["cache_mse"], 3)
         res["budgets"].append(row)
         log("budget %d spp: %s" % (b, row))
     fresh(cr).render(s, W, H, 256)
     res["cache_256spp_mse"] = mse(cr.image())  # bias floor of the trained cache
+    fresh(plain).render(s, W, H, 256)
+    res["plain_256spp_mse"] = mse(plain.image())
     s.close()
     return res
 

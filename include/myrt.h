@@ -357,6 +357,11 @@ int mrt_renderer_set_cubemap(mrt_renderer r, const float* faces, int size);
  * mrt_renderer_invalidate to restart it. */
 int mrt_renderer_set_cache(mrt_renderer r, mrt_cache c, int mode, float train_fraction, int train_steps,
                            int min_bounce);
+/* The sampler index of the first sample of the accumulation, 0 by default (no reference
+ * counterpart): renderers given disjoint ranges [base, base + spp) render independent samples of
+ * one image, so their means combine (say one renderer per GPU, or chunks of a reference image
+ * accumulated in binary64 on the host). Restarts the accumulation. */
+int mrt_renderer_set_sample_base(mrt_renderer r, uint32_t base);
 /* invalidate_counter / sample_count. */
 int mrt_renderer_invalidate(mrt_renderer r);
 int mrt_renderer_sample_count(mrt_renderer r, uint32_t* out);
@@ -365,6 +370,10 @@ int mrt_renderer_sample_count(mrt_renderer r, uint32_t* out);
 int mrt_render(mrt_renderer r, mrt_scene s, int width, int height, int spp);
 /* The accumulated mean radiance, width*height RGBA floats (alpha = 1). */
 int mrt_renderer_read(mrt_renderer r, float* rgba);
+/* The same image into a device buffer of width*height RGBA floats, asynchronously on `stream`
+ * (NULL: the renderer's stream, which mrt_render has synchronised). Replaces the GPU texture that
+ * run() returns (src/pathtracer/sequential_pathtracer.hpp:26-28): no host copy per frame. */
+int mrt_renderer_read_device(mrt_renderer r, float* d_rgba, void* stream);
 int mrt_renderer_stats(mrt_renderer r, mrt_render_stats* out);
 
 /* ---- neural radiance cache (no reference counterpart: the north star's learned part) ------

@@ -130,6 +130,7 @@ SIGNATURES = {
     "mrt_renderer_set_cubemap": ([_P, _P, ctypes.c_int], ctypes.c_int),
     "mrt_scene_push_texture": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
     "mrt_renderer_set_cache": ([_P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mrt_renderer_set_sample_base": ([_P, ctypes.c_uint32], ctypes.c_int),
     "mrt_renderer_sample_count": ([_P, _U32P], ctypes.c_int),
     "mrt_render": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "mrt_renderer_read": ([_P, _P], ctypes.c_int),
@@ -160,6 +161,7 @@ SIGNATURES = {
                                    ctypes.POINTER(ctypes.c_float), _P], ctypes.c_int),
     "mrt_denoiser_denoise": ([_P, _P, ctypes.c_int, ctypes.c_int, _P], ctypes.c_int),
     "mrt_denoiser_denoise_device": ([_P, _P, ctypes.c_int, ctypes.c_int, _P, _P], ctypes.c_int),
+    "mrt_renderer_read_device": ([_P, _P, _P], ctypes.c_int),
     "mrt_renderer_read_aux": ([_P, _P], ctypes.c_int),
     "mrt_post_pass_device": ([ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, _P],
                              ctypes.c_int),
@@ -622,6 +624,10 @@ class Renderer:
         _check(lib().mrt_renderer_set_cache(self.h, cache.h if cache is not None else None, int(mode),
                                             float(train_fraction), int(train_steps), int(min_bounce)))
 
+    def set_sample_base(self, base):
+        """Sampler index of the accumulation's first sample; restarts the accumulation."""
+        _check(lib().mrt_renderer_set_sample_base(self.h, int(base)))
+
     def sample_count(self):
         out = ctypes.c_uint32()
         _check(lib().mrt_renderer_sample_count(self.h, ctypes.byref(out)))
@@ -636,6 +642,10 @@ class Renderer:
         out = np.zeros((h, w, 4), np.float32)
         _check(lib().mrt_renderer_read(self.h, _ptr(out)))
         return out
+
+    def image_device(self, d_rgba, stream=None):
+        """Writes the mean image into the DeviceBuffer d_rgba (H * W * 16 bytes) on the device."""
+        _check(lib().mrt_renderer_read_device(self.h, d_rgba.ptr, stream))
 
     def stats(self):
         s = _RenderStats()

@@ -831,6 +831,23 @@ __global__ __launch_bounds__(kBlock) void denoise_input_kernel(const float4* acc
 }
 }  // namespace ptk
 
+namespace ptk {
+// Mean radiance of each pixel (running sum / sample count), alpha 1.
+__global__ __launch_bounds__(kBlock) void mean_kernel(const float4* accum, float inv_spp, float4* out, size_t npix) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= npix) return;
+  const float4 c = accum[i];
+  out[i] = make_float4(c.x * inv_spp, c.y * inv_spp, c.z * inv_spp, 1.0f);
+}
+}  // namespace ptk
+
+int launch_pt_mean(const float4* accum, uint32_t spp, float4* out, size_t npix, hipStream_t stream) {
+  const unsigned grid = unsigned((npix + ptk::kBlock - 1) / ptk::kBlock);
+  hipLaunchKernelGGL(ptk::mean_kernel, dim3(grid ? grid : 1), dim3(ptk::kBlock), 0, stream, accum,
+                     1.0f / float(spp ? spp : 1), out, npix);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_pt_denoise_input(const float4* accum, const float4* aux, uint32_t spp, float* raw, size_t npix,
                             hipStream_t stream) {
   const unsigned grid = unsigned((npix + ptk::kBlock - 1) / ptk::kBlock);

@@ -158,6 +158,7 @@ int launch_pt_shade(const PtScene& sc, const PtBuffers& b, uint32_t cur, uint32_
 int launch_pt_resolve(const PtBuffers& b, hipStream_t stream);
 int launch_pt_cache_resolve(const PtBuffers& b, hipStream_t stream);
 int launch_pt_train_batch(const PtBuffers& b, hipStream_t stream);
+int launch_pt_mean(const float4* accum, uint32_t spp, float4* out, size_t npix, hipStream_t stream);
 int launch_pt_denoise_input(const float4* accum, const float4* aux, uint32_t spp, float* raw, size_t npix,
                             hipStream_t stream);
 

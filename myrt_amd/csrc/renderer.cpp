@@ -95,6 +95,7 @@ struct Renderer {
   Camera cam{};
   int num_bounces = 10;      // sequential_pathtracer.hpp:134
   uint32_t sample_count = 0;
+  uint32_t sample_base = 0;  // sampler index of the accumulation's first sample (mrt_renderer_set_sample_base)
   uint32_t width = 0, height = 0;
   size_t cap_paths = 0, cap_accum = 0;
   // The sample pipeline's queues: ray and path queues (current and next bounce), hits, shadow
@@ -307,7 +308,8 @@ struct Renderer {
     if (e != hipSuccess) return hip_fail(e, "clearing the render counters");
     for (uint32_t done = 0; done < spp;) {
       const uint32_t nb = std::min(batch, spp - done);
-      if ((rc = run_batch(s, sc, pipe, nb, npix, sample_count + done, pc, cadence_spp)) != MRT_OK) return rc;
+      if ((rc = run_batch(s, sc, pipe, nb, npix, sample_base + sample_count + done, pc, cadence_spp)) != MRT_OK)
+        return rc;
       done += nb;
     }
     uint64_t ctr[2] = {0, 0};  // rays traced, non-finite values rejected
@@ -461,6 +463,15 @@ struct Renderer {
     hipError_t e = hipMemcpyAsync(rgba, den, npix * 16, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     return e == hipSuccess ? MRT_OK : hip_fail(e, "reading the denoised image");
+  }
+
+  // The mean image into a device buffer, asynchronously on `st` (null: the renderer's stream).
+  int read_device(float* d_rgba, hipStream_t st) {
+    if (!accum || sample_count == 0) return fail(MRT_ERR_INVALID, "nothing rendered yet");
+    const size_t npix = size_t(width) * height;
+    if (launch_pt_mean(accum, sample_count, reinterpret_cast<float4*>(d_rgba), npix, st ? st : stream))
+      return fail(MRT_ERR_HIP, "mean image");
+    return MRT_OK;
   }
 
   int read(float* rgba) {
@@ -738,6 +749,16 @@ int mrt_renderer_invalidate(mrt_renderer rh) {
   MRT_GUARD_END
 }
 
+int mrt_renderer_set_sample_base(mrt_renderer rh, uint32_t base) {
+  MRT_GUARD_BEGIN
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r) return fail(MRT_ERR_INVALID, "null renderer");
+  r->sample_base = base;
+  r->sample_count = 0;
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
 int mrt_renderer_sample_count(mrt_renderer rh, uint32_t* out) {
   MRT_GUARD_BEGIN
   Renderer* r = reinterpret_cast<Renderer*>(rh);
@@ -761,6 +782,14 @@ int mrt_renderer_read(mrt_renderer rh, float* rgba) {
   Renderer* r = reinterpret_cast<Renderer*>(rh);
   if (!r || !rgba) return fail(MRT_ERR_INVALID, "null argument");
   return r->read(rgba);
+  MRT_GUARD_END
+}
+
+int mrt_renderer_read_device(mrt_renderer rh, float* d_rgba, void* stream) {
+  MRT_GUARD_BEGIN
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || !d_rgba) return fail(MRT_ERR_INVALID, "null argument");
+  return r->read_device(d_rgba, static_cast<hipStream_t>(stream));
   MRT_GUARD_END
 }
 

@@ -325,3 +325,32 @@ def test_wavefront_scene_matches_fp64_oracle(ensure_built):
     say("relative MSE %.3e" % err)
     assert np.isfinite(img).all() and img.mean() > 0.01
     assert err <= REL_MSE_TOL
+
+
+def test_device_image_and_sample_ranges(ensure_built):
+    """mrt_renderer_read_device writes the host read's image into device memory bit for bit, and
+    disjoint sample ranges (mrt_renderer_set_sample_base) combine: the mean of samples 0-7 and of
+    samples 8-15 equals one 16-sample render up to fp32 summation order."""
+    sc = scenes.sphere_field_scene(30, 2, seed=9, glass_every=4, metal_every=3)
+    W, H = 64, 40
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], W / H)
+    s = gpu_scene(sc)
+    r = m.Renderer()
+    r.set_camera(view, proj)
+    r.set_num_bounces(5)
+    say("sample ranges %dx%d" % (W, H))
+    r.render(s, W, H, 16)
+    full = r.image().astype(np.float64)
+    d = m.DeviceBuffer(W * H * 16)
+    r.image_device(d)
+    m.synchronize()
+    assert np.array_equal(d.to_array(np.float32, W * H * 4).reshape(H, W, 4), r.image())
+    halves = []
+    for base in (0, 8):
+        r.set_sample_base(base)
+        r.render(s, W, H, 8)
+        assert r.sample_count() == 8
+        halves.append(r.image().astype(np.float64))
+    both = (halves[0] + halves[1]) / 2
+    assert np.abs(both - full).max() <= 1e-5 * max(1.0, np.abs(full).max())
+    assert np.abs(halves[0] - halves[1]).max() > 1e-3  # different samples
