@@ -255,15 +255,71 @@ int mrt_scene_copy_wide(mrt_scene s, void* blas_nodes, void* tris, void* tlas_no
  * rgba floats, row 0 at v = 0. Returns its id (>= 1) for mrt_material.texture; a textured hit
  * takes albedo = texture(u, 1 - v)^2.2 with bilinear filtering and wrap (material_new.glsl:29-31). */
 int mrt_scene_push_texture(mrt_scene s, const float* rgba, int w, int h, uint32_t* out_id);
-/* scene::push_material / set_parameter(material, name, value) (src/pathtracer/scene.hpp:134-140,
- * 249-252; src/material.hpp:81-102). Parameter names: "albedo" (3 or 4 floats), "ior",
- * "roughness", "metallic", "transmission"; a wrong count is an error (parameter.cpp:40-41). The
- * material index is what mrt_scene_enqueue_geometry takes. */
+/* The reference application's material (src/main.cpp:447-464, pushed through
+ * scene::push_material, src/pathtracer/scene.hpp:134-140) as one record: a material of the
+ * built-in 7-block layout {albedo: color_t{value: vec4unorm8}, ior, roughness, metallic,
+ * transmission, albedo_texture: uvec2} with texture = the uvec2's low word. The material index is
+ * what mrt_scene_enqueue_geometry and SDF programs take; materials of every type share it. */
 int mrt_scene_push_material(mrt_scene s, const mrt_material* mat, uint32_t* out_id);
+/* Writes a record through the material's type bindings (below); an input its type does not
+ * bind must keep its default (albedo 0xFFFFFFFF, ior 1.5, roughness 1, others 0). */
 int mrt_scene_set_material(mrt_scene s, uint32_t id, const mrt_material* mat);
-/* A material record as the shaders read it (scene::get_material_buffer, src/pathtracer/scene.hpp:178). */
+/* The record the shaders read for a material of any type (scene::get_material_buffer,
+ * src/pathtracer/scene.hpp:178, resolved through the type's bindings). */
 int mrt_scene_get_material(mrt_scene s, uint32_t id, mrt_material* out);
+/* set_parameter(material, name, value) with the value as floats: a vec4unorm8 parameter (or a
+ * layout wrapping one, like color_t) takes 3 or 4 values in [0, 1] packed to RGBA8 (alpha 1 if
+ * absent); integer types take integral values; a parameter feeding the texture input also takes
+ * the texture id alone and must name a pushed texture; float types take exactly their component
+ * count (parameter.cpp:40-41). An unknown name is an error here. */
 int mrt_scene_set_material_param(mrt_scene s, uint32_t id, const char* name, const float* value, size_t n);
+
+/* ---- named-parameter material layouts ---------------------------------------------------
+ * object_type / types_registry (src/types.hpp:216-310): a layout is a named list of parameters,
+ * each a constant type ("float", "int", "uint", "vec2".."vec4", "vec4unorm8", "ivec2".."ivec4",
+ * "uvec2".."uvec4", "mat2".."mat4"; 1, 1, 1, 2-4, 1, 2-4, 2-4, 4/9/16 blocks) or an earlier layout,
+ * packed as consecutive 4-byte blocks in declaration order (object_type::add_parameter,
+ * src/types.hpp:250-258). A layout can no longer grow once a material type or another layout
+ * uses it. Layout and material type ids are per scene. */
+/* object_type(name) registered with types_registry::push_back (src/types.hpp:216-224, 284-290). */
+int mrt_scene_register_layout(mrt_scene s, const char* name, uint32_t* out_layout);
+/* object_type::add_parameter(name, type) (src/types.hpp:250-258) with the type found by name as
+ * types_registry::find does (src/types.hpp:292-299): the first layout of that name, else the
+ * constant type of that identifier. */
+int mrt_scene_layout_add_parameter(mrt_scene s, uint32_t layout, const char* name, const char* type_name);
+/* types_registry::find: *constant = the constant type's parameter_type_id (src/types.hpp:27-44),
+ * or -1 and *layout = the layout id. Unknown names are an error. */
+int mrt_scene_find_type(mrt_scene s, const char* name, int32_t* constant, uint32_t* layout);
+/* object_type::blocks() and parameters() (src/types.hpp:240-262). */
+int mrt_scene_layout_info(mrt_scene s, uint32_t layout, uint32_t* blocks, uint32_t* n_params);
+int mrt_scene_layout_parameter(mrt_scene s, uint32_t layout, uint32_t index, char* name, size_t name_cap,
+                               uint32_t* offset, uint32_t* blocks, int32_t* constant, uint32_t* inner_layout);
+/* scene::push_material_type(type, glsl) (src/pathtracer/scene.hpp:131-133, material.hpp:18-37).
+ * Every material is shaded by the GGX model (the reference's material_new.glsl); in place of the
+ * GLSL `open()` that loads its inputs, input_paths names, for each input in the order albedo, ior,
+ * roughness, metallic, transmission, albedo_texture, the parameter path that feeds it ("albedo"
+ * or "albedo.value"; a layout of one parameter counts as that parameter). A NULL array, or a NULL
+ * entry, binds the input to the parameter of the input's name if the layout has one; "" leaves
+ * it unbound (its default). Albedo takes vec4unorm8, albedo_texture uint or uvec2 (low word = a
+ * texture id, a nonzero high word = no texture), the others float; a mismatch is an error. */
+int mrt_scene_push_material_type(mrt_scene s, uint32_t layout, const char* const* input_paths, uint32_t* out_type);
+/* scene::push_material(material_type) (src/pathtracer/scene.hpp:134-140, material.hpp:81-91):
+ * the material's blocks start at zero. */
+int mrt_scene_push_material_of_type(mrt_scene s, uint32_t type, uint32_t* out_id);
+/* scene::set_parameter<T>(material, name, value) (src/pathtracer/scene.hpp:156, 249-252;
+ * material.hpp:93-102): writes `bytes` bytes, the value's object representation (a vec4unorm8 is
+ * its 4 bytes r, g, b, a, parameter.cpp:130-133; a uvec2 its two words), at the parameter's
+ * offset. The size must be the parameter's blocks x 4, else MRT_ERR_INVALID ("Data size not
+ * suitable for this parameter"). A name the layout lacks is ignored, as in the reference, and
+ * *found (if not NULL) says which happened. */
+int mrt_scene_set_parameter(mrt_scene s, uint32_t material, const char* name, const void* data, size_t bytes,
+                            int* found);
+/* material_t {index, offset, type_info} (src/material.hpp:74-79). */
+int mrt_scene_material_slot(mrt_scene s, uint32_t material, uint32_t* type, uint32_t* offset);
+/* material_buffer::buffer() (src/material.hpp:104-106) as raw 4-byte blocks: copies
+ * min(n, total) blocks and reports the total. */
+int mrt_scene_material_blocks(mrt_scene s, uint32_t* out, size_t n, size_t* total);
+
 /* scene::push_sdf(sdf_info_t{root}) (src/pathtracer/scene.cpp:189-204) with the tree as a postfix
  * program; the program is validated (stack discipline, depth <= 8, modifiers <= 4 deep). */
 int mrt_scene_push_sdf(mrt_scene s, const mrt_sdf_node* nodes, size_t n, uint32_t* out_sdf);

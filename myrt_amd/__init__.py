@@ -111,6 +111,18 @@ SIGNATURES = {
     "mrt_scene_set_material": ([_P, ctypes.c_uint32, _P], ctypes.c_int),
     "mrt_scene_get_material": ([_P, ctypes.c_uint32, _P], ctypes.c_int),
     "mrt_scene_set_material_param": ([_P, ctypes.c_uint32, ctypes.c_char_p, _P, _SZ], ctypes.c_int),
+    "mrt_scene_register_layout": ([_P, ctypes.c_char_p, _U32P], ctypes.c_int),
+    "mrt_scene_layout_add_parameter": ([_P, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
+    "mrt_scene_find_type": ([_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), _U32P], ctypes.c_int),
+    "mrt_scene_layout_info": ([_P, ctypes.c_uint32, _U32P, _U32P], ctypes.c_int),
+    "mrt_scene_layout_parameter": ([_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p, _SZ, _U32P, _U32P,
+                                    ctypes.POINTER(ctypes.c_int32), _U32P], ctypes.c_int),
+    "mrt_scene_push_material_type": ([_P, ctypes.c_uint32, _P, _U32P], ctypes.c_int),
+    "mrt_scene_push_material_of_type": ([_P, ctypes.c_uint32, _U32P], ctypes.c_int),
+    "mrt_scene_set_parameter": ([_P, ctypes.c_uint32, ctypes.c_char_p, _P, _SZ, ctypes.POINTER(ctypes.c_int)],
+                                ctypes.c_int),
+    "mrt_scene_material_slot": ([_P, ctypes.c_uint32, _U32P, _U32P], ctypes.c_int),
+    "mrt_scene_material_blocks": ([_P, _P, _SZ, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "mrt_scene_set_lights": ([_P, _P, _SZ], ctypes.c_int),
     "mrt_scene_set_environment": ([_P, ctypes.c_float], ctypes.c_int),
     "mrt_scene_push_sdf": ([_P, _P, _SZ, _U32P], ctypes.c_int),
@@ -450,10 +462,82 @@ def quad_light(corner, u, v, emission):
 
 
 def _scene_push_material(self, mat):
-    mat = np.ascontiguousarray(mat, MATERIAL_DTYPE)
+    """scene::push_material: `mat` is a material type id (from push_material_type; the new
+    material's parameters start at zero) or a MATERIAL_DTYPE record of the built-in layout."""
     out = ctypes.c_uint32()
+    if isinstance(mat, (int, np.integer)):
+        _check(lib().mrt_scene_push_material_of_type(self.h, int(mat), ctypes.byref(out)))
+        return out.value
+    mat = np.ascontiguousarray(mat, MATERIAL_DTYPE)
     _check(lib().mrt_scene_push_material(self.h, _ptr(mat), ctypes.byref(out)))
     return out.value
+
+
+def _scene_register_type(self, name, parameters):
+    """object_type(name) with add_parameter(name, type) for each (name, type name) pair, type
+    names resolved like types_registry::find (layouts first, then "float", "vec4unorm8", ...)."""
+    out = ctypes.c_uint32()
+    _check(lib().mrt_scene_register_layout(self.h, name.encode(), ctypes.byref(out)))
+    for pname, tname in parameters:
+        _check(lib().mrt_scene_layout_add_parameter(self.h, out.value, pname.encode(), tname.encode()))
+    return out.value
+
+
+def _scene_find_type(self, name):
+    """types_registry::find: ("constant", parameter_type_id) or ("layout", layout id)."""
+    c, l = ctypes.c_int32(), ctypes.c_uint32()
+    _check(lib().mrt_scene_find_type(self.h, name.encode(), ctypes.byref(c), ctypes.byref(l)))
+    return ("constant", c.value) if c.value >= 0 else ("layout", l.value)
+
+
+def _scene_layout(self, layout):
+    """(blocks, [(name, offset, blocks, constant id or -1, inner layout or None), ...])."""
+    blocks, n = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(lib().mrt_scene_layout_info(self.h, int(layout), ctypes.byref(blocks), ctypes.byref(n)))
+    params = []
+    for i in range(n.value):
+        name = ctypes.create_string_buffer(256)
+        off, b, inner = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        c = ctypes.c_int32()
+        _check(lib().mrt_scene_layout_parameter(self.h, int(layout), i, name, 256, ctypes.byref(off),
+                                                ctypes.byref(b), ctypes.byref(c), ctypes.byref(inner)))
+        params.append((name.value.decode(), off.value, b.value, c.value, None if c.value >= 0 else inner.value))
+    return blocks.value, params
+
+
+def _scene_push_material_type(self, layout, inputs=None):
+    """scene::push_material_type: `inputs` maps GGX inputs (albedo, ior, roughness, metallic,
+    transmission, albedo_texture) to parameter paths; missing inputs bind by name, "" unbinds."""
+    names = ("albedo", "ior", "roughness", "metallic", "transmission", "albedo_texture")
+    arr = None
+    keep = []
+    if inputs is not None:
+        unknown = set(inputs) - set(names)
+        if unknown:
+            raise MrtError("unknown shading inputs %s" % sorted(unknown))
+        arr = (ctypes.c_char_p * 6)()
+        for i, k in enumerate(names):
+            if k in inputs:
+                keep.append(inputs[k].encode())
+                arr[i] = keep[-1]
+    out = ctypes.c_uint32()
+    _check(lib().mrt_scene_push_material_type(self.h, int(layout), arr, ctypes.byref(out)))
+    return out.value
+
+
+def _scene_material_slot(self, mid):
+    t, o = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(lib().mrt_scene_material_slot(self.h, int(mid), ctypes.byref(t), ctypes.byref(o)))
+    return t.value, o.value
+
+
+def _scene_material_blocks(self):
+    """material_buffer::buffer() as uint32 blocks (view floats with .view(np.float32))."""
+    total = ctypes.c_size_t()
+    _check(lib().mrt_scene_material_blocks(self.h, None, 0, ctypes.byref(total)))
+    out = np.zeros(max(total.value, 1), np.uint32)
+    _check(lib().mrt_scene_material_blocks(self.h, _ptr(out), out.size, ctypes.byref(total)))
+    return out[:total.value]
 
 
 def _scene_get_material(self, mid):
@@ -463,8 +547,20 @@ def _scene_get_material(self, mid):
 
 
 def _scene_set_material_param(self, mid, name, value):
+    """scene::set_parameter(material, name, value). A numpy value (array or scalar) or bytes is
+    written as its object representation, like the reference's set_parameter<T>: its size must
+    match the parameter, and a name the layout lacks is ignored (the call returns False). Python
+    numbers and lists go through the float form (vec4unorm8 from 3 or 4 floats in [0, 1],
+    integral values for integer types), where an unknown name is an error."""
+    if isinstance(value, (bytes, bytearray, memoryview, np.ndarray, np.generic)):
+        raw = bytes(value) if not isinstance(value, (np.ndarray, np.generic)) else np.ascontiguousarray(value).tobytes()
+        buf = ctypes.create_string_buffer(raw, max(len(raw), 1))
+        found = ctypes.c_int()
+        _check(lib().mrt_scene_set_parameter(self.h, int(mid), name.encode(), buf, len(raw), ctypes.byref(found)))
+        return bool(found.value)
     v = np.atleast_1d(np.asarray(value, np.float32))
     _check(lib().mrt_scene_set_material_param(self.h, int(mid), name.encode(), _ptr(v), v.size))
+    return True
 
 
 def _scene_set_lights(self, lights):
@@ -531,6 +627,12 @@ Scene.set_sdf_marching = _scene_set_sdf_marching
 Scene.push_material = _scene_push_material
 Scene.get_material = _scene_get_material
 Scene.set_parameter = _scene_set_material_param
+Scene.register_type = _scene_register_type
+Scene.find_type = _scene_find_type
+Scene.layout = _scene_layout
+Scene.push_material_type = _scene_push_material_type
+Scene.material_slot = _scene_material_slot
+Scene.material_blocks = _scene_material_blocks
 Scene.set_lights = _scene_set_lights
 Scene.set_environment = _scene_set_environment
 

@@ -513,82 +513,6 @@ int mrt_scene_push_texture(mrt_scene sh, const float* rgba, int w, int h, uint32
   MRT_GUARD_END
 }
 
-int mrt_scene_push_material(mrt_scene sh, const mrt_material* mat, uint32_t* out) {
-  MRT_GUARD_BEGIN
-  Scene* s = reinterpret_cast<Scene*>(sh);
-  if (!s || !mat) return fail(MRT_ERR_INVALID, "null argument");
-  static_assert(sizeof(mrt_material) == sizeof(Material), "material layouts differ");
-  Material m;
-  std::memcpy(&m, mat, sizeof(m));
-  s->materials.push_back(m);
-  s->shading_dirty = true;
-  if (out) *out = uint32_t(s->materials.size() - 1);
-  return MRT_OK;
-  MRT_GUARD_END
-}
-
-int mrt_scene_set_material(mrt_scene sh, uint32_t id, const mrt_material* mat) {
-  MRT_GUARD_BEGIN
-  Scene* s = reinterpret_cast<Scene*>(sh);
-  if (!s || !mat) return fail(MRT_ERR_INVALID, "null argument");
-  if (id >= s->materials.size()) return fail(MRT_ERR_INVALID, "unknown material");
-  std::memcpy(&s->materials[id], mat, sizeof(Material));
-  s->shading_dirty = true;
-  return MRT_OK;
-  MRT_GUARD_END
-}
-
-int mrt_scene_get_material(mrt_scene sh, uint32_t id, mrt_material* out) {
-  MRT_GUARD_BEGIN
-  Scene* s = reinterpret_cast<Scene*>(sh);
-  if (!s || !out) return fail(MRT_ERR_INVALID, "null argument");
-  if (id >= s->materials.size()) return fail(MRT_ERR_INVALID, "unknown material");
-  std::memcpy(out, &s->materials[id], sizeof(Material));
-  return MRT_OK;
-  MRT_GUARD_END
-}
-
-int mrt_scene_set_material_param(mrt_scene sh, uint32_t id, const char* name, const float* value, size_t n) {
-  MRT_GUARD_BEGIN
-  Scene* s = reinterpret_cast<Scene*>(sh);
-  if (!s || !name || !value) return fail(MRT_ERR_INVALID, "null argument");
-  if (id >= s->materials.size()) return fail(MRT_ERR_INVALID, "unknown material");
-  Material& m = s->materials[id];
-  const std::string k(name);
-  if (k == "albedo") {
-    // vec4unorm8 parameter (src/types.hpp:66-68): 3 or 4 floats in [0, 1] packed to RGBA8.
-    if (n != 3 && n != 4) return fail(MRT_ERR_INVALID, "albedo takes 3 or 4 values");
-    uint32_t packed = 0;
-    for (size_t c = 0; c < 4; ++c) {
-      const float v = c < n ? value[c] : 1.0f;
-      const uint32_t q = uint32_t(std::lround(std::fmin(std::fmax(v, 0.f), 1.f) * 255.f));
-      packed |= q << (8 * c);
-    }
-    m.albedo_rgba8 = packed;
-  } else if (n != 1) {
-    return fail(MRT_ERR_INVALID, "parameter '" + k + "' takes one value");  // parameter.cpp:40-41
-  } else if (k == "ior") {
-    m.ior = value[0];
-  } else if (k == "roughness") {
-    m.roughness = value[0];
-  } else if (k == "metallic") {
-    m.metallic = value[0];
-  } else if (k == "transmission") {
-    m.transmission = value[0];
-  } else if (k == "albedo_texture") {
-    // A texture id from mrt_scene_push_texture, or 0 for none.
-    const float v = value[0];
-    if (!(v >= 0.f) || v != std::floor(v) || v > float(s->textures.size()))
-      return fail(MRT_ERR_INVALID, "albedo_texture is not a texture id");
-    m.texture = uint32_t(v);
-  } else {
-    return fail(MRT_ERR_INVALID, "unknown material parameter '" + k + "'");
-  }
-  s->shading_dirty = true;
-  return MRT_OK;
-  MRT_GUARD_END
-}
-
 int mrt_scene_set_lights(mrt_scene sh, const mrt_quad_light* lights, size_t n) {
   MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);

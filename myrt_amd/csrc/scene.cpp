@@ -254,7 +254,8 @@ int Scene::upload_all() {
     sdf_dirty = false;
   }
   if (shading_dirty) {
-    std::vector<Material> mats = materials;
+    std::vector<Material> mats(this->mats.materials.size());
+    for (size_t i = 0; i < mats.size(); ++i) mats[i] = this->mats.resolve(uint32_t(i));
     if (mats.empty()) {
       Material def{};
       def.albedo_rgba8 = 0xffccccccu;
@@ -316,7 +317,7 @@ PtScene Scene::shading_view() const {
   v.materials = d_materials;
   v.lights = d_lights;
   v.num_lights = uint32_t(lights.size());
-  v.num_materials = uint32_t(std::max<size_t>(materials.size(), 1));
+  v.num_materials = uint32_t(std::max<size_t>(mats.materials.size(), 1));
   v.env_scale = env_scale;
   return v;
 }

@@ -7,6 +7,7 @@
 
 #include "bvh_build.hpp"
 #include "device_types.hpp"
+#include "material_layout.hpp"
 #include "pathtrace.hpp"
 #include "sdf.hpp"
 #include "wide_bvh.hpp"
@@ -67,8 +68,9 @@ struct Scene {
   std::vector<Instance> current;  // the prepared frame
   bool geoms_dirty = false, inst_dirty = false, prepared = false;
 
-  // Shading data: materials (material_buffer, src/material.hpp:72-118), quad lights, environment.
-  std::vector<Material> materials;
+  // Shading data: material layouts and parameters (material_buffer, src/material.hpp:72-118),
+  // albedo textures, quad lights, environment.
+  MaterialStore mats;
   std::vector<TexInfo> textures;   // albedo textures (mrt_scene_push_texture), ids from 1
   std::vector<float4> texels;
   std::vector<QuadLight> lights;
