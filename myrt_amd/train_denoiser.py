@@ -102,17 +102,28 @@ def main(argv=None):
                          "sqrt(spp) so that the smaller errors of the cleaner inputs weigh as much "
                          "in the L1 loss as the noisy ones")
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--max-seconds", type=float, default=0.0,
+                    help="wall-clock budget: rendering stops at 60 %% of it (fewer training scenes) and "
+                         "training steps shrink to fit the rest; 0 = no limit")
     args = ap.parse_args(argv)
 
     t0 = time.time()
     data = []
-    for k in range(args.scenes + args.val):
-        sc = scenes.training_scene(k)
+    # Validation scenes first (seeds 0 .. val-1, never trained on), then training scenes.
+    val = []
+    for k in range(args.val):
+        inputs, target = render_pairs(scenes.training_scene(k), args.res, args.target_spp)
+        val.append((inputs.astype(np.float16), target))
+    for k in range(args.scenes):
+        if args.max_seconds and time.time() - t0 > 0.6 * args.max_seconds:
+            log("render budget spent: %d training scenes" % k)
+            break
+        sc = scenes.training_scene(args.val + k)
         inputs, target = render_pairs(sc, args.res, args.target_spp)
-        data.append((inputs, target))
-        if k % 4 == 3 or k == args.scenes + args.val - 1:
-            log("rendered %d / %d scenes (%.0f s)" % (k + 1, args.scenes + args.val, time.time() - t0))
-    train, val = data[:args.scenes], data[args.scenes:]
+        data.append((inputs.astype(np.float16), target))  # float16 inputs: half the host memory
+        if k % 8 == 7 or k == args.scenes - 1:
+            log("rendered %d / %d scenes (%.0f s)" % (k + 1, args.scenes, time.time() - t0))
+    train = data
 
     d = m.Denoiser(seed=args.seed)
     rng = np.random.default_rng(args.seed)
@@ -121,8 +132,16 @@ def main(argv=None):
     B, C, R = args.batch, args.crop, args.res
     losses = []
     t1 = time.time()
+    steps = args.steps
     for step in range(args.steps):
-        lr = args.lr * (0.02 + 0.98 * 0.5 * (1 + np.cos(np.pi * step / args.steps)))  # cosine decay to 2 %
+        if step == 200 and args.max_seconds:
+            per_step = (time.time() - t1) / 200
+            left = args.max_seconds - (time.time() - t0) - 60.0  # validation and saving
+            steps = max(400, min(args.steps, int(left / per_step)))
+            log("%.1f ms per step: training %d steps" % (per_step * 1e3, steps))
+        if step >= steps:
+            break
+        lr = args.lr * (0.02 + 0.98 * 0.5 * (1 + np.cos(np.pi * step / steps)))  # cosine decay to 2 %
         if step % 50 == 0:
             d.set_adam(lr=float(lr))
         raw = np.empty((B, C, C, 10), np.float32)
@@ -131,24 +150,26 @@ def main(argv=None):
             inputs, target = train[int(rng.integers(len(train)))]
             lvl = int(rng.choice(len(INPUT_SPP), p=level_p))
             y, x = int(rng.integers(R - C + 1)), int(rng.integers(R - C + 1))
-            rr, tt = augment(inputs[lvl, y:y + C, x:x + C], target[y:y + C, x:x + C], rng)
+            rr, tt = augment(inputs[lvl, y:y + C, x:x + C].astype(np.float32), target[y:y + C, x:x + C], rng)
             raw[b], tgt[b] = rr, tt
         losses.append(d.train(raw, tgt))
-        if step % 250 == 249 or step == args.steps - 1:
+        if step % 250 == 249 or step == steps - 1:
             log("step %d loss %.4f lr %.2e (%.0f s)" % (step + 1, float(np.mean(losses[-250:])), lr, time.time() - t1))
 
     # Validation: MSE of the noisy input and of the denoised image against the target, per spp.
     report = {}
     for i, spp in enumerate(INPUT_SPP):
-        noisy = [mse(inp[i, ..., :3], tgt) for inp, tgt in val]
-        den = [mse(d.denoise(inp[i])[..., :3], tgt) for inp, tgt in val]
+        noisy = [mse(inp[i, ..., :3].astype(np.float32), tgt) for inp, tgt in val]
+        den = [mse(d.denoise(inp[i].astype(np.float32))[..., :3], tgt) for inp, tgt in val]
         report[spp] = {"noisy_mse": float(np.mean(noisy)), "denoised_mse": float(np.mean(den)),
                        "gain": float(np.mean(noisy) / max(np.mean(den), 1e-12))}
         log("validation %2d spp: noisy MSE %.4e, denoised %.4e (x%.1f)" % (
             spp, report[spp]["noisy_mse"], report[spp]["denoised_mse"], report[spp]["gain"]))
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     d.save(args.out)
-    meta = {"args": vars(args), "input_spp": INPUT_SPP, "final_loss": float(np.mean(losses[-100:])),
+    meta = {"args": vars(args), "train_scenes": len(train), "train_seeds": [args.val, args.val + len(train) - 1],
+            "validation_seeds": [0, args.val - 1], "steps": steps, "input_spp": INPUT_SPP,
+            "final_loss": float(np.mean(losses[-100:])),
             "validation": report, "render_s": round(t1 - t0, 1), "train_s": round(time.time() - t1, 1)}
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
         json.dump(meta, f, indent=1)

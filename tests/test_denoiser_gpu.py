@@ -196,12 +196,13 @@ def test_denoise_rendered_image(ensure_built):
 
 
 def test_trained_network_gains_on_a_held_out_scene(ensure_built):
-    """The shipped weights (myrt_amd/data/denoiser_unet.npy, trained on training scenes 0-131) on
-    training_scene(500), which they never saw: denoising a 1-spp render cuts the MSE against a
-    1024-spp render of the same scene at least 4x, a 16-spp render at least 2x."""
+    """The shipped weights (myrt_amd/data/denoiser_unet.npy, trained on training_scene seeds
+    4-515 at 512^2, as myrt_amd/data/denoiser_unet.json records) on training_scene(100000), which
+    they never saw: denoising a 1-spp render cuts the MSE against a 1024-spp render of the same scene at
+    least 4x, a 16-spp render at least 2x."""
     import os
     weights = os.path.join(os.path.dirname(m.__file__), "data", "denoiser_unet.npy")
-    sc = scenes.training_scene(500)
+    sc = scenes.training_scene(100000)
     s = m.Scene()
     gids = [s.push_geometry(i, p, n) for i, p, n in sc["geometries"]]
     for mat in sc["materials"]:
