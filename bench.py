@@ -58,6 +58,52 @@ def build_wavefront_scene(sc, with_sdf=True):
     return s
 
 
+def animated_frames(s, sc, d_r, n_rays, d_h, frames=5):
+    """Frames that move every instance of the wavefront scene (a random translation of about 0.02
+    scene units per axis and frame), prepared with a host rebuild of the instance BVH (the
+    reference's behaviour on a changed drawable hash) and with the GPU refit; the device-matrix
+    update alone; closest-hit throughput of the refitted TLAS after all frames."""
+    import myrt_amd as m
+    rng = np.random.default_rng(9)
+    inst = [(g, np.asarray(mt, np.float32), mi) for g, mt, mi in sc["instances"]]
+    out = {"instances": len(inst), "frames": frames}
+
+    def step_frame():
+        for k, (g, mt, mi) in enumerate(inst):
+            mt = mt.copy()
+            mt[:3, 3] += rng.normal(size=3).astype(np.float32) * 0.02
+            inst[k] = (g, mt, mi)
+            s.enqueue(g, mt, mi)
+
+    step = lambda: s.trace_closest_device(d_r, n_rays, d_h)
+    # Refits first; the closest-hit rate on the refitted TLAS is then compared with the rate on a
+    # TLAS rebuilt one frame later (the same rays; the scene has moved by one more small step).
+    for mode in ("refit", "rebuild"):
+        s.set_tlas_refit(0 if mode == "rebuild" else 1 << 30)
+        times, refitted = [], 0
+        for _ in range(frames):
+            step_frame()
+            m.synchronize()
+            t0 = time.perf_counter()
+            p = s.prepare()
+            m.synchronize()
+            times.append(time.perf_counter() - t0)
+            refitted += p["tlas_refitted"]
+            if mode == "rebuild" or refitted < frames:
+                continue
+            mats = np.stack([np.ascontiguousarray(mt.T).ravel() for _, mt, _ in inst]).astype(np.float32)
+            d_m = m.DeviceBuffer.from_array(mats)
+            _, ms_dev = timed_passes(lambda: s.update_transforms_device(d_m, len(inst)), 5)
+            out["device_update_ms"] = round(ms_dev, 4)
+        out[mode + "_prepare_ms"] = round(1e3 * float(np.median(times)), 3)
+        out[mode + "_frames_refitted"] = int(refitted)
+        step()
+        _, ms = timed_passes(step, 3)
+        out["incoherent_closest_mrays_after_" + mode] = round(n_rays / (ms * 1e-3) / 1e6, 2)
+    s.set_tlas_refit(0)
+    return out
+
+
 def wavefront_extras(args):
     """The 4M-triangle wavefront scene (2000 instanced icospheres, displaced ground, 64 SDF
     primitives): incoherent closest-hit rays/s and path-tracer samples/s at 1920x1080, 16 spp, depth 8."""
@@ -107,6 +153,8 @@ def wavefront_extras(args):
     log("wavefront render, shading in material order")
     r.render(s, W, H, spp)
     dt_sorted = (r.stats()["milliseconds"] - st0["milliseconds"]) * 1e-3
+    log("wavefront scene: animated frames")
+    anim = animated_frames(s, sc, d_r, rays.size, d_h)
     return {"triangles": n_tris, "sdf_primitives": 64, "prepare_s": round(prep_s, 3),
             "wide_nodes": prep["blas_wide_nodes"] + prep["tlas_wide_nodes"], "device_bytes": prep["device_bytes"],
             "incoherent_closest_mrays_triangles_only": round(rays.size / (ms_tri * 1e-3) / 1e6, 2),
@@ -116,7 +164,8 @@ def wavefront_extras(args):
             "incoherent_instance_visits_per_ray": round(st["instance_visits"] / rays.size, 3),
             "render": {"width": W, "height": H, "spp": spp, "depth": depth, "seconds": round(dt, 4),
                        "samples_per_s": round(W * H * spp / dt, 1), "rays_traced": int(n_rays),
-                       "mrays_per_s": round(n_rays / dt / 1e6, 2), "seconds_sorted_shading": round(dt_sorted, 4)}}
+                       "mrays_per_s": round(n_rays / dt / 1e6, 2), "seconds_sorted_shading": round(dt_sorted, 4)},
+            "animated_frames": anim}
 
 
 def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 64), warm_frames=16):

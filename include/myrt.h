@@ -179,6 +179,8 @@ typedef struct mrt_prepare_result {
   uint32_t blas_tri_records;
   uint32_t tlas_wide_nodes;
   uint32_t max_stack_entries;
+  int32_t tlas_refitted; /* the frame moved instances only and the TLAS was refitted on the GPU */
+  uint32_t reserved;
   uint64_t device_bytes;
 } mrt_prepare_result;
 
@@ -241,6 +243,22 @@ int mrt_scene_enqueue_geometry(mrt_scene s, uint32_t geom, uint32_t material, co
  * dont_split over this frame's drawables, collapses every BVH to the 64-B wide format and uploads
  * it. Drawables are consumed, as in the reference; an empty frame keeps the previous one. */
 int mrt_scene_prepare(mrt_scene s, mrt_prepare_result* out);
+/* Refit instead of rebuild (no reference counterpart; the reference rebuilds its instance BVH on
+ * every drawable-hash change, src/pathtracer/scene.cpp:255-271, 336-344). With max_refits > 0 a
+ * prepare whose drawables differ from the prepared frame only in their matrices (same count, same
+ * geometry and material at each position) keeps the TLAS topology and refits its boxes on the GPU;
+ * after max_refits refits in a row the next change rebuilds. 0 (the default) rebuilds every time.
+ * Closest and any hits do not depend on the choice; traversal cost may grow as instances move. */
+int mrt_scene_set_tlas_refit(mrt_scene s, uint32_t max_refits);
+/* Moves the prepared frame's instances to n device-resident column-major 4x4 matrices (n = the
+ * prepared instance count, in enqueue order) and refits the TLAS, all on `stream` (a hipStream_t;
+ * NULL = the null stream) with no host synchronisation: for transforms produced on the GPU. The
+ * inverses and world boxes are computed on the device exactly as mrt_scene_enqueue_geometry and
+ * mrt_scene_prepare compute them on the host. A singular matrix makes its instance unhittable and
+ * sets bit 0 of mrt_scene_update_status. The next prepare re-evaluates the enqueued drawables. */
+int mrt_scene_update_transforms_device(mrt_scene s, const float* d_m4x4_colmajor, size_t n, void* stream);
+/* Synchronises and reports the last refit's status bits (bit 0: a singular matrix). */
+int mrt_scene_update_status(mrt_scene s, uint32_t* status);
 /* The instance inverse matrices of the prepared frame (row-major 3x4 floats, 12 per instance). */
 int mrt_scene_instance_inverses(mrt_scene s, float* out, size_t n_inst);
 /* The host half of mrt_scene_prepare (BVH builds and the wide collapse) without touching the GPU,
@@ -323,6 +341,21 @@ int mrt_scene_material_blocks(mrt_scene s, uint32_t* out, size_t n, size_t* tota
 /* scene::push_sdf(sdf_info_t{root}) (src/pathtracer/scene.cpp:189-204) with the tree as a postfix
  * program; the program is validated (stack discipline, depth <= 8, modifiers <= 4 deep). */
 int mrt_scene_push_sdf(mrt_scene s, const mrt_sdf_node* nodes, size_t n, uint32_t* out_sdf);
+/* scene::set_parameter(sdf, parameter, value) (src/pathtracer/scene.hpp:150-153; sdf.cpp:346-407,
+ * parameter.cpp:57-109): writes count floats into p[slot .. slot + count) of node `node` of a
+ * pushed program (its tape index, as pushed) without re-pushing it. The program's bounds and
+ * compiled form are updated at once; a program's culling grid is rebuilt on the GPU at the next
+ * mrt_scene_prepare, which is also when the change reaches the device (as in the reference, whose
+ * prepare uploads the parameter buffer). Values must be finite; an update whose bounds come out
+ * invalid is rejected and changes nothing. */
+int mrt_scene_set_sdf_param(mrt_scene s, uint32_t sdf, uint32_t node, uint32_t slot, const float* values,
+                            size_t count);
+/* A program's culling grid: dim cells per axis over its object-space bounds (0 = none, every
+ * instruction is evaluated), origin lo and scale inv (cells per unit), and, if masks is not NULL,
+ * its dim^3 cell masks (bit j: chain instruction j may change the distance in the cell).
+ * Synchronises with the device when the masks were rebuilt there. */
+int mrt_scene_copy_sdf_grid(mrt_scene s, uint32_t sdf, float lo[3], float inv[3], uint32_t* dim, uint64_t* masks,
+                            size_t n);
 /* scene::enqueue(sdf, mat4) (src/pathtracer/scene.hpp:147-148, scene.cpp:411-417). SDF hits come
  * back with inst = 0x80000000 | drawable index, prim = material and an octahedral normal in bary. */
 int mrt_scene_enqueue_sdf(mrt_scene s, uint32_t sdf, const float m4x4_colmajor[16], uint32_t* out_drawable);

@@ -53,6 +53,7 @@ class _PrepareResult(ctypes.Structure):
                 ("num_geometries", ctypes.c_uint32), ("num_instances", ctypes.c_uint32),
                 ("blas_wide_nodes", ctypes.c_uint32), ("blas_tri_records", ctypes.c_uint32),
                 ("tlas_wide_nodes", ctypes.c_uint32), ("max_stack_entries", ctypes.c_uint32),
+                ("tlas_refitted", ctypes.c_int32), ("reserved", ctypes.c_uint32),
                 ("device_bytes", ctypes.c_uint64)]
 
 
@@ -98,6 +99,9 @@ SIGNATURES = {
     "mrt_scene_enqueue_geometry": ([_P, ctypes.c_uint32, ctypes.c_uint32, _P, _U32P], ctypes.c_int),
     "mrt_scene_prepare": ([_P, ctypes.POINTER(_PrepareResult)], ctypes.c_int),
     "mrt_scene_instance_inverses": ([_P, _P, _SZ], ctypes.c_int),
+    "mrt_scene_set_tlas_refit": ([_P, ctypes.c_uint32], ctypes.c_int),
+    "mrt_scene_update_transforms_device": ([_P, _P, _SZ, _P], ctypes.c_int),
+    "mrt_scene_update_status": ([_P, _U32P], ctypes.c_int),
     "mrt_scene_build_host": ([_P, ctypes.POINTER(_PrepareResult)], ctypes.c_int),
     "mrt_scene_copy_wide": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
     "mrt_trace_closest": ([_P, _P, _SZ, _P], ctypes.c_int),
@@ -128,6 +132,8 @@ SIGNATURES = {
     "mrt_scene_push_sdf": ([_P, _P, _SZ, _U32P], ctypes.c_int),
     "mrt_scene_enqueue_sdf": ([_P, ctypes.c_uint32, _P, _U32P], ctypes.c_int),
     "mrt_scene_set_sdf_marching": ([_P, ctypes.c_int, ctypes.c_float], ctypes.c_int),
+    "mrt_scene_set_sdf_param": ([_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, _SZ], ctypes.c_int),
+    "mrt_scene_copy_sdf_grid": ([_P, ctypes.c_uint32, _P, _P, _U32P, _P, _SZ], ctypes.c_int),
     "mrt_scene_sdf_drawables": ([_P, _P, _SZ], ctypes.c_int),
     "mrt_renderer_create": ([ctypes.POINTER(_P)], ctypes.c_int),
     "mrt_renderer_destroy": ([_P], ctypes.c_int),
@@ -403,6 +409,19 @@ class Scene:
         _check(lib().mrt_scene_copy_wide(self.h, _ptr(blas), _ptr(tris), _ptr(tlas), _ptr(items), _ptr(inst)))
         return blas, tris, tlas, items[:n_items], inst[:p["num_instances"]]
 
+    def set_tlas_refit(self, max_refits):
+        """Refit the TLAS on the GPU for frames that only move instances (0: always rebuild)."""
+        _check(lib().mrt_scene_set_tlas_refit(self.h, int(max_refits)))
+
+    def update_transforms_device(self, d_mats, n, stream=None):
+        """Move the prepared instances to n device-resident column-major 4x4 matrices."""
+        _check(lib().mrt_scene_update_transforms_device(self.h, d_mats.ptr, n, stream))
+
+    def update_status(self):
+        out = ctypes.c_uint32()
+        _check(lib().mrt_scene_update_status(self.h, ctypes.byref(out)))
+        return out.value
+
     def instance_inverses(self, n):
         out = np.zeros((max(n, 1), 12), np.float32)
         _check(lib().mrt_scene_instance_inverses(self.h, _ptr(out), max(n, 1)))
@@ -587,6 +606,25 @@ def _scene_enqueue_sdf(self, sdf, transformation=None):
     return out.value
 
 
+def _scene_set_sdf_param(self, sdf, node, slot, values):
+    """scene::set_parameter(sdf, parameter, value): p[slot:slot + len(values)] of tape node `node`."""
+    v = np.atleast_1d(np.asarray(values, np.float32))
+    _check(lib().mrt_scene_set_sdf_param(self.h, int(sdf), int(node), int(slot), _ptr(v), v.size))
+
+
+def _scene_sdf_grid(self, sdf, with_masks=True):
+    """(dim, lo, inv, masks or None) of a program's culling grid."""
+    lo, inv = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    dim = ctypes.c_uint32()
+    _check(lib().mrt_scene_copy_sdf_grid(self.h, int(sdf), _ptr(lo), _ptr(inv), ctypes.byref(dim), None, 0))
+    masks = None
+    if with_masks and dim.value:
+        masks = np.zeros(dim.value ** 3, np.uint64)
+        _check(lib().mrt_scene_copy_sdf_grid(self.h, int(sdf), _ptr(lo), _ptr(inv), ctypes.byref(dim), _ptr(masks),
+                                             masks.size))
+    return dim.value, lo, inv, masks
+
+
 def _scene_set_sdf_marching(self, steps=400, eps=1e-5):
     _check(lib().mrt_scene_set_sdf_marching(self.h, int(steps), float(eps)))
 
@@ -624,6 +662,8 @@ Scene.sdf_drawables = _scene_sdf_drawables
 Scene.push_sdf = _scene_push_sdf
 Scene.enqueue_sdf = _scene_enqueue_sdf
 Scene.set_sdf_marching = _scene_set_sdf_marching
+Scene.set_sdf_param = _scene_set_sdf_param
+Scene.sdf_grid = _scene_sdf_grid
 Scene.push_material = _scene_push_material
 Scene.get_material = _scene_get_material
 Scene.set_parameter = _scene_set_material_param

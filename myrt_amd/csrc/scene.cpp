@@ -23,6 +23,8 @@
 #include "bvh_build.hpp"
 #include "device_types.hpp"
 #include "scene_internal.hpp"
+#include "refit.hpp"
+#include "sdf_grid.hpp"
 #include "sdf_kernels.hpp"
 #include "trace.hpp"
 #include "wide_bvh.hpp"
@@ -122,6 +124,29 @@ Box transform_box(const Box& b, const float m[16]) {
   return out;
 }
 
+// Wide TLAS nodes grouped by depth, deepest level first, for the level-by-level refit.
+void Scene::tlas_levels() {
+  std::vector<uint32_t> depth(tlas_nodes.size(), 0);
+  uint32_t max_depth = 0;
+  // Children are numbered after their parent, so one forward pass sets every depth.
+  for (uint32_t i = 0; i < tlas_nodes.size(); ++i) {
+    const WideNode& n = tlas_nodes[i];
+    for (int c = 0; c < n.nchildren; ++c)
+      if (n.count[c] == 0) {
+        const uint32_t j = n.node_base + n.offset[c];
+        depth[j] = depth[i] + 1;
+        max_depth = std::max(max_depth, depth[j]);
+      }
+  }
+  std::vector<uint32_t> per(max_depth + 1, 0);
+  for (uint32_t d : depth) ++per[d];
+  tlas_level_start.assign(max_depth + 2, 0u);
+  for (uint32_t l = 0; l <= max_depth; ++l) tlas_level_start[l + 1] = tlas_level_start[l] + per[max_depth - l];
+  tlas_level_nodes.assign(tlas_nodes.size(), 0u);
+  std::vector<uint32_t> fill(tlas_level_start.begin(), tlas_level_start.end() - 1);
+  for (uint32_t i = 0; i < tlas_nodes.size(); ++i) tlas_level_nodes[fill[max_depth - depth[i]]++] = i;
+}
+
 int Scene::build_host() {
   try {
     if (geoms_dirty) {
@@ -154,6 +179,10 @@ int Scene::build_host() {
         }
       }
       if (tris.size() >= kMaxItemIndex) return fail(MRT_ERR_LIMIT, "scene exceeds 16M triangle references");
+      geo_boxes.clear();
+      for (const Geometry& geo : geoms)
+        for (float v : {geo.box.mn.x, geo.box.mn.y, geo.box.mn.z, geo.box.mx.x, geo.box.mx.y, geo.box.mx.z})
+          geo_boxes.push_back(v);
       geo_info.clear();
       all_idx.clear();
       all_pos.clear();
@@ -192,6 +221,7 @@ int Scene::build_host() {
         tlas_depth = w.depth;
         for (uint32_t item : w.items) tlas_items.push_back(tlas.indices[item]);
         if (tlas_items.size() >= kMaxItemIndex) return fail(MRT_ERR_LIMIT, "scene exceeds 16M instance references");
+        tlas_levels();
         inst_material.clear();
         for (const Instance& in : current) inst_material.push_back(in.material);
         for (size_t i = 0; i < current.size(); ++i) {
@@ -203,6 +233,8 @@ int Scene::build_host() {
         }
       } else {
         tlas = BinaryBvh{};
+        tlas_level_nodes.clear();
+        tlas_level_start.assign(1, 0u);
       }
     }
     for (InstanceRecord& r : inst_records) r.blas_root = blas_root[r.geom];
@@ -224,6 +256,11 @@ int Scene::upload_all() {
     if ((rc = upload(d_tlas_items, cap_items, tlas_items)) != MRT_OK) return rc;
     if ((rc = upload(d_inst, cap_inst, inst_records)) != MRT_OK) return rc;
     if ((rc = upload(d_inst_material, cap_inst_mat, inst_material)) != MRT_OK) return rc;
+    if ((rc = upload(d_level_nodes, cap_level_nodes, tlas_level_nodes)) != MRT_OK) return rc;
+    tlas_host_stale = false;
+  }
+  if (geoms_dirty) {
+    if ((rc = upload(d_geo_box, cap_geo_box, geo_boxes)) != MRT_OK) return rc;
   }
   if (geoms_dirty) {
     if ((rc = upload(d_geo_info, cap_geo_info, geo_info)) != MRT_OK) return rc;
@@ -245,12 +282,34 @@ int Scene::upload_all() {
       grids[j] = sdf_grid[sdf_current[j].sdf];
       if (grids[j].dim == 0) sdf_chain_only = false;
     }
-    std::vector<uint64_t> masks = sdf_masks;
-    if (masks.empty()) masks.push_back(~0ull);
     if ((rc = upload(d_sdf_prog, cap_sdf_tape, sdf_prog)) != MRT_OK) return rc;
     if ((rc = upload(d_sdf_drawables, cap_sdf_dr, dev)) != MRT_OK) return rc;
     if ((rc = upload(d_sdf_grids, cap_sdf_grids, grids)) != MRT_OK) return rc;
-    if ((rc = upload(d_sdf_masks, cap_sdf_masks, masks)) != MRT_OK) return rc;
+    // Masks travel only when programs were pushed; grids of changed programs are rebuilt on the
+    // GPU in place.
+    if (sdf_masks_dirty || !d_sdf_masks) {
+      if ((rc = sync_host_masks()) != MRT_OK) return rc;
+      std::vector<uint64_t> masks = sdf_masks;
+      if (masks.empty()) masks.push_back(~0ull);
+      if ((rc = upload(d_sdf_masks, cap_sdf_masks, masks)) != MRT_OK) return rc;
+      sdf_masks_dirty = false;
+    }
+    if (!sdf_grid_rebuild.empty()) {
+      if (!d_sdf_coarse) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_sdf_coarse),
+                                 sizeof(uint64_t) * kSdfGridCoarse * kSdfGridCoarse * kSdfGridCoarse);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+      }
+      for (uint32_t id : sdf_grid_rebuild) {
+        const SdfGrid& g = sdf_grid[id];
+        if (!g.dim) continue;
+        if (launch_sdf_grid(d_sdf_prog + sdf_prog_offset[id], sdf_prog_len[id], sdf_bounds[id], d_sdf_coarse,
+                            d_sdf_masks + g.mask_offset, nullptr))
+          return fail(MRT_ERR_HIP, "sdf grid rebuild");
+        sdf_masks_host_stale = true;
+      }
+      sdf_grid_rebuild.clear();
+    }
     sdf_dirty = false;
   }
   if (shading_dirty) {
@@ -296,6 +355,10 @@ Scene::~Scene() {
     for (void* q : {(void*)c.ctrl, (void*)c.spill, (void*)c.sdf_side, (void*)c.sdf_hits})
       if (q) hipFree(q);
   for (void* q : {d_host_rays, d_host_out})
+    if (q) hipFree(q);
+  if (d_sdf_coarse) hipFree(d_sdf_coarse);
+  for (void* q : {(void*)d_level_nodes, (void*)d_geo_box, (void*)d_inst_box, (void*)d_node_box, (void*)d_matrices,
+                  (void*)d_refit_status})
     if (q) hipFree(q);
   if (d_tlas) hipFree(d_tlas);
   if (d_tlas_items) hipFree(d_tlas_items);
@@ -776,6 +839,14 @@ static void build_sdf_drawables(Scene* s) {
   s->sdf_dirty = true;
 }
 
+// Same drawables but for their matrices: the prepared TLAS topology still fits.
+static bool same_but_transforms(const std::vector<Instance>& a, const std::vector<Instance>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); ++i)
+    if (a[i].geom != b[i].geom || a[i].material != b[i].material) return false;
+  return true;
+}
+
 static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
   if (!s->sdf_pending.empty()) {
     s->sdf_current = std::move(s->sdf_pending);
@@ -783,20 +854,36 @@ static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
     build_sdf_drawables(s);
   }
   // The instance BVH is rebuilt only when this frame's drawables differ from the prepared ones
-  // (the reference compares drawable hashes, scene.cpp:336-344).
+  // (the reference compares drawable hashes, scene.cpp:336-344). When only matrices differ and
+  // refits are enabled, the prepared TLAS is refitted on the GPU instead.
+  bool refit = false;
   if (!s->pending.empty()) {
-    if (!s->prepared || !same_drawables(s->pending, s->current)) {
+    if (!s->prepared || s->device_newer || !same_drawables(s->pending, s->current)) {
+      refit = upload_too && s->prepared && !s->geoms_dirty && !s->inst_dirty && s->refits_done < s->refit_limit &&
+              !s->tlas_nodes.empty() && same_but_transforms(s->pending, s->current);
       s->current = std::move(s->pending);
-      s->inst_dirty = true;
+      if (!refit) s->inst_dirty = true;
     }
     s->pending.clear();
   }
-  const bool gd = s->geoms_dirty, id = s->inst_dirty || s->sdf_dirty;
+  const bool gd = s->geoms_dirty, id = s->inst_dirty || s->sdf_dirty || refit;
+  if (s->inst_dirty) s->refits_done = 0;
   int rc = s->build_host();
   if (rc != MRT_OK) return rc;
   if (upload_too) {
     rc = s->upload_all();
     if (rc != MRT_OK) return rc;
+    if (refit) {
+      std::vector<float> mats(s->current.size() * 16);
+      for (size_t i = 0; i < s->current.size(); ++i) std::memcpy(&mats[16 * i], s->current[i].m, 64);
+      if ((rc = upload(s->d_matrices, s->cap_matrices, mats)) != MRT_OK) return rc;
+      if ((rc = s->refit(s->d_matrices, nullptr)) != MRT_OK) return rc;
+      // The host already holds these frames' inverses (computed at enqueue, bit-equal to the
+      // device's); the refitted nodes are read back only if asked for.
+      for (size_t i = 0; i < s->current.size(); ++i) std::memcpy(s->inst_records[i].inv, s->current[i].inv, 48);
+      s->device_newer = false;
+      ++s->refits_done;
+    }
     s->geoms_dirty = false;
     s->inst_dirty = false;
     s->prepared = true;
@@ -805,6 +892,7 @@ static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
     std::memset(out, 0, sizeof(*out));
     out->geometries_changed = gd ? 1 : 0;
     out->drawables_changed = id ? 1 : 0;
+    out->tlas_refitted = refit ? 1 : 0;
     out->num_geometries = uint32_t(s->geoms.size());
     out->num_instances = uint32_t(s->current.size());
     out->blas_wide_nodes = uint32_t(s->blas_nodes.size());
@@ -815,6 +903,71 @@ static int prepare_impl(Scene* s, mrt_prepare_result* out, bool upload_too) {
                         s->tlas_nodes.size() * sizeof(WideNode) + s->tlas_items.size() * 4 +
                         s->inst_records.size() * sizeof(InstanceRecord);
   }
+  return MRT_OK;
+}
+
+// Brings the host copy of the culling masks up to the GPU-rebuilt ones.
+int Scene::sync_host_masks() {
+  if (!sdf_masks_host_stale) return MRT_OK;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess && !sdf_masks.empty())
+    e = hipMemcpy(sdf_masks.data(), d_sdf_masks, sdf_masks.size() * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "reading the sdf culling masks");
+  sdf_masks_host_stale = false;
+  return MRT_OK;
+}
+
+int Scene::refit(const float* d_mats, hipStream_t stream) {
+  const size_t n = current.size();
+  int rc;
+  auto grow = [](float*& p, size_t& cap, size_t bytes) -> int {
+    if (p && cap >= bytes) return MRT_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    cap = bytes;
+    return MRT_OK;
+  };
+  if ((rc = grow(d_inst_box, cap_inst_box, std::max<size_t>(n, 1) * 24)) != MRT_OK) return rc;
+  if ((rc = grow(d_node_box, cap_node_box, std::max<size_t>(tlas_nodes.size(), 1) * 24)) != MRT_OK) return rc;
+  if (!d_refit_status) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_refit_status), 4);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  }
+  if (launch_fill_u32(d_refit_status, 1, 0u, stream)) return fail(MRT_ERR_HIP, "refit status clear");
+  RefitParams p{};
+  p.matrices = d_mats;
+  p.n_inst = uint32_t(n);
+  p.inst = d_inst;
+  p.geo_box = d_geo_box;
+  p.inst_box = d_inst_box;
+  p.tlas = d_tlas;
+  p.items = d_tlas_items;
+  p.level_nodes = d_level_nodes;
+  p.node_box = d_node_box;
+  p.status = d_refit_status;
+  p.pad_rel = kTlasPadRel;
+  if (launch_refit(p, tlas_level_start.data(), uint32_t(tlas_level_start.size() - 1), stream))
+    return fail(MRT_ERR_HIP, "refit launch");
+  tlas_host_stale = true;
+  return MRT_OK;
+}
+
+// Brings tlas_nodes and inst_records up to the device after refits.
+int Scene::sync_host_tlas() {
+  if (!tlas_host_stale) return MRT_OK;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess && !tlas_nodes.empty())
+    e = hipMemcpy(tlas_nodes.data(), d_tlas, tlas_nodes.size() * sizeof(WideNode), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && !inst_records.empty())
+    e = hipMemcpy(inst_records.data(), d_inst, inst_records.size() * sizeof(InstanceRecord), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "reading the refitted instance BVH");
+  if (device_newer)
+    for (size_t i = 0; i < current.size() && i < inst_records.size(); ++i)
+      std::memcpy(current[i].inv, inst_records[i].inv, 48);
+  tlas_host_stale = false;
   return MRT_OK;
 }
 
@@ -967,197 +1120,6 @@ static void compile_sdf(const SdfNode* t, size_t n, std::vector<SdfInsn>& out) {
   }
 }
 
-namespace {
-
-// Exact signed distance (double) and object-space bounding box of a chain primitive.
-double chain_prim_distance(const SdfInsn& in, double x, double y, double z) {
-  if (in.op & kSdfFuseTranslate) {
-    x -= in.t[0];
-    y -= in.t[1];
-    z -= in.t[2];
-  }
-  switch (in.op & 0xffu) {
-    case kSdfSphere:
-      return std::sqrt(x * x + y * y + z * z) - in.p[0];
-    case kSdfBox: {
-      const double qx = std::fabs(x) - in.p[0], qy = std::fabs(y) - in.p[1], qz = std::fabs(z) - in.p[2];
-      const double ox = std::max(qx, 0.0), oy = std::max(qy, 0.0), oz = std::max(qz, 0.0);
-      return std::sqrt(ox * ox + oy * oy + oz * oz) + std::min(std::max(qx, std::max(qy, qz)), 0.0);
-    }
-    case kSdfTorus: {
-      const double qx = std::sqrt(x * x + z * z) - in.p[0];
-      return std::sqrt(qx * qx + y * y) - in.p[1];
-    }
-    default: {  // capsule
-      const double yy = y - std::min(std::max(y, 0.0), double(in.p[0]));
-      return std::sqrt(x * x + yy * yy + z * z) - in.p[1];
-    }
-  }
-}
-
-void chain_prim_box(const SdfInsn& in, double lo[3], double hi[3]) {
-  double h[3], c[3] = {0, 0, 0};
-  switch (in.op & 0xffu) {
-    case kSdfSphere:
-      h[0] = h[1] = h[2] = in.p[0];
-      break;
-    case kSdfBox:
-      for (int a = 0; a < 3; ++a) h[a] = std::fabs(in.p[a]);
-      break;
-    case kSdfTorus:
-      h[0] = h[2] = double(in.p[0]) + in.p[1];
-      h[1] = in.p[1];
-      break;
-    default:  // capsule: y in [-r, height + r]
-      h[0] = h[2] = in.p[1];
-      h[1] = 0.5 * in.p[0] + in.p[1];
-      c[1] = 0.5 * in.p[0];
-      break;
-  }
-  for (int a = 0; a < 3; ++a) {
-    const double t = (in.op & kSdfFuseTranslate) ? in.t[a] : 0.0;
-    lo[a] = t + c[a] - h[a];
-    hi[a] = t + c[a] + h[a];
-  }
-}
-
-// Primitives whose distance is exact (1-Lipschitz and at least the distance to their box).
-bool chain_prim_ok(const SdfInsn& in) {
-  switch (in.op & 0xffu) {
-    case kSdfSphere:
-      return in.p[0] > 0.f;
-    case kSdfBox:
-      return true;
-    case kSdfTorus:
-      return in.p[0] >= 0.f && in.p[1] > 0.f;
-    case kSdfCapsule:
-      return in.p[0] >= 0.f && in.p[1] > 0.f;
-    default:
-      return false;
-  }
-}
-
-// Mask of one cell [clo, chi] (slack included): bit j is set unless primitive j provably leaves
-// the running value unchanged everywhere in the cell. Only the primitives in `candidates` are
-// considered; the others are known to drop over the cell. A dropped primitive never lowers the
-// bound on the running value (its distance anywhere in the cell exceeds the bound), so leaving the
-// non-candidates out of the bound changes no mask.
-uint64_t cell_mask(const SdfInsn* prog, uint32_t n, const double* blo, const double* bhi, const double clo[3],
-                   const double chi[3], uint64_t candidates) {
-  double ctr[3], half2 = 0;
-  for (int a = 0; a < 3; ++a) {
-    ctr[a] = 0.5 * (clo[a] + chi[a]);
-    half2 += 0.25 * (chi[a] - clo[a]) * (chi[a] - clo[a]);
-  }
-  const double half = std::sqrt(half2);
-  uint64_t mask = 1;
-  double upper = chain_prim_distance(prog[0], ctr[0], ctr[1], ctr[2]) + half;
-  for (uint32_t j = 1; j < n; ++j) {
-    if (!((candidates >> j) & 1u)) continue;
-    {
-      double gap2 = 0;
-      bool overlap = true;
-      for (int a = 0; a < 3; ++a) {
-        const double gap = std::max(0.0, std::max(blo[3 * j + a] - chi[a], clo[a] - bhi[3 * j + a]));
-        if (gap > 0) overlap = false;
-        gap2 += gap * gap;
-      }
-      const bool smooth = ((prog[j].op >> 8) & 0xffu) == kSdfSmoothUnion;
-      const double margin = (smooth ? double(prog[j].k) * (1.0 + 1e-3) : 0.0) + 1e-4 * (1.0 + std::fabs(upper));
-      bool drop = !overlap && std::sqrt(gap2) > upper + margin;
-      if ((prog[j].op & 0xffu) == kSdfSphere) {
-        // Tighter and valid everywhere: |p - c| - r >= (distance from the cell to c) - r.
-        double c2 = 0;
-        for (int a = 0; a < 3; ++a) {
-          const double c = (prog[j].op & kSdfFuseTranslate) ? double(prog[j].t[a]) : 0.0;
-          const double gap = std::max(0.0, std::max(c - chi[a], clo[a] - c));
-          c2 += gap * gap;
-        }
-        drop = std::sqrt(c2) - double(prog[j].p[0]) > upper + margin;
-      }
-      if (!drop) mask |= uint64_t(1) << j;
-    }
-    upper = std::min(upper, chain_prim_distance(prog[j], ctr[0], ctr[1], ctr[2]) + half);
-  }
-  return mask;
-}
-
-// Cells per axis of a chain's grid: finer for longer chains (more primitives to cull), 2 MiB of
-// masks at 64 and 16 MiB at 128.
-int sdf_grid_dim(uint32_t n) { return n <= 8 ? 32 : (n <= 24 ? 64 : 128); }
-
-// Builds the culling grid of a compiled program if it is one union chain of exact primitives.
-// A coarse grid of 32 cells per axis comes first; each fine cell then tests only the bits of the
-// coarse cell that contains it, since a primitive that provably drops over a cell drops over every
-// part of it. Slabs of cells are built in parallel.
-void build_sdf_grid(const SdfInsn* prog, uint32_t n, const Box& box, SdfGrid& g, std::vector<uint64_t>& masks) {
-  g = SdfGrid{};
-  if (n < 2 || n > uint32_t(kSdfChainMax)) return;
-  if (!chain_prim_ok(prog[0]) || (prog[0].op & ~(0xffu | kSdfFuseTranslate)) != 0) return;
-  for (uint32_t j = 1; j < n; ++j) {
-    const uint32_t fused = (prog[j].op >> 8) & 0xffu;
-    if (!chain_prim_ok(prog[j]) || (prog[j].op >> 17) != 0) return;
-    if (fused != kSdfUnion && !(fused == kSdfSmoothUnion && prog[j].k > 0.f)) return;
-  }
-  double ext[3];
-  for (int a = 0; a < 3; ++a) {
-    ext[a] = double(box.mx[a]) - double(box.mn[a]);
-    if (!(ext[a] > 0.0) || !std::isfinite(ext[a])) return;
-  }
-  std::vector<double> blo(3 * n), bhi(3 * n);
-  for (uint32_t j = 0; j < n; ++j) chain_prim_box(prog[j], &blo[3 * j], &bhi[3 * j]);
-  const int D = sdf_grid_dim(n), C = kSdfGridCoarse, R = D / C;
-  // Cell [lo, hi] of a grid with `dim` cells per axis, widened by a slack: the cell the device
-  // computes in binary32 may reach a little past the exact one.
-  auto cell_box = [&](int dim, const int ci[3], double clo[3], double chi[3]) {
-    for (int a = 0; a < 3; ++a) {
-      const double size = ext[a] / dim, slack = 1e-4 * ext[a];
-      clo[a] = double(box.mn[a]) + size * ci[a] - slack;
-      chi[a] = double(box.mn[a]) + size * (ci[a] + 1) + slack;
-    }
-  };
-  const uint64_t all = n >= 64 ? ~0ull : ((uint64_t(1) << n) - 1);
-  std::vector<uint64_t> coarse(size_t(C) * C * C), fine(size_t(D) * D * D);
-  auto run_slabs = [](int slabs, const std::function<void(int)>& body) {
-    const int threads = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
-    std::atomic<int> next{0};
-    std::vector<std::thread> pool;
-    for (int t = 0; t < threads; ++t)
-      pool.emplace_back([&] {
-        for (int z; (z = next.fetch_add(1)) < slabs;) body(z);
-      });
-    for (auto& th : pool) th.join();
-  };
-  run_slabs(C, [&](int cz) {
-    for (int cy = 0; cy < C; ++cy)
-      for (int cx = 0; cx < C; ++cx) {
-        const int ci[3] = {cx, cy, cz};
-        double clo[3], chi[3];
-        cell_box(C, ci, clo, chi);
-        coarse[(size_t(cz) * C + cy) * C + cx] = cell_mask(prog, n, blo.data(), bhi.data(), clo, chi, all);
-      }
-  });
-  run_slabs(D, [&](int cz) {
-    for (int cy = 0; cy < D; ++cy)
-      for (int cx = 0; cx < D; ++cx) {
-        const int ci[3] = {cx, cy, cz};
-        double clo[3], chi[3];
-        cell_box(D, ci, clo, chi);
-        const uint64_t parent = coarse[(size_t(cz / R) * C + cy / R) * C + cx / R];
-        fine[(size_t(cz) * D + cy) * D + cx] = R == 1 ? parent : cell_mask(prog, n, blo.data(), bhi.data(), clo, chi, parent);
-      }
-  });
-  g.dim = uint32_t(D);
-  g.mask_offset = uint32_t(masks.size());
-  for (int a = 0; a < 3; ++a) {
-    g.lo[a] = box.mn[a];
-    g.inv[a] = float(double(D) / ext[a]);
-  }
-  masks.insert(masks.end(), fine.begin(), fine.end());
-}
-
-}  // namespace
-
 int mrt_scene_push_sdf(mrt_scene sh, const mrt_sdf_node* nodes, size_t n, uint32_t* out) {
   MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
@@ -1175,10 +1137,91 @@ int mrt_scene_push_sdf(mrt_scene sh, const mrt_sdf_node* nodes, size_t n, uint32
   compile_sdf(t, n, s->sdf_prog);
   s->sdf_prog_len.push_back(uint32_t(s->sdf_prog.size()) - s->sdf_prog_offset.back());
   SdfGrid g;
+  int rc = s->sync_host_masks();
+  if (rc != MRT_OK) return rc;
+  const size_t masks_before = s->sdf_masks.size();
   build_sdf_grid(s->sdf_prog.data() + s->sdf_prog_offset.back(), s->sdf_prog_len.back(), b, g, s->sdf_masks);
   s->sdf_grid.push_back(g);
+  if (s->sdf_masks.size() != masks_before) s->sdf_masks_dirty = true;
   s->sdf_dirty = true;
   if (out) *out = uint32_t(s->sdf_len.size() - 1);
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_set_sdf_param(mrt_scene sh, uint32_t sdf, uint32_t node, uint32_t slot, const float* values,
+                            size_t count) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || (count && !values)) return fail(MRT_ERR_INVALID, "null argument");
+  if (sdf >= s->sdf_len.size()) return fail(MRT_ERR_INVALID, "unknown sdf");
+  const uint32_t n = s->sdf_len[sdf];
+  if (node >= n) return fail(MRT_ERR_INVALID, "node index out of range");
+  if (count == 0 || slot >= 6 || count > 6 - size_t(slot)) return fail(MRT_ERR_INVALID, "parameter slots out of range");
+  for (size_t k = 0; k < count; ++k)
+    if (!std::isfinite(values[k])) return fail(MRT_ERR_INVALID, "sdf parameters must be finite");
+  SdfNode* t = s->sdf_tape.data() + s->sdf_offset[sdf];
+  const SdfNode old = t[node];
+  for (size_t k = 0; k < count; ++k) t[node].p[slot + k] = values[k];
+  Box b;
+  std::string err;
+  if (!sdf_bounds(t, n, b, err)) {
+    t[node] = old;
+    return fail(MRT_ERR_INVALID, "invalid sdf program: " + err);
+  }
+  std::vector<SdfInsn> prog;
+  compile_sdf(t, n, prog);
+  if (prog.size() != s->sdf_prog_len[sdf]) {
+    t[node] = old;
+    return fail(MRT_ERR_INTERNAL, "recompiled program changed length");
+  }
+  std::copy(prog.begin(), prog.end(), s->sdf_prog.begin() + s->sdf_prog_offset[sdf]);
+  s->sdf_bounds[sdf] = b;
+  SdfGrid& g = s->sdf_grid[sdf];
+  const uint32_t np = uint32_t(prog.size());
+  auto& rb = s->sdf_grid_rebuild;
+  rb.erase(std::remove(rb.begin(), rb.end(), sdf), rb.end());
+  if (!sdf_grid_applies(prog.data(), np, b)) {
+    g.dim = 0;
+  } else if (g.dim == uint32_t(sdf_grid_dim(np))) {
+    const uint32_t off = g.mask_offset;
+    sdf_grid_frame(b, np, g);
+    g.mask_offset = off;
+    rb.push_back(sdf);  // the masks are rebuilt on the GPU at the next prepare
+  } else {
+    int rc = s->sync_host_masks();
+    if (rc != MRT_OK) return rc;
+    build_sdf_grid(prog.data(), np, b, g, s->sdf_masks);
+    s->sdf_masks_dirty = true;
+  }
+  bool used = false;
+  for (const SdfInstance& in : s->sdf_current) used = used || in.sdf == sdf;
+  if (used) build_sdf_drawables(s);
+  s->sdf_dirty = true;
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_copy_sdf_grid(mrt_scene sh, uint32_t sdf, float lo[3], float inv[3], uint32_t* dim, uint64_t* masks,
+                            size_t n) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !dim) return fail(MRT_ERR_INVALID, "null argument");
+  if (sdf >= s->sdf_len.size()) return fail(MRT_ERR_INVALID, "unknown sdf");
+  const SdfGrid& g = s->sdf_grid[sdf];
+  *dim = g.dim;
+  for (int a = 0; a < 3; ++a) {
+    if (lo) lo[a] = g.lo[a];
+    if (inv) inv[a] = g.inv[a];
+  }
+  if (!masks || !g.dim) return MRT_OK;
+  const size_t cells = size_t(g.dim) * g.dim * g.dim;
+  if (n < cells) return fail(MRT_ERR_INVALID, "mask buffer too small");
+  if (std::find(s->sdf_grid_rebuild.begin(), s->sdf_grid_rebuild.end(), sdf) != s->sdf_grid_rebuild.end())
+    return fail(MRT_ERR_NOT_PREPARED, "the grid is rebuilt at the next prepare");
+  const int rc = s->sync_host_masks();
+  if (rc != MRT_OK) return rc;
+  std::memcpy(masks, s->sdf_masks.data() + g.mask_offset, cells * sizeof(uint64_t));
   return MRT_OK;
   MRT_GUARD_END
 }
@@ -1231,6 +1274,8 @@ int mrt_scene_instance_inverses(mrt_scene sh, float* out, size_t n_inst) {
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s || !out) return fail(MRT_ERR_INVALID, "null argument");
   if (n_inst < s->current.size()) return fail(MRT_ERR_INVALID, "output too small");
+  const int rc = s->sync_host_tlas();
+  if (rc != MRT_OK) return rc;
   for (size_t i = 0; i < s->current.size(); ++i) std::memcpy(out + 12 * i, s->current[i].inv, 48);
   return MRT_OK;
   MRT_GUARD_END
@@ -1249,6 +1294,8 @@ int mrt_scene_copy_wide(mrt_scene sh, void* blas_nodes, void* tris, void* tlas_n
   MRT_GUARD_BEGIN
   Scene* s = reinterpret_cast<Scene*>(sh);
   if (!s) return fail(MRT_ERR_INVALID, "null scene");
+  const int rc = s->sync_host_tlas();
+  if (rc != MRT_OK) return rc;
   if (blas_nodes) std::memcpy(blas_nodes, s->blas_nodes.data(), s->blas_nodes.size() * sizeof(WideNode));
   if (tris) std::memcpy(tris, s->tris.data(), s->tris.size() * sizeof(TriRecord));
   if (tlas_nodes) std::memcpy(tlas_nodes, s->tlas_nodes.data(), s->tlas_nodes.size() * sizeof(WideNode));
@@ -1333,6 +1380,42 @@ int mrt_trace_stats_device(mrt_scene sh, const mrt_ray* d_rays, size_t n, int an
   out->node_visits = st4[1];
   out->tri_tests = st4[2];
   out->instance_visits = st4[3];
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_set_tlas_refit(mrt_scene sh, uint32_t max_refits) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s) return fail(MRT_ERR_INVALID, "null scene");
+  s->refit_limit = max_refits;
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_update_transforms_device(mrt_scene sh, const float* d_m4x4_colmajor, size_t n, void* stream) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !d_m4x4_colmajor) return fail(MRT_ERR_INVALID, "null argument");
+  if (!s->prepared) return fail(MRT_ERR_NOT_PREPARED, "scene not prepared");
+  if (n != s->current.size()) return fail(MRT_ERR_INVALID, "one matrix per prepared instance is required");
+  if (s->tlas_nodes.empty()) return fail(MRT_ERR_INVALID, "the prepared frame has no instance BVH");
+  const int rc = s->refit(d_m4x4_colmajor, static_cast<hipStream_t>(stream));
+  if (rc != MRT_OK) return rc;
+  s->device_newer = true;
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_scene_update_status(mrt_scene sh, uint32_t* status) {
+  MRT_GUARD_BEGIN
+  Scene* s = reinterpret_cast<Scene*>(sh);
+  if (!s || !status) return fail(MRT_ERR_INVALID, "null argument");
+  *status = 0;
+  if (!s->d_refit_status) return MRT_OK;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(status, s->d_refit_status, 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "reading the refit status");
   return MRT_OK;
   MRT_GUARD_END
 }

@@ -56,6 +56,10 @@ struct Scene {
   std::vector<SdfInstance> sdf_pending, sdf_current;
   std::vector<SdfDrawable> sdf_drawables;
   bool sdf_dirty = false;
+  bool sdf_masks_dirty = false;       // masks appended on the host since the last upload
+  bool sdf_masks_host_stale = false;  // the device holds GPU-rebuilt masks the host copy lacks
+  std::vector<uint32_t> sdf_grid_rebuild;  // programs whose grids the next prepare rebuilds on the GPU
+  uint64_t* d_sdf_coarse = nullptr;   // coarse-pass scratch of the GPU grid build
   int sdf_steps = 400;      // sequential_pathtracer.hpp:135
   float sdf_eps = 1e-5f;    // sequential_pathtracer.hpp:136
   SdfDrawable* d_sdf_drawables = nullptr;
@@ -136,6 +140,7 @@ struct Scene {
 
   ~Scene();
   int build_host();
+  void tlas_levels();
   int upload_all();
   int trace(const void* d_rays, size_t n, void* d_out, bool any, bool stats, hipStream_t stream,
             const uint32_t* n_dev = nullptr, int context = 0);
@@ -159,6 +164,23 @@ struct Scene {
                hipStream_t stream, const uint32_t* n_dev);
   PtScene shading_view() const;
   int read_ctrl(uint32_t* flags, unsigned long long* stats, hipStream_t stream);
+
+  // Instance-BVH refit (refit.hip). A frame that changes only the matrices of its drawables
+  // refits the prepared TLAS on the GPU when refit_limit allows (0: rebuild on every change, as
+  // the reference does); mrt_scene_update_transforms_device refits from device matrices.
+  uint32_t refit_limit = 0;
+  uint32_t refits_done = 0;           // refits since the last rebuild
+  bool device_newer = false;          // device transforms the host copies lack
+  bool tlas_host_stale = false;       // tlas_nodes / inst_records lag the device
+  std::vector<uint32_t> tlas_level_nodes, tlas_level_start;  // wide TLAS nodes, deepest level first
+  std::vector<float> geo_boxes;       // per geometry: min xyz, max xyz
+  uint32_t* d_level_nodes = nullptr;
+  float *d_geo_box = nullptr, *d_inst_box = nullptr, *d_node_box = nullptr, *d_matrices = nullptr;
+  uint32_t* d_refit_status = nullptr;
+  size_t cap_level_nodes = 0, cap_geo_box = 0, cap_inst_box = 0, cap_node_box = 0, cap_matrices = 0;
+  int refit(const float* d_matrices_colmajor, hipStream_t stream);
+  int sync_host_masks();
+  int sync_host_tlas();
 };
 
 }  // namespace mrt

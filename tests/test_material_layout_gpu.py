@@ -1,9 +1,11 @@
-"""Materials of caller-registered layouts render exactly as the same values in the built-in record.
+"""Materials of caller-registered layouts render like the same values in the built-in record.
 
 The shaders read each material through its type's bindings (src/pt/material_new.glsl:17-25
 `open()` loads the same six inputs from the reference's generated loader), so a scene whose
 materials come from a layout with other parameters around the bound ones, in another order and
-nested, must give a bit-identical image to the scene built from mrt_material records."""
+nested, must give the image of the scene built from mrt_material records: the resolved records
+are compared bit for bit, and the two renders agree to a few ulp (per-pixel radiance is summed with
+float atomics, whose order varies between runs)."""
 import sys
 
 import numpy as np
@@ -71,4 +73,6 @@ def test_generic_layout_renders_like_the_record(ensure_built):
         r.close()
         s.close()
     assert np.isfinite(imgs[0]).all() and imgs[0][..., :3].mean() > 0.01
-    assert np.array_equal(imgs[0], imgs[1])
+    diff = np.abs(imgs[0].astype(np.float64) - imgs[1])
+    say("largest pixel difference %.3g" % diff.max())
+    assert diff.max() <= 1e-5 * max(1.0, float(np.abs(imgs[0]).max()))
