@@ -94,8 +94,34 @@ __device__ inline float prim_distance(uint32_t kind, const SdfInsn& in, float x,
   }
 }
 
-// Binary operator on (lhs d1, m1) and (rhs d2, m2).
-__device__ inline float combine(uint32_t op, float k, float d1, uint32_t m1, float d2, uint32_t m2, uint32_t& m) {
+// Material of a smooth operator (raymarch.glsl:43-56, `_mxm` with t = 1 - h, sdf.hpp:462-522):
+// the right operand's material with probability 1 - h, else the left one's, drawn with one
+// uniform number per ray and operator. The number hashes the world ray's bits and the operator's
+// ordinal among its program's binary operators (MurmurHash3's 32-bit finaliser, Appleby), so it
+// does not depend on the ray's slot in a launch, and the culled chain evaluation draws the same
+// number for an operator as the full interpreter. Hard operators stay deterministic, as in the
+// reference (t is 0 or 1).
+__device__ inline uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+__device__ inline uint32_t blend_seed(const float* o, const float* d) {
+  uint32_t h = 0x2545F491u;
+  for (int a = 0; a < 3; ++a) h = fmix32(h ^ __float_as_uint(o[a]));
+  for (int a = 0; a < 3; ++a) h = fmix32(h ^ __float_as_uint(d[a]));
+  return h;
+}
+__device__ inline float blend_u(uint32_t seed, uint32_t ordinal) {
+  return float(fmix32(seed ^ (ordinal * 0x9E3779B9u + 0x7F4A7C15u)) >> 8) * (1.0f / 16777216.0f);
+}
+
+// Binary operator on (lhs d1, m1) and (rhs d2, m2); u is the operator's uniform number.
+__device__ inline float combine(uint32_t op, float k, float d1, uint32_t m1, float d2, uint32_t m2, uint32_t& m,
+                                float u) {
   switch (op) {
     case kSdfUnion: {
       const float r = fminf(d1, d2);
@@ -114,17 +140,17 @@ __device__ inline float combine(uint32_t op, float k, float d1, uint32_t m1, flo
     }
     case kSdfSmoothUnion: {
       const float h = clamp01(0.5f + 0.5f * (d2 - d1) / k);
-      m = h >= 0.5f ? m1 : m2;
+      m = u < 1.0f - h ? m2 : m1;
       return mixf(d2, d1, h) - k * h * (1.0f - h);
     }
     case kSdfSmoothSubtract: {
       const float h = clamp01(0.5f - 0.5f * (d2 + d1) / k);
-      m = h >= 0.5f ? m1 : m2;
+      m = u < 1.0f - h ? m2 : m1;
       return mixf(d2, -d1, h) + k * h * (1.0f - h);
     }
     default: {  // kSdfSmoothIntersect
       const float h = clamp01(0.5f - 0.5f * (d2 - d1) / k);
-      m = h >= 0.5f ? m1 : m2;
+      m = u < 1.0f - h ? m2 : m1;
       return mixf(d2, d1, h) + k * h * (1.0f - h);
     }
   }
@@ -133,10 +159,12 @@ __device__ inline float combine(uint32_t op, float k, float d1, uint32_t m1, flo
 // map(p): distance and material of a compiled program at object-space point p. The program is
 // uniform across the wave, so its instructions come through scalar loads; the next one is
 // fetched before the current one is evaluated.
-__device__ inline float map(const SdfInsn* prog, uint32_t len, float px, float py, float pz, uint32_t& mat) {
+__device__ inline float map(const SdfInsn* prog, uint32_t len, float px, float py, float pz, uint32_t& mat,
+                           uint32_t seed) {
   Stack st{};
   PosStack ps{};
   float x = px, y = py, z = pz;
+  uint32_t ordinal = 0;  // binary operators so far
   SdfInsn nxt = prog[0];
   for (uint32_t i = 0; i < len; ++i) {
     const SdfInsn in = nxt;
@@ -153,14 +181,14 @@ __device__ inline float map(const SdfInsn* prog, uint32_t len, float px, float p
       const uint32_t fused = (in.op >> 8) & 0xffu;
       if (fused) {
         uint32_t m;
-        st.d0 = combine(fused, in.k, st.d0, st.m0, d, in.material, m);
+        st.d0 = combine(fused, in.k, st.d0, st.m0, d, in.material, m, blend_u(seed, ordinal++));
         st.m0 = m;
       } else {
         st.push(d, in.material);
       }
     } else if (kind <= kSdfSmoothIntersect) {
       uint32_t m;
-      const float r = combine(kind, in.p[0], st.d1, st.m1, st.d0, st.m0, m);
+      const float r = combine(kind, in.p[0], st.d1, st.m1, st.d0, st.m0, m, blend_u(seed, ordinal++));
       st.reduce(r, m);
     } else if (kind == kSdfPushTranslate) {
       ps.push(x, y, z);
@@ -225,7 +253,7 @@ __device__ inline float chain_distance_at(const SdfInsn& in, float px, float py,
 // instructions leave the running distance and material unchanged bit for bit, so the result is
 // map()'s. Points outside the grid evaluate every instruction.
 __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGrid& g, const uint64_t* masks, float px,
-                                  float py, float pz, uint32_t& mat, CellCache& cc) {
+                                  float py, float pz, uint32_t& mat, CellCache& cc, uint32_t seed) {
   uint64_t mask = len >= 64 ? ~0ull : ((1ull << len) - 1ull);
   const float fx = (px - g.lo[0]) * g.inv[0], fy = (py - g.lo[1]) * g.inv[1], fz = (pz - g.lo[2]) * g.inv[2];
   const float dim = float(g.dim);
@@ -260,8 +288,9 @@ __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGr
     const SdfInsn a = prog[j0], b = prog[j1];
     const float da = chain_distance_at(a, px, py, pz), db = chain_distance_at(b, px, py, pz);
     uint32_t ma, mb;
-    const float ra = combine((a.op >> 8) & 0xffu, a.k, r, m, da, a.material, ma);
-    const float rb = combine((b.op >> 8) & 0xffu, b.k, ra, ma, db, b.material, mb);
+    // Chain instruction j carries binary operator j - 1.
+    const float ra = combine((a.op >> 8) & 0xffu, a.k, r, m, da, a.material, ma, blend_u(seed, j0 - 1));
+    const float rb = combine((b.op >> 8) & 0xffu, b.k, ra, ma, db, b.material, mb, blend_u(seed, j1 - 1));
     r = two ? rb : ra;
     m = two ? mb : ma;
   }
@@ -274,9 +303,9 @@ __device__ inline float map_chain(const SdfInsn* prog, uint32_t len, const SdfGr
 // is left out of the kernel.
 template <bool CHAIN = false>
 __device__ inline float map_any(const SdfInsn* prog, uint32_t len, const SdfGrid& g, const uint64_t* masks, float px,
-                                float py, float pz, uint32_t& mat, CellCache& cc) {
-  if (CHAIN) return map_chain(prog, len, g, masks, px, py, pz, mat, cc);
-  return g.dim ? map_chain(prog, len, g, masks, px, py, pz, mat, cc) : map(prog, len, px, py, pz, mat);
+                                float py, float pz, uint32_t& mat, CellCache& cc, uint32_t seed = 0) {
+  if (CHAIN) return map_chain(prog, len, g, masks, px, py, pz, mat, cc, seed);
+  return g.dim ? map_chain(prog, len, g, masks, px, py, pz, mat, cc, seed) : map(prog, len, px, py, pz, mat, seed);
 }
 
 __device__ inline uint32_t oct_encode(float x, float y, float z) {
@@ -386,8 +415,9 @@ __device__ inline void sdf_normal(const SdfParams& p, uint32_t ray) {
     ny = ny + ey * d;
     nz = nz + ez * d;
   }
+  const float* wr = reinterpret_cast<const float*>(p.rays) + size_t(ray) * 8;  // mrt_ray: o, tmin, d, tmax
   uint32_t mat;
-  map_any(tape, dr.tape_len, grid, p.masks, bx, by, bz, mat, cc);
+  map_any(tape, dr.tape_len, grid, p.masks, bx, by, bz, mat, cc, blend_seed(wr, wr + 4));
   const float* iv = dr.inv;  // world normal = (M^-1)^T n
   float wx = iv[0] * nx + iv[4] * ny + iv[8] * nz;
   float wy = iv[1] * nx + iv[5] * ny + iv[9] * nz;

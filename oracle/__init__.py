@@ -71,6 +71,8 @@ def lib():
             "orc_reference_time": ([_P, _P, _SZ, ctypes.c_int, ctypes.c_int, _SZ, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int)], ctypes.c_double),
             "orc_sdf_map": ([_P, ctypes.c_uint32, _P, _P], ctypes.c_float),
+            "orc_sdf_map_seeded": ([_P, ctypes.c_uint32, _P, ctypes.c_uint32, _P], ctypes.c_float),
+            "orc_sdf_blend_seed": ([_P, _P], ctypes.c_uint32),
             "orc_sdf_trace": ([_P, _P, ctypes.c_uint32, _P, _SZ, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P,
                                _P, ctypes.c_int], None),
             "orc_render": ([ctypes.POINTER(_PtScene), _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -229,6 +231,18 @@ def sdf_map(tape, p):
     mat = ctypes.c_uint32()
     d = lib().orc_sdf_map(_p(tape), tape.size, _p(np.ascontiguousarray(p, np.float32)), ctypes.byref(mat))
     return d, mat.value
+
+
+def sdf_map_seeded(tape, p, seed):
+    tape = np.ascontiguousarray(tape, SDF_NODE_DTYPE)
+    mat = ctypes.c_uint32()
+    d = lib().orc_sdf_map_seeded(_p(tape), tape.size, _p(np.ascontiguousarray(p, np.float32)), int(seed),
+                                 ctypes.byref(mat))
+    return d, mat.value
+
+
+def sdf_blend_seed(o, d):
+    return int(lib().orc_sdf_blend_seed(_p(np.ascontiguousarray(o, np.float32)), _p(np.ascontiguousarray(d, np.float32))))
 
 
 def sdf_trace(tape, drawables, rays, hits=None, any_hit=False, steps=400, eps=1e-5, threads=0):

@@ -195,6 +195,10 @@ typedef struct orc_sdf_drawable {
 
 /* map(p) of a tape (the formulas of src/pathtracer/sdf.hpp:399-633 in postfix form), binary32. */
 float orc_sdf_map(const orc_sdf_node* tape, uint32_t len, const float p[3], uint32_t* material);
+/* The same with a ray seed for the stochastic material of smooth operators (raymarch.glsl:43-56),
+ * and the seed of a world ray (what the closest-hit march uses at its hit point). */
+float orc_sdf_map_seeded(const orc_sdf_node* tape, uint32_t len, const float p[3], uint32_t seed, uint32_t* material);
+uint32_t orc_sdf_blend_seed(const float o[3], const float d[3]);
 /* march_sdf over every drawable (src/pt/raymarch.glsl:116-171), starting at the drawable box's
  * entry, on top of prior hits: closest (hits[i] from a triangle query; a closer SDF surface
  * replaces it, inst = 0x80000000 | drawable, prim = material, normals[3i..3i+2] = world normal

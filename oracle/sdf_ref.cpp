@@ -38,7 +38,37 @@ float menger(float x, float y, float z) {
   return (inner + len3(fmax_(dx, 0.0f), fmax_(dy, 0.0f), fmax_(dz, 0.0f))) * (1.0f / 81.0f);
 }
 
-float map(const orc_sdf_node* t, uint32_t len, float x, float y, float z, uint32_t* mat_out) {
+// Material of a smooth operator (src/pt/raymarch.glsl:43-56: `_mxm(m0, m1, t)` takes m1 when
+// random_next() < t, and src/pathtracer/sdf.hpp:462-522 call it with t = 1 - h): the right
+// operand's material when u < 1 - h. u is this ray's uniform number for the operator: MurmurHash3's
+// 32-bit finaliser (Appleby) over the world ray's bits gives the ray's seed, and the finaliser of
+// seed ^ (ordinal * 0x9E3779B9 + 0x7F4A7C15), top 24 bits, the number of the ordinal-th binary
+// operator of the tape.
+uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+uint32_t bits_of(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+uint32_t blend_seed(const float o[3], const float dd[3]) {
+  uint32_t h = 0x2545F491u;
+  for (int a = 0; a < 3; ++a) h = fmix32(h ^ bits_of(o[a]));
+  for (int a = 0; a < 3; ++a) h = fmix32(h ^ bits_of(dd[a]));
+  return h;
+}
+float blend_u(uint32_t seed, uint32_t ordinal) {
+  return float(fmix32(seed ^ (ordinal * 0x9E3779B9u + 0x7F4A7C15u)) >> 8) * (1.0f / 16777216.0f);
+}
+
+float map(const orc_sdf_node* t, uint32_t len, float x, float y, float z, uint32_t* mat_out, uint32_t seed = 0) {
+  uint32_t ordinal = 0;
   std::vector<float> d;
   std::vector<uint32_t> m;
   std::vector<float> ps;
@@ -81,6 +111,7 @@ float map(const orc_sdf_node* t, uint32_t len, float x, float y, float z, uint32
           m.pop_back();
           float r;
           uint32_t mr;
+          ++ordinal;
           if (nd.op <= 18) {
             r = nd.op == 16 ? fmin_(a, b) : (nd.op == 17 ? fmax_(-a, b) : fmax_(a, b));
             mr = r == b ? mb : ma;
@@ -97,7 +128,7 @@ float map(const orc_sdf_node* t, uint32_t len, float x, float y, float z, uint32
               h = clamp01(0.5f - 0.5f * (b - a) / k);
               r = mixf(b, a, h) + k * h * (1.0f - h);
             }
-            mr = h >= 0.5f ? ma : mb;
+            mr = blend_u(seed, ordinal - 1) < 1.0f - h ? mb : ma;
           }
           d.back() = r;
           m.back() = mr;
@@ -158,6 +189,12 @@ extern "C" {
 float orc_sdf_map(const orc_sdf_node* tape, uint32_t len, const float p[3], uint32_t* mat) {
   return map(tape, len, p[0], p[1], p[2], mat);
 }
+
+float orc_sdf_map_seeded(const orc_sdf_node* tape, uint32_t len, const float p[3], uint32_t seed, uint32_t* mat) {
+  return map(tape, len, p[0], p[1], p[2], mat, seed);
+}
+
+uint32_t orc_sdf_blend_seed(const float o[3], const float d[3]) { return blend_seed(o, d); }
 
 void orc_sdf_trace(const orc_sdf_node* tape, const orc_sdf_drawable* drs, uint32_t n_dr, const orc_ray* rays,
                    size_t n, int any_hit, int steps, float eps, orc_hit* hits, float* normals, int threads) {
@@ -230,7 +267,7 @@ void orc_sdf_trace(const orc_sdf_node* tape, const orc_sdf_drawable* drs, uint32
           for (int a = 0; a < 3; ++a) nrm[a] = nrm[a] + e[a] * d;
         }
         uint32_t mat = 0;
-        map(tape + dr.tape_offset, dr.tape_len, bp[0], bp[1], bp[2], &mat);
+        map(tape + dr.tape_offset, dr.tape_len, bp[0], bp[1], bp[2], &mat, blend_seed(r.o, r.d));
         const float* iv = dr.inv;
         float w[3];
         for (int a = 0; a < 3; ++a) w[a] = iv[a] * nrm[0] + iv[4 + a] * nrm[1] + iv[8 + a] * nrm[2];

@@ -21,11 +21,12 @@ def test_map_closed_forms(ensure_built):
     t = scenes.sdf_tape([("CAPSULE", 0, 2.0, 0.5)])
     assert orc.sdf_map(t, [0.0, 3.0, 0.0])[0] == 0.5
     assert orc.sdf_map(t, [1.0, 1.0, 0.0])[0] == 0.5
-    # smooth union of two spheres at the midpoint: d1 = d2 = 0.5 -> h = 0.5, d = 0.5 - k/4.
+    # smooth union of two spheres at the midpoint: d1 = d2 = 0.5 -> h = 0.5, d = 0.5 - k/4 (the
+    # material there is either operand's, drawn per ray: tests/test_sdf_blend.py).
     t = scenes.sdf_tape([("PUSH_TRANSLATE", 0, -1, 0, 0), ("SPHERE", 1, 0.5), ("POP", 0),
                          ("PUSH_TRANSLATE", 0, 1, 0, 0), ("SPHERE", 2, 0.5), ("POP", 0), ("SMOOTH_UNION", 0, 0.2)])
     d, mat = orc.sdf_map(t, [0.0, 0.0, 0.0])
-    assert abs(d - (0.5 - 0.05)) < 1e-7 and mat == 1
+    assert abs(d - (0.5 - 0.05)) < 1e-7 and mat in (1, 2)
     # hard union picks the closer operand's material; subtraction max(-a, b).
     t = scenes.sdf_tape([("PUSH_TRANSLATE", 0, -1, 0, 0), ("SPHERE", 1, 0.5), ("POP", 0),
                          ("PUSH_TRANSLATE", 0, 1, 0, 0), ("SPHERE", 2, 0.5), ("POP", 0), ("UNION", 0)])
