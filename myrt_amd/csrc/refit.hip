@@ -12,8 +12,10 @@
 //      (inner children from the level below, leaf children from their instances) and the
 //      requantisation of the child boxes, the host collapse's rule step for step, so a refit with
 //      unchanged matrices reproduces the host-built nodes bit for bit.
-// Everything is fixed-order float/double arithmetic with contraction off (-ffp-contract=off), so
-// host and device agree to the bit.
+// A TLAS of up to kRefitOneBlockNodes nodes runs both passes in one workgroup and one launch
+// (refit_one_block_kernel), with a barrier between levels: at a few thousand instances the
+// per-level launches would cost more than their work. Everything is fixed-order float/double
+// arithmetic with contraction off (-ffp-contract=off), so host and device agree to the bit.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -82,9 +84,7 @@ __device__ bool invert_rows(const float* m, float* out) {
   return true;
 }
 
-__global__ void instance_kernel(RefitParams p) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.n_inst) return;
+__device__ void update_instance(const RefitParams& p, uint32_t i) {
   float m[16];
   for (int k = 0; k < 16; ++k) m[k] = p.matrices[size_t(i) * 16 + k];
   InstanceRecord& rec = p.inst[i];
@@ -115,6 +115,11 @@ __global__ void instance_kernel(RefitParams p) {
   }
 }
 
+__global__ void instance_kernel(RefitParams p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p.n_inst) update_instance(p, i);
+}
+
 __device__ inline double clamp_coord(double v) {
   if (isnan(v)) return 0.0;
   return fmin(fmax(v, -1e37), 1e37);
@@ -137,103 +142,147 @@ __device__ inline float float_up(double v) {
   return f;
 }
 
-// The host collapse's quantisation of child boxes (wide_bvh.cpp `quantize`), for one node.
-__device__ void quantize(WideNode& n, const DBox* ch, int nc, double pad_rel) {
-  for (int a = 0; a < 3; ++a) {
-    float lreq[4], hreq[4];
-    for (int c = 0; c < nc; ++c) {
-      const DBox& bx = ch[c];
-      double m = 0.0;
-      for (int k = 0; k < 3; ++k) {
-        m = fmax(m, fabs(clamp_coord(bx.mn[k])));
-        m = fmax(m, fabs(clamp_coord(bx.mx[k])));
-      }
-      const double pad = pad_rel * m;
-      const float cmn = bx.mn[a], cmx = bx.mx[a];
-      if (!(cmn <= cmx)) {
-        lreq[c] = hreq[c] = 0.f;
-        continue;
-      }
-      const double lo = clamp_coord(double(cmn) - pad), hi = clamp_coord(double(cmx) + pad);
-      lreq[c] = fminf(float_down(lo), next_toward(float(clamp_coord(cmn)), false));
-      hreq[c] = fmaxf(float_up(hi), next_toward(float(clamp_coord(cmx)), true));
+// The host collapse's quantisation of child boxes (wide_bvh.cpp `quantize`) on one axis, given
+// the children's extents on that axis and their paddings. Loops over children are unrolled over
+// the 4 slots with a guard, so per-child values stay in registers (dynamically indexed arrays
+// would live in scratch, one memory round trip per access). Writes origin, exponent and the
+// child codes of the axis straight to the node in memory.
+__device__ void quantize_axis(WideNode* out, int a, const float (&cmn)[4], const float (&cmx)[4],
+                              const double (&pad)[4], int nc) {
+  float lreq[4], hreq[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    lreq[c] = hreq[c] = 0.f;
+    if (c < nc && cmn[c] <= cmx[c]) {  // an empty or NaN box keeps a point box at 0
+      const double lo = clamp_coord(double(cmn[c]) - pad[c]), hi = clamp_coord(double(cmx[c]) + pad[c]);
+      lreq[c] = fminf(float_down(lo), next_toward(float(clamp_coord(cmn[c])), false));
+      hreq[c] = fmaxf(float_up(hi), next_toward(float(clamp_coord(cmx[c])), true));
     }
-    float origin = lreq[0];
-    double top = hreq[0];
-    for (int c = 1; c < nc; ++c) {
+  }
+  float origin = lreq[0];
+  double top = hreq[0];
+#pragma unroll
+  for (int c = 1; c < 4; ++c) {
+    if (c < nc) {
       origin = fminf(origin, lreq[c]);
       top = fmax(top, double(hreq[c]));
     }
-    const double span = top - double(origin);
-    // Start at or below the host's first candidate (ceil(log2(span / 255)) - 1): every exponent
-    // below that cannot hold the span in 255 steps, so both searches end on the same exponent.
-    int e = span > 0 ? ilogb(span / 255.0) - 1 : -126;
-    if (e < -126) e = -126;
-    uint8_t qlo_out[4] = {0, 0, 0, 0}, qhi_out[4] = {0, 0, 0, 0};
-    for (; e <= 127; ++e) {
-      const float scale = ldexpf(1.0f, e);
-      bool ok = true;
-      for (int c = 0; c < nc && ok; ++c) {
-        const double ql = floor((double(lreq[c]) - double(origin)) / double(scale));
-        int qlo = int(fmin(fmax(ql, 0.0), 255.0));
-        while (qlo > 0 && fmaf(float(qlo), scale, origin) > lreq[c]) --qlo;
-        const double qh = ceil((double(hreq[c]) - double(origin)) / double(scale));
-        int qhi = int(fmin(fmax(qh, 0.0), 255.0));
-        while (fmaf(float(qhi), scale, origin) < hreq[c]) {
-          if (qhi == 255) {
-            ok = false;
-            break;
-          }
-          ++qhi;
-        }
-        qlo_out[c] = uint8_t(qlo);
-        qhi_out[c] = uint8_t(qhi);
-      }
-      if (ok) break;
-    }
-    // Coordinates are clamped to +-1e37, so a 255-step exponent <= 127 always exists.
-    n.origin[a] = origin;
-    n.exp[a] = int8_t(e);
-    for (int c = 0; c < 4; ++c) {
-      n.qlo[a][c] = c < nc ? qlo_out[c] : 0;
-      n.qhi[a][c] = c < nc ? qhi_out[c] : 0;
-    }
   }
+  const double span = top - double(origin);
+  // The host's first candidate, ceil(log2(x)) - 1 with x = span / 255, exactly: ilogb(x), less one
+  // when x is a power of two. Every smaller exponent leaves half the span or more outside 255
+  // steps, so both searches try the same exponents from here on.
+  int e = -126;
+  if (span > 0) {
+    const double x = span / 255.0;
+    e = ilogb(x);
+    if (ldexp(1.0, e) == x) --e;
+  }
+  if (e < -126) e = -126;
+  uint32_t lo_codes = 0, hi_codes = 0;
+  for (; e <= 127; ++e) {
+    const float scale = ldexpf(1.0f, e);
+    bool ok = true;
+    lo_codes = hi_codes = 0;
+    // All children are evaluated even after one fails (a failed exponent is discarded anyway),
+    // so their independent chains overlap.
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (c < nc) {
+        const double ql = floor((double(lreq[c]) - double(origin)) / double(scale));
+        int lo = int(fmin(fmax(ql, 0.0), 255.0));
+        while (lo > 0 && fmaf(float(lo), scale, origin) > lreq[c]) --lo;
+        const double qh = ceil((double(hreq[c]) - double(origin)) / double(scale));
+        int hi = int(fmin(fmax(qh, 0.0), 255.0));
+        while (fmaf(float(hi), scale, origin) < hreq[c] && hi < 255) ++hi;
+        ok = ok && !(fmaf(float(hi), scale, origin) < hreq[c]);
+        lo_codes |= uint32_t(lo) << (8 * c);
+        hi_codes |= uint32_t(hi) << (8 * c);
+      }
+    }
+    if (ok) break;
+  }
+  // Coordinates are clamped to +-1e37, so a 255-step exponent <= 127 always exists.
+  out->origin[a] = origin;
+  out->exp[a] = int8_t(e);
+  *reinterpret_cast<uint32_t*>(out->qlo[a]) = lo_codes;
+  *reinterpret_cast<uint32_t*>(out->qhi[a]) = hi_codes;
 }
 
-__global__ void level_kernel(RefitParams p, uint32_t first, uint32_t count) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  const uint32_t id = p.level_nodes[first + t];
-  WideNode n = p.tlas[id];
-  const int nc = n.nchildren;
-  DBox ch[4];
-  DBox all;
-  empty_box(all);
-  for (int c = 0; c < nc; ++c) {
-    empty_box(ch[c]);
-    if (n.count[c] == 0) {
-      const float* b = p.node_box + size_t(n.node_base + n.offset[c]) * 6;
-      enclose(ch[c], b, b + 3);
-    } else {
-      const uint32_t s = n.item_base + n.offset[c];
-      for (uint32_t k = 0; k < n.count[c]; ++k) {
-        const float* b = p.inst_box + size_t(p.items[s + k]) * 6;
-        enclose(ch[c], b, b + 3);
+// One axis of one node: the union of the node's children on that axis (for its parent) and the
+// requantisation of the children's codes on that axis. The paddings need each child's whole box.
+__device__ void refit_node_axis(const RefitParams& p, uint32_t id, int a) {
+  WideNode* node = p.tlas + id;
+  const int nc = node->nchildren;
+  const uint32_t node_base = node->node_base, item_base = node->item_base;
+  const uint32_t counts = *reinterpret_cast<const uint32_t*>(node->count);
+  const uint2 offs = *reinterpret_cast<const uint2*>(node->offset);
+  float mn[4], mx[4];
+  double pad[4];
+  float all_mn = FLT_MAX, all_mx = -FLT_MAX;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    DBox b;
+    empty_box(b);
+    if (c < nc) {
+      const uint32_t count = (counts >> (8 * c)) & 0xffu;
+      const uint32_t off = ((c < 2 ? offs.x : offs.y) >> (16 * (c & 1))) & 0xffffu;
+      if (count == 0) {
+        const float* q = p.node_box + size_t(node_base + off) * 6;
+        enclose(b, q, q + 3);
+      } else {
+        for (uint32_t k = 0; k < count; ++k) {
+          const float* q = p.inst_box + size_t(p.items[item_base + off + k]) * 6;
+          enclose(b, q, q + 3);
+        }
       }
     }
-    enclose(all, ch[c].mn, ch[c].mx);
+    double m = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      m = fmax(m, fabs(clamp_coord(b.mn[k])));
+      m = fmax(m, fabs(clamp_coord(b.mx[k])));
+    }
+    pad[c] = p.pad_rel * m;
+    mn[c] = a == 0 ? b.mn[0] : (a == 1 ? b.mn[1] : b.mn[2]);
+    mx[c] = a == 0 ? b.mx[0] : (a == 1 ? b.mx[1] : b.mx[2]);
+    if (c < nc) {
+      all_mn = ref_min(all_mn, mn[c]);
+      all_mx = ref_max(all_mx, mx[c]);
+    }
   }
-  quantize(n, ch, nc, p.pad_rel);
-  p.tlas[id] = n;
-  float* o = p.node_box + size_t(id) * 6;
-  for (int a = 0; a < 3; ++a) {
-    o[a] = all.mn[a];
-    o[3 + a] = all.mx[a];
+  quantize_axis(node, a, mn, mx, pad, nc);
+  p.node_box[size_t(id) * 6 + a] = all_mn;
+  p.node_box[size_t(id) * 6 + 3 + a] = all_mx;
+}
+
+// Three threads per node, one per axis.
+__global__ void __launch_bounds__(64) level_kernel(RefitParams p, uint32_t first, uint32_t count) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 3 * count) refit_node_axis(p, p.level_nodes[first + t / 3], int(t % 3));
+}
+
+// A small TLAS in one workgroup: the status word, the instances and every level, with a barrier
+// between the passes, in one launch (the multi-launch form is bound by launch latency there).
+__global__ void __launch_bounds__(1024) refit_one_block_kernel(RefitParams p, const uint32_t* level_start,
+                                                               uint32_t n_levels) {
+  if (threadIdx.x == 0) *p.status = 0u;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < p.n_inst; i += blockDim.x) update_instance(p, i);
+  __syncthreads();
+  for (uint32_t l = 0; l < n_levels; ++l) {
+    const uint32_t first = level_start[l], end = level_start[l + 1];
+    for (uint32_t t = 3 * first + threadIdx.x; t < 3 * end; t += blockDim.x)
+      refit_node_axis(p, p.level_nodes[t / 3], int(t % 3));
+    __syncthreads();
   }
 }
 
 }  // namespace
+
+int launch_refit_one_block(const RefitParams& p, const uint32_t* d_level_start, uint32_t n_levels, hipStream_t s) {
+  refit_one_block_kernel<<<1, 1024, 0, s>>>(p, d_level_start, n_levels);
+  return hipGetLastError() != hipSuccess ? 1 : 0;
+}
 
 int launch_refit(const RefitParams& p, const uint32_t* level_start, uint32_t n_levels, hipStream_t s) {
   if (p.n_inst) {
@@ -244,7 +293,7 @@ int launch_refit(const RefitParams& p, const uint32_t* level_start, uint32_t n_l
   for (uint32_t l = 0; l < n_levels; ++l) {
     const uint32_t first = level_start[l], count = level_start[l + 1] - first;
     if (!count) continue;
-    level_kernel<<<(count + 63) / 64, 64, 0, s>>>(p, first, count);
+    level_kernel<<<(3 * count + 63) / 64, 64, 0, s>>>(p, first, count);
     if (hipGetLastError() != hipSuccess) return 1;
   }
   return 0;

@@ -25,5 +25,9 @@ struct RefitParams {
 // Runs the instance pass and one launch per level on stream s (no host synchronisation).
 // level_start has n_levels + 1 entries into level_nodes. Returns nonzero on a launch error.
 int launch_refit(const RefitParams& p, const uint32_t* level_start, uint32_t n_levels, hipStream_t s);
+// The same passes in one workgroup and one launch, for TLASes of up to kRefitOneBlockNodes nodes;
+// d_level_start is the device copy of level_start, and the status word is cleared by the kernel.
+constexpr uint32_t kRefitOneBlockNodes = 4096;
+int launch_refit_one_block(const RefitParams& p, const uint32_t* d_level_start, uint32_t n_levels, hipStream_t s);
 
 }  // namespace mrt

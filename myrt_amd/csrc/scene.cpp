@@ -257,6 +257,7 @@ int Scene::upload_all() {
     if ((rc = upload(d_inst, cap_inst, inst_records)) != MRT_OK) return rc;
     if ((rc = upload(d_inst_material, cap_inst_mat, inst_material)) != MRT_OK) return rc;
     if ((rc = upload(d_level_nodes, cap_level_nodes, tlas_level_nodes)) != MRT_OK) return rc;
+    if ((rc = upload(d_level_start, cap_level_start, tlas_level_start)) != MRT_OK) return rc;
     tlas_host_stale = false;
   }
   if (geoms_dirty) {
@@ -357,7 +358,7 @@ Scene::~Scene() {
   for (void* q : {d_host_rays, d_host_out})
     if (q) hipFree(q);
   if (d_sdf_coarse) hipFree(d_sdf_coarse);
-  for (void* q : {(void*)d_level_nodes, (void*)d_geo_box, (void*)d_inst_box, (void*)d_node_box, (void*)d_matrices,
+  for (void* q : {(void*)d_level_nodes, (void*)d_level_start, (void*)d_geo_box, (void*)d_inst_box, (void*)d_node_box, (void*)d_matrices,
                   (void*)d_refit_status})
     if (q) hipFree(q);
   if (d_tlas) hipFree(d_tlas);
@@ -936,7 +937,6 @@ int Scene::refit(const float* d_mats, hipStream_t stream) {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_refit_status), 4);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
   }
-  if (launch_fill_u32(d_refit_status, 1, 0u, stream)) return fail(MRT_ERR_HIP, "refit status clear");
   RefitParams p{};
   p.matrices = d_mats;
   p.n_inst = uint32_t(n);
@@ -949,8 +949,13 @@ int Scene::refit(const float* d_mats, hipStream_t stream) {
   p.node_box = d_node_box;
   p.status = d_refit_status;
   p.pad_rel = kTlasPadRel;
-  if (launch_refit(p, tlas_level_start.data(), uint32_t(tlas_level_start.size() - 1), stream))
-    return fail(MRT_ERR_HIP, "refit launch");
+  const uint32_t n_levels = uint32_t(tlas_level_start.size() - 1);
+  if (tlas_nodes.size() <= kRefitOneBlockNodes) {
+    if (launch_refit_one_block(p, d_level_start, n_levels, stream)) return fail(MRT_ERR_HIP, "refit launch");
+  } else {
+    if (launch_fill_u32(d_refit_status, 1, 0u, stream)) return fail(MRT_ERR_HIP, "refit status clear");
+    if (launch_refit(p, tlas_level_start.data(), n_levels, stream)) return fail(MRT_ERR_HIP, "refit launch");
+  }
   tlas_host_stale = true;
   return MRT_OK;
 }

@@ -174,10 +174,10 @@ struct Scene {
   bool tlas_host_stale = false;       // tlas_nodes / inst_records lag the device
   std::vector<uint32_t> tlas_level_nodes, tlas_level_start;  // wide TLAS nodes, deepest level first
   std::vector<float> geo_boxes;       // per geometry: min xyz, max xyz
-  uint32_t* d_level_nodes = nullptr;
+  uint32_t *d_level_nodes = nullptr, *d_level_start = nullptr;
   float *d_geo_box = nullptr, *d_inst_box = nullptr, *d_node_box = nullptr, *d_matrices = nullptr;
   uint32_t* d_refit_status = nullptr;
-  size_t cap_level_nodes = 0, cap_geo_box = 0, cap_inst_box = 0, cap_node_box = 0, cap_matrices = 0;
+  size_t cap_level_nodes = 0, cap_level_start = 0, cap_geo_box = 0, cap_inst_box = 0, cap_node_box = 0, cap_matrices = 0;
   int refit(const float* d_matrices_colmajor, hipStream_t stream);
   int sync_host_masks();
   int sync_host_tlas();

@@ -66,9 +66,13 @@ def colmajor(instances):
     return np.stack([np.ascontiguousarray(np.asarray(mt, np.float32).T).ravel() for _, mt, _ in instances])
 
 
-@pytest.fixture(scope="module")
-def field(ensure_built):
-    return scenes.sphere_field_scene(N_SPHERES, 2, seed=4)
+@pytest.fixture(scope="module", params=["1500 instances", "30000 instances"])
+def field(ensure_built, request):
+    """1,500 icospheres (a TLAS small enough for the one-workgroup refit) or 30,000 (the
+    per-level launches)."""
+    if request.param.startswith("1500"):
+        return scenes.sphere_field_scene(N_SPHERES, 2, seed=4)
+    return scenes.sphere_field_scene(30000, 0, seed=5)
 
 
 def test_refit_with_unchanged_matrices_reproduces_the_build(field):
@@ -76,6 +80,7 @@ def test_refit_with_unchanged_matrices_reproduces_the_build(field):
     p = s.prepare()
     _, _, tlas0, items0, inst0 = s.wide_arrays()
     say("refit of %d instances / %d TLAS nodes with unchanged matrices" % (p["num_instances"], p["tlas_wide_nodes"]))
+    assert (p["tlas_wide_nodes"] <= 4096) == (p["num_instances"] < 10000)  # both refit forms are covered
     d = m.DeviceBuffer.from_array(colmajor(field["instances"]))
     s.update_transforms_device(d, len(field["instances"]))
     assert s.update_status() == 0
@@ -168,7 +173,8 @@ def test_device_matrices_and_singular_update(field):
     assert s.prepare()["drawables_changed"] == 1
     # A singular matrix: reported, and its instance is no longer hit.
     mats = colmajor(inst)
-    hits_per = np.bincount(b["inst"][b["inst"] < N_SPHERES].astype(np.int64), minlength=N_SPHERES)
+    n_spheres = len(field["instances"]) - 1
+    hits_per = np.bincount(b["inst"][b["inst"] < n_spheres].astype(np.int64), minlength=n_spheres)
     victim = int(np.argmax(hits_per))
     assert hits_per[victim] > 0
     mats[victim] = 0.0
