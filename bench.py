@@ -315,6 +315,43 @@ def unet_macs_per_pixel():
     return macs + UNET_LEVELS[0] * 3
 
 
+def c1_render(args, frames=20):
+    """The plain renderer on the reference's small configuration (ground quad + 200 icospheres,
+    64,002 triangles, 512x512, 1 spp, depth 4), one frame per call as an interactive loop renders
+    it: the median frame time and samples/s, for comparison with the CPU baseline's restated
+    reference renderer on the same scene."""
+    import myrt_amd as m
+    from myrt_amd import scenes
+    sc = scenes.sphere_field_scene(200, 2, seed=1)
+    s = m.Scene()
+    gids = [s.push_geometry(i, p, n) for i, p, n in sc["geometries"]]
+    for mat in sc["materials"]:
+        s.push_material(mat)
+    s.set_lights(sc["lights"])
+    s.set_environment(1.0)
+    for g, mt, mi in sc["instances"]:
+        s.enqueue(gids[g], mt, mi)
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], 1.0)
+    r = m.Renderer()
+    r.set_camera(view, proj)
+    r.set_num_bounces(4)
+    r.render(s, 512, 512, 1)
+    times = []
+    for _ in range(frames):
+        m.synchronize()
+        t0 = time.perf_counter()
+        r.render(s, 512, 512, 1)
+        m.synchronize()
+        times.append(time.perf_counter() - t0)
+    st = r.stats()
+    ms = float(np.median(times)) * 1e3
+    out = {"width": 512, "height": 512, "spp": 1, "depth": 4, "frame_ms": round(ms, 3),
+           "samples_per_s": round(512 * 512 / (ms * 1e-3), 1), "rays_per_sample": round(st["rays"] / st["samples"], 3)}
+    r.close()
+    s.close()
+    return out
+
+
 def learned_kernels(args):
     """Timings of the learned parts on the device: the denoiser at 1920x1080 (fp16 MFMA), the
     radiance-cache MLP inference on 2^20 queries and one training step on 2^16 samples (bf16 MFMA)."""
@@ -358,6 +395,61 @@ def learned_kernels(args):
     return out
 
 
+CPU_COMPILE = "g++ -O3 -march=native -ffp-contract=off -fno-fast-math (compiled on this host at bench time)"
+
+
+def cpu_baseline_child(npz_path, threads_all):
+    """The CPU baseline, run in a child process whose oracle library was built for this host.
+    Times the reference's traversal (stackless bit-stack walk and Moller-Trumbore, restated) on
+    bounded samples of the microbench's coherent, incoherent and shadow rays, and the reference's
+    plain renderer (restated generate / trace / color / filter passes) on its small configuration
+    (ground quad + 200 icospheres, 64,002 triangles, 512x512, 1 spp, depth 4), each at all host
+    threads and at one thread, under the thread_pool clone; median of 5 after a warm-up."""
+    import oracle
+    from myrt_amd import scenes
+    data = np.load(npz_path)
+    osc = oracle.OracleScene([(data["idx"], data["pts"])], [(0, np.eye(4, dtype=np.float32))])
+    out = {"traversal": {}, "render_c1": {}}
+    for cls, any_hit in (("incoherent", False), ("coherent", False), ("shadow", True)):
+        rays = data[cls]
+        row = {}
+        for label, threads, n in (("all_threads", 0, rays.size), ("one_thread", 1, rays.size // 8)):
+            rate, used = osc.reference_time(rays[:n], any_hit=any_hit, threads=threads, chunk=65536, reps=5)
+            row[label] = {"mrays_per_s": round(rate / 1e6, 4), "threads": used, "rays": int(n)}
+        out["traversal"][cls] = row
+    sc = scenes.sphere_field_scene(200, 2, seed=1)
+    o = oracle.OracleRender(sc["geometries"], sc["instances"], sc["materials"], sc["lights"])
+    view, proj = scenes.camera_matrices(sc["eye"], sc["target"], sc["fovy"], 1.0)
+    for label, threads in (("all_threads", 0), ("one_thread", 1)):
+        sps, rps, used, _ = o.reference_render_time(view, proj, 512, 512, 4, threads=threads, reps=5)
+        out["render_c1"][label] = {"samples_per_s": round(sps, 1), "mrays_per_s": round(rps / 1e6, 4), "threads": used}
+    print(json.dumps(out), flush=True)
+
+
+def run_cpu_baseline(args, idx, pts, coh, inc, sh):
+    """Builds the oracle for this host (falling back to the prebuilt -O2 library) and runs
+    cpu_baseline_child in a subprocess; returns its result dict and the compile note."""
+    import subprocess
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="myrt_cpu_baseline_")
+    env = dict(os.environ)
+    compile_note = CPU_COMPILE
+    try:
+        from oracle import build as ob
+        env["MYRT_ORACLE_LIBRARY"] = ob.build_native(tmp)
+    except Exception as ex:  # no compiler: the prebuilt library, which targets any x86-64
+        log("native oracle build failed (%s); timing the prebuilt library" % str(ex).splitlines()[0])
+        compile_note = "g++ -O2 -ffp-contract=off (prebuilt, generic x86-64)"
+    n = args.cpu_sample
+    npz = os.path.join(tmp, "rays.npz")
+    np.savez(npz, idx=idx, pts=pts, incoherent=inc[:n], coherent=coh[:n], shadow=sh[:n])
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", npz], env=env,
+                       capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline child failed:\n" + r.stderr[-2000:])
+    return json.loads(r.stdout.strip().splitlines()[-1]), compile_note
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,9 +465,13 @@ def main():
     ap.add_argument("--only-wavefront", action="store_true",
                     help="run only the 4M-triangle wavefront scene and print its fields (for profiler runs)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="incoherent rays timed on the host")
+    ap.add_argument("--cpu-baseline-child", metavar="NPZ", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel 3 times on BYTES (calibrates rocprof byte counters)")
     args = ap.parse_args()
+    if args.cpu_baseline_child:
+        cpu_baseline_child(args.cpu_baseline_child, 0)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
               "(the path would shard rays across GPUs with a replicated scene).", file=sys.stderr)
@@ -478,18 +574,24 @@ def main():
     wave = None if (args.only_incoherent or args.no_wavefront) else wavefront_extras(args)
     study = None if (args.only_incoherent or args.no_study) else equal_time_study(args)
     learned = None if args.only_incoherent else learned_kernels(args)
+    c1 = None if args.only_incoherent else c1_render(args)
 
     cpu = None
     if not args.no_cpu_baseline and not args.only_incoherent:
-        import oracle
-        sample = inc[: args.cpu_sample]
-        log("cpu baseline: reference traversal on %d incoherent rays" % sample.size)
-        osc = oracle.OracleScene([(idx, pts)], [(0, np.eye(4, dtype=np.float32))])
-        rate, used = osc.reference_time(sample, threads=0, chunk=65536, reps=5)
-        cpu = {"value": round(rate / 1e6, 4), "unit": "Mrays/s", "cores": used, "kind": "port",
-               "sample": "%d incoherent rays of this workload; the reference's stackless bit-stack traversal "
-                         "and Moller-Trumbore restated in C++ (oracle/), 65,536-ray jobs on a thread pool, "
-                         "median of 5 after one warm-up" % sample.size}
+        log("cpu baseline: reference traversal and renderer on the host (bounded samples)")
+        res, note = run_cpu_baseline(args, idx, pts, coh, inc, sh)
+        tr = res["traversal"]
+        inc_all = tr["incoherent"]["all_threads"]
+        cpu = {"value": inc_all["mrays_per_s"], "unit": "Mrays/s", "cores": inc_all["threads"], "kind": "port",
+               "sample": "%d incoherent rays of this workload (1/8 of that at one thread), the reference's stackless "
+                         "bit-stack traversal and Moller-Trumbore restated in C++ (oracle/), 65,536-ray jobs on a "
+                         "thread_pool clone, median of 5 after one warm-up" % inc_all["rays"],
+               "compile": note,
+               "single_thread_mrays_per_s": tr["incoherent"]["one_thread"]["mrays_per_s"],
+               "per_class": tr,
+               "render_c1": dict(res["render_c1"], config="the reference's plain renderer (pt_generate / pt_trace / "
+                                 "pt_color / pt_ray_filter and pbr.glsl restated, its two sphere lights) on ground "
+                                 "quad + 200 icospheres, 512x512, 1 spp, depth 4, 64-row jobs")}
 
     out = {
         "metric": "incoherent Mrays/s",
@@ -521,6 +623,7 @@ def main():
             "wavefront_4M": wave,
             "equal_time": study,
             "learned_kernels": learned,
+            "c1_plain_render": c1,
         },
     }
     print(json.dumps(out), flush=True)

@@ -75,6 +75,10 @@ def lib():
             "orc_sdf_blend_seed": ([_P, _P], ctypes.c_uint32),
             "orc_sdf_trace": ([_P, _P, ctypes.c_uint32, _P, _SZ, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P,
                                _P, ctypes.c_int], None),
+            "orc_reference_render_time": ([ctypes.POINTER(_PtScene), _P, _P, _P, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double)], ctypes.c_double),
             "orc_render": ([ctypes.POINTER(_PtScene), _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                             ctypes.c_int, ctypes.c_int, _P, ctypes.c_int], ctypes.c_int),
             "orc_fetch_counts": ([_P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_uint32, _P, _SZ, ctypes.c_int, _P, _P,
@@ -338,6 +342,18 @@ class OracleRender:
             rgba[..., :mk.shape[2]] = mk
             self._bokeh = np.ascontiguousarray(rgba)
             self.c.bokeh, self.c.bokeh_w, self.c.bokeh_h = self._bokeh.ctypes.data, rgba.shape[1], rgba.shape[0]
+
+    def reference_render_time(self, view, proj, width, height, max_depth, rows=0, threads=0, reps=5):
+        """CPU baseline: the reference's own plain renderer (pt_generate / pt_trace / pt_color
+        restated, its two sphere lights, its stackless traversal) on rows [0, rows) at 1 spp under
+        the thread_pool clone. Returns (samples/s, rays/s, threads used, radiance checksum)."""
+        v = np.ascontiguousarray(np.asarray(view, np.float32).T).ravel()
+        p = np.ascontiguousarray(np.asarray(proj, np.float32).T).ravel()
+        used, rps, chk = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        sps = lib().orc_reference_render_time(ctypes.byref(self.c), self.scene.accel(), _p(v), _p(p), width, height,
+                                              rows, max_depth, threads, reps, ctypes.byref(used), ctypes.byref(rps),
+                                              ctypes.byref(chk))
+        return sps, rps.value, used.value, chk.value
 
     def render(self, view, proj, width, height, spp, max_depth, sample_base=0, threads=0):
         """Mean radiance per pixel, shape (height, width, 3), float64 (rows from the bottom)."""

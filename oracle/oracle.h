@@ -157,6 +157,16 @@ typedef struct orc_pt_scene {
  * throughput ends the path). Returns the number of rejections (>= 0), or -1 on failure. */
 int orc_render(const orc_pt_scene* s, orc_accel a, const float view_colmajor[16], const float proj_colmajor[16],
                int width, int height, int spp, int sample_base, int max_depth, double* out_rgb, int threads);
+/* CPU baseline of the reference's plain renderer (sequential_pathtracer.cpp:45-64 with
+ * pt_generate/pt_trace/pt_color/pt_ray_filter and pbr.glsl restated, the reference's two sphere
+ * lights, its stackless traversal) on rows [0, rows) of a width x height image at 1 spp with
+ * max_depth bounces, each pass in 64-row jobs on a myrt::thread_pool clone of `threads` workers
+ * (0 = hardware_concurrency). One untimed frame, then the median of `reps`. Returns samples/s;
+ * *rays_per_s gets the median rays (closest + shadow) per second, *checksum the mean radiance
+ * sum of a frame. */
+double orc_reference_render_time(const orc_pt_scene* s, orc_accel a, const float view_colmajor[16],
+                                 const float proj_colmajor[16], int width, int height, int rows, int max_depth,
+                                 int threads, int reps, int* threads_used, double* rays_per_s, double* checksum);
 
 /* The estimator's material model alone (n = shading normal on wo's side, all vectors unit),
  * batched: orc_bsdf_eval gives, for `count` directions wi (3 doubles each), the non-delta BSDF

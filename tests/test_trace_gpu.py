@@ -88,13 +88,19 @@ def test_any_agrees_with_closest(sphere_scene):
     assert np.array_equal(occ.astype(bool), hits["inst"] != m.MISS)
 
 
-def test_tmin_tmax_window(sphere_scene):
+@pytest.mark.parametrize("start", [(0.0, 3.0), (-3.0, 0.0)], ids=["ahead", "behind"])
+def test_tmin_tmax_window(sphere_scene, start):
+    """Hit windows [tmin, tmax] ahead of the origin and reaching behind it (a negative tmin: the
+    far-plane widening works on the distance's magnitude, so boxes behind the origin stay in)."""
     s, o = sphere_scene
     rays = random_rays(5000, 14)
     rng = np.random.default_rng(15)
-    rays["tmin"] = rng.uniform(0.0, 3.0, rays.size).astype(np.float32)
+    rays["tmin"] = rng.uniform(start[0], start[1], rays.size).astype(np.float32)
     rays["tmax"] = rays["tmin"] + rng.uniform(0.0, 5.0, rays.size).astype(np.float32)
-    assert_closest_equal(s.trace_closest(rays), o.brute(rays), "window")
+    ref = o.brute(rays)
+    if start[1] <= 0.0:
+        assert (ref["t"][np.isfinite(ref["t"])] < 0).sum() > 100  # hits behind the origin are in the test
+    assert_closest_equal(s.trace_closest(rays), ref, "window")
 
 
 def test_degenerate_and_empty(ensure_built):
