@@ -16,15 +16,13 @@ struct Cache {
   CacheParams P{};
   float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;
   uint32_t step = 0;
-  uint16_t *h_buf = nullptr, *g_buf = nullptr;
   float *loss_tiles = nullptr, *partial = nullptr, *grad = nullptr;
   size_t cap_batch = 0;
-  uint32_t chunk = 512;  // queries per weight-gradient partial
   int infer_grid = 1024;
 
   ~Cache() {
-    for (void* p : {(void*)P.w, (void*)P.wt, (void*)P.master, (void*)P.m, (void*)P.v, (void*)h_buf, (void*)g_buf,
-                    (void*)loss_tiles, (void*)partial, (void*)grad})
+    for (void* p : {(void*)P.w, (void*)P.wt, (void*)P.master, (void*)P.m, (void*)P.v, (void*)loss_tiles,
+                    (void*)partial, (void*)grad})
       if (p) hipFree(p);
   }
 
@@ -94,16 +92,11 @@ struct Cache {
   int ensure_batch(size_t n) {
     const size_t n_pad = (n + 31) / 32 * 32;
     if (n_pad <= cap_batch) return MRT_OK;
-    for (void* p : {(void*)h_buf, (void*)g_buf, (void*)loss_tiles, (void*)partial})
-      if (p) hipFree(p);
-    h_buf = g_buf = nullptr;
-    loss_tiles = partial = nullptr;
-    const size_t act = size_t(kCacheLayers) * kCacheWidth * n_pad * 2;
-    const size_t chunks = (n_pad + chunk - 1) / chunk;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&h_buf), act);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g_buf), act);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&loss_tiles), n_pad / 32 * 4);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&partial), chunks * kCacheParamsPadded * 4);
+    if (loss_tiles) hipFree(loss_tiles);
+    loss_tiles = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&loss_tiles), n_pad / 32 * 4);
+    if (e == hipSuccess && !partial)
+      e = hipMalloc(reinterpret_cast<void**>(&partial), size_t(cache_train_groups()) * kCacheParamsPadded * 4);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc cache training buffers");
     cap_batch = n_pad;
     return MRT_OK;
@@ -115,7 +108,6 @@ struct Cache {
     if (n >= (1u << 26)) return fail(MRT_ERR_LIMIT, "at most 2^26 training samples per step");
     int rc = ensure_batch(n);
     if (rc != MRT_OK) return rc;
-    P.batch_stride = uint32_t(cap_batch);
     float bc1 = 1.f, bc2 = 1.f;
     if (lr_used != 0.f) {
       ++step;
@@ -123,8 +115,8 @@ struct Cache {
       bc2 = 1.0f - std::pow(beta2, float(step));
     }
     const float scale = 1.0f / float(3 * n);
-    if (launch_cache_train(P, d_q, d_t, uint32_t(n), scale, h_buf, g_buf, loss_tiles, partial, chunk, lr_used, beta1,
-                           beta2, eps, bc1, bc2, grad, s))
+    if (launch_cache_train(P, d_q, d_t, uint32_t(n), scale, loss_tiles, partial, lr_used, beta1, beta2, eps, bc1, bc2,
+                           grad, s))
       return fail(MRT_ERR_HIP, "cache training launch failed");
     return MRT_OK;
   }

@@ -84,8 +84,8 @@ __device__ inline uint32_t pk_bf16(float lo, float hi) {
 // Encodes the query of lane (s, h) (query index qi) into its four B fragments: lane h holds
 // input slots 16 kb + 8 h + (0..7) of fragment kb. Queries at or past n encode as zeros.
 __device__ inline void encode_fragments(const CacheQuery* qs, uint32_t qi, uint32_t n, const CacheParams& P,
-                                        bf16x8 (&b)[4]) {
-  const bool hi = (__lane_id() >> 5) != 0;
+                                        bf16x8 (&b)[4], int lane) {
+  const bool hi = (lane >> 5) != 0;
   float4 q0 = make_float4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
   const bool live = qi < n;
   if (live) {
@@ -121,89 +121,29 @@ __device__ inline void encode_fragments(const CacheQuery* qs, uint32_t qi, uint3
   }
 }
 
-// The encoding of a tile into LDS as [query][slot] rows (row stride kStride), for the training
-// kernel's LDS-operand products.
-__device__ inline void encode_tile(const CacheQuery* qs, uint32_t base, uint32_t n, const CacheParams& P,
-                                   uint16_t* act) {
-  const int lane = __lane_id();
-  const int s = lane & 31, h = lane >> 5;
-  bf16x8 b[4];
-  encode_fragments(qs, base + uint32_t(s), n, P, b);
+// Copies `count` 16-byte records to LDS with THREADS threads, src(rec) giving each record's global
+// address: every load of a thread is issued before the first store waits, so the staging costs one
+// memory round trip instead of one per record.
+template <int THREADS, int COUNT, typename Src>
+__device__ inline void stage_records(u32x4* dst, Src src) {
+  constexpr int kPer = (COUNT + THREADS - 1) / THREADS;
+  u32x4 v[kPer];
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) *reinterpret_cast<bf16x8*>(act + s * kStride + 16 * kb + 8 * h) = b[kb];
-}
-
-// D(64 x 32) = A(64 x 64) * act^T: A rows from `a` (row stride 64, global), act [32][kStride] in LDS.
-__device__ inline void product(const uint16_t* a, const uint16_t* act, f32x16& d0, f32x16& d1) {
-  const int lane = __lane_id();
-  const int r = lane & 31, h = lane >> 5;
-  d0 = f32x16{};
-  d1 = f32x16{};
+  for (int i = 0; i < kPer; ++i) {
+    const int rec = int(threadIdx.x) + i * THREADS;
+    if (rec < COUNT) v[i] = *src(rec);
+  }
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-    const int k = kb * 16 + 8 * h;
-    const bf16x8 b = ld8(act + r * kStride + k);
-    const bf16x8 a0 = ld8(a + r * kW + k);
-    const bf16x8 a1 = ld8(a + (32 + r) * kW + k);
-    d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, d0, 0, 0, 0);
-    d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, d1, 0, 0, 0);
+  for (int i = 0; i < kPer; ++i) {
+    const int rec = int(threadIdx.x) + i * THREADS;
+    if (rec < COUNT) dst[rec] = v[i];
   }
 }
 
-__device__ inline uint32_t pack_bf16(float lo, float hi) { return uint32_t(to_bf16(lo)) | (uint32_t(to_bf16(hi)) << 16); }
-
-// Writes D (neurons x queries) back as [query][neuron] bf16, optionally through ReLU. Lane (s, h)
-// holds neurons 8j + 4h + (0..3) of query s in d0[4j..4j+3] (and 32 more in d1): four consecutive
-// neurons, one 8-byte LDS write. Bit v of the returned mask is (stored value > 0) of d0[v], bit
-// 16 + v that of d1[v]; with use_mask, values whose bit in `mask` is clear are stored as zero
-// (the ReLU derivative of the backward pass).
-__device__ inline uint32_t store_tile(const f32x16& d0, const f32x16& d1, uint16_t* out, bool relu, bool use_mask,
-                                      uint32_t mask) {
-  const int lane = __lane_id();
-  const int s = lane & 31, h = lane >> 5;
-  uint32_t pos = 0;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    float x[2][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int v = 4 * j + i;
-      x[0][i] = d0[v];
-      x[1][i] = d1[v];
-      if (relu) {
-        x[0][i] = fmaxf(x[0][i], 0.f);
-        x[1][i] = fmaxf(x[1][i], 0.f);
-      }
-      if (use_mask) {
-        if (!((mask >> v) & 1u)) x[0][i] = 0.f;
-        if (!((mask >> (16 + v)) & 1u)) x[1][i] = 0.f;
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-      const uint2 w = make_uint2(pack_bf16(x[m][0], x[m][1]), pack_bf16(x[m][2], x[m][3]));
-      *reinterpret_cast<uint2*>(out + s * kStride + 32 * m + 8 * j + 4 * h) = w;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint16_t bits = uint16_t(((i < 2 ? w.x : w.y) >> (16 * (i & 1))) & 0xffffu);
-        if (from_bf16(bits) > 0.f) pos |= 1u << (16 * m + 4 * j + i);
-      }
-    }
-  }
-  return pos;
-}
-
-// Tile [query][neuron] (row stride kStride) to dst[neuron][32 queries]: lane = neuron gathers its
-// column and writes it as 64 contiguous bytes; the wave writes 4 KB in one run.
-__device__ inline void save_columns(const uint16_t* tile, uint16_t* dst) {
-  const int neuron = __lane_id();
-  uint32_t w[16];
-#pragma unroll
-  for (int q = 0; q < 32; q += 2)
-    w[q / 2] = uint32_t(tile[q * kStride + neuron]) | (uint32_t(tile[(q + 1) * kStride + neuron]) << 16);
-  uint4* o = reinterpret_cast<uint4*>(dst + size_t(neuron) * 32);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) o[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+// Global address of forward fragment record (l, mb, kb, lane): W[l][32 mb + lane % 32][16 kb + 8 (lane / 32) ..+8].
+__device__ inline const u32x4* forward_record_src(const CacheParams& P, int rec) {
+  const int lane = rec & 63, kb = (rec >> 6) & 3, lm = rec >> 8, l = lm >> 1, mb = lm & 1;
+  return reinterpret_cast<const u32x4*>(P.w + (size_t(l) * kW + 32 * mb + (lane & 31)) * kW + 16 * kb + 8 * (lane >> 5));
 }
 
 // ---- inference: weights in LDS, activations in registers -----------------------------------
@@ -271,11 +211,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                                                     const uint32_t* n_dev, float* out) {
   __shared__ u32x4 wlds[kFragRecords];
   // Weights to LDS in fragment order: record (l, mb, kb, lane) = W[l][32 mb + lane % 32][16 kb + 8 (lane / 32) ..+8].
-  for (int rec = threadIdx.x; rec < kFragRecords; rec += 256) {
-    const int lane = rec & 63, kb = (rec >> 6) & 3, lm = rec >> 8, l = lm >> 1, mb = lm & 1;
-    const uint16_t* src = P.w + (size_t(l) * kW + 32 * mb + (lane & 31)) * kW + 16 * kb + 8 * (lane >> 5);
-    wlds[rec] = *reinterpret_cast<const u32x4*>(src);
-  }
+  stage_records<256, kFragRecords>(wlds, [&](int rec) { return forward_record_src(P, rec); });
   __syncthreads();
   const int lane = __lane_id();
   const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;  // a device count is capped by the buffer size
@@ -286,8 +222,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     // compiler from hoisting all 44 fragment reads (176 registers) out of it.
     int wofs = 0;
     bf16x8 b[2][4];
-    encode_fragments(qs, pr * 64 + uint32_t(lane & 31), n, P, b[0]);
-    encode_fragments(qs, pr * 64 + 32 + uint32_t(lane & 31), n, P, b[1]);
+    encode_fragments(qs, pr * 64 + uint32_t(lane & 31), n, P, b[0], lane);
+    encode_fragments(qs, pr * 64 + 32 + uint32_t(lane & 31), n, P, b[1], lane);
     f32x16 d[2][2];
 #pragma unroll
     for (int l = 0; l < kLayers; ++l) {
@@ -324,15 +260,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   }
 }
 
-// ---- training: forward and backward of one tile per wave, activations in registers ------------
+// ---- training: forward, backward and weight gradients of one tile per wave ----------------------
 //
 // Weights W (forward, 44 KB) and W^T (backward: layers 1-5, 34 KB) sit in LDS in fragment order.
-// The forward and backward chains keep their operands in registers (to_fragments); each layer's
-// input H_l and output gradient G_l also go to HBM as [layer][tile][neuron][32 queries] columns for
-// the weight-gradient reduction, through a per-wave 4.5 KB LDS tile that transposes them. The
-// ReLU pattern of every hidden layer stays in registers (one bit per value a lane holds).
+// The forward and backward chains keep their operands in registers (to_fragments), and so do the
+// layer inputs H_l (the forward B fragments) and the ReLU patterns of the hidden layers (one bit per
+// value a lane holds). In the backward pass each layer's weight gradient dW_l = G_l H_l^T over the
+// workgroup's 8 tiles (256 queries) is one more set of products: every wave writes its H_l and G_l
+// transposed, [neuron][query], into the workgroup's two LDS tiles, where the run of 8 queries of
+// one neuron is the 16-byte operand that v_mfma_f32_16x16x32_bf16 takes along K; then each wave
+// sums two 16 x 16 blocks of dW_l over all 256 queries (the output layer: one block each for waves
+// 0-3) and adds them to the workgroup's partial in HBM. No block is summed by two waves, so the
+// reduction needs no atomics, and nothing per query goes to HBM: the optimiser step sums one partial
+// per workgroup.
 constexpr int kTransRecords = (4 * 8 + 2) * 64;  // W^T fragments: layers 1-4 full, layer 5 k block 0
 constexpr int kTrainWaves = 8;
+constexpr int kTrainThreads = 64 * kTrainWaves;
+// Row stride (bf16) of the transposed tiles: 256 queries + 8. Rows 132 dwords apart put the 8 rows
+// of a 16-byte read phase in distinct banks, and rows 8 apart (the two lane halves of a write) 32
+// banks apart.
+constexpr int kTQ = kTrainWaves * 32 + 8;
+constexpr int kOutRows = 3;  // rows of the output layer
+
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 __device__ inline int trans_record(int l, int mb, int kb, int lane) {
   return l < 5 ? (((l - 1) * 2 + mb) * 4 + kb) * 64 + lane : (32 + mb) * 64 + lane;
@@ -350,30 +300,27 @@ __device__ inline uint32_t positive_mask(const f32x16& d0, const f32x16& d1) {
   return m;
 }
 
-// Writes a tile's four B fragments (lane (s, h): 8 neurons 16 kb + 8 h of query s) to HBM as
-// [neuron][32 queries] columns through the wave's LDS tile.
-__device__ inline void save_fragments(const bf16x8 (&b)[4], uint16_t* tile, uint16_t* dst) {
-  const int lane = __lane_id();
-  const int s = lane & 31, h = lane >> 5;
+// Writes a tile's four B fragments (lane (s, h): neurons 16 kb + 8 h + (0..7) of query s) to
+// column `col` (the lane's query in the workgroup) of t[neuron][query].
+__device__ inline void write_transposed(const bf16x8 (&b)[4], uint16_t* t, int col, int h) {
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) *reinterpret_cast<bf16x8*>(tile + s * kStride + 16 * kb + 8 * h) = b[kb];
-  __builtin_amdgcn_wave_barrier();
-  save_columns(tile, dst);
-  __builtin_amdgcn_wave_barrier();
+  for (int kb = 0; kb < 4; ++kb) {
+    const u32x4 w = __builtin_bit_cast(u32x4, b[kb]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      t[(16 * kb + 8 * h + j) * kTQ + col] = uint16_t((w[j >> 1] >> (16 * (j & 1))) & 0xffffu);
+  }
 }
 
-__global__ __launch_bounds__(64 * kTrainWaves) void train_kernel(CacheParams P, const CacheQuery* qs,
-                                                                 const float* targets, uint32_t n, float grad_scale,
-                                                                 uint16_t* h_out, uint16_t* g_out, float* loss_out) {
+__global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, const CacheQuery* qs, const float* targets,
+                                                             uint32_t n, float grad_scale, float* loss_out,
+                                                             float* partial) {
   __shared__ u32x4 wf[kFragRecords];
   __shared__ u32x4 wb[kTransRecords];
-  __shared__ __attribute__((aligned(16))) uint16_t trans[kTrainWaves][32 * kStride];
-  for (int rec = threadIdx.x; rec < kFragRecords; rec += 64 * kTrainWaves) {
-    const int lane = rec & 63, kb = (rec >> 6) & 3, lm = rec >> 8, l = lm >> 1, mb = lm & 1;
-    wf[rec] = *reinterpret_cast<const u32x4*>(P.w + (size_t(l) * kW + 32 * mb + (lane & 31)) * kW + 16 * kb +
-                                               8 * (lane >> 5));
-  }
-  for (int rec = threadIdx.x; rec < kTransRecords; rec += 64 * kTrainWaves) {
+  __shared__ __attribute__((aligned(16))) uint16_t th[kW * kTQ];  // H_l^T
+  __shared__ __attribute__((aligned(16))) uint16_t tg[kW * kTQ];  // G_l^T
+  stage_records<kTrainThreads, kFragRecords>(wf, [&](int rec) { return forward_record_src(P, rec); });
+  stage_records<kTrainThreads, kTransRecords>(wb, [&](int rec) {
     const int lane = rec & 63;
     int l, mb, kb;
     if (rec < 32 * 64) {
@@ -386,29 +333,37 @@ __global__ __launch_bounds__(64 * kTrainWaves) void train_kernel(CacheParams P, 
       kb = 0;
       mb = (rec >> 6) - 32;
     }
-    wb[rec] = *reinterpret_cast<const u32x4*>(P.wt + (size_t(l) * kW + 32 * mb + (lane & 31)) * kW + 16 * kb +
-                                               8 * (lane >> 5));
-  }
+    return reinterpret_cast<const u32x4*>(P.wt + (size_t(l) * kW + 32 * mb + (lane & 31)) * kW + 16 * kb +
+                                          8 * (lane >> 5));
+  });
   __syncthreads();
-  const int lane = __lane_id();
-  const int s = lane & 31, h = lane >> 5;
   const int wave = threadIdx.x >> 6;
-  uint16_t* tile = trans[wave];
-  const size_t layer_stride = size_t(kW) * P.batch_stride;
   const uint32_t tiles = (n + 31) / 32;
-  for (uint32_t t = blockIdx.x * kTrainWaves + wave; t < tiles; t += gridDim.x * kTrainWaves) {
+  // Every wave runs the same number of rounds (the workgroup synchronises inside each); a wave past
+  // the last tile encodes zeros, whose gradients are zero.
+  const uint32_t per_round = gridDim.x * kTrainWaves;
+  const uint32_t rounds = (tiles + per_round - 1) / per_round;
+  float* part = partial + size_t(blockIdx.x) * kLayers * kW * kW;
+  for (uint32_t round = 0; round < rounds; ++round) {
+    const uint32_t t = round * per_round + blockIdx.x * kTrainWaves + uint32_t(wave);
     int wofs = 0;
+    // The lane index through an opaque copy: lane-dependent constants (the input encoding's
+    // per-half choices, LDS addresses) are formed in each round instead of kept live across rounds.
+    int lane = __lane_id();
+    asm volatile("" : "+v"(lane));
+    const int s = lane & 31, h = lane >> 5;
     const uint32_t qi = t * 32 + uint32_t(s);
-    const size_t tile_off = size_t(t) * kW * 32;
     bf16x8 b[4];
-    encode_fragments(qs, qi, n, P, b);
-    uint32_t relu_mask[kLayers];  // [l]: positive pattern of z_(l-1) = input H_l of layer l, l = 1..5
+    encode_fragments(qs, qi, n, P, b, lane);
+    bf16x8 hin[kLayers][4];       // H_l, the input of layer l, as B fragments
+    uint32_t relu_mask[kLayers];  // [l]: positive pattern of H_l, l = 1..5
     f32x16 d0, d1;
 #pragma unroll
     for (int l = 0; l < kLayers; ++l) {
-      save_fragments(b, tile, h_out + l * layer_stride + tile_off);
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) hin[l][kb] = b[kb];
       // An opaque zero per layer keeps each layer's fragment reads in place instead of hoisted to
-      // the top of the loop (176 registers for all of them).
+      // the top of the loop.
       asm volatile("" : "+v"(wofs));
       const u32x4* wl = wf + wofs;
       const int mbs = l < kLayers - 1 ? 2 : 1;
@@ -433,78 +388,80 @@ __global__ __launch_bounds__(64 * kTrainWaves) void train_kernel(CacheParams P, 
     if (h == 0 && qi < n) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const float y = d0[c], tg = targets[3 * size_t(qi) + c];
+        const float y = d0[c], tg_ = targets[3 * size_t(qi) + c];
         const float den = y * y + 0.01f;
-        const float e = y - tg;
+        const float e = y - tg_;
         loss += e * e / den;
         g[c] = 2.0f * e / den * grad_scale;
       }
     }
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-    if (lane == 0) loss_out[t] = loss;
+    if (lane == 0 && t < tiles) loss_out[t] = loss;
     {
       const u32x4 z = {0u, 0u, 0u, 0u};
       const u32x4 g0 = {pk_bf16(g[0], g[1]), pk_bf16(g[2], 0.f), 0u, 0u};
       b[0] = __builtin_bit_cast(bf16x8, h == 0 ? g0 : z);
       b[1] = b[2] = b[3] = __builtin_bit_cast(bf16x8, z);
     }
-    // Backward: G_(l-1) = (W_l^T G_l) masked by the ReLU pattern of H_l, l = 5..1.
+    // Backward, l = 5..0: dW_l = G_l H_l^T over the workgroup's queries, and G_(l-1) = (W_l^T G_l)
+    // masked by the ReLU pattern of H_l.
+    const int col = 32 * wave + s;
+    const int r = lane & 15, q8 = 8 * (lane >> 4);  // 16x16x32 operand lane: row r, queries q8..q8+7
 #pragma unroll
     for (int l = kLayers - 1; l >= 0; --l) {
-      save_fragments(b, tile, g_out + l * layer_stride + tile_off);
-      if (l == 0) break;
-      asm volatile("" : "+v"(wofs));
-      const u32x4* wt = wb + wofs;
-      const int kbs = l == kLayers - 1 ? 1 : 4;  // G_5 has rows 0-2 only
-      d0 = d1 = f32x16{};
+      write_transposed(hin[l], th, col, h);
+      write_transposed(b, tg, col, h);
+      if (l > 0) {
+        asm volatile("" : "+v"(wofs));
+        const u32x4* wt = wb + wofs;
+        const int kbs = l == kLayers - 1 ? 1 : 4;  // G_5 has rows 0-2 only
+        d0 = d1 = f32x16{};
 #pragma unroll
-      for (int kb = 0; kb < 4; ++kb) {
-        if (kb >= kbs) break;
-        d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 0, kb, lane)]),
-                                                      b[kb], d0, 0, 0, 0);
-        d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 1, kb, lane)]),
-                                                      b[kb], d1, 0, 0, 0);
+        for (int kb = 0; kb < 4; ++kb) {
+          if (kb >= kbs) break;
+          d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 0, kb, lane)]),
+                                                        b[kb], d0, 0, 0, 0);
+          d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 1, kb, lane)]),
+                                                        b[kb], d1, 0, 0, 0);
+        }
+        to_fragments<2>(d0, d1, b, relu_mask[l]);
       }
-      to_fragments<2>(d0, d1, b, relu_mask[l]);
-    }
-  }
-}
-
-// dW_l partial over a chunk of queries: D(64 out x 64 in) = G_l[out][q] * H_l[in][q]^T, with G_l and
-// H_l in the [tile][neuron][32 queries] layout of train_kernel. Wave w of the workgroup computes the
-// 32 x 32 quadrant (w / 2, w % 2); the output layer's G has rows 0-2 only, so its second row block
-// is zero and skipped.
-__global__ __launch_bounds__(256) void wgrad_kernel(const uint16_t* h_in, const uint16_t* g_in, uint32_t batch_stride,
-                                                    uint32_t n_pad, uint32_t chunk, float* partial) {
-  const int lane = __lane_id();
-  const int r = lane & 31, hh = lane >> 5;
-  const int wave = threadIdx.x >> 6, mb = wave >> 1, nb = wave & 1;
-  const int l = blockIdx.y;
-  const uint32_t c = blockIdx.x;
-  float* out = partial + (size_t(c) * kLayers + l) * kW * kW;
-  f32x16 acc = f32x16{};
-  if (!(l == kLayers - 1 && mb == 1)) {
-    const uint32_t q0 = c * chunk, q1 = min(n_pad, q0 + chunk);
-    const uint16_t* Gl = g_in + size_t(l) * kW * batch_stride;
-    const uint16_t* Hl = h_in + size_t(l) * kW * batch_stride;
-    for (uint32_t q = q0; q < q1; q += 16) {
-      // [tile][neuron][32 queries]: the 8 queries of a fragment lie in one tile row.
-      const uint32_t k = q + 8 * uint32_t(hh);
-      const size_t base = size_t(k >> 5) * kW * 32 + (k & 31u);
-      const bf16x8 a = ld8(Gl + base + size_t(32 * mb + r) * 32);
-      const bf16x8 b = ld8(Hl + base + size_t(32 * nb + r) * 32);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-    }
-  }
+      __syncthreads();
+      // This wave's blocks of dW_l: rows 16 ob + (0..15), columns 16 ib + (0..15) and, for the
+      // hidden layers, 16 (ib + 1) + (0..15). D of the 16x16 product: lane (r, g) holds rows
+      // 16 ob + 4 g + v, column r.
+      const bool out_layer = l == kLayers - 1;
+      if (!out_layer || wave < 4) {
+        const int ob = out_layer ? 0 : wave >> 1;
+        const int ib = out_layer ? wave : 2 * (wave & 1);
+        const uint16_t* ga = tg + (16 * ob + r) * kTQ + q8;
+        const uint16_t* hb = th + (16 * ib + r) * kTQ + q8;
+        f32x4 c0 = f32x4{}, c1 = f32x4{};
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int row = mb * 32 + 8 * (v >> 2) + 4 * hh + (v & 3);
-    out[row * kW + nb * 32 + r] = acc[v];
+        for (int w2 = 0; w2 < kTrainWaves; ++w2) {
+          const bf16x8 a = ld8(ga + 32 * w2);
+          c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ld8(hb + 32 * w2), c0, 0, 0, 0);
+          if (!out_layer) c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ld8(hb + 16 * kTQ + 32 * w2), c1, 0, 0, 0);
+        }
+        float* pl = part + l * kW * kW;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * ob + 4 * (lane >> 4) + v;
+          if (out_layer && row >= kOutRows) continue;
+          float* p0 = pl + row * kW + 16 * ib + r;
+          *p0 = round == 0 ? c0[v] : *p0 + c0[v];
+          if (!out_layer) p0[16] = round == 0 ? c1[v] : p0[16] + c1[v];
+        }
+      }
+      __syncthreads();
+    }
   }
 }
 
-// Sums the chunk partials of 64 parameters per workgroup (4 slices of the chunks per parameter,
-// combined in LDS) and takes the Adam step.
+// Sums the training workgroups' partials of 64 parameters per workgroup (4 slices of the partials
+// per parameter, combined in LDS) and takes the Adam step.
+// Launched over the parameters of layers 0-4 and rows 0-2 of layer 5 (the rest of the output layer
+// is padding and stays zero).
 __global__ __launch_bounds__(256) void adam_kernel(CacheParams P, const float* partial, uint32_t chunks, float lr,
                                                    float beta1, float beta2, float eps, float bc1, float bc2,
                                                    float* grad_out) {
@@ -518,7 +475,6 @@ __global__ __launch_bounds__(256) void adam_kernel(CacheParams P, const float* p
   if (slice != 0) return;
   const uint32_t i = p;
   const uint32_t l = i / (kW * kW), rem = i % (kW * kW), row = rem / kW, col = rem % kW;
-  if (l == kLayers - 1 && row >= 3) return;  // padding rows of the output layer stay zero
   g = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
   if (grad_out) grad_out[i] = g;
   if (lr == 0.f) return;
@@ -556,30 +512,31 @@ int launch_cache_infer(const CacheParams& P, const CacheQuery* q, uint32_t n, co
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_cache_train(const CacheParams& P, const CacheQuery* q, const float* targets, uint32_t n, float grad_scale,
-                       uint16_t* h_buf, uint16_t* g_buf, float* loss_tiles, float* partial, uint32_t chunk,
-                       float lr, float beta1, float beta2, float eps, float bc1, float bc2, float* grad_out,
-                       hipStream_t s) {
-  const uint32_t tiles = (n + 31) / 32;
-  const uint32_t n_pad = tiles * 32;
+int cache_train_groups() {
   static int max_groups = 0;  // one resident workgroup per CU (LDS-limited): a persistent grid
   if (!max_groups) {
     int dev = 0, cus = 0, per_cu = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mlpk::train_kernel, 64 * mlpk::kTrainWaves, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mlpk::train_kernel, mlpk::kTrainThreads, 0) !=
             hipSuccess ||
         per_cu < 1)
       per_cu = 1;
-    max_groups = std::max(1, cus * per_cu);
+    max_groups = std::min(kCacheMaxTrainGroups, std::max(1, cus * per_cu));
   }
-  const uint32_t groups = std::min<uint32_t>(uint32_t(max_groups), (tiles + mlpk::kTrainWaves - 1) / mlpk::kTrainWaves);
-  hipLaunchKernelGGL(mlpk::train_kernel, dim3(groups), dim3(64 * mlpk::kTrainWaves), 0, s, P, q, targets, n,
-                     grad_scale, h_buf, g_buf, loss_tiles);
-  const uint32_t chunks = (n_pad + chunk - 1) / chunk;
-  hipLaunchKernelGGL(mlpk::wgrad_kernel, dim3(chunks, 6), dim3(256), 0, s, h_buf, g_buf, P.batch_stride, n_pad, chunk,
-                     partial);
-  hipLaunchKernelGGL(mlpk::adam_kernel, dim3(6 * 64 * 64 / 64), dim3(256), 0, s, P, partial, chunks, lr, beta1, beta2,
+  return max_groups;
+}
+
+int launch_cache_train(const CacheParams& P, const CacheQuery* q, const float* targets, uint32_t n, float grad_scale,
+                       float* loss_tiles, float* partial, float lr, float beta1, float beta2, float eps, float bc1,
+                       float bc2, float* grad_out, hipStream_t s) {
+  const uint32_t tiles = (n + 31) / 32;
+  const uint32_t groups = std::min<uint32_t>(uint32_t(cache_train_groups()),
+                                             (tiles + mlpk::kTrainWaves - 1) / mlpk::kTrainWaves);
+  hipLaunchKernelGGL(mlpk::train_kernel, dim3(groups), dim3(mlpk::kTrainThreads), 0, s, P, q, targets, n, grad_scale,
+                     loss_tiles, partial);
+  constexpr uint32_t kTrained = (mlpk::kLayers - 1) * mlpk::kW * mlpk::kW + mlpk::kOutRows * mlpk::kW;
+  hipLaunchKernelGGL(mlpk::adam_kernel, dim3(kTrained / 64), dim3(256), 0, s, P, partial, groups, lr, beta1, beta2,
                      eps, bc1, bc2, grad_out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -27,7 +27,6 @@ struct CacheParams {
   float* v;          // Adam second moment
   float lo[3];       // position box of the encoding
   float inv_ext[3];  // 1 / box extent
-  uint32_t batch_stride;  // queries per row of the training activation buffers
 };
 
 constexpr int kCacheLayers = 6;
@@ -40,10 +39,13 @@ constexpr int kCacheParams = 5 * 64 * 64 + 3 * 64;                            //
 int cache_infer_blocks_per_cu();
 int launch_cache_infer(const CacheParams& P, const CacheQuery* q, uint32_t n, const uint32_t* n_dev, float* out,
                        int grid, hipStream_t s);
+// Workgroups of a training launch (at most kCacheMaxTrainGroups); `partial` holds that many
+// kCacheParamsPadded-float weight-gradient partials.
+constexpr int kCacheMaxTrainGroups = 512;
+int cache_train_groups();
 int launch_cache_train(const CacheParams& P, const CacheQuery* q, const float* targets, uint32_t n, float grad_scale,
-                       uint16_t* h_buf, uint16_t* g_buf, float* loss_tiles, float* partial, uint32_t chunk,
-                       float lr, float beta1, float beta2, float eps, float bc1, float bc2, float* grad_out,
-                       hipStream_t s);
+                       float* loss_tiles, float* partial, float lr, float beta1, float beta2, float eps, float bc1,
+                       float bc2, float* grad_out, hipStream_t s);
 int launch_cache_sync(const CacheParams& P, hipStream_t s);
 
 }  // namespace mrt
