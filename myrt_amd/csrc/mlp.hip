@@ -169,22 +169,14 @@ __device__ inline uint32_t relu_pk(uint32_t x) {
   return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, z));
 }
 
-// Keeps the bf16 halves of a packed pair whose bits (bit, bit + 1) of `mask` are set.
-__device__ inline uint32_t mask_pk(uint32_t x, uint32_t mask, int bit) {
-  const uint32_t t = (mask >> bit) & 3u;
-  return x & ((0u - (t & 1u)) & 0xffffu | ((0u - (t >> 1)) << 16));
-}
-
 // D (64 neurons x 32 queries, in d0 = rows 0-31 and d1 = rows 32-63) to the next layer's four B
-// fragments: MODE 0 as is, 1 through ReLU, 2 keeping the values whose bit in `mask` is set (bit v
-// of mask is d0[v], bit 16 + v is d1[v]: the ReLU derivative in the backward pass).
+// fragments: MODE 0 as is, 1 through ReLU.
 template <int MODE = 1>
-__device__ inline void to_fragments(const f32x16& d0, const f32x16& d1, bf16x8 (&b)[4], uint32_t mask = 0) {
+__device__ inline void to_fragments(const f32x16& d0, const f32x16& d1, bf16x8 (&b)[4]) {
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
     const f32x16& d = kb < 2 ? d0 : d1;
     const int v = 8 * (kb & 1);  // d[v..v+3]: rows 16kb' + 4h + (0..3); d[v+4..v+7]: rows 16kb' + 8 + 4h + (0..3)
-    const int mb = 16 * (kb >> 1);
     uint32_t x0 = pk_bf16(d[v + 0], d[v + 1]), x1 = pk_bf16(d[v + 2], d[v + 3]);
     uint32_t y0 = pk_bf16(d[v + 4], d[v + 5]), y1 = pk_bf16(d[v + 6], d[v + 7]);
     if (MODE == 1) {
@@ -192,11 +184,6 @@ __device__ inline void to_fragments(const f32x16& d0, const f32x16& d1, bf16x8 (
       x1 = relu_pk(x1);
       y0 = relu_pk(y0);
       y1 = relu_pk(y1);
-    } else if (MODE == 2) {
-      x0 = mask_pk(x0, mask, mb + v);
-      x1 = mask_pk(x1, mask, mb + v + 2);
-      y0 = mask_pk(y0, mask, mb + v + 4);
-      y1 = mask_pk(y1, mask, mb + v + 6);
     }
     // Lanes 32-63 of x trade with lanes 0-31 of y: h = 0 keeps x (rows +0..3) and gets the other
     // half's x (rows +4..7); h = 1 gets the other half's y (rows +8..11) and keeps its y (+12..15).
@@ -264,8 +251,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 //
 // Weights W (forward, 44 KB) and W^T (backward: layers 1-5, 34 KB) sit in LDS in fragment order.
 // The forward and backward chains keep their operands in registers (to_fragments), and so do the
-// layer inputs H_l (the forward B fragments) and the ReLU patterns of the hidden layers (one bit per
-// value a lane holds). In the backward pass each layer's weight gradient dW_l = G_l H_l^T over the
+// layer inputs H_l (the forward B fragments), whose nonzero entries are also the ReLU derivative
+// (keep_active). In the backward pass each layer's weight gradient dW_l = G_l H_l^T over the
 // workgroup's 8 tiles (256 queries) is one more set of products: every wave writes its H_l and G_l
 // transposed, [neuron][query], into the workgroup's two LDS tiles, where the run of 8 queries of
 // one neuron is the 16-byte operand that v_mfma_f32_16x16x32_bf16 takes along K; then each wave
@@ -277,8 +264,8 @@ constexpr int kTransRecords = (4 * 8 + 2) * 64;  // W^T fragments: layers 1-4 fu
 constexpr int kTrainWaves = 8;
 constexpr int kTrainThreads = 64 * kTrainWaves;
 // Row stride (bf16) of the transposed tiles: 256 queries + 8. Rows 132 dwords apart put the 8 rows
-// of a 16-byte read phase in distinct banks, and rows 8 apart (the two lane halves of a write) 32
-// banks apart.
+// of a 16-byte read phase in distinct banks, and rows 4, 8 and 12 apart (the row groups of one
+// 32-bit write) 16, 32 and 48 banks apart.
 constexpr int kTQ = kTrainWaves * 32 + 8;
 constexpr int kOutRows = 3;  // rows of the output layer
 
@@ -288,27 +275,53 @@ __device__ inline int trans_record(int l, int mb, int kb, int lane) {
   return l < 5 ? (((l - 1) * 2 + mb) * 4 + kb) * 64 + lane : (32 + mb) * 64 + lane;
 }
 
-// Positive-value pattern of D (the ReLU derivative): bit v of d0, 16 + v of d1. Rounding to bf16
-// keeps the sign and maps no positive value above the denormal range to zero.
-__device__ inline uint32_t positive_mask(const f32x16& d0, const f32x16& d1) {
-  uint32_t m = 0;
+// The ReLU derivative: keeps the values of G (packed bf16 fragments) whose layer input H (the
+// same positions of the forward fragments, after ReLU, so never negative) is nonzero. A positive
+// pre-activation rounds to a nonzero bf16 unless it is below 2^-133.
+__device__ inline void keep_active(bf16x8 (&g)[4], const bf16x8 (&hf)[4]) {
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    if (d0[v] > 0.f) m |= 1u << v;
-    if (d1[v] > 0.f) m |= 1u << (16 + v);
+  for (int kb = 0; kb < 4; ++kb) {
+    u32x4 gw = __builtin_bit_cast(u32x4, g[kb]);
+    const u32x4 hw = __builtin_bit_cast(u32x4, hf[kb]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const u16x2 one = {1, 1};
+      // A scalar copy first: clang's __builtin_bit_cast of a vector element reads element 0.
+      const uint32_t hv = hw[i];
+      const u16x2 nz = __builtin_elementwise_min(__builtin_bit_cast(u16x2, hv), one);  // 0 or 1 per half
+      gw[i] &= __builtin_bit_cast(uint32_t, u16x2{0, 0} - nz);                              // 0 or 0xffff
+    }
+    g[kb] = __builtin_bit_cast(bf16x8, gw);
   }
-  return m;
 }
 
 // Writes a tile's four B fragments (lane (s, h): neurons 16 kb + 8 h + (0..7) of query s) to
-// column `col` (the lane's query in the workgroup) of t[neuron][query].
-__device__ inline void write_transposed(const bf16x8 (&b)[4], uint16_t* t, int col, int h) {
+// t[neuron][query] at the queries 32 wave + s, as 32-bit pairs of queries: lanes s and s ^ 1 trade
+// halves so that the even lane writes neurons +0..3 and the odd lane neurons +4..7 of the pair
+// (s & ~1, s | 1). With rows 132 dwords apart, the four row groups of one write (even and odd lane,
+// h = 0 and 1: rows +0, +4, +8, +12) fall on four disjoint quarters of the banks.
+__device__ inline void write_transposed(const bf16x8 (&b)[4], uint16_t* t, int wave, int lane) {
+  const int s = lane & 31, h = lane >> 5;
+  const bool odd = (lane & 1) != 0;
+  // v_perm selectors over (src0 = received word, src1 = own word): the low halves (first word) and
+  // the high halves (second word) of the pair, own query first on the even lane, second on the odd.
+  const uint32_t sel_lo = odd ? 0x01000504u : 0x05040100u;
+  const uint32_t sel_hi = odd ? 0x03020706u : 0x07060302u;
+  uint32_t* row0 = reinterpret_cast<uint32_t*>(t + (8 * h + 4 * (odd ? 1 : 0)) * kTQ + 32 * wave + (s & ~1));
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) {
     const u32x4 w = __builtin_bit_cast(u32x4, b[kb]);
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-      t[(16 * kb + 8 * h + j) * kTQ + col] = uint16_t((w[j >> 1] >> (16 * (j & 1))) & 0xffffu);
+    for (int p = 0; p < 2; ++p) {
+      // Word p holds neurons 2p, 2p + 1; word p + 2 neurons 2p + 4, 2p + 5.
+      const uint32_t send = odd ? w[p] : w[p + 2];
+      const uint32_t own = odd ? w[p + 2] : w[p];
+      const uint32_t recv = uint32_t(__builtin_amdgcn_mov_dpp(int(send), 0xB1, 0xF, 0xF, false));  // lane ^ 1
+      uint32_t* r = row0 + (16 * kb + 2 * p) * (kTQ / 2);
+      r[0] = __builtin_amdgcn_perm(recv, own, sel_lo);
+      r[kTQ / 2] = __builtin_amdgcn_perm(recv, own, sel_hi);
+    }
   }
 }
 
@@ -337,7 +350,8 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
                                           8 * (lane >> 5));
   });
   __syncthreads();
-  const int wave = threadIdx.x >> 6;
+  // The wave index in a scalar register: addresses and branches that depend on it are uniform.
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const uint32_t tiles = (n + 31) / 32;
   // Every wave runs the same number of rounds (the workgroup synchronises inside each); a wave past
   // the last tile encodes zeros, whose gradients are zero.
@@ -355,13 +369,15 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
     const uint32_t qi = t * 32 + uint32_t(s);
     bf16x8 b[4];
     encode_fragments(qs, qi, n, P, b, lane);
-    bf16x8 hin[kLayers][4];       // H_l, the input of layer l, as B fragments
-    uint32_t relu_mask[kLayers];  // [l]: positive pattern of H_l, l = 1..5
+    bf16x8 hin[kLayers][4];       // H_l, the input of layer l, as B fragments (H_0 is encoded again
+                                  // when the backward pass reaches it, rather than held throughout)
     f32x16 d0, d1;
 #pragma unroll
     for (int l = 0; l < kLayers; ++l) {
+      if (l > 0) {
 #pragma unroll
-      for (int kb = 0; kb < 4; ++kb) hin[l][kb] = b[kb];
+        for (int kb = 0; kb < 4; ++kb) hin[l][kb] = b[kb];
+      }
       // An opaque zero per layer keeps each layer's fragment reads in place instead of hoisted to
       // the top of the loop.
       asm volatile("" : "+v"(wofs));
@@ -376,10 +392,7 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
           d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wl[frag_record(l, 1, kb, lane)]),
                                                         b[kb], d1, 0, 0, 0);
       }
-      if (l < kLayers - 1) {
-        relu_mask[l + 1] = positive_mask(d0, d1);
-        to_fragments(d0, d1, b);
-      }
+      if (l < kLayers - 1) to_fragments(d0, d1, b);
     }
     // Loss: relative L2 (y - t)^2 / (sg(y)^2 + 0.01) per channel; dL/dy = 2 (y - t) / (y^2 + 0.01).
     // Outputs y (rows 0-2) are d0[0..2] of lanes h = 0; G_5 as fragments: kb 0, lane h = 0 slots 0-2.
@@ -395,7 +408,9 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
         g[c] = 2.0f * e / den * grad_scale;
       }
     }
-    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    // Butterfly sum over the wave (ds_bpermute addressed from the round's lane copy).
+    for (int off = 32; off > 0; off >>= 1)
+      loss += __int_as_float(__builtin_amdgcn_ds_bpermute((lane ^ off) << 2, __float_as_int(loss)));
     if (lane == 0 && t < tiles) loss_out[t] = loss;
     {
       const u32x4 z = {0u, 0u, 0u, 0u};
@@ -405,12 +420,16 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
     }
     // Backward, l = 5..0: dW_l = G_l H_l^T over the workgroup's queries, and G_(l-1) = (W_l^T G_l)
     // masked by the ReLU pattern of H_l.
-    const int col = 32 * wave + s;
-    const int r = lane & 15, q8 = 8 * (lane >> 4);  // 16x16x32 operand lane: row r, queries q8..q8+7
 #pragma unroll
     for (int l = kLayers - 1; l >= 0; --l) {
-      write_transposed(hin[l], th, col, h);
-      write_transposed(b, tg, col, h);
+      // A fresh opaque lane copy per layer, so that this layer's addresses are formed here and not
+      // held in registers through the forward pass.
+      int ln = __lane_id();
+      asm volatile("" : "+v"(ln));
+      const int r = ln & 15, q8 = 8 * (ln >> 4);  // 16x16x32 operand lane: row r, queries q8..q8+7
+      if (l == 0) encode_fragments(qs, t * 32 + uint32_t(ln & 31), n, P, hin[0], ln);
+      write_transposed(hin[l], th, wave, ln);
+      write_transposed(b, tg, wave, ln);
       if (l > 0) {
         asm volatile("" : "+v"(wofs));
         const u32x4* wt = wb + wofs;
@@ -419,12 +438,13 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) {
           if (kb >= kbs) break;
-          d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 0, kb, lane)]),
+          d0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 0, kb, ln)]),
                                                         b[kb], d0, 0, 0, 0);
-          d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 1, kb, lane)]),
+          d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, wt[trans_record(l, 1, kb, ln)]),
                                                         b[kb], d1, 0, 0, 0);
         }
-        to_fragments<2>(d0, d1, b, relu_mask[l]);
+        to_fragments<0>(d0, d1, b);
+        keep_active(b, hin[l]);
       }
       __syncthreads();
       // This wave's blocks of dW_l: rows 16 ob + (0..15), columns 16 ib + (0..15) and, for the
@@ -439,6 +459,9 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
         f32x4 c0 = f32x4{}, c1 = f32x4{};
 #pragma unroll
         for (int w2 = 0; w2 < kTrainWaves; ++w2) {
+          // Operand reads in two groups of four tiles (a compiler-level fence between them) bound
+          // the registers they occupy.
+          if (w2 == kTrainWaves / 2) asm volatile("" ::: "memory");
           const bf16x8 a = ld8(ga + 32 * w2);
           c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ld8(hb + 32 * w2), c0, 0, 0, 0);
           if (!out_layer) c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ld8(hb + 16 * kTQ + 32 * w2), c1, 0, 0, 0);
@@ -446,7 +469,7 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
         float* pl = part + l * kW * kW;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int row = 16 * ob + 4 * (lane >> 4) + v;
+          const int row = 16 * ob + 4 * (ln >> 4) + v;
           if (out_layer && row >= kOutRows) continue;
           float* p0 = pl + row * kW + 16 * ib + r;
           *p0 = round == 0 ? c0[v] : *p0 + c0[v];

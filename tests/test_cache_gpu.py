@@ -36,9 +36,12 @@ def test_inference_matches_oracle(cache):
     assert rel(y, ref) < TOL
 
 
-def test_gradient_matches_oracle(cache):
-    q = M.random_queries(777, 6)
-    t = np.random.default_rng(7).uniform(0, 2, (777, 3)).astype(np.float32)
+@pytest.mark.parametrize("n", [777, (1 << 17) + 777])
+def test_gradient_matches_oracle(cache, n):
+    """777 samples: one workgroup, a ragged last tile. 2^17 + 777: every workgroup sums several
+    rounds of tiles into its partial, and the last round has waves past the end."""
+    q = M.random_queries(n, 6)
+    t = np.random.default_rng(7).uniform(0, 2, (n, 3)).astype(np.float32)
     g, loss = cache.gradient(q, t)
     p = cache.params().astype(np.float64)
     ref_loss, ref_g = M.loss_and_grad(p, M.encode(q, (-1, -1, -1), (1, 1, 1)), t, bf16=True)
