@@ -271,6 +271,10 @@ constexpr int kOutRows = 3;  // rows of the output layer
 
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
+// Offset (floats) of dW block (rows 16 ob.., columns 16 ib..) in a layer's partial: 16 blocks of
+// 64 lane records of 4 floats (lane (r, g) holds rows 16 ob + 4 g + (0..3) of column 16 ib + r).
+__device__ inline int partial_block(int ob, int ib) { return (ob * 4 + ib) * 256; }
+
 __device__ inline int trans_record(int l, int mb, int kb, int lane) {
   return l < 5 ? (((l - 1) * 2 + mb) * 4 + kb) * 64 + lane : (32 + mb) * 64 + lane;
 }
@@ -456,6 +460,18 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
         const int ib = out_layer ? wave : 2 * (wave & 1);
         const uint16_t* ga = tg + (16 * ob + r) * kTQ + q8;
         const uint16_t* hb = th + (16 * ib + r) * kTQ + q8;
+        // The partial of this wave's blocks: records of 16 bytes in the product's own lane layout
+        // (one coalesced store per block); the optimiser step maps them back to [row][column].
+        // Earlier rounds' sums are loaded before the products, so that no branch separates the
+        // products from the add and the store that read their results (a conditional branch there
+        // has come out of the compiler without the wait states the results need).
+        f32x4* p0 = reinterpret_cast<f32x4*>(part + l * kW * kW + partial_block(ob, ib)) + ln;
+        f32x4 prev0 = f32x4{}, prev1 = f32x4{};
+        if (round > 0) {
+          prev0 = p0[0];
+          if (!out_layer) prev1 = p0[64];
+        }
+        asm volatile("" : "+v"(prev0), "+v"(prev1));
         f32x4 c0 = f32x4{}, c1 = f32x4{};
 #pragma unroll
         for (int w2 = 0; w2 < kTrainWaves; ++w2) {
@@ -466,15 +482,8 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
           c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ld8(hb + 32 * w2), c0, 0, 0, 0);
           if (!out_layer) c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ld8(hb + 16 * kTQ + 32 * w2), c1, 0, 0, 0);
         }
-        float* pl = part + l * kW * kW;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * ob + 4 * (ln >> 4) + v;
-          if (out_layer && row >= kOutRows) continue;
-          float* p0 = pl + row * kW + 16 * ib + r;
-          *p0 = round == 0 ? c0[v] : *p0 + c0[v];
-          if (!out_layer) p0[16] = round == 0 ? c1[v] : p0[16] + c1[v];
-        }
+        p0[0] = c0 + prev0;
+        if (!out_layer) p0[64] = c1 + prev1;
       }
       __syncthreads();
     }
@@ -483,21 +492,27 @@ __global__ __launch_bounds__(kTrainThreads) void train_kernel(CacheParams P, con
 
 // Sums the training workgroups' partials of 64 parameters per workgroup (4 slices of the partials
 // per parameter, combined in LDS) and takes the Adam step.
-// Launched over the parameters of layers 0-4 and rows 0-2 of layer 5 (the rest of the output layer
-// is padding and stays zero).
+// Launched over the partial slots of layers 0-4 and of the output layer's first four blocks (rows
+// 0-15, of which rows 0-2 are parameters; the rest of the output layer is padding and stays zero).
+// Slot (block b, lane, v) of a layer is parameter row 16 (b / 4) + 4 (lane / 16) + v, column
+// 16 (b % 4) + lane % 16.
 __global__ __launch_bounds__(256) void adam_kernel(CacheParams P, const float* partial, uint32_t chunks, float lr,
                                                    float beta1, float beta2, float eps, float bc1, float bc2,
                                                    float* grad_out) {
   __shared__ float part[4][64];
-  const uint32_t p = blockIdx.x * 64 + (threadIdx.x & 63), slice = threadIdx.x >> 6;
+  const uint32_t q = blockIdx.x * 64 + (threadIdx.x & 63), slice = threadIdx.x >> 6;
+  const uint32_t l = q / (kW * kW), slot = q % (kW * kW), blk = slot >> 8, ln = (slot >> 2) & 63;
+  const uint32_t row = 16 * (blk >> 2) + 4 * (ln >> 4) + (slot & 3), col = 16 * (blk & 3) + (ln & 15);
+  const bool padding = l == kLayers - 1 && row >= kOutRows;
   float g = 0.f;
+  if (!padding) {
 #pragma unroll 8
-  for (uint32_t c = slice; c < chunks; c += 4) g += partial[size_t(c) * kLayers * kW * kW + p];
+    for (uint32_t c = slice; c < chunks; c += 4) g += partial[size_t(c) * kLayers * kW * kW + q];
+  }
   part[slice][threadIdx.x & 63] = g;
   __syncthreads();
-  if (slice != 0) return;
-  const uint32_t i = p;
-  const uint32_t l = i / (kW * kW), rem = i % (kW * kW), row = rem / kW, col = rem % kW;
+  if (slice != 0 || padding) return;
+  const uint32_t i = l * kW * kW + row * kW + col;
   g = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
   if (grad_out) grad_out[i] = g;
   if (lr == 0.f) return;
@@ -558,8 +573,8 @@ int launch_cache_train(const CacheParams& P, const CacheQuery* q, const float* t
                                              (tiles + mlpk::kTrainWaves - 1) / mlpk::kTrainWaves);
   hipLaunchKernelGGL(mlpk::train_kernel, dim3(groups), dim3(mlpk::kTrainThreads), 0, s, P, q, targets, n, grad_scale,
                      loss_tiles, partial);
-  constexpr uint32_t kTrained = (mlpk::kLayers - 1) * mlpk::kW * mlpk::kW + mlpk::kOutRows * mlpk::kW;
-  hipLaunchKernelGGL(mlpk::adam_kernel, dim3(kTrained / 64), dim3(256), 0, s, P, partial, groups, lr, beta1, beta2,
+  constexpr uint32_t kSlots = (mlpk::kLayers - 1) * mlpk::kW * mlpk::kW + 4 * 256;  // hidden layers, output blocks
+  hipLaunchKernelGGL(mlpk::adam_kernel, dim3(kSlots / 64), dim3(256), 0, s, P, partial, groups, lr, beta1, beta2,
                      eps, bc1, bc2, grad_out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -1,9 +1,11 @@
 """GPU radiance cache (bf16 MFMA, fp32 accumulation) against the FP64 oracle.
 
-Bar: initial weights bit-equal to the oracle's initialiser; outputs and gradients within the
-stated tolerance of the oracle run with the same bf16 roundings (2e-2 relative in the L2 norm,
-set by fp32 summation order and bf16 rounding flips); a few Adam steps track the oracle; the
-network learns a smooth radiance field."""
+Bar: initial weights bit-equal to the oracle's initialiser; outputs within 2e-2 relative in the
+L2 norm of the oracle run with the same bf16 roundings; gradients within 5e-3, both in the L2
+norm and entry by entry against the layer's largest entry (fp32 summation order and bf16 rounding
+flips give at most 1.5e-3 on these inputs, while a weight-gradient block that loses one tile's
+contribution is off by several per cent); a few Adam steps track the oracle; the network learns a
+smooth radiance field."""
 import numpy as np
 import pytest
 
@@ -12,6 +14,7 @@ from oracle import mlp_ref as M
 
 pytestmark = pytest.mark.gpu
 TOL = 2e-2
+GRAD_TOL = 5e-3
 
 
 def rel(a, b):
@@ -48,7 +51,9 @@ def test_gradient_matches_oracle(cache, n):
     assert abs(loss - ref_loss) / ref_loss < TOL
     k = 0
     for l, (r, c) in enumerate(M.SHAPES):
-        assert rel(g[k:k + r * c], ref_g[k:k + r * c]) < TOL, "layer %d" % l
+        a, b = g[k:k + r * c], ref_g[k:k + r * c]
+        assert rel(a, b) < GRAD_TOL, "layer %d" % l
+        assert np.abs(a - b).max() < GRAD_TOL * np.abs(b).max(), "layer %d" % l
         k += r * c
 
 
