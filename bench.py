@@ -211,13 +211,19 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
         m.synchronize()
         return out, time.perf_counter() - t0
 
+    def median_time(fn, k=3):
+        """Median of k timed runs (a single render's time varies by several per cent); the last
+        run's output is what the caller reads."""
+        return float(np.median([timed(fn)[1] for _ in range(k)]))
+
     def fresh(r):
         r.invalidate()
         return r
 
     def fit(render_spp, T, t_extra, first):
         """The largest sample count whose render plus t_extra fits T (1 % slack), by rescaling
-        each render's time (at most 6 renders); returns (spp, seconds)."""
+        each candidate's time (at most 6 candidates, each the median of 3 renders); returns
+        (spp, seconds)."""
         spp, times, last = max(1, first), {}, None
         while spp not in times and len(times) < 6:
             times[spp] = render_spp(spp)
@@ -253,7 +259,7 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
     cold_r = renderer()
     cold_r.render(s, W, H, 16)  # allocations stay out of the timings
     for b in budgets:
-        _, T = timed(lambda: fresh(plain).render(s, W, H, b))
+        T = median_time(lambda: fresh(plain).render(s, W, H, b))
         row = {"budget_spp": b, "budget_s": round(T, 4), "plain_mse": mse(plain.image())}
         # The reference's own post filter (bilateral, the app's sigmas) as the classical baseline.
         img, t_bl = timed(lambda: m.post_process(plain.image(), [m.POST_BILATERAL], 10.0, 0.05))
@@ -263,13 +269,13 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
         if den is not None:
             _, t_dn = timed(lambda: plain.denoise(den))
             row["denoise_s"] = round(t_dn, 4)
-            spp_pd, t_pd = fit(lambda k: timed(lambda: fresh(plain).render(s, W, H, k))[1], T, t_dn,
+            spp_pd, t_pd = fit(lambda k: median_time(lambda: fresh(plain).render(s, W, H, k)), T, t_dn,
                                int(b * (T - t_dn) / T))
             img, t_dn2 = timed(lambda: plain.denoise(den))
             row.update({"plain_denoised_spp": spp_pd, "plain_denoised_s": round(t_pd + t_dn2, 4),
                         "plain_denoised_mse": mse(img)})
         # Warm cache: as many samples as fit (with the denoiser when there is one).
-        spp_c, tc = fit(lambda k: timed(lambda: fresh(cr).render(s, W, H, k))[1], T, t_dn,
+        spp_c, tc = fit(lambda k: median_time(lambda: fresh(cr).render(s, W, H, k)), T, t_dn,
                         int((T - t_dn) / t16 * 16))
         row.update({"cache_spp": spp_c, "cache_s": round(tc, 4), "cache_mse": mse(cr.image())})
         if den is not None:
@@ -279,12 +285,14 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
             row["ratio_plain_over_plain_denoised"] = round(row["plain_mse"] / row["plain_denoised_mse"], 3)
             # Cold cache: a fresh network for every trial, trained only inside the budget.
             def cold_render(k):
-                cold = m.RadianceCache(seed=2)
-                cold.set_bounds((-12.5, -7.0, -9.5), (12.5, 6.0, 9.5))
-                cold_r.set_cache(cold, mode=2)
-                t = timed(lambda: fresh(cold_r).render(s, W, H, k))[1]
-                cold_render.keep = cold
-                return t
+                ts = []
+                for _ in range(3):  # the median of 3, each from a fresh network
+                    cold = m.RadianceCache(seed=2)
+                    cold.set_bounds((-12.5, -7.0, -9.5), (12.5, 6.0, 9.5))
+                    cold_r.set_cache(cold, mode=2)
+                    ts.append(timed(lambda: fresh(cold_r).render(s, W, H, k))[1])
+                    cold_render.keep = cold
+                return float(np.median(ts))
             spp_k, tk = fit(cold_render, T, t_dn, spp_c)
             img, t = timed(lambda: cold_r.denoise(den))
             row.update({"cold_cache_spp": spp_k, "cold_cache_s": round(tk + t, 4), "cold_cache_denoised_mse": mse(img),
