@@ -247,7 +247,7 @@ __global__ __launch_bounds__(ROWS * 64) void conv_tile_kernel(ConvArgs a, int gr
       const int i = tid + k * kThreads;
       if (i < kWItems) {
         const int hf = i & 1, rest = i >> 1;
-        const int co = rest / 9, tap = rest - co * 9;
+        const int tap = rest / kCout, co = rest - tap * kCout;
         r_w[k] = *reinterpret_cast<const h8*>(w + (size_t(m0 + co) * 9 + tap) * cin + c + 8 * hf);
       }
     }
@@ -266,8 +266,10 @@ __global__ __launch_bounds__(ROWS * 64) void conv_tile_kernel(ConvArgs a, int gr
     for (int k = 0; k < kWPer; ++k) {
       const int i = tid + k * kThreads;
       if (i < kWItems) {
+        // Consecutive threads take consecutive output channels of one tap, so a wave's records
+        // land on consecutive LDS addresses of each plane (tap-major order put 9 lanes on one bank).
         const int hf = i & 1, rest = i >> 1;
-        const int co = rest / 9, tap = rest - co * 9;
+        const int tap = rest / kCout, co = rest - tap * kCout;
         s_w[(hf * 9 + tap) * kCout + co] = r_w[k];
       }
     }
