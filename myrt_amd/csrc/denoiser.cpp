@@ -87,8 +87,16 @@ uint64_t splitmix_next(uint64_t& x) {
 
 int conv_call(const UnetLayer& u, const void* in0, int up0, const void* in1, const void* w, const float* bias,
               void* out, const void* mask, int relu, int B, int H, int W, bool half, hipStream_t s,
-              void* pool_out = nullptr) {
+              void* pool_out = nullptr, const ConvArgs* fuse = nullptr) {
   ConvArgs a{};
+  if (fuse) {  // the fused output step (raw, Hs, Ws, out_w, out_b, rgba)
+    a.raw = fuse->raw;
+    a.Hs = fuse->Hs;
+    a.Ws = fuse->Ws;
+    a.out_w = fuse->out_w;
+    a.out_b = fuse->out_b;
+    a.rgba = fuse->rgba;
+  }
   a.pool_out = pool_out;
   a.in0 = in0;
   a.in1 = in1;
@@ -230,7 +238,7 @@ int Denoiser::ensure_half(hipStream_t s) {
 
 template <typename T>
 int Denoiser::forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, int H, int W, hipStream_t s,
-                      bool with_out) {
+                      bool with_out, float* rgba) {
   constexpr bool half = sizeof(T) == 2;
   if (H % 16 || W % 16 || B < 1 || H < 16 || W < 16) return fail(MRT_ERR_INVALID, "denoiser tiles are 16-pixel multiples");
   int rc = A.ensure(B, H, W);
@@ -241,6 +249,15 @@ int Denoiser::forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, i
     return half ? static_cast<const void*>(wh.p + L[li].w_phys) : static_cast<const void*>(w.p + L[li].w_phys);
   };
   (void)wbase;
+  const bool fused = rgba != nullptr;
+  if (fused && (!half || with_out)) return fail(MRT_ERR_INTERNAL, "fused denoiser steps need the fp16 pass");
+  ConvArgs fout{};
+  fout.raw = raw;
+  fout.Hs = Hs;
+  fout.Ws = Ws;
+  fout.out_w = wl(kOut);
+  fout.out_b = b.p + L[kOut].b_phys;
+  fout.rgba = rgba;
   if (launch_unet_prep<T>(raw, A.x.p, B, Hs, Ws, H, W, s)) return fail(MRT_ERR_HIP, "denoiser input");
   for (int l = 0; l < kUnetLevels; ++l) {
     const int Hl = H >> l, Wl = W >> l;
@@ -259,14 +276,16 @@ int Denoiser::forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, i
     if ((rc = conv_call(L[kDec(l, 0)], src0, 1, A.eb[l].p, wl(kDec(l, 0)), b.p + L[kDec(l, 0)].b_phys, A.da[l].p,
                         nullptr, 1, B, Hl, Wl, half, s)) != MRT_OK ||
         (rc = conv_call(L[kDec(l, 1)], A.da[l].p, 0, nullptr, wl(kDec(l, 1)), b.p + L[kDec(l, 1)].b_phys, A.db[l].p,
-                        nullptr, 1, B, Hl, Wl, half, s)) != MRT_OK)
+                        nullptr, 1, B, Hl, Wl, half, s, nullptr, fused && l == 0 ? &fout : nullptr)) != MRT_OK)
       return rc;
   }
   if (!with_out) return MRT_OK;
   return conv_call(L[kOut], A.db[0].p, 0, nullptr, wl(kOut), b.p + L[kOut].b_phys, A.y.p, nullptr, 0, B, H, W, half, s);
 }
-template int Denoiser::forward<float>(UnetActs<float>&, const float*, int, int, int, int, int, hipStream_t, bool);
-template int Denoiser::forward<_Float16>(UnetActs<_Float16>&, const float*, int, int, int, int, int, hipStream_t, bool);
+template int Denoiser::forward<float>(UnetActs<float>&, const float*, int, int, int, int, int, hipStream_t, bool,
+                                      float*);
+template int Denoiser::forward<_Float16>(UnetActs<_Float16>&, const float*, int, int, int, int, int, hipStream_t, bool,
+                                         float*);
 
 int Denoiser::train(const float* d_raw, const float* d_target, int B, int H, int W, float lr_used, float* loss,
                     hipStream_t s) {
@@ -374,12 +393,9 @@ int Denoiser::train(const float* d_raw, const float* d_target, int B, int H, int
 
 int Denoiser::denoise(const float* d_raw, int Hs, int Ws, float* d_out, hipStream_t s) {
   const int H = (Hs + 15) / 16 * 16, W = (Ws + 15) / 16 * 16;
-  int rc = forward<_Float16>(hact, d_raw, 1, Hs, Ws, H, W, s, false);
-  if (rc != MRT_OK) return rc;
-  return launch_unet_finish_out(d_raw, hact.db[0].p, L[kOut].cin_phys, wh.p + L[kOut].w_phys, b.p + L[kOut].b_phys,
-                                d_out, Hs, Ws, W, s)
-             ? fail(MRT_ERR_HIP, "denoiser output")
-             : int(MRT_OK);
+  // The output layer and the finish step run inside the last convolution (the same arithmetic as
+  // the separate output pass, on the fp16 values that pass would read).
+  return forward<_Float16>(hact, d_raw, 1, Hs, Ws, H, W, s, false, d_out);
 }
 
 }  // namespace mrt

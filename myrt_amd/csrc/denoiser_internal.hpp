@@ -59,10 +59,12 @@ struct Denoiser {
   int set_logical(const float* p);
   int get_logical(float* p, bool grads);
   // Forward pass in T over raw features (device [B][Hs][Ws][10]) padded to H x W.
+  // with_out = false stops before the 1x1 output layer. rgba (fp16 only, with_out = false): the
+  // last convolution also runs the output layer and the finish step, writing the denoised image
+  // [B][Hs][Ws][4] instead of its 32 channels.
   template <typename T>
-  // with_out = false stops before the 1x1 output layer (the fused finish computes it).
   int forward(UnetActs<T>& A, const float* raw, int B, int Hs, int Ws, int H, int W, hipStream_t s,
-              bool with_out = true);
+              bool with_out = true, float* rgba = nullptr);
   // Loss, gradients into gw/gb (zeroed first) and, if lr > 0, one Adam step.
   int train(const float* d_raw, const float* d_target, int B, int H, int W, float lr_used, float* loss,
             hipStream_t s);

@@ -188,7 +188,50 @@ __global__ __launch_bounds__(kBlock) void conv_kernel(ConvArgs a) {
 constexpr int kTileCols = 64;
 constexpr int kHaloCols = kTileCols + 2;
 
-template <int NM, int ROWS>
+// Network input channel c (0 .. kUnetInPhys - 1) from a raw feature record: log1p of the clamped
+// radiance, albedo and normal as they are, 1 / (1 + depth) (0 for no hit), zero padding.
+__device__ inline float input_channel(const float* s, int c) {
+  if (c < 3) return log1pf(fmaxf(s[c], 0.0f));
+  if (c < 9) return s[c];
+  if (c == 9) {
+    const float d = s[9];
+    return (d > 0.0f && d < __int_as_float(0x7f800000)) ? 1.0f / (1.0f + d) : 0.0f;
+  }
+  return 0.0f;
+}
+
+// The 1x1 output convolution of one pixel: y = b + W x over cin channels, x read as 8-channel
+// records rec(k) for k = 0, 8, ...; the products are accumulated in this fixed order.
+template <typename Rec>
+__device__ inline void output_conv(Rec rec, int cin, const _Float16* w, const float* bias, float (&y)[3]) {
+  y[0] = bias[0];
+  y[1] = bias[1];
+  y[2] = bias[2];
+  for (int k = 0; k < cin; k += 8) {
+    const h8 v = rec(k);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const h8 wv = *reinterpret_cast<const h8*>(w + size_t(c) * cin + k);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) y[c] = fmaf(float(wv[e]), float(v[e]), y[c]);
+    }
+  }
+}
+
+// The finish step: the log-space residual y added to log1p(radiance) and mapped back, alpha 1.
+__device__ inline float4 finish_rgba(const float* raw_px, const float (&y)[3]) {
+  float o[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float z = log1pf(fmaxf(raw_px[c], 0.0f)) + y[c];
+    o[c] = expm1f(fmaxf(z, 0.0f));
+  }
+  return make_float4(o[0], o[1], o[2], 1.0f);
+}
+
+// FUSE: the output layer and the finish step run on the tile (ConvArgs::out_w); a separate
+// instantiation keeps the plain kernels' registers as they are.
+template <int NM, int ROWS, bool FUSE = false>
 __global__ __launch_bounds__(ROWS * 64) void conv_tile_kernel(ConvArgs a, int groups) {
   constexpr int kThreads = ROWS * 64;  // one wave per output row of the tile
   constexpr int kHRows = ROWS + 2;
@@ -327,15 +370,32 @@ __global__ __launch_bounds__(ROWS * 64) void conv_tile_kernel(ConvArgs a, int gr
       }
     }
     __syncthreads();
-    constexpr int kRecords = ROWS * kTileCols * 4;
+    if (FUSE) {
+      // The output layer and the finish step fused in (cout == 32: this is the only block): one
+      // pixel per thread from the staged tile, written as rgba; the 32 channels never reach HBM.
+      const _Float16* ow = static_cast<const _Float16*>(a.out_w);
+      for (int k = tid; k < ROWS * kTileCols; k += kThreads) {
+        const int row = k / kTileCols, pix = k % kTileCols;
+        const int y = y0 + row, px = x0 + pix;
+        if (y < a.Hs && px < a.Ws) {
+          float yv[3];
+          output_conv([&](int kk) { return s_buf[(row * kTileCols + pix) * kOutPitch + kk / 8]; }, 32, ow, a.out_b,
+                      yv);
+          const size_t q = (size_t(img) * a.Hs + y) * a.Ws + px;
+          reinterpret_cast<float4*>(a.rgba)[q] = finish_rgba(a.raw + q * kUnetInCh, yv);
+        }
+      }
+    } else {
+      constexpr int kRecords = ROWS * kTileCols * 4;
 #pragma unroll
-    for (int it = 0; it < kRecords / kThreads; ++it) {
-      const int k = tid + it * kThreads;
-      const int row = k / (kTileCols * 4), rem = k % (kTileCols * 4), pix = rem >> 2, q = rem & 3;
-      const int y = y0 + row, px = x0 + pix;
-      if (y < a.H && px < a.W)
-        *reinterpret_cast<h8*>(out + ((size_t(img) * a.H + y) * a.W + px) * a.cout + m0 + mb * 32 + 8 * q) =
-            s_buf[(row * kTileCols + pix) * kOutPitch + q];
+      for (int it = 0; it < kRecords / kThreads; ++it) {
+        const int k = tid + it * kThreads;
+        const int row = k / (kTileCols * 4), rem = k % (kTileCols * 4), pix = rem >> 2, q = rem & 3;
+        const int y = y0 + row, px = x0 + pix;
+        if (y < a.H && px < a.W)
+          *reinterpret_cast<h8*>(out + ((size_t(img) * a.H + y) * a.W + px) * a.cout + m0 + mb * 32 + 8 * q) =
+              s_buf[(row * kTileCols + pix) * kOutPitch + q];
+      }
     }
     if (a.pool_out) {
       // The 2x2 average of the staged block (tiles start on even rows and columns): one 16-byte
@@ -492,14 +552,8 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(const float* raw, T* x, in
   const int b = int(r / H);
   const int sx = px < Ws ? px : Ws - 1, sy = py < Hs ? py : Hs - 1;
   const float* s = raw + ((size_t(b) * Hs + sy) * Ws + sx) * kUnetInCh;
-  float v[kUnetInPhys];
-  for (int c = 0; c < 3; ++c) v[c] = log1pf(fmaxf(s[c], 0.0f));
-  for (int c = 3; c < 9; ++c) v[c] = s[c];
-  const float d = s[9];
-  v[9] = (d > 0.0f && d < __int_as_float(0x7f800000)) ? 1.0f / (1.0f + d) : 0.0f;
-  for (int c = 10; c < kUnetInPhys; ++c) v[c] = 0.0f;
   T* o = x + i * kUnetInPhys;
-  for (int c = 0; c < kUnetInPhys; ++c) o[c] = T(v[c]);
+  for (int c = 0; c < kUnetInPhys; ++c) o[c] = T(input_channel(s, c));
 }
 
 __global__ __launch_bounds__(kBlock) void loss_kernel(const float* x, const float* y, int yc, const float* target,
@@ -529,49 +583,7 @@ __global__ __launch_bounds__(kBlock) void loss_kernel(const float* x, const floa
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void finish_kernel(const float* raw, const T* y, int yc, float* out, int Hs, int Ws,
-                                                        int W) {
-  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  const size_t n = size_t(Hs) * Ws;
-  if (i >= n) return;
-  const int px = int(i % Ws), py = int(i / Ws);
-  const size_t q = size_t(py) * W + px;
-  float o[3];
-  for (int c = 0; c < 3; ++c) {
-    const float z = log1pf(fmaxf(raw[i * kUnetInCh + c], 0.0f)) + float(y[q * yc + c]);
-    o[c] = expm1f(fmaxf(z, 0.0f));
-  }
-  reinterpret_cast<float4*>(out)[i] = make_float4(o[0], o[1], o[2], 1.0f);
-}
 
-// The 1x1 output convolution (cin channels -> 3) fused with the finish step: per pixel the
-// residual y = W x + b is computed in fp32 from the fp16 decoder output and never stored.
-__global__ __launch_bounds__(kBlock) void finish_out_kernel(const float* raw, const _Float16* x, int cin,
-                                                            const _Float16* w, const float* bias, float* out, int Hs,
-                                                            int Ws, int W) {
-  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  const size_t n = size_t(Hs) * Ws;
-  if (i >= n) return;
-  const int px = int(i % Ws), py = int(i / Ws);
-  const _Float16* xp = x + (size_t(py) * W + px) * cin;
-  float y[3] = {bias[0], bias[1], bias[2]};
-  for (int k = 0; k < cin; k += 8) {
-    const h8 v = *reinterpret_cast<const h8*>(xp + k);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const h8 wv = *reinterpret_cast<const h8*>(w + size_t(c) * cin + k);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) y[c] = fmaf(float(wv[e]), float(v[e]), y[c]);
-    }
-  }
-  float o[3];
-  for (int c = 0; c < 3; ++c) {
-    const float z = log1pf(fmaxf(raw[i * kUnetInCh + c], 0.0f)) + y[c];
-    o[c] = expm1f(fmaxf(z, 0.0f));
-  }
-  reinterpret_cast<float4*>(out)[i] = make_float4(o[0], o[1], o[2], 1.0f);
-}
 
 __global__ __launch_bounds__(kBlock) void transpose_w_kernel(const float* w, float* wt, int cout, int taps, int cin) {
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -612,6 +624,9 @@ int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
   if (a.cout % 32 || a.c0 % step || a.c1 % step || (a.up0 && (a.H % 2 || a.W % 2)) || (a.c1 && !a.in1) ||
       (taps != 9 && taps != 1))
     return -1;
+  const bool fused = a.out_w != nullptr;
+  if (fused && (!a.raw || !a.rgba || !a.out_b || a.cout != 32 || a.Hs < 1 || a.Ws < 1 || a.Hs > a.H || a.Ws > a.W))
+    return -1;
   if (std::is_same<T, _Float16>::value && taps == 9 && !a.mask && a.cout <= 128 && a.c0 % 16 == 0 &&
       a.c1 % 16 == 0) {
     // Inference 3x3: the LDS-tiled kernel, one workgroup per 4 x 64 output tile and group of
@@ -640,7 +655,13 @@ int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
       groups = mt;
     }
     const dim3 grid((tall ? tiles_of(8) : tiles) * unsigned(groups));
-    if (tall) {
+    if (fused) {
+      if (nm != 1 || groups != 1) return -1;  // one 32-channel block (the checks above)
+      if (tall)
+        hipLaunchKernelGGL((unk::conv_tile_kernel<1, 8, true>), grid, dim3(8 * 64), 0, s, a, groups);
+      else
+        hipLaunchKernelGGL((unk::conv_tile_kernel<1, 4, true>), grid, dim3(4 * 64), 0, s, a, groups);
+    } else if (tall) {
       hipLaunchKernelGGL((unk::conv_tile_kernel<1, 8>), grid, dim3(8 * 64), 0, s, a, groups);
     } else {
       switch (nm) {
@@ -652,6 +673,7 @@ int launch_unet_conv(const ConvArgs& a, int taps, hipStream_t s) {
     }
     return unk::ok();
   }
+  if (fused) return -1;  // the fused output step exists in the fp16 tiled kernel only
   const size_t waves = size_t(a.B) * a.H * ((a.W + 63) / 64) * ((a.cout + 63) / 64);
   const unsigned grid = unsigned((waves * 64 + unk::kBlock - 1) / unk::kBlock);
   if (taps == 9)
@@ -728,24 +750,7 @@ int launch_unet_loss(const float* x, const float* y, int yc, const float* target
   return unk::ok();
 }
 
-template <typename T>
-int launch_unet_finish(const float* raw, const T* y, int yc, float* out_rgba, int Hs, int Ws, int W, hipStream_t s) {
-  const size_t n = size_t(Hs) * Ws;
-  hipLaunchKernelGGL(unk::finish_kernel<T>, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, raw, y, yc, out_rgba, Hs,
-                     Ws, W);
-  return unk::ok();
-}
-template int launch_unet_finish<float>(const float*, const float*, int, float*, int, int, int, hipStream_t);
-template int launch_unet_finish<_Float16>(const float*, const _Float16*, int, float*, int, int, int, hipStream_t);
 
-int launch_unet_finish_out(const float* raw, const _Float16* x, int cin, const _Float16* w, const float* bias,
-                           float* out_rgba, int Hs, int Ws, int W, hipStream_t s) {
-  if (cin % 8) return -1;
-  const size_t n = size_t(Hs) * Ws;
-  hipLaunchKernelGGL(unk::finish_out_kernel, dim3(unk::blocks_for(n)), dim3(unk::kBlock), 0, s, raw, x, cin, w, bias,
-                     out_rgba, Hs, Ws, W);
-  return unk::ok();
-}
 
 int launch_unet_transpose_w(const float* w, float* wt, int cout, int taps, int cin, hipStream_t s) {
   const size_t n = size_t(cout) * taps * cin;

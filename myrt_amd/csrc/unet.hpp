@@ -53,6 +53,14 @@ struct ConvArgs {
   int cout;
   int relu;
   int B, H, W;
+  // fp16 3x3 tiles only: with out_w (and cout == 32) the 1x1 output convolution (out_w [3][32],
+  // out_b [3]) and the finish step against the raw feature image raw [B][Hs][Ws][kUnetInCh] run on
+  // the tile; rgba [B][Hs][Ws] is written and `out` is not.
+  const float* raw;
+  int Hs, Ws;
+  const void* out_w;
+  const float* out_b;
+  float* rgba;
 };
 
 // Weight gradient dW[co][tap][ci] += sum over pixels of g[p][co] * x_tap[p][ci], where x_tap is the
@@ -88,12 +96,6 @@ int launch_unet_prep(const float* raw, T* x, int B, int Hs, int Ws, int H, int W
 // Writes per-block partial sums of |z - t| to partial[blocks].
 int launch_unet_loss(const float* x, const float* y, int yc, const float* target, float* g, float* partial,
                      int B, int H, int W, int* blocks, hipStream_t s);
-// Denoised radiance expm1(max(log1p(rgb) + y, 0)) for the Hs x Ws image region.
-template <typename T>
-int launch_unet_finish(const float* raw, const T* y, int yc, float* out_rgba, int Hs, int Ws, int W, hipStream_t s);
-// Fused 1x1 output convolution (fp16 weights [3+][cin], fp32 bias) and finish, for inference.
-int launch_unet_finish_out(const float* raw, const _Float16* x, int cin, const _Float16* w, const float* bias,
-                           float* out_rgba, int Hs, int Ws, int W, hipStream_t s);
 int launch_unet_transpose_w(const float* w, float* wt, int cout, int taps, int cin, hipStream_t s);
 int launch_unet_to_half(const float* in, _Float16* out, size_t n, hipStream_t s);
 int launch_unet_adam(float* w, float* m, float* v, const float* g, size_t n, float lr, float b1, float b2, float eps,

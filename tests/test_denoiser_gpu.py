@@ -96,6 +96,36 @@ def test_fp16_tiled_forward_matches_fp32_at_full_size(ensure_built):
     assert err16 < 3e-2
 
 
+@pytest.mark.parametrize("H,W", [(40, 72), (1000, 520)])
+def test_denoise_fused_steps_match_the_separate_passes(ensure_built, H, W):
+    """denoise() runs the 1x1 output layer and the finish step inside the last convolution
+    (40x72 takes the 4-row tiles, 1000x520 the 8-row ones; neither is a multiple of 16). It must
+    give expm1(max(log1p(max(radiance, 0)) + y, 0)) with y the
+    separate fp16 forward of the edge-padded features. The only difference is that the separate
+    output layer rounds y to fp16, so in log space (log1p of the output) the two differ by at most
+    half an fp16 ulp of y, 2^-11 |y|, plus fp32 rounding."""
+    d = m.Denoiser(seed=3)
+    rng = np.random.default_rng(12)
+    L = U.unpack(d.params().astype(np.float64))
+    for name in L:
+        Wt, b = L[name]
+        L[name] = (Wt * (5.0 if name == "out" else 1.0), rng.normal(scale=0.05, size=b.shape))
+    d.set_params(U.pack(L).astype(np.float32))
+    raw = raw_batch(1, H, W, seed=13)
+    say("denoise %dx%d: fused steps against the separate passes" % (W, H))
+    out = d.denoise(raw[0])
+    Hp, Wp = (H + 15) // 16 * 16, (W + 15) // 16 * 16
+    padded = np.pad(raw, ((0, 0), (0, Hp - H), (0, Wp - W), (0, 0)), mode="edge")
+    y = d.forward(padded, half=True)[0, :H, :W].astype(np.float64)
+    ref_log = np.maximum(np.log1p(np.maximum(raw[0, ..., :3].astype(np.float64), 0)) + y, 0)
+    diff = np.abs(np.log1p(out[..., :3].astype(np.float64)) - ref_log)
+    bound = 2.0 ** -11 * np.abs(y) + 1e-5 * (1 + ref_log)
+    say("largest log-space difference %.2e, largest share of the fp16 bound %.2f" % (diff.max(), (diff / bound).max()))
+    assert np.isfinite(out).all() and np.all(out[..., 3] == 1.0)
+    assert np.abs(y).max() > 0.05  # the residual is not negligible
+    assert np.all(diff <= bound)
+
+
 def test_gradient_and_adam_match_oracle(ensure_built):
     d = m.Denoiser(seed=8)
     p = d.params().astype(np.float64)
