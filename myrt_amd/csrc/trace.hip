@@ -106,7 +106,6 @@ __device__ inline uint32_t pack_bary(float b1, float b2) {
   return u | (v << 16);
 }
 
-__device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 __device__ inline float byte_of(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
 
 template <bool ANY, bool STATS, int WAVES>
@@ -269,9 +268,9 @@ trace_kernel(TraceParams p) {
         const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
         if (STATS) ++n_nodes;
         const float orx = __uint_as_float(q0.x), ory = __uint_as_float(q0.y), orz = __uint_as_float(q0.z);
-        const float scx = exp2i(int(int8_t(q0.w & 0xffu)));
-        const float scy = exp2i(int(int8_t((q0.w >> 8) & 0xffu)));
-        const float scz = exp2i(int(int8_t((q0.w >> 16) & 0xffu)));
+        // The per-axis exponents e (signed bytes, -126..127: 2^e is a normal power of two).
+        const int ex = int(int8_t(q0.w & 0xffu)), ey = int(int8_t((q0.w >> 8) & 0xffu)),
+                  ez = int(int8_t((q0.w >> 16) & 0xffu));
         const uint32_t nch = q0.w >> 24;
         // A child plane at origin + q * scale is crossed at t = q * a + b, with a = scale * i
         // (exact: scale is a power of two) and b = (origin - o) * i. Computing b rounds twice,
@@ -279,7 +278,10 @@ trace_kernel(TraceParams p) {
         // planes b + S with S = 2^-20 |b| + 2^-100, so no box that the exact slab test accepts
         // is culled. The remaining relative rounding is covered by kRobust on the far side. The
         // direction signs pick which quantised plane is near on each axis.
-        const float ax = scx * ix, ay = scy * iy, az = scz * iz;
+        // a = 2^e * i: v_ldexp_f32 scales exactly as the product with 2^e would (one rounding,
+        // the same result), in one instruction per axis.
+        const float ax = __builtin_amdgcn_ldexpf(ix, ex), ay = __builtin_amdgcn_ldexpf(iy, ey),
+                    az = __builtin_amdgcn_ldexpf(iz, ez);
         const float bx = (orx - ox) * ix, by = (ory - oy) * iy, bz = (orz - oz) * iz;
         const float sxx = fmaf(fabsf(bx), kSlack, kSlackFloor), syy = fmaf(fabsf(by), kSlack, kSlackFloor),
                     szz = fmaf(fabsf(bz), kSlack, kSlackFloor);
