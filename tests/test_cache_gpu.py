@@ -39,6 +39,21 @@ def test_inference_matches_oracle(cache):
     assert rel(y, ref) < TOL
 
 
+def test_inference_at_the_per_frame_size(cache):
+    """2^20 + 333 queries (the per-frame inference size, every wave of the persistent grid taking
+    several tile pairs, a ragged end): a random subset of 8192 against the oracle, and each query's
+    result independent of the batch it came in, bit for bit (the subset inferred alone)."""
+    n = (1 << 20) + 333
+    q = M.random_queries(n, 15)
+    y = cache.infer(q)
+    assert np.isfinite(y).all()
+    pick = np.sort(np.random.default_rng(16).choice(n, 8192, replace=False))
+    pick[-1] = n - 1  # the last query of the ragged end
+    ref, _, _ = M.forward(cache.params().astype(np.float64), M.encode(q[pick], (-1, -1, -1), (1, 1, 1)), bf16=True)
+    assert rel(y[pick], ref) < TOL
+    assert np.array_equal(cache.infer(q[pick]).view(np.uint32), y[pick].view(np.uint32))
+
+
 @pytest.mark.parametrize("n", [777, (1 << 17) + 777])
 def test_gradient_matches_oracle(cache, n):
     """777 samples: one workgroup, a ragged last tile. 2^17 + 777: every workgroup sums several
