@@ -244,18 +244,27 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
         den = m.Denoiser()
         den.load(weights)
         plain.denoise(den)
+    # Render paths end in the cache at vertex 2; training paths (one in 16, stratified) at vertex 4,
+    # learning their tail from the cache (self-training); 8 Adam steps per 2^21 paths.
+    cache_cfg = dict(mode=2, train_fraction=1.0 / 16, train_steps=8, min_bounce=2)
+    train_cfg = dict(train_end=4, unbiased_fraction=0.0)
+
+    def use_cache(r, cache):
+        r.set_cache(cache, **cache_cfg)
+        r.set_cache_training(**train_cfg)
+
     warm_cache = m.RadianceCache(seed=1)
     warm_cache.set_bounds((-12.5, -7.0, -9.5), (12.5, 6.0, 9.5))
     warm = renderer()
-    warm.set_cache(warm_cache, mode=2)
+    use_cache(warm, warm_cache)
     _, warm_s = timed(lambda: [warm.render(s, W, H, 16) for _ in range(warm_frames)])
     cr = renderer()
-    cr.set_cache(warm_cache, mode=2)
+    use_cache(cr, warm_cache)
     cr.render(s, W, H, 16)
     _, t16 = timed(lambda: fresh(cr).render(s, W, H, 16))
     res = {"width": W, "height": H, "depth": depth, "reference_spp": ref_spp, "reference_accumulation": "binary64 over "
            "256-sample fp32 chunks", "cache_warm_frames": warm_frames, "cache_warm_s": round(warm_s, 4),
-           "cache_seconds_per_16spp": round(t16, 4), "budgets": []}
+           "cache_seconds_per_16spp": round(t16, 4), "cache_settings": dict(cache_cfg, **train_cfg), "budgets": []}
     cold_r = renderer()
     cold_r.render(s, W, H, 16)  # allocations stay out of the timings
     for b in budgets:
@@ -289,7 +298,7 @@ def equal_time_study(args, W=480, H=270, depth=8, ref_spp=4096, budgets=(4, 16, 
                 for _ in range(3):  # the median of 3, each from a fresh network
                     cold = m.RadianceCache(seed=2)
                     cold.set_bounds((-12.5, -7.0, -9.5), (12.5, 6.0, 9.5))
-                    cold_r.set_cache(cold, mode=2)
+                    use_cache(cold_r, cold)
                     ts.append(timed(lambda: fresh(cold_r).render(s, W, H, k))[1])
                     cold_render.keep = cold
                 return float(np.median(ts))

@@ -439,13 +439,22 @@ int mrt_renderer_set_bokeh_mask(mrt_renderer r, const float* rgba, int w, int h)
  * (pt_color.comp:191-199). */
 int mrt_renderer_set_cubemap(mrt_renderer r, const float* faces, int size);
 /* Attaches a radiance cache (no reference counterpart). mode 0: off; 1: train only (every pass
- * builds a training batch of up to 2^18 path-vertex samples from a `train_fraction` of paths,
- * which stay unbiased, and takes `train_steps` Adam steps on disjoint slices of it); 2: train and
+ * builds a training batch of up to 2^18 path-vertex samples from its training paths, one drawn
+ * from every round(1 / train_fraction) consecutive paths of the pixel-major batch, which stay
+ * unbiased, and takes `train_steps` Adam steps on disjoint slices of it); 2: train and
  * end the other paths in the cache at vertex `min_bounce` (0 = primary hit) if its roughness is
  * >= 0.25, or at any later vertex; never on dielectrics. The accumulated image is kept; call
  * mrt_renderer_invalidate to restart it. */
 int mrt_renderer_set_cache(mrt_renderer r, mrt_cache c, int mode, float train_fraction, int train_steps,
                            int min_bounce);
+/* Path length under mode 2 (no reference counterpart). train_end > 0 (default 4): every path,
+ * training paths included, ends in the cache at vertex train_end, on dielectrics too (render paths
+ * have usually ended at min_bounce already), and for a training path the cache's prediction there
+ * is added to the radiance its earlier vertices learn from (self-training, as in Mueller et al.
+ * 2021); that vertex itself is not a training sample. An `unbiased_fraction` of the training
+ * paths (default 0) is exempt and runs to full depth. train_end = 0 keeps every training path
+ * unbiased and lets paths on dielectrics run to full depth. */
+int mrt_renderer_set_cache_training(mrt_renderer r, int train_end, float unbiased_fraction);
 /* The sampler index of the first sample of the accumulation, 0 by default (no reference
  * counterpart): renderers given disjoint ranges [base, base + spp) render independent samples of
  * one image, so their means combine (say one renderer per GPU, or chunks of a reference image

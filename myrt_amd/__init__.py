@@ -148,6 +148,7 @@ SIGNATURES = {
     "mrt_renderer_set_cubemap": ([_P, _P, ctypes.c_int], ctypes.c_int),
     "mrt_scene_push_texture": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
     "mrt_renderer_set_cache": ([_P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "mrt_renderer_set_cache_training": ([_P, ctypes.c_int, ctypes.c_float], ctypes.c_int),
     "mrt_renderer_set_sample_base": ([_P, ctypes.c_uint32], ctypes.c_int),
     "mrt_renderer_sample_count": ([_P, _U32P], ctypes.c_int),
     "mrt_render": ([_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -762,9 +763,15 @@ class Renderer:
             rgba = np.ascontiguousarray(rgba)
             _check(lib().mrt_renderer_set_bokeh_mask(self.h, _ptr(rgba), rgba.shape[1], rgba.shape[0]))
 
-    def set_cache(self, cache, mode=2, train_fraction=1.0 / 16, train_steps=16, min_bounce=2):
+    def set_cache(self, cache, mode=2, train_fraction=1.0 / 16, train_steps=8, min_bounce=2):
         _check(lib().mrt_renderer_set_cache(self.h, cache.h if cache is not None else None, int(mode),
                                             float(train_fraction), int(train_steps), int(min_bounce)))
+
+    def set_cache_training(self, train_end=4, unbiased_fraction=0.0):
+        """Every path ends in the cache at vertex train_end (0: training paths never do), training
+        paths learning their tail from its prediction; an unbiased_fraction of them runs to full
+        depth."""
+        _check(lib().mrt_renderer_set_cache_training(self.h, int(train_end), float(unbiased_fraction)))
 
     def set_sample_base(self, base):
         """Sampler index of the accumulation's first sample; restarts the accumulation."""

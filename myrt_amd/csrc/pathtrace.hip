@@ -25,6 +25,7 @@ namespace ptk {
 constexpr int kBlock = 256;
 constexpr int kShadeBlock = 512;  // shade_kernel: 8 waves share one queue atomic
 constexpr uint32_t kDelta = 1u << 8;
+constexpr uint32_t kUnbiased = 1u << 9;  // a training path that never ends in the cache
 constexpr float kPi = 3.14159265358979f;
 
 struct V3 {
@@ -72,6 +73,7 @@ __device__ inline float rnd(uint32_t pixel, uint32_t sample, uint32_t dim) {
 // 4-5 light point, 6 light choice}.
 __device__ inline uint32_t dim_of(uint32_t bounce, uint32_t k) { return 2u + 8u * bounce + k; }
 constexpr uint32_t kTrainDim = 0x40000000u;  // dimension that selects training paths
+constexpr uint32_t kUnbiasedDim = 0x40000001u;  // dimension that selects unbiased training paths
 constexpr uint32_t kLensDim = 0x30000000u;   // lens samples: kLensDim + 2 * try + {0, 1}
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
@@ -360,14 +362,22 @@ __global__ __launch_bounds__(kBlock) void generate_kernel(Camera cam, PtBuffers 
   ps.flags = kDelta;
   ps.last_pdf = 0.f;
   ps.pad = 0xFFFFFFFFu;
-  const bool want_train = live && b.cache.train && rnd(pixel, sample, kTrainDim) < b.cache.train_fraction;
-  uint32_t* const queues[1] = {&b.counts[3]};
-  const bool append[1] = {want_train};
-  uint32_t slots[1];
-  block_append<kBlock, 1>(queues, append, slots);
-  const uint32_t slot = slots[0];
+  // Training paths, stratified: one path of every train_group consecutive paths (the member drawn
+  // by a hash of the group's first path), whose training slot is the group index, so no queue
+  // atomic is needed (a queue atomic per block took 0.3 ms for 8M paths).
+  bool want_train = false;
+  uint32_t slot = 0;
+  if (b.cache.train) {
+    const uint32_t g = b.cache.train_group;
+    slot = ii / g;
+    const uint32_t first = slot * g, members = min(g, n_paths - first);
+    const uint32_t f_pixel = first / nspp, f_sample = sample_base + first % nspp;
+    want_train = live && ii - first == mix32(mix32(mix32(f_pixel) + f_sample) + kTrainDim) % members;
+    if (i == 0) b.counts[3] = (n_paths + g - 1) / g;
+  }
   if (want_train && slot < b.cache.max_train_paths) {
     ps.pad = slot;
+    if (rnd(pixel, sample, kUnbiasedDim) < b.cache.unbiased_fraction) ps.flags |= kUnbiased;
     TrainPath tp;
     tp.T[0] = tp.T[1] = tp.T[2] = 0.f;
     tp.nv = 0;
@@ -573,7 +583,16 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffer
         cq.albedo[2] = L.albedo.z;
         cq.pad2 = 0.f;
       }
-      if (ps.pad != 0xFFFFFFFFu) {
+      // At vertex train_end every path but the unbiased training paths ends in the cache, on
+      // dielectrics too. For a training path the cache's prediction becomes the tail radiance of
+      // its earlier vertices (self-training); this vertex is not recorded, since its target would
+      // be the prediction itself.
+      const bool at_end = b.cache.terminate && b.cache.train_end != 0u && bounce >= b.cache.train_end &&
+                          !(ps.flags & kUnbiased);
+      const bool train_ends = ps.pad != 0xFFFFFFFFu && at_end;
+      if (train_ends) {
+        cached = true;
+      } else if (ps.pad != 0xFFFFFFFFu) {
         TrainPath& tp = b.cache.tpaths[ps.pad];
         if (tp.nv < b.cache.max_vertices) {
           TrainVertex& tv = b.cache.tverts[size_t(ps.pad) * b.cache.max_vertices + tp.nv];
@@ -589,6 +608,8 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffer
       } else if (b.cache.terminate && bounce >= b.cache.min_bounce &&
                  (mat.roughness >= b.cache.min_roughness || bounce > b.cache.min_bounce) &&
                  mat.transmission < 0.5f) {
+        cached = true;
+      } else if (at_end) {
         cached = true;
       }
       if (cached) {
@@ -734,6 +755,8 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(PtScene sc, PtBuffer
     cr.thr[1] = cache_thr.y;
     cr.thr[2] = cache_thr.z;
     cr.pixel = ps.pixel;
+    cr.train_slot = ps.pad;
+    cr.pad[0] = cr.pad[1] = cr.pad[2] = 0;
     b.cache.qrec[qslot] = cr;
   }
 }
@@ -767,6 +790,7 @@ __global__ __launch_bounds__(kBlock) void cache_resolve_kernel(PtBuffers b) {
     bad = !finite3(c);
   }
   wave_add_pixel(b.accum, r.pixel, c, on && !bad);
+  if (on && !bad) add_train(b, r.train_slot, c);
   count_nonfinite(b.counts, bad);
 }
 

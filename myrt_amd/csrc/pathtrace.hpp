@@ -74,18 +74,25 @@ static_assert(sizeof(TrainVertex) == 96, "training vertex is 96 bytes");
 struct CacheRecord {
   float thr[3];
   uint32_t pixel;
+  uint32_t train_slot;  // training path that ended here (its tail radiance), or 0xFFFFFFFF
+  uint32_t pad[3];
 };
+static_assert(sizeof(CacheRecord) == 32, "cache record is 32 bytes");
 
 struct PtCache {
   uint32_t terminate;       // non-training paths end in the cache
   uint32_t train;           // training paths record vertices
-  float train_fraction;
+  uint32_t train_group;     // one training path per train_group consecutive paths
   uint32_t max_train_paths;
   uint32_t max_vertices;
   uint32_t max_queries;
   uint32_t max_batch;
   float min_roughness;      // vertex min_bounce ends in the cache if roughness >= this; later ones always
   uint32_t min_bounce;      // first vertex index (0 = primary hit) that may end in the cache
+  uint32_t train_end;       // every path ends in the cache at this vertex (0: training paths
+                            // never do); training paths' earlier vertices then learn from the
+                            // cache's own prediction
+  float unbiased_fraction;  // training paths exempt from train_end
   TrainPath* tpaths;
   TrainVertex* tverts;
   CacheQuery* queries;

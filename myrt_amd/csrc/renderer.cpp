@@ -164,10 +164,12 @@ struct Renderer {
   Cache* cache = nullptr;
   int cache_mode = 0;  // 0 off, 1 train only, 2 train and end paths in the cache
   float train_fraction = 1.0f / 16.0f;
-  int train_steps = 16;
+  int train_steps = 8;
   uint32_t max_batch = 1u << 18;
   float min_roughness = 0.25f;
   uint32_t min_bounce = 2;
+  uint32_t train_end = 4;         // every path ends in the cache at this vertex (0: training paths never)
+  float unbiased_fraction = 0.f;  // training paths exempt from train_end
   TrainPath* tpaths = nullptr;
   TrainVertex* tverts = nullptr;
   CacheQuery* queries = nullptr;
@@ -274,20 +276,23 @@ struct Renderer {
     PtCache pc{};
     if (use_cache) {
       const uint32_t depth = uint32_t(num_bounces);
-      const size_t max_train = size_t(double(cap) * double(train_fraction) * 1.25) + 1024;
+      const uint32_t group = uint32_t(std::max(1.0, std::round(1.0 / double(train_fraction))));
+      const size_t max_train = (cap + group - 1) / group;
       if ((rc = ensure(tpaths, c_tp, max_train)) || (rc = ensure(tverts, c_tv, max_train * depth)) ||
           (rc = ensure(queries, c_q, cap)) || (rc = ensure(qrec, c_qr, cap)) || (rc = ensure(qout, c_qo, 3 * cap)) ||
           (rc = ensure(batch_q, c_bq, batch_cap)) || (rc = ensure(batch_t, c_bt, 3 * size_t(batch_cap))))
         return rc;
       pc.terminate = cache_mode >= 2 ? 1u : 0u;
       pc.train = 1u;
-      pc.train_fraction = train_fraction;
+      pc.train_group = group;
       pc.max_train_paths = uint32_t(max_train);
       pc.max_vertices = depth;
       pc.max_queries = uint32_t(cap);
       pc.max_batch = batch_cap;
       pc.min_roughness = min_roughness;
       pc.min_bounce = min_bounce;
+      pc.train_end = train_end;
+      pc.unbiased_fraction = unbiased_fraction;
       pc.tpaths = tpaths;
       pc.tverts = tverts;
       pc.queries = queries;
@@ -660,6 +665,17 @@ int mrt_renderer_set_cache(mrt_renderer rh, mrt_cache c, int mode, float train_f
   r->cache_mode = mode;
   r->train_fraction = train_fraction;
   r->train_steps = train_steps;
+  return MRT_OK;
+  MRT_GUARD_END
+}
+
+int mrt_renderer_set_cache_training(mrt_renderer rh, int train_end, float unbiased_fraction) {
+  MRT_GUARD_BEGIN
+  Renderer* r = reinterpret_cast<Renderer*>(rh);
+  if (!r || train_end < 0 || train_end > 16 || !(unbiased_fraction >= 0.f && unbiased_fraction <= 1.f))
+    return fail(MRT_ERR_INVALID, "bad cache training settings");
+  r->train_end = uint32_t(train_end);
+  r->unbiased_fraction = unbiased_fraction;
   return MRT_OK;
   MRT_GUARD_END
 }

@@ -3,14 +3,19 @@ icospheres, displaced ground, 64-primitive SDF cluster) at 480x270, depth 8, wit
 trained online from 1/16 of the paths (2^20-query inference and 2^16-sample training per frame at
 this size and 16 samples per pixel).
 
-* Bias: after 16 warm frames, the cache's 256-sample render is compared with a 1024-sample plain
+The cache runs with the renderer's defaults: render paths end in it at vertex 2, training paths
+(one in 16, stratified) at vertex 4 with the cache's prediction as their tail (self-training), 8
+Adam steps per 2^21 paths.
+
+* Bias: after 16 warm frames, the cache's 256-sample render is compared with a 4096-sample plain
   reference (binary64 over disjoint 256-sample chunks): its error may exceed the plain 256-sample
   render's (the reference's own noise included) by at most 25 %, and its mean radiance stays
   within 1.5 % of the reference's.
 * Equal time: at the time the plain tracer takes for 16 and for 64 samples per pixel, the warm
-  cache's image has a lower MSE than plain's; with the denoiser on both, the cache is not worse at
-  16 samples and within 15 % at 64 (on this open scene most paths leave it before the cache's
-  vertex, so the cache saves little there, and the denoiser carries the gain).
+  cache's image has a lower MSE than plain's, cache + denoiser a lower MSE than plain + denoiser,
+  and cache + denoiser at least 2.5x lower than plain (the bench's equal-time study, which fits
+  each sample count to the budget, measures 3.4-3.5x at 64 samples; the bar leaves room for the
+  timing noise of single renders).
 """
 import sys
 import time
@@ -51,14 +56,14 @@ def setup(ensure_built):
         r.set_num_bounces(DEPTH)
         return r
 
-    say("reference: 1024 spp in 4 chunks")
+    say("reference: 4096 spp in 16 chunks")
     rr = renderer()
     acc = np.zeros((H, W, 3))
-    for k in range(4):
+    for k in range(16):
         rr.set_sample_base(1 << 20 | (256 * k))
         rr.render(s, W, H, 256)
         acc += rr.image()[..., :3]
-    ref = acc / 4
+    ref = acc / 16
     cache = m.RadianceCache(seed=1)
     cache.set_bounds((-12.5, -7.0, -9.5), (12.5, 6.0, 9.5))
     warm = renderer()
@@ -94,7 +99,7 @@ def test_cache_bias_is_bounded(setup):
     cr.render(s, W, H, 256)
     e_plain, e_cache = mse(plain.image(), ref), mse(cr.image(), ref)
     mean_ref, mean_cache = ref.mean(), cr.image()[..., :3].mean()
-    say("256 spp MSE vs 1024-spp reference: plain %.3e, cache %.3e; mean %.4f vs %.4f" % (
+    say("256 spp MSE vs 4096-spp reference: plain %.3e, cache %.3e; mean %.4f vs %.4f" % (
         e_plain, e_cache, mean_cache, mean_ref))
     assert np.isfinite(cr.image()).all()
     assert e_cache <= 1.25 * e_plain
@@ -135,8 +140,7 @@ def test_cache_gains_at_equal_time(setup, budget):
     e_cache_den = mse(cr.denoise(den), ref)
     say("MSE: plain %.3e, cache %.3e (x%.2f); plain+den %.3e, cache+den %.3e (x%.2f)" % (
         e_plain, e_cache, e_plain / e_cache, e_plain_den, e_cache_den, e_plain_den / e_cache_den))
+    say("plain / (cache+den) = %.2f" % (e_plain / e_cache_den))
     assert e_cache < e_plain
-    if budget == 16:
-        assert e_cache_den <= e_plain_den
-    else:
-        assert e_cache_den <= 1.15 * e_plain_den
+    assert e_cache_den < e_plain_den
+    assert e_plain / e_cache_den >= 2.5
