@@ -469,6 +469,7 @@ int Scene::trace_in(const TraceArena& a, const void* d_rays, size_t n, void* d_o
   p.lds_depth = lds_depth;
   p.waves = trace_waves ? trace_waves : (current.size() > 1 ? 7u : 6u);
   p.single_instance = current.size() == 1 ? 0u : 0xFFFFFFFFu;
+  p.leaf_wait = leaf_wait;
   if (launch_trace(p, any, stats, grid, stream) != 0) return fail(MRT_ERR_HIP, "trace kernel launch failed");
   if (sdf) {
     SdfParams sp{};
@@ -748,6 +749,10 @@ int mrt_scene_create(mrt_scene* out) {
   if (const char* v = std::getenv("MRT_TRACE_GRAB")) {
     const long g = std::strtol(v, nullptr, 10);
     if (g >= 1 && g <= 4096) s->trace_grab = uint32_t(g);
+  }
+  if (const char* v = std::getenv("MRT_TRACE_LEAF_WAIT")) {
+    const long w = std::strtol(v, nullptr, 10);
+    if (w >= 0 && w <= 64) s->leaf_wait = uint32_t(w);
   }
   if (const char* v = std::getenv("MRT_BVH_BUILDER")) s->sah_builder = std::strtol(v, nullptr, 10) != 0;
   if (const char* v = std::getenv("MRT_TRACE_WAVES")) {

@@ -376,13 +376,32 @@ trace_kernel(TraceParams p) {
     };
     const auto is_tri_leaf = [&](uint32_t e) { return level == 1 && e != kStackReturn && (e & 0x80000000u); };
 
-    // Every lane processes its current entry each iteration, whatever its kind.
+    // Every lane processes its current entry each iteration, whatever its kind, except that a
+    // lane at a triangle leaf, or at an instance leaf of the TLAS, waits while fewer than
+    // leaf_wait/64 of the busy lanes are at an entry of that kind and some lane has an inner node
+    // to test: the wave then runs the triangle block (~150 VALU instructions for two triangles) and
+    // the instance-entry block (~75) every few iterations instead of in almost every one, which is
+    // what a single lane's need made it do. A ray's sequence of entries does not change.
     {
       for (;;) {
-        if (ray_id >= 0) {
+        const bool busy = ray_id >= 0;
+        const bool at_leaf = busy && is_tri_leaf(entry);
+        // A frame of one instance enters it at refill, so it has no instance leaves to wait at.
+        const bool multi = p.single_instance == 0xFFFFFFFFu;
+        const bool at_inst = multi && busy && level == 0 && entry != kStackReturn && (entry & 0x80000000u);
+        bool run = busy;
+        if (p.leaf_wait) {
+          const uint32_t n_busy = uint32_t(__popcll(__ballot(busy)));
+          const uint32_t n_leaf = uint32_t(__popcll(__ballot(at_leaf)));
+          const uint32_t n_inst = multi ? uint32_t(__popcll(__ballot(at_inst))) : 0u;
+          bool leaf_ok = n_leaf * 64u >= p.leaf_wait * n_busy, inst_ok = n_inst * 64u >= p.leaf_wait * n_busy;
+          if (n_leaf + n_inst == n_busy && !leaf_ok && !inst_ok) leaf_ok = inst_ok = true;  // no node lane
+          run = busy && (at_leaf ? leaf_ok : (at_inst ? inst_ok : true));
+        }
+        if (run) {
           bool have_next = false, done = false;
           uint32_t next = 0;
-          if (is_tri_leaf(entry)) {
+          if (at_leaf) {
             done = leaf_step(entry);
           } else {
             node_step(entry, have_next, next, done);

@@ -29,6 +29,7 @@ struct TraceParams {
   uint32_t lds_depth;              // LDS stack entries per lane (a power of two)
   uint32_t waves;                  // waves per SIMD the kernel is compiled for: 6 or 7
   uint32_t single_instance;        // the frame's only instance record, or 0xFFFFFFFF
+  uint32_t leaf_wait;              // a lane at a leaf waits until leaf_wait/64 of the busy lanes are (0: never)
   unsigned long long* stats;       // rays, node visits, triangle tests, instance visits
 };
 
