@@ -508,6 +508,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     CHAIN ? MRT_SDF_CHAIN_WAVES : MRT_SDF_RAY_WAVES, 8))) void sdf_kernel(SdfParams p) {
   __shared__ uint4 s_prog[kProgCache * (sizeof(SdfInsn) / 16)];
   static_assert(sizeof(SdfInsn) % 16 == 0, "instruction records are whole 16-byte words");
+  // The grid covers the queue's capacity; a block past its live rays leaves before copying the
+  // program (the whole block at once, so the barrier below is never split).
+  if (blockIdx.x * kBlock >= (p.n_dev ? min(*p.n_dev, p.n) : p.n)) return;
   const bool cached = p.prog_len <= kProgCache;
   if (cached) {
     const uint4* src = reinterpret_cast<const uint4*>(p.prog);

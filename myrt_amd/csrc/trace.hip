@@ -122,6 +122,10 @@ trace_kernel(TraceParams p) {
   uint32_t res_next = 0, res_end = 0;  // rays reserved by this wave (wave-uniform): [res_next, res_end)
   // A device-side count (a wavefront queue) never exceeds the launch's capacity.
   const uint32_t n_rays = p.n_dev ? min(*p.n_dev, p.n) : p.n;
+  // The persistent grid is sized for full queues. Waves past the first ceil(n_rays / 64) leave at
+  // once: the waves before them take every ray from the counter, and a short queue (the last
+  // bounces of a render) then costs no counter atomic per resident wave.
+  if ((gtid & ~63u) >= n_rays) return;
 
   float wox = 0, woy = 0, woz = 0, wdx = 0, wdy = 0, wdz = 0, wix = 0, wiy = 0, wiz = 0;
   float tmin = 0, tmax = 0;
