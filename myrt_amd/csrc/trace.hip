@@ -69,7 +69,10 @@ __device__ inline void make_pre(float ox, float oy, float oz, float dx, float dy
 }
 
 // Watertight test. Returns true when the ray's line crosses the triangle (either side);
-// t = T / det is then the hit distance and (b1, b2) the weights of v1 and v2.
+// t = T / det is then the hit distance and (b1, b2) the weights of v1 and v2. The shear of each
+// vertex is one fused multiply-add per coordinate: a function of the vertex and the ray alone, so
+// triangles that share an edge see the same sheared endpoints and the edge functions stay
+// consistent (watertight).
 __device__ inline bool watertight(const RayPre& r, float4 a, float4 b, float4 c, float& t, float& b1, float& b2) {
   const float Axk = sel3(r.kx, a.x, a.y, a.z) - r.ox, Ayk = sel3(r.ky, a.x, a.y, a.z) - r.oy,
               Azk = sel3(r.kz, a.x, a.y, a.z) - r.oz;
@@ -77,9 +80,9 @@ __device__ inline bool watertight(const RayPre& r, float4 a, float4 b, float4 c,
               Bzk = sel3(r.kz, b.x, b.y, b.z) - r.oz;
   const float Cxk = sel3(r.kx, c.x, c.y, c.z) - r.ox, Cyk = sel3(r.ky, c.x, c.y, c.z) - r.oy,
               Czk = sel3(r.kz, c.x, c.y, c.z) - r.oz;
-  const float Ax = Axk - r.sx * Azk, Ay = Ayk - r.sy * Azk;
-  const float Bx = Bxk - r.sx * Bzk, By = Byk - r.sy * Bzk;
-  const float Cx = Cxk - r.sx * Czk, Cy = Cyk - r.sy * Czk;
+  const float Ax = fmaf(-r.sx, Azk, Axk), Ay = fmaf(-r.sy, Azk, Ayk);
+  const float Bx = fmaf(-r.sx, Bzk, Bxk), By = fmaf(-r.sy, Bzk, Byk);
+  const float Cx = fmaf(-r.sx, Czk, Cxk), Cy = fmaf(-r.sy, Czk, Cyk);
   float U = Cx * By - Cy * Bx;
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
@@ -92,7 +95,7 @@ __device__ inline bool watertight(const RayPre& r, float4 a, float4 b, float4 c,
   const float det = U + V + W;
   if (det == 0.f) return false;
   const float Az = r.sz * Azk, Bz = r.sz * Bzk, Cz = r.sz * Czk;
-  const float T = U * Az + V * Bz + W * Cz;
+  const float T = fmaf(U, Az, fmaf(V, Bz, W * Cz));
   t = T / det;
   const float rcp = 1.0f / det;
   b1 = V * rcp;

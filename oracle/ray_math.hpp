@@ -37,9 +37,11 @@ inline bool watertight(const Pre& p, const float* a, const float* b, const float
   const float Axk = a[p.kx] - p.o[0], Ayk = a[p.ky] - p.o[1], Azk = a[p.kz] - p.o[2];
   const float Bxk = b[p.kx] - p.o[0], Byk = b[p.ky] - p.o[1], Bzk = b[p.kz] - p.o[2];
   const float Cxk = c[p.kx] - p.o[0], Cyk = c[p.ky] - p.o[1], Czk = c[p.kz] - p.o[2];
-  const float Ax = Axk - p.sx * Azk, Ay = Ayk - p.sy * Azk;
-  const float Bx = Bxk - p.sx * Bzk, By = Byk - p.sy * Bzk;
-  const float Cx = Cxk - p.sx * Czk, Cy = Cyk - p.sy * Czk;
+  // The shear is one fused multiply-add per vertex coordinate (a function of the vertex and the
+  // ray alone, which keeps shared edges consistent).
+  const float Ax = std::fmaf(-p.sx, Azk, Axk), Ay = std::fmaf(-p.sy, Azk, Ayk);
+  const float Bx = std::fmaf(-p.sx, Bzk, Bxk), By = std::fmaf(-p.sy, Bzk, Byk);
+  const float Cx = std::fmaf(-p.sx, Czk, Cxk), Cy = std::fmaf(-p.sy, Czk, Cyk);
   float U = Cx * By - Cy * Bx;
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
@@ -52,7 +54,7 @@ inline bool watertight(const Pre& p, const float* a, const float* b, const float
   const float det = U + V + W;
   if (det == 0.f) return false;
   const float Az = p.sz * Azk, Bz = p.sz * Bzk, Cz = p.sz * Czk;
-  const float T = U * Az + V * Bz + W * Cz;
+  const float T = std::fmaf(U, Az, std::fmaf(V, Bz, W * Cz));
   t = T / det;
   const float rcp = 1.0f / det;
   b1 = V * rcp;
