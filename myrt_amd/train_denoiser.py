@@ -9,7 +9,8 @@ saved as a float32 .npy of the packed parameters.
 
 Usage:
   python -m myrt_amd.train_denoiser [--scenes 48] [--val 4] [--res 256] [--target-spp 1024]
-                                    [--steps 3000] [--batch 8] [--out myrt_amd/data/denoiser_unet.npy]
+                                    [--steps 3000] [--batch 8] [--init weights.npy]
+                                    [--out myrt_amd/data/denoiser_unet.npy]
 """
 import argparse
 import json
@@ -101,6 +102,9 @@ def main(argv=None):
                     help="how often each input spp level is drawn: uniformly, or in proportion to "
                          "sqrt(spp) so that the smaller errors of the cleaner inputs weigh as much "
                          "in the L1 loss as the noisy ones")
+    ap.add_argument("--init", default="",
+                    help="start from these packed parameters (.npy) instead of a seeded initialisation "
+                         "(fine-tuning; Adam's moments start at zero)")
     ap.add_argument("--out", default=DEFAULT_OUT)
     ap.add_argument("--max-seconds", type=float, default=0.0,
                     help="wall-clock budget: rendering stops at 60 %% of it (fewer training scenes) and "
@@ -126,6 +130,9 @@ def main(argv=None):
     train = data
 
     d = m.Denoiser(seed=args.seed)
+    if args.init:
+        d.load(args.init)
+        log("initialised from %s" % args.init)
     rng = np.random.default_rng(args.seed)
     level_p = np.sqrt(np.asarray(INPUT_SPP, np.float64)) if args.level_weight == "sqrt" else np.ones(len(INPUT_SPP))
     level_p /= level_p.sum()
