@@ -127,7 +127,7 @@ struct Scene {
   size_t cap_host_rays = 0, cap_host_out = 0;
   bool sdf_chain_only = false;  // every enqueued SDF program has a culling grid
   int num_cus = 0;
-  uint32_t refill_below = 48;
+  uint32_t refill_below = 56;
   uint32_t leaf_wait = 16;    // MRT_TRACE_LEAF_WAIT: leaf lanes wait for leaf_wait/64 of the busy lanes
   uint32_t trace_grab = 128;  // MRT_TRACE_GRAB: rays a traversal wave reserves per counter atomic
   bool sah_builder = true;  // traversal trees from build_sah_bvh (MRT_BVH_BUILDER=0: the reference builder's)
