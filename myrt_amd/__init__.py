@@ -17,9 +17,8 @@ HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("bary", "
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("type_and_parent", "<u4"), ("max", "<f4", 3), ("first", "<u4"),
                        ("second", "<u4"), ("pad", "<u4", 3)])
 AABB_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("centroid", "<f4", 3), ("has_centroid", "<u4")])
-WIDE_NODE_DTYPE = np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("nchildren", "u1"), ("node_base", "<u4"),
-                            ("item_base", "<u4"), ("qlo", "u1", (3, 4)), ("qhi", "u1", (3, 4)),
-                            ("count", "u1", 4), ("offset", "<u2", 4), ("pad", "<u4")])
+WIDE_NODE_DTYPE = np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("nchildren", "u1"), ("qlo", "u1", (3, 4)),
+                            ("qhi", "u1", (3, 4)), ("pad", "<u4", 2), ("entry", "<u4", 4)])
 TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("prim", "<u4"), ("v1", "<f4", 3), ("pad1", "<u4"), ("v2", "<f4", 3),
                       ("pad2", "<u4")])
 INSTANCE_DTYPE = np.dtype([("inv", "<f4", 12), ("blas_root", "<u4"), ("geom", "<u4"), ("inst", "<u4"),

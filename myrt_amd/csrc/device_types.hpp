@@ -8,21 +8,20 @@ namespace mrt {
 //   Child c's box on axis a is [fma(qlo[a][c], 2^e[a], origin[a]), fma(qhi[a][c], 2^e[a], origin[a])];
 //   the host rounds the codes outward and checks the decoded floats, so every decoded box
 //   strictly contains the child's exact box.
-//   count[c] == 0: inner child, node index node_base + offset[c].
-//   count[c] >  0: leaf child with count[c] items starting at item_base + offset[c]
-//                  (triangle records in a BLAS, instance indices in the TLAS).
+//   entry[c] is the child's stack entry, so the traversal pushes it without decoding:
+//   bit 31 clear: inner child, node index entry[c];
+//   bit 31 set:   leaf child of (entry[c] >> 24) & 0x7f items starting at entry[c] & 0xffffff
+//                 (triangle records in a BLAS, instance indices in the TLAS).
 //   Children 0 .. nchildren-1 are valid.
 struct alignas(64) WideNode {
   float origin[3];
   int8_t exp[3];
   uint8_t nchildren;
-  uint32_t node_base;
-  uint32_t item_base;
   uint8_t qlo[3][4];
   uint8_t qhi[3][4];
-  uint8_t count[4];
-  uint16_t offset[4];
-  uint32_t pad;
+  uint32_t pad[2];
+  uint32_t entry[4];  // each child's stack entry: an inner child's node index, or for a leaf
+                      // 0x80000000 | item count << 24 | first item (the traversal pushes it as is)
 };
 static_assert(sizeof(WideNode) == 64, "wide node must be one 64-byte line");
 

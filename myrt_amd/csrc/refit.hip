@@ -214,9 +214,7 @@ __device__ void quantize_axis(WideNode* out, int a, const float (&cmn)[4], const
 __device__ void refit_node_axis(const RefitParams& p, uint32_t id, int a) {
   WideNode* node = p.tlas + id;
   const int nc = node->nchildren;
-  const uint32_t node_base = node->node_base, item_base = node->item_base;
-  const uint32_t counts = *reinterpret_cast<const uint32_t*>(node->count);
-  const uint2 offs = *reinterpret_cast<const uint2*>(node->offset);
+  const uint4 ents = *reinterpret_cast<const uint4*>(node->entry);
   float mn[4], mx[4];
   double pad[4];
   float all_mn = FLT_MAX, all_mx = -FLT_MAX;
@@ -225,14 +223,14 @@ __device__ void refit_node_axis(const RefitParams& p, uint32_t id, int a) {
     DBox b;
     empty_box(b);
     if (c < nc) {
-      const uint32_t count = (counts >> (8 * c)) & 0xffu;
-      const uint32_t off = ((c < 2 ? offs.x : offs.y) >> (16 * (c & 1))) & 0xffffu;
-      if (count == 0) {
-        const float* q = p.node_box + size_t(node_base + off) * 6;
+      const uint32_t e = c == 0 ? ents.x : (c == 1 ? ents.y : (c == 2 ? ents.z : ents.w));
+      if (!(e & 0x80000000u)) {
+        const float* q = p.node_box + size_t(e) * 6;
         enclose(b, q, q + 3);
       } else {
+        const uint32_t count = (e >> 24) & 0x7fu, first = e & 0xffffffu;
         for (uint32_t k = 0; k < count; ++k) {
-          const float* q = p.inst_box + size_t(p.items[item_base + off + k]) * 6;
+          const float* q = p.inst_box + size_t(p.items[first + k]) * 6;
           enclose(b, q, q + 3);
         }
       }

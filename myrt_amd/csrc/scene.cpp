@@ -132,8 +132,8 @@ void Scene::tlas_levels() {
   for (uint32_t i = 0; i < tlas_nodes.size(); ++i) {
     const WideNode& n = tlas_nodes[i];
     for (int c = 0; c < n.nchildren; ++c)
-      if (n.count[c] == 0) {
-        const uint32_t j = n.node_base + n.offset[c];
+      if (!(n.entry[c] & 0x80000000u)) {
+        const uint32_t j = n.entry[c];
         depth[j] = depth[i] + 1;
         max_depth = std::max(max_depth, depth[j]);
       }
@@ -161,8 +161,9 @@ int Scene::build_host() {
         blas_root[g] = node_off;
         blas_depth = std::max(blas_depth, geo.wide.depth);
         for (WideNode n : geo.wide.nodes) {
-          n.node_base += node_off;
-          n.item_base += item_off;
+          // Entries index the geometry's own arrays; move them to the concatenated ones.
+          for (int c = 0; c < n.nchildren; ++c)
+            n.entry[c] += (n.entry[c] & 0x80000000u) ? item_off : node_off;
           blas_nodes.push_back(n);
         }
         for (uint32_t item : geo.wide.items) {

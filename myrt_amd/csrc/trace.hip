@@ -294,11 +294,12 @@ trace_kernel(TraceParams p) {
                     szz = fmaf(fabsf(bz), kSlack, kSlackFloor);
         const float bnx = bx - sxx, bfx = bx + sxx, bny = by - syy, bfy = by + syy, bnz = bz - szz, bfz = bz + szz;
         const bool nx = ix < 0.f, ny = iy < 0.f, nz = iz < 0.f;
-        const uint32_t wnx = nx ? q2.y : q1.z, wfx = nx ? q1.z : q2.y;
-        const uint32_t wny = ny ? q2.z : q1.w, wfy = ny ? q1.w : q2.z;
-        const uint32_t wnz = nz ? q2.w : q2.x, wfz = nz ? q2.x : q2.w;
+        // Low codes of x, y, z in q1.x, q1.y, q1.z; high codes in q1.w, q2.x, q2.y.
+        const uint32_t wnx = nx ? q1.w : q1.x, wfx = nx ? q1.x : q1.w;
+        const uint32_t wny = ny ? q2.x : q1.y, wfy = ny ? q1.y : q2.x;
+        const uint32_t wnz = nz ? q2.y : q1.z, wfz = nz ? q1.z : q2.y;
         float key[4];
-        uint32_t ent[4];
+        uint32_t ent[4] = {q3.x, q3.y, q3.z, q3.w};  // the children's stack entries, stored as such
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const float tn = fmaxf(fmaxf(fmaf(byte_of(wnx, c), ax, bnx), fmaf(byte_of(wny, c), ay, bny)),
@@ -309,9 +310,6 @@ trace_kernel(TraceParams p) {
           // t_min) moves outwards; for tf0 >= 0 this equals tf0 * kRobust bit for bit.
           const float tf = fmaf(fabsf(tf0), kRobust - 1.0f, tf0);
           const bool hit = (uint32_t(c) < nch) && tn <= tf;
-          const uint32_t cnt = (q3.x >> (8 * c)) & 0xffu;
-          const uint32_t off = (c < 2 ? (q3.y >> (16 * c)) : (q3.z >> (16 * (c - 2)))) & 0xffffu;
-          ent[c] = cnt == 0 ? (q1.x + off) : (0x80000000u | (cnt << 24) | (q1.y + off));
           key[c] = hit ? tn : __int_as_float(0x7f800000);
         }
         // Sort the four (distance, entry) pairs ascending; misses carry +inf and sort last.

@@ -205,16 +205,12 @@ WideBvh collapse_to_wide(const BinaryBvh& b, double pad_rel, uint32_t max_leaf) 
     WideNode& n = w.nodes[p.node];
     n = WideNode{};
     n.nchildren = uint8_t(ch.size());
-    n.node_base = node_base;
-    n.item_base = uint32_t(w.items.size());
     uint32_t inner = 0;
     for (size_t k = 0; k < ch.size(); ++k) {
       if (ch[k].internal) {
-        n.count[k] = 0;
-        n.offset[k] = uint16_t(inner++);
+        n.entry[k] = node_base + inner++;
       } else {
-        n.count[k] = uint8_t(ch[k].items.size());
-        n.offset[k] = uint16_t(w.items.size() - n.item_base);
+        n.entry[k] = 0x80000000u | (uint32_t(ch[k].items.size()) << 24) | uint32_t(w.items.size());
         w.items.insert(w.items.end(), ch[k].items.begin(), ch[k].items.end());
       }
     }

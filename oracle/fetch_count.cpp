@@ -29,13 +29,10 @@ struct WideNode {
   float origin[3];
   int8_t exp[3];
   uint8_t nchildren;
-  uint32_t node_base;
-  uint32_t item_base;
   uint8_t qlo[3][4];
   uint8_t qhi[3][4];
-  uint8_t count[4];
-  uint16_t offset[4];
-  uint32_t pad;
+  uint32_t pad[2];
+  uint32_t entry[4];  // an inner child's node index, or 0x80000000 | count << 24 | first item
 };
 static_assert(sizeof(WideNode) == 64, "wide node record");
 struct TriRecord {
@@ -185,8 +182,7 @@ Counts walk(const Arrays& A, const orc_ray& r, bool any, orc_hit* hit_out) {
         const float tf0 = std::fmin(std::fmin(tf3[0], tf3[1]), std::fmin(tf3[2], t_best));
         const float tf = std::fmaf(std::fabs(tf0), kRobust - 1.0f, tf0);
         const bool hit = uint32_t(ch) < n.nchildren && tn <= tf;
-        const uint32_t cnt = n.count[ch], off = n.offset[ch];
-        ent[ch] = cnt == 0 ? (n.node_base + off) : (0x80000000u | (cnt << 24) | (n.item_base + off));
+        ent[ch] = n.entry[ch];
         key[ch] = hit ? tn : INFINITY;
       }
       const auto cswap = [&](int i, int j) {

@@ -136,11 +136,12 @@ def test_refit_after_motion_matches_a_rebuild(field):
     checked = 0
     for node in tlas:
         for c in range(node["nchildren"]):
-            if node["count"][c] == 0:
+            e = int(node["entry"][c])
+            if not e & 0x80000000:
                 continue
             lo, hi = decoded_boxes(node, c)
-            first = node["item_base"] + node["offset"][c]
-            for it in items[first:first + node["count"][c]]:
+            first, count = e & 0xFFFFFF, (e >> 24) & 0x7F
+            for it in items[first:first + count]:
                 wl, wh = world[it]
                 assert np.all(lo <= wl + 1e-6 * (1 + np.abs(wl))) and np.all(hi >= wh - 1e-6 * (1 + np.abs(wh)))
                 checked += 1

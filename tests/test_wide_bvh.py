@@ -28,11 +28,11 @@ def walk(nodes):
         n = nodes[k]
         for c in range(n["nchildren"]):
             box = decode(n, c)
-            if n["count"][c] == 0:
-                stack.append((int(n["node_base"] + n["offset"][c]), path + [box]))
+            e = int(n["entry"][c])
+            if not e & 0x80000000:
+                stack.append((e, path + [box]))
             else:
-                start = int(n["item_base"] + n["offset"][c])
-                yield start, int(n["count"][c]), path + [box]
+                yield e & 0xFFFFFF, (e >> 24) & 0x7F, path + [box]
 
 
 @pytest.mark.parametrize("builder", ["sah", "reference"])
