@@ -12,7 +12,10 @@ namespace mrt {
 //   bit 31 clear: inner child, node index entry[c];
 //   bit 31 set:   leaf child of (entry[c] >> 24) & 0x7f items starting at entry[c] & 0xffffff
 //                 (triangle records in a BLAS, instance indices in the TLAS).
-//   Children 0 .. nchildren-1 are valid.
+//   Children 0 .. nchildren-1 are valid. An unused slot holds an inverted box (low codes 255,
+//   high codes 0), which the slab test rejects unless the node is smaller than about 2^-19 of
+//   its distance from the ray origin, and the entry of a leaf of no items, which is harmless if
+//   visited; so the traversal needs no child count.
 struct alignas(64) WideNode {
   float origin[3];
   int8_t exp[3];

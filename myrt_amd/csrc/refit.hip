@@ -184,6 +184,7 @@ __device__ void quantize_axis(WideNode* out, int a, const float (&cmn)[4], const
     const float scale = ldexpf(1.0f, e);
     bool ok = true;
     lo_codes = hi_codes = 0;
+    for (int c = nc; c < 4; ++c) lo_codes |= 255u << (8 * c);  // unused slots keep an inverted box
     // All children are evaluated even after one fails (a failed exponent is discarded anyway),
     // so their independent chains overlap.
 #pragma unroll

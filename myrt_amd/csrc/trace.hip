@@ -260,14 +260,17 @@ trace_kernel(TraceParams p) {
           ox = wox; oy = woy; oz = woz; ix = wix; iy = wiy; iz = wiz;
         }
       } else if (e & 0x80000000u) {
-        // Enter an instance: the remaining instances of this leaf stay on the stack.
+        // Enter an instance: the remaining instances of this leaf stay on the stack. A leaf of no
+        // items (an unused slot whose box was accepted) is skipped.
         const uint32_t cnt = (e >> 24) & 0x7fu;
         const uint32_t start = e & 0xffffffu;
-        const uint32_t inst = p.tlas_items[start];
-        if (cnt > 1) push(0x80000000u | ((cnt - 1) << 24) | (start + 1));
-        push(kStackReturn);
-        next = enter(inst);  // BLAS root node
-        have_next = true;
+        if (cnt) {
+          const uint32_t inst = p.tlas_items[start];
+          if (cnt > 1) push(0x80000000u | ((cnt - 1) << 24) | (start + 1));
+          push(kStackReturn);
+          next = enter(inst);  // BLAS root node
+          have_next = true;
+        }
       } else {
         // ---- inner node: test the four quantised child boxes ----
         const WideNode* nodes = level ? p.blas_nodes : p.tlas_nodes;
@@ -278,7 +281,6 @@ trace_kernel(TraceParams p) {
         // The per-axis exponents e (signed bytes, -126..127: 2^e is a normal power of two).
         const int ex = int(int8_t(q0.w & 0xffu)), ey = int(int8_t((q0.w >> 8) & 0xffu)),
                   ez = int(int8_t((q0.w >> 16) & 0xffu));
-        const uint32_t nch = q0.w >> 24;
         // A child plane at origin + q * scale is crossed at t = q * a + b, with a = scale * i
         // (exact: scale is a power of two) and b = (origin - o) * i. Computing b rounds twice,
         // an absolute error of at most 2.2 ulp of |b|; the near planes use b - S and the far
@@ -309,7 +311,7 @@ trace_kernel(TraceParams p) {
           // Widened by 2^-20 of its magnitude, so also a far plane behind the origin (a negative
           // t_min) moves outwards; for tf0 >= 0 this equals tf0 * kRobust bit for bit.
           const float tf = fmaf(fabsf(tf0), kRobust - 1.0f, tf0);
-          const bool hit = (uint32_t(c) < nch) && tn <= tf;
+          const bool hit = tn <= tf;  // unused slots hold inverted boxes and harmless entries
           key[c] = hit ? tn : __int_as_float(0x7f800000);
         }
         // Sort the four (distance, entry) pairs ascending; misses carry +inf and sort last.

@@ -164,8 +164,8 @@ void quantize(WideNode& n, const std::vector<Child>& ch, double pad_rel) {
     }
     n.origin[a] = origin;
     n.exp[a] = int8_t(e);
-    for (int c = nc; c < 4; ++c) {
-      n.qlo[a][c] = 0;
+    for (int c = nc; c < 4; ++c) {  // unused slot: an inverted box (low code above high)
+      n.qlo[a][c] = 255;
       n.qhi[a][c] = 0;
     }
   }
@@ -205,6 +205,7 @@ WideBvh collapse_to_wide(const BinaryBvh& b, double pad_rel, uint32_t max_leaf) 
     WideNode& n = w.nodes[p.node];
     n = WideNode{};
     n.nchildren = uint8_t(ch.size());
+    for (int k = 0; k < 4; ++k) n.entry[k] = 0x80000000u;  // unused slots: a leaf of no items
     uint32_t inner = 0;
     for (size_t k = 0; k < ch.size(); ++k) {
       if (ch[k].internal) {

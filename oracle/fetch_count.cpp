@@ -150,11 +150,13 @@ Counts walk(const Arrays& A, const orc_ray& r, bool any, orc_hit* hit_out) {
       }
     } else if (entry & 0x80000000u) {
       const uint32_t cnt = (entry >> 24) & 0x7fu, start = entry & 0xffffffu;
-      const uint32_t k = A.items[start];
-      if (cnt > 1) stack.push_back(0x80000000u | ((cnt - 1) << 24) | (start + 1));
-      stack.push_back(kReturn);
-      next = enter(k);
-      have_next = true;
+      if (cnt) {  // a leaf of no items is an unused slot
+        const uint32_t k = A.items[start];
+        if (cnt > 1) stack.push_back(0x80000000u | ((cnt - 1) << 24) | (start + 1));
+        stack.push_back(kReturn);
+        next = enter(k);
+        have_next = true;
+      }
     } else {
       const WideNode& n = (level ? A.blas : A.tlas)[entry];
       ++c.nodes;
@@ -181,7 +183,7 @@ Counts walk(const Arrays& A, const orc_ray& r, bool any, orc_hit* hit_out) {
         const float tn = std::fmax(std::fmax(tn3[0], tn3[1]), std::fmax(tn3[2], tmin));
         const float tf0 = std::fmin(std::fmin(tf3[0], tf3[1]), std::fmin(tf3[2], t_best));
         const float tf = std::fmaf(std::fabs(tf0), kRobust - 1.0f, tf0);
-        const bool hit = uint32_t(ch) < n.nchildren && tn <= tf;
+        const bool hit = tn <= tf;  // unused slots: inverted boxes, harmless entries
         ent[ch] = n.entry[ch];
         key[ch] = hit ? tn : INFINITY;
       }
