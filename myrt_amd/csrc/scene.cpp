@@ -393,7 +393,7 @@ int Scene::full_grid(bool any, bool stats, size_t n) const {
   // Seven waves per SIMD pay off where rays change instances often (the per-iteration work is
   // larger and more divergent); a single mesh runs best at six. MRT_TRACE_WAVES overrides.
   const uint32_t waves = trace_waves ? trace_waves : (current.size() > 1 ? 7u : 6u);
-  const int per_cu = trace_blocks_per_cu(any, stats, waves, lds_depth);
+  const int per_cu = trace_blocks_per_cu(any, stats, waves, lds_depth, current.size() == 1);
   const size_t max_grid = size_t(num_cus) * size_t(per_cu);
   const size_t want = (n + kTraceBlock - 1) / kTraceBlock;
   return int(std::max<size_t>(1, std::min(max_grid, want)));

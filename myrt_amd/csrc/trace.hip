@@ -111,7 +111,10 @@ __device__ inline uint32_t pack_bary(float b1, float b2) {
 
 __device__ inline float byte_of(uint32_t w, int c) { return float((w >> (8 * c)) & 0xffu); }
 
-template <bool ANY, bool STATS, int WAVES>
+// SINGLE: a frame of one instance (TraceParams::single_instance set). Each ray enters it at
+// refill and never returns to world space, so this instantiation keeps no world ray past the
+// refill, walks the BLAS nodes only and has no instance-leaf or RETURN handling in its loop.
+template <bool ANY, bool STATS, int WAVES, bool SINGLE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void
 trace_kernel(TraceParams p) {
   extern __shared__ uint32_t lds_stack[];  // [lds_depth][kBlock]
@@ -236,7 +239,7 @@ trace_kernel(TraceParams p) {
           if (STATS) n_nodes = n_tris = n_inst = 0;
           if (!p.has_tlas || !(wdx != 0.f || wdy != 0.f || wdz != 0.f)) {
             entry = kStackReturn;  // finishes at once
-          } else if (p.single_instance != 0xFFFFFFFFu) {
+          } else if (SINGLE) {
             // A frame of one instance: enter it here, with the other fresh rays of the wave,
             // instead of through the TLAS root; no RETURN marker, the ray ends with its BLAS.
             entry = enter(p.single_instance);
@@ -251,15 +254,20 @@ trace_kernel(TraceParams p) {
     // One step on an entry that is not a triangle leaf: an inner node (test the four children,
     // push the far ones, continue with the nearest), an instance leaf of the TLAS (move the ray
     // into object space and continue at the BLAS root) or a RETURN marker (back to world space).
-    const auto node_step = [&](uint32_t e, bool& have_next, uint32_t& next, bool& done) {
+    // Returns kStepPop (nothing to continue with: pop), kStepNext (continue at `next`) or kStepDone.
+    // A status value rather than flags written through references, which the compiler kept in a
+    // scratch slot in the single-instance instantiation.
+    constexpr int kStepPop = 0, kStepNext = 1, kStepDone = 2;
+    const auto node_step = [&](uint32_t e, uint32_t& next) -> int {
+      int status = kStepPop;
       if (e == kStackReturn) {
-        if (level == 0) {
-          done = true;  // no scene or degenerate ray
+        if (SINGLE || level == 0) {
+          status = kStepDone;  // no scene or degenerate ray
         } else {
           level = 0;
           ox = wox; oy = woy; oz = woz; ix = wix; iy = wiy; iz = wiz;
         }
-      } else if (e & 0x80000000u) {
+      } else if (!SINGLE && (e & 0x80000000u)) {
         // Enter an instance: the remaining instances of this leaf stay on the stack. A leaf of no
         // items (an unused slot whose box was accepted) is skipped.
         const uint32_t cnt = (e >> 24) & 0x7fu;
@@ -269,11 +277,11 @@ trace_kernel(TraceParams p) {
           if (cnt > 1) push(0x80000000u | ((cnt - 1) << 24) | (start + 1));
           push(kStackReturn);
           next = enter(inst);  // BLAS root node
-          have_next = true;
+          status = kStepNext;
         }
       } else {
         // ---- inner node: test the four quantised child boxes ----
-        const WideNode* nodes = level ? p.blas_nodes : p.tlas_nodes;
+        const WideNode* nodes = (SINGLE || level) ? p.blas_nodes : p.tlas_nodes;
         const uint4* np = reinterpret_cast<const uint4*>(nodes + e);
         const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
         if (STATS) ++n_nodes;
@@ -331,9 +339,10 @@ trace_kernel(TraceParams p) {
         if (nhit > 1) push(ent[1]);
         if (nhit > 0) {
           next = ent[0];
-          have_next = true;
+          status = kStepNext;
         }
       }
+      return status;
     };
     // The triangles of a BLAS leaf; returns true when an any-hit query is answered.
     const auto leaf_step = [&](uint32_t e) -> bool {
@@ -381,7 +390,9 @@ trace_kernel(TraceParams p) {
       }
       ray_id = -1;
     };
-    const auto is_tri_leaf = [&](uint32_t e) { return level == 1 && e != kStackReturn && (e & 0x80000000u); };
+    const auto is_tri_leaf = [&](uint32_t e) {
+      return (SINGLE || level == 1) && e != kStackReturn && (e & 0x80000000u);
+    };
 
     // Every lane processes its current entry each iteration, whatever its kind, except that a
     // lane at a triangle leaf, or at an instance leaf of the TLAS, waits while fewer than
@@ -394,7 +405,7 @@ trace_kernel(TraceParams p) {
         const bool busy = ray_id >= 0;
         const bool at_leaf = busy && is_tri_leaf(entry);
         // A frame of one instance enters it at refill, so it has no instance leaves to wait at.
-        const bool multi = p.single_instance == 0xFFFFFFFFu;
+        const bool multi = !SINGLE;
         const bool at_inst = multi && busy && level == 0 && entry != kStackReturn && (entry & 0x80000000u);
         bool run = busy;
         if (p.leaf_wait) {
@@ -406,13 +417,9 @@ trace_kernel(TraceParams p) {
           run = busy && (at_leaf ? leaf_ok : (at_inst ? inst_ok : true));
         }
         if (run) {
-          bool have_next = false, done = false;
           uint32_t next = 0;
-          if (at_leaf) {
-            done = leaf_step(entry);
-          } else {
-            node_step(entry, have_next, next, done);
-          }
+          const int status = at_leaf ? (leaf_step(entry) ? kStepDone : kStepPop) : node_step(entry, next);
+          bool done = status == kStepDone, have_next = status == kStepNext;
           if (!done && !have_next) {
             if (sp == 0) {
               done = true;
@@ -437,40 +444,36 @@ trace_kernel(TraceParams p) {
 
 }  // namespace dev
 
+namespace {
+// The instantiation for (any, stats, waves 6 or 7, single instance).
+template <bool A, bool S, int W>
+const void* pick_kernel(bool single) {
+  return single ? reinterpret_cast<const void*>(dev::trace_kernel<A, S, W, true>)
+                : reinterpret_cast<const void*>(dev::trace_kernel<A, S, W, false>);
+}
+template <bool A, bool S>
+const void* pick_kernel(uint32_t waves, bool single) {
+  return waves == 7 ? pick_kernel<A, S, 7>(single) : pick_kernel<A, S, 6>(single);
+}
+const void* pick_kernel(bool any, bool stats, uint32_t waves, bool single) {
+  if (any) return stats ? pick_kernel<true, true>(waves, single) : pick_kernel<true, false>(waves, single);
+  return stats ? pick_kernel<false, true>(waves, single) : pick_kernel<false, false>(waves, single);
+}
+}  // namespace
+
 int launch_trace(const TraceParams& p, bool any, bool stats, int grid, hipStream_t stream) {
-  const dim3 block(dev::kBlock), g(grid);
   const size_t lds = size_t(p.lds_depth) * dev::kBlock * 4;
-  const bool w7 = p.waves == 7;
-#define MRT_LAUNCH(A, S, W) hipLaunchKernelGGL((dev::trace_kernel<A, S, W>), g, block, lds, stream, p)
-  if (any) {
-    if (stats) {
-      if (w7) MRT_LAUNCH(true, true, 7); else MRT_LAUNCH(true, true, 6);
-    } else {
-      if (w7) MRT_LAUNCH(true, false, 7); else MRT_LAUNCH(true, false, 6);
-    }
-  } else {
-    if (stats) {
-      if (w7) MRT_LAUNCH(false, true, 7); else MRT_LAUNCH(false, true, 6);
-    } else {
-      if (w7) MRT_LAUNCH(false, false, 7); else MRT_LAUNCH(false, false, 6);
-    }
-  }
-#undef MRT_LAUNCH
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  const void* k = pick_kernel(any, stats, p.waves, p.single_instance != 0xFFFFFFFFu);
+  TraceParams arg = p;
+  void* args[] = {&arg};
+  return hipLaunchKernel(k, dim3(grid), dim3(dev::kBlock), args, lds, stream) == hipSuccess ? 0 : -1;
 }
 
-int trace_blocks_per_cu(bool any, bool stats, uint32_t waves, uint32_t lds_depth) {
+int trace_blocks_per_cu(bool any, bool stats, uint32_t waves, uint32_t lds_depth, bool single) {
   int blocks = 0;
   const size_t lds = size_t(lds_depth) * dev::kBlock * 4;
-  const bool w7 = waves == 7;
-  const void* k;
-  if (any)
-    k = stats ? (w7 ? (const void*)dev::trace_kernel<true, true, 7> : (const void*)dev::trace_kernel<true, true, 6>)
-              : (w7 ? (const void*)dev::trace_kernel<true, false, 7> : (const void*)dev::trace_kernel<true, false, 6>);
-  else
-    k = stats ? (w7 ? (const void*)dev::trace_kernel<false, true, 7> : (const void*)dev::trace_kernel<false, true, 6>)
-              : (w7 ? (const void*)dev::trace_kernel<false, false, 7> : (const void*)dev::trace_kernel<false, false, 6>);
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, dev::kBlock, lds);
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, pick_kernel(any, stats, waves, single), dev::kBlock, lds);
   if (e != hipSuccess || blocks < 1) blocks = 1;
   return blocks;
 }

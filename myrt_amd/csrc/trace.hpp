@@ -34,7 +34,7 @@ struct TraceParams {
 };
 
 int launch_trace(const TraceParams& p, bool any, bool stats, int grid, hipStream_t stream);
-int trace_blocks_per_cu(bool any, bool stats, uint32_t waves, uint32_t lds_depth);
+int trace_blocks_per_cu(bool any, bool stats, uint32_t waves, uint32_t lds_depth, bool single);
 constexpr int kTraceBlock = 256;
 constexpr uint32_t kTraceStackLds = 16;  // default LDS stack entries per lane
 
