@@ -465,12 +465,13 @@ int Scene::trace_in(const TraceArena& a, const void* d_rays, size_t n, void* d_o
   p.spill = a.spill;
   p.spill_depth = depth;
   p.total_threads = total_threads;
-  p.refill_below = refill_below;
+  const bool single = current.size() == 1;
+  p.refill_below = refill_below != kKnobUnset ? refill_below : (single ? 52u : 56u);
   p.grab = trace_grab;
   p.lds_depth = lds_depth;
   p.waves = trace_waves ? trace_waves : (current.size() > 1 ? 7u : 6u);
   p.single_instance = current.size() == 1 ? 0u : 0xFFFFFFFFu;
-  p.leaf_wait = leaf_wait;
+  p.leaf_wait = leaf_wait != kKnobUnset ? leaf_wait : (single ? 20u : 16u);
   if (launch_trace(p, any, stats, grid, stream) != 0) return fail(MRT_ERR_HIP, "trace kernel launch failed");
   if (sdf) {
     SdfParams sp{};

@@ -127,8 +127,13 @@ struct Scene {
   size_t cap_host_rays = 0, cap_host_out = 0;
   bool sdf_chain_only = false;  // every enqueued SDF program has a culling grid
   int num_cus = 0;
-  uint32_t refill_below = 56;
-  uint32_t leaf_wait = 16;    // MRT_TRACE_LEAF_WAIT: leaf lanes wait for leaf_wait/64 of the busy lanes
+  // MRT_TRACE_REFILL: refill a traversal wave below this many busy lanes; MRT_TRACE_LEAF_WAIT:
+  // lanes at a leaf wait for leaf_wait/64 of the busy lanes (0: never). Unset (kKnobUnset), each
+  // takes the measured default for the frame: 52 and 20 with a single instance, 56 and 16 with
+  // several.
+  static constexpr uint32_t kKnobUnset = 0xFFFFFFFFu;
+  uint32_t refill_below = kKnobUnset;
+  uint32_t leaf_wait = kKnobUnset;
   uint32_t trace_grab = 128;  // MRT_TRACE_GRAB: rays a traversal wave reserves per counter atomic
   bool sah_builder = true;  // traversal trees from build_sah_bvh (MRT_BVH_BUILDER=0: the reference builder's)
   uint32_t trace_waves = 0;  // 0: by scene (7 with several instances, else 6); MRT_TRACE_WAVES=6|7
