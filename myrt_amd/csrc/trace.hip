@@ -12,6 +12,10 @@
 //   * persistent waves that refill idle lanes when fewer than refill_below of them are busy. A
 //     wave takes its rays from a reserve of its own, which it refills with one atomic on the
 //     global ray counter for at least `grab` rays (128 by default, MRT_TRACE_GRAB overrides);
+//   * lanes at a leaf that wait until leaf_wait/64 of the wave's busy lanes are at one, so the
+//     triangle and instance-entry blocks run for many lanes at a time;
+//   * an instantiation for frames of one instance that enters it at refill and walks BLAS nodes
+//     only;
 //   * a per-lane ring stack in LDS (16 entries by default) that spills to a per-thread slice of
 //     global memory when a path needs more.
 #include <hip/hip_runtime.h>
