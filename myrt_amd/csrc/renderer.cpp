@@ -145,6 +145,7 @@ struct Renderer {
   };
   Pipe pipe;
   float4* accum = nullptr;
+  uint64_t* ctr_host = nullptr;  // pinned: rays traced and non-finite values rejected by a render
   float4* aux = nullptr;  // primary-hit feature sums, 2 per pixel
   size_t cap_aux = 0;
   float* raw = nullptr;   // denoiser input (10 floats per pixel)
@@ -212,6 +213,7 @@ struct Renderer {
     for (void* p : {(void*)accum, (void*)tpaths, (void*)tverts, (void*)queries, (void*)qrec, (void*)qout,
                     (void*)batch_q, (void*)batch_t, (void*)aux, (void*)raw, (void*)den, (void*)bokeh, (void*)cubemap})
       if (p) hipFree(p);
+    if (ctr_host) hipHostFree(ctr_host);
   }
 
   int update_camera() {
@@ -317,13 +319,18 @@ struct Renderer {
         return rc;
       done += nb;
     }
-    uint64_t ctr[2] = {0, 0};  // rays traced, non-finite values rejected
-    e = hipMemcpyAsync(ctr, pipe.counts + 8, 16, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    // Rays traced and non-finite values rejected: copied on the stream, and read_ctrl's one
+    // synchronisation of the same stream covers this copy too.
+    if (!ctr_host && (e = hipHostMalloc(reinterpret_cast<void**>(&ctr_host), 16, hipHostMallocDefault)) != hipSuccess) {
+      ctr_host = nullptr;
+      return hip_fail(e, "hipHostMalloc render counter staging");
+    }
+    e = hipMemcpyAsync(ctr_host, pipe.counts + 8, 16, hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) return hip_fail(e, "render");
-    const uint64_t rays_traced = ctr[0];
     uint32_t flags = 0;
     if ((rc = s.read_ctrl(&flags, nullptr, stream)) != MRT_OK) return rc;
+    const uint64_t ctr[2] = {ctr_host[0], ctr_host[1]};
+    const uint64_t rays_traced = ctr[0];
     if (flags & 1u) return fail(MRT_ERR_INTERNAL, "traversal stack overflow");
     const auto t1 = std::chrono::steady_clock::now();
     sample_count += spp;

@@ -363,6 +363,7 @@ Scene::~Scene() {
                   (void*)d_refit_status})
     if (q) hipFree(q);
   if (d_tlas) hipFree(d_tlas);
+  if (ctrl_host) hipHostFree(ctrl_host);
   if (d_tlas_items) hipFree(d_tlas_items);
   if (d_inst) hipFree(d_inst);
 }
@@ -498,8 +499,16 @@ int Scene::trace_in(const TraceArena& a, const void* d_rays, size_t n, void* d_o
 }
 
 int Scene::read_ctrl(uint32_t* flags, unsigned long long* stats, hipStream_t stream) {
-  uint32_t host[kTraceContexts][16] = {};
+  // Pinned staging: the copies are plain DMA on the stream and one synchronisation covers them
+  // and whatever the caller queued before (a render reads its own counters the same way).
   hipError_t e = hipSuccess;
+  if (!ctrl_host) e = hipHostMalloc(reinterpret_cast<void**>(&ctrl_host), kTraceContexts * 64, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ctrl_host = nullptr;
+    return hip_fail(e, "hipHostMalloc trace counter staging");
+  }
+  uint32_t (*host)[16] = reinterpret_cast<uint32_t (*)[16]>(ctrl_host);
+  std::memset(ctrl_host, 0, kTraceContexts * 64);
   for (int c = 0; c < kTraceContexts && e == hipSuccess; ++c)
     if (ctx[c].ctrl) e = hipMemcpyAsync(host[c], ctx[c].ctrl, 64, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);

@@ -121,6 +121,8 @@ struct Scene {
   };
   static constexpr int kTraceContexts = 4;
   TraceScratch ctx[kTraceContexts];
+  // Pinned host copy of every context's 64-B trace counter block, filled by read_ctrl.
+  uint32_t* ctrl_host = nullptr;
   // Device staging of the host-buffer queries (mrt_trace_closest/any, mrt_scene_pick); they
   // grow to the largest call and are reused.
   void *d_host_rays = nullptr, *d_host_out = nullptr;
